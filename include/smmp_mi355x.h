@@ -1,0 +1,134 @@
+/*
+ * smmp_mi355x: C-ABI of the MI355X-native SMMP energy-and-move engine.
+ *
+ * Replaces the f2py extension module `smmp` generated from the reference's
+ * smmp.pyf (file:line citations are relative to the reference's trunk/).
+ * Plain pointers and sizes only.  Every call returns 0 on success or a
+ * negative status; smmp_last_error() gives the message (the reference stops
+ * the process instead, e.g. mklist.f:112, enysol.f:136, metropolis.f:177).
+ *
+ * One smmp_ctx holds one system (the reference's COMMON blocks, INCL.H) on
+ * the host plus its device mirror on the current HIP device.  Contexts are
+ * independent; a context is not thread-safe (as the reference,
+ * universe.py:94-95).
+ */
+#ifndef SMMP_MI355X_H
+#define SMMP_MI355X_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct smmp_ctx smmp_ctx;
+
+/* status codes */
+#define SMMP_OK 0
+#define SMMP_EINVAL -1   /* bad argument */
+#define SMMP_EINPUT -2   /* bad input file / topology (a reference 'stop') */
+#define SMMP_EDEVICE -3  /* HIP error or no device */
+#define SMMP_EDRIFT -4   /* end-of-sweep energy drift > 1e-6 (metropolis.f:172-179) */
+#define SMMP_ESTATE -5   /* call out of order (e.g. energy before init_molecule) */
+
+/* dtype codes of state views */
+#define SMMP_I32 1
+#define SMMP_F64 2
+#define SMMP_CHR 3
+
+/* ---- context --------------------------------------------------------- */
+int smmp_ctx_create(smmp_ctx **out);
+void smmp_ctx_destroy(smmp_ctx *ctx);
+const char *smmp_last_error(const smmp_ctx *ctx);
+/* Writable view of a COMMON-block variable, named "block.field" as in
+ * smmp.pyf (e.g. "atm_r.xat", "bet.beta", "var_r.vlvr").  shape/lbound get
+ * up to 2 dims (Fortran order); CHR arrays report (len, n). */
+int smmp_state_view(smmp_ctx *ctx, const char *name, void **ptr, int *dtype, int *ndim,
+                    int64_t *shape, int64_t *lbound);
+/* number of names and the i-th name (for the shim's attribute table) */
+int smmp_state_count(void);
+const char *smmp_state_name(int i);
+
+/* ---- f2py entry points (smmp.pyf) ------------------------------------ */
+/* init_energy(libdir) smmp.pyf:4404, init_energy.f:14 (idempotent here) */
+int smmp_init_energy(smmp_ctx *ctx, const char *libdir);
+/* init_molecule(iabin,grpn,grpc,seqfile,varfile) smmp.pyf:5011, init_molecule.f:13 */
+int smmp_init_molecule(smmp_ctx *ctx, int iabin, const char *grpn, const char *grpc,
+                       const char *seqfile, const char *varfile);
+/* setvar(nml,vlvrx) smmp.pyf:949, setvar.f:13: host rebuild of molecule nml */
+int smmp_setvar(smmp_ctx *ctx, int nml, const double *vlvrx);
+/* energy() smmp.pyf:380, energy.f:12: GPU evaluation of the current vlvr;
+ * also sets eyel/eyvw/eyhb/eyvr/eysm/eysl and the coordinates */
+int smmp_energy(smmp_ctx *ctx, double *e);
+/* enysol(nmol) smmp.pyf:664, enysol.f:13 (solvent term of the current coordinates) */
+int smmp_enysol(smmp_ctx *ctx, int nmol, double *e);
+/* sgrnd(seed) smmp.pyf:5620, twister.f:64; grnd() twister.f:104 */
+int smmp_sgrnd(smmp_ctx *ctx, int32_t seed);
+int smmp_grnd(smmp_ctx *ctx, double *u);
+/* outvar(nml,file) outvar.f:12 ('' = stdout) */
+int smmp_outvar(smmp_ctx *ctx, int nml, const char *path);
+
+/* Weight of an energy in a Metropolis step (replaces the `dummy` callback,
+ * smmp.pyf:3-10): canonical can_weight (canon.f:135-149), muca_weight /
+ * muca_weight2 (mulcan_par_mod.f90:380-412). */
+typedef struct {
+  int32_t kind;      /* 0 = beta*x, 1 = (beta+b(k))x+alpha(k), 2 = b(k)x+alpha(k) */
+  double beta;
+  int32_t kmin, kmax;
+  double ebin;
+  const double *b, *alpha; /* kmax-kmin+1 values, may be NULL for kind 0 */
+} smmp_weight;
+
+/* metropolis(eol,acz,dummy) smmp.pyf:22, metropolis.f:13: one sweep on the
+ * GPU with the context's MT stream; eol/acz in,out. */
+int smmp_metropolis(smmp_ctx *ctx, double *eol, double *acz, const smmp_weight *w);
+
+/* partem_s(num_rep,nequi,nswp,nmes,nsave,newsta,switch) partem_s.f:13 with a
+ * new start (newsta) and random initial dihedrals (switch=1).  temps: R
+ * temperatures.  Serial-compatible mode: every replica draws from the
+ * context's single MT stream in slot order, as the reference does.
+ * Outputs (may be NULL): eol[R] per replica, ipoi[R] slot->replica (1-based),
+ * traj[nswp*R] energy per sweep and slot. */
+int smmp_partem_s(smmp_ctx *ctx, int num_rep, int nequi, int nswp, const double *temps,
+                  double *eol, int32_t *ipoi, double *traj);
+
+/* ---- batched entry points (no reference counterpart: the configs'
+ *      many-replica and many-conformation workloads) ------------------- */
+
+/* Energies of many conformations of one or several systems in one launch.
+ * ctxs[s] are initialised contexts; conf_sys[c] picks the system of
+ * conformation c; vlvr holds for each conformation the values of ALL
+ * variables of its system (offsets: sum of nvt of the previous ones).
+ * precision: 0 = FP64 throughout, 1 = FP32 pair terms with FP64
+ * accumulation and FP64 chain rebuild.
+ * out[c*8 + k] = total, eysm, eyel, eyvw, eyhb, eyvr, eysl, asa. */
+int smmp_energy_batch(smmp_ctx *const *ctxs, int nsys, const int32_t *conf_sys, int nconf,
+                      const double *vlvr, int precision, double *out);
+
+/* Independent-stream replica sampler: R replicas of the context's system,
+ * each with its own twister.f MT19937 stream seeded sgrnd(seeds[r]).
+ *   mode 0: canonical at temps[r]; mode 1: parallel tempering over temps
+ *   (exchange attempt after every sweep, partem_s.f:209-232 rule, exchange
+ *   draws from the stream of the lower slot); mode 2: multicanonical with
+ *   the weight w (muca_weight2) shared by all replicas.
+ * vlvr[r*nvt..] in: start conformations (all variables), out: final.
+ * eol[r] out: final energies; acc[r] out: accepted moves; ipoi[R] out.
+ * hist (mode 2, may be NULL): kmax-kmin+1 int32 counts of sweep-end bins.
+ * traj (may be NULL): nsweep*R energies at sweep ends, slot order.
+ * Returns SMMP_EDRIFT if a sweep-end recomputation differs by > 1e-6. */
+typedef struct {
+  int32_t nrep, nsweep, mode, precision;
+  const double *temps;
+  const uint32_t *seeds;
+  const smmp_weight *w;
+  int32_t resync_every;   /* full recomputation cadence in sweeps (1 = every sweep) */
+} smmp_sampler_cfg;
+int smmp_sample(smmp_ctx *ctx, const smmp_sampler_cfg *cfg, double *vlvr, double *eol,
+                double *acc, int32_t *ipoi, int32_t *hist, double *traj);
+
+/* Device / build information string (arch, kernels, tile shapes). */
+const char *smmp_build_info(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,370 @@
+! Fixture driver for the SMMP Fortran reference (test infrastructure only).
+!
+! Linked against the reference's own objects, built from the sources where
+! they lie (oracle/Makefile); nothing of the reference is copied here.  Each
+! mode prints '@name type count' headers followed by one value per line so
+! tests/golden/make_golden.py can parse the output.
+!
+! Modes (first command-line argument):
+!   mt     seed n                                  MT19937 stream (twister.f)
+!   topo   libdir seqfile varfile itysol           topology, lists, tables
+!   energy libdir seqfile varfile itysol nconf seed  random conformations
+!   canon  libdir seqfile varfile itysol seed temp nsweep  Metropolis run
+!   partem libdir seqfile itysol seed nrep nequi nswp      partem_s run
+!   muca   libdir seqfile itysol seed nsweep nup temp kmin kmax ebin
+program ref_driver
+  use multicanonical
+  include 'INCL.H'
+  double precision energy, grnd, addang
+  external can_weight
+  character(len=256) :: mode, arg
+  character(len=80) :: libdir, seqf, varf
+  character(len=4) :: grpn, grpc
+  integer :: iseed, n, i, j, k, nconf, nsw, isol, nrep, nequi, nswp
+  integer :: kmn, kmx, nup
+  double precision :: e, eolx, acz, tmp, ebw
+  double precision :: ggasc
+  parameter (ggasc = 1.98773d-3)
+
+  call get_command_argument(1, mode)
+  grpn = 'nh2 '
+  grpc = 'cooh'
+
+  select case (trim(mode))
+  case ('mt')
+    call iarg(2, iseed)
+    call iarg(3, n)
+    call sgrnd(iseed)
+    write(*,'(a,i0)') '@mt double ', n
+    do i = 1, n
+      write(*,'(es25.17)') grnd()
+    end do
+
+  case ('topo')
+    call setup(2, .true.)
+    call dump_params()
+    call dump_topo()
+    e = energy()
+    call dump_energy(e)
+    call dump_coords()
+
+  case ('energy')
+    call setup(2, .true.)
+    call iarg(6, nconf)
+    call iarg(7, iseed)
+    call sgrnd(iseed)
+    write(*,'(a,i0)') '@nconf int 1'
+    write(*,'(i0)') nconf
+    do k = 1, nconf
+      do i = 1, nvr
+        j = idvr(i)
+        vlvr(j) = addang(pi, axvr(j)*(grnd()-0.5))
+      end do
+      e = energy()
+      write(*,'(a,i0)') '@vlvr double ', nvr
+      do i = 1, nvr
+        write(*,'(es25.17)') vlvr(idvr(i))
+      end do
+      call dump_energy(e)
+      if (k .le. 2) call dump_coords()
+    end do
+
+  case ('canon')
+    call setup(2, .true.)
+    call iarg(6, iseed)
+    call rarg(7, tmp)
+    call iarg(8, nsw)
+    call sgrnd(iseed)
+    beta = 1.0d0 / (tmp * ggasc)
+    eolx = energy()
+    acz = 0.0d0
+    write(*,'(a,i0)') '@canon_e0 double 1'
+    write(*,'(es25.17)') eolx
+    do k = 1, nsw
+      call metropolis(eolx, acz, can_weight)
+      write(*,'(a,i0)') '@sweep double 2'
+      write(*,'(es25.17)') eolx
+      write(*,'(es25.17)') acz
+    end do
+    write(*,'(a,i0)') '@vlvr double ', nvr
+    do i = 1, nvr
+      write(*,'(es25.17)') vlvr(idvr(i))
+    end do
+    write(*,'(a,i0)') '@mti int 1'
+    write(*,'(i0)') mtstate_pos()
+
+  case ('partem')
+    call setup(2, .false.)
+    call iarg(5, iseed)
+    call iarg(6, nrep)
+    call iarg(7, nequi)
+    call iarg(8, nswp)
+    call sgrnd(iseed)
+    call partem_s(nrep, nequi, nswp, 1, nswp, .true., 1)
+    write(*,'(a,i0)') '@done int 1'
+    write(*,'(i0)') 1
+
+  case ('muca')
+    call setup(2, .false.)
+    call iarg(5, iseed)
+    call iarg(6, nsw)
+    call iarg(7, nup)
+    call rarg(8, tmp)
+    call iarg(9, kmn)
+    call iarg(10, kmx)
+    call rarg(11, ebw)
+    call sgrnd(iseed)
+    call mulcan_par(nsw, nup, tmp, kmn, kmx, ebw, .false.)
+    write(*,'(a,i0)') '@done int 1'
+    write(*,'(i0)') 1
+
+  case default
+    write(*,*) 'unknown mode ', trim(mode)
+    stop 2
+  end select
+
+contains
+
+  subroutine iarg(pos, val)
+    integer, intent(in) :: pos
+    integer, intent(out) :: val
+    character(len=256) :: s
+    call get_command_argument(pos, s)
+    read(s, *) val
+  end subroutine
+
+  subroutine rarg(pos, val)
+    integer, intent(in) :: pos
+    double precision, intent(out) :: val
+    character(len=256) :: s
+    call get_command_argument(pos, s)
+    read(s, *) val
+  end subroutine
+
+  ! ECEPP/3 set-up as universe.py does it, then init_molecule(1,'nh2','cooh').
+  subroutine setup(pos, hasvar)
+    integer, intent(in) :: pos
+    logical, intent(in) :: hasvar
+    character(len=256) :: s
+    character(len=80) :: nov
+    integer :: p
+    p = pos
+    call get_command_argument(p, s);     libdir = s
+    call get_command_argument(p+1, s);   seqf = s
+    if (hasvar) then
+      call get_command_argument(p+2, s); varf = s
+      if (trim(varf) .eq. '-') varf = ' '
+      call get_command_argument(p+3, s); read(s,*) isol
+    else
+      varf = ' '
+      call get_command_argument(p+2, s); read(s,*) isol
+    end if
+    flex = .false.
+    sh2 = .false.
+    epsd = .false.
+    ientyp = 0
+    tesgrd = .false.
+    itysol = isol
+    isolscl = .false.
+    upchswitch = 0
+    rndord = .false.
+    bgsprob = 0.0d0
+    ntlml = 0
+    call init_energy(libdir)
+    nov = ' '
+    call init_molecule(1, grpn, grpc, seqf, nov)
+    if (len_trim(varf) .gt. 0) call apply_vars(varf)
+  end subroutine
+
+  ! Applies 'index value_deg fixed' lines (variable index over all variables,
+  ! angle in degrees, fixed flag 0/1), then rebuilds and recounts the free
+  ! variables the way init_molecule.f:112-131 does after redvar.
+  subroutine apply_vars(fname)
+    character(len=*), intent(in) :: fname
+    integer :: iv, ifx, ios, a, nvt
+    double precision :: vdeg
+    open(77, file=trim(fname), status='old')
+    do
+      read(77, *, iostat=ios) iv, vdeg, ifx
+      if (ios .ne. 0) exit
+      vlvr(iv) = vdeg * cdr
+      fxvr(iv) = (ifx .ne. 0)
+    end do
+    close(77)
+    call setvar(1, vlvr)
+    nvt = ivrml1(ntlml) + nvrml(ntlml) - 1
+    nvr = 0
+    do a = 1, nvt
+      if (.not. fxvr(a)) then
+        nvr = nvr + 1
+        idvr(nvr) = a
+      end if
+      vlvr(a) = toat(iatvr(a))
+      olvlvr(a) = vlvr(a)
+      if (fxvr(a)) axvr(a) = 0.0d0
+    end do
+  end subroutine
+
+  integer function mtstate_pos()
+    integer :: mtl(0:623), mtil
+    common /myblock/ mtil, mtl
+    mtstate_pos = mtil
+  end function
+
+  subroutine dump_params()
+    integer :: a, b
+    write(*,'(a,i0)') '@conv double 1'
+    write(*,'(es25.17)') conv
+    write(*,'(a,i0)') '@cij double ', mxtyat*mxtyat
+    write(*,'(es25.17)') ((cij(a,b), b=1,mxtyat), a=1,mxtyat)
+    write(*,'(a,i0)') '@aij double ', mxtyat*mxtyat
+    write(*,'(es25.17)') ((aij(a,b), b=1,mxtyat), a=1,mxtyat)
+    write(*,'(a,i0)') '@a14 double ', mxtyat*mxtyat
+    write(*,'(es25.17)') ((a14(a,b), b=1,mxtyat), a=1,mxtyat)
+    write(*,'(a,i0)') '@chb double ', mxtyat*mxtyat
+    write(*,'(es25.17)') ((chb(a,b), b=1,mxtyat), a=1,mxtyat)
+    write(*,'(a,i0)') '@ahb double ', mxtyat*mxtyat
+    write(*,'(es25.17)') ((ahb(a,b), b=1,mxtyat), a=1,mxtyat)
+    write(*,'(a,i0)') '@ihbty int ', mxtyat*mxtyat
+    write(*,'(i0)') ((ihbty(a,b), b=1,mxtyat), a=1,mxtyat)
+    write(*,'(a,i0)') '@e0to double ', mxtyto
+    write(*,'(es25.17)') (e0to(a), a=1,mxtyto)
+    write(*,'(a,i0)') '@sgto double ', mxtyto
+    write(*,'(es25.17)') (sgto(a), a=1,mxtyto)
+    write(*,'(a,i0)') '@rnto double ', mxtyto
+    write(*,'(es25.17)') (rnto(a), a=1,mxtyto)
+    write(*,'(a,i0)') '@npnt int 1'
+    write(*,'(i0)') npnt
+    if (npnt .gt. 0) then
+      write(*,'(a,i0)') '@spoint double ', 3*npnt
+      write(*,'(es25.17)') ((spoint(a,b), b=1,3), a=1,npnt)
+    end if
+  end subroutine
+
+  subroutine dump_topo()
+    integer :: nat, nvt, nms, a, b
+    nat = iatrs2(irsml2(ntlml))
+    nvt = ivrml1(ntlml) + nvrml(ntlml) - 1
+    nms = imsml1(ntlml) + nmsml(ntlml) - 1
+    write(*,'(a,i0)') '@ntlml int 1'
+    write(*,'(i0)') ntlml
+    write(*,'(a,i0)') '@nres int 1'
+    write(*,'(i0)') irsml2(ntlml)
+    write(*,'(a,i0)') '@seq str ', irsml2(ntlml)
+    write(*,'(a)') (seq(a), a=1,irsml2(ntlml))
+    write(*,'(a,i0)') '@iatrs1 int ', irsml2(ntlml)
+    write(*,'(i0)') (iatrs1(a), a=1,irsml2(ntlml))
+    write(*,'(a,i0)') '@iatrs2 int ', irsml2(ntlml)
+    write(*,'(i0)') (iatrs2(a), a=1,irsml2(ntlml))
+    write(*,'(a,i0)') '@ixatrs int ', irsml2(ntlml)
+    write(*,'(i0)') (ixatrs(a), a=1,irsml2(ntlml))
+    write(*,'(a,i0)') '@ivrrs1 int ', irsml2(ntlml)
+    write(*,'(i0)') (ivrrs1(a), a=1,irsml2(ntlml))
+    write(*,'(a,i0)') '@nvrrs int ', irsml2(ntlml)
+    write(*,'(i0)') (nvrrs(a), a=1,irsml2(ntlml))
+    write(*,'(a,i0)') '@nat int 1'
+    write(*,'(i0)') nat
+    write(*,'(a,i0)') '@nmat str ', nat
+    write(*,'(a)') (nmat(a), a=1,nat)
+    write(*,'(a,i0)') '@ityat int ', nat
+    write(*,'(i0)') (ityat(a), a=1,nat)
+    write(*,'(a,i0)') '@iowat int ', nat
+    write(*,'(i0)') (iowat(a), a=1,nat)
+    write(*,'(a,i0)') '@iyowat int ', nat
+    write(*,'(i0)') (iyowat(a), a=1,nat)
+    write(*,'(a,i0)') '@nbdat int ', nat
+    write(*,'(i0)') (nbdat(a), a=1,nat)
+    write(*,'(a,i0)') '@ibdat int ', 3*nat
+    write(*,'(i0)') ((ibdat(b,a), b=1,3), a=1,nat)
+    write(*,'(a,i0)') '@iybdat int ', 3*nat
+    write(*,'(i0)') ((iybdat(b,a), b=1,3), a=1,nat)
+    write(*,'(a,i0)') '@cgat double ', nat
+    write(*,'(es25.17)') (cgat(a), a=1,nat)
+    write(*,'(a,i0)') '@blat double ', nat
+    write(*,'(es25.17)') (blat(a), a=1,nat)
+    write(*,'(a,i0)') '@baat double ', nat
+    write(*,'(es25.17)') (baat(a), a=1,nat)
+    write(*,'(a,i0)') '@toat double ', nat
+    write(*,'(es25.17)') (toat(a), a=1,nat)
+    write(*,'(a,i0)') '@ixmsat int ', nat
+    write(*,'(i0)') (ixmsat(a), a=1,nat)
+    write(*,'(a,i0)') '@sigma double ', nat
+    write(*,'(es25.17)') (sigma(a), a=1,nat)
+    write(*,'(a,i0)') '@rvdw double ', nat
+    write(*,'(es25.17)') (rvdw(a), a=1,nat)
+    write(*,'(a,i0)') '@nvt int 1'
+    write(*,'(i0)') nvt
+    write(*,'(a,i0)') '@nmvr str ', nvt
+    write(*,'(a)') (nmvr(a), a=1,nvt)
+    write(*,'(a,i0)') '@iatvr int ', nvt
+    write(*,'(i0)') (iatvr(a), a=1,nvt)
+    write(*,'(a,i0)') '@ityvr int ', nvt
+    write(*,'(i0)') (ityvr(a), a=1,nvt)
+    write(*,'(a,i0)') '@iclvr int ', nvt
+    write(*,'(i0)') (iclvr(a), a=1,nvt)
+    write(*,'(a,i0)') '@iorvr int ', nvt
+    write(*,'(i0)') (iorvr(a), a=1,nvt)
+    write(*,'(a,i0)') '@imsvr1 int ', nvt
+    write(*,'(i0)') (imsvr1(a), a=1,nvt)
+    write(*,'(a,i0)') '@imsvr2 int ', nvt
+    write(*,'(i0)') (imsvr2(a), a=1,nvt)
+    write(*,'(a,i0)') '@iadvr1 int ', nvt
+    write(*,'(i0)') (iadvr1(a), a=1,nvt)
+    write(*,'(a,i0)') '@iadvr2 int ', nvt
+    write(*,'(i0)') (iadvr2(a), a=1,nvt)
+    write(*,'(a,i0)') '@nadml int 1'
+    write(*,'(i0)') nadml(ntlml)
+    write(*,'(a,i0)') '@ladvr int ', nadml(ntlml)
+    write(*,'(i0)') (ladvr(a), a=1,nadml(ntlml))
+    write(*,'(a,i0)') '@fxvr int ', nvt
+    write(*,'(i0)') (merge(1,0,fxvr(a)), a=1,nvt)
+    write(*,'(a,i0)') '@axvr double ', nvt
+    write(*,'(es25.17)') (axvr(a), a=1,nvt)
+    write(*,'(a,i0)') '@vlvr_all double ', nvt
+    write(*,'(es25.17)') (vlvr(a), a=1,nvt)
+    write(*,'(a,i0)') '@nvr int 1'
+    write(*,'(i0)') nvr
+    write(*,'(a,i0)') '@idvr int ', nvr
+    write(*,'(i0)') (idvr(a), a=1,nvr)
+    write(*,'(a,i0)') '@nms int 1'
+    write(*,'(i0)') nms
+    write(*,'(a,i0)') '@latms1 int ', nms
+    write(*,'(i0)') (latms1(a), a=1,nms)
+    write(*,'(a,i0)') '@latms2 int ', nms
+    write(*,'(i0)') (latms2(a), a=1,nms)
+    write(*,'(a,i0)') '@ivwat1 int ', nat
+    write(*,'(i0)') (ivwat1(a), a=1,nat)
+    write(*,'(a,i0)') '@ivwat2 int ', nat
+    write(*,'(i0)') (ivwat2(a), a=1,nat)
+    write(*,'(a,i0)') '@nvw int 1'
+    write(*,'(i0)') nvwml(ntlml) + ivwml1(ntlml) - 1
+    write(*,'(a,i0)') '@lvwat1 int ', nvwml(ntlml) + ivwml1(ntlml) - 1
+    write(*,'(i0)') (lvwat1(a), a=1,nvwml(ntlml) + ivwml1(ntlml) - 1)
+    write(*,'(a,i0)') '@lvwat2 int ', nvwml(ntlml) + ivwml1(ntlml) - 1
+    write(*,'(i0)') (lvwat2(a), a=1,nvwml(ntlml) + ivwml1(ntlml) - 1)
+    write(*,'(a,i0)') '@i14at1 int ', nat
+    write(*,'(i0)') (i14at1(a), a=1,nat)
+    write(*,'(a,i0)') '@i14at2 int ', nat
+    write(*,'(i0)') (i14at2(a), a=1,nat)
+    write(*,'(a,i0)') '@n14 int 1'
+    write(*,'(i0)') n14ml(ntlml) + i14ml1(ntlml) - 1
+    write(*,'(a,i0)') '@l14at int ', n14ml(ntlml) + i14ml1(ntlml) - 1
+    write(*,'(i0)') (l14at(a), a=1,n14ml(ntlml) + i14ml1(ntlml) - 1)
+  end subroutine
+
+  subroutine dump_energy(etot)
+    double precision, intent(in) :: etot
+    write(*,'(a,i0)') '@energy double 7'
+    write(*,'(es25.17)') etot, eysm, eyel, eyvw, eyhb, eyvr, eysl
+    write(*,'(a,i0)') '@asa double 2'
+    write(*,'(es25.17)') asa, vdvol
+  end subroutine
+
+  subroutine dump_coords()
+    integer :: nat, a
+    nat = iatrs2(irsml2(ntlml))
+    write(*,'(a,i0)') '@xyz double ', 3*nat
+    write(*,'(es25.17)') (xat(a), yat(a), zat(a), a=1,nat)
+  end subroutine
+
+end program ref_driver

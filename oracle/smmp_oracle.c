@@ -1,0 +1,576 @@
+/*
+ * SMMP oracle (test infrastructure only; see smmp_oracle.h).
+ *
+ * A line-by-line-faithful FP64 restatement of the reference's arithmetic, in
+ * its evaluation order, so results agree with the reference objects to the
+ * last bits where the compilers round alike.  Built with -ffp-contract=off.
+ * Each function names the reference routine it restates.
+ */
+#include "smmp_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define PI 3.141592653589793
+#define PI2 (2.0 * PI)
+#define GASC 0.00198773
+
+/* ------------------------------------------------ twister.f:64-179 */
+
+void oracle_sgrnd(oracle_mt *g, int32_t seed) {
+  g->mt[0] = (uint32_t)seed;
+  for (int i = 1; i < 624; ++i) g->mt[i] = 69069u * g->mt[i - 1];
+  g->mti = 624;
+}
+
+uint32_t oracle_grnd_u32(oracle_mt *g) {
+  if (g->mti >= 624) {
+    if (g->mti == 625) oracle_sgrnd(g, 4357);
+    static const uint32_t mag01[2] = {0u, 0x9908B0DFu};
+    int kk;
+    uint32_t y;
+    for (kk = 0; kk < 624 - 397; ++kk) {
+      y = (g->mt[kk] & 0x80000000u) | (g->mt[kk + 1] & 0x7fffffffu);
+      g->mt[kk] = g->mt[kk + 397] ^ (y >> 1) ^ mag01[y & 1u];
+    }
+    for (; kk < 623; ++kk) {
+      y = (g->mt[kk] & 0x80000000u) | (g->mt[kk + 1] & 0x7fffffffu);
+      g->mt[kk] = g->mt[kk + (397 - 624)] ^ (y >> 1) ^ mag01[y & 1u];
+    }
+    y = (g->mt[623] & 0x80000000u) | (g->mt[0] & 0x7fffffffu);
+    g->mt[623] = g->mt[396] ^ (y >> 1) ^ mag01[y & 1u];
+    g->mti = 0;
+  }
+  uint32_t y = g->mt[g->mti++];
+  y ^= y >> 11;
+  y ^= (y << 7) & 0x9D2C5680u;
+  y ^= (y << 15) & 0xEFC60000u;
+  y ^= y >> 18;
+  return y;
+}
+
+double oracle_grnd(oracle_mt *g) { return (double)oracle_grnd_u32(g) / 4294967295.0; }
+
+/* ------------------------------------------------ difang.f:12-63 */
+
+double oracle_difang(double a1, double a2) {
+  double d = fmod(a2 - a1, PI2);
+  return fabs(d) <= PI ? d : d - copysign(PI2, d);
+}
+double oracle_addang(double a1, double a2) {
+  double d = fmod(a1 + a2, PI2);
+  return fabs(d) <= PI ? d : d - copysign(PI2, d);
+}
+
+/* ------------------------------------------------ conformation memory */
+
+int oracle_conf_alloc(oracle_conf *c, int nat) {
+  memset(c, 0, sizeof *c);
+  double **f[] = {&c->toat, &c->sntoat, &c->cstoat, &c->snbaat, &c->csbaat, &c->xat, &c->yat,
+                  &c->zat, &c->xtoat, &c->ytoat, &c->ztoat, &c->xbaat, &c->ybaat, &c->zbaat};
+  for (size_t k = 0; k < sizeof f / sizeof f[0]; ++k) {
+    *f[k] = (double *)calloc((size_t)nat + 1, sizeof(double));
+    if (!*f[k]) return -1;
+  }
+  return 0;
+}
+
+void oracle_conf_free(oracle_conf *c) {
+  double *f[] = {c->toat, c->sntoat, c->cstoat, c->snbaat, c->csbaat, c->xat, c->yat,
+                 c->zat, c->xtoat, c->ytoat, c->ztoat, c->xbaat, c->ybaat, c->zbaat};
+  for (size_t k = 0; k < sizeof f / sizeof f[0]; ++k) free(f[k]);
+  memset(c, 0, sizeof *c);
+}
+
+void oracle_conf_init(const oracle_topo *t, oracle_conf *c) {
+  for (int i = 0; i < t->nat; ++i) {
+    c->toat[i] = t->toat[i];
+    c->sntoat[i] = sin(t->toat[i]);
+    c->cstoat[i] = cos(t->toat[i]);
+    c->snbaat[i] = sin(t->baat[i]);
+    c->csbaat[i] = cos(t->baat[i]);
+  }
+}
+
+/* ------------------------------------------------ bldmol.f */
+
+#define A(p, i) ((p)[(i) - 1])
+
+/* bldmol.f:175-242 */
+static void eyring(const oracle_topo *t, oracle_conf *c, int i, int ia) {
+  double ct = A(c->cstoat, i), st = A(c->sntoat, i), ca = A(c->csbaat, i), sa = A(c->snbaat, i);
+  double bl = A(t->blat, i);
+  double x1 = A(c->xtoat, ia), x2 = A(c->ytoat, ia), x3 = A(c->ztoat, ia);
+  double z1 = A(c->xbaat, ia), z2 = A(c->ybaat, ia), z3 = A(c->zbaat, ia);
+  double y1 = z2 * x3 - z3 * x2, y2 = z3 * x1 - z1 * x3, y3 = z1 * x2 - z2 * x1;
+  double h2 = -sa * ct, h3 = -sa * st;
+  x1 = -ca * x1 + h2 * y1 + h3 * z1;
+  x2 = -ca * x2 + h2 * y2 + h3 * z2;
+  x3 = -ca * x3 + h2 * y3 + h3 * z3;
+  double dx = 1.0 / sqrt(x1 * x1 + x2 * x2 + x3 * x3);
+  x1 = x1 * dx;
+  x2 = x2 * dx;
+  x3 = x3 * dx;
+  A(c->xtoat, i) = x1;
+  A(c->ytoat, i) = x2;
+  A(c->ztoat, i) = x3;
+  z1 = -st * y1 + ct * z1;
+  z2 = -st * y2 + ct * z2;
+  z3 = -st * y3 + ct * z3;
+  double dz = 1.0 / sqrt(z1 * z1 + z2 * z2 + z3 * z3);
+  A(c->xbaat, i) = z1 * dz;
+  A(c->ybaat, i) = z2 * dz;
+  A(c->zbaat, i) = z3 * dz;
+  A(c->xat, i) = A(c->xat, ia) + x1 * bl;
+  A(c->yat, i) = A(c->yat, ia) + x2 * bl;
+  A(c->zat, i) = A(c->zat, ia) + x3 * bl;
+}
+
+/* bldmol.f:303-420 */
+static void setgbl(const oracle_topo *t, const oracle_conf *c, double *xg, double *zg) {
+  const double *g = t->gbpr;
+  double ca = cos(g[3]), sa = sin(g[3]), cb = cos(g[4]), sb = sin(g[4]), cg = cos(g[5]), sg = sin(g[5]);
+  double ag[3][3], d;
+  double x1 = ca * cg - sa * sb * sg, x2 = sa * cg + ca * sb * sg, x3 = -cb * sg;
+  d = sqrt(x1 * x1 + x2 * x2 + x3 * x3);
+  ag[0][0] = x1 / d; ag[1][0] = x2 / d; ag[2][0] = x3 / d;
+  double y1 = -sa * cb, y2 = ca * cb, y3 = sb;
+  d = sqrt(y1 * y1 + y2 * y2 + y3 * y3);
+  ag[0][1] = y1 / d; ag[1][1] = y2 / d; ag[2][1] = y3 / d;
+  double z1 = ca * sg + sa * sb * cg, z2 = sa * sg - ca * sb * cg, z3 = cb * cg;
+  d = sqrt(z1 * z1 + z2 * z2 + z3 * z3);
+  ag[0][2] = z1 / d; ag[1][2] = z2 / d; ag[2][2] = z3 / d;
+  int i2 = t->i2, i3 = t->i3;
+  double ct2 = A(c->cstoat, i2), st2 = A(c->sntoat, i2), sa2 = A(c->snbaat, i2);
+  x1 = -A(c->csbaat, i2); x2 = -sa2 * ct2; x3 = -sa2 * st2;
+  double dx = sqrt(x1 * x1 + x2 * x2 + x3 * x3);
+  x1 = x1 / dx; x2 = x2 / dx; x3 = x3 / dx;
+  double st3 = A(c->sntoat, i3), ct3 = A(c->cstoat, i3);
+  z1 = -st3 * (st2 * x3 + ct2 * x2);
+  z2 = st3 * ct2 * x1 + ct3 * st2;
+  z3 = st3 * st2 * x1 - ct3 * ct2;
+  double dz = sqrt(z1 * z1 + z2 * z2 + z3 * z3);
+  z1 = z1 / dz; z2 = z2 / dz; z3 = z3 / dz;
+  y1 = z2 * x3 - z3 * x2;
+  y3 = z1 * x2 - z2 * x1;
+  for (int i = 0; i < 3; ++i) {
+    xg[i] = ag[i][0] * x1 + ag[i][1] * y1 + ag[i][2] * z1;
+    zg[i] = ag[i][0] * x3 + ag[i][1] * y3 + ag[i][2] * z3;
+  }
+  dx = sqrt(xg[0] * xg[0] + xg[1] * xg[1] + xg[2] * xg[2]);
+  dz = sqrt(zg[0] * zg[0] + zg[1] * zg[1] + zg[2] * zg[2]);
+  for (int i = 0; i < 3; ++i) {
+    xg[i] = xg[i] / dx;
+    zg[i] = zg[i] / dz;
+  }
+}
+
+/* setvar.f:13-73 followed by bldmol.f:13-93 (one chain starting at atom 1) */
+void oracle_setvar(const oracle_topo *t, oracle_conf *c, const double *vlvr) {
+  for (int i = 1; i <= t->nvt; ++i) {
+    int iat = A(t->iatvr, i), ity = A(t->ityvr, i);
+    if (ity != 3) continue; /* the ECEPP/3 library defines torsion variables only */
+    double tsh = oracle_difang(A(c->toat, iat), vlvr[i - 1]);
+    int iow = A(t->iowat, iat);
+    for (int j = 1; j <= A(t->nbdat, iow); ++j) {
+      int jat = t->ibdat[(j - 1) + 3 * (iow - 1)];
+      if (A(t->iowat, jat) == iow) {
+        double to = A(c->toat, jat) + tsh;
+        if (fabs(to) > PI) to = to - copysign(PI2, to);
+        A(c->toat, jat) = to;
+        A(c->sntoat, jat) = sin(to);
+        A(c->cstoat, jat) = cos(to);
+      }
+    }
+  }
+  double xg[3], zg[3];
+  A(c->xat, 1) = t->gbpr[0];
+  A(c->yat, 1) = t->gbpr[1];
+  A(c->zat, 1) = t->gbpr[2];
+  setgbl(t, c, xg, zg);
+  A(c->xbaat, 1) = zg[0]; A(c->ybaat, 1) = zg[1]; A(c->zbaat, 1) = zg[2];
+  A(c->xtoat, 1) = xg[0]; A(c->ytoat, 1) = xg[1]; A(c->ztoat, 1) = xg[2];
+  for (int j = 2; j <= t->nat; ++j) eyring(t, c, j, A(t->iowat, j));
+}
+
+/* ------------------------------------------------ enyshe.f:13-195 */
+
+static inline double dielectric(const oracle_topo *t, double rij) {
+  if (!t->epsd) return 1.0;
+  double sr = t->slp * rij;
+  return t->plt - (sr * sr + 2.0 * sr + 2.0) * (t->plt - 1.0) * exp(-sr) / 2.0;
+}
+
+double oracle_enyshe(const oracle_topo *t, oracle_conf *c) {
+  double eyel = 0.0, eyvw = 0.0, eyhb = 0.0, eyvr = 0.0;
+  int i1s = t->nms + 1;
+  for (int io = t->nvt; io >= 1; --io) {
+    int iv = A(t->iorvr, io), ia = A(t->iatvr, iv), it = A(t->ityvr, iv), ic = A(t->iclvr, iv);
+    if (it == 3) {
+      double e0 = A(t->e0to, ic);
+      if (e0 != 0.0) eyvr = eyvr + e0 * (1.0 + A(t->sgto, ic) * cos(A(c->toat, ia) * A(t->rnto, ic)));
+    }
+    int i2s = i1s - 1;
+    i1s = A(t->imsvr1, iv);
+    for (int ims = i1s; ims <= i2s; ++ims) {
+      for (int i = A(t->latms1, ims); i <= A(t->latms2, ims); ++i) {
+        int ity = A(t->ityat, i);
+        double cqi = t->conv * A(t->cgat, i);
+        double xi = A(c->xat, i), yi = A(c->yat, i), zi = A(c->zat, i);
+        for (int ivw = A(t->ivwat1, i); ivw <= A(t->ivwat2, i); ++ivw) {
+          for (int j = A(t->lvwat1, ivw); j <= A(t->lvwat2, ivw); ++j) {
+            int jty = A(t->ityat, j);
+            int k = (ity - 1) + 18 * (jty - 1);
+            double xij = A(c->xat, j) - xi, yij = A(c->yat, j) - yi, zij = A(c->zat, j) - zi;
+            double rij2 = xij * xij + yij * yij + zij * zij;
+            double rij4 = rij2 * rij2, rij6 = rij4 * rij2, rij = sqrt(rij2);
+            double ep = dielectric(t, rij);
+            eyel = eyel + cqi * A(t->cgat, j) / (rij * ep);
+            if (t->ihbty[k] == 0)
+              eyvw = eyvw + t->aij[k] / (rij6 * rij6) - t->cij[k] / rij6;
+            else
+              eyhb = eyhb + t->ahb[k] / (rij6 * rij6) - t->chb[k] / (rij6 * rij4);
+          }
+        }
+        for (int i14 = A(t->i14at1, i); i14 <= A(t->i14at2, i); ++i14) {
+          int j = A(t->l14at, i14);
+          int jty = A(t->ityat, j);
+          int k = (ity - 1) + 18 * (jty - 1);
+          double xij = A(c->xat, j) - xi, yij = A(c->yat, j) - yi, zij = A(c->zat, j) - zi;
+          double rij2 = xij * xij + yij * yij + zij * zij;
+          double rij4 = rij2 * rij2, rij6 = rij4 * rij2, rij = sqrt(rij2);
+          double ep = dielectric(t, rij);
+          eyel = eyel + cqi * A(t->cgat, j) / (rij * ep);
+          if (t->ihbty[k] == 0)
+            eyvw = eyvw + t->a14[k] / (rij6 * rij6) - t->cij[k] / rij6;
+          else
+            eyhb = eyhb + t->ahb[k] / (rij6 * rij6) - t->chb[k] / (rij6 * rij4);
+        }
+      }
+    }
+  }
+  double eysm = eyel + eyvw + eyhb + eyvr;
+  c->e[1] = eysm;
+  c->e[2] = eyel;
+  c->e[3] = eyvw;
+  c->e[4] = eyhb;
+  c->e[5] = eyvr;
+  return eysm;
+}
+
+/* ------------------------------------------------ enysol.f:13-339 */
+
+double oracle_enysol(const oracle_topo *t, oracle_conf *c) {
+  const int nlow = 1, nup = t->nat, numat = nup - nlow + 1;
+  double asa = 0.0, vdvol = 0.0, eysl = 0.0;
+  double avr_x = A(c->xat, nlow), avr_y = A(c->yat, nlow), avr_z = A(c->zat, nlow);
+  double xmin = avr_x, ymin = avr_y, zmin = avr_z, xmax = xmin, ymax = ymin, zmax = zmin;
+  double rmax = A(t->rvdw, nlow);
+  for (int j = nlow + 1; j <= nup; ++j) {
+    double x = A(c->xat, j), y = A(c->yat, j), z = A(c->zat, j);
+    if (x <= xmin) xmin = x; else if (x >= xmax) xmax = x;
+    avr_x = avr_x + x;
+    if (y <= ymin) ymin = y; else if (y >= ymax) ymax = y;
+    avr_y = avr_y + y;
+    if (z <= zmin) zmin = z; else if (z >= zmax) zmax = z;
+    avr_z = avr_z + z;
+    if (A(t->rvdw, j) >= rmax) rmax = A(t->rvdw, j);
+  }
+  avr_x = avr_x / (double)numat;
+  avr_y = avr_y / (double)numat;
+  avr_z = avr_z / (double)numat;
+  double diamax = 2.0 * rmax;
+  double sizex = xmax - xmin, sizey = ymax - ymin, sizez = zmax - zmin;
+  int ndx = (int)(sizex / diamax + 1), ndy = (int)(sizey / diamax + 1), ndz = (int)(sizez / diamax + 1);
+  int nqxy = ndx * ndy, ncbox = nqxy * ndz;
+  if (ncbox >= 80000) return NAN; /* the reference stops here */
+  double shiftx = ((double)ndx * diamax - sizex) / 2.0;
+  double shifty = ((double)ndy * diamax - sizey) / 2.0;
+  double shiftz = ((double)ndz * diamax - sizez) / 2.0;
+  xmin = xmin - shiftx;
+  ymin = ymin - shifty;
+  zmin = zmin - shiftz;
+  int *inbox = (int *)calloc((size_t)ncbox + 2, sizeof(int));
+  int *numbox = (int *)calloc((size_t)numat + 1, sizeof(int));
+  int *indsort = (int *)calloc((size_t)numat + 1, sizeof(int));
+  int *look = (int *)calloc((size_t)numat + 1, sizeof(int));
+  double *xyz = (double *)calloc(3 * ((size_t)numat + 1), sizeof(double));
+  double *radb2 = (double *)calloc((size_t)numat + 1, sizeof(double));
+  int npnt = t->npnt;
+  char *surfc = (char *)calloc((size_t)npnt + 1, 1);
+  for (int j = nlow; j <= nup; ++j) {
+    double fx = (A(c->xat, j) - xmin) / diamax, fy = (A(c->yat, j) - ymin) / diamax, fz = (A(c->zat, j) - zmin) / diamax;
+    int mx = (int)(fx > 0.0 ? fx : 0.0), my = (int)(fy > 0.0 ? fy : 0.0), mz = (int)(fz > 0.0 ? fz : 0.0);
+    if (mx > ndx - 1) mx = ndx - 1;
+    if (my > ndy - 1) my = ndy - 1;
+    if (mz > ndz - 1) mz = ndz - 1;
+    int nboxj = mx + my * ndx + mz * nqxy + 1;
+    numbox[j] = nboxj;
+    inbox[nboxj] += 1;
+  }
+  for (int i = 1; i <= ncbox; ++i) inbox[i + 1] = inbox[i + 1] + inbox[i];
+  for (int i = nlow; i <= nup; ++i) {
+    int j = numbox[i], jj = inbox[j];
+    indsort[jj] = i;
+    inbox[j] = jj - 1;
+  }
+  for (int iz = 0; iz <= ndz - 1; ++iz)
+    for (int iy = 0; iy <= ndy - 1; ++iy)
+      for (int ix = 0; ix <= ndx - 1; ++ix) {
+        int ibox = ix + iy * ndx + iz * nqxy + 1;
+        int lbn = inbox[ibox + 1] - inbox[ibox];
+        if (lbn <= 0) continue;
+        int nsx = ix - 1 > 0 ? ix - 1 : 0, nsy = iy - 1 > 0 ? iy - 1 : 0, nsz = iz - 1 > 0 ? iz - 1 : 0;
+        int nex = ix + 1 < ndx - 1 ? ix + 1 : ndx - 1, ney = iy + 1 < ndy - 1 ? iy + 1 : ndy - 1;
+        int nez = iz + 1 < ndz - 1 ? iz + 1 : ndz - 1;
+        int jcnt = 1;
+        for (int jz = nsz; jz <= nez; ++jz)
+          for (int jy = nsy; jy <= ney; ++jy)
+            for (int jx = nsx; jx <= nex; ++jx) {
+              int jbox = jx + jy * ndx + jz * nqxy + 1;
+              for (int ii = inbox[jbox] + 1; ii <= inbox[jbox + 1]; ++ii)
+                if (A(t->rvdw, indsort[ii]) > 0.0) look[jcnt++] = indsort[ii];
+            }
+        for (int ia = inbox[ibox] + 1; ia <= inbox[ibox + 1]; ++ia) {
+          int jbi = indsort[ia];
+          double trad = A(t->rvdw, jbi);
+          if (!(trad > 0.0)) continue;
+          int nnei = 0;
+          for (int ib = 1; ib <= jcnt - 1; ++ib) {
+            int jtk = look[ib];
+            if (jtk == jbi) continue;
+            double dx = (A(c->xat, jbi) - A(c->xat, jtk)) / trad;
+            double dy = (A(c->yat, jbi) - A(c->yat, jtk)) / trad;
+            double dz = (A(c->zat, jbi) - A(c->zat, jtk)) / trad;
+            double dd = dx * dx + dy * dy + dz * dz;
+            double akrad = A(t->rvdw, jtk) / trad;
+            double dr = 1.0 + akrad;
+            dr = dr * dr;
+            if (dd <= dr) {
+              ++nnei;
+              xyz[3 * nnei + 0] = dx;
+              xyz[3 * nnei + 1] = dy;
+              xyz[3 * nnei + 2] = dz;
+              radb2[nnei] = akrad * akrad;
+            }
+          }
+          /* a surface point is free when no neighbour sphere holds it */
+          int icount = 0;
+          double sx = 0.0, sy = 0.0, sz = 0.0;
+          for (int il = 1; il <= npnt; ++il) {
+            double px = t->spoint[il - 1], py = t->spoint[npnt + il - 1], pz = t->spoint[2 * npnt + il - 1];
+            int free_pt = 1;
+            for (int ik = 1; ik <= nnei; ++ik) {
+              double ax = xyz[3 * ik] + px, ay = xyz[3 * ik + 1] + py, az = xyz[3 * ik + 2] + pz;
+              double sdd = 0.0;
+              sdd = sdd + ax * ax;
+              sdd = sdd + ay * ay;
+              sdd = sdd + az * az;
+              if (sdd <= radb2[ik]) {
+                free_pt = 0;
+                break;
+              }
+            }
+            surfc[il] = (char)free_pt;
+          }
+          for (int il = 1; il <= npnt; ++il)
+            if (surfc[il]) {
+              ++icount;
+              sx = sx + t->spoint[il - 1];
+              sy = sy + t->spoint[npnt + il - 1];
+              sz = sz + t->spoint[2 * npnt + il - 1];
+            }
+          double sdr = 4.0 * PI * trad * trad / (double)npnt;
+          double area = sdr * (double)icount;
+          double volume = sdr / 3.0 * (trad * (double)icount + (A(c->xat, jbi) - avr_x) * sx +
+                                       (A(c->yat, jbi) - avr_y) * sy + (A(c->zat, jbi) - avr_z) * sz);
+          asa = asa + area;
+          vdvol = vdvol + volume;
+          eysl = eysl + area * A(t->sigma, jbi);
+        }
+      }
+  free(inbox); free(numbox); free(indsort); free(look); free(xyz); free(radb2); free(surfc);
+  c->e[6] = eysl;
+  c->e[7] = asa;
+  c->e[8] = vdvol;
+  return eysl;
+}
+
+/* energy.f:12-119 for a single chain (eninteract is zero) */
+double oracle_energy(const oracle_topo *t, oracle_conf *c, const double *vlvr) {
+  oracle_setvar(t, c, vlvr);
+  double esm = 0.0;
+  esm = esm + oracle_enyshe(t, c);
+  if (t->itysol > 0) {
+    esm = esm + oracle_enysol(t, c);
+  } else {
+    c->e[6] = 0.0;
+  }
+  c->e[0] = esm;
+  return esm;
+}
+
+/* ------------------------------------------------ weights */
+
+double oracle_weight_eval(const oracle_weight *w, double x) {
+  if (w->kind == 0) return w->beta * x; /* canon.f:135-149 */
+  double xmin = (double)w->kmin, xmax = (double)w->kmax;
+  double v = x / w->ebin + copysign(0.5, x);
+  if (v < xmin) v = xmin;
+  if (v > xmax) v = xmax;
+  int k = (int)v - w->kmin;
+  if (w->kind == 1) return (w->beta + w->b[k]) * x + w->alpha[k]; /* muca_weight */
+  return w->b[k] * x + w->alpha[k];                                 /* muca_weight2 */
+}
+
+/* ------------------------------------------------ metropolis.f:13-184 */
+
+int oracle_metropolis(const oracle_topo *t, oracle_conf *c, oracle_mt *g, double *vlvr,
+                      int nvr, const int32_t *idvr, const double *axvr,
+                      const oracle_weight *w, double *eol1, double *acz) {
+  double *vrol = (double *)malloc(sizeof(double) * (size_t)t->nvt);
+  double eol = oracle_energy(t, c, vlvr);
+  for (int nsw = 1; nsw <= nvr; ++nsw) {
+    int jv = idvr[nsw - 1];
+    memcpy(vrol, vlvr, sizeof(double) * (size_t)t->nvt);
+    double dv = axvr[jv - 1] * (oracle_grnd(g) - 0.5);
+    vlvr[jv - 1] = oracle_addang(vrol[jv - 1], dv);
+    double enw = oracle_energy(t, c, vlvr);
+    double delta = oracle_weight_eval(w, enw) - oracle_weight_eval(w, eol);
+    double up = 0.0;
+    if (delta <= 0.0) {
+      eol = enw;
+      up = 1.0;
+    } else {
+      double rd = oracle_grnd(g);
+      if (delta > 100.0) delta = 100.0;
+      double ex = exp(-delta);
+      if (ex >= rd) {
+        eol = enw;
+        up = 1.0;
+      } else {
+        memcpy(vlvr, vrol, sizeof(double) * (size_t)t->nvt);
+      }
+    }
+    *acz = *acz + up;
+  }
+  double enw = oracle_energy(t, c, vlvr);
+  int bad = fabs(eol - enw) > 0.000001 && eol < 100000;
+  *eol1 = eol;
+  free(vrol);
+  return bad;
+}
+
+/* ------------------------------------------------ partem_s.f:13-258 */
+
+int oracle_partem(const oracle_topo *t, oracle_mt *g, const double *vlvr0, int nvr,
+                  const int32_t *idvr, const double *axvr, int nrep, const double *temps,
+                  int nequi, int nswp, double *eol, int32_t *ipoi, double *coor, double *traj) {
+  oracle_conf c;
+  if (oracle_conf_alloc(&c, t->nat)) return -1;
+  oracle_conf_init(t, &c);
+  double *vlvr = (double *)malloc(sizeof(double) * (size_t)t->nvt);
+  double *pbe = (double *)malloc(sizeof(double) * (size_t)nrep);
+  memcpy(vlvr, vlvr0, sizeof(double) * (size_t)t->nvt);
+  for (int j = 0; j < nrep; ++j) {
+    pbe[j] = 1.0 / (temps[j] * GASC);
+    ipoi[j] = j + 1;
+  }
+  for (int i = 0; i < nrep; ++i)
+    for (int j = 0; j < nvr; ++j) {
+      int iv = idvr[j];
+      double dv = axvr[iv - 1] * (oracle_grnd(g) - 0.5);
+      coor[(size_t)i * nvr + j] = oracle_addang(PI, dv);
+    }
+  oracle_weight w = {0};
+  int bad = 0;
+  double acz = 0.0;
+  for (int k = 0; k < nrep; ++k) {
+    w.beta = pbe[k];
+    for (int i = 0; i < nvr; ++i) vlvr[idvr[i] - 1] = coor[(size_t)k * nvr + i];
+    double energ = oracle_energy(t, &c, vlvr);
+    for (int nsw = 1; nsw <= nequi; ++nsw) bad |= oracle_metropolis(t, &c, g, vlvr, nvr, idvr, axvr, &w, &energ, &acz);
+    for (int i = 0; i < nvr; ++i) coor[(size_t)k * nvr + i] = vlvr[idvr[i] - 1];
+    eol[k] = energ;
+  }
+  int iswitch = 1;
+  for (int nsw = 1; nsw <= nswp; ++nsw) {
+    for (int k1 = 1; k1 <= nrep; ++k1) {
+      int k = ipoi[k1 - 1] - 1;
+      w.beta = pbe[k1 - 1];
+      double energ = eol[k];
+      for (int i = 0; i < nvr; ++i) vlvr[idvr[i] - 1] = coor[(size_t)k * nvr + i];
+      oracle_setvar(t, &c, vlvr);
+      bad |= oracle_metropolis(t, &c, g, vlvr, nvr, idvr, axvr, &w, &energ, &acz);
+      if (traj) traj[(size_t)(nsw - 1) * nrep + (k1 - 1)] = energ;
+      for (int i = 0; i < nvr; ++i) coor[(size_t)k * nvr + i] = vlvr[idvr[i] - 1];
+      eol[k] = energ;
+    }
+    if (!(nrep > 1)) continue;
+    int num_rep1, nu;
+    if (iswitch == 1) {
+      num_rep1 = nrep - 1;
+      nu = 1;
+      iswitch = 2;
+    } else {
+      num_rep1 = nrep;
+      nu = 2;
+      iswitch = 1;
+    }
+    for (int i = nu; i <= num_rep1; i += 2) {
+      int j = i + 1;
+      if (i == nrep) j = 1;
+      int in = ipoi[i - 1], jn = ipoi[j - 1];
+      double delta = -pbe[i - 1] * eol[jn - 1] - pbe[j - 1] * eol[in - 1] + pbe[i - 1] * eol[in - 1] +
+                     pbe[j - 1] * eol[jn - 1];
+      double ra = oracle_grnd(g);
+      if (ra <= exp(delta)) {
+        ipoi[i - 1] = jn;
+        ipoi[j - 1] = in;
+      }
+    }
+  }
+  free(vlvr);
+  free(pbe);
+  oracle_conf_free(&c);
+  return bad;
+}
+
+/* ------------------------------------------------ mulcan_par_mod.f90:122-155
+ * The reference converts the counts with FLOAT() and takes LOG() of those
+ * single-precision values; the restatement does the same. */
+void oracle_muca_update(int kmin, int kmax, double ebin, double beta, const int32_t *ihist,
+                        double *xhist, double *g1, double *b, double *alpha) {
+#define H(k) ihist[(k) - kmin]
+#define X(k) xhist[(k) - kmin]
+#define G(k) g1[(k) - kmin]
+#define B(k) b[(k) - kmin]
+#define AL(k) alpha[(k) - kmin]
+  (void)beta;
+  X(kmax) = X(kmax) + H(kmax);
+  X(kmax - 1) = X(kmax - 1) + H(kmax - 1);
+  for (int i = kmax - 2; i >= kmin; --i) {
+    X(i) = X(i) + H(i);
+    if (H(i) > 1 && H(i + 1) > 1) {
+      double g0 = (double)((float)H(i + 1) * (float)H(i));
+      g0 = g0 / (double)(float)(H(i) + H(i + 1));
+      G(i) = G(i) + g0;
+      g0 = g0 / G(i);
+      double bi_1 = (double)logf((float)H(i + 1));
+      double bi = (double)logf((float)H(i));
+      B(i) = B(i) + g0 * (bi_1 - bi) / ebin;
+    } else if (X(i) < 2.0) {
+      B(i) = B(i + 1) > 0.0 ? B(i + 1) : 0.0;
+    }
+  }
+  B(kmax) = B(kmax - 2);
+  B(kmax - 1) = B(kmax - 2);
+  AL(kmax) = 0;
+  for (int i = kmax - 1; i >= kmin; --i) AL(i) = AL(i + 1) + (B(i + 1) - B(i)) * (i + 0.5) * ebin;
+#undef H
+#undef X
+#undef G
+#undef B
+#undef AL
+}

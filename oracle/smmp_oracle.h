@@ -1,0 +1,104 @@
+/*
+ * SMMP oracle: a plain-C, FP64, single-threaded restatement of the SMMP
+ * energy-and-move hot path.  TEST INFRASTRUCTURE ONLY: tests/, the smoke()
+ * check and bench.py's CPU-baseline leg may load it; the product never does.
+ *
+ * Arrays use the reference's 1-based indices stored from element 0
+ * (a(i) -> p[i-1]); 2-D tables are column-major like Fortran.
+ */
+#ifndef SMMP_ORACLE_H
+#define SMMP_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Topology and parameters of one single-chain system (inputs, read only). */
+typedef struct {
+  int nat, nvt, nms, npnt;
+  const int32_t *ityat, *iowat, *nbdat, *ibdat; /* ibdat: 3*nat */
+  const double *cgat, *blat, *baat, *toat;      /* toat: initial torsions */
+  const int32_t *iatvr, *ityvr, *iclvr, *iorvr, *imsvr1;
+  const int32_t *latms1, *latms2;
+  const int32_t *ivwat1, *ivwat2, *lvwat1, *lvwat2, *i14at1, *i14at2, *l14at;
+  const double *sigma, *rvdw; /* solvent radii (incl. water probe) */
+  const double *cij, *aij, *a14, *chb, *ahb; /* 18x18 */
+  const int32_t *ihbty;                      /* 18x18 */
+  const double *e0to, *sgto, *rnto;          /* 19 */
+  const double *spoint;                      /* npnt x 3, column-major */
+  double conv, plt, slp;
+  int epsd, itysol;
+  int i2, i3;       /* second and third backbone atom (fnd3ba) */
+  double gbpr[6];   /* global position/orientation */
+} oracle_topo;
+
+/* Mutable state of one conformation (allocated by the caller). */
+typedef struct {
+  double *toat, *sntoat, *cstoat, *snbaat, *csbaat; /* nat */
+  double *xat, *yat, *zat;                          /* nat */
+  double *xtoat, *ytoat, *ztoat, *xbaat, *ybaat, *zbaat;
+  /* components of the last energy: total, eysm, eyel, eyvw, eyhb, eyvr, eysl, asa, vdvol */
+  double e[9];
+} oracle_conf;
+
+/* Mersenne twister of twister.f (state 624 words + position). */
+typedef struct {
+  uint32_t mt[624];
+  int mti;
+} oracle_mt;
+
+void oracle_sgrnd(oracle_mt *g, int32_t seed);
+double oracle_grnd(oracle_mt *g);
+uint32_t oracle_grnd_u32(oracle_mt *g);
+double oracle_difang(double a1, double a2);
+double oracle_addang(double a1, double a2);
+
+/* conformation memory helpers */
+int oracle_conf_alloc(oracle_conf *c, int nat);
+void oracle_conf_free(oracle_conf *c);
+void oracle_conf_init(const oracle_topo *t, oracle_conf *c);
+
+/* setvar.f + bldmol.f: apply all variables vlvr[0..nvt-1] and rebuild. */
+void oracle_setvar(const oracle_topo *t, oracle_conf *c, const double *vlvr);
+/* enyshe.f: fills e[2..5]; returns eysm. */
+double oracle_enyshe(const oracle_topo *t, oracle_conf *c);
+/* enysol.f: returns eysl; fills e[7], e[8]. */
+double oracle_enysol(const oracle_topo *t, oracle_conf *c);
+/* energy.f for one chain: setvar + enyshe + solvent; returns total. */
+double oracle_energy(const oracle_topo *t, oracle_conf *c, const double *vlvr);
+
+/* Weight of an energy (canon.f:135 / mulcan_par_mod.f90:380-412). */
+typedef struct {
+  int kind;          /* 0 canonical: beta*x; 1 muca_weight; 2 muca_weight2 */
+  double beta;
+  int kmin, kmax;    /* bins */
+  double ebin;
+  const double *b, *alpha; /* kmax-kmin+1 */
+} oracle_weight;
+double oracle_weight_eval(const oracle_weight *w, double x);
+
+/* metropolis.f:13-184 for one chain.  idvr: free variable indices (1-based),
+ * axvr: step widths (nvt).  vlvr is updated in place.  Returns 0, or 1 if
+ * the end-of-sweep drift check fails (the reference stops there). */
+int oracle_metropolis(const oracle_topo *t, oracle_conf *c, oracle_mt *g, double *vlvr,
+                      int nvr, const int32_t *idvr, const double *axvr,
+                      const oracle_weight *w, double *eol, double *acz);
+
+/* partem_s.f:13-258 (new start, switch=1), all replicas on one stream.
+ * temps: R temperatures; outputs: eol[R] (by replica), ipoi[R] (slot ->
+ * replica, 1-based), coor[R*nvr] final free variables, and per sweep and slot
+ * the energy (traj[nswp*R], slot order, may be NULL). */
+int oracle_partem(const oracle_topo *t, oracle_mt *g, const double *vlvr0, int nvr,
+                  const int32_t *idvr, const double *axvr, int nrep, const double *temps,
+                  int nequi, int nswp, double *eol, int32_t *ipoi, double *coor, double *traj);
+
+/* mulcan_par_mod.f90:23-190 weight recursion on one histogram update.
+ * ihist (kmin..kmax) counts; b, alpha, g1, xhist updated in place. */
+void oracle_muca_update(int kmin, int kmax, double ebin, double beta, const int32_t *ihist,
+                        double *xhist, double *g1, double *b, double *alpha);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

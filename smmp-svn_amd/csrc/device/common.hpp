@@ -1,0 +1,76 @@
+// Device-side system description and small math helpers for gfx950.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace smmp {
+
+constexpr int kWave = 64;
+constexpr int kTy = 18;          // ECEPP atom types
+constexpr int kMaxNb = 128;      // solvent neighbours kept per atom
+constexpr double kPi = 3.141592653589793;
+constexpr double kPi2 = 2.0 * kPi;
+
+// Read-only description of one chain in device memory (see HostSys).
+struct DevSys {
+  int32_t nat, nvt, nlev, words, nheavy, npnt, i2, i3, epsd, itysol, np;  // np: nat rounded up to 64
+  double conv, plt, slp;
+  double gbpr[6];
+  const int16_t* parent;
+  const int16_t* lev_atoms;
+  const int32_t* lev_off;
+  const int8_t* type;
+  const double *q, *bl, *cba, *sba, *toat0, *tref, *rvdw, *sigma;
+  const int16_t* tvar;
+  const int16_t *var_atom, *var_base, *var_axis0, *var_m1, *var_m2;
+  const double *var_e0, *var_sg, *var_rn;
+  const uint64_t* bits;
+  const int16_t* heavy;
+  const double *pA, *pA14, *pC;
+  const int32_t* pHB;
+  const double* spoint;  // npnt*3
+};
+
+// difang/addang of difang.f with the same fmod/sign steps.
+__device__ __forceinline__ double d_difang(double a1, double a2) {
+  double d = fmod(a2 - a1, kPi2);
+  return fabs(d) <= kPi ? d : d - copysign(kPi2, d);
+}
+__device__ __forceinline__ double d_addang(double a1, double a2) {
+  double d = fmod(a1 + a2, kPi2);
+  return fabs(d) <= kPi ? d : d - copysign(kPi2, d);
+}
+
+// 1/sqrt(x) in FP64: hardware estimate refined by two Newton steps.
+__device__ __forceinline__ double rsqrt_f64(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  double h = 0.5 * x;
+  double e = fma(-h * y, y, 0.5);
+  y = fma(y, e, y);
+  e = fma(-h * y, y, 0.5);
+  y = fma(y, e, y);
+  return y;
+}
+
+// Wave-wide sum (wave64) by butterfly shuffles.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+// Orders LDS traffic of one wave (cross-lane hand-off without s_barrier).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+}  // namespace smmp

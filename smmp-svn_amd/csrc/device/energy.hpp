@@ -1,0 +1,480 @@
+// Full ECEPP/3 (+OONS) energy of one conformation by one workgroup:
+//   1. torsions from the variables (setvar.f:43-57) and their sin/cos,
+//   2. Eyring chain build level by level (bldmol.f:13-93, one wave),
+//   3. pair terms over 64x64 tiles of the interaction bit rows
+//      (enyshe.f:112-184; Coulomb, 6-12 LJ, 10-12 H-bond, 1-4 with a14),
+//   4. torsion terms (enyshe.f:73-79),
+//   5. OONS solvation by 122-point occlusion (enysol.f:192-324).
+// MIXED = FP32 pair terms (per-tile FP32 partial sums, FP64 totals); the
+// chain, the solvent term and every total stay FP64.
+#pragma once
+#include "common.hpp"
+
+namespace smmp {
+
+// LDS carve-up for one conformation (doubles unless noted).
+struct EnergyLds {
+  double *x, *y, *z, *q;   // np each
+  double* fr;              // frames: 6*np (or the solvent scratch)
+  double *pA, *pA14, *pC, *pHB;  // 324 each
+  double* sp;              // npnt*3
+  double* red;             // nwaves*8
+  int8_t* t;               // np bytes
+  float *xf, *yf, *zf, *qf;  // mixed-mode copies (alias fr after the build)
+};
+
+__host__ __device__ inline int energy_region1(int np, int nwaves) {
+  int a = 6 * np, b = nwaves * kMaxNb * 4;
+  return a > b ? a : b;
+}
+__host__ __device__ inline size_t energy_lds_bytes(int np, int npnt, int nwaves) {
+  size_t d = 4 * (size_t)np + energy_region1(np, nwaves) + 4 * 324 + 3 * (size_t)npnt + 8 * (size_t)nwaves;
+  return d * sizeof(double) + (size_t)np;
+}
+
+__device__ inline EnergyLds carve_energy_lds(double* base, int np, int npnt, int nwaves) {
+  EnergyLds L;
+  double* p = base;
+  L.x = p; p += np;
+  L.y = p; p += np;
+  L.z = p; p += np;
+  L.q = p; p += np;
+  L.fr = p; p += energy_region1(np, nwaves);
+  L.pA = p; p += 324;
+  L.pA14 = p; p += 324;
+  L.pC = p; p += 324;
+  L.pHB = p; p += 324;
+  L.sp = p; p += 3 * npnt;
+  L.red = p; p += 8 * nwaves;
+  L.t = reinterpret_cast<int8_t*>(p);
+  L.xf = reinterpret_cast<float*>(L.fr);
+  L.yf = L.xf + np;
+  L.zf = L.yf + np;
+  L.qf = L.zf + np;
+  return L;
+}
+
+// Torsion of atom a for the variable values vl (setvar.f:43-57 from the
+// snapshot state toat0/tref).
+__device__ __forceinline__ double atom_torsion(const DevSys& S, const double* vl, int a) {
+  int v = S.tvar[a];
+  double t0 = S.toat0[a];
+  if (v < 0) return t0;
+  double to = t0 + d_difang(S.tref[a], vl[v]);
+  if (fabs(to) > kPi) to = to - copysign(kPi2, to);
+  return to;
+}
+
+// Stage tables, tessellation points, charges and types (once per launch).
+__device__ void energy_stage_static(const DevSys& S, EnergyLds& L) {
+  const int tid = threadIdx.x, nt = blockDim.x, np = S.np;
+  for (int k = tid; k < 324; k += nt) {
+    L.pA[k] = S.pA[k];
+    L.pA14[k] = S.pA14[k];
+    L.pC[k] = S.pC[k];
+    L.pHB[k] = (double)S.pHB[k];
+  }
+  for (int k = tid; k < 3 * S.npnt; k += nt) L.sp[k] = S.spoint[k];
+  for (int a = tid; a < np; a += nt) {
+    if (a < S.nat) {
+      L.q[a] = S.q[a];
+      L.t[a] = S.type[a];
+    } else {
+      L.q[a] = 0.0;
+      L.t[a] = 0;
+      L.x[a] = 0.0;
+      L.y[a] = 0.0;
+      L.z[a] = 0.0;
+    }
+  }
+}
+
+// Torsion sin/cos of every atom into its frame slot (read by eyring_atom).
+__device__ void energy_stage_torsions(const DevSys& S, const double* vl, EnergyLds& L) {
+  for (int a = threadIdx.x; a < S.nat; a += blockDim.x) {
+    double s, c;
+    sincos(atom_torsion(S, vl, a), &s, &c);
+    L.fr[6 * a + 0] = c;  // cos(toat), replaced by the frame when built
+    L.fr[6 * a + 1] = s;
+  }
+}
+
+// bldmol.f:303-420 (setgbl) and the first atom; one lane.
+__device__ void build_root(const DevSys& S, EnergyLds& L) {
+  const double* g = S.gbpr;
+  double ca = cos(g[3]), sa = sin(g[3]), cb = cos(g[4]), sb = sin(g[4]), cg = cos(g[5]), sg = sin(g[5]);
+  double ag[3][3], d;
+  double x1 = ca * cg - sa * sb * sg, x2 = sa * cg + ca * sb * sg, x3 = -cb * sg;
+  d = sqrt(x1 * x1 + x2 * x2 + x3 * x3);
+  ag[0][0] = x1 / d; ag[1][0] = x2 / d; ag[2][0] = x3 / d;
+  double y1 = -sa * cb, y2 = ca * cb, y3 = sb;
+  d = sqrt(y1 * y1 + y2 * y2 + y3 * y3);
+  ag[0][1] = y1 / d; ag[1][1] = y2 / d; ag[2][1] = y3 / d;
+  double z1 = ca * sg + sa * sb * cg, z2 = sa * sg - ca * sb * cg, z3 = cb * cg;
+  d = sqrt(z1 * z1 + z2 * z2 + z3 * z3);
+  ag[0][2] = z1 / d; ag[1][2] = z2 / d; ag[2][2] = z3 / d;
+  const int i2 = S.i2, i3 = S.i3;
+  double ct2 = L.fr[6 * i2 + 0], st2 = L.fr[6 * i2 + 1], sa2 = S.sba[i2];
+  x1 = -S.cba[i2]; x2 = -sa2 * ct2; x3 = -sa2 * st2;
+  double dx = sqrt(x1 * x1 + x2 * x2 + x3 * x3);
+  x1 /= dx; x2 /= dx; x3 /= dx;
+  double ct3 = L.fr[6 * i3 + 0], st3 = L.fr[6 * i3 + 1];
+  z1 = -st3 * (st2 * x3 + ct2 * x2);
+  z2 = st3 * ct2 * x1 + ct3 * st2;
+  z3 = st3 * st2 * x1 - ct3 * ct2;
+  double dz = sqrt(z1 * z1 + z2 * z2 + z3 * z3);
+  z1 /= dz; z2 /= dz; z3 /= dz;
+  y1 = z2 * x3 - z3 * x2;
+  y3 = z1 * x2 - z2 * x1;
+  double xg[3], zg[3];
+  for (int i = 0; i < 3; ++i) {
+    xg[i] = ag[i][0] * x1 + ag[i][1] * y1 + ag[i][2] * z1;
+    zg[i] = ag[i][0] * x3 + ag[i][1] * y3 + ag[i][2] * z3;
+  }
+  dx = sqrt(xg[0] * xg[0] + xg[1] * xg[1] + xg[2] * xg[2]);
+  dz = sqrt(zg[0] * zg[0] + zg[1] * zg[1] + zg[2] * zg[2]);
+  L.x[0] = g[0];
+  L.y[0] = g[1];
+  L.z[0] = g[2];
+  L.fr[0] = xg[0] / dx;
+  L.fr[1] = xg[1] / dx;
+  L.fr[2] = xg[2] / dx;
+  L.fr[3] = zg[0] / dz;
+  L.fr[4] = zg[1] / dz;
+  L.fr[5] = zg[2] / dz;
+}
+
+// bldmol.f:175-242 (eyring) for atom a from its parent's frame.
+__device__ __forceinline__ void eyring_atom(const DevSys& S, EnergyLds& L, int a) {
+  const int p = S.parent[a];
+  double ct = L.fr[6 * a + 0], st = L.fr[6 * a + 1];
+  double ca = S.cba[a], sa = S.sba[a], bl = S.bl[a];
+  double x1 = L.fr[6 * p + 0], x2 = L.fr[6 * p + 1], x3 = L.fr[6 * p + 2];
+  double z1 = L.fr[6 * p + 3], z2 = L.fr[6 * p + 4], z3 = L.fr[6 * p + 5];
+  double px = L.x[p], py = L.y[p], pz = L.z[p];
+  double y1 = z2 * x3 - z3 * x2, y2 = z3 * x1 - z1 * x3, y3 = z1 * x2 - z2 * x1;
+  double h2 = -sa * ct, h3 = -sa * st;
+  double u1 = -ca * x1 + h2 * y1 + h3 * z1;
+  double u2 = -ca * x2 + h2 * y2 + h3 * z2;
+  double u3 = -ca * x3 + h2 * y3 + h3 * z3;
+  double dx = 1.0 / sqrt(u1 * u1 + u2 * u2 + u3 * u3);
+  u1 *= dx; u2 *= dx; u3 *= dx;
+  double w1 = -st * y1 + ct * z1, w2 = -st * y2 + ct * z2, w3 = -st * y3 + ct * z3;
+  double dz = 1.0 / sqrt(w1 * w1 + w2 * w2 + w3 * w3);
+  L.fr[6 * a + 0] = u1;
+  L.fr[6 * a + 1] = u2;
+  L.fr[6 * a + 2] = u3;
+  L.fr[6 * a + 3] = w1 * dz;
+  L.fr[6 * a + 4] = w2 * dz;
+  L.fr[6 * a + 5] = w3 * dz;
+  L.x[a] = px + u1 * bl;
+  L.y[a] = py + u2 * bl;
+  L.z[a] = pz + u3 * bl;
+}
+
+// Chain build by wave 0, level by level; the other waves wait.
+__device__ void build_chain(const DevSys& S, EnergyLds& L) {
+  if (threadIdx.x < kWave) {
+    if (threadIdx.x == 0) build_root(S, L);
+    wave_lds_sync();
+    const int lane = threadIdx.x;
+    for (int lv = 0; lv < S.nlev; ++lv) {
+      const int o0 = S.lev_off[lv], o1 = S.lev_off[lv + 1];
+      for (int o = o0 + lane; o < o1; o += kWave) eyring_atom(S, L, S.lev_atoms[o]);
+      wave_lds_sync();
+    }
+  }
+  __syncthreads();
+}
+
+// Upper-triangle tile index -> (bi, bj), bi <= bj, row-major.
+__device__ __forceinline__ void tile_coords(int t, int nt, int& bi, int& bj) {
+  int b = 0, rem = t;
+  while (rem >= nt - b) {
+    rem -= nt - b;
+    ++b;
+  }
+  bi = b;
+  bj = b + rem;
+}
+
+struct PairSums {
+  double el, vw, hb;
+};
+
+// Pair terms over the tiles this wave owns (FP64).
+__device__ PairSums pair_terms_f64(const DevSys& S, const EnergyLds& L, int wave, int nwaves) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int nat = S.nat, nt = (nat + 63) / 64, ntile = nt * (nt + 1) / 2, words = S.words;
+  double el = 0.0, vw = 0.0, hb = 0.0;
+  for (int t = wave; t < ntile; t += nwaves) {
+    int bi, bj;
+    tile_coords(t, nt, bi, bj);
+    const int i = bi * 64 + lane;
+    const bool vi = i < nat;
+    uint64_t w0 = 0, w1 = 0;
+    double xi = 0, yi = 0, zi = 0, qi = 0;
+    int ti = 0;
+    if (vi) {
+      w0 = S.bits[((size_t)i * words + bj) * 2 + 0];
+      w1 = S.bits[((size_t)i * words + bj) * 2 + 1];
+      xi = L.x[i]; yi = L.y[i]; zi = L.z[i];
+      qi = S.conv * L.q[i];
+      ti = L.t[i] * kTy;
+    }
+    if (bi == bj) {
+      uint64_t keep = (lane == 63) ? 0ull : ~((2ull << lane) - 1ull);
+      w0 &= keep;
+      w1 &= keep;
+    }
+    const int kmax = min(64, nat - bj * 64);
+    for (int k = 0; k < kmax; ++k) {
+      const bool on = (w0 >> k) & 1ull;
+      if (!__any(on)) continue;
+      if (on) {
+        const int j = bj * 64 + k;
+        double dx = L.x[j] - xi, dy = L.y[j] - yi, dz = L.z[j] - zi;
+        double r2 = dx * dx + dy * dy + dz * dz;
+        double ri = rsqrt_f64(r2);
+        double r2i = ri * ri;
+        double r6i = r2i * r2i * r2i;
+        const int tt = ti + L.t[j];
+        const bool is14 = (w1 >> k) & 1ull;
+        double A = is14 ? L.pA14[tt] : L.pA[tt];
+        double Cc = L.pC[tt];
+        double ep_inv = 1.0;
+        if (S.epsd) {
+          double rij = r2 * ri;
+          double sr = S.slp * rij;
+          ep_inv = 1.0 / (S.plt - (sr * sr + 2.0 * sr + 2.0) * (S.plt - 1.0) * exp(-sr) / 2.0);
+        }
+        el += qi * L.q[j] * ri * ep_inv;
+        if (L.pHB[tt] != 0.0)
+          hb += A * (r6i * r6i) - Cc * (r6i * r2i * r2i);
+        else
+          vw += A * (r6i * r6i) - Cc * r6i;
+      }
+    }
+  }
+  return {el, vw, hb};
+}
+
+// Mixed precision: FP32 terms, FP32 sums per tile, FP64 accumulation.
+__device__ PairSums pair_terms_mixed(const DevSys& S, const EnergyLds& L, int wave, int nwaves) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int nat = S.nat, nt = (nat + 63) / 64, ntile = nt * (nt + 1) / 2, words = S.words;
+  double el = 0.0, vw = 0.0, hb = 0.0;
+  const float conv = (float)S.conv;
+  for (int t = wave; t < ntile; t += nwaves) {
+    int bi, bj;
+    tile_coords(t, nt, bi, bj);
+    const int i = bi * 64 + lane;
+    const bool vi = i < nat;
+    uint64_t w0 = 0, w1 = 0;
+    float xi = 0, yi = 0, zi = 0, qi = 0;
+    int ti = 0;
+    if (vi) {
+      w0 = S.bits[((size_t)i * words + bj) * 2 + 0];
+      w1 = S.bits[((size_t)i * words + bj) * 2 + 1];
+      xi = L.xf[i]; yi = L.yf[i]; zi = L.zf[i];
+      qi = conv * L.qf[i];
+      ti = L.t[i] * kTy;
+    }
+    if (bi == bj) {
+      uint64_t keep = (lane == 63) ? 0ull : ~((2ull << lane) - 1ull);
+      w0 &= keep;
+      w1 &= keep;
+    }
+    float fel = 0.f, fvw = 0.f, fhb = 0.f;
+    const int kmax = min(64, nat - bj * 64);
+    for (int k = 0; k < kmax; ++k) {
+      const bool on = (w0 >> k) & 1ull;
+      if (!__any(on)) continue;
+      if (on) {
+        const int j = bj * 64 + k;
+        float dx = L.xf[j] - xi, dy = L.yf[j] - yi, dz = L.zf[j] - zi;
+        float r2 = dx * dx + dy * dy + dz * dz;
+        float ri = __builtin_amdgcn_rsqf(r2);
+        float r2i = ri * ri;
+        float r6i = r2i * r2i * r2i;
+        const int tt = ti + L.t[j];
+        const bool is14 = (w1 >> k) & 1ull;
+        float A = (float)(is14 ? L.pA14[tt] : L.pA[tt]);
+        float Cc = (float)L.pC[tt];
+        fel += qi * L.qf[j] * ri;
+        if (L.pHB[tt] != 0.0)
+          fhb += A * (r6i * r6i) - Cc * (r6i * r2i * r2i);
+        else
+          fvw += A * (r6i * r6i) - Cc * r6i;
+      }
+    }
+    el += (double)fel;
+    vw += (double)fvw;
+    hb += (double)fhb;
+  }
+  return {el, vw, hb};
+}
+
+// enyshe.f:73-79 over all variables; lanes over variables.
+__device__ double torsion_terms(const DevSys& S, const double* vl) {
+  double e = 0.0;
+  for (int v = threadIdx.x; v < S.nvt; v += blockDim.x) {
+    double e0 = S.var_e0[v];
+    if (e0 != 0.0) {
+      double to = atom_torsion(S, vl, S.var_atom[v]);
+      e += e0 * (1.0 + S.var_sg[v] * cos(to * S.var_rn[v]));
+    }
+  }
+  return e;
+}
+
+// enysol.f:192-324: area of heavy atom i from its free tessellation points.
+// Returns the number of free points, or -1 if the neighbour list overflows.
+__device__ int solvent_atom(const DevSys& S, const EnergyLds& L, double* nb, int i) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const double xi = L.x[i], yi = L.y[i], zi = L.z[i];
+  const double trad = S.rvdw[i];
+  int cnt = 0;
+  for (int base = 0; base < S.nheavy; base += kWave) {
+    const int h = base + lane;
+    bool on = false;
+    double dx = 0, dy = 0, dz = 0, r2b = 0;
+    if (h < S.nheavy) {
+      const int j = S.heavy[h];
+      if (j != i) {
+        const double rj = S.rvdw[j];
+        const double ax = xi - L.x[j], ay = yi - L.y[j], az = zi - L.z[j];
+        const double lim = (trad + rj) * (1.0 + 1e-9);
+        if (ax * ax + ay * ay + az * az <= lim * lim) {  // cheap filter, then the reference test
+          dx = ax / trad;
+          dy = ay / trad;
+          dz = az / trad;
+          double dd = dx * dx + dy * dy + dz * dz;
+          double akrad = rj / trad;
+          double dr = 1.0 + akrad;
+          dr = dr * dr;
+          on = dd <= dr;
+          r2b = akrad * akrad;
+        }
+      }
+    }
+    const uint64_t m = __ballot(on);
+    if (on) {
+      const int pos = cnt + __popcll(m & ((1ull << lane) - 1ull));
+      if (pos < kMaxNb) {
+        nb[4 * pos + 0] = dx;
+        nb[4 * pos + 1] = dy;
+        nb[4 * pos + 2] = dz;
+        nb[4 * pos + 3] = r2b;
+      }
+    }
+    cnt += __popcll(m);
+  }
+  wave_lds_sync();
+  if (cnt > kMaxNb) return -1;
+  int nfree = 0;
+  for (int p0 = 0; p0 < S.npnt; p0 += kWave) {
+    const int p = p0 + lane;
+    bool fr = false;
+    if (p < S.npnt) {
+      const double px = L.sp[3 * p + 0], py = L.sp[3 * p + 1], pz = L.sp[3 * p + 2];
+      fr = true;
+      for (int k = 0; k < cnt; ++k) {
+        double ax = nb[4 * k + 0] + px, ay = nb[4 * k + 1] + py, az = nb[4 * k + 2] + pz;
+        double sdd = ax * ax;
+        sdd = sdd + ay * ay;
+        sdd = sdd + az * az;
+        if (sdd <= nb[4 * k + 3]) {
+          fr = false;
+          break;
+        }
+      }
+    }
+    nfree += __popcll(__ballot(fr));
+  }
+  wave_lds_sync();
+  return nfree;
+}
+
+struct EnergyOut {
+  double el, vw, hb, tor, sol, asa;
+  int err;
+};
+
+// Pair, torsion and solvent terms of the coordinates in L (all threads).
+// nfree_out (LDS, nheavy) receives the free-point count of every heavy atom
+// when not null.  On return thread 0 holds the totals in 'out'.
+template <bool MIXED>
+__device__ void energy_from_coords(const DevSys& S, const double* vl, EnergyLds& L, int32_t* nfree_out,
+                                   EnergyOut& out) {
+  const int tid = threadIdx.x, nwaves = blockDim.x / kWave, wave = tid / kWave, lane = tid & (kWave - 1);
+  if (MIXED) {  // FP32 copies centred on the first atom (frames are dead now)
+    const double cx = L.x[0], cy = L.y[0], cz = L.z[0];
+    for (int a = tid; a < S.np; a += blockDim.x) {
+      L.xf[a] = (float)(L.x[a] - cx);
+      L.yf[a] = (float)(L.y[a] - cy);
+      L.zf[a] = (float)(L.z[a] - cz);
+      L.qf[a] = (float)L.q[a];
+    }
+    __syncthreads();
+  }
+  PairSums ps = MIXED ? pair_terms_mixed(S, L, wave, nwaves) : pair_terms_f64(S, L, wave, nwaves);
+  double tor = torsion_terms(S, vl);
+  if (MIXED) __syncthreads();  // the solvent scratch overlays the FP32 copies
+  double sol = 0.0, asa = 0.0;
+  int err = 0;
+  if (S.itysol > 0) {
+    double* nb = L.fr + (size_t)wave * kMaxNb * 4;
+    const double c4 = 4.0 * kPi / (double)S.npnt;
+    for (int h = wave; h < S.nheavy; h += nwaves) {
+      const int i = S.heavy[h];
+      int nfree = solvent_atom(S, L, nb, i);
+      if (nfree < 0) {
+        err = 1;
+        nfree = 0;
+      }
+      if (lane == 0) {
+        if (nfree_out) nfree_out[h] = nfree;
+        const double trad = S.rvdw[i];
+        double area = c4 * trad * trad * (double)nfree;
+        sol += area * S.sigma[i];
+        asa += area;
+      }
+    }
+  }
+  double v[6] = {ps.el, ps.vw, ps.hb, tor, sol, asa};
+#pragma unroll
+  for (int k = 0; k < 6; ++k) v[k] = wave_sum(v[k]);
+  const int errw = __any(err);
+  __syncthreads();  // L.red may still be read by a previous reduction
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) L.red[8 * wave + k] = v[k];
+    L.red[8 * wave + 6] = (double)errw;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double s[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int w = 0; w < nwaves; ++w)
+      for (int k = 0; k < 7; ++k) s[k] += L.red[8 * w + k];
+    out.el = s[0];
+    out.vw = s[1];
+    out.hb = s[2];
+    out.tor = s[3];
+    out.sol = s[4];
+    out.asa = s[5];
+    out.err = s[6] != 0.0;
+  }
+}
+
+// Whole energy of one conformation from its variables (energy.f:12-119).
+template <bool MIXED>
+__device__ void conformation_energy(const DevSys& S, const double* vl, EnergyLds& L, int32_t* nfree_out,
+                                    EnergyOut& out) {
+  energy_stage_torsions(S, vl, L);
+  __syncthreads();
+  build_chain(S, L);
+  energy_from_coords<MIXED>(S, vl, L, nfree_out, out);
+}
+
+}  // namespace smmp

@@ -1,0 +1,381 @@
+// HIP engine: device mirrors of contexts and the kernel launches behind the
+// C-ABI.  Kernels: k_energy (full energy per conformation), the sweep and
+// exchange kernels of sampler.hpp.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../host/devtopo.hpp"
+#include "../host/engine.hpp"
+#include "common.hpp"
+#include "energy.hpp"
+#include "sampler.hpp"
+
+namespace smmp {
+
+#define HIPCHK(x)                                                                                   \
+  do {                                                                                              \
+    hipError_t e_ = (x);                                                                            \
+    if (e_ != hipSuccess) throw DeviceError(std::string(#x) + ": " + hipGetErrorString(e_));        \
+  } while (0)
+
+constexpr int kEnergyThreads = 512;
+
+// ------------------------------------------------------------- kernels
+
+template <bool MIXED>
+__global__ void __launch_bounds__(kEnergyThreads) k_energy(const DevSys* __restrict__ systems,
+                                                           const int32_t* __restrict__ conf_sys,
+                                                           const int64_t* __restrict__ conf_off,
+                                                           const double* __restrict__ vlvr, double* __restrict__ out,
+                                                           double* __restrict__ xyz_out, int32_t* __restrict__ err) {
+  extern __shared__ double smem[];
+  const int c = blockIdx.x;
+  const DevSys& S = systems[conf_sys[c]];
+  const double* vl = vlvr + conf_off[c];
+  EnergyLds L = carve_energy_lds(smem, S.np, S.npnt, blockDim.x / kWave);
+  EnergyOut o;
+  energy_stage_static(S, L);
+  conformation_energy<MIXED>(S, vl, L, nullptr, o);
+  if (threadIdx.x == 0) {
+    double eysm = o.el + o.vw + o.hb + o.tor;  // energy.f / enyshe.f:187
+    double* r = out + (size_t)c * 8;
+    r[0] = eysm + o.sol;
+    r[1] = eysm;
+    r[2] = o.el;
+    r[3] = o.vw;
+    r[4] = o.hb;
+    r[5] = o.tor;
+    r[6] = o.sol;
+    r[7] = o.asa;
+    if (o.err) atomicOr(err, 1);
+  }
+  if (xyz_out) {
+    for (int a = threadIdx.x; a < S.nat; a += blockDim.x) {
+      xyz_out[3 * a + 0] = L.x[a];
+      xyz_out[3 * a + 1] = L.y[a];
+      xyz_out[3 * a + 2] = L.z[a];
+    }
+  }
+}
+
+// ------------------------------------------------------ device buffers
+
+struct DevMem {
+  void* p = nullptr;
+  size_t n = 0;
+  DevMem() = default;
+  DevMem(const DevMem&) = delete;
+  DevMem& operator=(const DevMem&) = delete;
+  ~DevMem() {
+    if (p) (void)hipFree(p);
+  }
+  void* ensure(size_t bytes) {
+    if (bytes > n) {
+      if (p) HIPCHK(hipFree(p));
+      p = nullptr;
+      n = 0;
+      HIPCHK(hipMalloc(&p, bytes));
+      n = bytes;
+    }
+    return p;
+  }
+};
+
+// Replica state of the samplers (grown on demand, reused across calls).
+struct SamplerBuffers {
+  DevMem idvr, axvr, vlvr, xyz, nfree, valid, eol, acc, mt, rng_of, slot_of, beta, err, dbg;
+  DevMem wb, walpha, hist, traj, ipoi, iswitch;
+};
+
+// One system uploaded to the device: a single allocation holding every
+// array, plus the DevSys with the device pointers.
+struct DevSystem {
+  HostSys h;
+  DevSys d;
+  DevMem mem;
+};
+
+static void upload_system(DevSystem& ds, hipStream_t st) {
+  const HostSys& h = ds.h;
+  size_t off = 0;
+  std::vector<std::pair<size_t, std::pair<const void*, size_t>>> parts;
+  auto add = [&](const void* src, size_t bytes) {
+    off = (off + 255) & ~size_t(255);
+    size_t o = off;
+    parts.push_back({o, {src, bytes}});
+    off += std::max<size_t>(bytes, 1);
+    return o;
+  };
+  size_t o_parent = add(h.parent.data(), h.parent.size() * 2);
+  size_t o_lev = add(h.lev_atoms.data(), h.lev_atoms.size() * 2);
+  size_t o_levoff = add(h.lev_off.data(), h.lev_off.size() * 4);
+  size_t o_type = add(h.type.data(), h.type.size());
+  size_t o_q = add(h.q.data(), h.q.size() * 8);
+  size_t o_bl = add(h.bl.data(), h.bl.size() * 8);
+  size_t o_cba = add(h.cba.data(), h.cba.size() * 8);
+  size_t o_sba = add(h.sba.data(), h.sba.size() * 8);
+  size_t o_to = add(h.toat0.data(), h.toat0.size() * 8);
+  size_t o_tref = add(h.tref.data(), h.tref.size() * 8);
+  size_t o_rv = add(h.rvdw.data(), h.rvdw.size() * 8);
+  size_t o_sg = add(h.sigma.data(), h.sigma.size() * 8);
+  size_t o_tvar = add(h.tvar.data(), h.tvar.size() * 2);
+  size_t o_va = add(h.var_atom.data(), h.var_atom.size() * 2);
+  size_t o_vb = add(h.var_base.data(), h.var_base.size() * 2);
+  size_t o_vx = add(h.var_axis0.data(), h.var_axis0.size() * 2);
+  size_t o_m1 = add(h.var_m1.data(), h.var_m1.size() * 2);
+  size_t o_m2 = add(h.var_m2.data(), h.var_m2.size() * 2);
+  size_t o_e0 = add(h.var_e0.data(), h.var_e0.size() * 8);
+  size_t o_vs = add(h.var_sg.data(), h.var_sg.size() * 8);
+  size_t o_vr = add(h.var_rn.data(), h.var_rn.size() * 8);
+  size_t o_bits = add(h.bits.data(), h.bits.size() * 8);
+  size_t o_heavy = add(h.heavy.data(), h.heavy.size() * 2);
+  size_t o_pA = add(h.pA.data(), h.pA.size() * 8);
+  size_t o_pA14 = add(h.pA14.data(), h.pA14.size() * 8);
+  size_t o_pC = add(h.pC.data(), h.pC.size() * 8);
+  size_t o_pHB = add(h.pHB.data(), h.pHB.size() * 4);
+  size_t o_sp = add(h.spoint.data(), h.spoint.size() * 8);
+  std::vector<uint8_t> blob(off, 0);
+  for (auto& pr : parts)
+    if (pr.second.second) std::memcpy(blob.data() + pr.first, pr.second.first, pr.second.second);
+  char* base = static_cast<char*>(ds.mem.ensure(off));
+  HIPCHK(hipMemcpyAsync(base, blob.data(), off, hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
+  DevSys& d = ds.d;
+  d.nat = h.nat; d.nvt = h.nvt; d.nlev = h.nlev; d.words = h.words; d.nheavy = h.nheavy;
+  d.npnt = h.npnt; d.i2 = h.i2; d.i3 = h.i3; d.epsd = h.epsd; d.itysol = h.itysol;
+  d.np = (h.nat + 63) / 64 * 64;
+  d.conv = h.conv; d.plt = h.plt; d.slp = h.slp;
+  for (int k = 0; k < 6; ++k) d.gbpr[k] = h.gbpr[k];
+  d.parent = (const int16_t*)(base + o_parent);
+  d.lev_atoms = (const int16_t*)(base + o_lev);
+  d.lev_off = (const int32_t*)(base + o_levoff);
+  d.type = (const int8_t*)(base + o_type);
+  d.q = (const double*)(base + o_q);
+  d.bl = (const double*)(base + o_bl);
+  d.cba = (const double*)(base + o_cba);
+  d.sba = (const double*)(base + o_sba);
+  d.toat0 = (const double*)(base + o_to);
+  d.tref = (const double*)(base + o_tref);
+  d.rvdw = (const double*)(base + o_rv);
+  d.sigma = (const double*)(base + o_sg);
+  d.tvar = (const int16_t*)(base + o_tvar);
+  d.var_atom = (const int16_t*)(base + o_va);
+  d.var_base = (const int16_t*)(base + o_vb);
+  d.var_axis0 = (const int16_t*)(base + o_vx);
+  d.var_m1 = (const int16_t*)(base + o_m1);
+  d.var_m2 = (const int16_t*)(base + o_m2);
+  d.var_e0 = (const double*)(base + o_e0);
+  d.var_sg = (const double*)(base + o_vs);
+  d.var_rn = (const double*)(base + o_vr);
+  d.bits = (const uint64_t*)(base + o_bits);
+  d.heavy = (const int16_t*)(base + o_heavy);
+  d.pA = (const double*)(base + o_pA);
+  d.pA14 = (const double*)(base + o_pA14);
+  d.pC = (const double*)(base + o_pC);
+  d.pHB = (const int32_t*)(base + o_pHB);
+  d.spoint = (const double*)(base + o_sp);
+}
+
+struct Engine {
+  hipStream_t stream = nullptr;
+  int device = -1;
+  bool valid = false;           // system upload matches the host state
+  DevSystem sys;
+  DevMem d_sys, d_confsys, d_confoff, d_vlvr, d_out, d_xyz, d_err;
+  SamplerBuffers smp;           // replica state of the samplers
+  std::vector<std::unique_ptr<DevSystem>> batch_sys;  // energy_batch systems
+  ~Engine() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+static Engine* make_engine() {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n < 1) throw DeviceError("no HIP device available (hipGetDeviceCount)");
+  auto* eng = new Engine();
+  HIPCHK(hipGetDevice(&eng->device));
+  HIPCHK(hipStreamCreateWithFlags(&eng->stream, hipStreamNonBlocking));
+  return eng;
+}
+
+static Engine& engine_of(Engine*& e) {
+  if (!e) e = make_engine();
+  return *e;
+}
+
+void engine_destroy(Engine* e) { delete e; }
+void engine_invalidate(Engine* e) {
+  if (e) e->valid = false;
+}
+
+// Upload the context's system when the topology changed, or refresh the
+// global parameters gbpr when only those changed.
+static DevSystem& current_system(Engine& E, const State& s) {
+  bool gb_same = true;
+  for (int k = 0; k < 6; ++k)
+    if (E.sys.h.gbpr[k] != s.gbpr(k + 1, 1)) gb_same = false;
+  if (!E.valid || !gb_same) {
+    E.sys.h = build_host_sys(s);
+    upload_system(E.sys, E.stream);
+    DevSys* dp = static_cast<DevSys*>(E.d_sys.ensure(sizeof(DevSys)));
+    HIPCHK(hipMemcpyAsync(dp, &E.sys.d, sizeof(DevSys), hipMemcpyHostToDevice, E.stream));
+    HIPCHK(hipStreamSynchronize(E.stream));
+    E.valid = true;
+  }
+  return E.sys;
+}
+
+static size_t lds_for(const DevSys& d, int threads) { return energy_lds_bytes(d.np, d.npnt, threads / kWave); }
+
+static void check_lds(size_t bytes) {
+  if (bytes > 160 * 1024) throw DeviceError("system too large for one workgroup's LDS");
+}
+
+// Launches k_energy for nconf conformations of the systems at d_systems.
+static void launch_energy(Engine& E, const DevSys* d_systems, const int32_t* d_cs, const int64_t* d_off,
+                          const double* d_vl, double* d_out, double* d_xyz, int nconf, size_t lds, int precision) {
+  check_lds(lds);
+  int32_t* d_err = static_cast<int32_t*>(E.d_err.ensure(sizeof(int32_t)));
+  HIPCHK(hipMemsetAsync(d_err, 0, sizeof(int32_t), E.stream));
+  if (precision == 0) {
+    HIPCHK(hipFuncSetAttribute((const void*)k_energy<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_energy<false>, dim3(nconf), dim3(kEnergyThreads), lds, E.stream, d_systems, d_cs, d_off, d_vl,
+                       d_out, d_xyz, d_err);
+  } else {
+    HIPCHK(hipFuncSetAttribute((const void*)k_energy<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_energy<true>, dim3(nconf), dim3(kEnergyThreads), lds, E.stream, d_systems, d_cs, d_off, d_vl,
+                       d_out, d_xyz, d_err);
+  }
+  HIPCHK(hipGetLastError());
+}
+
+static int read_err(Engine& E) {
+  int32_t h = 0;
+  HIPCHK(hipMemcpyAsync(&h, E.d_err.p, sizeof(int32_t), hipMemcpyDeviceToHost, E.stream));
+  HIPCHK(hipStreamSynchronize(E.stream));
+  return h;
+}
+
+// Single energy of the context's current variables; updates the host state
+// (components, coordinates) like energy.f does for the COMMON blocks.
+static void energy_into_state(Engine& E, State& s) {
+  DevSystem& ds = current_system(E, s);
+  const int nvt = ds.h.nvt, nat = ds.h.nat;
+  double* d_vl = static_cast<double*>(E.d_vlvr.ensure(sizeof(double) * nvt));
+  int32_t* d_cs = static_cast<int32_t*>(E.d_confsys.ensure(sizeof(int32_t)));
+  int64_t* d_off = static_cast<int64_t*>(E.d_confoff.ensure(sizeof(int64_t)));
+  double* d_out = static_cast<double*>(E.d_out.ensure(sizeof(double) * 8));
+  double* d_xyz = static_cast<double*>(E.d_xyz.ensure(sizeof(double) * 3 * nat));
+  const int32_t zero = 0;
+  const int64_t zoff = 0;
+  HIPCHK(hipMemcpyAsync(d_vl, s.vlvr.data(), sizeof(double) * nvt, hipMemcpyHostToDevice, E.stream));
+  HIPCHK(hipMemcpyAsync(d_cs, &zero, sizeof(zero), hipMemcpyHostToDevice, E.stream));
+  HIPCHK(hipMemcpyAsync(d_off, &zoff, sizeof(zoff), hipMemcpyHostToDevice, E.stream));
+  launch_energy(E, static_cast<const DevSys*>(E.d_sys.p), d_cs, d_off, d_vl, d_out, d_xyz, 1,
+                lds_for(ds.d, kEnergyThreads), 0);
+  double o[8];
+  std::vector<double> xyz(3 * (size_t)nat);
+  HIPCHK(hipMemcpyAsync(o, d_out, sizeof o, hipMemcpyDeviceToHost, E.stream));
+  HIPCHK(hipMemcpyAsync(xyz.data(), d_xyz, sizeof(double) * 3 * nat, hipMemcpyDeviceToHost, E.stream));
+  HIPCHK(hipStreamSynchronize(E.stream));
+  if (read_err(E)) throw Error("enysol> too many solvent neighbours for one atom");
+  s.eysm = o[1];
+  s.eyel = o[2];
+  s.eyvw = o[3];
+  s.eyhb = o[4];
+  s.eyvr = o[5];
+  s.eysl = o[6];
+  s.asa = o[7];
+  for (int a = 0; a < nat; ++a) {
+    s.xat(a + 1) = xyz[3 * a + 0];
+    s.yat(a + 1) = xyz[3 * a + 1];
+    s.zat(a + 1) = xyz[3 * a + 2];
+  }
+  // keep the host torsion state in step with the variables (setvar.f)
+  setvar(s, 1, s.vlvr.data());
+}
+
+double engine_energy(Engine*& e, State& s) {
+  Engine& E = engine_of(e);
+  energy_into_state(E, s);
+  return s.eysm + s.eysl;
+}
+
+double engine_enysol(Engine*& e, State& s) {
+  Engine& E = engine_of(e);
+  energy_into_state(E, s);
+  return s.eysl;
+}
+
+int engine_energy_batch(Engine*& e, std::vector<State*>& sys, const int32_t* conf_sys, int nconf, const double* vlvr,
+                        int precision, double* out) {
+  Engine& E = engine_of(e);
+  if (nconf == 0) return SMMP_OK;
+  const int nsys = (int)sys.size();
+  E.batch_sys.clear();
+  std::vector<DevSys> dsys(nsys);
+  int maxnp = 0, maxpnt = 0;
+  for (int i = 0; i < nsys; ++i) {
+    auto p = std::make_unique<DevSystem>();
+    p->h = build_host_sys(*sys[i]);
+    upload_system(*p, E.stream);
+    dsys[i] = p->d;
+    maxnp = std::max(maxnp, p->d.np);
+    maxpnt = std::max(maxpnt, p->d.npnt);
+    E.batch_sys.push_back(std::move(p));
+  }
+  std::vector<int64_t> off(nconf);
+  int64_t tot = 0;
+  for (int c = 0; c < nconf; ++c) {
+    off[c] = tot;
+    tot += dsys[conf_sys[c]].nvt;
+  }
+  DevSys* d_systems = static_cast<DevSys*>(E.d_sys.ensure(sizeof(DevSys) * nsys));
+  E.valid = false;  // d_sys now holds the batch systems
+  int32_t* d_cs = static_cast<int32_t*>(E.d_confsys.ensure(sizeof(int32_t) * nconf));
+  int64_t* d_off = static_cast<int64_t*>(E.d_confoff.ensure(sizeof(int64_t) * nconf));
+  double* d_vl = static_cast<double*>(E.d_vlvr.ensure(sizeof(double) * tot));
+  double* d_out = static_cast<double*>(E.d_out.ensure(sizeof(double) * 8 * nconf));
+  HIPCHK(hipMemcpyAsync(d_systems, dsys.data(), sizeof(DevSys) * nsys, hipMemcpyHostToDevice, E.stream));
+  HIPCHK(hipMemcpyAsync(d_cs, conf_sys, sizeof(int32_t) * nconf, hipMemcpyHostToDevice, E.stream));
+  HIPCHK(hipMemcpyAsync(d_off, off.data(), sizeof(int64_t) * nconf, hipMemcpyHostToDevice, E.stream));
+  HIPCHK(hipMemcpyAsync(d_vl, vlvr, sizeof(double) * tot, hipMemcpyHostToDevice, E.stream));
+  size_t lds = energy_lds_bytes(maxnp, maxpnt, kEnergyThreads / kWave);
+  launch_energy(E, d_systems, d_cs, d_off, d_vl, d_out, nullptr, nconf, lds, precision);
+  HIPCHK(hipMemcpyAsync(out, d_out, sizeof(double) * 8 * nconf, hipMemcpyDeviceToHost, E.stream));
+  HIPCHK(hipStreamSynchronize(E.stream));
+  if (read_err(E)) throw Error("enysol> too many solvent neighbours for one atom");
+  return SMMP_OK;
+}
+
+// --------------------------------------------------- host MT (twister.f)
+
+double host_grnd(State& s) {
+  if (s.mti >= 624) {
+    if (s.mti == 625) {
+      s.mt(0) = 4357;
+      for (int i = 1; i < 624; ++i) s.mt(i) = (int32_t)(69069u * (uint32_t)s.mt(i - 1));
+    }
+    uint32_t* mt = reinterpret_cast<uint32_t*>(s.mt.data());
+    mt_twist(mt);
+    s.mti = 0;
+  }
+  uint32_t y = (uint32_t)s.mt(s.mti++);
+  return (double)mt_temper(y) / 4294967295.0;
+}
+
+const char* engine_build_info() {
+  return "smmp_mi355x: gfx950 HIP engine; energy kernel 512 threads, 64x64 pair tiles, FP64 and mixed FP32/FP64";
+}
+
+}  // namespace smmp
+
+#include "sampler_host.inc"

@@ -1,0 +1,263 @@
+// C-ABI: context management, COMMON-block views and the host-side entry
+// points.  The GPU entry points (energy, metropolis, samplers) live in
+// device/engine.hip and are reached through engine.hpp.
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../../include/smmp_mi355x.h"
+#include "engine.hpp"
+#include "state.hpp"
+
+using namespace smmp;
+
+struct smmp_ctx {
+  State s;
+  std::string err;
+  Engine* eng = nullptr;  // device mirror, created lazily
+};
+
+namespace {
+
+struct ViewDesc {
+  const char* name;
+  int dtype;
+  int ndim;
+  int64_t shape[2];
+  int64_t lbound[2];
+  void* (*ptr)(State&);
+};
+
+#define V1(block, f, dt, n, lb) {#block "." #f, dt, 1, {n, 0}, {lb, 0}, [](State& s) -> void* { return (void*)s.f.data(); }}
+#define V2(block, f, dt, n1, n2) {#block "." #f, dt, 2, {n1, n2}, {1, 1}, [](State& s) -> void* { return (void*)s.f.data(); }}
+#define VS(block, f, dt) {#block "." #f, dt, 0, {1, 0}, {1, 0}, [](State& s) -> void* { return (void*)&s.f; }}
+#define VC(block, f, L, n) {#block "." #f, SMMP_CHR, 2, {L, n}, {1, 1}, [](State& s) -> void* { return (void*)s.f.data(); }}
+
+const ViewDesc kViews[] = {
+    VS(epar_l, flex, SMMP_I32), VS(epar_l, tesgrd, SMMP_I32), VS(epar_l, sh2, SMMP_I32),
+    VS(epar_l, epsd, SMMP_I32), VS(epar_l, ientyp, SMMP_I32),
+    VS(isolty, itysol, SMMP_I32), VS(isolty, isolscl, SMMP_I32),
+    VS(updchois, rndord, SMMP_I32), VS(updchois, upchswitch, SMMP_I32), VS(updchois, bgsprob, SMMP_F64),
+    VS(bet, beta, SMMP_F64),
+    VS(mol_par, nvr, SMMP_I32),
+    VS(mol_i, ntlml, SMMP_I32), V1(mol_i, irsml1, SMMP_I32, mxml, 1), V1(mol_i, irsml2, SMMP_I32, mxml, 1),
+    V1(mol_i, ivrml1, SMMP_I32, mxml, 1), V1(mol_i, nvrml, SMMP_I32, mxml, 1), V1(mol_i, imsml1, SMMP_I32, mxml, 1),
+    V1(mol_i, nmsml, SMMP_I32, mxml, 1), V2(mol_i, ixrfpt, SMMP_I32, 3, mxml), VS(mol_i, ireg, SMMP_I32),
+    VC(mol_c, nmml, 80, mxml),
+    V2(mol_r, gbpr, SMMP_F64, 6, mxml), V2(mol_r, rfpt, SMMP_F64, 3, mxml),
+    V1(res_i, iatrs1, SMMP_I32, mxrs, 1), V1(res_i, iatrs2, SMMP_I32, mxrs, 1), V1(res_i, ixatrs, SMMP_I32, mxrs, 1),
+    V1(res_i, ivrrs1, SMMP_I32, mxrs, 1), V1(res_i, nvrrs, SMMP_I32, mxrs, 1),
+    VC(res_c, seq, 4, mxrs),
+    V1(atm_i, ityat, SMMP_I32, mxat, 1), V1(atm_i, iowat, SMMP_I32, mxat, 1), V1(atm_i, iyowat, SMMP_I32, mxat, 1),
+    V1(atm_i, nbdat, SMMP_I32, mxat, 1), V2(atm_i, ibdat, SMMP_I32, mxbd, mxat), V2(atm_i, iybdat, SMMP_I32, mxbd, mxat),
+    V1(atm_i, ixmsat, SMMP_I32, mxat, 1),
+    V1(atm_r, cgat, SMMP_F64, mxat, 1), V1(atm_r, blat, SMMP_F64, mxat, 1), V1(atm_r, baat, SMMP_F64, mxat, 1),
+    V1(atm_r, toat, SMMP_F64, mxat, 1), V1(atm_r, sntoat, SMMP_F64, mxat, 1), V1(atm_r, cstoat, SMMP_F64, mxat, 1),
+    V1(atm_r, snbaat, SMMP_F64, mxat, 1), V1(atm_r, csbaat, SMMP_F64, mxat, 1),
+    V1(atm_r, xat, SMMP_F64, mxat + 2, -1), V1(atm_r, yat, SMMP_F64, mxat + 2, -1), V1(atm_r, zat, SMMP_F64, mxat + 2, -1),
+    VC(atm_c, nmat, 4, mxat),
+    V1(var_r, vlvr, SMMP_F64, mxvr, 1), V1(var_r, olvlvr, SMMP_F64, mxvr, 1), V1(var_r, axvr, SMMP_F64, mxvr, 1),
+    V1(var_i, iatvr, SMMP_I32, mxvr, 1), V1(var_i, ityvr, SMMP_I32, mxvr, 1), V1(var_i, iclvr, SMMP_I32, mxvr, 1),
+    V1(var_i, iorvr, SMMP_I32, mxvr, 1), V1(var_i, imsvr1, SMMP_I32, mxvr, 1), V1(var_i, imsvr2, SMMP_I32, mxvr, 1),
+    V1(var_i, latms1, SMMP_I32, mxms, 1), V1(var_i, latms2, SMMP_I32, mxms, 1), V1(var_i, iadvr1, SMMP_I32, mxvr, 1),
+    V1(var_i, iadvr2, SMMP_I32, mxvr, 1), V1(var_i, ladvr, SMMP_I32, mxvr, 1), V1(var_i, idvr, SMMP_I32, mxvr, 1),
+    VC(var_c, nmvr, 3, mxvr),
+    V1(var_l, fxvr, SMMP_I32, mxvr, 1),
+    V2(epar_i, ihbty, SMMP_I32, mxtyat, mxtyat),
+    V2(epar_r, cij, SMMP_F64, mxtyat, mxtyat), V2(epar_r, aij, SMMP_F64, mxtyat, mxtyat),
+    V2(epar_r, a14, SMMP_F64, mxtyat, mxtyat), V2(epar_r, chb, SMMP_F64, mxtyat, mxtyat),
+    V2(epar_r, ahb, SMMP_F64, mxtyat, mxtyat), V1(epar_r, e0to, SMMP_F64, mxtyto, 1),
+    V1(epar_r, sgto, SMMP_F64, mxtyto, 1), V1(epar_r, rnto, SMMP_F64, mxtyto, 1),
+    VS(epar_r, conv, SMMP_F64), VS(epar_r, plt, SMMP_F64), VS(epar_r, slp, SMMP_F64),
+    V1(eny_i, ivwat1, SMMP_I32, mxat, 1), V1(eny_i, ivwat2, SMMP_I32, mxat, 1),
+    V1(eny_i, i14at1, SMMP_I32, mxat, 1), V1(eny_i, i14at2, SMMP_I32, mxat, 1),
+    V1(eny_i, lvwat1, SMMP_I32, mxvw, 1), V1(eny_i, lvwat2, SMMP_I32, mxvw, 1), V1(eny_i, l14at, SMMP_I32, mx14, 1),
+    V1(eny_i, nvwml, SMMP_I32, mxml, 1), V1(eny_i, ivwml1, SMMP_I32, mxml, 1),
+    V1(eny_i, n14ml, SMMP_I32, mxml, 1), V1(eny_i, i14ml1, SMMP_I32, mxml, 1),
+    VS(eny_r, eyel, SMMP_F64), VS(eny_r, eyvw, SMMP_F64), VS(eny_r, eyhb, SMMP_F64), VS(eny_r, eyvr, SMMP_F64),
+    VS(eny_r, eysm, SMMP_F64), VS(eny_r, eysl, SMMP_F64),
+    V1(solvent, rvdw, SMMP_F64, mxat, 1), V1(solvent, sigma, SMMP_F64, mxat, 1),
+    V2(tes, spoint, SMMP_F64, mxpoint, 3), VS(pnt, npnt, SMMP_I32),
+    VS(v_sol, vdvol, SMMP_F64), VS(v_sol, asa, SMMP_F64),
+    V2(h_bond, mmhb, SMMP_I32, mxml, mxml),
+    VS(myblock, mti, SMMP_I32), V1(myblock, mt, SMMP_I32, 624, 0),
+};
+constexpr int kNViews = sizeof(kViews) / sizeof(kViews[0]);
+
+int fail(smmp_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+template <typename F>
+int guarded(smmp_ctx* c, int code, F&& f) {
+  if (!c) return SMMP_EINVAL;
+  try {
+    c->err.clear();
+    return f();
+  } catch (const Error& e) {
+    return fail(c, code, e.msg);
+  } catch (const DeviceError& e) {
+    return fail(c, SMMP_EDEVICE, e.msg);
+  } catch (const std::exception& e) {
+    return fail(c, SMMP_EINVAL, e.what());
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int smmp_ctx_create(smmp_ctx** out) {
+  if (!out) return SMMP_EINVAL;
+  auto* c = new (std::nothrow) smmp_ctx();
+  if (!c) return SMMP_EINVAL;
+  reset_state(c->s);
+  *out = c;
+  return SMMP_OK;
+}
+
+void smmp_ctx_destroy(smmp_ctx* c) {
+  if (!c) return;
+  engine_destroy(c->eng);
+  delete c;
+}
+
+const char* smmp_last_error(const smmp_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int smmp_state_count(void) { return kNViews; }
+const char* smmp_state_name(int i) { return (i >= 0 && i < kNViews) ? kViews[i].name : nullptr; }
+
+int smmp_state_view(smmp_ctx* c, const char* name, void** ptr, int* dtype, int* ndim, int64_t* shape,
+                    int64_t* lbound) {
+  if (!c || !name) return SMMP_EINVAL;
+  for (const auto& v : kViews) {
+    if (std::strcmp(v.name, name) != 0) continue;
+    if (ptr) *ptr = v.ptr(c->s);
+    if (dtype) *dtype = v.dtype;
+    if (ndim) *ndim = v.ndim;
+    for (int k = 0; k < 2; ++k) {
+      if (shape) shape[k] = v.shape[k];
+      if (lbound) lbound[k] = v.lbound[k];
+    }
+    return SMMP_OK;
+  }
+  return fail(c, SMMP_EINVAL, std::string("unknown state variable ") + name);
+}
+
+int smmp_init_energy(smmp_ctx* c, const char* libdir) {
+  return guarded(c, SMMP_EINPUT, [&] {
+    if (!libdir) throw Error("init_energy> libdir is NULL");
+    init_energy(c->s, libdir);
+    engine_invalidate(c->eng);
+    return SMMP_OK;
+  });
+}
+
+int smmp_init_molecule(smmp_ctx* c, int iabin, const char* grpn, const char* grpc, const char* seqfile,
+                       const char* varfile) {
+  return guarded(c, SMMP_EINPUT, [&] {
+    init_molecule(c->s, iabin, grpn ? grpn : "nh2", grpc ? grpc : "cooh", seqfile ? seqfile : "",
+                  varfile ? varfile : "");
+    engine_invalidate(c->eng);
+    return SMMP_OK;
+  });
+}
+
+int smmp_setvar(smmp_ctx* c, int nml, const double* vlvrx) {
+  return guarded(c, SMMP_EINVAL, [&] {
+    if (nml < 1 || nml > c->s.ntlml) throw Error("setvar> no such molecule");
+    setvar(c->s, nml, vlvrx ? vlvrx : c->s.vlvr.data());
+    return SMMP_OK;
+  });
+}
+
+int smmp_outvar(smmp_ctx* c, int nml, const char* path) {
+  return guarded(c, SMMP_EINVAL, [&] {
+    write_var_file(c->s, nml, path ? path : "");
+    return SMMP_OK;
+  });
+}
+
+int smmp_sgrnd(smmp_ctx* c, int32_t seed) {
+  if (!c) return SMMP_EINVAL;
+  c->s.mt(0) = seed;
+  for (int i = 1; i < 624; ++i) c->s.mt(i) = (int32_t)(69069u * (uint32_t)c->s.mt(i - 1));
+  c->s.mti = 624;
+  return SMMP_OK;
+}
+
+int smmp_grnd(smmp_ctx* c, double* u) {
+  if (!c || !u) return SMMP_EINVAL;
+  *u = host_grnd(c->s);
+  return SMMP_OK;
+}
+
+int smmp_energy(smmp_ctx* c, double* e) {
+  return guarded(c, SMMP_ESTATE, [&] {
+    if (c->s.ntlml < 1) throw Error("energy> no molecule initialised");
+    double v = engine_energy(c->eng, c->s);
+    if (e) *e = v;
+    return SMMP_OK;
+  });
+}
+
+int smmp_enysol(smmp_ctx* c, int nmol, double* e) {
+  return guarded(c, SMMP_ESTATE, [&] {
+    if (c->s.ntlml < 1) throw Error("enysol> no molecule initialised");
+    if (nmol != 0 && (nmol < 1 || nmol > c->s.ntlml)) throw Error("enysol> no such molecule");
+    double v = engine_enysol(c->eng, c->s);
+    if (e) *e = v;
+    return SMMP_OK;
+  });
+}
+
+int smmp_metropolis(smmp_ctx* c, double* eol, double* acz, const smmp_weight* w) {
+  return guarded(c, SMMP_ESTATE, [&] {
+    if (!eol || !acz || !w) throw Error("metropolis> NULL argument");
+    if (c->s.ntlml < 1) throw Error("metropolis> no molecule initialised");
+    return engine_metropolis(c->eng, c->s, eol, acz, *w);
+  });
+}
+
+int smmp_partem_s(smmp_ctx* c, int num_rep, int nequi, int nswp, const double* temps, double* eol,
+                  int32_t* ipoi, double* traj) {
+  return guarded(c, SMMP_ESTATE, [&] {
+    if (num_rep < 1 || num_rep > 2048 || nequi < 0 || nswp < 0 || !temps) throw Error("partem_s> bad arguments");
+    if (c->s.ntlml < 1) throw Error("partem_s> no molecule initialised");
+    return engine_partem_s(c->eng, c->s, num_rep, nequi, nswp, temps, eol, ipoi, traj);
+  });
+}
+
+int smmp_energy_batch(smmp_ctx* const* ctxs, int nsys, const int32_t* conf_sys, int nconf, const double* vlvr,
+                      int precision, double* out) {
+  if (!ctxs || nsys < 1 || !conf_sys || nconf < 0 || !vlvr || !out) return SMMP_EINVAL;
+  smmp_ctx* c0 = ctxs[0];
+  return guarded(c0, SMMP_ESTATE, [&] {
+    std::vector<State*> sys(nsys);
+    for (int i = 0; i < nsys; ++i) {
+      if (!ctxs[i] || ctxs[i]->s.ntlml < 1) throw Error("energy_batch> context without a molecule");
+      sys[i] = &ctxs[i]->s;
+    }
+    for (int k = 0; k < nconf; ++k)
+      if (conf_sys[k] < 0 || conf_sys[k] >= nsys) throw Error("energy_batch> conf_sys out of range");
+    if (precision != 0 && precision != 1) throw Error("energy_batch> precision must be 0 or 1");
+    return engine_energy_batch(c0->eng, sys, conf_sys, nconf, vlvr, precision, out);
+  });
+}
+
+int smmp_sample(smmp_ctx* c, const smmp_sampler_cfg* cfg, double* vlvr, double* eol, double* acc, int32_t* ipoi,
+                int32_t* hist, double* traj) {
+  return guarded(c, SMMP_ESTATE, [&] {
+    if (!cfg || !vlvr || !eol || !acc || !ipoi) throw Error("sample> NULL argument");
+    if (cfg->nrep < 1 || cfg->nsweep < 0 || !cfg->temps || !cfg->seeds) throw Error("sample> bad configuration");
+    if (cfg->mode == 2 && !cfg->w) throw Error("sample> multicanonical mode needs a weight");
+    if (c->s.ntlml != 1) throw Error("sample> the replica sampler handles one chain");
+    return engine_sample(c->eng, c->s, *cfg, vlvr, eol, acc, ipoi, hist, traj);
+  });
+}
+
+const char* smmp_build_info(void) { return engine_build_info(); }
+
+}  // extern "C"

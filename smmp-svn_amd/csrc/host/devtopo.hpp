@@ -1,0 +1,50 @@
+// Device-ready description of one single-chain system, derived from the
+// host State after init_molecule: the chain-build schedule, the interaction
+// set as symmetric 2-plane bit rows, the torsion terms, the solvent atoms
+// and, per variable, the atoms a torsion change moves.
+#pragma once
+#include <cstdint>
+#include <vector>
+
+#include "state.hpp"
+
+namespace smmp {
+
+constexpr int kTypes = mxtyat;  // 18 ECEPP atom types
+
+struct HostSys {
+  int nat = 0, nvt = 0, nlev = 0, words = 0, nheavy = 0;
+  int i2 = 0, i3 = 0;  // 0-based second/third backbone atoms (bldmol fnd3ba)
+  double gbpr[6] = {0, 0, 0, 0, 0, 0};
+  // atoms (0-based)
+  std::vector<int16_t> parent;      // iowat-1, -1 for the first atom
+  std::vector<int16_t> lev_atoms;   // atoms in build order (first atom excluded)
+  std::vector<int32_t> lev_off;     // nlev+1 offsets into lev_atoms
+  std::vector<int8_t> type;         // ityat-1
+  std::vector<double> q, bl, cba, sba, toat0, tref, rvdw, sigma;
+  std::vector<int16_t> tvar;        // variable (0-based) whose value sets this torsion, -1 if fixed
+  // variables (0-based over all variables of the chain)
+  std::vector<int16_t> var_atom;    // iatvr-1
+  std::vector<int16_t> var_base;    // iowat(iatvr)-1: axis end, not moved
+  std::vector<int16_t> var_axis0;   // iowat(base)-1: axis start
+  std::vector<int16_t> var_m1, var_m2;  // moved atoms [m1, m2]
+  std::vector<double> var_e0, var_sg, var_rn;  // torsion potential (e0to/2, sign, period)
+  std::vector<int32_t> var_is_torsion;
+  // interaction set: bits[(i*words + w)*2 + plane], plane 0 = pair counted,
+  // plane 1 = 1-4 pair (half repulsion); symmetric in (i, j)
+  std::vector<uint64_t> bits;
+  int64_t npairs = 0, n14 = 0;
+  std::vector<int16_t> heavy;       // atoms with rvdw > 0
+  // parameters, [ti*18 + tj]: A (aij or ahb), A14 (a14 or ahb), C (cij or chb), hb flag
+  std::vector<double> pA, pA14, pC;
+  std::vector<int32_t> pHB;
+  double conv = 0, plt = 0, slp = 0;
+  int epsd = 0, itysol = 0, npnt = 0;
+  std::vector<double> spoint;       // npnt*3, point-major (x,y,z)
+};
+
+// Builds the description for molecule 1 of s (single chain).  Throws Error
+// when a list pair occurs twice or a moving set is not contiguous.
+HostSys build_host_sys(const State& s);
+
+}  // namespace smmp

@@ -1,0 +1,40 @@
+// Boundary between the host library and the HIP engine (device/engine.hip).
+#pragma once
+#include <string>
+#include <vector>
+
+#include "../../../include/smmp_mi355x.h"
+#include "state.hpp"
+
+namespace smmp {
+
+struct Engine;  // device mirror of one context (opaque to the host code)
+
+struct DeviceError {
+  std::string msg;
+  explicit DeviceError(std::string m) : msg(std::move(m)) {}
+};
+
+void engine_destroy(Engine* e);
+void engine_invalidate(Engine* e);  // topology changed: rebuild device data on next use
+
+// energy.f:12 for the context's current vlvr; writes eyel.. and coordinates
+double engine_energy(Engine*& e, State& s);
+// enysol.f:13 for the current coordinates
+double engine_enysol(Engine*& e, State& s);
+// metropolis.f:13 (context MT stream)
+int engine_metropolis(Engine*& e, State& s, double* eol, double* acz, const smmp_weight& w);
+// partem_s.f:13, serial-compatible single stream
+int engine_partem_s(Engine*& e, State& s, int nrep, int nequi, int nswp, const double* temps, double* eol,
+                    int32_t* ipoi, double* traj);
+int engine_energy_batch(Engine*& e, std::vector<State*>& sys, const int32_t* conf_sys, int nconf,
+                        const double* vlvr, int precision, double* out);
+int engine_sample(Engine*& e, State& s, const smmp_sampler_cfg& cfg, double* vlvr, double* eol, double* acc,
+                  int32_t* ipoi, int32_t* hist, double* traj);
+const char* engine_build_info();
+
+// twister.f:104 grnd() on the context's stream (host side; the draws of the
+// device samplers come from the same state copied to the device)
+double host_grnd(State& s);
+
+}  // namespace smmp

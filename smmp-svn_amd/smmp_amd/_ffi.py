@@ -1,0 +1,151 @@
+"""ctypes binding of the C-ABI in include/smmp_mi355x.h.
+
+Loads ``libsmmp_mi355x.so`` built in-tree (``smmp-svn_amd/csrc/build``).
+There is no fallback: a missing or unloadable library raises ImportError.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+LIBPATH = os.environ.get("SMMP_LIB") or os.path.join(PKG, "csrc", "build", "libsmmp_mi355x.so")
+
+SMMP_I32, SMMP_F64, SMMP_CHR = 1, 2, 3
+ERRORS = {-1: "invalid argument", -2: "input error", -3: "device error", -4: "energy drift",
+          -5: "state error"}
+
+
+class SmmpError(RuntimeError):
+    """An error status of the C-ABI (the reference stops the process instead)."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Weight(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("beta", C.c_double), ("kmin", C.c_int32), ("kmax", C.c_int32),
+                ("ebin", C.c_double), ("b", C.POINTER(C.c_double)), ("alpha", C.POINTER(C.c_double))]
+
+
+class SamplerCfg(C.Structure):
+    _fields_ = [("nrep", C.c_int32), ("nsweep", C.c_int32), ("mode", C.c_int32), ("precision", C.c_int32),
+                ("temps", C.POINTER(C.c_double)), ("seeds", C.POINTER(C.c_uint32)),
+                ("w", C.POINTER(Weight)), ("resync_every", C.c_int32)]
+
+
+def _load():
+    if not os.path.exists(LIBPATH):
+        raise ImportError(f"libsmmp_mi355x.so not built ({LIBPATH}); run __graft_entry__.build()")
+    lib = C.CDLL(LIBPATH)
+    P = C.c_void_p
+    dp = C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int32)
+    sig = {
+        "smmp_ctx_create": (C.c_int, [C.POINTER(P)]),
+        "smmp_ctx_destroy": (None, [P]),
+        "smmp_last_error": (C.c_char_p, [P]),
+        "smmp_state_view": (C.c_int, [P, C.c_char_p, C.POINTER(P), ip, ip, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+        "smmp_state_count": (C.c_int, []),
+        "smmp_state_name": (C.c_char_p, [C.c_int]),
+        "smmp_init_energy": (C.c_int, [P, C.c_char_p]),
+        "smmp_init_molecule": (C.c_int, [P, C.c_int, C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p]),
+        "smmp_setvar": (C.c_int, [P, C.c_int, dp]),
+        "smmp_energy": (C.c_int, [P, dp]),
+        "smmp_enysol": (C.c_int, [P, C.c_int, dp]),
+        "smmp_sgrnd": (C.c_int, [P, C.c_int32]),
+        "smmp_grnd": (C.c_int, [P, dp]),
+        "smmp_outvar": (C.c_int, [P, C.c_int, C.c_char_p]),
+        "smmp_metropolis": (C.c_int, [P, dp, dp, C.POINTER(Weight)]),
+        "smmp_partem_s": (C.c_int, [P, C.c_int, C.c_int, C.c_int, dp, dp, ip, dp]),
+        "smmp_energy_batch": (C.c_int, [C.POINTER(P), C.c_int, ip, C.c_int, dp, C.c_int, dp]),
+        "smmp_sample": (C.c_int, [P, C.POINTER(SamplerCfg), dp, dp, dp, ip, ip, dp]),
+        "smmp_build_info": (C.c_char_p, []),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+lib = _load()
+
+#: every symbol the header declares (checked by the CPU tests)
+EXPORTS = ("smmp_ctx_create", "smmp_ctx_destroy", "smmp_last_error", "smmp_state_view", "smmp_state_count",
+           "smmp_state_name", "smmp_init_energy", "smmp_init_molecule", "smmp_setvar", "smmp_energy",
+           "smmp_enysol", "smmp_sgrnd", "smmp_grnd", "smmp_outvar", "smmp_metropolis", "smmp_partem_s",
+           "smmp_energy_batch", "smmp_sample", "smmp_build_info")
+
+
+def dptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def iptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+class Context:
+    """Owns one smmp_ctx (one system: the reference's COMMON blocks)."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        self.check(lib.smmp_ctx_create(C.byref(h)), None)
+        self.h = h
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h:
+            lib.smmp_ctx_destroy(h)
+            self.h = None
+
+    def check(self, rc: int, h=None) -> None:
+        if rc != 0:
+            hh = h if h is not None else getattr(self, "h", None)
+            msg = lib.smmp_last_error(hh).decode() if hh else ""
+            raise SmmpError(rc, msg)
+
+    def view(self, name: str) -> np.ndarray:
+        """numpy view (Fortran order) of a COMMON variable, e.g. 'atm_r.xat'."""
+        ptr = C.c_void_p()
+        dt = C.c_int32()
+        nd = C.c_int32()
+        shape = (C.c_int64 * 2)()
+        lb = (C.c_int64 * 2)()
+        self.check(lib.smmp_state_view(self.h, name.encode(), C.byref(ptr), C.byref(dt), C.byref(nd), shape, lb))
+        if dt.value == SMMP_I32:
+            ct, npdt = C.c_int32, np.int32
+        elif dt.value == SMMP_F64:
+            ct, npdt = C.c_double, np.float64
+        else:
+            ct, npdt = C.c_char, "S1"
+        if nd.value == 0:
+            n = 1
+            shp = (1,)
+        elif nd.value == 1:
+            n = shape[0]
+            shp = (shape[0],)
+        else:
+            n = shape[0] * shape[1]
+            shp = (shape[0], shape[1])
+        buf = (ct * n).from_address(ptr.value)
+        arr = np.ctypeslib.as_array(buf) if npdt != "S1" else np.frombuffer(buf, dtype="S1", count=n)
+        arr = arr.view(npdt) if npdt != "S1" else arr
+        return arr.reshape(shp, order="F")
+
+    def init_energy(self, libdir: str) -> None:
+        self.check(lib.smmp_init_energy(self.h, libdir.encode()))
+
+    def init_molecule(self, iabin: int, grpn: str, grpc: str, seqfile: str, varfile: str = " ") -> None:
+        self.check(lib.smmp_init_molecule(self.h, iabin, grpn.encode(), grpc.encode(), seqfile.encode(),
+                                          varfile.encode()))
+
+    def energy(self) -> float:
+        e = C.c_double()
+        self.check(lib.smmp_energy(self.h, C.byref(e)))
+        return e.value

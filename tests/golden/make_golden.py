@@ -1,0 +1,176 @@
+#!/usr/bin/env python3
+"""Writes the golden fixtures under tests/golden/ from the SMMP Fortran reference.
+
+Runs ``oracle/_ref/ref_driver`` (the reference's own objects compiled with
+amdflang by ``make -C oracle ref``, linked with our driver) in a scratch
+directory and stores its outputs as small ``.npz`` files (loadable with
+``allow_pickle=False``).  Needs /root/reference; run it in the build container,
+never on a GPU box.
+
+    python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(REPO, "smmp-svn_amd"))
+from smmp_amd import workloads  # noqa: E402
+
+DRIVER = os.path.join(REPO, "oracle", "_ref", "ref_driver")
+LIBDIR = os.path.join(HERE, "SMMP") + "/"
+
+
+def parse(text: str) -> dict[str, list]:
+    """Parses '@name type count' blocks; repeated names become lists of arrays."""
+    out: dict[str, list] = {}
+    lines = text.splitlines()
+    i = 0
+    while i < len(lines):
+        line = lines[i]
+        if not line.startswith("@"):
+            i += 1
+            continue
+        name, typ, n = line[1:].split()
+        n = int(n)
+        vals = lines[i + 1:i + 1 + n]
+        if typ == "int":
+            arr = np.array([int(v) for v in vals], dtype=np.int64)
+        elif typ == "double":
+            arr = np.array([float(v.replace("D", "E")) for v in vals], dtype=np.float64)
+        else:
+            arr = np.array([v.rstrip() for v in vals], dtype="<U8")
+        out.setdefault(name, []).append(arr)
+        i += n + 1
+    return out
+
+
+def run(args: list[str], cwd: str) -> dict[str, list]:
+    res = subprocess.run([DRIVER] + args, cwd=cwd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"ref_driver {' '.join(args)} failed:\n{res.stdout[-2000:]}\n{res.stderr[-2000:]}")
+    return parse(res.stdout)
+
+
+def first(d: dict[str, list]) -> dict[str, np.ndarray]:
+    return {k: v[0] for k, v in d.items()}
+
+
+def vars_file(path: str, topo: dict[str, np.ndarray], varfile: str, fix_omega: bool) -> str:
+    """Maps a reference .var file (residue : name : degrees, and 'om*:..&')
+    onto variable indices for the driver's 'index value fixed' format."""
+    nm = [s.strip() for s in topo["nmvr"]]
+    ivrrs1 = topo["ivrrs1"]
+    nvrrs = topo["nvrrs"]
+    vals: dict[int, tuple[float, int]] = {}
+    for line in open(varfile):
+        line = line.strip()
+        if not line or "#" in line:
+            continue
+        parts = [p.strip() for p in line.split(":")]
+        if len(parts) != 3:
+            continue
+        r, name, val = int(parts[0]), parts[1], float(parts[2])
+        for iv in range(ivrrs1[r - 1], ivrrs1[r - 1] + nvrrs[r - 1]):
+            if nm[iv - 1] == name:
+                vals[iv] = (val, 0)
+    if fix_omega:
+        for iv, name in enumerate(nm, 1):
+            if name.startswith("om"):
+                vals[iv] = (180.0, 1)
+    with open(path, "w") as f:
+        for iv in sorted(vals):
+            f.write(f"{iv} {vals[iv][0]!r} {vals[iv][1]}\n")
+    return path
+
+
+def save(name: str, **arrays) -> None:
+    path = os.path.join(HERE, name)
+    np.savez_compressed(path, **arrays)
+    print(f"wrote {path} ({os.path.getsize(path)} bytes)")
+
+
+def topo_fixture(tag: str, seq: list[str], tmp: str, nconf: int, seed: int,
+                 varf: str = "-") -> dict[str, np.ndarray]:
+    seqf = workloads.write_seq_file(os.path.join(tmp, f"{tag}.seq"), seq, tag)
+    topo = first(run(["topo", LIBDIR, seqf, varf, "1"], tmp))
+    ene = run(["energy", LIBDIR, seqf, varf, "1", str(nconf), str(seed)], tmp)
+    arrays = dict(topo)
+    arrays["seq_in"] = np.array(seq, dtype="<U8")
+    arrays["conf_vlvr"] = np.stack(ene["vlvr"])
+    arrays["conf_energy"] = np.stack(ene["energy"])
+    arrays["conf_asa"] = np.stack(ene["asa"])
+    arrays["conf_xyz"] = np.stack(ene["xyz"])
+    arrays["conf_seed"] = np.array([seed])
+    save(f"topo_{tag}.npz", **arrays)
+    return topo
+
+
+def main() -> None:
+    if not os.path.exists(DRIVER):
+        raise SystemExit("build the reference first: make -C oracle ref")
+    tmp = tempfile.mkdtemp(prefix="smmp_golden_")
+
+    # MT19937 streams (twister.f): the default seed and a few others.
+    mt = {}
+    for s in (4357, 81236, 1, 2147483647, -5):
+        u = run(["mt", str(s), "2000"], tmp)["mt"][0]
+        mt[f"seed_{s}".replace("-", "m")] = np.rint(u * 4294967295.0).astype(np.uint64)
+    save("mt19937.npz", **mt)
+
+    # Met-enkephalin (EXAMPLES/enkefa.seq): topology, energies, the known answer.
+    enk = [s.lower() for s in open(os.path.join(HERE, "enkefa.seq")).read().split()[2:]]
+    t_enk = topo_fixture("enkefa", enk, tmp, nconf=8, seed=11)
+    vf = vars_file(os.path.join(tmp, "enk.vars"), t_enk, os.path.join(HERE, "enkefa.var"), False)
+    seqf = os.path.join(tmp, "enkefa.seq")
+    kn = first(run(["topo", LIBDIR, seqf, vf, "0"], tmp))
+    vf_fix = vars_file(os.path.join(tmp, "enk_fix.vars"), t_enk, os.path.join(HERE, "enkefa.var"), True)
+    kn_fix = first(run(["topo", LIBDIR, seqf, vf_fix, "0"], tmp))
+    canon = run(["canon", LIBDIR, seqf, vf_fix, "0", "81236", "300.0", "40"], tmp)
+    canon_sol = run(["canon", LIBDIR, seqf, vf, "1", "4357", "500.0", "10"], tmp)
+    save("enkefa_runs.npz",
+         known_energy=kn["energy"], known_vlvr=kn["vlvr_all"], known_xyz=kn["xyz"],
+         fixed_nvr=kn_fix["nvr"], fixed_idvr=kn_fix["idvr"], fixed_vlvr=kn_fix["vlvr_all"],
+         fixed_axvr=kn_fix["axvr"], fixed_energy=kn_fix["energy"],
+         canon_e0=canon["canon_e0"][0], canon_sweeps=np.stack(canon["sweep"]),
+         canon_vlvr=canon["vlvr"][0], canon_mti=canon["mti"][0],
+         canon_sol_e0=canon_sol["canon_e0"][0], canon_sol_sweeps=np.stack(canon_sol["sweep"]),
+         canon_sol_vlvr=canon_sol["vlvr"][0])
+
+    # Sequences of the benchmark shapes and edge cases of the topology builder.
+    cases = {
+        "c2_20mer": workloads.random_sequence(20, 2),
+        "c5_36mer": workloads.random_sequence(36, 5),
+        "len10": workloads.random_sequence(10, 10),
+        "len60": workloads.random_sequence(60, 60),
+        "glyends": ["gly", "trp", "his", "arg", "pro", "cys", "gly"],
+        "pro_first": ["pro", "lys", "gln", "glu", "asn", "asp", "ile", "thr", "ser", "gly"],
+        "allres": list(workloads.RESIDUES),
+    }
+    for tag, seq in cases.items():
+        topo_fixture(tag, seq, tmp, nconf=6, seed=len(seq) + 7)
+
+    # partem_s on a 10-mer: 4 replicas, time.d at every sweep, par_R.in at the end.
+    seqf = workloads.write_seq_file(os.path.join(tmp, "pt.seq"), cases["len10"], "pt")
+    temps = workloads.geometric_temperatures(4, 250.0, 1000.0)
+    workloads.write_temperatures(os.path.join(tmp, "temperatures"), temps)
+    run(["partem", LIBDIR, seqf, "1", "777", "4", "2", "12"], tmp)
+    td = np.loadtxt(os.path.join(tmp, "time.d"))
+    raw = open(os.path.join(tmp, "par_R.in"), "rb").read()
+    # unformatted sequential: [4-byte len][int nsw][4-byte len][4-byte len][R doubles][4-byte len]
+    nrep = 4
+    rec = 4 + 4 + 4 + 4 + 8 * nrep + 4
+    eol_final = np.frombuffer(raw[12 + 4:12 + 4 + 8 * nrep], dtype="<f8").copy()
+    assert len(raw) == nrep * rec, len(raw)
+    save("partem_len10.npz", temps=temps, time_d=td, eol_final=eol_final,
+         seed=np.array([777]), nequi=np.array([2]), nswp=np.array([12]))
+
+
+if __name__ == "__main__":
+    main()
