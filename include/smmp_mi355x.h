@@ -83,12 +83,13 @@ typedef struct {
 int smmp_metropolis(smmp_ctx *ctx, double *eol, double *acz, const smmp_weight *w);
 
 /* partem_s(num_rep,nequi,nswp,nmes,nsave,newsta,switch) partem_s.f:13 with a
- * new start (newsta) and random initial dihedrals (switch=1).  temps: R
- * temperatures.  Serial-compatible mode: every replica draws from the
- * context's single MT stream in slot order, as the reference does.
+ * new start (newsta): switch > 0 random initial torsions, < 0 all 180
+ * degrees, 0 all zero (partem_s.f:114-126).  temps: R temperatures (the
+ * 'temperatures' file).  Serial-compatible mode: every replica draws from
+ * the context's single MT stream in slot order, as the reference does.
  * Outputs (may be NULL): eol[R] per replica, ipoi[R] slot->replica (1-based),
- * traj[nswp*R] energy per sweep and slot. */
-int smmp_partem_s(smmp_ctx *ctx, int num_rep, int nequi, int nswp, const double *temps,
+ * traj[nswp*R] energy per sweep and slot (the energies of time.d). */
+int smmp_partem_s(smmp_ctx *ctx, int num_rep, int nequi, int nswp, int switch_, const double *temps,
                   double *eol, int32_t *ipoi, double *traj);
 
 /* ---- batched entry points (no reference counterpart: the configs'
@@ -121,12 +122,22 @@ typedef struct {
   const uint32_t *seeds;
   const smmp_weight *w;
   int32_t resync_every;   /* full recomputation cadence in sweeps (1 = every sweep) */
+  int32_t verify;         /* 1: recompute every trial in full and fail on the first
+                             incremental component that disagrees (slow; tests) */
 } smmp_sampler_cfg;
 int smmp_sample(smmp_ctx *ctx, const smmp_sampler_cfg *cfg, double *vlvr, double *eol,
                 double *acc, int32_t *ipoi, int32_t *hist, double *traj);
 
 /* Device / build information string (arch, kernels, tile shapes). */
 const char *smmp_build_info(void);
+
+/* Kernel timing (no reference counterpart): when enabled, every launch on
+ * the context's stream is bracketed by HIP events.  kernel: 0 = sweep,
+ * 1 = exchange, 2 = energy.  work[4]: pair-term evaluations, solvent
+ * contacts (each tested against all tessellation points), atoms placed by
+ * chain builds, atoms rotated by trial moves, summed over the launches. */
+int smmp_profile(smmp_ctx *ctx, int enable);
+int smmp_profile_read(smmp_ctx *ctx, int kernel, double *total_ms, int64_t *launches, double *work);
 
 #ifdef __cplusplus
 }

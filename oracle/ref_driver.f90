@@ -10,7 +10,7 @@
 !   topo   libdir seqfile varfile itysol           topology, lists, tables
 !   energy libdir seqfile varfile itysol nconf seed  random conformations
 !   canon  libdir seqfile varfile itysol seed temp nsweep  Metropolis run
-!   partem libdir seqfile itysol seed nrep nequi nswp      partem_s run
+!   partem libdir seqfile itysol seed nrep nequi nswp [switch]  partem_s run
 !   muca   libdir seqfile itysol seed nsweep nup temp kmin kmax ebin
 program ref_driver
   use multicanonical
@@ -21,7 +21,7 @@ program ref_driver
   character(len=80) :: libdir, seqf, varf
   character(len=4) :: grpn, grpc
   integer :: iseed, n, i, j, k, nconf, nsw, isol, nrep, nequi, nswp
-  integer :: kmn, kmx, nup
+  integer :: kmn, kmx, nup, iswt
   double precision :: e, eolx, acz, tmp, ebw
   double precision :: ggasc
   parameter (ggasc = 1.98773d-3)
@@ -100,7 +100,9 @@ program ref_driver
     call iarg(7, nequi)
     call iarg(8, nswp)
     call sgrnd(iseed)
-    call partem_s(nrep, nequi, nswp, 1, nswp, .true., 1)
+    iswt = 1
+    if (command_argument_count() .ge. 9) call iarg(9, iswt)
+    call partem_s(nrep, nequi, nswp, 1, nswp, .true., iswt)
     write(*,'(a,i0)') '@done int 1'
     write(*,'(i0)') 1
 

@@ -424,6 +424,27 @@ double oracle_weight_eval(const oracle_weight *w, double x) {
   return w->b[k] * x + w->alpha[k];                                 /* muca_weight2 */
 }
 
+/* ------------------------------------------------ per-trial trace (tests) */
+static double *g_trace = NULL;
+static int g_trace_cap = 0, g_trace_n = 0, g_trace_rep = -1, g_cur_rep = -1;
+
+void oracle_set_trace(double *buf, int cap, int rep) {
+  g_trace = buf;
+  g_trace_cap = cap;
+  g_trace_n = 0;
+  g_trace_rep = rep;
+}
+int oracle_trace_count(void) { return g_trace_n; }
+
+static void trace_trial(int jv, double enw, double eol, int acc) {
+  if (!g_trace || g_cur_rep != g_trace_rep || g_trace_n >= g_trace_cap) return;
+  double *t = g_trace + 4 * g_trace_n++;
+  t[0] = jv - 1;
+  t[1] = enw;
+  t[2] = eol;
+  t[3] = acc;
+}
+
 /* ------------------------------------------------ metropolis.f:13-184 */
 
 int oracle_metropolis(const oracle_topo *t, oracle_conf *c, oracle_mt *g, double *vlvr,
@@ -439,6 +460,7 @@ int oracle_metropolis(const oracle_topo *t, oracle_conf *c, oracle_mt *g, double
     double enw = oracle_energy(t, c, vlvr);
     double delta = oracle_weight_eval(w, enw) - oracle_weight_eval(w, eol);
     double up = 0.0;
+    const double eprev = eol;
     if (delta <= 0.0) {
       eol = enw;
       up = 1.0;
@@ -453,6 +475,7 @@ int oracle_metropolis(const oracle_topo *t, oracle_conf *c, oracle_mt *g, double
         memcpy(vlvr, vrol, sizeof(double) * (size_t)t->nvt);
       }
     }
+    trace_trial(jv, enw, eprev, up != 0.0);
     *acz = *acz + up;
   }
   double enw = oracle_energy(t, c, vlvr);
@@ -466,7 +489,7 @@ int oracle_metropolis(const oracle_topo *t, oracle_conf *c, oracle_mt *g, double
 
 int oracle_partem(const oracle_topo *t, oracle_mt *g, const double *vlvr0, int nvr,
                   const int32_t *idvr, const double *axvr, int nrep, const double *temps,
-                  int nequi, int nswp, double *eol, int32_t *ipoi, double *coor, double *traj) {
+                  int nequi, int nswp, int switch_, double *eol, int32_t *ipoi, double *coor, double *traj) {
   oracle_conf c;
   if (oracle_conf_alloc(&c, t->nat)) return -1;
   oracle_conf_init(t, &c);
@@ -480,13 +503,20 @@ int oracle_partem(const oracle_topo *t, oracle_mt *g, const double *vlvr0, int n
   for (int i = 0; i < nrep; ++i)
     for (int j = 0; j < nvr; ++j) {
       int iv = idvr[j];
-      double dv = axvr[iv - 1] * (oracle_grnd(g) - 0.5);
-      coor[(size_t)i * nvr + j] = oracle_addang(PI, dv);
+      double vr = 0.0; /* switch = 0 leaves the zero-initialised start */
+      if (switch_ > 0) {
+        double dv = axvr[iv - 1] * (oracle_grnd(g) - 0.5);
+        vr = oracle_addang(PI, dv);
+      } else if (switch_ < 0) {
+        vr = PI;
+      }
+      coor[(size_t)i * nvr + j] = vr;
     }
   oracle_weight w = {0};
   int bad = 0;
   double acz = 0.0;
   for (int k = 0; k < nrep; ++k) {
+    g_cur_rep = k;
     w.beta = pbe[k];
     for (int i = 0; i < nvr; ++i) vlvr[idvr[i] - 1] = coor[(size_t)k * nvr + i];
     double energ = oracle_energy(t, &c, vlvr);
@@ -498,6 +528,7 @@ int oracle_partem(const oracle_topo *t, oracle_mt *g, const double *vlvr0, int n
   for (int nsw = 1; nsw <= nswp; ++nsw) {
     for (int k1 = 1; k1 <= nrep; ++k1) {
       int k = ipoi[k1 - 1] - 1;
+      g_cur_rep = k;
       w.beta = pbe[k1 - 1];
       double energ = eol[k];
       for (int i = 0; i < nvr; ++i) vlvr[idvr[i] - 1] = coor[(size_t)k * nvr + i];

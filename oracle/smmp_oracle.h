@@ -85,16 +85,22 @@ int oracle_metropolis(const oracle_topo *t, oracle_conf *c, oracle_mt *g, double
                       int nvr, const int32_t *idvr, const double *axvr,
                       const oracle_weight *w, double *eol, double *acz);
 
-/* partem_s.f:13-258 (new start, switch=1), all replicas on one stream.
+/* partem_s.f:13-258 (new start), all replicas on one stream.  switch_ > 0:
+ * random initial torsions, < 0: all at 180 degrees, 0: all at zero.
  * temps: R temperatures; outputs: eol[R] (by replica), ipoi[R] (slot ->
  * replica, 1-based), coor[R*nvr] final free variables, and per sweep and slot
  * the energy (traj[nswp*R], slot order, may be NULL). */
 int oracle_partem(const oracle_topo *t, oracle_mt *g, const double *vlvr0, int nvr,
                   const int32_t *idvr, const double *axvr, int nrep, const double *temps,
-                  int nequi, int nswp, double *eol, int32_t *ipoi, double *coor, double *traj);
+                  int nequi, int nswp, int switch_, double *eol, int32_t *ipoi, double *coor, double *traj);
 
 /* mulcan_par_mod.f90:23-190 weight recursion on one histogram update.
  * ihist (kmin..kmax) counts; b, alpha, g1, xhist updated in place. */
+/* Records (variable, trial energy, old energy, accepted) of every trial of
+ * replica 'rep' of oracle_partem (rep -1: of oracle_metropolis calls). */
+void oracle_set_trace(double *buf, int cap, int rep);
+int oracle_trace_count(void);
+
 void oracle_muca_update(int kmin, int kmax, double ebin, double beta, const int32_t *ihist,
                         double *xhist, double *g1, double *b, double *alpha);
 

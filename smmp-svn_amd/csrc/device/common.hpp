@@ -23,12 +23,20 @@ struct DevSys {
   const double *q, *bl, *cba, *sba, *toat0, *tref, *rvdw, *sigma;
   const int16_t* tvar;
   const int16_t *var_atom, *var_base, *var_axis0, *var_m1, *var_m2;
+  const double* var_sign;  // rotation of [m1,m2] is sign * (change of the torsion)
   const double *var_e0, *var_sg, *var_rn;
   const uint64_t* bits;
   const int16_t* heavy;
   const double *pA, *pA14, *pC;
   const int32_t* pHB;
   const double* spoint;  // npnt*3
+};
+
+// Work done by one thread, for the algorithmic flop count of a launch:
+// pair-term evaluations, solvent contacts (each tested against every
+// tessellation point), atoms placed by the chain build, atoms rotated.
+struct Work {
+  double pairs = 0, contacts = 0, builds = 0, moved = 0;
 };
 
 // difang/addang of difang.f with the same fmod/sign steps.

@@ -173,9 +173,12 @@ __device__ __forceinline__ void eyring_atom(const DevSys& S, EnergyLds& L, int a
 }
 
 // Chain build by wave 0, level by level; the other waves wait.
-__device__ void build_chain(const DevSys& S, EnergyLds& L) {
+__device__ void build_chain(const DevSys& S, EnergyLds& L, Work& wk) {
   if (threadIdx.x < kWave) {
-    if (threadIdx.x == 0) build_root(S, L);
+    if (threadIdx.x == 0) {
+      build_root(S, L);
+      wk.builds += S.nat;
+    }
     wave_lds_sync();
     const int lane = threadIdx.x;
     for (int lv = 0; lv < S.nlev; ++lv) {
@@ -203,7 +206,7 @@ struct PairSums {
 };
 
 // Pair terms over the tiles this wave owns (FP64).
-__device__ PairSums pair_terms_f64(const DevSys& S, const EnergyLds& L, int wave, int nwaves) {
+__device__ PairSums pair_terms_f64(const DevSys& S, const EnergyLds& L, int wave, int nwaves, Work& wk) {
   const int lane = threadIdx.x & (kWave - 1);
   const int nat = S.nat, nt = (nat + 63) / 64, ntile = nt * (nt + 1) / 2, words = S.words;
   double el = 0.0, vw = 0.0, hb = 0.0;
@@ -228,6 +231,7 @@ __device__ PairSums pair_terms_f64(const DevSys& S, const EnergyLds& L, int wave
       w1 &= keep;
     }
     const int kmax = min(64, nat - bj * 64);
+    wk.pairs += __popcll(w0 & (kmax == 64 ? ~0ull : ((1ull << kmax) - 1ull)));
     for (int k = 0; k < kmax; ++k) {
       const bool on = (w0 >> k) & 1ull;
       if (!__any(on)) continue;
@@ -260,7 +264,7 @@ __device__ PairSums pair_terms_f64(const DevSys& S, const EnergyLds& L, int wave
 }
 
 // Mixed precision: FP32 terms, FP32 sums per tile, FP64 accumulation.
-__device__ PairSums pair_terms_mixed(const DevSys& S, const EnergyLds& L, int wave, int nwaves) {
+__device__ PairSums pair_terms_mixed(const DevSys& S, const EnergyLds& L, int wave, int nwaves, Work& wk) {
   const int lane = threadIdx.x & (kWave - 1);
   const int nat = S.nat, nt = (nat + 63) / 64, ntile = nt * (nt + 1) / 2, words = S.words;
   double el = 0.0, vw = 0.0, hb = 0.0;
@@ -271,12 +275,13 @@ __device__ PairSums pair_terms_mixed(const DevSys& S, const EnergyLds& L, int wa
     const int i = bi * 64 + lane;
     const bool vi = i < nat;
     uint64_t w0 = 0, w1 = 0;
-    float xi = 0, yi = 0, zi = 0, qi = 0;
+    double xi = 0, yi = 0, zi = 0;
+    float qi = 0;
     int ti = 0;
     if (vi) {
       w0 = S.bits[((size_t)i * words + bj) * 2 + 0];
       w1 = S.bits[((size_t)i * words + bj) * 2 + 1];
-      xi = L.xf[i]; yi = L.yf[i]; zi = L.zf[i];
+      xi = L.x[i]; yi = L.y[i]; zi = L.z[i];
       qi = conv * L.qf[i];
       ti = L.t[i] * kTy;
     }
@@ -287,12 +292,14 @@ __device__ PairSums pair_terms_mixed(const DevSys& S, const EnergyLds& L, int wa
     }
     float fel = 0.f, fvw = 0.f, fhb = 0.f;
     const int kmax = min(64, nat - bj * 64);
+    wk.pairs += __popcll(w0 & (kmax == 64 ? ~0ull : ((1ull << kmax) - 1ull)));
     for (int k = 0; k < kmax; ++k) {
       const bool on = (w0 >> k) & 1ull;
       if (!__any(on)) continue;
       if (on) {
         const int j = bj * 64 + k;
-        float dx = L.xf[j] - xi, dy = L.yf[j] - yi, dz = L.zf[j] - zi;
+        // differences in FP64 (coordinates reach tens of Angstrom), the rest in FP32
+        float dx = (float)(L.x[j] - xi), dy = (float)(L.y[j] - yi), dz = (float)(L.z[j] - zi);
         float r2 = dx * dx + dy * dy + dz * dz;
         float ri = __builtin_amdgcn_rsqf(r2);
         float r2i = ri * ri;
@@ -328,71 +335,81 @@ __device__ double torsion_terms(const DevSys& S, const double* vl) {
   return e;
 }
 
+constexpr int kMaxPntPerLane = 4;  // tessellations of up to 256 points
+
 // enysol.f:192-324: area of heavy atom i from its free tessellation points.
-// Returns the number of free points, or -1 if the neighbour list overflows.
-__device__ int solvent_atom(const DevSys& S, const EnergyLds& L, double* nb, int i) {
+// Neighbours in contact are gathered into the wave's LDS scratch in chunks
+// of at most kMaxNb; a point is free when no neighbour of any chunk holds it
+// (the reference's 'last hit' cache only reorders this OR).  Returns the
+// number of free points, or -1 for a tessellation with more than 256 points.
+__device__ int solvent_atom(const DevSys& S, const EnergyLds& L, double* nb, int i, Work& wk) {
   const int lane = threadIdx.x & (kWave - 1);
+  if (S.npnt > kWave * kMaxPntPerLane) return -1;
   const double xi = L.x[i], yi = L.y[i], zi = L.z[i];
   const double trad = S.rvdw[i];
-  int cnt = 0;
-  for (int base = 0; base < S.nheavy; base += kWave) {
-    const int h = base + lane;
-    bool on = false;
-    double dx = 0, dy = 0, dz = 0, r2b = 0;
-    if (h < S.nheavy) {
-      const int j = S.heavy[h];
-      if (j != i) {
-        const double rj = S.rvdw[j];
-        const double ax = xi - L.x[j], ay = yi - L.y[j], az = zi - L.z[j];
-        const double lim = (trad + rj) * (1.0 + 1e-9);
-        if (ax * ax + ay * ay + az * az <= lim * lim) {  // cheap filter, then the reference test
-          dx = ax / trad;
-          dy = ay / trad;
-          dz = az / trad;
-          double dd = dx * dx + dy * dy + dz * dz;
-          double akrad = rj / trad;
-          double dr = 1.0 + akrad;
-          dr = dr * dr;
-          on = dd <= dr;
-          r2b = akrad * akrad;
+  bool occ[kMaxPntPerLane];
+#pragma unroll
+  for (int q = 0; q < kMaxPntPerLane; ++q) occ[q] = (q * kWave + lane) >= S.npnt;
+  int base = 0;
+  while (base < S.nheavy) {
+    int cnt = 0;
+    while (base < S.nheavy && cnt <= kMaxNb - kWave) {
+      const int h = base + lane;
+      bool on = false;
+      double dx = 0, dy = 0, dz = 0, r2b = 0;
+      if (h < S.nheavy) {
+        const int j = S.heavy[h];
+        if (j != i) {
+          const double rj = S.rvdw[j];
+          const double ax = xi - L.x[j], ay = yi - L.y[j], az = zi - L.z[j];
+          const double lim = (trad + rj) * (1.0 + 1e-9);
+          if (ax * ax + ay * ay + az * az <= lim * lim) {  // cheap filter, then the reference test
+            dx = ax / trad;
+            dy = ay / trad;
+            dz = az / trad;
+            double dd = dx * dx + dy * dy + dz * dz;
+            double akrad = rj / trad;
+            double dr = 1.0 + akrad;
+            dr = dr * dr;
+            on = dd <= dr;
+            r2b = akrad * akrad;
+          }
         }
       }
-    }
-    const uint64_t m = __ballot(on);
-    if (on) {
-      const int pos = cnt + __popcll(m & ((1ull << lane) - 1ull));
-      if (pos < kMaxNb) {
+      const uint64_t m = __ballot(on);
+      if (on) {
+        const int pos = cnt + __popcll(m & ((1ull << lane) - 1ull));
         nb[4 * pos + 0] = dx;
         nb[4 * pos + 1] = dy;
         nb[4 * pos + 2] = dz;
         nb[4 * pos + 3] = r2b;
       }
+      cnt += __popcll(m);
+      base += kWave;
     }
-    cnt += __popcll(m);
-  }
-  wave_lds_sync();
-  if (cnt > kMaxNb) return -1;
-  int nfree = 0;
-  for (int p0 = 0; p0 < S.npnt; p0 += kWave) {
-    const int p = p0 + lane;
-    bool fr = false;
-    if (p < S.npnt) {
+    if (lane == 0) wk.contacts += cnt;
+    wave_lds_sync();
+#pragma unroll
+    for (int q = 0; q < kMaxPntPerLane; ++q) {
+      const int p = q * kWave + lane;
+      if (occ[q]) continue;
       const double px = L.sp[3 * p + 0], py = L.sp[3 * p + 1], pz = L.sp[3 * p + 2];
-      fr = true;
       for (int k = 0; k < cnt; ++k) {
         double ax = nb[4 * k + 0] + px, ay = nb[4 * k + 1] + py, az = nb[4 * k + 2] + pz;
         double sdd = ax * ax;
         sdd = sdd + ay * ay;
         sdd = sdd + az * az;
         if (sdd <= nb[4 * k + 3]) {
-          fr = false;
+          occ[q] = true;
           break;
         }
       }
     }
-    nfree += __popcll(__ballot(fr));
+    wave_lds_sync();
   }
-  wave_lds_sync();
+  int nfree = 0;
+#pragma unroll
+  for (int q = 0; q < kMaxPntPerLane; ++q) nfree += __popcll(__ballot(!occ[q]));
   return nfree;
 }
 
@@ -406,7 +423,7 @@ struct EnergyOut {
 // when not null.  On return thread 0 holds the totals in 'out'.
 template <bool MIXED>
 __device__ void energy_from_coords(const DevSys& S, const double* vl, EnergyLds& L, int32_t* nfree_out,
-                                   EnergyOut& out) {
+                                   EnergyOut& out, Work& wk) {
   const int tid = threadIdx.x, nwaves = blockDim.x / kWave, wave = tid / kWave, lane = tid & (kWave - 1);
   if (MIXED) {  // FP32 copies centred on the first atom (frames are dead now)
     const double cx = L.x[0], cy = L.y[0], cz = L.z[0];
@@ -418,7 +435,7 @@ __device__ void energy_from_coords(const DevSys& S, const double* vl, EnergyLds&
     }
     __syncthreads();
   }
-  PairSums ps = MIXED ? pair_terms_mixed(S, L, wave, nwaves) : pair_terms_f64(S, L, wave, nwaves);
+  PairSums ps = MIXED ? pair_terms_mixed(S, L, wave, nwaves, wk) : pair_terms_f64(S, L, wave, nwaves, wk);
   double tor = torsion_terms(S, vl);
   if (MIXED) __syncthreads();  // the solvent scratch overlays the FP32 copies
   double sol = 0.0, asa = 0.0;
@@ -428,7 +445,7 @@ __device__ void energy_from_coords(const DevSys& S, const double* vl, EnergyLds&
     const double c4 = 4.0 * kPi / (double)S.npnt;
     for (int h = wave; h < S.nheavy; h += nwaves) {
       const int i = S.heavy[h];
-      int nfree = solvent_atom(S, L, nb, i);
+      int nfree = solvent_atom(S, L, nb, i, wk);
       if (nfree < 0) {
         err = 1;
         nfree = 0;
@@ -470,11 +487,11 @@ __device__ void energy_from_coords(const DevSys& S, const double* vl, EnergyLds&
 // Whole energy of one conformation from its variables (energy.f:12-119).
 template <bool MIXED>
 __device__ void conformation_energy(const DevSys& S, const double* vl, EnergyLds& L, int32_t* nfree_out,
-                                    EnergyOut& out) {
+                                    EnergyOut& out, Work& wk) {
   energy_stage_torsions(S, vl, L);
   __syncthreads();
-  build_chain(S, L);
-  energy_from_coords<MIXED>(S, vl, L, nfree_out, out);
+  build_chain(S, L, wk);
+  energy_from_coords<MIXED>(S, vl, L, nfree_out, out, wk);
 }
 
 }  // namespace smmp

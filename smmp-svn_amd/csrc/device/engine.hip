@@ -34,15 +34,17 @@ __global__ void __launch_bounds__(kEnergyThreads) k_energy(const DevSys* __restr
                                                            const int32_t* __restrict__ conf_sys,
                                                            const int64_t* __restrict__ conf_off,
                                                            const double* __restrict__ vlvr, double* __restrict__ out,
-                                                           double* __restrict__ xyz_out, int32_t* __restrict__ err) {
+                                                           double* __restrict__ xyz_out, int32_t* __restrict__ err,
+                                                           double* __restrict__ work) {
   extern __shared__ double smem[];
   const int c = blockIdx.x;
   const DevSys& S = systems[conf_sys[c]];
   const double* vl = vlvr + conf_off[c];
   EnergyLds L = carve_energy_lds(smem, S.np, S.npnt, blockDim.x / kWave);
   EnergyOut o;
+  Work wk;
   energy_stage_static(S, L);
-  conformation_energy<MIXED>(S, vl, L, nullptr, o);
+  conformation_energy<MIXED>(S, vl, L, nullptr, o, wk);
   if (threadIdx.x == 0) {
     double eysm = o.el + o.vw + o.hb + o.tor;  // energy.f / enyshe.f:187
     double* r = out + (size_t)c * 8;
@@ -62,6 +64,21 @@ __global__ void __launch_bounds__(kEnergyThreads) k_energy(const DevSys* __restr
       xyz_out[3 * a + 1] = L.y[a];
       xyz_out[3 * a + 2] = L.z[a];
     }
+  }
+  if (work) {  // block totals of the work counters (one row per conformation)
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave, nw = blockDim.x / kWave;
+    double v[3] = {wk.pairs, wk.contacts, wk.builds};
+    for (int k = 0; k < 3; ++k) v[k] = wave_sum(v[k]);
+    __syncthreads();
+    if (lane == 0)
+      for (int k = 0; k < 3; ++k) L.red[8 * wave + k] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int k = 0; k < 3; ++k) {
+        double t = 0.0;
+        for (int w = 0; w < nw; ++w) t += L.red[8 * w + k];
+        work[4 * (size_t)c + k] = t;
+      }
   }
 }
 
@@ -91,7 +108,7 @@ struct DevMem {
 // Replica state of the samplers (grown on demand, reused across calls).
 struct SamplerBuffers {
   DevMem idvr, axvr, vlvr, xyz, nfree, valid, eol, acc, mt, rng_of, slot_of, beta, err, dbg;
-  DevMem wb, walpha, hist, traj, ipoi, iswitch;
+  DevMem wb, walpha, hist, traj, ipoi, iswitch, work, trace, trace_n;
 };
 
 // One system uploaded to the device: a single allocation holding every
@@ -131,6 +148,7 @@ static void upload_system(DevSystem& ds, hipStream_t st) {
   size_t o_vx = add(h.var_axis0.data(), h.var_axis0.size() * 2);
   size_t o_m1 = add(h.var_m1.data(), h.var_m1.size() * 2);
   size_t o_m2 = add(h.var_m2.data(), h.var_m2.size() * 2);
+  size_t o_vsign = add(h.var_sign.data(), h.var_sign.size() * 8);
   size_t o_e0 = add(h.var_e0.data(), h.var_e0.size() * 8);
   size_t o_vs = add(h.var_sg.data(), h.var_sg.size() * 8);
   size_t o_vr = add(h.var_rn.data(), h.var_rn.size() * 8);
@@ -171,6 +189,7 @@ static void upload_system(DevSystem& ds, hipStream_t st) {
   d.var_axis0 = (const int16_t*)(base + o_vx);
   d.var_m1 = (const int16_t*)(base + o_m1);
   d.var_m2 = (const int16_t*)(base + o_m2);
+  d.var_sign = (const double*)(base + o_vsign);
   d.var_e0 = (const double*)(base + o_e0);
   d.var_sg = (const double*)(base + o_vs);
   d.var_rn = (const double*)(base + o_vr);
@@ -183,18 +202,86 @@ static void upload_system(DevSystem& ds, hipStream_t st) {
   d.spoint = (const double*)(base + o_sp);
 }
 
+// Kernel timing with HIP events recorded on the engine's stream around each
+// launch, plus the algorithmic work the kernels report (see Work).
+enum KernelId { kKSweep = 0, kKExchange = 1, kKEnergy = 2, kNKernels = 3 };
+struct KernelStat {
+  double ms = 0.0;
+  int64_t launches = 0;
+  double work[4] = {0, 0, 0, 0};
+};
+struct Profiler {
+  bool on = false;
+  std::vector<hipEvent_t> pool;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
+  KernelStat stat[kNKernels];
+};
+
 struct Engine {
   hipStream_t stream = nullptr;
   int device = -1;
   bool valid = false;           // system upload matches the host state
   DevSystem sys;
-  DevMem d_sys, d_confsys, d_confoff, d_vlvr, d_out, d_xyz, d_err;
+  DevMem d_sys, d_confsys, d_confoff, d_vlvr, d_out, d_xyz, d_err, d_work;
   SamplerBuffers smp;           // replica state of the samplers
   std::vector<std::unique_ptr<DevSystem>> batch_sys;  // energy_batch systems
+  Profiler prof;
   ~Engine() {
+    for (hipEvent_t ev : prof.pool) (void)hipEventDestroy(ev);
+    for (auto& p : prof.pending) {
+      (void)hipEventDestroy(p.second.first);
+      (void)hipEventDestroy(p.second.second);
+    }
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
+
+static hipEvent_t prof_event(Engine& E) {
+  if (!E.prof.pool.empty()) {
+    hipEvent_t ev = E.prof.pool.back();
+    E.prof.pool.pop_back();
+    return ev;
+  }
+  hipEvent_t ev;
+  HIPCHK(hipEventCreate(&ev));
+  return ev;
+}
+// Brackets one launch: call with start=true before and start=false after.
+static void prof_mark(Engine& E, int kid, bool start) {
+  if (!E.prof.on) return;
+  hipEvent_t ev = prof_event(E);
+  HIPCHK(hipEventRecord(ev, E.stream));
+  if (start) {
+    E.prof.pending.push_back({kid, {ev, nullptr}});
+  } else {
+    E.prof.pending.back().second.second = ev;
+  }
+}
+// Folds finished launches into the statistics (after a stream sync).
+static void prof_collect(Engine& E) {
+  if (E.prof.pending.empty()) return;
+  HIPCHK(hipStreamSynchronize(E.stream));
+  for (auto& p : E.prof.pending) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, p.second.first, p.second.second));
+    E.prof.stat[p.first].ms += ms;
+    E.prof.stat[p.first].launches += 1;
+    E.prof.pool.push_back(p.second.first);
+    E.prof.pool.push_back(p.second.second);
+  }
+  E.prof.pending.clear();
+}
+static void prof_add_work(Engine& E, int kid, const double* dev_work, int rows) {
+  if (!E.prof.on || rows <= 0) return;
+  std::vector<double> h(4 * (size_t)rows);
+  HIPCHK(hipMemcpyAsync(h.data(), dev_work, sizeof(double) * h.size(), hipMemcpyDeviceToHost, E.stream));
+  HIPCHK(hipStreamSynchronize(E.stream));
+  for (int r = 0; r < rows; ++r)
+    for (int k = 0; k < 4; ++k) E.prof.stat[kid].work[k] += h[4 * (size_t)r + k];
+}
+
+void engine_profile(Engine*& e, int enable);
+int engine_profile_read(Engine*& e, int kernel, double* ms, int64_t* launches, double* work);
 
 static Engine* make_engine() {
   int n = 0;
@@ -245,16 +332,43 @@ static void launch_energy(Engine& E, const DevSys* d_systems, const int32_t* d_c
   check_lds(lds);
   int32_t* d_err = static_cast<int32_t*>(E.d_err.ensure(sizeof(int32_t)));
   HIPCHK(hipMemsetAsync(d_err, 0, sizeof(int32_t), E.stream));
+  double* d_work = E.prof.on ? static_cast<double*>(E.d_work.ensure(sizeof(double) * 4 * nconf)) : nullptr;
+  if (d_work) HIPCHK(hipMemsetAsync(d_work, 0, sizeof(double) * 4 * nconf, E.stream));
+  prof_mark(E, kKEnergy, true);
   if (precision == 0) {
     HIPCHK(hipFuncSetAttribute((const void*)k_energy<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(k_energy<false>, dim3(nconf), dim3(kEnergyThreads), lds, E.stream, d_systems, d_cs, d_off, d_vl,
-                       d_out, d_xyz, d_err);
+                       d_out, d_xyz, d_err, d_work);
   } else {
     HIPCHK(hipFuncSetAttribute((const void*)k_energy<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(k_energy<true>, dim3(nconf), dim3(kEnergyThreads), lds, E.stream, d_systems, d_cs, d_off, d_vl,
-                       d_out, d_xyz, d_err);
+                       d_out, d_xyz, d_err, d_work);
   }
   HIPCHK(hipGetLastError());
+  prof_mark(E, kKEnergy, false);
+  if (d_work) {
+    prof_collect(E);
+    prof_add_work(E, kKEnergy, d_work, nconf);
+  }
+}
+
+void engine_profile(Engine*& e, int enable) {
+  Engine& E = engine_of(e);
+  prof_collect(E);
+  E.prof.on = enable != 0;
+  for (auto& st : E.prof.stat) st = KernelStat();
+}
+
+int engine_profile_read(Engine*& e, int kernel, double* ms, int64_t* launches, double* work) {
+  Engine& E = engine_of(e);
+  if (kernel < 0 || kernel >= kNKernels) return SMMP_EINVAL;
+  prof_collect(E);
+  const KernelStat& st = E.prof.stat[kernel];
+  if (ms) *ms = st.ms;
+  if (launches) *launches = st.launches;
+  if (work)
+    for (int k = 0; k < 4; ++k) work[k] = st.work[k];
+  return SMMP_OK;
 }
 
 static int read_err(Engine& E) {

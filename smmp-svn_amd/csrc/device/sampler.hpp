@@ -84,6 +84,11 @@ struct SamplerParams {
   int32_t rep0;          // replica of block 0
   int32_t* err;          // bit 0 drift, bit 1 neighbour overflow
   double* dbg;           // optional per-replica diagnostics
+  double* work;          // [R][4] algorithmic work counters (accumulated)
+  int32_t verify;        // recompute every trial in full and record the first mismatch in dbg
+  double* trace;         // optional: per trial of replica trace_rep, (var, E_trial, E_old, accepted)
+  int32_t trace_rep, trace_cap;
+  int32_t* trace_n;
 };
 
 // Extra LDS of the sampler beyond the energy carve-up.
@@ -161,7 +166,7 @@ __device__ __forceinline__ void pair_energy(const DevSys& S, const EnergyLds& L,
 // Energy change of the pairs between the moved range [m1,m2] (new
 // coordinates in L, old in the backup) and the fixed atoms.  The smaller
 // set is walked uniformly, the larger one across the lanes.
-__device__ void pair_delta(const DevSys& S, const SamplerLds& Q, int m1, int m2, PairE& d) {
+__device__ void pair_delta(const DevSys& S, const SamplerLds& Q, int m1, int m2, PairE& d, Work& wk) {
   const EnergyLds& L = Q.L;
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave, nwaves = blockDim.x / kWave;
   const int nat = S.nat, nM = m2 - m1 + 1, nO = nat - nM, words = S.words;
@@ -183,6 +188,7 @@ __device__ void pair_delta(const DevSys& S, const SamplerLds& Q, int m1, int m2,
     }
     const uint64_t w0 = S.bits[((size_t)a * words + (b >> 6)) * 2 + 0];
     if (!((w0 >> (b & 63)) & 1ull)) continue;
+    wk.pairs += 2.0;
     const bool is14 = (S.bits[((size_t)a * words + (b >> 6)) * 2 + 1] >> (b & 63)) & 1ull;
     const int tt = L.t[a] * kTy + L.t[b];
     const double qq = S.conv * L.q[a] * L.q[b];
@@ -193,8 +199,10 @@ __device__ void pair_delta(const DevSys& S, const SamplerLds& Q, int m1, int m2,
   }
 }
 
-// Marks heavy atoms in contact across the moved/fixed boundary before or
-// after the move (their solvent-accessible area may change).
+// Marks the heavy atoms whose accessible area may change: every moved atom
+// (its 122 points stay fixed in the laboratory frame while it rotates, so
+// even a rigidly moved environment changes its discretised area) and every
+// fixed atom in contact with a moved one before or after the move.
 __device__ void mark_affected(const DevSys& S, const SamplerLds& Q, int m1, int m2) {
   const int nh = S.nheavy;
   int h1 = 0, h2 = nh;  // heavy atoms of [m1,m2]: one index range (heavy is sorted)
@@ -213,6 +221,7 @@ __device__ void mark_affected(const DevSys& S, const SamplerLds& Q, int m1, int 
     h2 = lo;
   }
   const int nM = h2 - h1, nO = nh - nM;
+  for (int h = h1 + threadIdx.x; h < h2; h += blockDim.x) Q.aff[h] = 1;
   for (int it = threadIdx.x; it < nM * nO; it += blockDim.x) {
     const int r = it / nO, c = it - r * nO;
     const int ha = h1 + r, hb = c < h1 ? c : c + nM;
@@ -241,6 +250,7 @@ __global__ void __launch_bounds__(512) k_sweep(SamplerParams P) {
   const int tid = threadIdx.x, nwaves = blockDim.x / kWave, wave = tid / kWave, lane = tid & (kWave - 1);
   const int nvt = P.nvt, np = S.np, nat = S.nat, nh = S.nheavy;
   SamplerLds Q = carve_sampler_lds(smem, S, nvt, nwaves);
+  Work wk;
   EnergyLds& L = Q.L;
   double* gvl = P.vlvr + (size_t)r * nvt;
   double* gx = P.xyz + (size_t)r * 3 * np;
@@ -264,7 +274,7 @@ __global__ void __launch_bounds__(512) k_sweep(SamplerParams P) {
   //     [6] jv, [7] drift flag, [8..10] axis point, [11..13] axis unit, [14] beta, [15] old value
   if (!valid) {
     EnergyOut o;
-    conformation_energy<MIXED>(S, Q.vl, L, Q.nf, o);
+    conformation_energy<MIXED>(S, Q.vl, L, Q.nf, o, wk);
     if (tid == 0) {
       Q.sc[0] = o.el + o.vw + o.hb + o.tor + o.sol;
       if (o.err) atomicOr(P.err, 2);
@@ -309,12 +319,18 @@ __global__ void __launch_bounds__(512) k_sweep(SamplerParams P) {
       const int jv = (int)Q.sc[6];
       const int m1 = S.var_m1[jv], m2 = S.var_m2[jv];
       const bool full = Q.sc[5] != 0.0;
+      EnergyOut vref;  // verify mode: full energy before the move (thread 0)
+      if (P.verify && !full) {
+        energy_from_coords<MIXED>(S, Q.vl, L, nullptr, vref, wk);
+        __syncthreads();
+      }
       // ---- rigid rotation of the moved atoms about the bond (Rodrigues)
       {
         double sn, cs;
-        sincos(Q.sc[3], &sn, &cs);
+        sincos(S.var_sign[jv] * Q.sc[3], &sn, &cs);
         const double px = Q.sc[8], py = Q.sc[9], pz = Q.sc[10];
         const double kx = Q.sc[11], ky = Q.sc[12], kz = Q.sc[13];
+        if (tid == 0) wk.moved += m2 - m1 + 1;
         for (int a = m1 + tid; a <= m2; a += blockDim.x) {
           const double vx = L.x[a] - px, vy = L.y[a] - py, vz = L.z[a] - pz;
           Q.bx[a - m1] = L.x[a];
@@ -335,7 +351,7 @@ __global__ void __launch_bounds__(512) k_sweep(SamplerParams P) {
       if (!full) {
         // ---- incremental energy change
         PairE d = {0.0, 0.0, 0.0};
-        pair_delta(S, Q, m1, m2, d);
+        pair_delta(S, Q, m1, m2, d, wk);
         if (S.itysol > 0) mark_affected(S, Q, m1, m2);
         double dt = 0.0;
         if (tid == 0) {
@@ -357,7 +373,7 @@ __global__ void __launch_bounds__(512) k_sweep(SamplerParams P) {
           for (int h = wave; h < nh; h += nwaves) {
             if (!Q.aff[h]) continue;
             const int i = S.heavy[h];
-            int nfree = solvent_atom(S, L, nb, i);
+            int nfree = solvent_atom(S, L, nb, i, wk);
             if (nfree < 0) {
               err = 1;
               nfree = Q.nf[h];
@@ -379,23 +395,53 @@ __global__ void __launch_bounds__(512) k_sweep(SamplerParams P) {
         }
         __syncthreads();
         if (tid == 0) {
-          double s = dt;
+          double s = dt, comp[4] = {0, 0, 0, 0};
           int e = 0;
           for (int w = 0; w < nwaves; ++w) {
-            for (int k = 0; k < 4; ++k) s += L.red[8 * w + k];
+            for (int k = 0; k < 4; ++k) {
+              s += L.red[8 * w + k];
+              comp[k] += L.red[8 * w + k];
+            }
             e |= L.red[8 * w + 6] != 0.0;
           }
           if (e) atomicOr(P.err, 2);
           Q.sc[16] = Q.sc[0] + s;
+          for (int k = 0; k < 4; ++k) Q.sc[24 + k] = comp[k];  // el, vw, hb, sol deltas (verify mode)
+          Q.sc[28] = dt;
         }
       } else {
         // ---- full recomputation from the trial coordinates
         EnergyOut o;
-        energy_from_coords<MIXED>(S, Q.vl, L, Q.nf2, o);
+        energy_from_coords<MIXED>(S, Q.vl, L, Q.nf2, o, wk);
         if (tid == 0) {
           Q.sc[16] = o.el + o.vw + o.hb + o.tor + o.sol;
           if (o.err) atomicOr(P.err, 2);
         }
+      }
+      if (P.verify && !full) {  // compare each incremental component with a full recomputation
+        EnergyOut vnew;
+        energy_from_coords<MIXED>(S, Q.vl, L, nullptr, vnew, wk);
+        if (tid == 0 && P.dbg[16 * r] == 0.0) {
+          const double fd[5] = {vnew.el - vref.el, vnew.vw - vref.vw, vnew.hb - vref.hb, vnew.tor - vref.tor,
+                                vnew.sol - vref.sol};
+          const double id[5] = {Q.sc[24], Q.sc[25], Q.sc[26], Q.sc[28], Q.sc[27]};
+          const double scale = 1.0 + fabs(vref.el) + fabs(vref.vw) + fabs(vref.hb) + fabs(vref.sol);
+          bool bad = false;
+          for (int k = 0; k < 5; ++k) bad |= fabs(fd[k] - id[k]) > 1e-9 * (scale + fabs(fd[k]));
+          if (bad) {
+            double* d = P.dbg + 16 * r;
+            d[0] = 1.0;
+            d[1] = sw;
+            d[2] = nsw;
+            d[3] = jv;
+            for (int k = 0; k < 5; ++k) {
+              d[4 + k] = id[k];
+              d[9 + k] = fd[k];
+            }
+            atomicOr(P.err, 4);
+          }
+        }
+        __syncthreads();
       }
       // ---- acceptance (leader): metropolis.f:72-87
       if (tid == 0) {
@@ -411,6 +457,14 @@ __global__ void __launch_bounds__(512) k_sweep(SamplerParams P) {
           acc = exp(-delta) >= rd;
         }
         Q.sc[4] = acc ? 1.0 : 0.0;
+        if (P.trace && r == P.trace_rep && *P.trace_n < P.trace_cap) {
+          double* t = P.trace + 4 * (size_t)(*P.trace_n);
+          t[0] = jv;
+          t[1] = enew;
+          t[2] = eol;
+          t[3] = acc ? 1.0 : 0.0;
+          *P.trace_n += 1;
+        }
         if (acc) {
           Q.sc[0] = enew;
           Q.sc[1] += 1.0;
@@ -435,15 +489,15 @@ __global__ void __launch_bounds__(512) k_sweep(SamplerParams P) {
     }
     // ---- sweep end: rebuild from the variables and check (metropolis.f:172-179)
     EnergyOut o;
-    conformation_energy<MIXED>(S, Q.vl, L, Q.nf, o);
+    conformation_energy<MIXED>(S, Q.vl, L, Q.nf, o, wk);
     if (tid == 0) {
       const double enw = o.el + o.vw + o.hb + o.tor + o.sol;
       const double eol = Q.sc[0];
       if (fabs(eol - enw) > 0.000001 && eol < 100000.0) {
         atomicOr(P.err, 1);
-        if (P.dbg) {
-          P.dbg[2 * r] = eol;
-          P.dbg[2 * r + 1] = enw;
+        if (P.dbg[16 * r + 14] == 0.0) {  // first drift of this replica
+          P.dbg[16 * r + 14] = eol;
+          P.dbg[16 * r + 15] = enw;
         }
       }
       if (o.err) atomicOr(P.err, 2);
@@ -471,6 +525,20 @@ __global__ void __launch_bounds__(512) k_sweep(SamplerParams P) {
     P.eol[r] = Q.sc[0];
     P.acc[r] = Q.sc[1];
     P.valid[r] = 1;
+  }
+  if (P.work) {
+    double v[4] = {wk.pairs, wk.contacts, wk.builds, wk.moved};
+    for (int k = 0; k < 4; ++k) v[k] = wave_sum(v[k]);
+    __syncthreads();
+    if (lane == 0)
+      for (int k = 0; k < 4; ++k) L.red[8 * wave + k] = v[k];
+    __syncthreads();
+    if (tid == 0)
+      for (int k = 0; k < 4; ++k) {
+        double t = 0.0;
+        for (int w = 0; w < nwaves; ++w) t += L.red[8 * w + k];
+        P.work[4 * r + k] += t;
+      }
   }
 }
 

@@ -100,6 +100,8 @@ int guarded(smmp_ctx* c, int code, F&& f) {
     return fail(c, code, e.msg);
   } catch (const DeviceError& e) {
     return fail(c, SMMP_EDEVICE, e.msg);
+  } catch (const DriftError& e) {
+    return fail(c, SMMP_EDRIFT, e.msg);
   } catch (const std::exception& e) {
     return fail(c, SMMP_EINVAL, e.what());
   }
@@ -221,12 +223,12 @@ int smmp_metropolis(smmp_ctx* c, double* eol, double* acz, const smmp_weight* w)
   });
 }
 
-int smmp_partem_s(smmp_ctx* c, int num_rep, int nequi, int nswp, const double* temps, double* eol,
+int smmp_partem_s(smmp_ctx* c, int num_rep, int nequi, int nswp, int switch_, const double* temps, double* eol,
                   int32_t* ipoi, double* traj) {
   return guarded(c, SMMP_ESTATE, [&] {
     if (num_rep < 1 || num_rep > 2048 || nequi < 0 || nswp < 0 || !temps) throw Error("partem_s> bad arguments");
     if (c->s.ntlml < 1) throw Error("partem_s> no molecule initialised");
-    return engine_partem_s(c->eng, c->s, num_rep, nequi, nswp, temps, eol, ipoi, traj);
+    return engine_partem_s(c->eng, c->s, num_rep, nequi, nswp, switch_, temps, eol, ipoi, traj);
   });
 }
 
@@ -259,5 +261,16 @@ int smmp_sample(smmp_ctx* c, const smmp_sampler_cfg* cfg, double* vlvr, double* 
 }
 
 const char* smmp_build_info(void) { return engine_build_info(); }
+
+int smmp_profile(smmp_ctx* c, int enable) {
+  return guarded(c, SMMP_EDEVICE, [&] {
+    engine_profile(c->eng, enable);
+    return SMMP_OK;
+  });
+}
+
+int smmp_profile_read(smmp_ctx* c, int kernel, double* total_ms, int64_t* launches, double* work) {
+  return guarded(c, SMMP_EDEVICE, [&] { return engine_profile_read(c->eng, kernel, total_ms, launches, work); });
+}
 
 }  // extern "C"

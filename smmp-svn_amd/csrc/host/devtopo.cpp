@@ -59,7 +59,7 @@ HostSys build_host_sys(const State& s) {
   for (int k = 0; k < 6; ++k) h.gbpr[k] = s.gbpr(k + 1, 1);
   // variables: torsion terms, controlled atoms, moved atoms
   h.var_atom.resize(nvt); h.var_base.resize(nvt); h.var_axis0.resize(nvt);
-  h.var_m1.resize(nvt); h.var_m2.resize(nvt);
+  h.var_m1.resize(nvt); h.var_m2.resize(nvt); h.var_sign.resize(nvt);
   h.var_e0.resize(nvt); h.var_sg.resize(nvt); h.var_rn.resize(nvt); h.var_is_torsion.resize(nvt);
   std::vector<std::vector<int>> children(nat);
   for (int i = 1; i < nat; ++i) children[h.parent[i]].push_back(i);
@@ -98,6 +98,19 @@ HostSys build_host_sys(const State& s) {
     if (cnt == 0 || m2 - m1 + 1 != cnt) throw Error("device engine: moved atoms of a variable are not contiguous");
     h.var_m1[k] = (int16_t)m1;
     h.var_m2[k] = (int16_t)m2;
+    h.var_sign[k] = 1.0;
+  }
+  // The torsions of the second and third backbone atoms also set the frame of
+  // the first atom (setgbl keeps bond 1->2 and the plane 1-2-3 fixed in the
+  // laboratory), so changing them turns the atoms before the moved range the
+  // other way while the moved range stays put.
+  for (int k = 0; k < nvt; ++k) {
+    if (h.tvar[h.i2] != k && h.tvar[h.i3] != k) continue;
+    if (h.var_m2[k] != nat - 1 || h.var_m1[k] < 1)
+      throw Error("device engine: a torsion of the first backbone atoms does not move the chain's end");
+    h.var_m2[k] = (int16_t)(h.var_m1[k] - 1);
+    h.var_m1[k] = 0;
+    h.var_sign[k] = -1.0;
   }
   // interaction set from the reference lists (mklist), as symmetric bits
   h.bits.assign((size_t)nat * h.words * 2, 0ull);

@@ -27,7 +27,8 @@ struct HostSys {
   std::vector<int16_t> var_atom;    // iatvr-1
   std::vector<int16_t> var_base;    // iowat(iatvr)-1: axis end, not moved
   std::vector<int16_t> var_axis0;   // iowat(base)-1: axis start
-  std::vector<int16_t> var_m1, var_m2;  // moved atoms [m1, m2]
+  std::vector<int16_t> var_m1, var_m2;  // atoms a trial move rotates, [m1, m2]
+  std::vector<double> var_sign;         // +1: by the torsion change, -1: by its opposite
   std::vector<double> var_e0, var_sg, var_rn;  // torsion potential (e0to/2, sign, period)
   std::vector<int32_t> var_is_torsion;
   // interaction set: bits[(i*words + w)*2 + plane], plane 0 = pair counted,

@@ -14,6 +14,12 @@ struct DeviceError {
   std::string msg;
   explicit DeviceError(std::string m) : msg(std::move(m)) {}
 };
+// End-of-sweep energy differs from the tracked one by more than 1e-6
+// (metropolis.f:172-179); the results of the call are still written.
+struct DriftError {
+  std::string msg;
+  explicit DriftError(std::string m) : msg(std::move(m)) {}
+};
 
 void engine_destroy(Engine* e);
 void engine_invalidate(Engine* e);  // topology changed: rebuild device data on next use
@@ -25,13 +31,15 @@ double engine_enysol(Engine*& e, State& s);
 // metropolis.f:13 (context MT stream)
 int engine_metropolis(Engine*& e, State& s, double* eol, double* acz, const smmp_weight& w);
 // partem_s.f:13, serial-compatible single stream
-int engine_partem_s(Engine*& e, State& s, int nrep, int nequi, int nswp, const double* temps, double* eol,
-                    int32_t* ipoi, double* traj);
+int engine_partem_s(Engine*& e, State& s, int nrep, int nequi, int nswp, int switch_, const double* temps,
+                    double* eol, int32_t* ipoi, double* traj);
 int engine_energy_batch(Engine*& e, std::vector<State*>& sys, const int32_t* conf_sys, int nconf,
                         const double* vlvr, int precision, double* out);
 int engine_sample(Engine*& e, State& s, const smmp_sampler_cfg& cfg, double* vlvr, double* eol, double* acc,
                   int32_t* ipoi, int32_t* hist, double* traj);
 const char* engine_build_info();
+void engine_profile(Engine*& e, int enable);
+int engine_profile_read(Engine*& e, int kernel, double* ms, int64_t* launches, double* work);
 
 // twister.f:104 grnd() on the context's stream (host side; the draws of the
 // device samplers come from the same state copied to the device)

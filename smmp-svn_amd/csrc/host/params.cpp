@@ -16,11 +16,9 @@
 namespace smmp {
 
 static inline double F(float x) { return (double)x; }
-// x**6 for an integer exponent, evaluated by squaring as Fortran compilers do.
-static inline double pow6(double x) {
-  double x2 = x * x, x4 = x2 * x2;
-  return x4 * x2;
-}
+// x**6 for an integer exponent as the reference build evaluates it:
+// repeated multiplication from the left (matches its tables bit for bit).
+static inline double pow6(double x) { return x * x * x * x * x * x; }
 
 void block_data(State& s) {
   s.conv = 332.0;

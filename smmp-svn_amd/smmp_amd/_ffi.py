@@ -35,7 +35,7 @@ class Weight(C.Structure):
 class SamplerCfg(C.Structure):
     _fields_ = [("nrep", C.c_int32), ("nsweep", C.c_int32), ("mode", C.c_int32), ("precision", C.c_int32),
                 ("temps", C.POINTER(C.c_double)), ("seeds", C.POINTER(C.c_uint32)),
-                ("w", C.POINTER(Weight)), ("resync_every", C.c_int32)]
+                ("w", C.POINTER(Weight)), ("resync_every", C.c_int32), ("verify", C.c_int32)]
 
 
 def _load():
@@ -61,10 +61,12 @@ def _load():
         "smmp_grnd": (C.c_int, [P, dp]),
         "smmp_outvar": (C.c_int, [P, C.c_int, C.c_char_p]),
         "smmp_metropolis": (C.c_int, [P, dp, dp, C.POINTER(Weight)]),
-        "smmp_partem_s": (C.c_int, [P, C.c_int, C.c_int, C.c_int, dp, dp, ip, dp]),
+        "smmp_partem_s": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp, ip, dp]),
         "smmp_energy_batch": (C.c_int, [C.POINTER(P), C.c_int, ip, C.c_int, dp, C.c_int, dp]),
         "smmp_sample": (C.c_int, [P, C.POINTER(SamplerCfg), dp, dp, dp, ip, ip, dp]),
         "smmp_build_info": (C.c_char_p, []),
+        "smmp_profile": (C.c_int, [P, C.c_int]),
+        "smmp_profile_read": (C.c_int, [P, C.c_int, dp, C.POINTER(C.c_int64), dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -79,7 +81,9 @@ lib = _load()
 EXPORTS = ("smmp_ctx_create", "smmp_ctx_destroy", "smmp_last_error", "smmp_state_view", "smmp_state_count",
            "smmp_state_name", "smmp_init_energy", "smmp_init_molecule", "smmp_setvar", "smmp_energy",
            "smmp_enysol", "smmp_sgrnd", "smmp_grnd", "smmp_outvar", "smmp_metropolis", "smmp_partem_s",
-           "smmp_energy_batch", "smmp_sample", "smmp_build_info")
+           "smmp_energy_batch", "smmp_sample", "smmp_build_info", "smmp_profile", "smmp_profile_read")
+
+KERNEL_SWEEP, KERNEL_EXCHANGE, KERNEL_ENERGY = 0, 1, 2
 
 
 def dptr(a: np.ndarray):
@@ -149,3 +153,100 @@ class Context:
         e = C.c_double()
         self.check(lib.smmp_energy(self.h, C.byref(e)))
         return e.value
+
+    # ---- convenience accessors of the chain
+    @property
+    def nvt(self) -> int:
+        v = self.view
+        nt = int(v("mol_i.ntlml")[0])
+        return int(v("mol_i.ivrml1")[nt - 1] + v("mol_i.nvrml")[nt - 1] - 1) if nt else 0
+
+    @property
+    def nvr(self) -> int:
+        return int(self.view("mol_par.nvr")[0])
+
+    @property
+    def idvr(self) -> np.ndarray:
+        return self.view("var_i.idvr")[:self.nvr].copy()
+
+    def vlvr(self) -> np.ndarray:
+        return self.view("var_r.vlvr")[:self.nvt].copy()
+
+    def setvar(self, nml: int, vl) -> None:
+        full = np.array(self.view("var_r.vlvr"), dtype=np.float64)
+        full[:len(vl)] = vl
+        self.check(lib.smmp_setvar(self.h, nml, dptr(full)))
+
+    def profile(self, enable: bool = True) -> None:
+        self.check(lib.smmp_profile(self.h, 1 if enable else 0))
+
+    def profile_read(self, kernel: int) -> dict:
+        ms = C.c_double()
+        n = C.c_int64()
+        work = np.zeros(4)
+        self.check(lib.smmp_profile_read(self.h, kernel, C.byref(ms), C.byref(n), dptr(work)))
+        return {"ms": ms.value, "launches": n.value, "pairs": work[0], "contacts": work[1],
+                "builds": work[2], "moved": work[3]}
+
+    def sgrnd(self, seed: int) -> None:
+        self.check(lib.smmp_sgrnd(self.h, int(np.int32(np.uint32(seed & 0xffffffff)))))
+
+    def grnd(self) -> float:
+        u = C.c_double()
+        self.check(lib.smmp_grnd(self.h, C.byref(u)))
+        return u.value
+
+    def metropolis(self, eol: float, acz: float, weight: Weight) -> tuple[float, float]:
+        e = C.c_double(eol)
+        a = C.c_double(acz)
+        self.check(lib.smmp_metropolis(self.h, C.byref(e), C.byref(a), C.byref(weight)))
+        return e.value, a.value
+
+    def partem_s(self, temps, nequi: int, nswp: int, switch: int = 1):
+        temps = np.ascontiguousarray(temps, dtype=np.float64)
+        R = len(temps)
+        eol = np.zeros(R)
+        ipoi = np.zeros(R, dtype=np.int32)
+        traj = np.zeros(max(nswp, 1) * R)
+        rc = lib.smmp_partem_s(self.h, R, nequi, nswp, switch, dptr(temps), dptr(eol), iptr(ipoi), dptr(traj))
+        self.check(rc)
+        return eol, ipoi, traj[:nswp * R].reshape(nswp, R)
+
+    def sample(self, temps, seeds, nsweep: int, vlvr, mode: int = 0, weight: Weight | None = None,
+               precision: int = 0, with_traj: bool = True, allow_drift: bool = False, verify: bool = False):
+        """Independent-stream replica sampler (smmp_sample)."""
+        temps = np.ascontiguousarray(temps, dtype=np.float64)
+        R = len(temps)
+        seeds = np.ascontiguousarray(seeds, dtype=np.uint32)
+        need = R + (1 if mode == 1 else 0)
+        if len(seeds) < need:
+            raise ValueError(f"need {need} seeds")
+        vl = np.ascontiguousarray(vlvr, dtype=np.float64).reshape(R, self.nvt).copy()
+        eol = np.zeros(R)
+        acc = np.zeros(R)
+        ipoi = np.zeros(R, dtype=np.int32)
+        nb = (weight.kmax - weight.kmin + 1) if weight is not None else 1
+        hist = np.zeros(nb, dtype=np.int32)
+        traj = np.zeros(max(nsweep, 1) * R) if with_traj else None
+        cfg = SamplerCfg(nrep=R, nsweep=nsweep, mode=mode, precision=precision,
+                         temps=dptr(temps), seeds=seeds.ctypes.data_as(C.POINTER(C.c_uint32)),
+                         w=C.pointer(weight) if weight is not None else None, resync_every=1,
+                         verify=1 if verify else 0)
+        rc = lib.smmp_sample(self.h, C.byref(cfg), dptr(vl), dptr(eol), dptr(acc), iptr(ipoi),
+                             iptr(hist) if weight is not None else None,
+                             dptr(traj) if traj is not None else None)
+        if rc != 0 and not (allow_drift and rc == -4):
+            self.check(rc)
+        return {"vlvr": vl, "eol": eol, "acc": acc, "ipoi": ipoi, "hist": hist, "rc": rc,
+                "traj": traj[:nsweep * R].reshape(nsweep, R) if traj is not None else None}
+
+
+def energy_batch(ctxs, conf_sys, vlvr_flat, precision: int = 0) -> np.ndarray:
+    """Energies of many conformations (rows: total, eysm, eyel, eyvw, eyhb, eyvr, eysl, asa)."""
+    arr = (C.c_void_p * len(ctxs))(*[c.h for c in ctxs])
+    cs = np.ascontiguousarray(conf_sys, dtype=np.int32)
+    vl = np.ascontiguousarray(vlvr_flat, dtype=np.float64)
+    out = np.zeros((len(cs), 8))
+    rc = lib.smmp_energy_batch(arr, len(ctxs), iptr(cs), len(cs), dptr(vl), precision, dptr(out))
+    ctxs[0].check(rc)
+    return out

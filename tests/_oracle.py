@@ -67,7 +67,9 @@ def _load():
     lib.oracle_weight_eval.restype = C.c_double
     lib.oracle_metropolis.argtypes = [P(Topo), P(Conf), P(MT), F64P, C.c_int, I32P, F64P, P(Weight), F64P, F64P]
     lib.oracle_partem.argtypes = [P(Topo), P(MT), F64P, C.c_int, I32P, F64P, C.c_int, F64P, C.c_int, C.c_int,
-                                  F64P, I32P, F64P, F64P]
+                                  C.c_int, F64P, I32P, F64P, F64P]
+    lib.oracle_set_trace.argtypes = [F64P, C.c_int, C.c_int]
+    lib.oracle_trace_count.restype = C.c_int
     lib.oracle_muca_update.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, I32P, F64P, F64P, F64P, F64P]
     lib.oracle_difang.argtypes = [C.c_double, C.c_double]
     lib.oracle_difang.restype = C.c_double
@@ -195,15 +197,15 @@ class OracleSystem:
             out.append((eol.value, acz.value, bad))
         return vl, out
 
-    def partem(self, mt: OracleMT, temps, nequi: int, nswp: int):
+    def partem(self, mt: OracleMT, temps, nequi: int, nswp: int, switch: int = 1):
         R = len(temps)
         temps = f64(temps)
         eol = np.zeros(R)
         ipoi = np.zeros(R, dtype=np.int32)
         coor = np.zeros(R * self.nvr)
-        traj = np.zeros(nswp * R)
+        traj = np.zeros(max(nswp, 1) * R)
         rc = lib.oracle_partem(C.byref(self.t), C.byref(mt.s), self.vlvr0.ctypes.data_as(F64P), self.nvr,
                                self.idvr.ctypes.data_as(I32P), self.axvr.ctypes.data_as(F64P), R,
-                               temps.ctypes.data_as(F64P), nequi, nswp, eol.ctypes.data_as(F64P),
+                               temps.ctypes.data_as(F64P), nequi, nswp, switch, eol.ctypes.data_as(F64P),
                                ipoi.ctypes.data_as(I32P), coor.ctypes.data_as(F64P), traj.ctypes.data_as(F64P))
-        return rc, eol, ipoi, coor.reshape(R, self.nvr), traj.reshape(nswp, R)
+        return rc, eol, ipoi, coor.reshape(R, self.nvr), traj[:nswp * R].reshape(nswp, R)

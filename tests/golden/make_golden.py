@@ -171,6 +171,21 @@ def main() -> None:
     save("partem_len10.npz", temps=temps, time_d=td, eol_final=eol_final,
          seed=np.array([777]), nequi=np.array([2]), nswp=np.array([12]))
 
+    # partem_s from the extended chain (switch < 0: every free torsion at 180
+    # degrees), so the run stays in the well-conditioned energy range.
+    for tag, nrep, nequi, nswp in (("len10", 4, 3, 25), ("c2_20mer", 4, 2, 10)):
+        seqf = workloads.write_seq_file(os.path.join(tmp, f"pt_{tag}.seq"), cases[tag], tag)
+        temps = workloads.geometric_temperatures(nrep, 250.0, 1000.0)
+        workloads.write_temperatures(os.path.join(tmp, "temperatures"), temps)
+        run(["partem", LIBDIR, seqf, "1", "4242", str(nrep), str(nequi), str(nswp), "-1"], tmp)
+        td = np.loadtxt(os.path.join(tmp, "time.d"))
+        raw = open(os.path.join(tmp, "par_R.in"), "rb").read()
+        rec = 4 + 4 + 4 + 4 + 8 * nrep + 4
+        assert len(raw) == nrep * rec, len(raw)
+        eol_final = np.frombuffer(raw[16:16 + 8 * nrep], dtype="<f8").copy()
+        save(f"partem_ext_{tag}.npz", temps=temps, time_d=td, eol_final=eol_final, seed=np.array([4242]),
+             nequi=np.array([nequi]), nswp=np.array([nswp]), switch=np.array([-1]))
+
 
 if __name__ == "__main__":
     main()

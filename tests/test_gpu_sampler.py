@@ -1,0 +1,214 @@
+"""Metropolis, parallel tempering and multicanonical sweeps on the GPU.
+
+Trajectories are compared move by move: the GPU draws from the same
+twister.f stream as the oracle / the Fortran reference, so acceptance
+counts must be identical and energies agree to 1e-8 kcal/mol (the GPU tracks
+the energy incrementally between the per-sweep full recomputations).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, golden, make_ctx
+
+pytestmark = pytest.mark.gpu
+GASC = 1.98773e-3
+
+
+def say(*a):
+    print("[gpu-sampler]", *a, flush=True)
+
+
+def test_metropolis_c1_trajectory_vs_reference():
+    """C1: Met-enkephalin, omegas fixed (19 free), ECEPP/3 vacuum, 300 K."""
+    from smmp_amd import _ffi
+    r = golden("enkefa_runs.npz")
+    c = make_ctx(tag="enkefa", itysol=0, varfile=os.path.join(GOLDEN, "enkefa_c1.var"))
+    assert c.nvr == 19
+    c.sgrnd(81236)
+    beta = 1.0 / (300.0 * GASC)
+    c.view("bet.beta")[0] = beta
+    w = _ffi.Weight(kind=0, beta=beta)
+    eol, acz = 0.0, 0.0
+    say("smmp_metropolis C1 40 sweeps")
+    for k in range(40):
+        eol, acz = c.metropolis(eol, acz, w)
+        assert acz == r["canon_sweeps"][k, 1], k
+        assert abs(eol - r["canon_sweeps"][k, 0]) <= 1e-8 * max(1.0, abs(eol)), k
+    assert np.abs(c.vlvr()[c.idvr - 1] - r["canon_vlvr"]).max() < 1e-12
+    assert int(c.view("myblock.mti")[0]) == int(r["canon_mti"][0])
+
+
+def test_metropolis_with_solvent_vs_reference():
+    from smmp_amd import _ffi
+    r = golden("enkefa_runs.npz")
+    c = make_ctx(tag="enkefa", itysol=1, varfile=os.path.join(GOLDEN, "enkefa.var"))
+    c.sgrnd(4357)
+    beta = 1.0 / (500.0 * GASC)
+    w = _ffi.Weight(kind=0, beta=beta)
+    eol, acz = 0.0, 0.0
+    say("smmp_metropolis enkefa OONS 10 sweeps")
+    for k in range(10):
+        eol, acz = c.metropolis(eol, acz, w)
+        assert acz == r["canon_sol_sweeps"][k, 1], k
+        assert abs(eol - r["canon_sol_sweeps"][k, 0]) <= 1e-8 * max(1.0, abs(eol)), k
+
+
+@pytest.mark.parametrize("tag", ["len10", "c2_20mer"])
+def test_partem_s_serial_compatible_vs_reference(tag):
+    """partem_s.f on one stream from the extended chain (switch < 0): the
+    energies of time.d (3 decimals) and of par_R.in.  (From random starts the
+    first sweeps sit at clash energies ~1e13 kcal/mol where a move's energy
+    change is below the rounding of the total, so the reference's own
+    decisions there depend on its summation order.)"""
+    p = golden(f"partem_ext_{tag}.npz")
+    nswp, R = int(p["nswp"][0]), len(p["temps"])
+    c = make_ctx(tag=tag, itysol=1)
+    c.sgrnd(int(p["seed"][0]))
+    say(f"smmp_partem_s {tag} {R} replicas {nswp} sweeps")
+    eol, ipoi, traj = c.partem_s(p["temps"], int(p["nequi"][0]), nswp, int(p["switch"][0]))
+    ref = p["time_d"][:, 3].reshape(nswp, R)
+    assert np.abs(traj - ref).max() <= 6e-4
+    assert np.abs(eol - p["eol_final"]).max() <= 1e-8 * max(1.0, np.abs(p["eol_final"]).max())
+
+
+def test_batched_canonical_replicas_match_oracle_streams():
+    """Each replica r of the batched sampler = oracle metropolis on stream seed_r."""
+    from _oracle import OracleMT, OracleSystem
+    c = make_ctx(tag="len10", itysol=1)
+    osys = OracleSystem(c)
+    R, nsw = 6, 4
+    temps = np.array([250.0, 300.0, 400.0, 550.0, 750.0, 1000.0])
+    seeds = np.array([11, 22, 33, 44, 55, 66], dtype=np.uint32)
+    rng = np.random.default_rng(1)
+    v0 = np.tile(c.vlvr(), (R, 1))
+    v0[:, osys.idvr - 1] = rng.uniform(-np.pi, np.pi, size=(R, osys.nvr))
+    say("smmp_sample canonical len10 R=6 4 sweeps")
+    out = c.sample(temps, seeds, nsw, v0, mode=0)
+    for r in range(R):
+        mt = OracleMT(int(seeds[r]))
+        osys.reset()
+        vl, tr = osys.metropolis(mt, v0[r], 1.0 / (temps[r] * GASC), nsw)
+        assert out["acc"][r] == tr[-1][1], r
+        for k in range(nsw):
+            assert abs(out["traj"][k, r] - tr[k][0]) <= 1e-8 * max(1.0, abs(tr[k][0])), (r, k)
+        assert np.abs(out["vlvr"][r] - vl).max() < 1e-12
+
+
+def test_batched_parallel_tempering_matches_python_restatement():
+    """PT with one stream per replica plus an exchange stream (partem_s.f:209-232 rule)."""
+    from _oracle import OracleMT, OracleSystem
+    c = make_ctx(tag="len10", itysol=1)
+    osys = OracleSystem(c)
+    R, nsw = 4, 5
+    temps = np.array([250.0, 400.0, 630.0, 1000.0])
+    seeds = np.array([7, 8, 9, 10, 99], dtype=np.uint32)
+    v0 = np.tile(c.vlvr(), (R, 1))
+    v0[:, osys.idvr - 1] = np.pi  # extended chain: well-conditioned energies from the start
+    say("smmp_sample PT len10 R=4 5 sweeps")
+    out = c.sample(temps, seeds, nsw, v0, mode=1)
+    # restatement: replica k sweeps at the temperature of its slot, then exchanges
+    mts = [OracleMT(int(s)) for s in seeds[:R]]
+    mtx = OracleMT(int(seeds[R]))
+    vl = [v0[k].copy() for k in range(R)]
+    eol = np.zeros(R)
+    ipoi = list(range(1, R + 1))
+    pbe = 1.0 / (temps * GASC)
+    iswitch = 1
+    for sw in range(nsw):
+        slot_of = {ipoi[s] - 1: s for s in range(R)}
+        for k in range(R):
+            osys.reset()
+            vl[k], tr = osys.metropolis(mts[k], vl[k], pbe[slot_of[k]], 1)
+            eol[k] = tr[-1][0]
+        if iswitch == 1:
+            nu, last, iswitch = 1, R - 1, 2
+        else:
+            nu, last, iswitch = 2, R, 1
+        for i in range(nu, last + 1, 2):
+            j = 1 if i == R else i + 1
+            a, b = ipoi[i - 1], ipoi[j - 1]
+            d = -pbe[i - 1] * eol[b - 1] - pbe[j - 1] * eol[a - 1] + pbe[i - 1] * eol[a - 1] + pbe[j - 1] * eol[b - 1]
+            if mtx.grnd() <= np.exp(d):
+                ipoi[i - 1], ipoi[j - 1] = b, a
+    assert list(out["ipoi"]) == ipoi
+    assert np.abs(out["eol"] - eol).max() <= 1e-8 * max(1.0, np.abs(eol).max())
+
+
+def test_multicanonical_histogram_and_consistency():
+    from smmp_amd import _ffi
+    from _oracle import OracleSystem
+    c = make_ctx(tag="enkefa", itysol=0, varfile=os.path.join(GOLDEN, "enkefa.var"))
+    osys = OracleSystem(c)
+    kmin, kmax, ebin = -20, 179, 1.0
+    nb = kmax - kmin + 1
+    b = np.full(nb, 1.0 / (600.0 * GASC))
+    alpha = np.zeros(nb)
+    w = _ffi.Weight(kind=2, beta=0.0, kmin=kmin, kmax=kmax, ebin=ebin,
+                    b=_ffi.dptr(b), alpha=_ffi.dptr(alpha))
+    R, nsw = 4, 6
+    v0 = np.tile(c.vlvr(), (R, 1))
+    say("smmp_sample multicanonical enkefa R=4")
+    out = c.sample(np.full(R, 600.0), np.arange(1, R + 1, dtype=np.uint32), nsw, v0, mode=2, weight=w)
+    assert out["hist"].sum() == R * nsw
+    for r in range(R):
+        ref = osys.energy(out["vlvr"][r])
+        assert abs(out["eol"][r] - ref[0]) <= 1e-9 * max(1.0, abs(ref[0]))
+
+
+def test_verify_mode_incremental_equals_full_recomputation():
+    """Every trial's incremental energy change (moved/fixed pairs, torsion,
+    solvent of the affected atoms) equals a full recomputation, C2 system."""
+    c = make_ctx(tag="c2_20mer", itysol=1)
+    R = 4
+    v0 = np.tile(c.vlvr(), (R, 1))
+    rng = np.random.default_rng(4)
+    idv = c.idvr - 1
+    v0[:2, idv] = rng.uniform(-np.pi, np.pi, size=(2, len(idv)))  # clash starts: full-path trials
+    v0[2:, idv] = np.pi + rng.normal(0.0, 0.3, size=(2, len(idv)))  # near-extended starts
+    say("smmp_sample verify c2_20mer R=4 2 sweeps")
+    out = c.sample(np.array([250.0, 400.0, 630.0, 1000.0]), np.arange(31, 35, dtype=np.uint32), 2, v0,
+                   mode=0, verify=True)
+    assert out["rc"] == 0
+
+
+def test_c2_system_sweeps_consistent_with_oracle():
+    from _oracle import OracleSystem
+    c = make_ctx(tag="c2_20mer", itysol=1)
+    o = OracleSystem(c)
+    R = 16
+    v0 = np.tile(c.vlvr(), (R, 1))
+    v0[:, o.idvr - 1] = np.random.default_rng(8).uniform(-np.pi, np.pi, size=(R, o.nvr))
+    say("smmp_sample PT c2_20mer R=16 15 sweeps")
+    out = c.sample(np.geomspace(250.0, 1000.0, R), np.arange(1, R + 2, dtype=np.uint32), 15, v0, mode=1)
+    for r in range(R):
+        ref = o.energy(out["vlvr"][r])
+        assert abs(out["eol"][r] - ref[0]) <= 1e-9 * max(1.0, abs(ref[0]))
+    assert sorted(out["ipoi"]) == list(range(1, R + 1))
+
+
+def test_energy_histogram_matches_oracle_sampler_ks():
+    """Sampled energies of the batched GPU sampler and of the CPU oracle's
+    Metropolis chains (different streams) come from one distribution."""
+    from scipy.stats import ks_2samp
+    from _oracle import OracleMT, OracleSystem
+    c = make_ctx(tag="enkefa", itysol=1)
+    o = OracleSystem(c)
+    T, nsw, burn, every = 600.0, 240, 80, 20
+    R = 64
+    v0 = np.tile(c.vlvr(), (R, 1))
+    v0[:, o.idvr - 1] = np.pi
+    say("smmp_sample canonical enkefa R=64 240 sweeps")
+    out = c.sample(np.full(R, T), np.arange(100, 100 + R, dtype=np.uint32), nsw, v0, mode=0)
+    gpu = out["traj"][burn - 1::every, :].ravel()
+    cpu = []
+    for k in range(8):
+        o.reset()
+        _, tr = o.metropolis(OracleMT(9000 + k), v0[0], 1.0 / (T * GASC), nsw)
+        e = np.array([t[0] for t in tr])
+        cpu.append(e[burn - 1::every])
+    cpu = np.concatenate(cpu)
+    p = ks_2samp(gpu, cpu).pvalue
+    say(f"KS p = {p:.3g}, gpu mean {gpu.mean():.3f}, cpu mean {cpu.mean():.3f}")
+    assert p > 1e-3
