@@ -106,6 +106,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=30, help="untimed sweeps from the random start")
     ap.add_argument("--precision", type=int, default=0, help="0 FP64, 1 FP32 pair terms")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy a MIB-MiB device buffer 5 times (calibrates HBM byte counters) and exit")
+    ap.add_argument("--phases", action="store_true",
+                    help="also time the sweep kernel's phases on the device clock (adds barriers)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of this build",
@@ -115,6 +119,14 @@ def main():
     from smmp_amd import _ffi, workloads
 
     torch.cuda.set_device(0)
+    if args.copy_probe:
+        x = torch.ones(args.copy_probe << 18, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        log(f"copied {args.copy_probe} MiB x 5")
+        return
     tmp = tempfile.mkdtemp(prefix="smmp_bench_")
     c, seq, seqf = make_system(tmp)
     temps = workloads.geometric_temperatures(NREP, TMIN, TMAX)
@@ -126,7 +138,7 @@ def main():
     log(f"warmup {args.warmup} sweeps")
     out = c.sample(temps, seeds_w, args.warmup, v0, mode=1, precision=args.precision, with_traj=False)
     v1 = out["vlvr"]
-    c.profile(True)
+    c.profile(2 if args.phases else 1)
     torch.cuda.synchronize()
     log(f"timed {args.steps} sweeps")
     t0 = time.perf_counter()
@@ -135,6 +147,12 @@ def main():
     dt = time.perf_counter() - t0
     ks = c.profile_read(_ffi.KERNEL_SWEEP)
     kx = c.profile_read(_ffi.KERNEL_EXCHANGE)
+    ntrial = NREP * args.steps * nvr
+    log("work per trial: " + ", ".join(f"{k} {ks[k] / ntrial:.1f}" for k in ("pairs", "contacts", "builds", "moved")))
+    if args.phases:
+        ph = c.profile_phases()
+        log("phase us per trial: " + ", ".join(f"{k} {ph[k] * 1e3 / ntrial:.2f}" for k in _ffi.Context.PHASES)
+            + f"; full-path trials {ph['full_trials']:.0f}")
     c.profile(False)
     value = NREP * args.steps / dt
     npnt = int(c.view("pnt.npnt")[0])

@@ -138,6 +138,13 @@ const char *smmp_build_info(void);
  * chain builds, atoms rotated by trial moves, summed over the launches. */
 int smmp_profile(smmp_ctx *ctx, int enable);
 int smmp_profile_read(smmp_ctx *ctx, int kernel, double *total_ms, int64_t *launches, double *work);
+/* With smmp_profile(ctx, 2) the sweep kernel also times its phases on the
+ * device clock (one replica = one workgroup, thread 0 after each barrier).
+ * out[8]: milliseconds summed over replicas for proposal, rotation, pair
+ * change (+ contact marking, torsion term), solvent change, acceptance and
+ * commit, full-recomputation trials, sweep-end rebuild; out[7] = number of
+ * trials that took the full-recomputation path (|E| >= 1e5). */
+int smmp_profile_phases(smmp_ctx *ctx, double *out);
 
 #ifdef __cplusplus
 }

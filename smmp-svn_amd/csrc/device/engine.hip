@@ -108,7 +108,7 @@ struct DevMem {
 // Replica state of the samplers (grown on demand, reused across calls).
 struct SamplerBuffers {
   DevMem idvr, axvr, vlvr, xyz, nfree, valid, eol, acc, mt, rng_of, slot_of, beta, err, dbg;
-  DevMem wb, walpha, hist, traj, ipoi, iswitch, work, trace, trace_n;
+  DevMem wb, walpha, hist, traj, ipoi, iswitch, work, trace, trace_n, phase;
 };
 
 // One system uploaded to the device: a single allocation holding every
@@ -212,6 +212,9 @@ struct KernelStat {
 };
 struct Profiler {
   bool on = false;
+  bool phases = false;          // level 2: per-phase wall-clock of the sweep kernel
+  double phase_ticks[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  double tick_khz = 0.0;
   std::vector<hipEvent_t> pool;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
   KernelStat stat[kNKernels];
@@ -356,7 +359,23 @@ void engine_profile(Engine*& e, int enable) {
   Engine& E = engine_of(e);
   prof_collect(E);
   E.prof.on = enable != 0;
+  E.prof.phases = enable >= 2;
   for (auto& st : E.prof.stat) st = KernelStat();
+  for (double& t : E.prof.phase_ticks) t = 0.0;
+  if (E.prof.phases && E.prof.tick_khz == 0.0) {
+    int khz = 0;
+    HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, E.device));
+    E.prof.tick_khz = khz;
+  }
+}
+
+int engine_profile_phases(Engine*& e, double* out) {
+  Engine& E = engine_of(e);
+  for (int k = 0; k < 8; ++k) {
+    const double t = E.prof.phase_ticks[k];
+    out[k] = (k == 7 || E.prof.tick_khz <= 0.0) ? t : t / E.prof.tick_khz;
+  }
+  return SMMP_OK;
 }
 
 int engine_profile_read(Engine*& e, int kernel, double* ms, int64_t* launches, double* work) {

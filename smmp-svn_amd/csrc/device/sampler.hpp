@@ -89,7 +89,11 @@ struct SamplerParams {
   double* trace;         // optional: per trial of replica trace_rep, (var, E_trial, E_old, accepted)
   int32_t trace_rep, trace_cap;
   int32_t* trace_n;
+  double* phase;         // optional: [R][8] wall-clock ticks per trial phase (see PhaseId)
 };
+
+// Phases of a sweep timed by thread 0 when P.phase is set.
+enum PhaseId { kPhPropose = 0, kPhRotate, kPhPairs, kPhSolvent, kPhAccept, kPhFull, kPhSweepEnd, kPhNFull };
 
 // Extra LDS of the sampler beyond the energy carve-up.
 __host__ __device__ inline size_t sampler_lds_bytes(int np, int npnt, int nheavy, int nvt, int nwaves) {
@@ -288,6 +292,15 @@ __global__ void __launch_bounds__(512) k_sweep(SamplerParams P) {
   }
   __syncthreads();
   const double c4 = 4.0 * kPi / (double)(S.npnt > 0 ? S.npnt : 1);
+  double ph[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  uint64_t tp = wall_clock64();
+  auto lap = [&](int k) {
+    if (P.phase && tid == 0) {
+      const uint64_t now = wall_clock64();
+      ph[k] += (double)(now - tp);
+      tp = now;
+    }
+  };
 
   for (int sw = 0; sw < P.nsweep; ++sw) {
     const double beta = P.beta[P.slot_of[r]];
@@ -316,6 +329,7 @@ __global__ void __launch_bounds__(512) k_sweep(SamplerParams P) {
         Q.sc[11] = ux * un; Q.sc[12] = uy * un; Q.sc[13] = uz * un;
       }
       __syncthreads();
+      lap(kPhPropose);
       const int jv = (int)Q.sc[6];
       const int m1 = S.var_m1[jv], m2 = S.var_m2[jv];
       const bool full = Q.sc[5] != 0.0;
@@ -347,6 +361,7 @@ __global__ void __launch_bounds__(512) k_sweep(SamplerParams P) {
         if (tid == 0) Q.vl[jv] = Q.sc[2];
       }
       __syncthreads();
+      lap(kPhRotate);
       double enew;
       if (!full) {
         // ---- incremental energy change
@@ -366,6 +381,7 @@ __global__ void __launch_bounds__(512) k_sweep(SamplerParams P) {
           }
         }
         __syncthreads();
+        lap(kPhPairs);
         double dsol = 0.0;
         int err = 0;
         if (S.itysol > 0) {
@@ -409,6 +425,8 @@ __global__ void __launch_bounds__(512) k_sweep(SamplerParams P) {
           for (int k = 0; k < 4; ++k) Q.sc[24 + k] = comp[k];  // el, vw, hb, sol deltas (verify mode)
           Q.sc[28] = dt;
         }
+        if (P.phase) __syncthreads();  // uniform: timing only
+        lap(kPhSolvent);
       } else {
         // ---- full recomputation from the trial coordinates
         EnergyOut o;
@@ -416,7 +434,10 @@ __global__ void __launch_bounds__(512) k_sweep(SamplerParams P) {
         if (tid == 0) {
           Q.sc[16] = o.el + o.vw + o.hb + o.tor + o.sol;
           if (o.err) atomicOr(P.err, 2);
+          ph[kPhNFull] += 1.0;
         }
+        if (P.phase) __syncthreads();
+        lap(kPhFull);
       }
       if (P.verify && !full) {  // compare each incremental component with a full recomputation
         EnergyOut vnew;
@@ -486,6 +507,7 @@ __global__ void __launch_bounds__(512) k_sweep(SamplerParams P) {
         }
       }
       __syncthreads();
+      lap(kPhAccept);
     }
     // ---- sweep end: rebuild from the variables and check (metropolis.f:172-179)
     EnergyOut o;
@@ -511,7 +533,10 @@ __global__ void __launch_bounds__(512) k_sweep(SamplerParams P) {
       }
     }
     __syncthreads();
+    lap(kPhSweepEnd);
   }
+  if (P.phase && tid == 0)
+    for (int k = 0; k < 8; ++k) P.phase[8 * (size_t)r + k] += ph[k];
   // ---- write the replica state back
   for (int k = tid; k < nvt; k += blockDim.x) gvl[k] = Q.vl[k];
   for (int k = tid; k < 625; k += blockDim.x) gmt[k] = Q.mt[k];

@@ -273,4 +273,9 @@ int smmp_profile_read(smmp_ctx* c, int kernel, double* total_ms, int64_t* launch
   return guarded(c, SMMP_EDEVICE, [&] { return engine_profile_read(c->eng, kernel, total_ms, launches, work); });
 }
 
+int smmp_profile_phases(smmp_ctx* c, double* out) {
+  if (!out) return SMMP_EINVAL;
+  return guarded(c, SMMP_EDEVICE, [&] { return engine_profile_phases(c->eng, out); });
+}
+
 }  // extern "C"

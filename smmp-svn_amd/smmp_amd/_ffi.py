@@ -67,6 +67,7 @@ def _load():
         "smmp_build_info": (C.c_char_p, []),
         "smmp_profile": (C.c_int, [P, C.c_int]),
         "smmp_profile_read": (C.c_int, [P, C.c_int, dp, C.POINTER(C.c_int64), dp]),
+        "smmp_profile_phases": (C.c_int, [P, dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -81,7 +82,8 @@ lib = _load()
 EXPORTS = ("smmp_ctx_create", "smmp_ctx_destroy", "smmp_last_error", "smmp_state_view", "smmp_state_count",
            "smmp_state_name", "smmp_init_energy", "smmp_init_molecule", "smmp_setvar", "smmp_energy",
            "smmp_enysol", "smmp_sgrnd", "smmp_grnd", "smmp_outvar", "smmp_metropolis", "smmp_partem_s",
-           "smmp_energy_batch", "smmp_sample", "smmp_build_info", "smmp_profile", "smmp_profile_read")
+           "smmp_energy_batch", "smmp_sample", "smmp_build_info", "smmp_profile", "smmp_profile_read",
+           "smmp_profile_phases")
 
 KERNEL_SWEEP, KERNEL_EXCHANGE, KERNEL_ENERGY = 0, 1, 2
 
@@ -177,8 +179,18 @@ class Context:
         full[:len(vl)] = vl
         self.check(lib.smmp_setvar(self.h, nml, dptr(full)))
 
-    def profile(self, enable: bool = True) -> None:
-        self.check(lib.smmp_profile(self.h, 1 if enable else 0))
+    PHASES = ("propose", "rotate", "pairs", "solvent", "accept", "full", "sweep_end")
+
+    def profile(self, enable=True) -> None:
+        """enable: False/0 off, True/1 kernel timing and work, 2 also sweep phases."""
+        self.check(lib.smmp_profile(self.h, int(enable)))
+
+    def profile_phases(self) -> dict:
+        out = np.zeros(8)
+        self.check(lib.smmp_profile_phases(self.h, dptr(out)))
+        d = {k: out[i] for i, k in enumerate(self.PHASES)}
+        d["full_trials"] = out[7]
+        return d
 
     def profile_read(self, kernel: int) -> dict:
         ms = C.c_double()
