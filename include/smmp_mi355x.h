@@ -124,6 +124,9 @@ typedef struct {
   int32_t resync_every;   /* full recomputation cadence in sweeps (1 = every sweep) */
   int32_t verify;         /* 1: recompute every trial in full and fail on the first
                              incremental component that disagrees (slow; tests) */
+  int32_t groups;         /* workgroups per replica (upper bound; 0 = automatic).  With
+                             more than one, the workgroups of a replica split each
+                             energy change and reduce it on the device every trial */
 } smmp_sampler_cfg;
 int smmp_sample(smmp_ctx *ctx, const smmp_sampler_cfg *cfg, double *vlvr, double *eol,
                 double *acc, int32_t *ipoi, int32_t *hist, double *traj);
@@ -140,10 +143,11 @@ int smmp_profile(smmp_ctx *ctx, int enable);
 int smmp_profile_read(smmp_ctx *ctx, int kernel, double *total_ms, int64_t *launches, double *work);
 /* With smmp_profile(ctx, 2) the sweep kernel also times its phases on the
  * device clock (one replica = one workgroup, thread 0 after each barrier).
- * out[8]: milliseconds summed over replicas for proposal, rotation, pair
- * change (+ contact marking, torsion term), solvent change, acceptance and
- * commit, full-recomputation trials, sweep-end rebuild; out[7] = number of
- * trials that took the full-recomputation path (|E| >= 1e5). */
+ * out[9]: milliseconds summed over replicas for proposal, rotation, pair
+ * change (+ contact marking, torsion term), solvent change, the reduction
+ * across a replica's workgroups, acceptance and commit, full-recomputation
+ * trials, sweep-end rebuild; out[8] = number of trials that took the
+ * full-recomputation path (|E| >= 1e5). */
 int smmp_profile_phases(smmp_ctx *ctx, double *out);
 
 #ifdef __cplusplus

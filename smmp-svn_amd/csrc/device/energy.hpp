@@ -19,21 +19,32 @@ struct EnergyLds {
   double *pA, *pA14, *pC, *pHB;  // 324 each
   double* sp;              // npnt*3
   double* red;             // nwaves*8
+  double *rv, *sg;         // solvent radius and sigma per atom (np each)
+  const uint64_t* bits;    // pair bit rows: an LDS copy, or the system's array in HBM
   int8_t* t;               // np bytes
+  int16_t* hv;             // heavy-atom list (nheavy)
   float *xf, *yf, *zf, *qf;  // mixed-mode copies (alias fr after the build)
 };
 
+// Number of u64 words of the pair bit rows (two planes per 64-column word).
+__host__ __device__ inline size_t pair_bit_words(int nat, int words) { return (size_t)nat * words * 2; }
+
+// Per-wave solvent scratch: kMaxNb neighbours x 4 doubles, then kMaxNb
+// int32 candidate indices (kMaxNb/2 doubles).
+constexpr int kNbDoubles = kMaxNb * 4 + kMaxNb / 2;
 __host__ __device__ inline int energy_region1(int np, int nwaves) {
-  int a = 6 * np, b = nwaves * kMaxNb * 4;
+  int a = 6 * np, b = nwaves * kNbDoubles;
   return a > b ? a : b;
 }
-__host__ __device__ inline size_t energy_lds_bytes(int np, int npnt, int nwaves) {
-  size_t d = 4 * (size_t)np + energy_region1(np, nwaves) + 4 * 324 + 3 * (size_t)npnt + 8 * (size_t)nwaves;
-  return d * sizeof(double) + (size_t)np;
+// nbits: u64 words of pair bits staged in LDS (0: read from HBM).
+__host__ __device__ inline size_t energy_lds_bytes(int np, int npnt, int nwaves, int nheavy, size_t nbits) {
+  size_t d = 6 * (size_t)np + energy_region1(np, nwaves) + 4 * 324 + 3 * (size_t)npnt + 8 * (size_t)nwaves + nbits;
+  return d * sizeof(double) + (size_t)np + 2 * (size_t)nheavy;
 }
 
-__device__ inline EnergyLds carve_energy_lds(double* base, int np, int npnt, int nwaves) {
+__device__ inline EnergyLds carve_energy_lds(double* base, const DevSys& S, int nwaves, bool bits_lds) {
   EnergyLds L;
+  const int np = S.np, npnt = S.npnt;
   double* p = base;
   L.x = p; p += np;
   L.y = p; p += np;
@@ -46,7 +57,16 @@ __device__ inline EnergyLds carve_energy_lds(double* base, int np, int npnt, int
   L.pHB = p; p += 324;
   L.sp = p; p += 3 * npnt;
   L.red = p; p += 8 * nwaves;
+  L.rv = p; p += np;
+  L.sg = p; p += np;
+  if (bits_lds) {
+    L.bits = reinterpret_cast<const uint64_t*>(p);
+    p += pair_bit_words(S.nat, S.words);
+  } else {
+    L.bits = S.bits;
+  }
   L.t = reinterpret_cast<int8_t*>(p);
+  L.hv = reinterpret_cast<int16_t*>(L.t + np);  // np is a multiple of 64
   L.xf = reinterpret_cast<float*>(L.fr);
   L.yf = L.xf + np;
   L.zf = L.yf + np;
@@ -75,13 +95,23 @@ __device__ void energy_stage_static(const DevSys& S, EnergyLds& L) {
     L.pHB[k] = (double)S.pHB[k];
   }
   for (int k = tid; k < 3 * S.npnt; k += nt) L.sp[k] = S.spoint[k];
+  for (int h = tid; h < S.nheavy; h += nt) L.hv[h] = S.heavy[h];
+  if (L.bits != S.bits) {
+    uint64_t* b = const_cast<uint64_t*>(L.bits);
+    const size_t n = pair_bit_words(S.nat, S.words);
+    for (size_t k = tid; k < n; k += nt) b[k] = S.bits[k];
+  }
   for (int a = tid; a < np; a += nt) {
     if (a < S.nat) {
       L.q[a] = S.q[a];
       L.t[a] = S.type[a];
+      L.rv[a] = S.rvdw[a];
+      L.sg[a] = S.sigma[a];
     } else {
       L.q[a] = 0.0;
       L.t[a] = 0;
+      L.rv[a] = 0.0;
+      L.sg[a] = 0.0;
       L.x[a] = 0.0;
       L.y[a] = 0.0;
       L.z[a] = 0.0;
@@ -205,12 +235,12 @@ struct PairSums {
   double el, vw, hb;
 };
 
-// Pair terms over the tiles this wave owns (FP64).
-__device__ PairSums pair_terms_f64(const DevSys& S, const EnergyLds& L, int wave, int nwaves, Work& wk) {
+// Pair terms over the tiles this wave owns (FP64): tiles t0, t0 + stride, ...
+__device__ PairSums pair_terms_f64(const DevSys& S, const EnergyLds& L, int t0, int stride, Work& wk) {
   const int lane = threadIdx.x & (kWave - 1);
   const int nat = S.nat, nt = (nat + 63) / 64, ntile = nt * (nt + 1) / 2, words = S.words;
   double el = 0.0, vw = 0.0, hb = 0.0;
-  for (int t = wave; t < ntile; t += nwaves) {
+  for (int t = t0; t < ntile; t += stride) {
     int bi, bj;
     tile_coords(t, nt, bi, bj);
     const int i = bi * 64 + lane;
@@ -219,8 +249,8 @@ __device__ PairSums pair_terms_f64(const DevSys& S, const EnergyLds& L, int wave
     double xi = 0, yi = 0, zi = 0, qi = 0;
     int ti = 0;
     if (vi) {
-      w0 = S.bits[((size_t)i * words + bj) * 2 + 0];
-      w1 = S.bits[((size_t)i * words + bj) * 2 + 1];
+      w0 = L.bits[((size_t)i * words + bj) * 2 + 0];
+      w1 = L.bits[((size_t)i * words + bj) * 2 + 1];
       xi = L.x[i]; yi = L.y[i]; zi = L.z[i];
       qi = S.conv * L.q[i];
       ti = L.t[i] * kTy;
@@ -264,12 +294,12 @@ __device__ PairSums pair_terms_f64(const DevSys& S, const EnergyLds& L, int wave
 }
 
 // Mixed precision: FP32 terms, FP32 sums per tile, FP64 accumulation.
-__device__ PairSums pair_terms_mixed(const DevSys& S, const EnergyLds& L, int wave, int nwaves, Work& wk) {
+__device__ PairSums pair_terms_mixed(const DevSys& S, const EnergyLds& L, int t0, int stride, Work& wk) {
   const int lane = threadIdx.x & (kWave - 1);
   const int nat = S.nat, nt = (nat + 63) / 64, ntile = nt * (nt + 1) / 2, words = S.words;
   double el = 0.0, vw = 0.0, hb = 0.0;
   const float conv = (float)S.conv;
-  for (int t = wave; t < ntile; t += nwaves) {
+  for (int t = t0; t < ntile; t += stride) {
     int bi, bj;
     tile_coords(t, nt, bi, bj);
     const int i = bi * 64 + lane;
@@ -279,8 +309,8 @@ __device__ PairSums pair_terms_mixed(const DevSys& S, const EnergyLds& L, int wa
     float qi = 0;
     int ti = 0;
     if (vi) {
-      w0 = S.bits[((size_t)i * words + bj) * 2 + 0];
-      w1 = S.bits[((size_t)i * words + bj) * 2 + 1];
+      w0 = L.bits[((size_t)i * words + bj) * 2 + 0];
+      w1 = L.bits[((size_t)i * words + bj) * 2 + 1];
       xi = L.x[i]; yi = L.y[i]; zi = L.z[i];
       qi = conv * L.qf[i];
       ti = L.t[i] * kTy;
@@ -308,7 +338,12 @@ __device__ PairSums pair_terms_mixed(const DevSys& S, const EnergyLds& L, int wa
         const bool is14 = (w1 >> k) & 1ull;
         float A = (float)(is14 ? L.pA14[tt] : L.pA[tt]);
         float Cc = (float)L.pC[tt];
-        fel += qi * L.qf[j] * ri;
+        float ep_inv = 1.f;
+        if (S.epsd) {  // sigmoidal dielectric (enyshe.f:129-134)
+          const float sr = (float)S.slp * (r2 * ri), plt = (float)S.plt;
+          ep_inv = 1.f / (plt - (sr * sr + 2.f * sr + 2.f) * (plt - 1.f) * expf(-sr) / 2.f);
+        }
+        fel += qi * L.qf[j] * ri * ep_inv;
         if (L.pHB[tt] != 0.0)
           fhb += A * (r6i * r6i) - Cc * (r6i * r2i * r2i);
         else
@@ -337,44 +372,106 @@ __device__ double torsion_terms(const DevSys& S, const double* vl) {
 
 constexpr int kMaxPntPerLane = 4;  // tessellations of up to 256 points
 
-// enysol.f:192-324: area of heavy atom i from its free tessellation points.
-// Neighbours in contact are gathered into the wave's LDS scratch in chunks
-// of at most kMaxNb; a point is free when no neighbour of any chunk holds it
-// (the reference's 'last hit' cache only reorders this OR).  Returns the
-// number of free points, or -1 for a tessellation with more than 256 points.
+// Point tests of one chunk of neighbours: lane l owns points l, l+64, ...
+// (NS slots).  All slots advance together over the neighbours (padded to a
+// multiple of 4 with entries that occlude nothing), and the wave stops early
+// once every point it owns is occluded.  The expressions keep the
+// reference's operation order with contraction off, so every decision
+// equals the FP64 evaluation of enysol.f.
+template <int NS>
+__device__ __forceinline__ void solvent_points(const EnergyLds& L, int npnt, const double* nb, int cnt,
+                                               bool (&occ)[kMaxPntPerLane], int lane) {
+#pragma clang fp contract(off)
+  double px[NS], py[NS], pz[NS];
+#pragma unroll
+  for (int q = 0; q < NS; ++q) {
+    const int p = q * kWave + lane;
+    const bool in = p < npnt;
+    px[q] = in ? L.sp[3 * p + 0] : 0.0;
+    py[q] = in ? L.sp[3 * p + 1] : 0.0;
+    pz[q] = in ? L.sp[3 * p + 2] : 0.0;
+  }
+  for (int k0 = 0; k0 < cnt; k0 += 8) {
+#pragma unroll
+    for (int u = 0; u < 8; u += 4) {
+      if (k0 + u >= cnt) break;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int k = k0 + u + v;
+        const double nx = nb[4 * k + 0], ny = nb[4 * k + 1], nz = nb[4 * k + 2], r2 = nb[4 * k + 3];
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
+          const double ax = nx + px[q], ay = ny + py[q], az = nz + pz[q];
+          double sdd = ax * ax;
+          sdd = sdd + ay * ay;
+          sdd = sdd + az * az;
+          occ[q] |= sdd <= r2;
+        }
+      }
+    }
+    bool all = true;
+#pragma unroll
+    for (int q = 0; q < NS; ++q) all = all && occ[q];
+    if (__all(all)) break;
+  }
+}
+
+// enysol.f:192-324: area of heavy atom i from its free tessellation points,
+// by one wave.  Candidates come from a conservative distance filter; the
+// reference's contact test (with its divisions) runs once per candidate; the
+// contacts go to the wave's LDS scratch (nb, kNbDoubles) in chunks of at most
+// kMaxNb, and a point is free when no neighbour of any chunk holds it (the
+// reference's 'last hit' cache only reorders this OR).  Returns the number of
+// free points, or -1 for a tessellation with more than 256 points.
 __device__ int solvent_atom(const DevSys& S, const EnergyLds& L, double* nb, int i, Work& wk) {
+#pragma clang fp contract(off)
   const int lane = threadIdx.x & (kWave - 1);
-  if (S.npnt > kWave * kMaxPntPerLane) return -1;
+  const int npnt = S.npnt, nh = S.nheavy;
+  if (npnt > kWave * kMaxPntPerLane) return -1;
+  int32_t* cand = reinterpret_cast<int32_t*>(nb + kMaxNb * 4);
   const double xi = L.x[i], yi = L.y[i], zi = L.z[i];
-  const double trad = S.rvdw[i];
+  const double trad = L.rv[i];
   bool occ[kMaxPntPerLane];
 #pragma unroll
-  for (int q = 0; q < kMaxPntPerLane; ++q) occ[q] = (q * kWave + lane) >= S.npnt;
+  for (int q = 0; q < kMaxPntPerLane; ++q) occ[q] = (q * kWave + lane) >= npnt;
   int base = 0;
-  while (base < S.nheavy) {
-    int cnt = 0;
-    while (base < S.nheavy && cnt <= kMaxNb - kWave) {
+  while (base < nh) {
+    // 1. candidates: centres closer than the sum of radii (a superset of contacts)
+    int nc = 0;
+    while (base < nh && nc <= kMaxNb - kWave) {
       const int h = base + lane;
       bool on = false;
+      int j = 0;
+      if (h < nh) {
+        j = L.hv[h];
+        const double ax = xi - L.x[j], ay = yi - L.y[j], az = zi - L.z[j];
+        const double lim = (trad + L.rv[j]) * (1.0 + 1e-9);
+        on = j != i && ax * ax + ay * ay + az * az <= lim * lim;
+      }
+      const uint64_t m = __ballot(on);
+      if (on) cand[nc + __popcll(m & ((1ull << lane) - 1ull))] = j;
+      nc += __popcll(m);
+      base += kWave;
+    }
+    wave_lds_sync();
+    // 2. the reference's contact test, one lane per candidate
+    int cnt = 0;
+    for (int c0 = 0; c0 < nc; c0 += kWave) {
+      const int c = c0 + lane;
+      bool on = false;
       double dx = 0, dy = 0, dz = 0, r2b = 0;
-      if (h < S.nheavy) {
-        const int j = S.heavy[h];
-        if (j != i) {
-          const double rj = S.rvdw[j];
-          const double ax = xi - L.x[j], ay = yi - L.y[j], az = zi - L.z[j];
-          const double lim = (trad + rj) * (1.0 + 1e-9);
-          if (ax * ax + ay * ay + az * az <= lim * lim) {  // cheap filter, then the reference test
-            dx = ax / trad;
-            dy = ay / trad;
-            dz = az / trad;
-            double dd = dx * dx + dy * dy + dz * dz;
-            double akrad = rj / trad;
-            double dr = 1.0 + akrad;
-            dr = dr * dr;
-            on = dd <= dr;
-            r2b = akrad * akrad;
-          }
-        }
+      if (c < nc) {
+        const int j = cand[c];
+        const double rj = L.rv[j];
+        dx = (xi - L.x[j]) / trad;
+        dy = (yi - L.y[j]) / trad;
+        dz = (zi - L.z[j]) / trad;
+        double dd = dx * dx + dy * dy + dz * dz;
+        double akrad = rj / trad;
+        double dr = 1.0 + akrad;
+        dr = dr * dr;
+        on = dd <= dr;
+        r2b = akrad * akrad;
       }
       const uint64_t m = __ballot(on);
       if (on) {
@@ -385,26 +482,23 @@ __device__ int solvent_atom(const DevSys& S, const EnergyLds& L, double* nb, int
         nb[4 * pos + 3] = r2b;
       }
       cnt += __popcll(m);
-      base += kWave;
+    }
+    const int padded = (cnt + 3) & ~3;
+    if (lane < padded - cnt) {  // never occludes: sdd >= 0 > -1
+      const int pos = cnt + lane;
+      nb[4 * pos + 0] = 0.0;
+      nb[4 * pos + 1] = 0.0;
+      nb[4 * pos + 2] = 0.0;
+      nb[4 * pos + 3] = -1.0;
     }
     if (lane == 0) wk.contacts += cnt;
     wave_lds_sync();
-#pragma unroll
-    for (int q = 0; q < kMaxPntPerLane; ++q) {
-      const int p = q * kWave + lane;
-      if (occ[q]) continue;
-      const double px = L.sp[3 * p + 0], py = L.sp[3 * p + 1], pz = L.sp[3 * p + 2];
-      for (int k = 0; k < cnt; ++k) {
-        double ax = nb[4 * k + 0] + px, ay = nb[4 * k + 1] + py, az = nb[4 * k + 2] + pz;
-        double sdd = ax * ax;
-        sdd = sdd + ay * ay;
-        sdd = sdd + az * az;
-        if (sdd <= nb[4 * k + 3]) {
-          occ[q] = true;
-          break;
-        }
-      }
-    }
+    if (npnt <= kWave)
+      solvent_points<1>(L, npnt, nb, padded, occ, lane);
+    else if (npnt <= 2 * kWave)
+      solvent_points<2>(L, npnt, nb, padded, occ, lane);
+    else
+      solvent_points<4>(L, npnt, nb, padded, occ, lane);
     wave_lds_sync();
   }
   int nfree = 0;
@@ -420,10 +514,13 @@ struct EnergyOut {
 
 // Pair, torsion and solvent terms of the coordinates in L (all threads).
 // nfree_out (LDS, nheavy) receives the free-point count of every heavy atom
-// when not null.  On return thread 0 holds the totals in 'out'.
+// this block evaluates when not null.  On return thread 0 holds the totals
+// in 'out'.  With nparts > 1 the block evaluates share 'part' only (tiles
+// and heavy atoms k with k % nparts == part; torsions in part 0), and the
+// totals are partial sums.
 template <bool MIXED>
 __device__ void energy_from_coords(const DevSys& S, const double* vl, EnergyLds& L, int32_t* nfree_out,
-                                   EnergyOut& out, Work& wk) {
+                                   EnergyOut& out, Work& wk, int part = 0, int nparts = 1) {
   const int tid = threadIdx.x, nwaves = blockDim.x / kWave, wave = tid / kWave, lane = tid & (kWave - 1);
   if (MIXED) {  // FP32 copies centred on the first atom (frames are dead now)
     const double cx = L.x[0], cy = L.y[0], cz = L.z[0];
@@ -435,16 +532,17 @@ __device__ void energy_from_coords(const DevSys& S, const double* vl, EnergyLds&
     }
     __syncthreads();
   }
-  PairSums ps = MIXED ? pair_terms_mixed(S, L, wave, nwaves, wk) : pair_terms_f64(S, L, wave, nwaves, wk);
-  double tor = torsion_terms(S, vl);
+  const int t0 = part + nparts * wave, stride = nparts * nwaves;
+  PairSums ps = MIXED ? pair_terms_mixed(S, L, t0, stride, wk) : pair_terms_f64(S, L, t0, stride, wk);
+  double tor = part == 0 ? torsion_terms(S, vl) : 0.0;
   if (MIXED) __syncthreads();  // the solvent scratch overlays the FP32 copies
   double sol = 0.0, asa = 0.0;
   int err = 0;
   if (S.itysol > 0) {
-    double* nb = L.fr + (size_t)wave * kMaxNb * 4;
+    double* nb = L.fr + (size_t)wave * kNbDoubles;
     const double c4 = 4.0 * kPi / (double)S.npnt;
-    for (int h = wave; h < S.nheavy; h += nwaves) {
-      const int i = S.heavy[h];
+    for (int h = t0; h < S.nheavy; h += stride) {
+      const int i = L.hv[h];
       int nfree = solvent_atom(S, L, nb, i, wk);
       if (nfree < 0) {
         err = 1;
@@ -452,9 +550,9 @@ __device__ void energy_from_coords(const DevSys& S, const double* vl, EnergyLds&
       }
       if (lane == 0) {
         if (nfree_out) nfree_out[h] = nfree;
-        const double trad = S.rvdw[i];
+        const double trad = L.rv[i];
         double area = c4 * trad * trad * (double)nfree;
-        sol += area * S.sigma[i];
+        sol += area * L.sg[i];
         asa += area;
       }
     }
@@ -487,11 +585,11 @@ __device__ void energy_from_coords(const DevSys& S, const double* vl, EnergyLds&
 // Whole energy of one conformation from its variables (energy.f:12-119).
 template <bool MIXED>
 __device__ void conformation_energy(const DevSys& S, const double* vl, EnergyLds& L, int32_t* nfree_out,
-                                    EnergyOut& out, Work& wk) {
+                                    EnergyOut& out, Work& wk, int part = 0, int nparts = 1) {
   energy_stage_torsions(S, vl, L);
   __syncthreads();
   build_chain(S, L, wk);
-  energy_from_coords<MIXED>(S, vl, L, nfree_out, out, wk);
+  energy_from_coords<MIXED>(S, vl, L, nfree_out, out, wk, part, nparts);
 }
 
 }  // namespace smmp

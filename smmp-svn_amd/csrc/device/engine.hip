@@ -35,12 +35,12 @@ __global__ void __launch_bounds__(kEnergyThreads) k_energy(const DevSys* __restr
                                                            const int64_t* __restrict__ conf_off,
                                                            const double* __restrict__ vlvr, double* __restrict__ out,
                                                            double* __restrict__ xyz_out, int32_t* __restrict__ err,
-                                                           double* __restrict__ work) {
+                                                           double* __restrict__ work, int bits_lds) {
   extern __shared__ double smem[];
   const int c = blockIdx.x;
   const DevSys& S = systems[conf_sys[c]];
   const double* vl = vlvr + conf_off[c];
-  EnergyLds L = carve_energy_lds(smem, S.np, S.npnt, blockDim.x / kWave);
+  EnergyLds L = carve_energy_lds(smem, S, blockDim.x / kWave, bits_lds != 0);
   EnergyOut o;
   Work wk;
   energy_stage_static(S, L);
@@ -108,7 +108,7 @@ struct DevMem {
 // Replica state of the samplers (grown on demand, reused across calls).
 struct SamplerBuffers {
   DevMem idvr, axvr, vlvr, xyz, nfree, valid, eol, acc, mt, rng_of, slot_of, beta, err, dbg;
-  DevMem wb, walpha, hist, traj, ipoi, iswitch, work, trace, trace_n, phase;
+  DevMem wb, walpha, hist, traj, ipoi, iswitch, work, trace, trace_n, phase, part, sync;
 };
 
 // One system uploaded to the device: a single allocation holding every
@@ -213,7 +213,7 @@ struct KernelStat {
 struct Profiler {
   bool on = false;
   bool phases = false;          // level 2: per-phase wall-clock of the sweep kernel
-  double phase_ticks[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  double phase_ticks[kNPhase] = {};
   double tick_khz = 0.0;
   std::vector<hipEvent_t> pool;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
@@ -323,15 +323,34 @@ static DevSystem& current_system(Engine& E, const State& s) {
   return E.sys;
 }
 
-static size_t lds_for(const DevSys& d, int threads) { return energy_lds_bytes(d.np, d.npnt, threads / kWave); }
+constexpr size_t kLdsBytes = 160 * 1024;  // per workgroup on gfx950
+
+// LDS of k_energy for a set of systems: the largest carve-up, with the pair
+// bit rows staged in LDS when every system's copy fits.
+struct LdsPlan {
+  size_t bytes = 0;
+  int bits_lds = 0;
+};
+static LdsPlan plan_energy_lds(const DevSys* const* sys, int n, int threads) {
+  LdsPlan with, without;
+  for (int i = 0; i < n; ++i) {
+    const DevSys& d = *sys[i];
+    with.bytes = std::max(with.bytes, energy_lds_bytes(d.np, d.npnt, threads / kWave, d.nheavy,
+                                                       pair_bit_words(d.nat, d.words)));
+    without.bytes = std::max(without.bytes, energy_lds_bytes(d.np, d.npnt, threads / kWave, d.nheavy, 0));
+  }
+  with.bits_lds = 1;
+  return with.bytes <= kLdsBytes ? with : without;
+}
 
 static void check_lds(size_t bytes) {
-  if (bytes > 160 * 1024) throw DeviceError("system too large for one workgroup's LDS");
+  if (bytes > kLdsBytes) throw DeviceError("system too large for one workgroup's LDS");
 }
 
 // Launches k_energy for nconf conformations of the systems at d_systems.
 static void launch_energy(Engine& E, const DevSys* d_systems, const int32_t* d_cs, const int64_t* d_off,
-                          const double* d_vl, double* d_out, double* d_xyz, int nconf, size_t lds, int precision) {
+                          const double* d_vl, double* d_out, double* d_xyz, int nconf, LdsPlan plan, int precision) {
+  const size_t lds = plan.bytes;
   check_lds(lds);
   int32_t* d_err = static_cast<int32_t*>(E.d_err.ensure(sizeof(int32_t)));
   HIPCHK(hipMemsetAsync(d_err, 0, sizeof(int32_t), E.stream));
@@ -341,11 +360,11 @@ static void launch_energy(Engine& E, const DevSys* d_systems, const int32_t* d_c
   if (precision == 0) {
     HIPCHK(hipFuncSetAttribute((const void*)k_energy<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(k_energy<false>, dim3(nconf), dim3(kEnergyThreads), lds, E.stream, d_systems, d_cs, d_off, d_vl,
-                       d_out, d_xyz, d_err, d_work);
+                       d_out, d_xyz, d_err, d_work, plan.bits_lds);
   } else {
     HIPCHK(hipFuncSetAttribute((const void*)k_energy<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(k_energy<true>, dim3(nconf), dim3(kEnergyThreads), lds, E.stream, d_systems, d_cs, d_off, d_vl,
-                       d_out, d_xyz, d_err, d_work);
+                       d_out, d_xyz, d_err, d_work, plan.bits_lds);
   }
   HIPCHK(hipGetLastError());
   prof_mark(E, kKEnergy, false);
@@ -371,9 +390,9 @@ void engine_profile(Engine*& e, int enable) {
 
 int engine_profile_phases(Engine*& e, double* out) {
   Engine& E = engine_of(e);
-  for (int k = 0; k < 8; ++k) {
+  for (int k = 0; k < kNPhase; ++k) {
     const double t = E.prof.phase_ticks[k];
-    out[k] = (k == 7 || E.prof.tick_khz <= 0.0) ? t : t / E.prof.tick_khz;
+    out[k] = (k == kPhNFull || E.prof.tick_khz <= 0.0) ? t : t / E.prof.tick_khz;
   }
   return SMMP_OK;
 }
@@ -413,7 +432,7 @@ static void energy_into_state(Engine& E, State& s) {
   HIPCHK(hipMemcpyAsync(d_cs, &zero, sizeof(zero), hipMemcpyHostToDevice, E.stream));
   HIPCHK(hipMemcpyAsync(d_off, &zoff, sizeof(zoff), hipMemcpyHostToDevice, E.stream));
   launch_energy(E, static_cast<const DevSys*>(E.d_sys.p), d_cs, d_off, d_vl, d_out, d_xyz, 1,
-                lds_for(ds.d, kEnergyThreads), 0);
+                [&] { const DevSys* one = &ds.d; return plan_energy_lds(&one, 1, kEnergyThreads); }(), 0);
   double o[8];
   std::vector<double> xyz(3 * (size_t)nat);
   HIPCHK(hipMemcpyAsync(o, d_out, sizeof o, hipMemcpyDeviceToHost, E.stream));
@@ -455,14 +474,11 @@ int engine_energy_batch(Engine*& e, std::vector<State*>& sys, const int32_t* con
   const int nsys = (int)sys.size();
   E.batch_sys.clear();
   std::vector<DevSys> dsys(nsys);
-  int maxnp = 0, maxpnt = 0;
   for (int i = 0; i < nsys; ++i) {
     auto p = std::make_unique<DevSystem>();
     p->h = build_host_sys(*sys[i]);
     upload_system(*p, E.stream);
     dsys[i] = p->d;
-    maxnp = std::max(maxnp, p->d.np);
-    maxpnt = std::max(maxpnt, p->d.npnt);
     E.batch_sys.push_back(std::move(p));
   }
   std::vector<int64_t> off(nconf);
@@ -481,8 +497,10 @@ int engine_energy_batch(Engine*& e, std::vector<State*>& sys, const int32_t* con
   HIPCHK(hipMemcpyAsync(d_cs, conf_sys, sizeof(int32_t) * nconf, hipMemcpyHostToDevice, E.stream));
   HIPCHK(hipMemcpyAsync(d_off, off.data(), sizeof(int64_t) * nconf, hipMemcpyHostToDevice, E.stream));
   HIPCHK(hipMemcpyAsync(d_vl, vlvr, sizeof(double) * tot, hipMemcpyHostToDevice, E.stream));
-  size_t lds = energy_lds_bytes(maxnp, maxpnt, kEnergyThreads / kWave);
-  launch_energy(E, d_systems, d_cs, d_off, d_vl, d_out, nullptr, nconf, lds, precision);
+  std::vector<const DevSys*> host_sys(nsys);
+  for (int i = 0; i < nsys; ++i) host_sys[i] = &dsys[i];
+  launch_energy(E, d_systems, d_cs, d_off, d_vl, d_out, nullptr, nconf,
+                plan_energy_lds(host_sys.data(), nsys, kEnergyThreads), precision);
   HIPCHK(hipMemcpyAsync(out, d_out, sizeof(double) * 8 * nconf, hipMemcpyDeviceToHost, E.stream));
   HIPCHK(hipStreamSynchronize(E.stream));
   if (read_err(E)) throw Error("enysol> too many solvent neighbours for one atom");

@@ -90,46 +90,125 @@ struct SamplerParams {
   int32_t trace_rep, trace_cap;
   int32_t* trace_n;
   double* phase;         // optional: [R][8] wall-clock ticks per trial phase (see PhaseId)
+  int32_t bits_lds;      // stage the pair bit rows in LDS
+  // Workgroups per replica (G).  With G > 1 the G blocks of a replica hold
+  // the same state and split every energy change between them; they meet
+  // once per trial in group_sum.  The launch is cooperative (all blocks
+  // resident).
+  int32_t G;
+  int32_t xcd_groups;    // place the G blocks of a replica on one XCD
+  double* part;          // [R][2][G][kPartVals] partial sums
+  uint32_t* sync;        // [R] arrival counters, zero at launch
+  uint64_t wait_ticks;   // bound of one wait, device wall-clock ticks
 };
+constexpr int kPartVals = 8;
 
 // Phases of a sweep timed by thread 0 when P.phase is set.
-enum PhaseId { kPhPropose = 0, kPhRotate, kPhPairs, kPhSolvent, kPhAccept, kPhFull, kPhSweepEnd, kPhNFull };
+enum PhaseId { kPhPropose = 0, kPhRotate, kPhPairs, kPhSolvent, kPhReduce, kPhAccept, kPhFull, kPhSweepEnd,
+               kPhNFull, kNPhase };
 
 // Extra LDS of the sampler beyond the energy carve-up.
-__host__ __device__ inline size_t sampler_lds_bytes(int np, int npnt, int nheavy, int nvt, int nwaves) {
-  size_t b = energy_lds_bytes(np, npnt, nwaves);
+__host__ __device__ inline size_t sampler_lds_bytes(int np, int npnt, int nheavy, int nvt, int nwaves, size_t nbits) {
+  size_t b = energy_lds_bytes(np, npnt, nwaves, nheavy, nbits);
   b = (b + 15) & ~size_t(15);
   b += 3 * (size_t)np * sizeof(double);           // backup of moved coordinates
   b += (size_t)nvt * sizeof(double);              // variables
   b += 2 * (size_t)nheavy * sizeof(int32_t);      // nfree current / trial
   b += 625 * sizeof(uint32_t);                    // MT state
-  b += 32 * sizeof(double);                       // broadcast scalars
+  b += 48 * sizeof(double);                       // broadcast scalars and phase clocks
+  b += 7 * (size_t)nvt * sizeof(double);          // per-variable tables (VarLds)
+  b += 6 * (size_t)nvt * sizeof(int16_t);
+  b += 2 * (size_t)nheavy;                        // list of the affected heavy atoms
   b += (size_t)nheavy;                            // affected flags
   return b + 64;
 }
 
+// Per-variable tables read by every trial, staged in LDS: step width,
+// rotation sign, torsion-energy parameters, the torsion atom's snapshot
+// (toat0, tref) and controlling variable, moved range, axis atoms.
+struct VarLds {
+  double *ax, *sign, *e0, *sg, *rn, *t0, *tref;
+  int16_t *m1, *m2, *base, *axis0, *tv, *idvr;
+};
+
 struct SamplerLds {
   EnergyLds L;
+  VarLds V;
   double *bx, *by, *bz, *vl, *sc;
   int32_t *nf, *nf2;
   uint32_t* mt;
+  int16_t* alist;
   int8_t* aff;
 };
 
-__device__ inline SamplerLds carve_sampler_lds(double* base, const DevSys& S, int nvt, int nwaves) {
+__device__ inline SamplerLds carve_sampler_lds(double* base, const DevSys& S, int nvt, int nwaves, bool bits_lds) {
   SamplerLds Q;
-  Q.L = carve_energy_lds(base, S.np, S.npnt, nwaves);
-  char* p = reinterpret_cast<char*>(base) + ((energy_lds_bytes(S.np, S.npnt, nwaves) + 15) & ~size_t(15));
+  Q.L = carve_energy_lds(base, S, nwaves, bits_lds);
+  const size_t nbits = bits_lds ? pair_bit_words(S.nat, S.words) : 0;
+  char* p = reinterpret_cast<char*>(base) + ((energy_lds_bytes(S.np, S.npnt, nwaves, S.nheavy, nbits) + 15) & ~size_t(15));
   Q.bx = reinterpret_cast<double*>(p); p += S.np * sizeof(double);
   Q.by = reinterpret_cast<double*>(p); p += S.np * sizeof(double);
   Q.bz = reinterpret_cast<double*>(p); p += S.np * sizeof(double);
   Q.vl = reinterpret_cast<double*>(p); p += nvt * sizeof(double);
-  Q.sc = reinterpret_cast<double*>(p); p += 32 * sizeof(double);
+  Q.sc = reinterpret_cast<double*>(p); p += 48 * sizeof(double);
+  // doubles first (8-byte aligned), then 32-, 16- and 8-bit arrays
+  Q.V.ax = reinterpret_cast<double*>(p); p += nvt * sizeof(double);
+  Q.V.sign = reinterpret_cast<double*>(p); p += nvt * sizeof(double);
+  Q.V.e0 = reinterpret_cast<double*>(p); p += nvt * sizeof(double);
+  Q.V.sg = reinterpret_cast<double*>(p); p += nvt * sizeof(double);
+  Q.V.rn = reinterpret_cast<double*>(p); p += nvt * sizeof(double);
+  Q.V.t0 = reinterpret_cast<double*>(p); p += nvt * sizeof(double);
+  Q.V.tref = reinterpret_cast<double*>(p); p += nvt * sizeof(double);
   Q.nf = reinterpret_cast<int32_t*>(p); p += S.nheavy * sizeof(int32_t);
   Q.nf2 = reinterpret_cast<int32_t*>(p); p += S.nheavy * sizeof(int32_t);
   Q.mt = reinterpret_cast<uint32_t*>(p); p += 625 * sizeof(uint32_t);
+  Q.V.m1 = reinterpret_cast<int16_t*>(p); p += nvt * sizeof(int16_t);
+  Q.V.m2 = reinterpret_cast<int16_t*>(p); p += nvt * sizeof(int16_t);
+  Q.V.base = reinterpret_cast<int16_t*>(p); p += nvt * sizeof(int16_t);
+  Q.V.axis0 = reinterpret_cast<int16_t*>(p); p += nvt * sizeof(int16_t);
+  Q.V.tv = reinterpret_cast<int16_t*>(p); p += nvt * sizeof(int16_t);
+  Q.V.idvr = reinterpret_cast<int16_t*>(p); p += nvt * sizeof(int16_t);
+  Q.alist = reinterpret_cast<int16_t*>(p); p += S.nheavy * sizeof(int16_t);
   Q.aff = reinterpret_cast<int8_t*>(p);
   return Q;
+}
+
+// Stages the per-variable tables (all threads; a barrier must follow).
+__device__ void stage_variables(const SamplerParams& P, const DevSys& S, const VarLds& V) {
+  for (int k = threadIdx.x; k < P.nvt; k += blockDim.x) {
+    V.ax[k] = P.axvr[k];
+    V.sign[k] = S.var_sign[k];
+    const double e0 = S.var_e0[k];
+    V.e0[k] = e0;
+    V.sg[k] = S.var_sg[k];
+    V.rn[k] = S.var_rn[k];
+    V.m1[k] = S.var_m1[k];
+    V.m2[k] = S.var_m2[k];
+    V.base[k] = S.var_base[k];
+    V.axis0[k] = S.var_axis0[k];
+    double t0 = 0.0, tref = 0.0;
+    int16_t tv = -1;
+    if (e0 != 0.0) {
+      const int ia = S.var_atom[k];
+      t0 = S.toat0[ia];
+      tref = S.tref[ia];
+      tv = S.tvar[ia];
+    }
+    V.t0[k] = t0;
+    V.tref[k] = tref;
+    V.tv[k] = tv;
+  }
+  for (int k = threadIdx.x; k < P.nvr; k += blockDim.x) V.idvr[k] = (int16_t)P.idvr[k];
+}
+
+// Torsion of variable k's torsion atom for the values vl (as atom_torsion).
+__device__ __forceinline__ double var_torsion(const VarLds& V, const double* vl, int k) {
+  const int v = V.tv[k];
+  const double t0 = V.t0[k];
+  if (v < 0) return t0;
+  double to = t0 + d_difang(V.tref[k], vl[v]);
+  if (fabs(to) > kPi) to = to - copysign(kPi2, to);
+  return to;
 }
 
 // Weight of an energy (canon.f:135-149, mulcan_par_mod.f90:380-412).
@@ -142,43 +221,40 @@ __device__ inline double weight_of(const SamplerParams& P, double beta, double x
   return P.wb[k] * x + P.walpha[k];
 }
 
-// ECEPP pair energy (Coulomb, LJ or H-bond; 1-4 repulsion from A14).
+// ECEPP pair energy split into Coulomb and the 6-12 (or 10-12 H-bond) term;
+// A carries a14 for 1-4 pairs.  Branch-free in the pair type.
 struct PairE {
   double el, vw, hb;
 };
-__device__ __forceinline__ void pair_energy(const DevSys& S, const EnergyLds& L, double dx, double dy, double dz,
-                                            double qq, int tt, bool is14, PairE& e, double sgn) {
-  double r2 = dx * dx + dy * dy + dz * dz;
-  double ri = rsqrt_f64(r2);
-  double r2i = ri * ri;
-  double r6i = r2i * r2i * r2i;
-  double A = is14 ? L.pA14[tt] : L.pA[tt];
-  double Cc = L.pC[tt];
+__device__ __forceinline__ void pair_split(const DevSys& S, double dx, double dy, double dz, double qq, double A,
+                                           double C, bool hb, double& el, double& lj) {
+  const double r2 = dx * dx + dy * dy + dz * dz;
+  const double ri = rsqrt_f64(r2);
+  const double r2i = ri * ri;
+  const double r6i = r2i * r2i * r2i;
   double ep_inv = 1.0;
-  if (S.epsd) {
-    double rij = r2 * ri;
-    double sr = S.slp * rij;
+  if (S.epsd) {  // sigmoidal dielectric (enyshe.f:129-134)
+    const double sr = S.slp * (r2 * ri);
     ep_inv = 1.0 / (S.plt - (sr * sr + 2.0 * sr + 2.0) * (S.plt - 1.0) * exp(-sr) / 2.0);
   }
-  e.el += sgn * (qq * ri * ep_inv);
-  if (L.pHB[tt] != 0.0)
-    e.hb += sgn * (A * (r6i * r6i) - Cc * (r6i * r2i * r2i));
-  else
-    e.vw += sgn * (A * (r6i * r6i) - Cc * r6i);
+  el = qq * ri * ep_inv;
+  lj = A * (r6i * r6i) - C * (hb ? r6i * r2i * r2i : r6i);
 }
 
 // Energy change of the pairs between the moved range [m1,m2] (new
 // coordinates in L, old in the backup) and the fixed atoms.  The smaller
-// set is walked uniformly, the larger one across the lanes.
-__device__ void pair_delta(const DevSys& S, const SamplerLds& Q, int m1, int m2, PairE& d, Work& wk) {
+// set is walked uniformly, the larger one across the lanes.  Work items
+// item0, item0 + stride, ... of the (row, 64-column block) grid.
+__device__ void pair_delta(const DevSys& S, const SamplerLds& Q, int m1, int m2, int item0, int stride, PairE& d,
+                           Work& wk) {
   const EnergyLds& L = Q.L;
-  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave, nwaves = blockDim.x / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
   const int nat = S.nat, nM = m2 - m1 + 1, nO = nat - nM, words = S.words;
   if (nM <= 0 || nO <= 0) return;
   const bool rowsM = nM <= nO;            // uniform rows from the smaller set
   const int nrow = rowsM ? nM : nO, ncol = rowsM ? nO : nM;
   const int nob = (ncol + 63) / 64;
-  for (int item = wave; item < nrow * nob; item += nwaves) {
+  for (int item = item0; item < nrow * nob; item += stride) {
     const int r = item / nob, ob = item - r * nob;
     const int c = ob * 64 + lane;
     if (c >= ncol) continue;
@@ -190,78 +266,194 @@ __device__ void pair_delta(const DevSys& S, const SamplerLds& Q, int m1, int m2,
       b = r < m1 ? r : r + nM;
       a = m1 + c;
     }
-    const uint64_t w0 = S.bits[((size_t)a * words + (b >> 6)) * 2 + 0];
+    const size_t w = ((size_t)a * words + (b >> 6)) * 2;
+    const uint64_t w0 = L.bits[w];
     if (!((w0 >> (b & 63)) & 1ull)) continue;
     wk.pairs += 2.0;
-    const bool is14 = (S.bits[((size_t)a * words + (b >> 6)) * 2 + 1] >> (b & 63)) & 1ull;
+    const bool is14 = (L.bits[w + 1] >> (b & 63)) & 1ull;
     const int tt = L.t[a] * kTy + L.t[b];
     const double qq = S.conv * L.q[a] * L.q[b];
+    const double A = is14 ? L.pA14[tt] : L.pA[tt];
+    const double C = L.pC[tt];
+    const bool hb = L.pHB[tt] != 0.0;
     const double xb = L.x[b], yb = L.y[b], zb = L.z[b];
-    pair_energy(S, L, L.x[a] - xb, L.y[a] - yb, L.z[a] - zb, qq, tt, is14, d, 1.0);
     const int k = a - m1;
-    pair_energy(S, L, Q.bx[k] - xb, Q.by[k] - yb, Q.bz[k] - zb, qq, tt, is14, d, -1.0);
+    double eln, ljn, elo, ljo;
+    pair_split(S, L.x[a] - xb, L.y[a] - yb, L.z[a] - zb, qq, A, C, hb, eln, ljn);
+    pair_split(S, Q.bx[k] - xb, Q.by[k] - yb, Q.bz[k] - zb, qq, A, C, hb, elo, ljo);
+    d.el += eln - elo;
+    const double dlj = ljn - ljo;
+    d.hb += hb ? dlj : 0.0;
+    d.vw += hb ? 0.0 : dlj;
   }
 }
 
 // Marks the heavy atoms whose accessible area may change: every moved atom
 // (its 122 points stay fixed in the laboratory frame while it rotates, so
 // even a rigidly moved environment changes its discretised area) and every
-// fixed atom in contact with a moved one before or after the move.
-__device__ void mark_affected(const DevSys& S, const SamplerLds& Q, int m1, int m2) {
+// fixed atom in contact with a moved one before or after the move.  Only
+// the fixed atoms of this block's share (h % G == g) are tested.
+__device__ void mark_affected(const DevSys& S, const SamplerLds& Q, int m1, int m2, int g, int G) {
   const int nh = S.nheavy;
   int h1 = 0, h2 = nh;  // heavy atoms of [m1,m2]: one index range (heavy is sorted)
   {
     int lo = 0, hi = nh;
     while (lo < hi) {
       int mid = (lo + hi) >> 1;
-      if (S.heavy[mid] < m1) lo = mid + 1; else hi = mid;
+      if (Q.L.hv[mid] < m1) lo = mid + 1; else hi = mid;
     }
     h1 = lo;
     lo = h1, hi = nh;
     while (lo < hi) {
       int mid = (lo + hi) >> 1;
-      if (S.heavy[mid] <= m2) lo = mid + 1; else hi = mid;
+      if (Q.L.hv[mid] <= m2) lo = mid + 1; else hi = mid;
     }
     h2 = lo;
   }
-  const int nM = h2 - h1, nO = nh - nM;
+  const int nM = h2 - h1, nown = (nh - g + G - 1) / G;
   for (int h = h1 + threadIdx.x; h < h2; h += blockDim.x) Q.aff[h] = 1;
-  for (int it = threadIdx.x; it < nM * nO; it += blockDim.x) {
-    const int r = it / nO, c = it - r * nO;
-    const int ha = h1 + r, hb = c < h1 ? c : c + nM;
-    const int a = S.heavy[ha], b = S.heavy[hb];
-    const double lim = (S.rvdw[a] + S.rvdw[b]) * (1.0 + 1e-9);
+  for (int it = threadIdx.x; it < nM * nown; it += blockDim.x) {
+    const int r = it / nown, k = it - r * nown;
+    const int ha = h1 + r, hb = g + G * k;
+    if (hb >= h1 && hb < h2) continue;
+    const int a = Q.L.hv[ha], b = Q.L.hv[hb];
+    const double lim = (Q.L.rv[a] + Q.L.rv[b]) * (1.0 + 1e-9);
     const double l2 = lim * lim;
     const double xb = Q.L.x[b], yb = Q.L.y[b], zb = Q.L.z[b];
     double dx = Q.L.x[a] - xb, dy = Q.L.y[a] - yb, dz = Q.L.z[a] - zb;
     bool hit = dx * dx + dy * dy + dz * dz <= l2;
-    const int k = a - m1;
-    dx = Q.bx[k] - xb; dy = Q.by[k] - yb; dz = Q.bz[k] - zb;
+    const int k2 = a - m1;
+    dx = Q.bx[k2] - xb; dy = Q.by[k2] - yb; dz = Q.bz[k2] - zb;
     hit = hit || (dx * dx + dy * dy + dz * dz <= l2);
-    if (hit) {
-      Q.aff[ha] = 1;
-      Q.aff[hb] = 1;
-    }
+    if (hit) Q.aff[hb] = 1;
   }
 }
 
-// One block = one replica; P.nsweep Metropolis sweeps.
+// Wave 0: Q.alist = the affected heavy atoms of share g in increasing
+// order; returns their number (valid in wave 0; a barrier must follow).
+__device__ int list_affected(const DevSys& S, const SamplerLds& Q, int g, int G) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int nown = (S.nheavy - g + G - 1) / G;
+  int cnt = 0;
+  for (int k0 = 0; k0 < nown; k0 += kWave) {
+    const int k = k0 + lane;
+    const int h = g + G * k;
+    const bool on = k < nown && Q.aff[h] != 0;
+    const uint64_t m = __ballot(on);
+    if (on) Q.alist[cnt + __popcll(m & ((1ull << lane) - 1ull))] = (int16_t)h;
+    cnt += __popcll(m);
+  }
+  return cnt;
+}
+
+// Full energy of the replica in LDS: from the coordinates, or (rebuild) from
+// the variables through the chain build.  Out of line, so that the trial
+// loop's register budget does not carry the full evaluation.
+struct ReplicaEnergy {
+  EnergyOut o;
+  Work w;
+};
+template <bool MIXED>
+__device__ __attribute__((noinline)) ReplicaEnergy replica_energy(const DevSys& S, const double* vl, EnergyLds L,
+                                                                  int32_t* nfree_out, bool rebuild, int part,
+                                                                  int nparts) {
+  ReplicaEnergy r;
+  if (rebuild)
+    conformation_energy<MIXED>(S, vl, L, nfree_out, r.o, r.w, part, nparts);
+  else
+    energy_from_coords<MIXED>(S, vl, L, nfree_out, r.o, r.w, part, nparts);
+  return r;
+}
+__device__ __forceinline__ void add_work(Work& a, const Work& b) {
+  a.pairs += b.pairs;
+  a.contacts += b.contacts;
+  a.builds += b.builds;
+  a.moved += b.moved;
+}
+
+// Replica and share of block b.  Blocks are dispatched round-robin over
+// the 8 XCDs, so with xcd_groups the G blocks of a replica are b = x + 8q
+// for one x (one L2 for their reductions).
+__device__ __forceinline__ void replica_of_block(const SamplerParams& P, int b, int& r, int& g) {
+  const int G = P.G;
+  if (G == 1) {
+    r = P.rep0 + b;
+    g = 0;
+  } else if (P.xcd_groups) {
+    const int x = b & 7, q = b >> 3;
+    r = P.rep0 + (q / G) * 8 + x;
+    g = q % G;
+  } else {
+    r = P.rep0 + b / G;
+    g = b % G;
+  }
+}
+
+// Thread 0 of each of the G blocks of replica r: replaces v[0..nv) (this
+// block's partial sums) by the totals over the G blocks.  Every block adds
+// the partials in block order, so all get bit-identical totals and take the
+// same decisions.  Partials are double-buffered by the parity of the
+// reduction index: a block can be at most one reduction ahead of another
+// (it waits for all of them at each reduction), so it never overwrites
+// values still being read.  The wait is bounded on the device clock; on
+// expiry the error flag gets bit 3 and the block goes on.
+__device__ void group_sum(const SamplerParams& P, int r, int g, uint32_t& epoch, double* v, int nv) {
+  const int G = P.G;
+  if (G == 1) return;
+  double* base = P.part + ((size_t)r * 2 + (epoch & 1u)) * G * kPartVals;
+  double* mine = base + (size_t)g * kPartVals;
+  for (int k = 0; k < nv; ++k) mine[k] = v[k];
+  __hip_atomic_fetch_add(&P.sync[r], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  const uint32_t target = (epoch + 1u) * (uint32_t)G;
+  const uint64_t t0 = wall_clock64();
+  // relaxed polls (no cache invalidation per poll), one acquire fence after
+  while (__hip_atomic_load(&P.sync[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    __builtin_amdgcn_s_sleep(1);
+    // after one expiry anywhere, no block waits again: the launch drains
+    if (wall_clock64() - t0 > P.wait_ticks ||
+        (__hip_atomic_load(P.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & 8)) {
+      atomicOr(P.err, 8);
+      break;
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the partials are visible now
+  for (int k = 0; k < nv; ++k) v[k] = 0.0;
+  for (int q = 0; q < G; ++q) {
+    const double* src = base + (size_t)q * kPartVals;
+    for (int k = 0; k < nv; ++k) v[k] += src[k];
+  }
+  ++epoch;
+}
+
+// P.nsweep Metropolis sweeps of one replica per group of G blocks.
 template <bool MIXED>
 __global__ void __launch_bounds__(512) k_sweep(SamplerParams P) {
   extern __shared__ double smem[];
-  const DevSys& S = *P.sys;
-  const int r = P.rep0 + blockIdx.x;
+  // A local copy of the system description for the inlined trial loop (its
+  // fields stay in scalar registers instead of being re-read from memory
+  // that a reduction's device-scope acquire has invalidated); the
+  // out-of-line full evaluation gets the global copy, so the address of no
+  // local or kernel argument escapes.
+  const DevSys Sv = *P.sys;
+  const DevSys& S = Sv;
+  const DevSys& Sg = *P.sys;
+  const int G = P.G;
+  int r, g;
+  replica_of_block(P, blockIdx.x, r, g);
   const int tid = threadIdx.x, nwaves = blockDim.x / kWave, wave = tid / kWave, lane = tid & (kWave - 1);
   const int nvt = P.nvt, np = S.np, nat = S.nat, nh = S.nheavy;
-  SamplerLds Q = carve_sampler_lds(smem, S, nvt, nwaves);
+  SamplerLds Q = carve_sampler_lds(smem, S, nvt, nwaves, P.bits_lds != 0);
   Work wk;
   EnergyLds& L = Q.L;
   double* gvl = P.vlvr + (size_t)r * nvt;
   double* gx = P.xyz + (size_t)r * 3 * np;
   int32_t* gnf = P.nfree + (size_t)r * nh;
   uint32_t* gmt = P.mt + (size_t)P.rng_of[r] * 625;
+  uint32_t epoch = 0;  // reductions done (thread 0)
 
   energy_stage_static(S, L);
+  stage_variables(P, S, Q.V);
+  const VarLds& V = Q.V;
   for (int k = tid; k < nvt; k += blockDim.x) Q.vl[k] = gvl[k];
   for (int k = tid; k < 625; k += blockDim.x) Q.mt[k] = gmt[k];
   const bool valid = P.valid[r] != 0;
@@ -275,13 +467,17 @@ __global__ void __launch_bounds__(512) k_sweep(SamplerParams P) {
   }
   __syncthreads();
   // sc: [0] eol, [1] acz, [2] trial value, [3] dphi, [4] accept, [5] full-trial flag,
-  //     [6] jv, [7] drift flag, [8..10] axis point, [11..13] axis unit, [14] beta, [15] old value
+  //     [6] jv, [7] drift flag, [8..10] axis point, [11..13] axis unit, [15] old value,
+  //     [16] trial energy, [24..28] incremental components (verify), [32..40] phase clocks
   if (!valid) {
-    EnergyOut o;
-    conformation_energy<MIXED>(S, Q.vl, L, Q.nf, o, wk);
+    ReplicaEnergy re = replica_energy<MIXED>(Sg, Q.vl, L, Q.nf, true, g, G);
+    if (g != 0) re.w.builds = 0.0;
+    add_work(wk, re.w);
     if (tid == 0) {
-      Q.sc[0] = o.el + o.vw + o.hb + o.tor + o.sol;
-      if (o.err) atomicOr(P.err, 2);
+      double v[2] = {re.o.el + re.o.vw + re.o.hb + re.o.tor + re.o.sol, (double)re.o.err};
+      group_sum(P, r, g, epoch, v, 2);
+      Q.sc[0] = v[0];
+      if (v[1] != 0.0) atomicOr(P.err, 2);
     }
   } else if (tid == 0) {
     Q.sc[0] = P.eol[r];
@@ -292,20 +488,23 @@ __global__ void __launch_bounds__(512) k_sweep(SamplerParams P) {
   }
   __syncthreads();
   const double c4 = 4.0 * kPi / (double)(S.npnt > 0 ? S.npnt : 1);
-  double ph[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  uint64_t tp = wall_clock64();
+  double* ph = Q.sc + 32;  // phase clocks (thread 0): [0, kNPhase) sums, [kNPhase] last stamp
+  if (tid == 0) {
+    for (int k = 0; k < kNPhase; ++k) ph[k] = 0.0;
+    ph[kNPhase] = (double)wall_clock64();
+  }
   auto lap = [&](int k) {
     if (P.phase && tid == 0) {
-      const uint64_t now = wall_clock64();
-      ph[k] += (double)(now - tp);
-      tp = now;
+      const double now = (double)wall_clock64();
+      ph[k] += now - ph[kNPhase];
+      ph[kNPhase] = now;
     }
   };
 
   for (int sw = 0; sw < P.nsweep; ++sw) {
-    const double beta = P.beta[P.slot_of[r]];
+    if (tid == 0) Q.sc[14] = P.beta[P.slot_of[r]];  // LDS: no global reload inside the loop
     for (int nsw = 0; nsw < P.nvr; ++nsw) {
-      // ---- proposal (leader): metropolis.f:42-66
+      // ---- proposal (leader): metropolis.f:42-66; identical in the G blocks
       if (tid == 0) {
         uint32_t* mti = &Q.mt[624];
         int ivar = nsw;
@@ -313,16 +512,16 @@ __global__ void __launch_bounds__(512) k_sweep(SamplerParams P) {
           ivar = (int)(P.nvr * mt_grnd(Q.mt, mti));
           if (ivar >= P.nvr) ivar = P.nvr - 1;
         }
-        const int jv = P.idvr[ivar];
+        const int jv = V.idvr[ivar];
         const double vold = Q.vl[jv];
-        const double dv = P.axvr[jv] * (mt_grnd(Q.mt, mti) - 0.5);
+        const double dv = V.ax[jv] * (mt_grnd(Q.mt, mti) - 0.5);
         const double vnew = d_addang(vold, dv);
         Q.sc[2] = vnew;
         Q.sc[15] = vold;
         Q.sc[3] = d_difang(vold, vnew);
         Q.sc[6] = (double)jv;
         Q.sc[5] = fabs(Q.sc[0]) >= 1e5 ? 1.0 : 0.0;
-        const int b = S.var_base[jv], a0 = S.var_axis0[jv];
+        const int b = V.base[jv], a0 = V.axis0[jv];
         double ux = L.x[b] - L.x[a0], uy = L.y[b] - L.y[a0], uz = L.z[b] - L.z[a0];
         double un = rsqrt_f64(ux * ux + uy * uy + uz * uz);
         Q.sc[8] = L.x[b]; Q.sc[9] = L.y[b]; Q.sc[10] = L.z[b];
@@ -331,20 +530,21 @@ __global__ void __launch_bounds__(512) k_sweep(SamplerParams P) {
       __syncthreads();
       lap(kPhPropose);
       const int jv = (int)Q.sc[6];
-      const int m1 = S.var_m1[jv], m2 = S.var_m2[jv];
+      const int m1 = V.m1[jv], m2 = V.m2[jv];
       const bool full = Q.sc[5] != 0.0;
-      EnergyOut vref;  // verify mode: full energy before the move (thread 0)
+      EnergyOut vref;  // verify mode (G = 1): full energy before the move (thread 0)
       if (P.verify && !full) {
-        energy_from_coords<MIXED>(S, Q.vl, L, nullptr, vref, wk);
+        const ReplicaEnergy re = replica_energy<MIXED>(Sg, Q.vl, L, nullptr, false, 0, 1);
+        vref = re.o;
         __syncthreads();
       }
       // ---- rigid rotation of the moved atoms about the bond (Rodrigues)
       {
         double sn, cs;
-        sincos(S.var_sign[jv] * Q.sc[3], &sn, &cs);
+        sincos(V.sign[jv] * Q.sc[3], &sn, &cs);
         const double px = Q.sc[8], py = Q.sc[9], pz = Q.sc[10];
         const double kx = Q.sc[11], ky = Q.sc[12], kz = Q.sc[13];
-        if (tid == 0) wk.moved += m2 - m1 + 1;
+        if (tid == 0 && g == 0) wk.moved += m2 - m1 + 1;
         for (int a = m1 + tid; a <= m2; a += blockDim.x) {
           const double vx = L.x[a] - px, vy = L.y[a] - py, vz = L.z[a] - pz;
           Q.bx[a - m1] = L.x[a];
@@ -364,31 +564,37 @@ __global__ void __launch_bounds__(512) k_sweep(SamplerParams P) {
       lap(kPhRotate);
       double enew;
       if (!full) {
-        // ---- incremental energy change
+        // ---- incremental energy change; block g takes pair items and
+        //      heavy atoms k with k % G == g
         PairE d = {0.0, 0.0, 0.0};
-        pair_delta(S, Q, m1, m2, d, wk);
-        if (S.itysol > 0) mark_affected(S, Q, m1, m2);
+        pair_delta(S, Q, m1, m2, g + G * wave, G * nwaves, d, wk);
+        if (S.itysol > 0) mark_affected(S, Q, m1, m2, g, G);
         double dt = 0.0;
-        if (tid == 0) {
-          const double e0 = S.var_e0[jv];
+        if (tid == 0 && g == 0) {
+          const double e0 = V.e0[jv];
           if (e0 != 0.0) {
-            const int ia = S.var_atom[jv];
-            const double tn = atom_torsion(S, Q.vl, ia);
+            const double tn = var_torsion(V, Q.vl, jv);
             Q.vl[jv] = Q.sc[15];
-            const double to = atom_torsion(S, Q.vl, ia);
+            const double to = var_torsion(V, Q.vl, jv);
             Q.vl[jv] = Q.sc[2];
-            dt = e0 * (1.0 + S.var_sg[jv] * cos(tn * S.var_rn[jv])) - e0 * (1.0 + S.var_sg[jv] * cos(to * S.var_rn[jv]));
+            dt = e0 * (1.0 + V.sg[jv] * cos(tn * V.rn[jv])) - e0 * (1.0 + V.sg[jv] * cos(to * V.rn[jv]));
           }
         }
         __syncthreads();
+        if (S.itysol > 0) {
+          const int na = wave == 0 ? list_affected(S, Q, g, G) : 0;
+          if (tid == 0) Q.sc[29] = (double)na;
+          __syncthreads();
+        }
         lap(kPhPairs);
         double dsol = 0.0;
         int err = 0;
         if (S.itysol > 0) {
-          double* nb = L.fr + (size_t)wave * kMaxNb * 4;
-          for (int h = wave; h < nh; h += nwaves) {
-            if (!Q.aff[h]) continue;
-            const int i = S.heavy[h];
+          double* nb = L.fr + (size_t)wave * kNbDoubles;
+          const int na = (int)Q.sc[29];
+          for (int idx = wave; idx < na; idx += nwaves) {
+            const int h = Q.alist[idx];
+            const int i = L.hv[h];
             int nfree = solvent_atom(S, L, nb, i, wk);
             if (nfree < 0) {
               err = 1;
@@ -396,8 +602,8 @@ __global__ void __launch_bounds__(512) k_sweep(SamplerParams P) {
             }
             if (lane == 0) {
               Q.nf2[h] = nfree;
-              const double trad = S.rvdw[i];
-              dsol += S.sigma[i] * (c4 * trad * trad * (double)(nfree - Q.nf[h]));
+              const double trad = L.rv[i];
+              dsol += L.sg[i] * (c4 * trad * trad * (double)(nfree - Q.nf[h]));
             }
           }
         }
@@ -410,38 +616,38 @@ __global__ void __launch_bounds__(512) k_sweep(SamplerParams P) {
           L.red[8 * wave + 6] = (double)errw;
         }
         __syncthreads();
+        lap(kPhSolvent);
         if (tid == 0) {
-          double s = dt, comp[4] = {0, 0, 0, 0};
-          int e = 0;
+          double tot[6] = {0, 0, 0, 0, dt, 0};  // el, vw, hb, sol, torsion, error
           for (int w = 0; w < nwaves; ++w) {
-            for (int k = 0; k < 4; ++k) {
-              s += L.red[8 * w + k];
-              comp[k] += L.red[8 * w + k];
-            }
-            e |= L.red[8 * w + 6] != 0.0;
+            for (int k = 0; k < 4; ++k) tot[k] += L.red[8 * w + k];
+            tot[5] += L.red[8 * w + 6];
           }
-          if (e) atomicOr(P.err, 2);
-          Q.sc[16] = Q.sc[0] + s;
-          for (int k = 0; k < 4; ++k) Q.sc[24 + k] = comp[k];  // el, vw, hb, sol deltas (verify mode)
-          Q.sc[28] = dt;
+          group_sum(P, r, g, epoch, tot, 6);
+          if (tot[5] != 0.0) atomicOr(P.err, 2);
+          Q.sc[16] = Q.sc[0] + (tot[4] + tot[0] + tot[1] + tot[2] + tot[3]);
+          for (int k = 0; k < 4; ++k) Q.sc[24 + k] = tot[k];  // el, vw, hb, sol deltas (verify mode)
+          Q.sc[28] = tot[4];
         }
         if (P.phase) __syncthreads();  // uniform: timing only
-        lap(kPhSolvent);
+        lap(kPhReduce);
       } else {
         // ---- full recomputation from the trial coordinates
-        EnergyOut o;
-        energy_from_coords<MIXED>(S, Q.vl, L, Q.nf2, o, wk);
+        const ReplicaEnergy re = replica_energy<MIXED>(Sg, Q.vl, L, Q.nf2, false, g, G);
+        add_work(wk, re.w);
         if (tid == 0) {
-          Q.sc[16] = o.el + o.vw + o.hb + o.tor + o.sol;
-          if (o.err) atomicOr(P.err, 2);
+          double v[2] = {re.o.el + re.o.vw + re.o.hb + re.o.tor + re.o.sol, (double)re.o.err};
+          group_sum(P, r, g, epoch, v, 2);
+          Q.sc[16] = v[0];
+          if (v[1] != 0.0) atomicOr(P.err, 2);
           ph[kPhNFull] += 1.0;
         }
         if (P.phase) __syncthreads();
         lap(kPhFull);
       }
       if (P.verify && !full) {  // compare each incremental component with a full recomputation
-        EnergyOut vnew;
-        energy_from_coords<MIXED>(S, Q.vl, L, nullptr, vnew, wk);
+        const ReplicaEnergy rn = replica_energy<MIXED>(Sg, Q.vl, L, nullptr, false, 0, 1);
+        const EnergyOut& vnew = rn.o;
         if (tid == 0 && P.dbg[16 * r] == 0.0) {
           const double fd[5] = {vnew.el - vref.el, vnew.vw - vref.vw, vnew.hb - vref.hb, vnew.tor - vref.tor,
                                 vnew.sol - vref.sol};
@@ -464,10 +670,11 @@ __global__ void __launch_bounds__(512) k_sweep(SamplerParams P) {
         }
         __syncthreads();
       }
-      // ---- acceptance (leader): metropolis.f:72-87
+      // ---- acceptance (leader): metropolis.f:72-87; identical in the G blocks
       if (tid == 0) {
         enew = Q.sc[16];
         const double eol = Q.sc[0];
+        const double beta = Q.sc[14];
         double delta = weight_of(P, beta, enew) - weight_of(P, beta, eol);
         bool acc;
         if (delta <= 0.0) {
@@ -478,7 +685,7 @@ __global__ void __launch_bounds__(512) k_sweep(SamplerParams P) {
           acc = exp(-delta) >= rd;
         }
         Q.sc[4] = acc ? 1.0 : 0.0;
-        if (P.trace && r == P.trace_rep && *P.trace_n < P.trace_cap) {
+        if (P.trace && g == 0 && r == P.trace_rep && *P.trace_n < P.trace_cap) {
           double* t = P.trace + 4 * (size_t)(*P.trace_n);
           t[0] = jv;
           t[1] = enew;
@@ -510,47 +717,56 @@ __global__ void __launch_bounds__(512) k_sweep(SamplerParams P) {
       lap(kPhAccept);
     }
     // ---- sweep end: rebuild from the variables and check (metropolis.f:172-179)
-    EnergyOut o;
-    conformation_energy<MIXED>(S, Q.vl, L, Q.nf, o, wk);
+    ReplicaEnergy re = replica_energy<MIXED>(Sg, Q.vl, L, Q.nf, true, g, G);
+    if (g != 0) re.w.builds = 0.0;
+    add_work(wk, re.w);
     if (tid == 0) {
-      const double enw = o.el + o.vw + o.hb + o.tor + o.sol;
+      double v[2] = {re.o.el + re.o.vw + re.o.hb + re.o.tor + re.o.sol, (double)re.o.err};
+      group_sum(P, r, g, epoch, v, 2);
+      const double enw = v[0];
       const double eol = Q.sc[0];
       if (fabs(eol - enw) > 0.000001 && eol < 100000.0) {
         atomicOr(P.err, 1);
-        if (P.dbg[16 * r + 14] == 0.0) {  // first drift of this replica
+        if (g == 0 && P.dbg[16 * r + 14] == 0.0) {  // first drift of this replica
           P.dbg[16 * r + 14] = eol;
           P.dbg[16 * r + 15] = enw;
         }
       }
-      if (o.err) atomicOr(P.err, 2);
+      if (v[1] != 0.0) atomicOr(P.err, 2);
       Q.sc[0] = enw;
-      const int slot = P.slot_of[r];
-      if (P.traj) P.traj[(size_t)(P.traj_base + sw) * P.nrep + slot] = enw;
-      if (P.hist) {
-        double v = enw / P.ebin + copysign(0.5, enw);
-        v = fmin(fmax(v, (double)P.kmin), (double)P.kmax);
-        atomicAdd(&P.hist[(int)v - P.kmin], 1);
+      if (g == 0) {
+        const int slot = P.slot_of[r];
+        if (P.traj) P.traj[(size_t)(P.traj_base + sw) * P.nrep + slot] = enw;
+        if (P.hist) {
+          double hv = enw / P.ebin + copysign(0.5, enw);
+          hv = fmin(fmax(hv, (double)P.kmin), (double)P.kmax);
+          atomicAdd(&P.hist[(int)hv - P.kmin], 1);
+        }
       }
     }
     __syncthreads();
     lap(kPhSweepEnd);
   }
-  if (P.phase && tid == 0)
-    for (int k = 0; k < 8; ++k) P.phase[8 * (size_t)r + k] += ph[k];
-  // ---- write the replica state back
-  for (int k = tid; k < nvt; k += blockDim.x) gvl[k] = Q.vl[k];
-  for (int k = tid; k < 625; k += blockDim.x) gmt[k] = Q.mt[k];
-  for (int a = tid; a < nat; a += blockDim.x) {
-    gx[a] = L.x[a];
-    gx[np + a] = L.y[a];
-    gx[2 * np + a] = L.z[a];
+  if (P.phase && tid == 0 && g == 0)
+    for (int k = 0; k < kNPhase; ++k) P.phase[kNPhase * (size_t)r + k] += ph[k];
+  // ---- write the replica state back (block 0 of the group; each block its
+  //      own free-point counts)
+  if (g == 0) {
+    for (int k = tid; k < nvt; k += blockDim.x) gvl[k] = Q.vl[k];
+    for (int k = tid; k < 625; k += blockDim.x) gmt[k] = Q.mt[k];
+    for (int a = tid; a < nat; a += blockDim.x) {
+      gx[a] = L.x[a];
+      gx[np + a] = L.y[a];
+      gx[2 * np + a] = L.z[a];
+    }
+    if (tid == 0) {
+      P.eol[r] = Q.sc[0];
+      P.acc[r] = Q.sc[1];
+      P.valid[r] = 1;
+    }
   }
-  for (int h = tid; h < nh; h += blockDim.x) gnf[h] = Q.nf[h];
-  if (tid == 0) {
-    P.eol[r] = Q.sc[0];
-    P.acc[r] = Q.sc[1];
-    P.valid[r] = 1;
-  }
+  for (int h = tid; h < nh; h += blockDim.x)
+    if (h % G == g) gnf[h] = Q.nf[h];
   if (P.work) {
     double v[4] = {wk.pairs, wk.contacts, wk.builds, wk.moved};
     for (int k = 0; k < 4; ++k) v[k] = wave_sum(v[k]);
@@ -562,7 +778,7 @@ __global__ void __launch_bounds__(512) k_sweep(SamplerParams P) {
       for (int k = 0; k < 4; ++k) {
         double t = 0.0;
         for (int w = 0; w < nwaves; ++w) t += L.red[8 * w + k];
-        P.work[4 * r + k] += t;
+        atomicAdd(&P.work[4 * r + k], t);
       }
   }
 }

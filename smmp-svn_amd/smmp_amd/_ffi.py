@@ -35,7 +35,8 @@ class Weight(C.Structure):
 class SamplerCfg(C.Structure):
     _fields_ = [("nrep", C.c_int32), ("nsweep", C.c_int32), ("mode", C.c_int32), ("precision", C.c_int32),
                 ("temps", C.POINTER(C.c_double)), ("seeds", C.POINTER(C.c_uint32)),
-                ("w", C.POINTER(Weight)), ("resync_every", C.c_int32), ("verify", C.c_int32)]
+                ("w", C.POINTER(Weight)), ("resync_every", C.c_int32), ("verify", C.c_int32),
+                ("groups", C.c_int32)]
 
 
 def _load():
@@ -179,17 +180,17 @@ class Context:
         full[:len(vl)] = vl
         self.check(lib.smmp_setvar(self.h, nml, dptr(full)))
 
-    PHASES = ("propose", "rotate", "pairs", "solvent", "accept", "full", "sweep_end")
+    PHASES = ("propose", "rotate", "pairs", "solvent", "reduce", "accept", "full", "sweep_end")
 
     def profile(self, enable=True) -> None:
         """enable: False/0 off, True/1 kernel timing and work, 2 also sweep phases."""
         self.check(lib.smmp_profile(self.h, int(enable)))
 
     def profile_phases(self) -> dict:
-        out = np.zeros(8)
+        out = np.zeros(9)
         self.check(lib.smmp_profile_phases(self.h, dptr(out)))
         d = {k: out[i] for i, k in enumerate(self.PHASES)}
-        d["full_trials"] = out[7]
+        d["full_trials"] = out[8]
         return d
 
     def profile_read(self, kernel: int) -> dict:
@@ -225,7 +226,8 @@ class Context:
         return eol, ipoi, traj[:nswp * R].reshape(nswp, R)
 
     def sample(self, temps, seeds, nsweep: int, vlvr, mode: int = 0, weight: Weight | None = None,
-               precision: int = 0, with_traj: bool = True, allow_drift: bool = False, verify: bool = False):
+               precision: int = 0, with_traj: bool = True, allow_drift: bool = False, verify: bool = False,
+               groups: int = 0):
         """Independent-stream replica sampler (smmp_sample)."""
         temps = np.ascontiguousarray(temps, dtype=np.float64)
         R = len(temps)
@@ -243,7 +245,7 @@ class Context:
         cfg = SamplerCfg(nrep=R, nsweep=nsweep, mode=mode, precision=precision,
                          temps=dptr(temps), seeds=seeds.ctypes.data_as(C.POINTER(C.c_uint32)),
                          w=C.pointer(weight) if weight is not None else None, resync_every=1,
-                         verify=1 if verify else 0)
+                         verify=1 if verify else 0, groups=groups)
         rc = lib.smmp_sample(self.h, C.byref(cfg), dptr(vl), dptr(eol), dptr(acc), iptr(ipoi),
                              iptr(hist) if weight is not None else None,
                              dptr(traj) if traj is not None else None)

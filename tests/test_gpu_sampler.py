@@ -212,3 +212,22 @@ def test_energy_histogram_matches_oracle_sampler_ks():
     p = ks_2samp(gpu, cpu).pvalue
     say(f"KS p = {p:.3g}, gpu mean {gpu.mean():.3f}, cpu mean {cpu.mean():.3f}")
     assert p > 1e-3
+
+
+@pytest.mark.parametrize("groups", [2, 3, 4, 8])
+def test_group_split_matches_single_workgroup(groups):
+    """G workgroups per replica (device reductions every trial) take the same
+    decisions as one workgroup; energies agree to rounding."""
+    c = make_ctx(tag="c2_20mer", itysol=1)
+    R = 8
+    temps = np.geomspace(250.0, 1000.0, R)
+    seeds = np.arange(1, R + 2, dtype=np.uint32)
+    v0 = np.tile(c.vlvr(), (R, 1))
+    v0[:, c.idvr - 1] = np.pi + np.random.default_rng(5).normal(0.0, 0.5, size=(R, c.nvr))
+    say(f"smmp_sample PT c2_20mer R=8 6 sweeps groups=1 vs {groups}")
+    a = c.sample(temps, seeds, 6, v0, mode=1, groups=1)
+    b = c.sample(temps, seeds, 6, v0, mode=1, groups=groups)
+    assert np.array_equal(a["acc"], b["acc"])
+    assert np.array_equal(a["ipoi"], b["ipoi"])
+    assert np.abs(a["traj"] - b["traj"]).max() <= 1e-8 * max(1.0, np.abs(a["traj"]).max())
+    assert np.abs(a["vlvr"] - b["vlvr"]).max() < 1e-12
