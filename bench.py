@@ -152,7 +152,7 @@ def main():
     if args.phases:
         ph = c.profile_phases()
         log("phase us per trial: " + ", ".join(f"{k} {ph[k] * 1e3 / ntrial:.2f}" for k in _ffi.Context.PHASES)
-            + f"; full-path trials {ph['full_trials']:.0f}")
+            + f"; full-path trials {ph['full_trials']:.0f}; core clock {ph['core_mhz']:.0f} MHz")
     c.profile(False)
     value = NREP * args.steps / dt
     npnt = int(c.view("pnt.npnt")[0])

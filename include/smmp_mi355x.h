@@ -143,11 +143,12 @@ int smmp_profile(smmp_ctx *ctx, int enable);
 int smmp_profile_read(smmp_ctx *ctx, int kernel, double *total_ms, int64_t *launches, double *work);
 /* With smmp_profile(ctx, 2) the sweep kernel also times its phases on the
  * device clock (one replica = one workgroup, thread 0 after each barrier).
- * out[9]: milliseconds summed over replicas for proposal, rotation, pair
+ * out[11]: milliseconds summed over replicas for proposal, rotation, pair
  * change (+ contact marking, torsion term), solvent change, the reduction
  * across a replica's workgroups, acceptance and commit, full-recomputation
  * trials, sweep-end rebuild; out[8] = number of trials that took the
- * full-recomputation path (|E| >= 1e5). */
+ * full-recomputation path (|E| >= 1e5); out[9] = core clock cycles and
+ * out[10] = milliseconds of the whole launches (their ratio is the clock). */
 int smmp_profile_phases(smmp_ctx *ctx, double *out);
 
 #ifdef __cplusplus

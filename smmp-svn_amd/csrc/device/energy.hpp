@@ -233,6 +233,7 @@ __device__ __forceinline__ void tile_coords(int t, int nt, int& bi, int& bj) {
 
 struct PairSums {
   double el, vw, hb;
+  double mag;  // mixed mode: sum of |pair term| (scale of the FP32 rounding), else 0
 };
 
 // Pair terms over the tiles this wave owns (FP64): tiles t0, t0 + stride, ...
@@ -290,14 +291,14 @@ __device__ PairSums pair_terms_f64(const DevSys& S, const EnergyLds& L, int t0, 
       }
     }
   }
-  return {el, vw, hb};
+  return {el, vw, hb, 0.0};
 }
 
 // Mixed precision: FP32 terms, FP32 sums per tile, FP64 accumulation.
 __device__ PairSums pair_terms_mixed(const DevSys& S, const EnergyLds& L, int t0, int stride, Work& wk) {
   const int lane = threadIdx.x & (kWave - 1);
   const int nat = S.nat, nt = (nat + 63) / 64, ntile = nt * (nt + 1) / 2, words = S.words;
-  double el = 0.0, vw = 0.0, hb = 0.0;
+  double el = 0.0, vw = 0.0, hb = 0.0, mag = 0.0;
   const float conv = (float)S.conv;
   for (int t = t0; t < ntile; t += stride) {
     int bi, bj;
@@ -320,7 +321,7 @@ __device__ PairSums pair_terms_mixed(const DevSys& S, const EnergyLds& L, int t0
       w0 &= keep;
       w1 &= keep;
     }
-    float fel = 0.f, fvw = 0.f, fhb = 0.f;
+    float fel = 0.f, fvw = 0.f, fhb = 0.f, fmag = 0.f;
     const int kmax = min(64, nat - bj * 64);
     wk.pairs += __popcll(w0 & (kmax == 64 ? ~0ull : ((1ull << kmax) - 1ull)));
     for (int k = 0; k < kmax; ++k) {
@@ -343,18 +344,22 @@ __device__ PairSums pair_terms_mixed(const DevSys& S, const EnergyLds& L, int t0
           const float sr = (float)S.slp * (r2 * ri), plt = (float)S.plt;
           ep_inv = 1.f / (plt - (sr * sr + 2.f * sr + 2.f) * (plt - 1.f) * expf(-sr) / 2.f);
         }
-        fel += qi * L.qf[j] * ri * ep_inv;
+        const float tel = qi * L.qf[j] * ri * ep_inv;
+        fel += tel;
+        const float tlj = A * (r6i * r6i) - Cc * (L.pHB[tt] != 0.0 ? r6i * r2i * r2i : r6i);
         if (L.pHB[tt] != 0.0)
-          fhb += A * (r6i * r6i) - Cc * (r6i * r2i * r2i);
+          fhb += tlj;
         else
-          fvw += A * (r6i * r6i) - Cc * r6i;
+          fvw += tlj;
+        fmag += fabsf(tel) + fabsf(A * (r6i * r6i)) + fabsf(Cc * r6i);
       }
     }
     el += (double)fel;
     vw += (double)fvw;
     hb += (double)fhb;
+    mag += (double)fmag;
   }
-  return {el, vw, hb};
+  return {el, vw, hb, mag};
 }
 
 // enyshe.f:73-79 over all variables; lanes over variables.
@@ -370,7 +375,7 @@ __device__ double torsion_terms(const DevSys& S, const double* vl) {
   return e;
 }
 
-constexpr int kMaxPntPerLane = 4;  // tessellations of up to 256 points
+constexpr int kMaxPntPerLane = 2;  // tessellations of up to 128 points (tes.dat: 122)
 
 // Point tests of one chunk of neighbours: lane l owns points l, l+64, ...
 // (NS slots).  All slots advance together over the neighbours (padded to a
@@ -427,7 +432,7 @@ __device__ int solvent_atom(const DevSys& S, const EnergyLds& L, double* nb, int
 #pragma clang fp contract(off)
   const int lane = threadIdx.x & (kWave - 1);
   const int npnt = S.npnt, nh = S.nheavy;
-  if (npnt > kWave * kMaxPntPerLane) return -1;
+  if (npnt > kWave * kMaxPntPerLane) return -1;  // the host rejects such tessellations first
   int32_t* cand = reinterpret_cast<int32_t*>(nb + kMaxNb * 4);
   const double xi = L.x[i], yi = L.y[i], zi = L.z[i];
   const double trad = L.rv[i];
@@ -493,12 +498,7 @@ __device__ int solvent_atom(const DevSys& S, const EnergyLds& L, double* nb, int
     }
     if (lane == 0) wk.contacts += cnt;
     wave_lds_sync();
-    if (npnt <= kWave)
-      solvent_points<1>(L, npnt, nb, padded, occ, lane);
-    else if (npnt <= 2 * kWave)
-      solvent_points<2>(L, npnt, nb, padded, occ, lane);
-    else
-      solvent_points<4>(L, npnt, nb, padded, occ, lane);
+    solvent_points<kMaxPntPerLane>(L, npnt, nb, padded, occ, lane);
     wave_lds_sync();
   }
   int nfree = 0;
@@ -509,6 +509,7 @@ __device__ int solvent_atom(const DevSys& S, const EnergyLds& L, double* nb, int
 
 struct EnergyOut {
   double el, vw, hb, tor, sol, asa;
+  double mag;  // mixed mode: sum of |pair term|; 0 in FP64 mode
   int err;
 };
 
@@ -557,28 +558,29 @@ __device__ void energy_from_coords(const DevSys& S, const double* vl, EnergyLds&
       }
     }
   }
-  double v[6] = {ps.el, ps.vw, ps.hb, tor, sol, asa};
+  double v[7] = {ps.el, ps.vw, ps.hb, tor, sol, asa, ps.mag};
 #pragma unroll
-  for (int k = 0; k < 6; ++k) v[k] = wave_sum(v[k]);
+  for (int k = 0; k < 7; ++k) v[k] = wave_sum(v[k]);
   const int errw = __any(err);
   __syncthreads();  // L.red may still be read by a previous reduction
   if (lane == 0) {
 #pragma unroll
-    for (int k = 0; k < 6; ++k) L.red[8 * wave + k] = v[k];
-    L.red[8 * wave + 6] = (double)errw;
+    for (int k = 0; k < 7; ++k) L.red[8 * wave + k] = v[k];
+    L.red[8 * wave + 7] = (double)errw;
   }
   __syncthreads();
   if (tid == 0) {
-    double s[7] = {0, 0, 0, 0, 0, 0, 0};
+    double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int w = 0; w < nwaves; ++w)
-      for (int k = 0; k < 7; ++k) s[k] += L.red[8 * w + k];
+      for (int k = 0; k < 8; ++k) s[k] += L.red[8 * w + k];
     out.el = s[0];
     out.vw = s[1];
     out.hb = s[2];
     out.tor = s[3];
     out.sol = s[4];
     out.asa = s[5];
-    out.err = s[6] != 0.0;
+    out.mag = s[6];
+    out.err = s[7] != 0.0;
   }
 }
 

@@ -105,7 +105,7 @@ constexpr int kPartVals = 8;
 
 // Phases of a sweep timed by thread 0 when P.phase is set.
 enum PhaseId { kPhPropose = 0, kPhRotate, kPhPairs, kPhSolvent, kPhReduce, kPhAccept, kPhFull, kPhSweepEnd,
-               kPhNFull, kNPhase };
+               kPhNFull, kPhCoreCycles, kPhRealTicks, kNPhase };
 
 // Extra LDS of the sampler beyond the energy carve-up.
 __host__ __device__ inline size_t sampler_lds_bytes(int np, int npnt, int nheavy, int nvt, int nwaves, size_t nbits) {
@@ -347,14 +347,14 @@ __device__ int list_affected(const DevSys& S, const SamplerLds& Q, int g, int G)
 }
 
 // Full energy of the replica in LDS: from the coordinates, or (rebuild) from
-// the variables through the chain build.  Out of line, so that the trial
-// loop's register budget does not carry the full evaluation.
+// the variables through the chain build.  Inlined: out of line, the call
+// ABI takes the LDS pointers as flat pointers and doubles the registers.
 struct ReplicaEnergy {
   EnergyOut o;
   Work w;
 };
 template <bool MIXED>
-__device__ __attribute__((noinline)) ReplicaEnergy replica_energy(const DevSys& S, const double* vl, EnergyLds L,
+__device__ __forceinline__ ReplicaEnergy replica_energy(const DevSys& S, const double* vl, EnergyLds L,
                                                                   int32_t* nfree_out, bool rebuild, int part,
                                                                   int nparts) {
   ReplicaEnergy r;
@@ -429,14 +429,8 @@ __device__ void group_sum(const SamplerParams& P, int r, int g, uint32_t& epoch,
 template <bool MIXED>
 __global__ void __launch_bounds__(512) k_sweep(SamplerParams P) {
   extern __shared__ double smem[];
-  // A local copy of the system description for the inlined trial loop (its
-  // fields stay in scalar registers instead of being re-read from memory
-  // that a reduction's device-scope acquire has invalidated); the
-  // out-of-line full evaluation gets the global copy, so the address of no
-  // local or kernel argument escapes.
-  const DevSys Sv = *P.sys;
-  const DevSys& S = Sv;
-  const DevSys& Sg = *P.sys;
+  const DevSys& S = *P.sys;
+  const DevSys& Sg = S;
   const int G = P.G;
   int r, g;
   replica_of_block(P, blockIdx.x, r, g);
@@ -489,9 +483,10 @@ __global__ void __launch_bounds__(512) k_sweep(SamplerParams P) {
   __syncthreads();
   const double c4 = 4.0 * kPi / (double)(S.npnt > 0 ? S.npnt : 1);
   double* ph = Q.sc + 32;  // phase clocks (thread 0): [0, kNPhase) sums, [kNPhase] last stamp
+  const uint64_t core0 = clock64(), real0 = wall_clock64();
   if (tid == 0) {
     for (int k = 0; k < kNPhase; ++k) ph[k] = 0.0;
-    ph[kNPhase] = (double)wall_clock64();
+    ph[kNPhase] = (double)real0;
   }
   auto lap = [&](int k) {
     if (P.phase && tid == 0) {
@@ -721,11 +716,15 @@ __global__ void __launch_bounds__(512) k_sweep(SamplerParams P) {
     if (g != 0) re.w.builds = 0.0;
     add_work(wk, re.w);
     if (tid == 0) {
-      double v[2] = {re.o.el + re.o.vw + re.o.hb + re.o.tor + re.o.sol, (double)re.o.err};
-      group_sum(P, r, g, epoch, v, 2);
+      double v[3] = {re.o.el + re.o.vw + re.o.hb + re.o.tor + re.o.sol, (double)re.o.err, re.o.mag};
+      group_sum(P, r, g, epoch, v, 3);
       const double enw = v[0];
       const double eol = Q.sc[0];
-      if (fabs(eol - enw) > 0.000001 && eol < 100000.0) {
+      // metropolis.f:172-179.  Mixed mode: the tracked energy starts from the
+      // previous FP32-term recomputation, so two recomputations' rounding
+      // (each within 5e-6 of the summed pair-term magnitudes) are allowed.
+      const double tol = MIXED ? 0.000001 + 1e-5 * v[2] : 0.000001;
+      if (fabs(eol - enw) > tol && eol < 100000.0) {
         atomicOr(P.err, 1);
         if (g == 0 && P.dbg[16 * r + 14] == 0.0) {  // first drift of this replica
           P.dbg[16 * r + 14] = eol;
@@ -747,8 +746,11 @@ __global__ void __launch_bounds__(512) k_sweep(SamplerParams P) {
     __syncthreads();
     lap(kPhSweepEnd);
   }
-  if (P.phase && tid == 0 && g == 0)
+  if (P.phase && tid == 0 && g == 0) {
+    ph[kPhCoreCycles] = (double)(clock64() - core0);  // core clock vs the constant wall clock
+    ph[kPhRealTicks] = (double)(wall_clock64() - real0);
     for (int k = 0; k < kNPhase; ++k) P.phase[kNPhase * (size_t)r + k] += ph[k];
+  }
   // ---- write the replica state back (block 0 of the group; each block its
   //      own free-point counts)
   if (g == 0) {

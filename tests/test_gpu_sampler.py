@@ -231,3 +231,26 @@ def test_group_split_matches_single_workgroup(groups):
     assert np.array_equal(a["ipoi"], b["ipoi"])
     assert np.abs(a["traj"] - b["traj"]).max() <= 1e-8 * max(1.0, np.abs(a["traj"]).max())
     assert np.abs(a["vlvr"] - b["vlvr"]).max() < 1e-12
+
+
+def test_mixed_precision_sampler_runs_and_tracks_fp64():
+    """precision=1: FP32 pair terms in the full recomputations, FP64 trial
+    changes; the drift check uses the mixed bound.  Final energies against
+    the FP64 oracle at the sampled conformations."""
+    from _oracle import OracleSystem
+    c = make_ctx(tag="c2_20mer", itysol=1)
+    o = OracleSystem(c)
+    R = 8
+    v0 = np.tile(c.vlvr(), (R, 1))
+    v0[:, o.idvr - 1] = np.pi + np.random.default_rng(12).normal(0.0, 0.6, size=(R, o.nvr))
+    say("smmp_sample PT c2_20mer R=8 12 sweeps mixed precision")
+    out = c.sample(np.geomspace(250.0, 1000.0, R), np.arange(1, R + 2, dtype=np.uint32), 12, v0, mode=1,
+                   precision=1)
+    worst = 0.0
+    for r in range(R):
+        ref = o.energy(out["vlvr"][r])
+        scale = abs(ref[2]) + abs(ref[3]) + abs(ref[4])
+        err = abs(out["eol"][r] - ref[0])
+        worst = max(worst, err / (scale + 1.0))
+        assert err <= 2e-5 * scale + 1e-3, (r, out["eol"][r], ref[0])
+    say(f"mixed: worst |E - E_fp64| / (|el|+|vw|+|hb|+1) = {worst:.2e}")
