@@ -153,6 +153,10 @@ def main():
         ph = c.profile_phases()
         log("phase us per trial: " + ", ".join(f"{k} {ph[k] * 1e3 / ntrial:.2f}" for k in _ffi.Context.PHASES)
             + f"; full-path trials {ph['full_trials']:.0f}; core clock {ph['core_mhz']:.0f} MHz")
+        na = max(ph["sol_atoms"], 1.0)
+        log(f"solvent per trial (block 0 of each replica): slowest wave {ph['sol_max_wave'] * 1e3 / ntrial:.2f} us; "
+            f"per atom: gather {ph['sol_gather'] * 1e3 / na:.3f} us, point tests {ph['sol_points'] * 1e3 / na:.3f} us; "
+            f"atoms per trial {na / ntrial:.1f}")
     c.profile(False)
     value = NREP * args.steps / dt
     npnt = int(c.view("pnt.npnt")[0])

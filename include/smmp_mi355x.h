@@ -143,12 +143,15 @@ int smmp_profile(smmp_ctx *ctx, int enable);
 int smmp_profile_read(smmp_ctx *ctx, int kernel, double *total_ms, int64_t *launches, double *work);
 /* With smmp_profile(ctx, 2) the sweep kernel also times its phases on the
  * device clock (one replica = one workgroup, thread 0 after each barrier).
- * out[11]: milliseconds summed over replicas for proposal, rotation, pair
+ * out[16]: milliseconds summed over replicas for proposal, rotation, pair
  * change (+ contact marking, torsion term), solvent change, the reduction
  * across a replica's workgroups, acceptance and commit, full-recomputation
  * trials, sweep-end rebuild; out[8] = number of trials that took the
  * full-recomputation path (|E| >= 1e5); out[9] = core clock cycles and
- * out[10] = milliseconds of the whole launches (their ratio is the clock). */
+ * out[10] = milliseconds of the whole launches (their ratio is the clock);
+ * solvent change of the first workgroup of each replica: out[11] ms of its
+ * slowest wave, out[12]/out[13] ms of neighbour gathering / point tests
+ * summed over waves, out[14] atoms evaluated, out[15] unused. */
 int smmp_profile_phases(smmp_ctx *ctx, double *out);
 
 #ifdef __cplusplus

@@ -7,7 +7,7 @@ namespace smmp {
 
 constexpr int kWave = 64;
 constexpr int kTy = 18;          // ECEPP atom types
-constexpr int kMaxNb = 128;      // solvent neighbours kept per atom
+constexpr int kMaxNb = 64;       // solvent neighbours per chunk of one atom's gather
 constexpr double kPi = 3.141592653589793;
 constexpr double kPi2 = 2.0 * kPi;
 

@@ -12,6 +12,10 @@
 
 namespace smmp {
 
+// A pointer into LDS typed as such, so that loads through it are ds_read
+// even where the same data may live in HBM on another path.
+typedef const uint64_t __attribute__((address_space(3))) lds_u64;
+
 // LDS carve-up for one conformation (doubles unless noted).
 struct EnergyLds {
   double *x, *y, *z, *q;   // np each
@@ -20,7 +24,9 @@ struct EnergyLds {
   double* sp;              // npnt*3
   double* red;             // nwaves*8
   double *rv, *sg;         // solvent radius and sigma per atom (np each)
+  double* hp;              // heavy atoms packed by heavy index: x, y, z, radius (4 * nheavy)
   const uint64_t* bits;    // pair bit rows: an LDS copy, or the system's array in HBM
+  const lds_u64* lbits;    // the LDS copy as an LDS pointer (ds_read), or null
   int8_t* t;               // np bytes
   int16_t* hv;             // heavy-atom list (nheavy)
   float *xf, *yf, *zf, *qf;  // mixed-mode copies (alias fr after the build)
@@ -29,16 +35,17 @@ struct EnergyLds {
 // Number of u64 words of the pair bit rows (two planes per 64-column word).
 __host__ __device__ inline size_t pair_bit_words(int nat, int words) { return (size_t)nat * words * 2; }
 
-// Per-wave solvent scratch: kMaxNb neighbours x 4 doubles, then kMaxNb
-// int32 candidate indices (kMaxNb/2 doubles).
-constexpr int kNbDoubles = kMaxNb * 4 + kMaxNb / 2;
+// Per-wave solvent scratch: kMaxNb neighbours x kNbFloats floats.
+constexpr int kNbFloats = 8;  // per neighbour: 2d (3), 1/M, -t/M, j, 2 unused (see solvent_points)
+constexpr int kNbDoubles = kMaxNb * kNbFloats / 2;
 __host__ __device__ inline int energy_region1(int np, int nwaves) {
   int a = 6 * np, b = nwaves * kNbDoubles;
   return a > b ? a : b;
 }
 // nbits: u64 words of pair bits staged in LDS (0: read from HBM).
 __host__ __device__ inline size_t energy_lds_bytes(int np, int npnt, int nwaves, int nheavy, size_t nbits) {
-  size_t d = 6 * (size_t)np + energy_region1(np, nwaves) + 4 * 324 + 3 * (size_t)npnt + 8 * (size_t)nwaves + nbits;
+  size_t d = 6 * (size_t)np + energy_region1(np, nwaves) + 4 * 324 + 3 * (size_t)npnt + 8 * (size_t)nwaves + nbits +
+             4 * (size_t)nheavy;
   return d * sizeof(double) + (size_t)np + 2 * (size_t)nheavy;
 }
 
@@ -59,11 +66,14 @@ __device__ inline EnergyLds carve_energy_lds(double* base, const DevSys& S, int 
   L.red = p; p += 8 * nwaves;
   L.rv = p; p += np;
   L.sg = p; p += np;
+  L.hp = p; p += 4 * S.nheavy;
   if (bits_lds) {
     L.bits = reinterpret_cast<const uint64_t*>(p);
+    L.lbits = (const lds_u64*)(p);
     p += pair_bit_words(S.nat, S.words);
   } else {
     L.bits = S.bits;
+    L.lbits = nullptr;
   }
   L.t = reinterpret_cast<int8_t*>(p);
   L.hv = reinterpret_cast<int16_t*>(L.t + np);  // np is a multiple of 64
@@ -72,6 +82,11 @@ __device__ inline EnergyLds carve_energy_lds(double* base, const DevSys& S, int 
   L.zf = L.yf + np;
   L.qf = L.zf + np;
   return L;
+}
+
+// Word k of the pair bit rows.
+__device__ __forceinline__ uint64_t pair_bits(const EnergyLds& L, size_t k) {
+  return L.lbits ? L.lbits[k] : L.bits[k];
 }
 
 // Torsion of atom a for the variable values vl (setvar.f:43-57 from the
@@ -250,8 +265,8 @@ __device__ PairSums pair_terms_f64(const DevSys& S, const EnergyLds& L, int t0, 
     double xi = 0, yi = 0, zi = 0, qi = 0;
     int ti = 0;
     if (vi) {
-      w0 = L.bits[((size_t)i * words + bj) * 2 + 0];
-      w1 = L.bits[((size_t)i * words + bj) * 2 + 1];
+      w0 = pair_bits(L, ((size_t)i * words + bj) * 2 + 0);
+      w1 = pair_bits(L, ((size_t)i * words + bj) * 2 + 1);
       xi = L.x[i]; yi = L.y[i]; zi = L.z[i];
       qi = S.conv * L.q[i];
       ti = L.t[i] * kTy;
@@ -310,8 +325,8 @@ __device__ PairSums pair_terms_mixed(const DevSys& S, const EnergyLds& L, int t0
     float qi = 0;
     int ti = 0;
     if (vi) {
-      w0 = L.bits[((size_t)i * words + bj) * 2 + 0];
-      w1 = L.bits[((size_t)i * words + bj) * 2 + 1];
+      w0 = pair_bits(L, ((size_t)i * words + bj) * 2 + 0);
+      w1 = pair_bits(L, ((size_t)i * words + bj) * 2 + 1);
       xi = L.x[i]; yi = L.y[i]; zi = L.z[i];
       qi = conv * L.qf[i];
       ti = L.t[i] * kTy;
@@ -377,133 +392,291 @@ __device__ double torsion_terms(const DevSys& S, const double* vl) {
 
 constexpr int kMaxPntPerLane = 2;  // tessellations of up to 128 points (tes.dat: 122)
 
-// Point tests of one chunk of neighbours: lane l owns points l, l+64, ...
-// (NS slots).  All slots advance together over the neighbours (padded to a
-// multiple of 4 with entries that occlude nothing), and the wave stops early
-// once every point it owns is occluded.  The expressions keep the
-// reference's operation order with contraction off, so every decision
-// equals the FP64 evaluation of enysol.f.
-template <int NS>
-__device__ __forceinline__ void solvent_points(const EnergyLds& L, int npnt, const double* nb, int cnt,
-                                               bool (&occ)[kMaxPntPerLane], int lane) {
-#pragma clang fp contract(off)
-  double px[NS], py[NS], pz[NS];
-#pragma unroll
-  for (int q = 0; q < NS; ++q) {
-    const int p = q * kWave + lane;
-    const bool in = p < npnt;
-    px[q] = in ? L.sp[3 * p + 0] : 0.0;
-    py[q] = in ? L.sp[3 * p + 1] : 0.0;
-    pz[q] = in ? L.sp[3 * p + 2] : 0.0;
-  }
+// Solvent point test.  The reference occludes tessellation point p of heavy
+// atom i by neighbour j when its FP64
+//   sdd = (dx+px)^2 + (dy+py)^2 + (dz+pz)^2   (left to right, no contraction)
+// is <= r2 = (rj/ri)^2, with d = (x_i - x_j)/r_i (enysol.f).  In exact
+// arithmetic |d+p|^2 = |d|^2 + v, v = 2 d.p + |p|^2, so with t = r2 - |d|^2
+// the test is v <= t.  Three levels decide it:
+//  1. FP32 (the point loop; 20 bytes of LDS per neighbour, which is what
+//     bounds this loop):  s = (v - t)/M32, M32 = 24 u32 (r2 + 21|d|^2 + 20)
+//     with u32 = 2^-24, at least twice the FP32 rounding of d, p, |p|^2, v,
+//     t and s for |p| < 2: s <= -1 means the reference occludes, and a point
+//     whose s exceeds 1 for every neighbour is free.
+//  2. FP64 (undecided points only): the same form with M64 = 128 u (r2 +
+//     3|d|^2 + 9), u = 2^-53, d and r2 from a reciprocal of r_i.
+//  3. The reference's own expression with its divisions, for what is left.
+// So every free-point count equals enysol.f's.
+constexpr double kU32 = 3.552713678800501e-15;    // 32 * 2^-53
+constexpr double kU128 = 1.4210854715202004e-14;  // 128 * 2^-53
+constexpr double kF24 = 1.430511474609375e-06;    // 24 * 2^-24
+
+// Minimum of s over the neighbours of one chunk, per owned point (lane l
+// owns points l and l+64).  Stops once every valid point is occluded.
+__device__ __forceinline__ void solvent_points(const float* nb, int cnt, const float (&px)[kMaxPntPerLane],
+                                               const float (&py)[kMaxPntPerLane], const float (&pz)[kMaxPntPerLane],
+                                               const float (&pe)[kMaxPntPerLane], const bool (&done)[kMaxPntPerLane],
+                                               float (&smin)[kMaxPntPerLane]) {
   for (int k0 = 0; k0 < cnt; k0 += 8) {
 #pragma unroll
     for (int u = 0; u < 8; u += 4) {
       if (k0 + u >= cnt) break;
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int k = k0 + u + v;
-        const double nx = nb[4 * k + 0], ny = nb[4 * k + 1], nz = nb[4 * k + 2], r2 = nb[4 * k + 3];
+      for (int v4 = 0; v4 < 4; ++v4) {
+        const float* e = nb + kNbFloats * (k0 + u + v4);
+        const float4 a = *reinterpret_cast<const float4*>(e);
+        const float c = e[4];
 #pragma unroll
-        for (int q = 0; q < NS; ++q) {
-          const double ax = nx + px[q], ay = ny + py[q], az = nz + pz[q];
-          double sdd = ax * ax;
-          sdd = sdd + ay * ay;
-          sdd = sdd + az * az;
-          occ[q] |= sdd <= r2;
+        for (int q = 0; q < kMaxPntPerLane; ++q) {
+          const float v = fmaf(a.x, px[q], fmaf(a.y, py[q], fmaf(a.z, pz[q], pe[q])));
+          smin[q] = fminf(smin[q], fmaf(v, a.w, c));
         }
       }
     }
     bool all = true;
 #pragma unroll
-    for (int q = 0; q < NS; ++q) all = all && occ[q];
+    for (int q = 0; q < kMaxPntPerLane; ++q) all = all && (done[q] || smin[q] <= -1.f);
     if (__all(all)) break;
   }
 }
 
+// Levels 2 and 3 for one undecided point p of this lane over the
+// neighbours of the chunk (rarely run).
+__device__ __forceinline__ bool resolve_point(const EnergyLds& L, const float* nb, int cnt, double xi, double yi,
+                                              double zi, double trad, double rinv, int p) {
+#pragma clang fp contract(off)
+  const double px = L.sp[3 * p + 0], py = L.sp[3 * p + 1], pz = L.sp[3 * p + 2];
+  const double pe = px * px + py * py + pz * pz;
+  for (int k = 0; k < cnt; ++k) {
+    const int j = __float_as_int(nb[kNbFloats * k + 5]);
+    const double ax = xi - L.x[j], ay = yi - L.y[j], az = zi - L.z[j];
+    const double dxa = ax * rinv, dya = ay * rinv, dza = az * rinv;
+    const double dda = dxa * dxa + dya * dya + dza * dza;
+    const double aka = L.rv[j] * rinv;
+    const double r2a = aka * aka;
+    const double w = fma(2.0 * dxa, px, fma(2.0 * dya, py, fma(2.0 * dza, pz, pe))) - (r2a - dda);
+    const double m = kU128 * (r2a + 3.0 * dda + 9.0);
+    if (w <= -m) return true;
+    if (w <= m) {  // the reference's expression
+      const double dx = ax / trad, dy = ay / trad, dz = az / trad;
+      const double akrad = L.rv[j] / trad;
+      const double r2b = akrad * akrad;
+      const double sx = dx + px, sy = dy + py, sz = dz + pz;
+      double sdd = sx * sx;
+      sdd = sdd + sy * sy;
+      sdd = sdd + sz * sz;
+      if (sdd <= r2b) return true;
+    }
+  }
+  return false;
+}
+
+// The reference's contact test (enysol.f) for candidate j of atom i:
+// dd = |d|^2 <= dr = (1 + rj/ri)^2 with exact divisions, decided from the
+// reciprocal form when the two sides are further apart than its rounding.
+__device__ __forceinline__ bool contact_exact(double ax, double ay, double az, double rj, double trad) {
+#pragma clang fp contract(off)
+  const double dx = ax / trad, dy = ay / trad, dz = az / trad;
+  const double dd = dx * dx + dy * dy + dz * dz;
+  double akrad = rj / trad;
+  double dr = 1.0 + akrad;
+  dr = dr * dr;
+  return dd <= dr;
+}
+
+// The packed heavy-atom copy of the current coordinates (all threads; a
+// barrier must follow before solvent_atom reads it).
+__device__ __forceinline__ void fill_heavy_packed(const DevSys& S, const EnergyLds& L) {
+  for (int h = threadIdx.x; h < S.nheavy; h += blockDim.x) {
+    const int j = L.hv[h];
+    L.hp[4 * h + 0] = L.x[j];
+    L.hp[4 * h + 1] = L.y[j];
+    L.hp[4 * h + 2] = L.z[j];
+    L.hp[4 * h + 3] = L.rv[j];
+  }
+}
+
+// Point tests of the contacts gathered so far (padded to a multiple of 4),
+// then the fallback levels for the undecided points.
+__device__ __forceinline__ void solvent_chunk(const EnergyLds& L, float* nb, int cnt, double xi, double yi,
+                                              double zi, double trad, double rinv, const float (&px)[kMaxPntPerLane],
+                                              const float (&py)[kMaxPntPerLane], const float (&pz)[kMaxPntPerLane],
+                                              const float (&pe)[kMaxPntPerLane], bool (&occ)[kMaxPntPerLane]) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int padded = (cnt + 3) & ~3;
+  if (lane < padded - cnt) {  // s = 4: never occludes, never undecided
+    float* e = nb + kNbFloats * (cnt + lane);
+    e[0] = e[1] = e[2] = e[3] = 0.f;
+    e[4] = 4.f;
+    e[5] = __int_as_float(-1);
+  }
+  wave_lds_sync();
+  float smin[kMaxPntPerLane];
+#pragma unroll
+  for (int q = 0; q < kMaxPntPerLane; ++q) smin[q] = 4.f;
+  solvent_points(nb, padded, px, py, pz, pe, occ, smin);
+  bool undecided = false;
+#pragma unroll
+  for (int q = 0; q < kMaxPntPerLane; ++q) {
+    if (!occ[q] && smin[q] <= -1.f) occ[q] = true;
+    undecided = undecided || (!occ[q] && smin[q] <= 1.f);
+  }
+  if (__any(undecided)) {
+#pragma unroll
+    for (int q = 0; q < kMaxPntPerLane; ++q)
+      if (!occ[q] && smin[q] <= 1.f) occ[q] = resolve_point(L, nb, cnt, xi, yi, zi, trad, rinv, q * kWave + lane);
+  }
+  wave_lds_sync();
+}
+
+// Per-lane tessellation points (FP32) of the points this lane owns.
+struct LanePoints {
+  float x[kMaxPntPerLane], y[kMaxPntPerLane], z[kMaxPntPerLane], e[kMaxPntPerLane];
+  bool occ[kMaxPntPerLane];
+};
+__device__ __forceinline__ void lane_points(const EnergyLds& L, int npnt, LanePoints& lp) {
+  const int lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+  for (int q = 0; q < kMaxPntPerLane; ++q) {
+    const int p = q * kWave + lane;
+    const bool in = p < npnt;
+    lp.x[q] = in ? (float)L.sp[3 * p + 0] : 0.f;
+    lp.y[q] = in ? (float)L.sp[3 * p + 1] : 0.f;
+    lp.z[q] = in ? (float)L.sp[3 * p + 2] : 0.f;
+    lp.e[q] = lp.x[q] * lp.x[q] + lp.y[q] * lp.y[q] + lp.z[q] * lp.z[q];
+    lp.occ[q] = !in;
+  }
+}
+
+// Contact test of heavy atom j against atom i: dd <= (1 + rj/ri)^2 (see
+// contact_exact); on a contact, its neighbour entry (see solvent_points).
+struct ContactGeom {
+  double dxa, dya, dza, dda, r2a;
+};
+__device__ __forceinline__ bool contact_of(int i, int j, double xi, double yi, double zi, double trad, double rinv,
+                                           double xj, double yj, double zj, double rj, ContactGeom& cg) {
+#pragma clang fp contract(off)
+  if (j < 0 || j == i) return false;
+  const double ax = xi - xj, ay = yi - yj, az = zi - zj;
+  cg.dxa = ax * rinv;
+  cg.dya = ay * rinv;
+  cg.dza = az * rinv;
+  cg.dda = cg.dxa * cg.dxa + cg.dya * cg.dya + cg.dza * cg.dza;
+  const double aka = rj * rinv;
+  double dra = 1.0 + aka;
+  dra = dra * dra;
+  cg.r2a = aka * aka;
+  const double mc = kU32 * (cg.dda + dra);
+  if (cg.dda <= dra - mc) return true;
+  if (cg.dda > dra + mc) return false;
+  return contact_exact(ax, ay, az, rj, trad);
+}
+__device__ __forceinline__ void write_contact(float* e, int j, const ContactGeom& cg) {
+  const double t = cg.r2a - cg.dda;
+  const float im = (float)__builtin_amdgcn_rcp(kF24 * (cg.r2a + 21.0 * cg.dda + 20.0));  // 1/M32, any rounding
+  e[0] = (float)(2.0 * cg.dxa);
+  e[1] = (float)(2.0 * cg.dya);
+  e[2] = (float)(2.0 * cg.dza);
+  e[3] = im;
+  e[4] = (float)(-t * (double)im);
+  e[5] = __int_as_float(j);
+}
+
+// Atoms with more than kMaxNb contacts (not met in proteins of the usual
+// packing): the contacts are taken in chunks of kMaxNb, all from LDS.  Out
+// of line, with its own LDS carve-up, to keep the common path small.
+__device__ __attribute__((noinline)) int solvent_atom_chunked(const DevSys* sys, int nwaves, int bits_lds, int i) {
+#pragma clang fp contract(off)
+  extern __shared__ double smem[];
+  const DevSys& S = *sys;
+  const EnergyLds L = carve_energy_lds(smem, S, nwaves, bits_lds != 0);
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  float* nb = reinterpret_cast<float*>(L.fr + (size_t)wave * kNbDoubles);
+  const int nh = S.nheavy;
+  const double xi = L.x[i], yi = L.y[i], zi = L.z[i], trad = L.rv[i], rinv = 1.0 / trad;
+  LanePoints lp;
+  lane_points(L, S.npnt, lp);
+  int cnt = 0;
+  for (int h0 = 0; h0 < nh; h0 += kWave) {
+    const int h = h0 + lane;
+    const int j = h < nh ? L.hv[h] : -1;
+    const int jj = j < 0 ? 0 : j;
+    ContactGeom cg;
+    const bool on = contact_of(i, j, xi, yi, zi, trad, rinv, L.x[jj], L.y[jj], L.z[jj], L.rv[jj], cg);
+    const uint64_t m = __ballot(on);
+    const int n = __popcll(m);
+    if (cnt + n > kMaxNb) {
+      solvent_chunk(L, nb, cnt, xi, yi, zi, trad, rinv, lp.x, lp.y, lp.z, lp.e, lp.occ);
+      cnt = 0;
+    }
+    if (on) write_contact(nb + kNbFloats * (cnt + __popcll(m & ((1ull << lane) - 1ull))), j, cg);
+    cnt += n;
+  }
+  solvent_chunk(L, nb, cnt, xi, yi, zi, trad, rinv, lp.x, lp.y, lp.z, lp.e, lp.occ);
+  int nfree = 0;
+#pragma unroll
+  for (int q = 0; q < kMaxPntPerLane; ++q) nfree += __popcll(__ballot(!lp.occ[q]));
+  return nfree;
+}
+
 // enysol.f:192-324: area of heavy atom i from its free tessellation points,
-// by one wave.  Candidates come from a conservative distance filter; the
-// reference's contact test (with its divisions) runs once per candidate; the
-// contacts go to the wave's LDS scratch (nb, kNbDoubles) in chunks of at most
-// kMaxNb, and a point is free when no neighbour of any chunk holds it (the
-// reference's 'last hit' cache only reorders this OR).  Returns the number of
-// free points, or -1 for a tessellation with more than 256 points.
-__device__ int solvent_atom(const DevSys& S, const EnergyLds& L, double* nb, int i, Work& wk) {
+// by one wave.  The reference's contact test runs on every heavy atom (read
+// from the packed copy L.hp, which must be current); the contacts go to the
+// wave's LDS scratch (nb, kNbDoubles), and a point is
+// free when no contact holds it (the reference's 'last hit' cache only
+// reorders this OR).  More than kMaxNb contacts: solvent_atom_chunked.
+// Returns the number of free points, or -1 for a tessellation with more
+// than 128 points.
+__device__ int solvent_atom(const DevSys& S, const EnergyLds& L, double* nbd, int i, Work& wk, int bits_lds,
+                            double* tsplit = nullptr) {
 #pragma clang fp contract(off)
   const int lane = threadIdx.x & (kWave - 1);
+  // optional timing (lane 0): tsplit[0] += gather, [1] += point tests, [2] += 1
+  double tmark = tsplit ? (double)wall_clock64() : 0.0;
   const int npnt = S.npnt, nh = S.nheavy;
   if (npnt > kWave * kMaxPntPerLane) return -1;  // the host rejects such tessellations first
-  int32_t* cand = reinterpret_cast<int32_t*>(nb + kMaxNb * 4);
+  float* nb = reinterpret_cast<float*>(nbd);
   const double xi = L.x[i], yi = L.y[i], zi = L.z[i];
   const double trad = L.rv[i];
-  bool occ[kMaxPntPerLane];
-#pragma unroll
-  for (int q = 0; q < kMaxPntPerLane; ++q) occ[q] = (q * kWave + lane) >= npnt;
-  int base = 0;
-  while (base < nh) {
-    // 1. candidates: centres closer than the sum of radii (a superset of contacts)
-    int nc = 0;
-    while (base < nh && nc <= kMaxNb - kWave) {
-      const int h = base + lane;
-      bool on = false;
-      int j = 0;
-      if (h < nh) {
-        j = L.hv[h];
-        const double ax = xi - L.x[j], ay = yi - L.y[j], az = zi - L.z[j];
-        const double lim = (trad + L.rv[j]) * (1.0 + 1e-9);
-        on = j != i && ax * ax + ay * ay + az * az <= lim * lim;
-      }
-      const uint64_t m = __ballot(on);
-      if (on) cand[nc + __popcll(m & ((1ull << lane) - 1ull))] = j;
-      nc += __popcll(m);
-      base += kWave;
-    }
+  const double rinv = 1.0 / trad;
+  int cnt = 0;
+  bool overflow = false;
+  auto visit = [&](int j, double xj, double yj, double zj, double rj) {
+    ContactGeom cg;
+    const bool on = contact_of(i, j, xi, yi, zi, trad, rinv, xj, yj, zj, rj, cg);
+    const uint64_t m = __ballot(on);
+    const int n = __popcll(m);
+    if (cnt + n > kMaxNb) overflow = true;
+    if (on && !overflow) write_contact(nb + kNbFloats * (cnt + __popcll(m & ((1ull << lane) - 1ull))), j, cg);
+    cnt += n;
+  };
+  for (int h0 = 0; h0 < nh; h0 += kWave) {
+    const int h = h0 + lane;
+    const bool in = h < nh;
+    const double2 a = in ? *reinterpret_cast<const double2*>(L.hp + 4 * h) : double2{0.0, 0.0};
+    const double2 b = in ? *reinterpret_cast<const double2*>(L.hp + 4 * h + 2) : double2{0.0, 0.0};
+    visit(in ? (int)L.hv[h] : -1, a.x, a.y, b.x, b.y);
+  }
+  if (lane == 0) wk.contacts += cnt;
+  if (overflow) {
     wave_lds_sync();
-    // 2. the reference's contact test, one lane per candidate
-    int cnt = 0;
-    for (int c0 = 0; c0 < nc; c0 += kWave) {
-      const int c = c0 + lane;
-      bool on = false;
-      double dx = 0, dy = 0, dz = 0, r2b = 0;
-      if (c < nc) {
-        const int j = cand[c];
-        const double rj = L.rv[j];
-        dx = (xi - L.x[j]) / trad;
-        dy = (yi - L.y[j]) / trad;
-        dz = (zi - L.z[j]) / trad;
-        double dd = dx * dx + dy * dy + dz * dz;
-        double akrad = rj / trad;
-        double dr = 1.0 + akrad;
-        dr = dr * dr;
-        on = dd <= dr;
-        r2b = akrad * akrad;
-      }
-      const uint64_t m = __ballot(on);
-      if (on) {
-        const int pos = cnt + __popcll(m & ((1ull << lane) - 1ull));
-        nb[4 * pos + 0] = dx;
-        nb[4 * pos + 1] = dy;
-        nb[4 * pos + 2] = dz;
-        nb[4 * pos + 3] = r2b;
-      }
-      cnt += __popcll(m);
-    }
-    const int padded = (cnt + 3) & ~3;
-    if (lane < padded - cnt) {  // never occludes: sdd >= 0 > -1
-      const int pos = cnt + lane;
-      nb[4 * pos + 0] = 0.0;
-      nb[4 * pos + 1] = 0.0;
-      nb[4 * pos + 2] = 0.0;
-      nb[4 * pos + 3] = -1.0;
-    }
-    if (lane == 0) wk.contacts += cnt;
-    wave_lds_sync();
-    solvent_points<kMaxPntPerLane>(L, npnt, nb, padded, occ, lane);
-    wave_lds_sync();
+    return solvent_atom_chunked(&S, blockDim.x / kWave, bits_lds, i);
+  }
+  if (tsplit && lane == 0) {
+    const double now = (double)wall_clock64();
+    tsplit[0] += now - tmark;
+    tmark = now;
+  }
+  LanePoints lp;
+  lane_points(L, npnt, lp);
+  solvent_chunk(L, nb, cnt, xi, yi, zi, trad, rinv, lp.x, lp.y, lp.z, lp.e, lp.occ);
+  if (tsplit && lane == 0) {
+    tsplit[1] += (double)wall_clock64() - tmark;
+    tsplit[2] += 1.0;
   }
   int nfree = 0;
 #pragma unroll
-  for (int q = 0; q < kMaxPntPerLane; ++q) nfree += __popcll(__ballot(!occ[q]));
+  for (int q = 0; q < kMaxPntPerLane; ++q) nfree += __popcll(__ballot(!lp.occ[q]));
   return nfree;
 }
 
@@ -523,6 +696,7 @@ template <bool MIXED>
 __device__ void energy_from_coords(const DevSys& S, const double* vl, EnergyLds& L, int32_t* nfree_out,
                                    EnergyOut& out, Work& wk, int part = 0, int nparts = 1) {
   const int tid = threadIdx.x, nwaves = blockDim.x / kWave, wave = tid / kWave, lane = tid & (kWave - 1);
+  if (S.itysol > 0) fill_heavy_packed(S, L);  // read after the barrier below
   if (MIXED) {  // FP32 copies centred on the first atom (frames are dead now)
     const double cx = L.x[0], cy = L.y[0], cz = L.z[0];
     for (int a = tid; a < S.np; a += blockDim.x) {
@@ -536,7 +710,7 @@ __device__ void energy_from_coords(const DevSys& S, const double* vl, EnergyLds&
   const int t0 = part + nparts * wave, stride = nparts * nwaves;
   PairSums ps = MIXED ? pair_terms_mixed(S, L, t0, stride, wk) : pair_terms_f64(S, L, t0, stride, wk);
   double tor = part == 0 ? torsion_terms(S, vl) : 0.0;
-  if (MIXED) __syncthreads();  // the solvent scratch overlays the FP32 copies
+  __syncthreads();  // packed heavy atoms written; the solvent scratch overlays the FP32 copies
   double sol = 0.0, asa = 0.0;
   int err = 0;
   if (S.itysol > 0) {
@@ -544,7 +718,7 @@ __device__ void energy_from_coords(const DevSys& S, const double* vl, EnergyLds&
     const double c4 = 4.0 * kPi / (double)S.npnt;
     for (int h = t0; h < S.nheavy; h += stride) {
       const int i = L.hv[h];
-      int nfree = solvent_atom(S, L, nb, i, wk);
+      int nfree = solvent_atom(S, L, nb, i, wk, L.lbits != nullptr);
       if (nfree < 0) {
         err = 1;
         nfree = 0;

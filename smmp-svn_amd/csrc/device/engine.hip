@@ -392,7 +392,8 @@ int engine_profile_phases(Engine*& e, double* out) {
   Engine& E = engine_of(e);
   for (int k = 0; k < kNPhase; ++k) {
     const double t = E.prof.phase_ticks[k];
-    out[k] = (k == kPhNFull || k == kPhCoreCycles || E.prof.tick_khz <= 0.0) ? t : t / E.prof.tick_khz;
+    const bool count = k == kPhNFull || k == kPhCoreCycles || k == kPhSolAtoms || k == kPhSolContacts;
+    out[k] = (count || E.prof.tick_khz <= 0.0) ? t : t / E.prof.tick_khz;
   }
   return SMMP_OK;
 }

@@ -102,10 +102,15 @@ struct SamplerParams {
   uint64_t wait_ticks;   // bound of one wait, device wall-clock ticks
 };
 constexpr int kPartVals = 8;
+constexpr int kSweepThreads = 512;  // threads of one sweep workgroup (8 waves; 16 spill and run slower)
 
 // Phases of a sweep timed by thread 0 when P.phase is set.
 enum PhaseId { kPhPropose = 0, kPhRotate, kPhPairs, kPhSolvent, kPhReduce, kPhAccept, kPhFull, kPhSweepEnd,
-               kPhNFull, kPhCoreCycles, kPhRealTicks, kNPhase };
+               kPhNFull, kPhCoreCycles, kPhRealTicks,
+               // solvent change, per trial: slowest wave, and sums over waves of the
+               // gather and point-test times, atoms and contacts
+               kPhSolMax, kPhSolGather, kPhSolPoints, kPhSolAtoms, kPhSolContacts, kNPhase };
+constexpr int kScDoubles = 128;  // broadcast scalars [0,32), phase clocks [32,64), per-wave stats [64,128)
 
 // Extra LDS of the sampler beyond the energy carve-up.
 __host__ __device__ inline size_t sampler_lds_bytes(int np, int npnt, int nheavy, int nvt, int nwaves, size_t nbits) {
@@ -115,7 +120,7 @@ __host__ __device__ inline size_t sampler_lds_bytes(int np, int npnt, int nheavy
   b += (size_t)nvt * sizeof(double);              // variables
   b += 2 * (size_t)nheavy * sizeof(int32_t);      // nfree current / trial
   b += 625 * sizeof(uint32_t);                    // MT state
-  b += 48 * sizeof(double);                       // broadcast scalars and phase clocks
+  b += kScDoubles * sizeof(double);               // broadcast scalars and phase clocks
   b += 7 * (size_t)nvt * sizeof(double);          // per-variable tables (VarLds)
   b += 6 * (size_t)nvt * sizeof(int16_t);
   b += 2 * (size_t)nheavy;                        // list of the affected heavy atoms
@@ -150,7 +155,7 @@ __device__ inline SamplerLds carve_sampler_lds(double* base, const DevSys& S, in
   Q.by = reinterpret_cast<double*>(p); p += S.np * sizeof(double);
   Q.bz = reinterpret_cast<double*>(p); p += S.np * sizeof(double);
   Q.vl = reinterpret_cast<double*>(p); p += nvt * sizeof(double);
-  Q.sc = reinterpret_cast<double*>(p); p += 48 * sizeof(double);
+  Q.sc = reinterpret_cast<double*>(p); p += kScDoubles * sizeof(double);
   // doubles first (8-byte aligned), then 32-, 16- and 8-bit arrays
   Q.V.ax = reinterpret_cast<double*>(p); p += nvt * sizeof(double);
   Q.V.sign = reinterpret_cast<double*>(p); p += nvt * sizeof(double);
@@ -267,10 +272,10 @@ __device__ void pair_delta(const DevSys& S, const SamplerLds& Q, int m1, int m2,
       a = m1 + c;
     }
     const size_t w = ((size_t)a * words + (b >> 6)) * 2;
-    const uint64_t w0 = L.bits[w];
+    const uint64_t w0 = pair_bits(L, w);
     if (!((w0 >> (b & 63)) & 1ull)) continue;
     wk.pairs += 2.0;
-    const bool is14 = (L.bits[w + 1] >> (b & 63)) & 1ull;
+    const bool is14 = (pair_bits(L, w + 1) >> (b & 63)) & 1ull;
     const int tt = L.t[a] * kTy + L.t[b];
     const double qq = S.conv * L.q[a] * L.q[b];
     const double A = is14 ? L.pA14[tt] : L.pA[tt];
@@ -347,21 +352,27 @@ __device__ int list_affected(const DevSys& S, const SamplerLds& Q, int g, int G)
 }
 
 // Full energy of the replica in LDS: from the coordinates, or (rebuild) from
-// the variables through the chain build.  Inlined: out of line, the call
-// ABI takes the LDS pointers as flat pointers and doubles the registers.
+// the variables through the chain build.  One out-of-line copy serves every
+// call site (inlined, five copies made the kernel twice the size of the
+// instruction cache).  It takes no LDS pointers: it carves the layout from
+// the dynamic LDS itself, so its accesses stay LDS accesses.
+// nf_sel: 0 = no free-point output, 1 = Q.nf, 2 = Q.nf2.
 struct ReplicaEnergy {
   EnergyOut o;
   Work w;
 };
 template <bool MIXED>
-__device__ __forceinline__ ReplicaEnergy replica_energy(const DevSys& S, const double* vl, EnergyLds L,
-                                                                  int32_t* nfree_out, bool rebuild, int part,
-                                                                  int nparts) {
+__device__ __attribute__((noinline)) ReplicaEnergy replica_energy(const DevSys* sys, int nvt, int bits_lds,
+                                                                  int nf_sel, bool rebuild, int part, int nparts) {
+  extern __shared__ double smem[];
+  const DevSys& S = *sys;
+  SamplerLds Q = carve_sampler_lds(smem, S, nvt, blockDim.x / kWave, bits_lds != 0);
+  int32_t* nfree_out = nf_sel == 1 ? Q.nf : (nf_sel == 2 ? Q.nf2 : nullptr);
   ReplicaEnergy r;
   if (rebuild)
-    conformation_energy<MIXED>(S, vl, L, nfree_out, r.o, r.w, part, nparts);
+    conformation_energy<MIXED>(S, Q.vl, Q.L, nfree_out, r.o, r.w, part, nparts);
   else
-    energy_from_coords<MIXED>(S, vl, L, nfree_out, r.o, r.w, part, nparts);
+    energy_from_coords<MIXED>(S, Q.vl, Q.L, nfree_out, r.o, r.w, part, nparts);
   return r;
 }
 __device__ __forceinline__ void add_work(Work& a, const Work& b) {
@@ -427,10 +438,9 @@ __device__ void group_sum(const SamplerParams& P, int r, int g, uint32_t& epoch,
 
 // P.nsweep Metropolis sweeps of one replica per group of G blocks.
 template <bool MIXED>
-__global__ void __launch_bounds__(512) k_sweep(SamplerParams P) {
+__global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
   extern __shared__ double smem[];
   const DevSys& S = *P.sys;
-  const DevSys& Sg = S;
   const int G = P.G;
   int r, g;
   replica_of_block(P, blockIdx.x, r, g);
@@ -464,7 +474,7 @@ __global__ void __launch_bounds__(512) k_sweep(SamplerParams P) {
   //     [6] jv, [7] drift flag, [8..10] axis point, [11..13] axis unit, [15] old value,
   //     [16] trial energy, [24..28] incremental components (verify), [32..40] phase clocks
   if (!valid) {
-    ReplicaEnergy re = replica_energy<MIXED>(Sg, Q.vl, L, Q.nf, true, g, G);
+    ReplicaEnergy re = replica_energy<MIXED>(P.sys, nvt, P.bits_lds, 1, true, g, G);
     if (g != 0) re.w.builds = 0.0;
     add_work(wk, re.w);
     if (tid == 0) {
@@ -487,6 +497,7 @@ __global__ void __launch_bounds__(512) k_sweep(SamplerParams P) {
   if (tid == 0) {
     for (int k = 0; k < kNPhase; ++k) ph[k] = 0.0;
     ph[kNPhase] = (double)real0;
+    for (int k = 64; k < kScDoubles; ++k) Q.sc[k] = 0.0;
   }
   auto lap = [&](int k) {
     if (P.phase && tid == 0) {
@@ -529,7 +540,7 @@ __global__ void __launch_bounds__(512) k_sweep(SamplerParams P) {
       const bool full = Q.sc[5] != 0.0;
       EnergyOut vref;  // verify mode (G = 1): full energy before the move (thread 0)
       if (P.verify && !full) {
-        const ReplicaEnergy re = replica_energy<MIXED>(Sg, Q.vl, L, nullptr, false, 0, 1);
+        const ReplicaEnergy re = replica_energy<MIXED>(P.sys, nvt, P.bits_lds, 0, false, 0, 1);
         vref = re.o;
         __syncthreads();
       }
@@ -563,7 +574,10 @@ __global__ void __launch_bounds__(512) k_sweep(SamplerParams P) {
         //      heavy atoms k with k % G == g
         PairE d = {0.0, 0.0, 0.0};
         pair_delta(S, Q, m1, m2, g + G * wave, G * nwaves, d, wk);
-        if (S.itysol > 0) mark_affected(S, Q, m1, m2, g, G);
+        if (S.itysol > 0) {
+          mark_affected(S, Q, m1, m2, g, G);
+          fill_heavy_packed(S, L);  // trial coordinates, read after the barrier below
+        }
         double dt = 0.0;
         if (tid == 0 && g == 0) {
           const double e0 = V.e0[jv];
@@ -576,9 +590,13 @@ __global__ void __launch_bounds__(512) k_sweep(SamplerParams P) {
           }
         }
         __syncthreads();
+        int* next_atom = reinterpret_cast<int*>(Q.sc + 30);  // LDS work counter of the solvent loop
         if (S.itysol > 0) {
           const int na = wave == 0 ? list_affected(S, Q, g, G) : 0;
-          if (tid == 0) Q.sc[29] = (double)na;
+          if (tid == 0) {
+            Q.sc[29] = (double)na;
+            *next_atom = 0;
+          }
           __syncthreads();
         }
         lap(kPhPairs);
@@ -587,10 +605,17 @@ __global__ void __launch_bounds__(512) k_sweep(SamplerParams P) {
         if (S.itysol > 0) {
           double* nb = L.fr + (size_t)wave * kNbDoubles;
           const int na = (int)Q.sc[29];
-          for (int idx = wave; idx < na; idx += nwaves) {
+          double* wst = P.phase ? Q.sc + 64 + 4 * wave : nullptr;  // this wave's timing (lane 0)
+          const double tw0 = wst ? (double)wall_clock64() : 0.0;
+          // waves take the next atom as they finish (atom costs vary with their contacts)
+          for (;;) {
+            int idx = 0;
+            if (lane == 0) idx = atomicAdd(next_atom, 1);
+            idx = __shfl(idx, 0, kWave);
+            if (idx >= na) break;
             const int h = Q.alist[idx];
             const int i = L.hv[h];
-            int nfree = solvent_atom(S, L, nb, i, wk);
+            int nfree = solvent_atom(S, L, nb, i, wk, P.bits_lds, wst);
             if (nfree < 0) {
               err = 1;
               nfree = Q.nf[h];
@@ -601,6 +626,7 @@ __global__ void __launch_bounds__(512) k_sweep(SamplerParams P) {
               dsol += L.sg[i] * (c4 * trad * trad * (double)(nfree - Q.nf[h]));
             }
           }
+          if (wst && lane == 0) wst[3] = (double)wall_clock64() - tw0;
         }
         double v[4] = {d.el, d.vw, d.hb, dsol};
 #pragma unroll
@@ -612,6 +638,18 @@ __global__ void __launch_bounds__(512) k_sweep(SamplerParams P) {
         }
         __syncthreads();
         lap(kPhSolvent);
+        if (P.phase && tid == 0 && S.itysol > 0) {
+          double mx = 0.0;
+          for (int w = 0; w < nwaves; ++w) {
+            double* st = Q.sc + 64 + 4 * w;
+            mx = fmax(mx, st[3]);
+            ph[kPhSolGather] += st[0];
+            ph[kPhSolPoints] += st[1];
+            ph[kPhSolAtoms] += st[2];
+            st[0] = st[1] = st[2] = st[3] = 0.0;
+          }
+          ph[kPhSolMax] += mx;
+        }
         if (tid == 0) {
           double tot[6] = {0, 0, 0, 0, dt, 0};  // el, vw, hb, sol, torsion, error
           for (int w = 0; w < nwaves; ++w) {
@@ -628,7 +666,7 @@ __global__ void __launch_bounds__(512) k_sweep(SamplerParams P) {
         lap(kPhReduce);
       } else {
         // ---- full recomputation from the trial coordinates
-        const ReplicaEnergy re = replica_energy<MIXED>(Sg, Q.vl, L, Q.nf2, false, g, G);
+        const ReplicaEnergy re = replica_energy<MIXED>(P.sys, nvt, P.bits_lds, 2, false, g, G);
         add_work(wk, re.w);
         if (tid == 0) {
           double v[2] = {re.o.el + re.o.vw + re.o.hb + re.o.tor + re.o.sol, (double)re.o.err};
@@ -641,7 +679,7 @@ __global__ void __launch_bounds__(512) k_sweep(SamplerParams P) {
         lap(kPhFull);
       }
       if (P.verify && !full) {  // compare each incremental component with a full recomputation
-        const ReplicaEnergy rn = replica_energy<MIXED>(Sg, Q.vl, L, nullptr, false, 0, 1);
+        const ReplicaEnergy rn = replica_energy<MIXED>(P.sys, nvt, P.bits_lds, 0, false, 0, 1);
         const EnergyOut& vnew = rn.o;
         if (tid == 0 && P.dbg[16 * r] == 0.0) {
           const double fd[5] = {vnew.el - vref.el, vnew.vw - vref.vw, vnew.hb - vref.hb, vnew.tor - vref.tor,
@@ -712,7 +750,7 @@ __global__ void __launch_bounds__(512) k_sweep(SamplerParams P) {
       lap(kPhAccept);
     }
     // ---- sweep end: rebuild from the variables and check (metropolis.f:172-179)
-    ReplicaEnergy re = replica_energy<MIXED>(Sg, Q.vl, L, Q.nf, true, g, G);
+    ReplicaEnergy re = replica_energy<MIXED>(P.sys, nvt, P.bits_lds, 1, true, g, G);
     if (g != 0) re.w.builds = 0.0;
     add_work(wk, re.w);
     if (tid == 0) {
