@@ -157,6 +157,9 @@ def main():
         log(f"solvent per trial (block 0 of each replica): slowest wave {ph['sol_max_wave'] * 1e3 / ntrial:.2f} us; "
             f"per atom: gather {ph['sol_gather'] * 1e3 / na:.3f} us, point tests {ph['sol_points'] * 1e3 / na:.3f} us; "
             f"atoms per trial {na / ntrial:.1f}")
+        log(f"pair change per trial: slowest wave {ph['pair_max_wave'] * 1e3 / ntrial:.2f} us, "
+            f"wave average {ph['pair_sum'] * 1e3 / ntrial / 8:.2f} us; contact marking {ph['mark_sum'] * 1e3 / ntrial / 8:.2f} us; "
+            f"torsion term (thread 0) {ph['torsion'] * 1e3 / ntrial:.2f} us")
     c.profile(False)
     value = NREP * args.steps / dt
     npnt = int(c.view("pnt.npnt")[0])

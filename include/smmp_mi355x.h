@@ -143,7 +143,7 @@ int smmp_profile(smmp_ctx *ctx, int enable);
 int smmp_profile_read(smmp_ctx *ctx, int kernel, double *total_ms, int64_t *launches, double *work);
 /* With smmp_profile(ctx, 2) the sweep kernel also times its phases on the
  * device clock (one replica = one workgroup, thread 0 after each barrier).
- * out[16]: milliseconds summed over replicas for proposal, rotation, pair
+ * out[...]: milliseconds summed over replicas for proposal, rotation, pair
  * change (+ contact marking, torsion term), solvent change, the reduction
  * across a replica's workgroups, acceptance and commit, full-recomputation
  * trials, sweep-end rebuild; out[8] = number of trials that took the
@@ -151,7 +151,10 @@ int smmp_profile_read(smmp_ctx *ctx, int kernel, double *total_ms, int64_t *laun
  * out[10] = milliseconds of the whole launches (their ratio is the clock);
  * solvent change of the first workgroup of each replica: out[11] ms of its
  * slowest wave, out[12]/out[13] ms of neighbour gathering / point tests
- * summed over waves, out[14] atoms evaluated, out[15] unused. */
+ * summed over waves, out[14] atoms evaluated, out[15] unused; pair change
+ * of the same workgroups: out[16] ms of the slowest wave, out[17] ms summed
+ * over waves, out[18] ms of contact marking summed over waves, out[19] ms of
+ * thread 0's torsion term.  out must hold 20 values. */
 int smmp_profile_phases(smmp_ctx *ctx, double *out);
 
 #ifdef __cplusplus

@@ -23,10 +23,12 @@ struct DevSys {
   const double *q, *bl, *cba, *sba, *toat0, *tref, *rvdw, *sigma;
   const int16_t* tvar;
   const int16_t *var_atom, *var_base, *var_axis0, *var_m1, *var_m2;
+  const int16_t *var_h1, *var_h2;  // heavy-index range [h1, h2) of the moved atoms
   const double* var_sign;  // rotation of [m1,m2] is sign * (change of the torsion)
   const double *var_e0, *var_sg, *var_rn;
   const uint64_t* bits;
   const int16_t* heavy;
+  const int16_t* heavy_of;  // per atom: index in heavy, or -1
   const double *pA, *pA14, *pC;
   const int32_t* pHB;
   const double* spoint;  // npnt*3

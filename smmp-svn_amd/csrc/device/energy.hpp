@@ -20,7 +20,7 @@ typedef const uint64_t __attribute__((address_space(3))) lds_u64;
 struct EnergyLds {
   double *x, *y, *z, *q;   // np each
   double* fr;              // frames: 6*np (or the solvent scratch)
-  double *pA, *pA14, *pC, *pHB;  // 324 each
+  double* ptab;            // pair types (18x18) x {A, A14, C, H-bond flag}: 4 * 324
   double* sp;              // npnt*3
   double* red;             // nwaves*8
   double *rv, *sg;         // solvent radius and sigma per atom (np each)
@@ -29,6 +29,7 @@ struct EnergyLds {
   const lds_u64* lbits;    // the LDS copy as an LDS pointer (ds_read), or null
   int8_t* t;               // np bytes
   int16_t* hv;             // heavy-atom list (nheavy)
+  int16_t* hidx;           // per atom (np): index in hv, or -1
   float *xf, *yf, *zf, *qf;  // mixed-mode copies (alias fr after the build)
 };
 
@@ -46,7 +47,7 @@ __host__ __device__ inline int energy_region1(int np, int nwaves) {
 __host__ __device__ inline size_t energy_lds_bytes(int np, int npnt, int nwaves, int nheavy, size_t nbits) {
   size_t d = 6 * (size_t)np + energy_region1(np, nwaves) + 4 * 324 + 3 * (size_t)npnt + 8 * (size_t)nwaves + nbits +
              4 * (size_t)nheavy;
-  return d * sizeof(double) + (size_t)np + 2 * (size_t)nheavy;
+  return d * sizeof(double) + (size_t)np + 2 * (size_t)nheavy + 2 * (size_t)np;
 }
 
 __device__ inline EnergyLds carve_energy_lds(double* base, const DevSys& S, int nwaves, bool bits_lds) {
@@ -58,10 +59,7 @@ __device__ inline EnergyLds carve_energy_lds(double* base, const DevSys& S, int 
   L.z = p; p += np;
   L.q = p; p += np;
   L.fr = p; p += energy_region1(np, nwaves);
-  L.pA = p; p += 324;
-  L.pA14 = p; p += 324;
-  L.pC = p; p += 324;
-  L.pHB = p; p += 324;
+  L.ptab = p; p += 4 * 324;
   L.sp = p; p += 3 * npnt;
   L.red = p; p += 8 * nwaves;
   L.rv = p; p += np;
@@ -77,6 +75,7 @@ __device__ inline EnergyLds carve_energy_lds(double* base, const DevSys& S, int 
   }
   L.t = reinterpret_cast<int8_t*>(p);
   L.hv = reinterpret_cast<int16_t*>(L.t + np);  // np is a multiple of 64
+  L.hidx = L.hv + S.nheavy;
   L.xf = reinterpret_cast<float*>(L.fr);
   L.yf = L.xf + np;
   L.zf = L.yf + np;
@@ -104,10 +103,10 @@ __device__ __forceinline__ double atom_torsion(const DevSys& S, const double* vl
 __device__ void energy_stage_static(const DevSys& S, EnergyLds& L) {
   const int tid = threadIdx.x, nt = blockDim.x, np = S.np;
   for (int k = tid; k < 324; k += nt) {
-    L.pA[k] = S.pA[k];
-    L.pA14[k] = S.pA14[k];
-    L.pC[k] = S.pC[k];
-    L.pHB[k] = (double)S.pHB[k];
+    L.ptab[4 * k + 0] = S.pA[k];
+    L.ptab[4 * k + 1] = S.pA14[k];
+    L.ptab[4 * k + 2] = S.pC[k];
+    L.ptab[4 * k + 3] = (double)S.pHB[k];
   }
   for (int k = tid; k < 3 * S.npnt; k += nt) L.sp[k] = S.spoint[k];
   for (int h = tid; h < S.nheavy; h += nt) L.hv[h] = S.heavy[h];
@@ -122,11 +121,13 @@ __device__ void energy_stage_static(const DevSys& S, EnergyLds& L) {
       L.t[a] = S.type[a];
       L.rv[a] = S.rvdw[a];
       L.sg[a] = S.sigma[a];
+      L.hidx[a] = S.heavy_of[a];
     } else {
       L.q[a] = 0.0;
       L.t[a] = 0;
       L.rv[a] = 0.0;
       L.sg[a] = 0.0;
+      L.hidx[a] = -1;
       L.x[a] = 0.0;
       L.y[a] = 0.0;
       L.z[a] = 0.0;
@@ -290,8 +291,8 @@ __device__ PairSums pair_terms_f64(const DevSys& S, const EnergyLds& L, int t0, 
         double r6i = r2i * r2i * r2i;
         const int tt = ti + L.t[j];
         const bool is14 = (w1 >> k) & 1ull;
-        double A = is14 ? L.pA14[tt] : L.pA[tt];
-        double Cc = L.pC[tt];
+        double A = L.ptab[4 * tt + (is14 ? 1 : 0)];
+        double Cc = L.ptab[4 * tt + 2];
         double ep_inv = 1.0;
         if (S.epsd) {
           double rij = r2 * ri;
@@ -299,7 +300,7 @@ __device__ PairSums pair_terms_f64(const DevSys& S, const EnergyLds& L, int t0, 
           ep_inv = 1.0 / (S.plt - (sr * sr + 2.0 * sr + 2.0) * (S.plt - 1.0) * exp(-sr) / 2.0);
         }
         el += qi * L.q[j] * ri * ep_inv;
-        if (L.pHB[tt] != 0.0)
+        if (L.ptab[4 * tt + 3] != 0.0)
           hb += A * (r6i * r6i) - Cc * (r6i * r2i * r2i);
         else
           vw += A * (r6i * r6i) - Cc * r6i;
@@ -352,8 +353,9 @@ __device__ PairSums pair_terms_mixed(const DevSys& S, const EnergyLds& L, int t0
         float r6i = r2i * r2i * r2i;
         const int tt = ti + L.t[j];
         const bool is14 = (w1 >> k) & 1ull;
-        float A = (float)(is14 ? L.pA14[tt] : L.pA[tt]);
-        float Cc = (float)L.pC[tt];
+        float A = (float)L.ptab[4 * tt + (is14 ? 1 : 0)];
+        float Cc = (float)L.ptab[4 * tt + 2];
+        const bool hbt = L.ptab[4 * tt + 3] != 0.0;
         float ep_inv = 1.f;
         if (S.epsd) {  // sigmoidal dielectric (enyshe.f:129-134)
           const float sr = (float)S.slp * (r2 * ri), plt = (float)S.plt;
@@ -361,8 +363,8 @@ __device__ PairSums pair_terms_mixed(const DevSys& S, const EnergyLds& L, int t0
         }
         const float tel = qi * L.qf[j] * ri * ep_inv;
         fel += tel;
-        const float tlj = A * (r6i * r6i) - Cc * (L.pHB[tt] != 0.0 ? r6i * r2i * r2i : r6i);
-        if (L.pHB[tt] != 0.0)
+        const float tlj = A * (r6i * r6i) - Cc * (hbt ? r6i * r2i * r2i : r6i);
+        if (hbt)
           fhb += tlj;
         else
           fvw += tlj;

@@ -148,12 +148,15 @@ static void upload_system(DevSystem& ds, hipStream_t st) {
   size_t o_vx = add(h.var_axis0.data(), h.var_axis0.size() * 2);
   size_t o_m1 = add(h.var_m1.data(), h.var_m1.size() * 2);
   size_t o_m2 = add(h.var_m2.data(), h.var_m2.size() * 2);
+  size_t o_h1 = add(h.var_h1.data(), h.var_h1.size() * 2);
+  size_t o_h2 = add(h.var_h2.data(), h.var_h2.size() * 2);
   size_t o_vsign = add(h.var_sign.data(), h.var_sign.size() * 8);
   size_t o_e0 = add(h.var_e0.data(), h.var_e0.size() * 8);
   size_t o_vs = add(h.var_sg.data(), h.var_sg.size() * 8);
   size_t o_vr = add(h.var_rn.data(), h.var_rn.size() * 8);
   size_t o_bits = add(h.bits.data(), h.bits.size() * 8);
   size_t o_heavy = add(h.heavy.data(), h.heavy.size() * 2);
+  size_t o_heavy_of = add(h.heavy_of.data(), h.heavy_of.size() * 2);
   size_t o_pA = add(h.pA.data(), h.pA.size() * 8);
   size_t o_pA14 = add(h.pA14.data(), h.pA14.size() * 8);
   size_t o_pC = add(h.pC.data(), h.pC.size() * 8);
@@ -189,12 +192,15 @@ static void upload_system(DevSystem& ds, hipStream_t st) {
   d.var_axis0 = (const int16_t*)(base + o_vx);
   d.var_m1 = (const int16_t*)(base + o_m1);
   d.var_m2 = (const int16_t*)(base + o_m2);
+  d.var_h1 = (const int16_t*)(base + o_h1);
+  d.var_h2 = (const int16_t*)(base + o_h2);
   d.var_sign = (const double*)(base + o_vsign);
   d.var_e0 = (const double*)(base + o_e0);
   d.var_sg = (const double*)(base + o_vs);
   d.var_rn = (const double*)(base + o_vr);
   d.bits = (const uint64_t*)(base + o_bits);
   d.heavy = (const int16_t*)(base + o_heavy);
+  d.heavy_of = (const int16_t*)(base + o_heavy_of);
   d.pA = (const double*)(base + o_pA);
   d.pA14 = (const double*)(base + o_pA14);
   d.pC = (const double*)(base + o_pC);

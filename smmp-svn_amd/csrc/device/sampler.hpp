@@ -109,8 +109,11 @@ enum PhaseId { kPhPropose = 0, kPhRotate, kPhPairs, kPhSolvent, kPhReduce, kPhAc
                kPhNFull, kPhCoreCycles, kPhRealTicks,
                // solvent change, per trial: slowest wave, and sums over waves of the
                // gather and point-test times, atoms and contacts
-               kPhSolMax, kPhSolGather, kPhSolPoints, kPhSolAtoms, kPhSolContacts, kNPhase };
-constexpr int kScDoubles = 128;  // broadcast scalars [0,32), phase clocks [32,64), per-wave stats [64,128)
+               kPhSolMax, kPhSolGather, kPhSolPoints, kPhSolAtoms, kPhSolContacts,
+               // pair change, per trial: slowest wave and wave sum of pair_delta,
+               // wave sum of the contact marking, thread 0's torsion term
+               kPhPairMax, kPhPairSum, kPhMarkSum, kPhTorsion, kNPhase };
+constexpr int kScDoubles = 160;  // broadcast scalars [0,32), phase clocks [32,64), per-wave stats [64,160)
 
 // Extra LDS of the sampler beyond the energy carve-up.
 __host__ __device__ inline size_t sampler_lds_bytes(int np, int npnt, int nheavy, int nvt, int nwaves, size_t nbits) {
@@ -122,7 +125,7 @@ __host__ __device__ inline size_t sampler_lds_bytes(int np, int npnt, int nheavy
   b += 625 * sizeof(uint32_t);                    // MT state
   b += kScDoubles * sizeof(double);               // broadcast scalars and phase clocks
   b += 7 * (size_t)nvt * sizeof(double);          // per-variable tables (VarLds)
-  b += 6 * (size_t)nvt * sizeof(int16_t);
+  b += 8 * (size_t)nvt * sizeof(int16_t);
   b += 2 * (size_t)nheavy;                        // list of the affected heavy atoms
   b += (size_t)nheavy;                            // affected flags
   return b + 64;
@@ -133,7 +136,7 @@ __host__ __device__ inline size_t sampler_lds_bytes(int np, int npnt, int nheavy
 // (toat0, tref) and controlling variable, moved range, axis atoms.
 struct VarLds {
   double *ax, *sign, *e0, *sg, *rn, *t0, *tref;
-  int16_t *m1, *m2, *base, *axis0, *tv, *idvr;
+  int16_t *m1, *m2, *base, *axis0, *tv, *idvr, *h1, *h2;
 };
 
 struct SamplerLds {
@@ -173,6 +176,8 @@ __device__ inline SamplerLds carve_sampler_lds(double* base, const DevSys& S, in
   Q.V.axis0 = reinterpret_cast<int16_t*>(p); p += nvt * sizeof(int16_t);
   Q.V.tv = reinterpret_cast<int16_t*>(p); p += nvt * sizeof(int16_t);
   Q.V.idvr = reinterpret_cast<int16_t*>(p); p += nvt * sizeof(int16_t);
+  Q.V.h1 = reinterpret_cast<int16_t*>(p); p += nvt * sizeof(int16_t);
+  Q.V.h2 = reinterpret_cast<int16_t*>(p); p += nvt * sizeof(int16_t);
   Q.alist = reinterpret_cast<int16_t*>(p); p += S.nheavy * sizeof(int16_t);
   Q.aff = reinterpret_cast<int8_t*>(p);
   return Q;
@@ -191,6 +196,8 @@ __device__ void stage_variables(const SamplerParams& P, const DevSys& S, const V
     V.m2[k] = S.var_m2[k];
     V.base[k] = S.var_base[k];
     V.axis0[k] = S.var_axis0[k];
+    V.h1[k] = S.var_h1[k];
+    V.h2[k] = S.var_h2[k];
     double t0 = 0.0, tref = 0.0;
     int16_t tv = -1;
     if (e0 != 0.0) {
@@ -206,15 +213,6 @@ __device__ void stage_variables(const SamplerParams& P, const DevSys& S, const V
   for (int k = threadIdx.x; k < P.nvr; k += blockDim.x) V.idvr[k] = (int16_t)P.idvr[k];
 }
 
-// Torsion of variable k's torsion atom for the values vl (as atom_torsion).
-__device__ __forceinline__ double var_torsion(const VarLds& V, const double* vl, int k) {
-  const int v = V.tv[k];
-  const double t0 = V.t0[k];
-  if (v < 0) return t0;
-  double to = t0 + d_difang(V.tref[k], vl[v]);
-  if (fabs(to) > kPi) to = to - copysign(kPi2, to);
-  return to;
-}
 
 // Weight of an energy (canon.f:135-149, mulcan_par_mod.f90:380-412).
 __device__ inline double weight_of(const SamplerParams& P, double beta, double x) {
@@ -261,8 +259,9 @@ __device__ void pair_delta(const DevSys& S, const SamplerLds& Q, int m1, int m2,
   const int nob = (ncol + 63) / 64;
   for (int item = item0; item < nrow * nob; item += stride) {
     const int r = item / nob, ob = item - r * nob;
-    const int c = ob * 64 + lane;
-    if (c >= ncol) continue;
+    const int c0 = ob * 64 + lane;
+    const bool inb = c0 < ncol;
+    const int c = inb ? c0 : 0;
     int a, b;  // a in M, b fixed
     if (rowsM) {
       a = m1 + r;
@@ -271,25 +270,32 @@ __device__ void pair_delta(const DevSys& S, const SamplerLds& Q, int m1, int m2,
       b = r < m1 ? r : r + nM;
       a = m1 + c;
     }
+    // every load up front (the bit only masks the sums), so the LDS round
+    // trips of an item overlap instead of chaining behind a branch
     const size_t w = ((size_t)a * words + (b >> 6)) * 2;
-    const uint64_t w0 = pair_bits(L, w);
-    if (!((w0 >> (b & 63)) & 1ull)) continue;
-    wk.pairs += 2.0;
-    const bool is14 = (pair_bits(L, w + 1) >> (b & 63)) & 1ull;
-    const int tt = L.t[a] * kTy + L.t[b];
-    const double qq = S.conv * L.q[a] * L.q[b];
-    const double A = is14 ? L.pA14[tt] : L.pA[tt];
-    const double C = L.pC[tt];
-    const bool hb = L.pHB[tt] != 0.0;
-    const double xb = L.x[b], yb = L.y[b], zb = L.z[b];
+    const uint64_t w0 = pair_bits(L, w), w1 = pair_bits(L, w + 1);
     const int k = a - m1;
+    const double xa = L.x[a], ya = L.y[a], za = L.z[a];
+    const double xo = Q.bx[k], yo = Q.by[k], zo = Q.bz[k];
+    const double xb = L.x[b], yb = L.y[b], zb = L.z[b];
+    const double qa = L.q[a], qb = L.q[b];
+    const int tt = L.t[a] * kTy + L.t[b];
+    const double2 aa = *reinterpret_cast<const double2*>(L.ptab + 4 * tt);
+    const double2 cf = *reinterpret_cast<const double2*>(L.ptab + 4 * tt + 2);
+    const bool on = inb && ((w0 >> (b & 63)) & 1ull);
+    const bool is14 = (w1 >> (b & 63)) & 1ull;
+    const double A = is14 ? aa.y : aa.x, C = cf.x;
+    const bool hb = cf.y != 0.0;
+    const double qq = S.conv * qa * qb;
     double eln, ljn, elo, ljo;
-    pair_split(S, L.x[a] - xb, L.y[a] - yb, L.z[a] - zb, qq, A, C, hb, eln, ljn);
-    pair_split(S, Q.bx[k] - xb, Q.by[k] - yb, Q.bz[k] - zb, qq, A, C, hb, elo, ljo);
-    d.el += eln - elo;
-    const double dlj = ljn - ljo;
+    pair_split(S, xa - xb, ya - yb, za - zb, qq, A, C, hb, eln, ljn);
+    pair_split(S, xo - xb, yo - yb, zo - zb, qq, A, C, hb, elo, ljo);
+    const double del = on ? eln - elo : 0.0;
+    const double dlj = on ? ljn - ljo : 0.0;
+    d.el += del;
     d.hb += hb ? dlj : 0.0;
     d.vw += hb ? 0.0 : dlj;
+    wk.pairs += on ? 2.0 : 0.0;
   }
 }
 
@@ -298,39 +304,34 @@ __device__ void pair_delta(const DevSys& S, const SamplerLds& Q, int m1, int m2,
 // even a rigidly moved environment changes its discretised area) and every
 // fixed atom in contact with a moved one before or after the move.  Only
 // the fixed atoms of this block's share (h % G == g) are tested.
-__device__ void mark_affected(const DevSys& S, const SamplerLds& Q, int m1, int m2, int g, int G) {
+__device__ void mark_affected(const DevSys& S, const SamplerLds& Q, int jv, int m1, int g, int G) {
   const int nh = S.nheavy;
-  int h1 = 0, h2 = nh;  // heavy atoms of [m1,m2]: one index range (heavy is sorted)
-  {
-    int lo = 0, hi = nh;
-    while (lo < hi) {
-      int mid = (lo + hi) >> 1;
-      if (Q.L.hv[mid] < m1) lo = mid + 1; else hi = mid;
-    }
-    h1 = lo;
-    lo = h1, hi = nh;
-    while (lo < hi) {
-      int mid = (lo + hi) >> 1;
-      if (Q.L.hv[mid] <= m2) lo = mid + 1; else hi = mid;
-    }
-    h2 = lo;
-  }
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave, nwaves = blockDim.x / kWave;
+  const int h1 = Q.V.h1[jv], h2 = Q.V.h2[jv];  // heavy atoms of the moved range
   const int nM = h2 - h1, nown = (nh - g + G - 1) / G;
   for (int h = h1 + threadIdx.x; h < h2; h += blockDim.x) Q.aff[h] = 1;
-  for (int it = threadIdx.x; it < nM * nown; it += blockDim.x) {
-    const int r = it / nown, k = it - r * nown;
-    const int ha = h1 + r, hb = g + G * k;
-    if (hb >= h1 && hb < h2) continue;
-    const int a = Q.L.hv[ha], b = Q.L.hv[hb];
-    const double lim = (Q.L.rv[a] + Q.L.rv[b]) * (1.0 + 1e-9);
-    const double l2 = lim * lim;
-    const double xb = Q.L.x[b], yb = Q.L.y[b], zb = Q.L.z[b];
-    double dx = Q.L.x[a] - xb, dy = Q.L.y[a] - yb, dz = Q.L.z[a] - zb;
-    bool hit = dx * dx + dy * dy + dz * dz <= l2;
-    const int k2 = a - m1;
-    dx = Q.bx[k2] - xb; dy = Q.by[k2] - yb; dz = Q.bz[k2] - zb;
-    hit = hit || (dx * dx + dy * dy + dz * dz <= l2);
-    if (hit) Q.aff[hb] = 1;
+  // a wave per moved heavy atom, lanes over this share's heavy atoms; the
+  // packed copy holds the trial positions, the backup the old ones
+  const double* hp = Q.L.hp;
+  for (int r = wave; r < nM; r += nwaves) {
+    const int ha = h1 + r;
+    const int k2 = Q.L.hv[ha] - m1;
+    const double xn = hp[4 * ha + 0], yn = hp[4 * ha + 1], zn = hp[4 * ha + 2], ra = hp[4 * ha + 3];
+    const double xo = Q.bx[k2], yo = Q.by[k2], zo = Q.bz[k2];
+    for (int k0 = 0; k0 < nown; k0 += kWave) {
+      const int k = k0 + lane;
+      const int hb = g + G * k;
+      if (k >= nown || (hb >= h1 && hb < h2)) continue;
+      const double2 pb = *reinterpret_cast<const double2*>(hp + 4 * hb);
+      const double2 qb = *reinterpret_cast<const double2*>(hp + 4 * hb + 2);
+      const double lim = (ra + qb.y) * (1.0 + 1e-9);
+      const double l2 = lim * lim;
+      double dx = xn - pb.x, dy = yn - pb.y, dz = zn - qb.x;
+      bool hit = dx * dx + dy * dy + dz * dz <= l2;
+      dx = xo - pb.x; dy = yo - pb.y; dz = zo - qb.x;
+      hit = hit || (dx * dx + dy * dy + dz * dz <= l2);
+      if (hit) Q.aff[hb] = 1;
+    }
   }
 }
 
@@ -470,6 +471,10 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
     for (int h = tid; h < nh; h += blockDim.x) Q.nf[h] = gnf[h];
   }
   __syncthreads();
+  if (valid && S.itysol > 0) {  // the packed heavy copy of the loaded coordinates
+    fill_heavy_packed(S, L);
+    __syncthreads();
+  }
   // sc: [0] eol, [1] acz, [2] trial value, [3] dphi, [4] accept, [5] full-trial flag,
   //     [6] jv, [7] drift flag, [8..10] axis point, [11..13] axis unit, [15] old value,
   //     [16] trial energy, [24..28] incremental components (verify), [32..40] phase clocks
@@ -559,9 +564,36 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
           const double kv = kx * vx + ky * vy + kz * vz;
           const double cx = ky * vz - kz * vy, cy = kz * vx - kx * vz, cz = kx * vy - ky * vx;
           const double t = kv * (1.0 - cs);
-          L.x[a] = px + vx * cs + cx * sn + kx * t;
-          L.y[a] = py + vy * cs + cy * sn + ky * t;
-          L.z[a] = pz + vz * cs + cz * sn + kz * t;
+          const double xn = px + vx * cs + cx * sn + kx * t;
+          const double yn = py + vy * cs + cy * sn + ky * t;
+          const double zn = pz + vz * cs + cz * sn + kz * t;
+          L.x[a] = xn;
+          L.y[a] = yn;
+          L.z[a] = zn;
+          const int h = L.hidx[a];
+          if (h >= 0) {
+            L.hp[4 * h + 0] = xn;
+            L.hp[4 * h + 1] = yn;
+            L.hp[4 * h + 2] = zn;
+          }
+        }
+        // torsion term of the variable (enyshe.f:73-79), the new and the old
+        // angle on two idle lanes of the last wave
+        const int tl = tid - ((int)blockDim.x - kWave);
+        if (g == 0 && (tl == 0 || tl == 1)) {
+          const double e0 = V.e0[jv];
+          double e = 0.0;
+          if (e0 != 0.0) {
+            const int v = V.tv[jv];
+            double to = V.t0[jv];
+            if (v >= 0) {
+              const double val = v == jv ? (tl == 0 ? Q.sc[2] : Q.sc[15]) : Q.vl[v];
+              to = to + d_difang(V.tref[jv], val);
+              if (fabs(to) > kPi) to = to - copysign(kPi2, to);
+            }
+            e = e0 * (1.0 + V.sg[jv] * cos(to * V.rn[jv]));
+          }
+          Q.sc[17 + tl] = e;  // [17] new, [18] old
         }
         for (int h = tid; h < nh; h += blockDim.x) Q.aff[h] = 0;
         if (tid == 0) Q.vl[jv] = Q.sc[2];
@@ -573,23 +605,34 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
         // ---- incremental energy change; block g takes pair items and
         //      heavy atoms k with k % G == g
         PairE d = {0.0, 0.0, 0.0};
+        double* pst = P.phase ? Q.sc + 96 + 4 * wave : nullptr;  // this wave's pair timing (lane 0)
+        double tpa = pst ? (double)wall_clock64() : 0.0;
         pair_delta(S, Q, m1, m2, g + G * wave, G * nwaves, d, wk);
-        if (S.itysol > 0) {
-          mark_affected(S, Q, m1, m2, g, G);
-          fill_heavy_packed(S, L);  // trial coordinates, read after the barrier below
+        if (pst && lane == 0) {
+          const double now = (double)wall_clock64();
+          pst[0] += now - tpa;
+          tpa = now;
         }
-        double dt = 0.0;
-        if (tid == 0 && g == 0) {
-          const double e0 = V.e0[jv];
-          if (e0 != 0.0) {
-            const double tn = var_torsion(V, Q.vl, jv);
-            Q.vl[jv] = Q.sc[15];
-            const double to = var_torsion(V, Q.vl, jv);
-            Q.vl[jv] = Q.sc[2];
-            dt = e0 * (1.0 + V.sg[jv] * cos(tn * V.rn[jv])) - e0 * (1.0 + V.sg[jv] * cos(to * V.rn[jv]));
-          }
+        if (S.itysol > 0) mark_affected(S, Q, jv, m1, g, G);
+        if (pst && lane == 0) {
+          const double now = (double)wall_clock64();
+          pst[1] += now - tpa;
+          tpa = now;
         }
+        const double dt = g == 0 ? Q.sc[17] - Q.sc[18] : 0.0;  // torsion change, from the rotation phase
         __syncthreads();
+        if (P.phase && tid == 0) {
+          double mx = 0.0;
+          for (int w = 0; w < nwaves; ++w) {
+            double* st = Q.sc + 96 + 4 * w;
+            mx = fmax(mx, st[0]);
+            ph[kPhPairSum] += st[0];
+            ph[kPhMarkSum] += st[1];
+            ph[kPhTorsion] += st[2];
+            st[0] = st[1] = st[2] = st[3] = 0.0;
+          }
+          ph[kPhPairMax] += mx;
+        }
         int* next_atom = reinterpret_cast<int*>(Q.sc + 30);  // LDS work counter of the solvent loop
         if (S.itysol > 0) {
           const int na = wave == 0 ? list_affected(S, Q, g, G) : 0;
@@ -741,9 +784,16 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
         }
       } else {
         for (int a = m1 + tid; a <= m2; a += blockDim.x) {
-          L.x[a] = Q.bx[a - m1];
-          L.y[a] = Q.by[a - m1];
-          L.z[a] = Q.bz[a - m1];
+          const double xo = Q.bx[a - m1], yo = Q.by[a - m1], zo = Q.bz[a - m1];
+          L.x[a] = xo;
+          L.y[a] = yo;
+          L.z[a] = zo;
+          const int h = L.hidx[a];
+          if (h >= 0) {
+            L.hp[4 * h + 0] = xo;
+            L.hp[4 * h + 1] = yo;
+            L.hp[4 * h + 2] = zo;
+          }
         }
       }
       __syncthreads();

@@ -35,6 +35,8 @@ HostSys build_host_sys(const State& s) {
     if (s.itysol != 0 && s.rvdw(a) > 0.0) h.heavy.push_back((int16_t)i);
   }
   h.nheavy = (int)h.heavy.size();
+  h.heavy_of.assign(nat, (int16_t)-1);
+  for (int k = 0; k < h.nheavy; ++k) h.heavy_of[h.heavy[k]] = (int16_t)k;
   // build levels: level(a) = level(parent(a)) + 1, first atom at level 0
   std::vector<int> lev(nat, 0);
   int maxlev = 0;
@@ -111,6 +113,15 @@ HostSys build_host_sys(const State& s) {
     h.var_m2[k] = (int16_t)(h.var_m1[k] - 1);
     h.var_m1[k] = 0;
     h.var_sign[k] = -1.0;
+  }
+  // heavy-index range of each moved range (heavy is sorted by atom)
+  h.var_h1.resize(nvt);
+  h.var_h2.resize(nvt);
+  for (int k = 0; k < nvt; ++k) {
+    const auto lo = std::lower_bound(h.heavy.begin(), h.heavy.end(), h.var_m1[k]);
+    const auto hi = std::upper_bound(h.heavy.begin(), h.heavy.end(), h.var_m2[k]);
+    h.var_h1[k] = (int16_t)(lo - h.heavy.begin());
+    h.var_h2[k] = (int16_t)(hi - h.heavy.begin());
   }
   // interaction set from the reference lists (mklist), as symmetric bits
   h.bits.assign((size_t)nat * h.words * 2, 0ull);

@@ -28,6 +28,7 @@ struct HostSys {
   std::vector<int16_t> var_base;    // iowat(iatvr)-1: axis end, not moved
   std::vector<int16_t> var_axis0;   // iowat(base)-1: axis start
   std::vector<int16_t> var_m1, var_m2;  // atoms a trial move rotates, [m1, m2]
+  std::vector<int16_t> var_h1, var_h2;  // their heavy atoms: heavy indices [h1, h2)
   std::vector<double> var_sign;         // +1: by the torsion change, -1: by its opposite
   std::vector<double> var_e0, var_sg, var_rn;  // torsion potential (e0to/2, sign, period)
   std::vector<int32_t> var_is_torsion;
@@ -36,6 +37,7 @@ struct HostSys {
   std::vector<uint64_t> bits;
   int64_t npairs = 0, n14 = 0;
   std::vector<int16_t> heavy;       // atoms with rvdw > 0
+  std::vector<int16_t> heavy_of;    // per atom: its index in 'heavy', or -1
   // parameters, [ti*18 + tj]: A (aij or ahb), A14 (a14 or ahb), C (cij or chb), hb flag
   std::vector<double> pA, pA14, pC;
   std::vector<int32_t> pHB;
