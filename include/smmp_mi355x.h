@@ -131,6 +131,18 @@ typedef struct {
 int smmp_sample(smmp_ctx *ctx, const smmp_sampler_cfg *cfg, double *vlvr, double *eol,
                 double *acc, int32_t *ipoi, int32_t *hist, double *traj);
 
+/* Multicanonical weight iteration (host arithmetic; arrays indexed from
+ * kmin, kmax-kmin+1 values each).
+ * smmp_muca_update: one update of mulcan_par (mulcan_par_mod.f90:104-155):
+ * adds the sweep histogram ihist to xhist, advances Berg's recursion in
+ * g1/b, recomputes alpha and (if not NULL) ent = (beta+b)k ebin + alpha,
+ * and clears ihist.  beta is mulcan_par's 1/(k_B temp).
+ * smmp_muca_final: the weights written to muca.d (mulcan_par_mod.f90:168-176):
+ * b += beta and alpha recomputed, in place. */
+int smmp_muca_update(int kmin, int kmax, double ebin, double beta, int32_t *ihist, double *xhist, double *g1,
+                     double *b, double *alpha, double *ent);
+int smmp_muca_final(int kmin, int kmax, double ebin, double beta, double *b, double *alpha);
+
 /* Device / build information string (arch, kernels, tile shapes). */
 const char *smmp_build_info(void);
 

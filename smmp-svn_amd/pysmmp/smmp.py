@@ -1,0 +1,287 @@
+"""Drop-in replacement of the f2py extension module ``smmp`` (smmp.pyf).
+
+The reference's Python layer (universe.py, protein.py, algorithms.py) calls
+``smmp.<function>(...)`` and reads and writes COMMON-block variables as
+``smmp.<block>.<field>``.  This module offers the same surface over the
+MI355X engine's C-ABI (libsmmp_mi355x.so): one process-wide context holds
+the COMMON state on the host; energies and Metropolis sweeps run on the GPU.
+
+Differences from the f2py module (each stated where it applies):
+  * errors raise ``smmp_amd._ffi.SmmpError`` instead of stopping the process;
+  * ``metropolis``'s weight callback must be linear in the energy (the
+    canonical weight, e.g. ``lambda x: x * smmp.bet.beta``: it is evaluated
+    on the host to read off beta) or a weight descriptor from
+    ``smmp_amd.weights``; other callables are rejected, because the weight is
+    evaluated inside the GPU sweep;
+  * ``partem_s`` is exported (partem_s.f:13 has no f2py entry);
+  * observables and minimisers (rgyr, helix, hbond, contacts, zimmer,
+    outpdb, minim, rmsdfun, anneal) are outside the energy-and-move engine
+    and raise NotImplementedError.
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+_here = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(_here))
+from smmp_amd import _ffi, weights  # noqa: E402
+
+_ctx = _ffi.Context()
+
+
+def context() -> _ffi.Context:
+    """The process-wide engine context behind this module."""
+    return _ctx
+
+
+class _Block:
+    """A COMMON block: attribute access yields numpy views of its fields
+    (Fortran layout; scalars as 0-d arrays, as f2py gives them)."""
+
+    def __init__(self, name: str):
+        object.__setattr__(self, "_name", name)
+
+    def _view(self, field: str) -> np.ndarray:
+        v = _ctx.view(f"{self._name}.{field}")
+        return v.reshape(()) if v.shape == (1,) and field in _SCALARS.get(self._name, ()) else v
+
+    def __getattr__(self, field: str):
+        if field.startswith("_"):
+            raise AttributeError(field)
+        return self._view(field)
+
+    def __setattr__(self, field: str, value) -> None:
+        self._view(field)[...] = value
+
+    def __repr__(self) -> str:
+        return f"<smmp COMMON block {self._name}>"
+
+
+# scalar fields (rank 0 in the Fortran declarations)
+_SCALARS = {
+    "epar_l": ("flex", "tesgrd", "sh2", "epsd", "ientyp"),
+    "isolty": ("itysol", "isolscl"),
+    "updchois": ("rndord", "upchswitch", "bgsprob"),
+    "bet": ("beta",),
+    "mol_par": ("nvr",),
+    "mol_i": ("ntlml",),
+    "epar_r": ("conv", "plt", "slp"),
+    "eny_r": ("eyel", "eyvw", "eyhb", "eyvr", "eysm", "eysl"),
+    "pnt": ("npnt",),
+    "v_sol": ("vdvol", "asa"),
+    "myblock": ("mti",),
+}
+
+
+def _blocks() -> list[str]:
+    n = _ffi.lib.smmp_state_count()
+    names = {_ffi.lib.smmp_state_name(i).decode().split(".")[0] for i in range(n)}
+    return sorted(names)
+
+
+for _b in _blocks():
+    globals()[_b] = _Block(_b)
+
+
+# ---------------------------------------------------------------- functions
+def init_energy(libdir: str) -> None:
+    """smmp.pyf:4404, init_energy.f:14 (idempotent here)."""
+    _ctx.init_energy(libdir)
+
+
+def init_molecule(iabin: int, grpn: str, grpc: str, seqfile: str = " ", varfile: str = " ") -> None:
+    """smmp.pyf:5011, init_molecule.f:13."""
+    _ctx.init_molecule(int(iabin), grpn, grpc, seqfile, varfile)
+
+
+def setvar(nml: int, vlvrx) -> None:
+    """smmp.pyf:949, setvar.f:13 (host rebuild of molecule nml)."""
+    _ctx.setvar(int(nml), np.asarray(vlvrx, dtype=np.float64))
+
+
+def energy() -> float:
+    """smmp.pyf:380, energy.f:12: total energy of the current variables (GPU)."""
+    return _ctx.energy()
+
+
+def enysol(nmol: int = 0) -> float:
+    """smmp.pyf:664, enysol.f:13: solvent term of the current coordinates."""
+    return _ctx.enysol(int(nmol))
+
+
+def sgrnd(seed: int) -> None:
+    """smmp.pyf:5620, twister.f:64."""
+    _ctx.sgrnd(int(seed))
+
+
+def grnd() -> float:
+    """twister.f:104."""
+    return _ctx.grnd()
+
+
+def outvar(nml: int, fileout: str = "") -> None:
+    """outvar.f:12: the variables of molecule nml (0 = all) in .var format."""
+    _ctx.outvar(int(nml), fileout)
+
+
+def metropolis(eol1: float, acz: float, dummy) -> tuple[float, float]:
+    """smmp.pyf:22, metropolis.f:13: one sweep over the free variables on the
+    GPU.  ``dummy`` is the weight: a descriptor from smmp_amd.weights, or a
+    callable linear in the energy (its slope is beta)."""
+    return _ctx.metropolis(float(eol1), float(acz), weights.as_descriptor(dummy))
+
+
+def partem_s(num_rep: int, nequi: int, nswp: int, nmes: int = 1, nsave: int = 1, newsta: bool = True,
+             switch: int = 1):
+    """partem_s.f:13: parallel tempering on one random-number stream.
+    Reads the 'temperatures' file of the current directory (partem_s.f:88-95)
+    and appends the energies to 'time.d' every nmes sweeps in the reference's
+    format (end-to-end distance and radius of gyration, outside this engine,
+    are written as 0).  Returns (eol, ipoi, energies per sweep and slot).
+    Restarts (newsta false, par_R.in) are not provided."""
+    if not newsta:
+        raise NotImplementedError("partem_s: restart from par_R.in is not provided")
+    temps = np.zeros(num_rep)
+    with open("temperatures") as f:
+        for _ in range(num_rep):
+            j, t = f.readline().split()[:2]
+            temps[int(j) - 1] = float(t)
+    eol, ipoi, traj = _ctx.partem_s(temps, int(nequi), int(nswp), int(switch))
+    with open("time.d", "w") as out:
+        for sw in range(nswp):
+            if (sw + 1) % max(int(nmes), 1) == 0:
+                for k1 in range(num_rep):
+                    out.write(f"{sw + 1:8d}{k1 + 1:3d}{temps[k1]:12.3f}{traj[sw, k1]:12.3f}{0.0:12.3f}{0.0:12.3f}\n")
+    return eol, ipoi, traj
+
+
+def _outside(name: str):
+    def f(*args, **kwargs):
+        raise NotImplementedError(f"smmp.{name}: outside the energy-and-move engine")
+    f.__name__ = name
+    return f
+
+
+for _n in ("rgyr", "helix", "hbond", "interhbond", "contacts", "zimmer", "outpdb", "minim", "rmsdfun",
+           "anneal", "canon", "regul", "gradient", "esolan", "eninteract"):
+    globals()[_n] = _outside(_n)
+
+
+# ----------------------------------------------------------- multicanonical
+class _Multicanonical:
+    """The f2py module 'multicanonical' (mulcan_par_mod.f90): weight
+    iteration and multicanonical simulation of the current molecule, one
+    chain on the context's random-number stream, sweeps on the GPU."""
+
+    def __init__(self):
+        self.b = self.alpha = None
+        self.kmin = self.kmax = 0
+        self.ebin = 1.0
+
+    @staticmethod
+    def _random_start() -> None:
+        # mulcan_par_mod.f90:93-98: vlvr(idvr(i)) = addang(pi, axvr (grnd() - 1/2))
+        vl = _ctx.vlvr()
+        ax = _ctx.view("var_r.axvr")
+        for iv in _ctx.idvr:
+            dv = float(ax[iv - 1]) * (grnd() - 0.5)
+            vl[iv - 1] = _addang(np.pi, dv)
+        _ctx.set_vlvr(vl)
+
+    def mulcan_par(self, nsweep: int, nup: int, temp: float, kmin: int, kmax: int, binWidth: float,
+                   l_iter: bool = False):
+        """mulcan_par_mod.f90:23-190.  Files as the reference: mpar_full.d
+        (i, xhist, g1, b, alpha, ent per bin after each update; read back when
+        l_iter) and muca.d (i, b, alpha of the final weights).  The sweeps use
+        muca_weight, (beta + b(k)) x + alpha(k), with beta of COMMON /bet/."""
+        n = kmax - kmin + 1
+        beta_t = 1.0 / (temp * 1.98773e-3)
+        ihist = np.zeros(n, dtype=np.int32)
+        if l_iter:
+            rows = np.loadtxt("mpar_full.d", ndmin=2)
+            xhist = np.zeros(n); g1 = np.zeros(n); b = np.zeros(n); alpha = np.zeros(n); ent = np.zeros(n)
+            for r in rows[:n]:
+                k = int(r[0]) - kmin
+                xhist[k], g1[k], b[k], alpha[k], ent[k] = r[1:6]
+        else:
+            xhist, g1, b, alpha, ent = (np.zeros(n) for _ in range(5))
+        self._random_start()
+        eol = energy()
+        acz = 0.0
+        w = weights.Multicanonical(b, alpha, kmin, kmax, binWidth, beta=float(bet.beta))
+        for nsw in range(1, nsweep + 1):
+            w.beta = float(bet.beta)
+            eol, acz = metropolis(eol, acz, w)
+            ihist[w.bin(eol) - kmin] += 1
+            if nsw % nup == 0:
+                _ffi.lib.smmp_muca_update(kmin, kmax, binWidth, beta_t, _ffi.iptr(ihist), _ffi.dptr(xhist),
+                                          _ffi.dptr(g1), _ffi.dptr(b), _ffi.dptr(alpha), _ffi.dptr(ent))
+                with open("mpar_full.d", "w") as f:
+                    for k in range(kmax, kmin - 1, -1):
+                        i = k - kmin
+                        f.write(f"{k:5d}{xhist[i]:15.4f}{g1[i]:15.4f}{b[i]:15.4f}{alpha[i]:15.4f}{ent[i]:15.4f}\n")
+        _ffi.lib.smmp_muca_final(kmin, kmax, binWidth, beta_t, _ffi.dptr(b), _ffi.dptr(alpha))
+        with open("muca.d", "w") as f:
+            for k in range(kmin, kmax + 1):
+                f.write(f"{k} {float(b[k - kmin])!r} {float(alpha[k - kmin])!r}\n")
+        self.b, self.alpha, self.kmin, self.kmax, self.ebin = b, alpha, kmin, kmax, binWidth
+        return eol, acz / (nsweep * int(mol_par.nvr)), xhist
+
+    def mulcan_sim(self, nequi: int, nsweep: int, nmes: int, nsave: int, kmin: int, kmax: int, binWidth: float,
+                   restart: bool = False):
+        """mulcan_par_mod.f90:193-377 with the weights of muca.d (muca_weight2,
+        b(k) x + alpha(k)).  start.d holds the restart state (sweep, energy,
+        histogram, variables) every nsave sweeps; time.d gets 'nsw, eol' every
+        nmes sweeps with the contact columns (outside this engine) as 0."""
+        n = kmax - kmin + 1
+        b = np.zeros(n); alpha = np.zeros(n)
+        for r in np.loadtxt("muca.d", ndmin=2)[:n]:
+            b[int(r[0]) - kmin], alpha[int(r[0]) - kmin] = r[1], r[2]
+        w = weights.Multicanonical(b, alpha, kmin, kmax, binWidth)
+        ihist = np.zeros(n, dtype=np.int64)
+        xhist = np.zeros(n)
+        nswm = 1
+        if restart:
+            with open("start.d") as f:
+                nswm = int(float(f.readline().split()[0]))
+                xhist[:] = [float(x) for x in f.readline().split()]
+                vl = _ctx.vlvr()
+                for _ in range(int(mol_par.nvr)):
+                    j, x = f.readline().split()
+                    vl[_ctx.idvr[int(j) - 1] - 1] = float(x)
+                _ctx.set_vlvr(vl)
+        else:
+            self._random_start()
+        eol = energy()
+        acz = 0.0
+        if not restart:
+            for _ in range(nequi):
+                eol, acz = metropolis(eol, acz, w)
+        acz = 0.0
+        with open("time.d", "w") as tf:
+            for nsw in range(nswm, nsweep + 1):
+                eol, acz = metropolis(eol, acz, w)
+                ihist[w.bin(eol) - kmin] += 1
+                if nsw % nsave == 0:
+                    vl = _ctx.vlvr()
+                    with open("start.d", "w") as f:
+                        f.write(f"{nswm} {float(eol)!r}\n" + " ".join(repr(float(x)) for x in xhist) + "\n")
+                        for i, iv in enumerate(_ctx.idvr, 1):
+                            f.write(f"{i} {float(vl[iv - 1])!r}\n")
+                if nsw % nmes == 0:
+                    xhist += ihist
+                    ihist[:] = 0
+                    tf.write(f"{nsw:7d}{eol:12.2f}{0:8d}{0:8d}{0.0:12.4f}\n")
+        return eol, acz / (max(nsweep - nswm + 1, 1) * int(mol_par.nvr)), xhist
+
+
+multicanonical = _Multicanonical()
+
+
+def _addang(a1: float, a2: float) -> float:
+    """addang of difang.f:38-63: a1 + a2 wrapped into (-pi, pi]."""
+    d = np.fmod(a1 + a2, 2.0 * np.pi)
+    return float(d if abs(d) <= np.pi else d - np.copysign(2.0 * np.pi, d))

@@ -69,6 +69,8 @@ def _load():
         "smmp_profile": (C.c_int, [P, C.c_int]),
         "smmp_profile_read": (C.c_int, [P, C.c_int, dp, C.POINTER(C.c_int64), dp]),
         "smmp_profile_phases": (C.c_int, [P, dp]),
+        "smmp_muca_update": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, ip, dp, dp, dp, dp, dp]),
+        "smmp_muca_final": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, dp, dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -84,7 +86,7 @@ EXPORTS = ("smmp_ctx_create", "smmp_ctx_destroy", "smmp_last_error", "smmp_state
            "smmp_state_name", "smmp_init_energy", "smmp_init_molecule", "smmp_setvar", "smmp_energy",
            "smmp_enysol", "smmp_sgrnd", "smmp_grnd", "smmp_outvar", "smmp_metropolis", "smmp_partem_s",
            "smmp_energy_batch", "smmp_sample", "smmp_build_info", "smmp_profile", "smmp_profile_read",
-           "smmp_profile_phases")
+           "smmp_profile_phases", "smmp_muca_update", "smmp_muca_final")
 
 KERNEL_SWEEP, KERNEL_EXCHANGE, KERNEL_ENERGY = 0, 1, 2
 
@@ -174,6 +176,18 @@ class Context:
 
     def vlvr(self) -> np.ndarray:
         return self.view("var_r.vlvr")[:self.nvt].copy()
+
+    def set_vlvr(self, vl) -> None:
+        """Writes the variables (the next energy()/metropolis() builds from them)."""
+        self.view("var_r.vlvr")[:self.nvt] = np.asarray(vl, dtype=np.float64)[:self.nvt]
+
+    def enysol(self, nmol: int = 0) -> float:
+        e = C.c_double()
+        self.check(lib.smmp_enysol(self.h, int(nmol), C.byref(e)))
+        return e.value
+
+    def outvar(self, nml: int, path: str = "") -> None:
+        self.check(lib.smmp_outvar(self.h, int(nml), path.encode()))
 
     def setvar(self, nml: int, vl) -> None:
         full = np.array(self.view("var_r.vlvr"), dtype=np.float64)
