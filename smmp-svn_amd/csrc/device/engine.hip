@@ -108,7 +108,7 @@ struct DevMem {
 // Replica state of the samplers (grown on demand, reused across calls).
 struct SamplerBuffers {
   DevMem idvr, axvr, vlvr, xyz, nfree, valid, eol, acc, mt, rng_of, slot_of, beta, err, dbg;
-  DevMem wb, walpha, hist, traj, ipoi, iswitch, work, trace, trace_n, phase, part, sync;
+  DevMem wb, walpha, hist, traj, ipoi, iswitch, work, trace, trace_n, phase, part, sync, gsync;
 };
 
 // One system uploaded to the device: a single allocation holding every

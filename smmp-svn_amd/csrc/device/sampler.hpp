@@ -100,6 +100,14 @@ struct SamplerParams {
   double* part;          // [R][2][G][kPartVals] partial sums
   uint32_t* sync;        // [R] arrival counters, zero at launch
   uint64_t wait_ticks;   // bound of one wait, device wall-clock ticks
+  // Parallel tempering inside one persistent launch: after every sweep all
+  // blocks meet (grid barrier), block 0 attempts the exchanges, all meet
+  // again and read their new temperature slot.  Cooperative launch.
+  int32_t pt_exchange;
+  int32_t* ipoi;         // slot -> replica (1-based), partem_s.f:209-232
+  int32_t* iswitch;      // which half of the pairs is tried next
+  uint32_t* mt_x;        // the exchange stream (625 words)
+  uint32_t* gsync;       // grid-barrier arrival counter, zero at launch
 };
 constexpr int kPartVals = 8;
 constexpr int kSweepThreads = 512;  // threads of one sweep workgroup (8 waves; 16 spill and run slower)
@@ -437,6 +445,56 @@ __device__ void group_sum(const SamplerParams& P, int r, int g, uint32_t& epoch,
   ++epoch;
 }
 
+// Temperature exchanges between neighbouring slots (partem_s.f:209-232):
+// pairs (1,2),(3,4).. then (2,3),..,(R,1) alternately; the random number is
+// drawn for every pair.  One thread; stream mt_x.
+__device__ void exchange_step(int nrep, int32_t* ipoi, int32_t* slot_of, const double* beta, const double* eol,
+                              uint32_t* mt_x, int32_t* iswitch) {
+  int num_rep1, nu;
+  if (*iswitch == 1) {
+    num_rep1 = nrep - 1;
+    nu = 1;
+    *iswitch = 2;
+  } else {
+    num_rep1 = nrep;
+    nu = 2;
+    *iswitch = 1;
+  }
+  for (int i = nu; i <= num_rep1; i += 2) {
+    int j = i + 1;
+    if (i == nrep) j = 1;
+    const int in = ipoi[i - 1], jn = ipoi[j - 1];
+    const double bi = beta[i - 1], bj = beta[j - 1];
+    const double delta = -bi * eol[jn - 1] - bj * eol[in - 1] + bi * eol[in - 1] + bj * eol[jn - 1];
+    const double ra = mt_grnd(mt_x, &mt_x[624]);
+    if (ra <= exp(delta)) {
+      ipoi[i - 1] = jn;
+      ipoi[j - 1] = in;
+      slot_of[jn - 1] = i - 1;
+      slot_of[in - 1] = j - 1;
+    }
+  }
+}
+
+// All blocks of the launch meet (thread 0 of each; the callers hold block
+// barriers on either side).  Bounded on the device clock like group_sum.
+__device__ void grid_sync(const SamplerParams& P, uint32_t& gepoch) {
+  const uint32_t nb = gridDim.x;
+  __hip_atomic_fetch_add(P.gsync, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  const uint32_t target = (gepoch + 1u) * nb;
+  const uint64_t t0 = wall_clock64();
+  while (__hip_atomic_load(P.gsync, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    __builtin_amdgcn_s_sleep(2);
+    if (wall_clock64() - t0 > P.wait_ticks ||
+        (__hip_atomic_load(P.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & 8)) {
+      atomicOr(P.err, 8);
+      break;
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  ++gepoch;
+}
+
 // P.nsweep Metropolis sweeps of one replica per group of G blocks.
 template <bool MIXED>
 __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
@@ -454,7 +512,8 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
   double* gx = P.xyz + (size_t)r * 3 * np;
   int32_t* gnf = P.nfree + (size_t)r * nh;
   uint32_t* gmt = P.mt + (size_t)P.rng_of[r] * 625;
-  uint32_t epoch = 0;  // reductions done (thread 0)
+  uint32_t epoch = 0;   // reductions done (thread 0)
+  uint32_t gepoch = 0;  // grid barriers done (thread 0)
 
   energy_stage_static(S, L);
   stage_variables(P, S, Q.V);
@@ -829,9 +888,19 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
           hv = fmin(fmax(hv, (double)P.kmin), (double)P.kmax);
           atomicAdd(&P.hist[(int)hv - P.kmin], 1);
         }
+        if (P.pt_exchange) P.eol[r] = enw;
       }
     }
     __syncthreads();
+    if (P.pt_exchange) {  // exchange attempt of this sweep (partem_s.f:204-232)
+      if (tid == 0) grid_sync(P, gepoch);
+      __syncthreads();
+      if (blockIdx.x == 0 && tid == 0 && P.nrep > 1)
+        exchange_step(P.nrep, P.ipoi, const_cast<int32_t*>(P.slot_of), P.beta, P.eol, P.mt_x, P.iswitch);
+      __syncthreads();
+      if (tid == 0) grid_sync(P, gepoch);
+      __syncthreads();
+    }
     lap(kPhSweepEnd);
   }
   if (P.phase && tid == 0 && g == 0) {
@@ -873,36 +942,11 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
   }
 }
 
-// Temperature exchanges between neighbouring slots (partem_s.f:209-232):
-// pairs (1,2),(3,4).. then (2,3),..,(R,1) alternately; the random number is
-// drawn for every pair.  One thread; stream 'xs' of P.mt.
+// One exchange attempt as its own launch (the per-sweep launch path).
 __global__ void k_exchange(int nrep, int32_t* ipoi, int32_t* slot_of, const double* beta, const double* eol,
                            uint32_t* mt_x, int32_t* iswitch) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  int num_rep1, nu;
-  if (*iswitch == 1) {
-    num_rep1 = nrep - 1;
-    nu = 1;
-    *iswitch = 2;
-  } else {
-    num_rep1 = nrep;
-    nu = 2;
-    *iswitch = 1;
-  }
-  for (int i = nu; i <= num_rep1; i += 2) {
-    int j = i + 1;
-    if (i == nrep) j = 1;
-    const int in = ipoi[i - 1], jn = ipoi[j - 1];
-    const double bi = beta[i - 1], bj = beta[j - 1];
-    const double delta = -bi * eol[jn - 1] - bj * eol[in - 1] + bi * eol[in - 1] + bj * eol[jn - 1];
-    const double ra = mt_grnd(mt_x, &mt_x[624]);
-    if (ra <= exp(delta)) {
-      ipoi[i - 1] = jn;
-      ipoi[j - 1] = in;
-      slot_of[jn - 1] = i - 1;
-      slot_of[in - 1] = j - 1;
-    }
-  }
+  exchange_step(nrep, ipoi, slot_of, beta, eol, mt_x, iswitch);
 }
 
 }  // namespace smmp
