@@ -108,6 +108,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-MiB device buffer 5 times (calibrates HBM byte counters) and exit")
+    ap.add_argument("--groups", type=int, default=0,
+                    help="workgroups per replica: 0 automatic, -1 measured (autotuner), >0 upper bound")
+    ap.add_argument("--graph-sweeps", type=int, default=0,
+                    help="replay a captured HIP graph of this many sweeps (must divide --steps and --warmup)")
     ap.add_argument("--phases", action="store_true",
                     help="also time the sweep kernel's phases on the device clock (adds barriers)")
     args = ap.parse_args()
@@ -136,15 +140,19 @@ def main():
     seeds_w = np.arange(1, NREP + 2, dtype=np.uint32)
     seeds_t = np.arange(1001, 1001 + NREP + 1, dtype=np.uint32)
     log(f"warmup {args.warmup} sweeps")
-    out = c.sample(temps, seeds_w, args.warmup, v0, mode=1, precision=args.precision, with_traj=False)
+    kw = dict(mode=1, precision=args.precision, with_traj=False, groups=args.groups, graph_sweeps=args.graph_sweeps)
+    out = c.sample(temps, seeds_w, args.warmup, v0, **kw)
     v1 = out["vlvr"]
     c.profile(2 if args.phases else 1)
     torch.cuda.synchronize()
     log(f"timed {args.steps} sweeps")
     t0 = time.perf_counter()
-    out = c.sample(temps, seeds_t, args.steps, v1, mode=1, precision=args.precision, with_traj=False)
+    out = c.sample(temps, seeds_t, args.steps, v1, **kw)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    info = c.sampler_info()
+    log(f"launch: {info['groups']} workgroups per replica, persistent {info['persistent']}, "
+        f"graph replays {info['graph_replays']} (kept graph {info['graph_reused']})")
     ks = c.profile_read(_ffi.KERNEL_SWEEP)
     kx = c.profile_read(_ffi.KERNEL_EXCHANGE)
     ntrial = NREP * args.steps * nvr
@@ -186,7 +194,8 @@ def main():
         "config": {"workload": f"parallel tempering, {NREP} replicas x {LENGTH}-residue random sequence, "
                                f"ECEPP/3+OONS, T geometric {TMIN:g}-{TMAX:g} K, exchange every sweep",
                    "replicas": NREP, "residues": LENGTH, "free_torsions": nvr, "solvent": "OONS",
-                   "precision": "fp64" if args.precision == 0 else "mixed"},
+                   "precision": "fp64" if args.precision == 0 else "mixed",
+                   "workgroups_per_replica": info["groups"]},
         "energy_evals_per_s_equiv": round(value * (nvr + 2), 1),
         "roofline": {"bound": "valu", "achieved": round(achieved, 4), "peak": PEAK_FP64, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP64, 5), "traffic": None,

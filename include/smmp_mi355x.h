@@ -124,12 +124,28 @@ typedef struct {
   int32_t resync_every;   /* full recomputation cadence in sweeps (1 = every sweep) */
   int32_t verify;         /* 1: recompute every trial in full and fail on the first
                              incremental component that disagrees (slow; tests) */
-  int32_t groups;         /* workgroups per replica (upper bound; 0 = automatic).  With
+  int32_t groups;         /* workgroups per replica (upper bound; 0 = automatic,
+                             -1 = measured: 1, 2, 4, 8 are timed on the call's own
+                             replicas and the fastest is kept for later calls on the
+                             same system, replica count, precision and mode).  With
                              more than one, the workgroups of a replica split each
                              energy change and reduce it on the device every trial */
+  int32_t graph_sweeps;   /* > 0: the launches of that many sweeps (and their
+                             exchanges) are captured once into a HIP graph and
+                             replayed nsweep / graph_sweeps times (nsweep must be a
+                             multiple); the graph stays with the context and later
+                             calls with the same configuration replay it without
+                             capturing.  0: launched directly. */
 } smmp_sampler_cfg;
 int smmp_sample(smmp_ctx *ctx, const smmp_sampler_cfg *cfg, double *vlvr, double *eol,
                 double *acc, int32_t *ipoi, int32_t *hist, double *traj);
+
+/* How the last smmp_sample call ran (no reference counterpart):
+ * out[0] workgroups per replica, out[1] 1 if the whole run was one
+ * persistent launch per graph replay or call (parallel tempering with the
+ * exchanges inside the kernel), out[2] graph replays (0: direct launches),
+ * out[3] 1 if the graph was reused from an earlier call.  out holds 4. */
+int smmp_sampler_info(smmp_ctx *ctx, int32_t *out);
 
 /* Multicanonical weight iteration (host arithmetic; arrays indexed from
  * kmin, kmax-kmin+1 values each).

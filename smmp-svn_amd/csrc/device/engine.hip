@@ -109,6 +109,7 @@ struct DevMem {
 struct SamplerBuffers {
   DevMem idvr, axvr, vlvr, xyz, nfree, valid, eol, acc, mt, rng_of, slot_of, beta, err, dbg;
   DevMem wb, walpha, hist, traj, ipoi, iswitch, work, trace, trace_n, phase, part, sync, gsync;
+  DevMem sweep_base, tune_vl, tune_mt;
 };
 
 // One system uploaded to the device: a single allocation holding every
@@ -235,7 +236,15 @@ struct Engine {
   SamplerBuffers smp;           // replica state of the samplers
   std::vector<std::unique_ptr<DevSystem>> batch_sys;  // energy_batch systems
   Profiler prof;
+  uint64_t sys_gen = 0;         // counts uploads of the context's system
+  bool capturing = false;       // launches are being recorded into a graph
+  // captured sampler launches and what they were captured for
+  hipGraphExec_t gexec = nullptr;
+  std::vector<uint8_t> gkey;
+  std::map<std::vector<int64_t>, int> tuned_groups;  // (system, R, precision, mode) -> G
+  int32_t info[4] = {0, 0, 0, 0};                     // how the last sampler call ran
   ~Engine() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
     for (hipEvent_t ev : prof.pool) (void)hipEventDestroy(ev);
     for (auto& p : prof.pending) {
       (void)hipEventDestroy(p.second.first);
@@ -257,7 +266,7 @@ static hipEvent_t prof_event(Engine& E) {
 }
 // Brackets one launch: call with start=true before and start=false after.
 static void prof_mark(Engine& E, int kid, bool start) {
-  if (!E.prof.on) return;
+  if (!E.prof.on || E.capturing) return;
   hipEvent_t ev = prof_event(E);
   HIPCHK(hipEventRecord(ev, E.stream));
   if (start) {
@@ -321,6 +330,7 @@ static DevSystem& current_system(Engine& E, const State& s) {
   if (!E.valid || !gb_same) {
     E.sys.h = build_host_sys(s);
     upload_system(E.sys, E.stream);
+    ++E.sys_gen;
     DevSys* dp = static_cast<DevSys*>(E.d_sys.ensure(sizeof(DevSys)));
     HIPCHK(hipMemcpyAsync(dp, &E.sys.d, sizeof(DevSys), hipMemcpyHostToDevice, E.stream));
     HIPCHK(hipStreamSynchronize(E.stream));
@@ -401,6 +411,12 @@ int engine_profile_phases(Engine*& e, double* out) {
     const bool count = k == kPhNFull || k == kPhCoreCycles || k == kPhSolAtoms || k == kPhSolContacts;
     out[k] = (count || E.prof.tick_khz <= 0.0) ? t : t / E.prof.tick_khz;
   }
+  return SMMP_OK;
+}
+
+int engine_sampler_info(Engine*& e, int32_t* out) {
+  Engine& E = engine_of(e);
+  for (int k = 0; k < 4; ++k) out[k] = E.info[k];
   return SMMP_OK;
 }
 

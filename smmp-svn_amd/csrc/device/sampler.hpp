@@ -108,7 +108,20 @@ struct SamplerParams {
   int32_t* iswitch;      // which half of the pairs is tried next
   uint32_t* mt_x;        // the exchange stream (625 words)
   uint32_t* gsync;       // grid-barrier arrival counter, zero at launch
+  // Sweeps already done when launches are replayed from a graph (a kernel
+  // argument cannot change between replays); null: traj_base.
+  int32_t* sweep_base;
 };
+
+// Fills n words (graph-safe zeroing of counters: no memset nodes).
+__global__ void k_fill_u32(uint32_t* p, int n, uint32_t v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+// Advances the sweep counter of graph replays past a finished launch.
+__global__ void k_add_i32(int32_t* p, int v) {
+  if (threadIdx.x == 0) *p += v;
+}
 constexpr int kPartVals = 8;
 constexpr int kSweepThreads = 512;  // threads of one sweep workgroup (8 waves; 16 spill and run slower)
 
@@ -571,6 +584,9 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
     }
   };
 
+  // first trajectory row of this launch (the counter advances in a launch
+  // after this one)
+  const int traj_base = P.sweep_base ? *P.sweep_base : P.traj_base;
   for (int sw = 0; sw < P.nsweep; ++sw) {
     if (tid == 0) Q.sc[14] = P.beta[P.slot_of[r]];  // LDS: no global reload inside the loop
     for (int nsw = 0; nsw < P.nvr; ++nsw) {
@@ -722,23 +738,31 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
               err = 1;
               nfree = Q.nf[h];
             }
-            if (lane == 0) {
-              Q.nf2[h] = nfree;
-              const double trad = L.rv[i];
-              dsol += L.sg[i] * (c4 * trad * trad * (double)(nfree - Q.nf[h]));
-            }
+            if (lane == 0) Q.nf2[h] = nfree;
           }
           if (wst && lane == 0) wst[3] = (double)wall_clock64() - tw0;
         }
-        double v[4] = {d.el, d.vw, d.hb, dsol};
+        double v[3] = {d.el, d.vw, d.hb};
 #pragma unroll
-        for (int k = 0; k < 4; ++k) v[k] = wave_sum(v[k]);
+        for (int k = 0; k < 3; ++k) v[k] = wave_sum(v[k]);
         const int errw = __any(err);
         if (lane == 0) {
-          for (int k = 0; k < 4; ++k) L.red[8 * wave + k] = v[k];
+          for (int k = 0; k < 3; ++k) L.red[8 * wave + k] = v[k];
           L.red[8 * wave + 6] = (double)errw;
         }
         __syncthreads();
+        if (S.itysol > 0 && wave == 0) {
+          // summed in list order, so the result does not depend on which
+          // wave took which atom
+          const int na = (int)Q.sc[29];
+          for (int k = lane; k < na; k += kWave) {
+            const int h = Q.alist[k];
+            const int i = L.hv[h];
+            const double trad = L.rv[i];
+            dsol += L.sg[i] * (c4 * trad * trad * (double)(Q.nf2[h] - Q.nf[h]));
+          }
+          dsol = wave_sum(dsol);
+        }
         lap(kPhSolvent);
         if (P.phase && tid == 0 && S.itysol > 0) {
           double mx = 0.0;
@@ -753,9 +777,9 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
           ph[kPhSolMax] += mx;
         }
         if (tid == 0) {
-          double tot[6] = {0, 0, 0, 0, dt, 0};  // el, vw, hb, sol, torsion, error
+          double tot[6] = {0, 0, 0, dsol, dt, 0};  // el, vw, hb, sol, torsion, error
           for (int w = 0; w < nwaves; ++w) {
-            for (int k = 0; k < 4; ++k) tot[k] += L.red[8 * w + k];
+            for (int k = 0; k < 3; ++k) tot[k] += L.red[8 * w + k];
             tot[5] += L.red[8 * w + 6];
           }
           group_sum(P, r, g, epoch, tot, 6);
@@ -882,7 +906,7 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
       Q.sc[0] = enw;
       if (g == 0) {
         const int slot = P.slot_of[r];
-        if (P.traj) P.traj[(size_t)(P.traj_base + sw) * P.nrep + slot] = enw;
+        if (P.traj) P.traj[(size_t)(traj_base + sw) * P.nrep + slot] = enw;
         if (P.hist) {
           double hv = enw / P.ebin + copysign(0.5, enw);
           hv = fmin(fmax(hv, (double)P.kmin), (double)P.kmax);

@@ -273,6 +273,11 @@ int smmp_profile_read(smmp_ctx* c, int kernel, double* total_ms, int64_t* launch
   return guarded(c, SMMP_EDEVICE, [&] { return engine_profile_read(c->eng, kernel, total_ms, launches, work); });
 }
 
+int smmp_sampler_info(smmp_ctx* c, int32_t* out) {
+  if (!out) return SMMP_EINVAL;
+  return guarded(c, SMMP_EDEVICE, [&] { return engine_sampler_info(c->eng, out); });
+}
+
 int smmp_profile_phases(smmp_ctx* c, double* out) {
   if (!out) return SMMP_EINVAL;
   return guarded(c, SMMP_EDEVICE, [&] { return engine_profile_phases(c->eng, out); });

@@ -41,6 +41,7 @@ const char* engine_build_info();
 void engine_profile(Engine*& e, int enable);
 int engine_profile_read(Engine*& e, int kernel, double* ms, int64_t* launches, double* work);
 int engine_profile_phases(Engine*& e, double* out);
+int engine_sampler_info(Engine*& e, int32_t* out);
 
 // twister.f:104 grnd() on the context's stream (host side; the draws of the
 // device samplers come from the same state copied to the device)

@@ -36,7 +36,7 @@ class SamplerCfg(C.Structure):
     _fields_ = [("nrep", C.c_int32), ("nsweep", C.c_int32), ("mode", C.c_int32), ("precision", C.c_int32),
                 ("temps", C.POINTER(C.c_double)), ("seeds", C.POINTER(C.c_uint32)),
                 ("w", C.POINTER(Weight)), ("resync_every", C.c_int32), ("verify", C.c_int32),
-                ("groups", C.c_int32)]
+                ("groups", C.c_int32), ("graph_sweeps", C.c_int32)]
 
 
 def _load():
@@ -69,6 +69,7 @@ def _load():
         "smmp_profile": (C.c_int, [P, C.c_int]),
         "smmp_profile_read": (C.c_int, [P, C.c_int, dp, C.POINTER(C.c_int64), dp]),
         "smmp_profile_phases": (C.c_int, [P, dp]),
+        "smmp_sampler_info": (C.c_int, [P, ip]),
         "smmp_muca_update": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, ip, dp, dp, dp, dp, dp]),
         "smmp_muca_final": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, dp, dp]),
     }
@@ -86,7 +87,7 @@ EXPORTS = ("smmp_ctx_create", "smmp_ctx_destroy", "smmp_last_error", "smmp_state
            "smmp_state_name", "smmp_init_energy", "smmp_init_molecule", "smmp_setvar", "smmp_energy",
            "smmp_enysol", "smmp_sgrnd", "smmp_grnd", "smmp_outvar", "smmp_metropolis", "smmp_partem_s",
            "smmp_energy_batch", "smmp_sample", "smmp_build_info", "smmp_profile", "smmp_profile_read",
-           "smmp_profile_phases", "smmp_muca_update", "smmp_muca_final")
+           "smmp_profile_phases", "smmp_muca_update", "smmp_muca_final", "smmp_sampler_info")
 
 KERNEL_SWEEP, KERNEL_EXCHANGE, KERNEL_ENERGY = 0, 1, 2
 
@@ -200,6 +201,13 @@ class Context:
         """enable: False/0 off, True/1 kernel timing and work, 2 also sweep phases."""
         self.check(lib.smmp_profile(self.h, int(enable)))
 
+    def sampler_info(self) -> dict:
+        """How the last sample() ran (smmp_sampler_info)."""
+        out = np.zeros(4, dtype=np.int32)
+        self.check(lib.smmp_sampler_info(self.h, iptr(out)))
+        return {"groups": int(out[0]), "persistent": bool(out[1]), "graph_replays": int(out[2]),
+                "graph_reused": bool(out[3])}
+
     def profile_phases(self) -> dict:
         out = np.zeros(20)
         self.check(lib.smmp_profile_phases(self.h, dptr(out)))
@@ -245,8 +253,10 @@ class Context:
 
     def sample(self, temps, seeds, nsweep: int, vlvr, mode: int = 0, weight: Weight | None = None,
                precision: int = 0, with_traj: bool = True, allow_drift: bool = False, verify: bool = False,
-               groups: int = 0):
-        """Independent-stream replica sampler (smmp_sample)."""
+               groups: int = 0, graph_sweeps: int = 0):
+        """Independent-stream replica sampler (smmp_sample).  groups: workgroups
+        per replica (0 automatic, -1 measured); graph_sweeps: sweeps per
+        replay of a captured graph (0: direct launches)."""
         temps = np.ascontiguousarray(temps, dtype=np.float64)
         R = len(temps)
         seeds = np.ascontiguousarray(seeds, dtype=np.uint32)
@@ -263,7 +273,7 @@ class Context:
         cfg = SamplerCfg(nrep=R, nsweep=nsweep, mode=mode, precision=precision,
                          temps=dptr(temps), seeds=seeds.ctypes.data_as(C.POINTER(C.c_uint32)),
                          w=C.pointer(weight) if weight is not None else None, resync_every=1,
-                         verify=1 if verify else 0, groups=groups)
+                         verify=1 if verify else 0, groups=groups, graph_sweeps=graph_sweeps)
         rc = lib.smmp_sample(self.h, C.byref(cfg), dptr(vl), dptr(eol), dptr(acc), iptr(ipoi),
                              iptr(hist) if weight is not None else None,
                              dptr(traj) if traj is not None else None)

@@ -254,3 +254,74 @@ def test_mixed_precision_sampler_runs_and_tracks_fp64():
         worst = max(worst, err / (scale + 1.0))
         assert err <= 2e-5 * scale + 1e-3, (r, out["eol"][r], ref[0])
     say(f"mixed: worst |E - E_fp64| / (|el|+|vw|+|hb|+1) = {worst:.2e}")
+
+
+def _pt_inputs(c, R, seed):
+    temps = np.geomspace(250.0, 1000.0, R)
+    seeds = np.arange(7, 7 + R + 1, dtype=np.uint32)
+    v0 = np.tile(c.vlvr(), (R, 1))
+    v0[:, c.idvr - 1] = np.random.default_rng(seed).uniform(-np.pi, np.pi, size=(R, c.nvr))
+    return temps, seeds, v0
+
+
+def _same_run(a, b):
+    for k in ("acc", "ipoi", "eol", "vlvr", "traj"):
+        assert np.array_equal(a[k], b[k]), k
+
+
+@pytest.mark.parametrize("launch", ["persistent", "per-sweep", "canonical"])
+def test_graph_replay_matches_direct_launches(launch, monkeypatch):
+    """Sweeps replayed from a captured HIP graph (3 sweeps per replay) give
+    the run of direct launches bit for bit; a later call with the same
+    configuration replays the kept graph, and its trajectory rows start
+    again at sweep 0."""
+    if launch == "per-sweep":
+        monkeypatch.setenv("SMMP_PT_LAUNCH", "per-sweep")
+    c = make_ctx(tag="c2_20mer", itysol=1)
+    R = 8
+    mode = 0 if launch == "canonical" else 1
+    temps, seeds, v0 = _pt_inputs(c, R, 31)
+    say(f"smmp_sample c2_20mer R=8 12 sweeps {launch}: direct vs graph of 3 sweeps")
+    a = c.sample(temps, seeds, 12, v0, mode=mode, groups=4)
+    assert c.sampler_info()["graph_replays"] == 0
+    b = c.sample(temps, seeds, 12, v0, mode=mode, groups=4, graph_sweeps=3)
+    info = c.sampler_info()
+    assert info["graph_replays"] == 4 and not info["graph_reused"]
+    assert info["persistent"] == (launch == "persistent")
+    _same_run(a, b)
+    say("second graph call (kept graph)")
+    b2 = c.sample(temps, seeds, 12, v0, mode=mode, groups=4, graph_sweeps=3)
+    assert c.sampler_info()["graph_reused"]
+    _same_run(a, b2)
+    say("6 sweeps: direct vs graph")
+    a6 = c.sample(temps, seeds, 6, v0, mode=mode, groups=4)
+    b6 = c.sample(temps, seeds, 6, v0, mode=mode, groups=4, graph_sweeps=3)
+    assert c.sampler_info()["graph_replays"] == 2
+    _same_run(a6, b6)
+    assert np.array_equal(a6["traj"], a["traj"][:6])
+
+
+def test_graph_sweeps_must_divide_nsweep():
+    c = make_ctx(tag="len10", itysol=1)
+    temps, seeds, v0 = _pt_inputs(c, 2, 3)
+    with pytest.raises(RuntimeError, match="graph_sweeps"):
+        c.sample(temps, seeds, 7, v0, mode=1, graph_sweeps=3)
+
+
+def test_measured_groups_keep_the_start_state():
+    """groups=-1 times 1, 2, 4, 8 workgroups per replica on the call's own
+    replicas and puts their start state back: the run equals an explicit
+    run at the chosen count, and the choice is kept for the next call."""
+    c = make_ctx(tag="c2_20mer", itysol=1)
+    R = 16
+    temps, seeds, v0 = _pt_inputs(c, R, 44)
+    say("smmp_sample c2_20mer R=16 5 sweeps groups=-1")
+    t = c.sample(temps, seeds, 5, v0, mode=1, groups=-1)
+    G = c.sampler_info()["groups"]
+    say(f"measured groups: {G}")
+    assert G in (1, 2, 4, 8)
+    e = c.sample(temps, seeds, 5, v0, mode=1, groups=G)
+    _same_run(t, e)
+    t2 = c.sample(temps, seeds, 5, v0, mode=1, groups=-1)
+    assert c.sampler_info()["groups"] == G
+    _same_run(t2, e)
