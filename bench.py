@@ -38,6 +38,11 @@ FLOP_PAIR = 22      # ECEPP pair term: 3 sub + 5 r^2 + 1 rsqrt + 3 powers + 2 Co
 FLOP_POINT = 9      # one tessellation point against one solvent neighbour
 FLOP_BUILD = 60     # one atom placed by the Eyring step
 FLOP_ROTATE = 21    # one atom rotated rigidly about a bond (Rodrigues)
+# HBM-side bytes per sweep of this workload in FP64: FETCH_SIZE x 2.0 plus
+# WRITE_SIZE x 1.0 (the factors make a 256 MiB device copy read 256 MiB and
+# write 256 MiB) of the timed launch, rocprofv3 --pmc, 1000 sweeps:
+# 4,587,491 KiB + 9,686,131 KiB.
+HBM_BYTES_PER_SWEEP = 14.616e6
 
 
 def log(*a):
@@ -198,7 +203,9 @@ def main():
                    "workgroups_per_replica": info["groups"]},
         "energy_evals_per_s_equiv": round(value * (nvr + 2), 1),
         "roofline": {"bound": "valu", "achieved": round(achieved, 4), "peak": PEAK_FP64, "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK_FP64, 5), "traffic": None,
+                     "frac": round(achieved / PEAK_FP64, 5),
+                     "traffic": (round(HBM_BYTES_PER_SWEEP * args.steps / max(ks["launches"], 1))
+                                 if args.precision == 0 else None),
                      "kernel": "k_sweep", "avg_launch_ms": round(ks["ms"] / max(ks["launches"], 1), 4),
                      "flop_per_launch": round(flops / max(ks["launches"], 1), 1)},
         "cpu_baseline": cpu,
