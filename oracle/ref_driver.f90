@@ -12,6 +12,12 @@
 !   canon  libdir seqfile varfile itysol seed temp nsweep  Metropolis run
 !   partem libdir seqfile itysol seed nrep nequi nswp [switch]  partem_s run
 !   muca   libdir seqfile itysol seed nsweep nup temp kmin kmax ebin
+!   multi  libdir seqfile itysol gbfile boxsize seed nconf temp nequi nsweep
+!          several chains: gbfile holds 'molecule k value' lines for gbpr;
+!          topology, energies of random conformations with the
+!          intermolecular terms (eninteract.f), then a Metropolis run with
+!          the rigid-body trials of metropolis.f:94-168 (nequi unrecorded
+!          sweeps, then the state and nsweep recorded sweeps)
 program ref_driver
   use multicanonical
   include 'INCL.H'
@@ -120,6 +126,72 @@ program ref_driver
     write(*,'(a,i0)') '@done int 1'
     write(*,'(i0)') 1
 
+  case ('multi')
+    call setup(2, .false.)
+    call get_command_argument(5, arg)
+    call rarg(6, boxsize)
+    call iarg(7, iseed)
+    call iarg(8, nconf)
+    call rarg(9, tmp)
+    call iarg(10, nequi)
+    call iarg(11, nsw)
+    call read_gbpr(arg)
+    call dump_params()
+    call dump_topo()
+    call dump_mols()
+    e = energy()
+    call dump_energy(e)
+    call dump_inter()
+    call dump_coords()
+    call sgrnd(iseed)
+    write(*,'(a,i0)') '@nconf int 1'
+    write(*,'(i0)') nconf
+    do k = 1, nconf
+      do i = 1, nvr
+        j = idvr(i)
+        vlvr(j) = addang(pi, axvr(j)*(grnd()-0.5))
+      end do
+      e = energy()
+      write(*,'(a,i0)') '@vlvr double ', nvr
+      do i = 1, nvr
+        write(*,'(es25.17)') vlvr(idvr(i))
+      end do
+      call dump_energy(e)
+      call dump_inter()
+      if (k .le. 2) call dump_coords()
+    end do
+    ! Metropolis from the last random conformation: nequi sweeps to leave
+    ! the clash regime, then the recorded run
+    beta = 1.0d0 / (tmp * ggasc)
+    eolx = energy()
+    acz = 0.0d0
+    do k = 1, nequi
+      call metropolis(eolx, acz, can_weight)
+    end do
+    acz = 0.0d0
+    eolx = energy()
+    write(*,'(a,i0)') '@gbpr_start double ', 6*ntlml
+    write(*,'(es25.17)') ((gbpr(j,i), j=1,6), i=1,ntlml)
+    call dump_mt()
+    write(*,'(a,i0)') '@canon_e0 double 1'
+    write(*,'(es25.17)') eolx
+    write(*,'(a,i0)') '@vlvr_start double ', ivrml1(ntlml) + nvrml(ntlml) - 1
+    write(*,'(es25.17)') (vlvr(i), i=1,ivrml1(ntlml) + nvrml(ntlml) - 1)
+    write(*,'(a,i0)') '@mti_start int 1'
+    write(*,'(i0)') mtstate_pos()
+    do k = 1, nsw
+      call metropolis(eolx, acz, can_weight)
+      write(*,'(a,i0)') '@sweep double 2'
+      write(*,'(es25.17)') eolx
+      write(*,'(es25.17)') acz
+    end do
+    write(*,'(a,i0)') '@vlvr_end double ', ivrml1(ntlml) + nvrml(ntlml) - 1
+    write(*,'(es25.17)') (vlvr(i), i=1,ivrml1(ntlml) + nvrml(ntlml) - 1)
+    write(*,'(a,i0)') '@gbpr_end double ', 6*ntlml
+    write(*,'(es25.17)') ((gbpr(j,i), j=1,6), i=1,ntlml)
+    write(*,'(a,i0)') '@mti int 1'
+    write(*,'(i0)') mtstate_pos()
+
   case default
     write(*,*) 'unknown mode ', trim(mode)
     stop 2
@@ -205,6 +277,46 @@ contains
       olvlvr(a) = vlvr(a)
       if (fxvr(a)) axvr(a) = 0.0d0
     end do
+  end subroutine
+
+  subroutine read_gbpr(fname)
+    character(len=*), intent(in) :: fname
+    integer :: iml, kk, ios
+    double precision :: v
+    open(78, file=trim(fname), status='old')
+    do
+      read(78, *, iostat=ios) iml, kk, v
+      if (ios .ne. 0) exit
+      gbpr(kk, iml) = v
+    end do
+    close(78)
+  end subroutine
+
+  subroutine dump_mols()
+    integer :: m
+    write(*,'(a,i0)') '@irsml1 int ', ntlml
+    write(*,'(i0)') (irsml1(m), m=1,ntlml)
+    write(*,'(a,i0)') '@irsml2 int ', ntlml
+    write(*,'(i0)') (irsml2(m), m=1,ntlml)
+    write(*,'(a,i0)') '@ivrml1 int ', ntlml
+    write(*,'(i0)') (ivrml1(m), m=1,ntlml)
+    write(*,'(a,i0)') '@nvrml int ', ntlml
+    write(*,'(i0)') (nvrml(m), m=1,ntlml)
+    write(*,'(a,i0)') '@gbpr double ', 6*ntlml
+    write(*,'(es25.17)') ((gbpr(j,m), j=1,6), m=1,ntlml)
+  end subroutine
+
+  subroutine dump_inter()
+    write(*,'(a,i0)') '@inter double 4'
+    write(*,'(es25.17)') eysmi, eyeli, eyvwi, eyhbi
+  end subroutine
+
+  subroutine dump_mt()
+    integer :: mtl(0:623), mtil, q
+    common /myblock/ mtil, mtl
+    write(*,'(a,i0)') '@mt_state int 625'
+    write(*,'(i0)') (mtl(q), q=0,623)
+    write(*,'(i0)') mtil
   end subroutine
 
   integer function mtstate_pos()

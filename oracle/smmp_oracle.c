@@ -84,6 +84,12 @@ void oracle_conf_free(oracle_conf *c) {
 }
 
 void oracle_conf_init(const oracle_topo *t, oracle_conf *c) {
+  memset(c->ei, 0, sizeof c->ei);
+  memset(c->gb, 0, sizeof c->gb);
+  if (t->nml > 0)
+    memcpy(c->gb, t->gbm, sizeof(double) * 6 * (size_t)t->nml);
+  else
+    memcpy(c->gb, t->gbpr, sizeof(double) * 6);
   for (int i = 0; i < t->nat; ++i) {
     c->toat[i] = t->toat[i];
     c->sntoat[i] = sin(t->toat[i]);
@@ -127,9 +133,18 @@ static void eyring(const oracle_topo *t, oracle_conf *c, int i, int ia) {
   A(c->zat, i) = A(c->zat, ia) + x3 * bl;
 }
 
-/* bldmol.f:303-420 */
-static void setgbl(const oracle_topo *t, const oracle_conf *c, double *xg, double *zg) {
-  const double *g = t->gbpr;
+/* chain m (0-based): first and last atom, second and third backbone atom */
+static int nchains(const oracle_topo *t) { return t->nml > 0 ? t->nml : 1; }
+static void chain_atoms(const oracle_topo *t, int m, int *a1, int *a2, int *i2, int *i3) {
+  if (t->nml > 0) {
+    *a1 = t->mol_a1[m]; *a2 = t->mol_a2[m]; *i2 = t->mol_i2[m]; *i3 = t->mol_i3[m];
+  } else {
+    *a1 = 1; *a2 = t->nat; *i2 = t->i2; *i3 = t->i3;
+  }
+}
+
+/* bldmol.f:303-420 for a chain with global parameters g and backbone atoms i2, i3 */
+static void setgbl(const oracle_conf *c, const double *g, int i2, int i3, double *xg, double *zg) {
   double ca = cos(g[3]), sa = sin(g[3]), cb = cos(g[4]), sb = sin(g[4]), cg = cos(g[5]), sg = sin(g[5]);
   double ag[3][3], d;
   double x1 = ca * cg - sa * sb * sg, x2 = sa * cg + ca * sb * sg, x3 = -cb * sg;
@@ -141,7 +156,6 @@ static void setgbl(const oracle_topo *t, const oracle_conf *c, double *xg, doubl
   double z1 = ca * sg + sa * sb * cg, z2 = sa * sg - ca * sb * cg, z3 = cb * cg;
   d = sqrt(z1 * z1 + z2 * z2 + z3 * z3);
   ag[0][2] = z1 / d; ag[1][2] = z2 / d; ag[2][2] = z3 / d;
-  int i2 = t->i2, i3 = t->i3;
   double ct2 = A(c->cstoat, i2), st2 = A(c->sntoat, i2), sa2 = A(c->snbaat, i2);
   x1 = -A(c->csbaat, i2); x2 = -sa2 * ct2; x3 = -sa2 * st2;
   double dx = sqrt(x1 * x1 + x2 * x2 + x3 * x3);
@@ -166,7 +180,7 @@ static void setgbl(const oracle_topo *t, const oracle_conf *c, double *xg, doubl
   }
 }
 
-/* setvar.f:13-73 followed by bldmol.f:13-93 (one chain starting at atom 1) */
+/* setvar.f:13-73 followed by bldmol.f:13-93 for every chain (energy.f:40-46) */
 void oracle_setvar(const oracle_topo *t, oracle_conf *c, const double *vlvr) {
   for (int i = 1; i <= t->nvt; ++i) {
     int iat = A(t->iatvr, i), ity = A(t->ityvr, i);
@@ -184,14 +198,19 @@ void oracle_setvar(const oracle_topo *t, oracle_conf *c, const double *vlvr) {
       }
     }
   }
-  double xg[3], zg[3];
-  A(c->xat, 1) = t->gbpr[0];
-  A(c->yat, 1) = t->gbpr[1];
-  A(c->zat, 1) = t->gbpr[2];
-  setgbl(t, c, xg, zg);
-  A(c->xbaat, 1) = zg[0]; A(c->ybaat, 1) = zg[1]; A(c->zbaat, 1) = zg[2];
-  A(c->xtoat, 1) = xg[0]; A(c->ytoat, 1) = xg[1]; A(c->ztoat, 1) = xg[2];
-  for (int j = 2; j <= t->nat; ++j) eyring(t, c, j, A(t->iowat, j));
+  for (int m = 0; m < nchains(t); ++m) {
+    int a1, a2, i2, i3;
+    chain_atoms(t, m, &a1, &a2, &i2, &i3);
+    const double *g = c->gb + 6 * m;
+    double xg[3], zg[3];
+    A(c->xat, a1) = g[0];
+    A(c->yat, a1) = g[1];
+    A(c->zat, a1) = g[2];
+    setgbl(c, g, i2, i3, xg, zg);
+    A(c->xbaat, a1) = zg[0]; A(c->ybaat, a1) = zg[1]; A(c->zbaat, a1) = zg[2];
+    A(c->xtoat, a1) = xg[0]; A(c->ytoat, a1) = xg[1]; A(c->ztoat, a1) = xg[2];
+    for (int j = a1 + 1; j <= a2; ++j) eyring(t, c, j, A(t->iowat, j));
+  }
 }
 
 /* ------------------------------------------------ enyshe.f:13-195 */
@@ -397,7 +416,45 @@ double oracle_enysol(const oracle_topo *t, oracle_conf *c) {
   return eysl;
 }
 
-/* energy.f:12-119 for a single chain (eninteract is zero) */
+/* eninteract.f:14-86: every atom pair between two chains with the
+ * reference's terms; the sums run over atoms, not over residue blocks, so
+ * they may differ from the reference's in the last bits */
+double oracle_eninteract(const oracle_topo *t, oracle_conf *c) {
+  double eyeli = 0.0, eyvwi = 0.0, eyhbi = 0.0;
+  const int n = nchains(t);
+  for (int im = 0; im < n; ++im)
+    for (int jm = im + 1; jm < n; ++jm) {
+      int a1, a2, b1, b2, u, v;
+      chain_atoms(t, im, &a1, &a2, &u, &v);
+      chain_atoms(t, jm, &b1, &b2, &u, &v);
+      for (int i = a1; i <= a2; ++i) {
+        const int ity = A(t->ityat, i);
+        const double cqi = t->conv * A(t->cgat, i);
+        const double xi = A(c->xat, i), yi = A(c->yat, i), zi = A(c->zat, i);
+        for (int j = b1; j <= b2; ++j) {
+          const int jty = A(t->ityat, j);
+          const int k = (ity - 1) + 18 * (jty - 1);
+          const double xij = A(c->xat, j) - xi, yij = A(c->yat, j) - yi, zij = A(c->zat, j) - zi;
+          const double rij2 = xij * xij + yij * yij + zij * zij;
+          const double rij4 = rij2 * rij2, rij6 = rij4 * rij2, rij = sqrt(rij2);
+          const double ep = dielectric(t, rij);
+          eyeli = eyeli + cqi * A(t->cgat, j) / (rij * ep);
+          if (t->ihbty[k] == 0)
+            eyvwi = eyvwi + t->aij[k] / (rij6 * rij6) - t->cij[k] / rij6;
+          else
+            eyhbi = eyhbi + t->ahb[k] / (rij6 * rij6) - t->chb[k] / (rij6 * rij4);
+        }
+      }
+    }
+  c->ei[0] = eyeli + eyvwi + eyhbi;
+  c->ei[1] = eyeli;
+  c->ei[2] = eyvwi;
+  c->ei[3] = eyhbi;
+  return c->ei[0];
+}
+
+/* energy.f:12-119: intra-chain terms of all chains, solvent over all atoms,
+ * then the terms between chains (added to the total only, energy.f:114) */
 double oracle_energy(const oracle_topo *t, oracle_conf *c, const double *vlvr) {
   oracle_setvar(t, c, vlvr);
   double esm = 0.0;
@@ -407,6 +464,7 @@ double oracle_energy(const oracle_topo *t, oracle_conf *c, const double *vlvr) {
   } else {
     c->e[6] = 0.0;
   }
+  esm = esm + oracle_eninteract(t, c);
   c->e[0] = esm;
   return esm;
 }
@@ -477,6 +535,47 @@ int oracle_metropolis(const oracle_topo *t, oracle_conf *c, oracle_mt *g, double
     }
     trace_trial(jv, enw, eprev, up != 0.0);
     *acz = *acz + up;
+  }
+  /* rigid moves of each chain when there are several (metropolis.f:94-168):
+   * three shifts of its position, then three new Euler angles */
+  if (t->nml > 1) {
+    for (int m = 0; m < t->nml; ++m) {
+      for (int i = 0; i < 6; ++i) {
+        double *gp = &c->gb[6 * m + i];
+        const double gbol = *gp;
+        if (i < 3) {
+          *gp = *gp + (oracle_grnd(g) - 0.5);
+          if (t->boxsize > 0.0) {
+            if (*gp > t->boxsize)
+              *gp = t->boxsize;
+            else if (*gp < -t->boxsize)
+              *gp = -t->boxsize;
+          }
+        } else {
+          *gp = (oracle_grnd(g) - 0.5) * (i == 4 ? PI : PI2);
+        }
+        double enw = oracle_energy(t, c, vlvr);
+        double delta = oracle_weight_eval(w, enw) - oracle_weight_eval(w, eol);
+        double up = 0.0;
+        const double eprev = eol;
+        if (delta <= 0.0) {
+          eol = enw;
+          up = 1.0;
+        } else {
+          double rd = oracle_grnd(g);
+          if (delta > 100.0) delta = 100.0;
+          double ex = exp(-delta);
+          if (ex >= rd) {
+            eol = enw;
+            up = 1.0;
+          } else {
+            *gp = gbol;
+          }
+        }
+        trace_trial(-1 - (6 * m + i), enw, eprev, up != 0.0);
+        *acz = *acz + up;
+      }
+    }
   }
   double enw = oracle_energy(t, c, vlvr);
   int bad = fabs(eol - enw) > 0.000001 && eol < 100000;

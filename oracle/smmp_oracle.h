@@ -14,7 +14,9 @@
 extern "C" {
 #endif
 
-/* Topology and parameters of one single-chain system (inputs, read only). */
+/* Topology and parameters of one system (inputs, read only).  nml = 0
+ * describes one chain through i2, i3 and gbpr; nml >= 1 chains use the
+ * mol_* arrays (1-based atoms) and gbm (6 values per chain). */
 typedef struct {
   int nat, nvt, nms, npnt;
   const int32_t *ityat, *iowat, *nbdat, *ibdat; /* ibdat: 3*nat */
@@ -31,6 +33,10 @@ typedef struct {
   int epsd, itysol;
   int i2, i3;       /* second and third backbone atom (fnd3ba) */
   double gbpr[6];   /* global position/orientation */
+  int nml;
+  int mol_a1[10], mol_a2[10], mol_i2[10], mol_i3[10];
+  double gbm[60];
+  double boxsize;   /* COMMON /bounds/: rigid-move positions stay in [-boxsize, boxsize] when > 0 */
 } oracle_topo;
 
 /* Mutable state of one conformation (allocated by the caller). */
@@ -40,6 +46,8 @@ typedef struct {
   double *xtoat, *ytoat, *ztoat, *xbaat, *ybaat, *zbaat;
   /* components of the last energy: total, eysm, eyel, eyvw, eyhb, eyvr, eysl, asa, vdvol */
   double e[9];
+  double ei[4];   /* between chains: eysmi, eyeli, eyvwi, eyhbi (eninteract.f) */
+  double gb[60];  /* global parameters of the chains (6 per chain), set by oracle_conf_init */
 } oracle_conf;
 
 /* Mersenne twister of twister.f (state 624 words + position). */
@@ -65,7 +73,9 @@ void oracle_setvar(const oracle_topo *t, oracle_conf *c, const double *vlvr);
 double oracle_enyshe(const oracle_topo *t, oracle_conf *c);
 /* enysol.f: returns eysl; fills e[7], e[8]. */
 double oracle_enysol(const oracle_topo *t, oracle_conf *c);
-/* energy.f for one chain: setvar + enyshe + solvent; returns total. */
+/* eninteract.f:14-86: pairs between chains; fills ei[], returns eysmi. */
+double oracle_eninteract(const oracle_topo *t, oracle_conf *c);
+/* energy.f: setvar + enyshe + solvent + eninteract; returns total. */
 double oracle_energy(const oracle_topo *t, oracle_conf *c, const double *vlvr);
 
 /* Weight of an energy (canon.f:135 / mulcan_par_mod.f90:380-412). */
@@ -78,8 +88,9 @@ typedef struct {
 } oracle_weight;
 double oracle_weight_eval(const oracle_weight *w, double x);
 
-/* metropolis.f:13-184 for one chain.  idvr: free variable indices (1-based),
- * axvr: step widths (nvt).  vlvr is updated in place.  Returns 0, or 1 if
+/* metropolis.f:13-184.  idvr: free variable indices (1-based), axvr: step
+ * widths (nvt).  vlvr (and, with several chains, c->gb through the rigid
+ * trials of metropolis.f:94-168) is updated in place.  Returns 0, or 1 if
  * the end-of-sweep drift check fails (the reference stops there). */
 int oracle_metropolis(const oracle_topo *t, oracle_conf *c, oracle_mt *g, double *vlvr,
                       int nvr, const int32_t *idvr, const double *axvr,

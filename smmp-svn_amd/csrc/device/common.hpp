@@ -8,14 +8,20 @@ namespace smmp {
 constexpr int kWave = 64;
 constexpr int kTy = 18;          // ECEPP atom types
 constexpr int kMaxNb = 64;       // solvent neighbours per chunk of one atom's gather
+constexpr int kMaxMol = 10;      // chains per system (mxml)
 constexpr double kPi = 3.141592653589793;
 constexpr double kPi2 = 2.0 * kPi;
 
-// Read-only description of one chain in device memory (see HostSys).
+// Read-only description of one system of chains in device memory (see HostSys).
 struct DevSys {
-  int32_t nat, nvt, nlev, words, nheavy, npnt, i2, i3, epsd, itysol, np;  // np: nat rounded up to 64
-  double conv, plt, slp;
-  double gbpr[6];
+  int32_t nat, nvt, nlev, words, nheavy, npnt, epsd, itysol, np;  // np: nat rounded up to 64
+  int32_t nml;                           // chains
+  int16_t mol_a1[kMaxMol], mol_a2[kMaxMol];  // first and last atom of each chain
+  int16_t mol_i2[kMaxMol], mol_i3[kMaxMol];  // its second and third backbone atoms
+  int16_t mol_h1[kMaxMol], mol_h2[kMaxMol];  // its heavy atoms: heavy indices [h1, h2)
+  double conv, plt, slp, boxsize;
+  double gbpr[6 * kMaxMol];              // global parameters of the chains (the context's)
+  const int16_t* mol_of;                 // per atom: its chain
   const int16_t* parent;
   const int16_t* lev_atoms;
   const int32_t* lev_off;

@@ -145,9 +145,9 @@ __device__ void energy_stage_torsions(const DevSys& S, const double* vl, EnergyL
   }
 }
 
-// bldmol.f:303-420 (setgbl) and the first atom; one lane.
-__device__ void build_root(const DevSys& S, EnergyLds& L) {
-  const double* g = S.gbpr;
+// bldmol.f:303-420 (setgbl) and the first atom of chain m from its global
+// parameters g[0..5]; one lane.
+__device__ void build_root(const DevSys& S, EnergyLds& L, const double* g, int m) {
   double ca = cos(g[3]), sa = sin(g[3]), cb = cos(g[4]), sb = sin(g[4]), cg = cos(g[5]), sg = sin(g[5]);
   double ag[3][3], d;
   double x1 = ca * cg - sa * sb * sg, x2 = sa * cg + ca * sb * sg, x3 = -cb * sg;
@@ -159,7 +159,7 @@ __device__ void build_root(const DevSys& S, EnergyLds& L) {
   double z1 = ca * sg + sa * sb * cg, z2 = sa * sg - ca * sb * cg, z3 = cb * cg;
   d = sqrt(z1 * z1 + z2 * z2 + z3 * z3);
   ag[0][2] = z1 / d; ag[1][2] = z2 / d; ag[2][2] = z3 / d;
-  const int i2 = S.i2, i3 = S.i3;
+  const int a1 = S.mol_a1[m], i2 = S.mol_i2[m], i3 = S.mol_i3[m];
   double ct2 = L.fr[6 * i2 + 0], st2 = L.fr[6 * i2 + 1], sa2 = S.sba[i2];
   x1 = -S.cba[i2]; x2 = -sa2 * ct2; x3 = -sa2 * st2;
   double dx = sqrt(x1 * x1 + x2 * x2 + x3 * x3);
@@ -179,15 +179,16 @@ __device__ void build_root(const DevSys& S, EnergyLds& L) {
   }
   dx = sqrt(xg[0] * xg[0] + xg[1] * xg[1] + xg[2] * xg[2]);
   dz = sqrt(zg[0] * zg[0] + zg[1] * zg[1] + zg[2] * zg[2]);
-  L.x[0] = g[0];
-  L.y[0] = g[1];
-  L.z[0] = g[2];
-  L.fr[0] = xg[0] / dx;
-  L.fr[1] = xg[1] / dx;
-  L.fr[2] = xg[2] / dx;
-  L.fr[3] = zg[0] / dz;
-  L.fr[4] = zg[1] / dz;
-  L.fr[5] = zg[2] / dz;
+  L.x[a1] = g[0];
+  L.y[a1] = g[1];
+  L.z[a1] = g[2];
+  double* f = L.fr + 6 * a1;
+  f[0] = xg[0] / dx;
+  f[1] = xg[1] / dx;
+  f[2] = xg[2] / dx;
+  f[3] = zg[0] / dz;
+  f[4] = zg[1] / dz;
+  f[5] = zg[2] / dz;
 }
 
 // bldmol.f:175-242 (eyring) for atom a from its parent's frame.
@@ -218,13 +219,12 @@ __device__ __forceinline__ void eyring_atom(const DevSys& S, EnergyLds& L, int a
   L.z[a] = pz + u3 * bl;
 }
 
-// Chain build by wave 0, level by level; the other waves wait.
-__device__ void build_chain(const DevSys& S, EnergyLds& L, Work& wk) {
+// Chain build by wave 0, level by level; the other waves wait.  gb holds
+// six global parameters per chain; lane m places the first atom of chain m.
+__device__ void build_chain(const DevSys& S, EnergyLds& L, Work& wk, const double* gb) {
   if (threadIdx.x < kWave) {
-    if (threadIdx.x == 0) {
-      build_root(S, L);
-      wk.builds += S.nat;
-    }
+    if (threadIdx.x < S.nml) build_root(S, L, gb + 6 * threadIdx.x, threadIdx.x);
+    if (threadIdx.x == 0) wk.builds += S.nat;
     wave_lds_sync();
     const int lane = threadIdx.x;
     for (int lv = 0; lv < S.nlev; ++lv) {
@@ -252,8 +252,19 @@ struct PairSums {
   double mag;  // mixed mode: sum of |pair term| (scale of the FP32 rounding), else 0
 };
 
+// Columns of word bj (atoms 64 bj .. 64 bj + 63) in the chain of atom i.
+__device__ __forceinline__ uint64_t chain_cols(const DevSys& S, int i, int bj) {
+  const int m = S.mol_of[i];
+  const int lo = max(S.mol_a1[m] - 64 * bj, 0), hi = min(S.mol_a2[m] - 64 * bj, 63);
+  if (lo > hi) return 0ull;
+  const uint64_t upto = hi == 63 ? ~0ull : ((1ull << (hi + 1)) - 1ull);
+  return upto & ~((1ull << lo) - 1ull);
+}
+
 // Pair terms over the tiles this wave owns (FP64): tiles t0, t0 + stride, ...
-__device__ PairSums pair_terms_f64(const DevSys& S, const EnergyLds& L, int t0, int stride, Work& wk) {
+// sel: 0 every pair, 1 pairs within a chain, 2 pairs between chains.
+__device__ PairSums pair_terms_f64(const DevSys& S, const EnergyLds& L, int t0, int stride, Work& wk,
+                                   int sel = 0) {
   const int lane = threadIdx.x & (kWave - 1);
   const int nat = S.nat, nt = (nat + 63) / 64, ntile = nt * (nt + 1) / 2, words = S.words;
   double el = 0.0, vw = 0.0, hb = 0.0;
@@ -274,6 +285,12 @@ __device__ PairSums pair_terms_f64(const DevSys& S, const EnergyLds& L, int t0, 
     }
     if (bi == bj) {
       uint64_t keep = (lane == 63) ? 0ull : ~((2ull << lane) - 1ull);
+      w0 &= keep;
+      w1 &= keep;
+    }
+    if (sel != 0 && vi) {
+      const uint64_t in = chain_cols(S, i, bj);
+      const uint64_t keep = sel == 1 ? in : ~in;
       w0 &= keep;
       w1 &= keep;
     }
@@ -311,7 +328,8 @@ __device__ PairSums pair_terms_f64(const DevSys& S, const EnergyLds& L, int t0, 
 }
 
 // Mixed precision: FP32 terms, FP32 sums per tile, FP64 accumulation.
-__device__ PairSums pair_terms_mixed(const DevSys& S, const EnergyLds& L, int t0, int stride, Work& wk) {
+__device__ PairSums pair_terms_mixed(const DevSys& S, const EnergyLds& L, int t0, int stride, Work& wk,
+                                     int sel = 0) {
   const int lane = threadIdx.x & (kWave - 1);
   const int nat = S.nat, nt = (nat + 63) / 64, ntile = nt * (nt + 1) / 2, words = S.words;
   double el = 0.0, vw = 0.0, hb = 0.0, mag = 0.0;
@@ -334,6 +352,12 @@ __device__ PairSums pair_terms_mixed(const DevSys& S, const EnergyLds& L, int t0
     }
     if (bi == bj) {
       uint64_t keep = (lane == 63) ? 0ull : ~((2ull << lane) - 1ull);
+      w0 &= keep;
+      w1 &= keep;
+    }
+    if (sel != 0 && vi) {
+      const uint64_t in = chain_cols(S, i, bj);
+      const uint64_t keep = sel == 1 ? in : ~in;
       w0 &= keep;
       w1 &= keep;
     }
@@ -686,6 +710,7 @@ struct EnergyOut {
   double el, vw, hb, tor, sol, asa;
   double mag;  // mixed mode: sum of |pair term|; 0 in FP64 mode
   int err;
+  double eli, vwi, hbi;  // with split: the pair terms between chains (not in el, vw, hb)
 };
 
 // Pair, torsion and solvent terms of the coordinates in L (all threads).
@@ -693,10 +718,11 @@ struct EnergyOut {
 // this block evaluates when not null.  On return thread 0 holds the totals
 // in 'out'.  With nparts > 1 the block evaluates share 'part' only (tiles
 // and heavy atoms k with k % nparts == part; torsions in part 0), and the
-// totals are partial sums.
+// totals are partial sums.  With split (several chains) the pair terms
+// between chains go to eli, vwi, hbi instead of el, vw, hb (eninteract.f).
 template <bool MIXED>
 __device__ void energy_from_coords(const DevSys& S, const double* vl, EnergyLds& L, int32_t* nfree_out,
-                                   EnergyOut& out, Work& wk, int part = 0, int nparts = 1) {
+                                   EnergyOut& out, Work& wk, int part = 0, int nparts = 1, bool split = false) {
   const int tid = threadIdx.x, nwaves = blockDim.x / kWave, wave = tid / kWave, lane = tid & (kWave - 1);
   if (S.itysol > 0) fill_heavy_packed(S, L);  // read after the barrier below
   if (MIXED) {  // FP32 copies centred on the first atom (frames are dead now)
@@ -710,7 +736,14 @@ __device__ void energy_from_coords(const DevSys& S, const double* vl, EnergyLds&
     __syncthreads();
   }
   const int t0 = part + nparts * wave, stride = nparts * nwaves;
-  PairSums ps = MIXED ? pair_terms_mixed(S, L, t0, stride, wk) : pair_terms_f64(S, L, t0, stride, wk);
+  split = split && S.nml > 1;
+  PairSums ps = MIXED ? pair_terms_mixed(S, L, t0, stride, wk, split ? 1 : 0)
+                      : pair_terms_f64(S, L, t0, stride, wk, split ? 1 : 0);
+  PairSums pi = {0.0, 0.0, 0.0, 0.0};
+  if (split) {
+    pi = MIXED ? pair_terms_mixed(S, L, t0, stride, wk, 2) : pair_terms_f64(S, L, t0, stride, wk, 2);
+    ps.mag += pi.mag;
+  }
   double tor = part == 0 ? torsion_terms(S, vl) : 0.0;
   __syncthreads();  // packed heavy atoms written; the solvent scratch overlays the FP32 copies
   double sol = 0.0, asa = 0.0;
@@ -757,17 +790,36 @@ __device__ void energy_from_coords(const DevSys& S, const double* vl, EnergyLds&
     out.asa = s[5];
     out.mag = s[6];
     out.err = s[7] != 0.0;
+    out.eli = out.vwi = out.hbi = 0.0;
+  }
+  if (split) {  // second round through L.red: the terms between chains
+    double u[3] = {pi.el, pi.vw, pi.hb};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) u[k] = wave_sum(u[k]);
+    __syncthreads();
+    if (lane == 0)
+      for (int k = 0; k < 3; ++k) L.red[8 * wave + k] = u[k];
+    __syncthreads();
+    if (tid == 0) {
+      double t[3] = {0.0, 0.0, 0.0};
+      for (int w = 0; w < nwaves; ++w)
+        for (int k = 0; k < 3; ++k) t[k] += L.red[8 * w + k];
+      out.eli = t[0];
+      out.vwi = t[1];
+      out.hbi = t[2];
+    }
   }
 }
 
 // Whole energy of one conformation from its variables (energy.f:12-119).
 template <bool MIXED>
-__device__ void conformation_energy(const DevSys& S, const double* vl, EnergyLds& L, int32_t* nfree_out,
-                                    EnergyOut& out, Work& wk, int part = 0, int nparts = 1) {
+__device__ void conformation_energy(const DevSys& S, const double* vl, const double* gb, EnergyLds& L,
+                                    int32_t* nfree_out, EnergyOut& out, Work& wk, int part = 0, int nparts = 1,
+                                    bool split = false) {
   energy_stage_torsions(S, vl, L);
   __syncthreads();
-  build_chain(S, L, wk);
-  energy_from_coords<MIXED>(S, vl, L, nfree_out, out, wk, part, nparts);
+  build_chain(S, L, wk, gb);
+  energy_from_coords<MIXED>(S, vl, L, nfree_out, out, wk, part, nparts, split);
 }
 
 }  // namespace smmp

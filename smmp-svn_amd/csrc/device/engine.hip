@@ -26,6 +26,9 @@ namespace smmp {
   } while (0)
 
 constexpr int kEnergyThreads = 512;
+// k_energy values per conformation: total, eysm, eyel, eyvw, eyhb, eyvr,
+// eysl, asa, then between chains eysmi, eyeli, eyvwi, eyhbi
+constexpr int kEnergyOut = 12;
 
 // ------------------------------------------------------------- kernels
 
@@ -44,11 +47,12 @@ __global__ void __launch_bounds__(kEnergyThreads) k_energy(const DevSys* __restr
   EnergyOut o;
   Work wk;
   energy_stage_static(S, L);
-  conformation_energy<MIXED>(S, vl, L, nullptr, o, wk);
+  conformation_energy<MIXED>(S, vl, S.gbpr, L, nullptr, o, wk, 0, 1, true);
   if (threadIdx.x == 0) {
     double eysm = o.el + o.vw + o.hb + o.tor;  // energy.f / enyshe.f:187
-    double* r = out + (size_t)c * 8;
-    r[0] = eysm + o.sol;
+    const double eysmi = o.eli + o.vwi + o.hbi;  // eninteract.f:83
+    double* r = out + (size_t)c * kEnergyOut;
+    r[0] = eysm + o.sol + eysmi;
     r[1] = eysm;
     r[2] = o.el;
     r[3] = o.vw;
@@ -56,6 +60,10 @@ __global__ void __launch_bounds__(kEnergyThreads) k_energy(const DevSys* __restr
     r[5] = o.tor;
     r[6] = o.sol;
     r[7] = o.asa;
+    r[8] = eysmi;
+    r[9] = o.eli;
+    r[10] = o.vwi;
+    r[11] = o.hbi;
     if (o.err) atomicOr(err, 1);
   }
   if (xyz_out) {
@@ -109,7 +117,7 @@ struct DevMem {
 struct SamplerBuffers {
   DevMem idvr, axvr, vlvr, xyz, nfree, valid, eol, acc, mt, rng_of, slot_of, beta, err, dbg;
   DevMem wb, walpha, hist, traj, ipoi, iswitch, work, trace, trace_n, phase, part, sync, gsync;
-  DevMem sweep_base, tune_vl, tune_mt;
+  DevMem sweep_base, tune_vl, tune_mt, tune_gb, gb;
 };
 
 // One system uploaded to the device: a single allocation holding every
@@ -158,6 +166,7 @@ static void upload_system(DevSystem& ds, hipStream_t st) {
   size_t o_bits = add(h.bits.data(), h.bits.size() * 8);
   size_t o_heavy = add(h.heavy.data(), h.heavy.size() * 2);
   size_t o_heavy_of = add(h.heavy_of.data(), h.heavy_of.size() * 2);
+  size_t o_mol_of = add(h.mol_of.data(), h.mol_of.size() * 2);
   size_t o_pA = add(h.pA.data(), h.pA.size() * 8);
   size_t o_pA14 = add(h.pA14.data(), h.pA14.size() * 8);
   size_t o_pC = add(h.pC.data(), h.pC.size() * 8);
@@ -171,10 +180,17 @@ static void upload_system(DevSystem& ds, hipStream_t st) {
   HIPCHK(hipStreamSynchronize(st));
   DevSys& d = ds.d;
   d.nat = h.nat; d.nvt = h.nvt; d.nlev = h.nlev; d.words = h.words; d.nheavy = h.nheavy;
-  d.npnt = h.npnt; d.i2 = h.i2; d.i3 = h.i3; d.epsd = h.epsd; d.itysol = h.itysol;
+  d.npnt = h.npnt; d.epsd = h.epsd; d.itysol = h.itysol;
+  d.nml = h.nml;
+  for (int m = 0; m < kMaxMol; ++m) {
+    d.mol_a1[m] = h.mol_a1[m]; d.mol_a2[m] = h.mol_a2[m];
+    d.mol_i2[m] = h.mol_i2[m]; d.mol_i3[m] = h.mol_i3[m];
+    d.mol_h1[m] = h.mol_h1[m]; d.mol_h2[m] = h.mol_h2[m];
+  }
+  d.boxsize = h.boxsize;
   d.np = (h.nat + 63) / 64 * 64;
   d.conv = h.conv; d.plt = h.plt; d.slp = h.slp;
-  for (int k = 0; k < 6; ++k) d.gbpr[k] = h.gbpr[k];
+  for (int k = 0; k < 6 * kMaxMol; ++k) d.gbpr[k] = h.gbpr[k];
   d.parent = (const int16_t*)(base + o_parent);
   d.lev_atoms = (const int16_t*)(base + o_lev);
   d.lev_off = (const int32_t*)(base + o_levoff);
@@ -202,6 +218,7 @@ static void upload_system(DevSystem& ds, hipStream_t st) {
   d.bits = (const uint64_t*)(base + o_bits);
   d.heavy = (const int16_t*)(base + o_heavy);
   d.heavy_of = (const int16_t*)(base + o_heavy_of);
+  d.mol_of = (const int16_t*)(base + o_mol_of);
   d.pA = (const double*)(base + o_pA);
   d.pA14 = (const double*)(base + o_pA14);
   d.pC = (const double*)(base + o_pC);
@@ -324,10 +341,18 @@ void engine_invalidate(Engine* e) {
 // Upload the context's system when the topology changed, or refresh the
 // global parameters gbpr when only those changed.
 static DevSystem& current_system(Engine& E, const State& s) {
-  bool gb_same = true;
-  for (int k = 0; k < 6; ++k)
-    if (E.sys.h.gbpr[k] != s.gbpr(k + 1, 1)) gb_same = false;
-  if (!E.valid || !gb_same) {
+  bool gb_same = E.valid && E.sys.h.nml == s.ntlml && E.sys.h.boxsize == s.boxsize;
+  for (int m = 1; gb_same && m <= s.ntlml; ++m)
+    for (int k = 0; k < 6; ++k)
+      if (E.sys.h.gbpr[6 * (m - 1) + k] != s.gbpr(k + 1, m)) gb_same = false;
+  if (E.valid && !gb_same && E.sys.h.nml == s.ntlml && s.ntlml >= 1) {  // only the global parameters moved
+    for (int m = 1; m <= s.ntlml; ++m)
+      for (int k = 0; k < 6; ++k) E.sys.h.gbpr[6 * (m - 1) + k] = E.sys.d.gbpr[6 * (m - 1) + k] = s.gbpr(k + 1, m);
+    E.sys.h.boxsize = E.sys.d.boxsize = s.boxsize;
+    DevSys* dp = static_cast<DevSys*>(E.d_sys.ensure(sizeof(DevSys)));
+    HIPCHK(hipMemcpyAsync(dp, &E.sys.d, sizeof(DevSys), hipMemcpyHostToDevice, E.stream));
+    HIPCHK(hipStreamSynchronize(E.stream));
+  } else if (!gb_same) {
     E.sys.h = build_host_sys(s);
     upload_system(E.sys, E.stream);
     ++E.sys_gen;
@@ -447,7 +472,7 @@ static void energy_into_state(Engine& E, State& s) {
   double* d_vl = static_cast<double*>(E.d_vlvr.ensure(sizeof(double) * nvt));
   int32_t* d_cs = static_cast<int32_t*>(E.d_confsys.ensure(sizeof(int32_t)));
   int64_t* d_off = static_cast<int64_t*>(E.d_confoff.ensure(sizeof(int64_t)));
-  double* d_out = static_cast<double*>(E.d_out.ensure(sizeof(double) * 8));
+  double* d_out = static_cast<double*>(E.d_out.ensure(sizeof(double) * kEnergyOut));
   double* d_xyz = static_cast<double*>(E.d_xyz.ensure(sizeof(double) * 3 * nat));
   const int32_t zero = 0;
   const int64_t zoff = 0;
@@ -456,7 +481,7 @@ static void energy_into_state(Engine& E, State& s) {
   HIPCHK(hipMemcpyAsync(d_off, &zoff, sizeof(zoff), hipMemcpyHostToDevice, E.stream));
   launch_energy(E, static_cast<const DevSys*>(E.d_sys.p), d_cs, d_off, d_vl, d_out, d_xyz, 1,
                 [&] { const DevSys* one = &ds.d; return plan_energy_lds(&one, 1, kEnergyThreads); }(), 0);
-  double o[8];
+  double o[kEnergyOut];
   std::vector<double> xyz(3 * (size_t)nat);
   HIPCHK(hipMemcpyAsync(o, d_out, sizeof o, hipMemcpyDeviceToHost, E.stream));
   HIPCHK(hipMemcpyAsync(xyz.data(), d_xyz, sizeof(double) * 3 * nat, hipMemcpyDeviceToHost, E.stream));
@@ -469,19 +494,23 @@ static void energy_into_state(Engine& E, State& s) {
   s.eyvr = o[5];
   s.eysl = o[6];
   s.asa = o[7];
+  s.eysmi = o[8];
+  s.eyeli = o[9];
+  s.eyvwi = o[10];
+  s.eyhbi = o[11];
   for (int a = 0; a < nat; ++a) {
     s.xat(a + 1) = xyz[3 * a + 0];
     s.yat(a + 1) = xyz[3 * a + 1];
     s.zat(a + 1) = xyz[3 * a + 2];
   }
   // keep the host torsion state in step with the variables (setvar.f)
-  setvar(s, 1, s.vlvr.data());
+  for (int m = 1; m <= s.ntlml; ++m) setvar(s, m, s.vlvr.data());
 }
 
 double engine_energy(Engine*& e, State& s) {
   Engine& E = engine_of(e);
   energy_into_state(E, s);
-  return s.eysm + s.eysl;
+  return s.eysm + s.eysl + s.eysmi;  // energy.f:114
 }
 
 double engine_enysol(Engine*& e, State& s) {
@@ -515,7 +544,7 @@ int engine_energy_batch(Engine*& e, std::vector<State*>& sys, const int32_t* con
   int32_t* d_cs = static_cast<int32_t*>(E.d_confsys.ensure(sizeof(int32_t) * nconf));
   int64_t* d_off = static_cast<int64_t*>(E.d_confoff.ensure(sizeof(int64_t) * nconf));
   double* d_vl = static_cast<double*>(E.d_vlvr.ensure(sizeof(double) * tot));
-  double* d_out = static_cast<double*>(E.d_out.ensure(sizeof(double) * 8 * nconf));
+  double* d_out = static_cast<double*>(E.d_out.ensure(sizeof(double) * kEnergyOut * nconf));
   HIPCHK(hipMemcpyAsync(d_systems, dsys.data(), sizeof(DevSys) * nsys, hipMemcpyHostToDevice, E.stream));
   HIPCHK(hipMemcpyAsync(d_cs, conf_sys, sizeof(int32_t) * nconf, hipMemcpyHostToDevice, E.stream));
   HIPCHK(hipMemcpyAsync(d_off, off.data(), sizeof(int64_t) * nconf, hipMemcpyHostToDevice, E.stream));
@@ -524,9 +553,12 @@ int engine_energy_batch(Engine*& e, std::vector<State*>& sys, const int32_t* con
   for (int i = 0; i < nsys; ++i) host_sys[i] = &dsys[i];
   launch_energy(E, d_systems, d_cs, d_off, d_vl, d_out, nullptr, nconf,
                 plan_energy_lds(host_sys.data(), nsys, kEnergyThreads), precision);
-  HIPCHK(hipMemcpyAsync(out, d_out, sizeof(double) * 8 * nconf, hipMemcpyDeviceToHost, E.stream));
+  std::vector<double> full((size_t)kEnergyOut * nconf);
+  HIPCHK(hipMemcpyAsync(full.data(), d_out, sizeof(double) * full.size(), hipMemcpyDeviceToHost, E.stream));
   HIPCHK(hipStreamSynchronize(E.stream));
   if (read_err(E)) throw Error("enysol> too many solvent neighbours for one atom");
+  for (int c = 0; c < nconf; ++c)
+    for (int k = 0; k < 8; ++k) out[8 * (size_t)c + k] = full[(size_t)kEnergyOut * c + k];
   return SMMP_OK;
 }
 

@@ -108,6 +108,7 @@ struct SamplerParams {
   int32_t* iswitch;      // which half of the pairs is tried next
   uint32_t* mt_x;        // the exchange stream (625 words)
   uint32_t* gsync;       // grid-barrier arrival counter, zero at launch
+  double* gbpr;          // per replica: 6 global parameters per chain (kMaxMol chains)
   // Sweeps already done when launches are replayed from a graph (a kernel
   // argument cannot change between replays); null: traj_base.
   int32_t* sweep_base;
@@ -145,6 +146,7 @@ __host__ __device__ inline size_t sampler_lds_bytes(int np, int npnt, int nheavy
   b += 2 * (size_t)nheavy * sizeof(int32_t);      // nfree current / trial
   b += 625 * sizeof(uint32_t);                    // MT state
   b += kScDoubles * sizeof(double);               // broadcast scalars and phase clocks
+  b += 6 * kMaxMol * sizeof(double);              // global parameters of the chains
   b += 7 * (size_t)nvt * sizeof(double);          // per-variable tables (VarLds)
   b += 8 * (size_t)nvt * sizeof(int16_t);
   b += 2 * (size_t)nheavy;                        // list of the affected heavy atoms
@@ -164,6 +166,7 @@ struct SamplerLds {
   EnergyLds L;
   VarLds V;
   double *bx, *by, *bz, *vl, *sc;
+  double* gb;  // global parameters of the replica's chains (6 per chain)
   int32_t *nf, *nf2;
   uint32_t* mt;
   int16_t* alist;
@@ -180,6 +183,7 @@ __device__ inline SamplerLds carve_sampler_lds(double* base, const DevSys& S, in
   Q.bz = reinterpret_cast<double*>(p); p += S.np * sizeof(double);
   Q.vl = reinterpret_cast<double*>(p); p += nvt * sizeof(double);
   Q.sc = reinterpret_cast<double*>(p); p += kScDoubles * sizeof(double);
+  Q.gb = reinterpret_cast<double*>(p); p += 6 * kMaxMol * sizeof(double);
   // doubles first (8-byte aligned), then 32-, 16- and 8-bit arrays
   Q.V.ax = reinterpret_cast<double*>(p); p += nvt * sizeof(double);
   Q.V.sign = reinterpret_cast<double*>(p); p += nvt * sizeof(double);
@@ -392,7 +396,7 @@ __device__ __attribute__((noinline)) ReplicaEnergy replica_energy(const DevSys* 
   int32_t* nfree_out = nf_sel == 1 ? Q.nf : (nf_sel == 2 ? Q.nf2 : nullptr);
   ReplicaEnergy r;
   if (rebuild)
-    conformation_energy<MIXED>(S, Q.vl, Q.L, nfree_out, r.o, r.w, part, nparts);
+    conformation_energy<MIXED>(S, Q.vl, Q.gb, Q.L, nfree_out, r.o, r.w, part, nparts);
   else
     energy_from_coords<MIXED>(S, Q.vl, Q.L, nfree_out, r.o, r.w, part, nparts);
   return r;
@@ -533,6 +537,8 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
   const VarLds& V = Q.V;
   for (int k = tid; k < nvt; k += blockDim.x) Q.vl[k] = gvl[k];
   for (int k = tid; k < 625; k += blockDim.x) Q.mt[k] = gmt[k];
+  double* ggb = P.gbpr + (size_t)r * 6 * kMaxMol;
+  for (int k = tid; k < 6 * S.nml; k += blockDim.x) Q.gb[k] = ggb[k];
   const bool valid = P.valid[r] != 0;
   if (valid) {
     for (int a = tid; a < nat; a += blockDim.x) {
@@ -937,6 +943,7 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
   if (g == 0) {
     for (int k = tid; k < nvt; k += blockDim.x) gvl[k] = Q.vl[k];
     for (int k = tid; k < 625; k += blockDim.x) gmt[k] = Q.mt[k];
+    for (int k = tid; k < 6 * S.nml; k += blockDim.x) ggb[k] = Q.gb[k];
     for (int a = tid; a < nat; a += blockDim.x) {
       gx[a] = L.x[a];
       gx[np + a] = L.y[a];

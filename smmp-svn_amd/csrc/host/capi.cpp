@@ -77,6 +77,8 @@ const ViewDesc kViews[] = {
     V1(eny_i, n14ml, SMMP_I32, mxml, 1), V1(eny_i, i14ml1, SMMP_I32, mxml, 1),
     VS(eny_r, eyel, SMMP_F64), VS(eny_r, eyvw, SMMP_F64), VS(eny_r, eyhb, SMMP_F64), VS(eny_r, eyvr, SMMP_F64),
     VS(eny_r, eysm, SMMP_F64), VS(eny_r, eysl, SMMP_F64),
+    VS(eny_r, eysmi, SMMP_F64), VS(eny_r, eyeli, SMMP_F64), VS(eny_r, eyvwi, SMMP_F64), VS(eny_r, eyhbi, SMMP_F64),
+    VS(bounds, boxsize, SMMP_F64),
     V1(solvent, rvdw, SMMP_F64, mxat, 1), V1(solvent, sigma, SMMP_F64, mxat, 1),
     V2(tes, spoint, SMMP_F64, mxpoint, 3), VS(pnt, npnt, SMMP_I32),
     VS(v_sol, vdvol, SMMP_F64), VS(v_sol, asa, SMMP_F64),
@@ -209,6 +211,8 @@ int smmp_enysol(smmp_ctx* c, int nmol, double* e) {
   return guarded(c, SMMP_ESTATE, [&] {
     if (c->s.ntlml < 1) throw Error("enysol> no molecule initialised");
     if (nmol != 0 && (nmol < 1 || nmol > c->s.ntlml)) throw Error("enysol> no such molecule");
+    if (nmol != 0 && c->s.ntlml > 1)  // enysol.f:53-59: that chain alone, as if the others were absent
+      throw Error("enysol> the solvent term of one chain among several is not provided (use nmol = 0)");
     double v = engine_enysol(c->eng, c->s);
     if (e) *e = v;
     return SMMP_OK;

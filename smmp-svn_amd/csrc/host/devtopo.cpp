@@ -6,14 +6,30 @@
 namespace smmp {
 
 HostSys build_host_sys(const State& s) {
-  if (s.ntlml != 1) throw Error("device engine: one chain per context (ntlml = 1)");
+  if (s.ntlml < 1 || s.ntlml > mxml) throw Error("device engine: no chain, or more chains than mxml");
   HostSys h;
   const int nat = s.natoms(), nvt = s.nvars();
-  if (nat < 4) throw Error("device engine: chain too short");
   if (nat > 4096) throw Error("device engine: more than 4096 atoms");
   h.nat = nat;
   h.nvt = nvt;
   h.words = (nat + 63) / 64;
+  h.nml = s.ntlml;
+  h.boxsize = s.boxsize;
+  h.mol_of.assign(nat, 0);
+  for (int m = 1; m <= h.nml; ++m) {
+    const int a1 = s.iatrs1(s.irsml1(m)), a2 = s.iatrs2(s.irsml2(m));
+    if (a2 - a1 + 1 < 4) throw Error("device engine: chain " + std::to_string(m) + " too short");
+    if (a1 != (m == 1 ? 1 : h.mol_a2[m - 2] + 2)) throw Error("device engine: chains are not numbered in order");
+    h.mol_a1[m - 1] = (int16_t)(a1 - 1);
+    h.mol_a2[m - 1] = (int16_t)(a2 - 1);
+    for (int a = a1; a <= a2; ++a) h.mol_of[a - 1] = (int16_t)(m - 1);
+    for (int k = 0; k < 6; ++k) h.gbpr[6 * (m - 1) + k] = s.gbpr(k + 1, m);
+    int b1, b2, b3;
+    first_backbone_atoms(s, m, b1, b2, b3);
+    if (b1 != a1) throw Error("device engine: a chain must start with its first backbone atom");
+    h.mol_i2[m - 1] = (int16_t)(b2 - 1);
+    h.mol_i3[m - 1] = (int16_t)(b3 - 1);
+  }
   h.parent.assign(nat, -1);
   h.type.resize(nat);
   h.q.resize(nat); h.bl.resize(nat); h.cba.resize(nat); h.sba.resize(nat);
@@ -21,8 +37,9 @@ HostSys build_host_sys(const State& s) {
   h.rvdw.resize(nat); h.sigma.resize(nat);
   for (int a = 1; a <= nat; ++a) {
     int i = a - 1;
-    h.parent[i] = (a == 1) ? -1 : (int16_t)(s.iowat(a) - 1);
-    if (a > 1 && !(s.iowat(a) >= 1 && s.iowat(a) < a))
+    const int a1 = h.mol_a1[h.mol_of[i]] + 1;  // first atom of this chain (1-based)
+    h.parent[i] = (a == a1) ? -1 : (int16_t)(s.iowat(a) - 1);
+    if (a > a1 && !(s.iowat(a) >= a1 && s.iowat(a) < a))
       throw Error("device engine: atom " + std::to_string(a) + " precedes its predecessor");
     h.type[i] = (int8_t)(s.ityat(a) - 1);
     h.q[i] = s.cgat(a);
@@ -37,10 +54,18 @@ HostSys build_host_sys(const State& s) {
   h.nheavy = (int)h.heavy.size();
   h.heavy_of.assign(nat, (int16_t)-1);
   for (int k = 0; k < h.nheavy; ++k) h.heavy_of[h.heavy[k]] = (int16_t)k;
-  // build levels: level(a) = level(parent(a)) + 1, first atom at level 0
+  for (int m = 0; m < h.nml; ++m) {
+    const auto lo = std::lower_bound(h.heavy.begin(), h.heavy.end(), h.mol_a1[m]);
+    const auto hi = std::upper_bound(h.heavy.begin(), h.heavy.end(), h.mol_a2[m]);
+    h.mol_h1[m] = (int16_t)(lo - h.heavy.begin());
+    h.mol_h2[m] = (int16_t)(hi - h.heavy.begin());
+  }
+  // build levels: level(a) = level(parent(a)) + 1, the first atom of every
+  // chain at level 0, so the chains are built side by side
   std::vector<int> lev(nat, 0);
   int maxlev = 0;
-  for (int i = 1; i < nat; ++i) {
+  for (int i = 0; i < nat; ++i) {
+    if (h.parent[i] < 0) continue;
     lev[i] = lev[h.parent[i]] + 1;
     maxlev = std::max(maxlev, lev[i]);
   }
@@ -48,17 +73,10 @@ HostSys build_host_sys(const State& s) {
   h.lev_off.assign(maxlev + 1, 0);
   for (int L = 1; L <= maxlev; ++L) {
     h.lev_off[L - 1] = (int)h.lev_atoms.size();
-    for (int i = 1; i < nat; ++i)
-      if (lev[i] == L) h.lev_atoms.push_back((int16_t)i);
+    for (int i = 0; i < nat; ++i)
+      if (h.parent[i] >= 0 && lev[i] == L) h.lev_atoms.push_back((int16_t)i);
   }
   h.lev_off[maxlev] = (int)h.lev_atoms.size();
-  // first three backbone atoms (i1 is atom 1)
-  int b1, b2, b3;
-  first_backbone_atoms(s, 1, b1, b2, b3);
-  if (b1 != 1) throw Error("device engine: the chain must start at atom 1");
-  h.i2 = b2 - 1;
-  h.i3 = b3 - 1;
-  for (int k = 0; k < 6; ++k) h.gbpr[k] = s.gbpr(k + 1, 1);
   // variables: torsion terms, controlled atoms, moved atoms
   h.var_atom.resize(nvt); h.var_base.resize(nvt); h.var_axis0.resize(nvt);
   h.var_m1.resize(nvt); h.var_m2.resize(nvt); h.var_sign.resize(nvt);
@@ -102,16 +120,17 @@ HostSys build_host_sys(const State& s) {
     h.var_m2[k] = (int16_t)m2;
     h.var_sign[k] = 1.0;
   }
-  // The torsions of the second and third backbone atoms also set the frame of
-  // the first atom (setgbl keeps bond 1->2 and the plane 1-2-3 fixed in the
-  // laboratory), so changing them turns the atoms before the moved range the
-  // other way while the moved range stays put.
+  // The torsions of the second and third backbone atoms of a chain also set
+  // the frame of its first atom (setgbl keeps bond 1->2 and the plane 1-2-3
+  // fixed in the laboratory), so changing them turns the chain's atoms
+  // before the moved range the other way while the moved range stays put.
   for (int k = 0; k < nvt; ++k) {
-    if (h.tvar[h.i2] != k && h.tvar[h.i3] != k) continue;
-    if (h.var_m2[k] != nat - 1 || h.var_m1[k] < 1)
+    const int m = h.mol_of[h.var_base[k]];
+    if (h.tvar[h.mol_i2[m]] != k && h.tvar[h.mol_i3[m]] != k) continue;
+    if (h.var_m2[k] != h.mol_a2[m] || h.var_m1[k] <= h.mol_a1[m])
       throw Error("device engine: a torsion of the first backbone atoms does not move the chain's end");
     h.var_m2[k] = (int16_t)(h.var_m1[k] - 1);
-    h.var_m1[k] = 0;
+    h.var_m1[k] = h.mol_a1[m];
     h.var_sign[k] = -1.0;
   }
   // heavy-index range of each moved range (heavy is sorted by atom)
@@ -123,9 +142,16 @@ HostSys build_host_sys(const State& s) {
     h.var_h1[k] = (int16_t)(lo - h.heavy.begin());
     h.var_h2[k] = (int16_t)(hi - h.heavy.begin());
   }
-  // interaction set from the reference lists (mklist), as symmetric bits
+  // interaction set from the reference lists (mklist) of each chain, as
+  // symmetric bits, plus every pair between chains (eninteract.f:29-80)
   h.bits.assign((size_t)nat * h.words * 2, 0ull);
-  PairList pl = expand_pairs(s, 1);
+  PairList pl;
+  for (int m = 1; m <= h.nml; ++m) {
+    PairList pm = expand_pairs(s, m);
+    pl.i.insert(pl.i.end(), pm.i.begin(), pm.i.end());
+    pl.j.insert(pl.j.end(), pm.j.begin(), pm.j.end());
+    pl.kind.insert(pl.kind.end(), pm.kind.begin(), pm.kind.end());
+  }
   for (size_t p = 0; p < pl.i.size(); ++p) {
     int i = pl.i[p] - 1, j = pl.j[p] - 1;
     if (i == j) throw Error("device engine: self pair in the interaction lists");
@@ -139,8 +165,16 @@ HostSys build_host_sys(const State& s) {
       h.bits[((size_t)j * h.words + i / 64) * 2 + 1] |= 1ull << (i % 64);
       ++h.n14;
     }
+    if (h.mol_of[i] != h.mol_of[j]) throw Error("device engine: a list pair joins two chains");
     ++h.npairs;
   }
+  for (int i = 0; i < nat; ++i)
+    for (int j = h.mol_a2[h.mol_of[i]] + 1; j < nat; ++j) {
+      h.bits[((size_t)i * h.words + j / 64) * 2 + 0] |= 1ull << (j % 64);
+      h.bits[((size_t)j * h.words + i / 64) * 2 + 0] |= 1ull << (i % 64);
+      ++h.ninter;
+    }
+  h.npairs += h.ninter;
   // parameter tables
   h.pA.resize(kTypes * kTypes); h.pA14.resize(kTypes * kTypes); h.pC.resize(kTypes * kTypes); h.pHB.resize(kTypes * kTypes);
   for (int ti = 1; ti <= kTypes; ++ti)

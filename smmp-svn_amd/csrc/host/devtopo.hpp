@@ -1,7 +1,8 @@
-// Device-ready description of one single-chain system, derived from the
-// host State after init_molecule: the chain-build schedule, the interaction
-// set as symmetric 2-plane bit rows, the torsion terms, the solvent atoms
-// and, per variable, the atoms a torsion change moves.
+// Device-ready description of one system of one or more chains, derived
+// from the host State after init_molecule: the chain-build schedule, the
+// interaction set as symmetric 2-plane bit rows (pairs within a chain from
+// the reference's lists, every pair between chains), the torsion terms, the
+// solvent atoms and, per variable, the atoms a torsion change moves.
 #pragma once
 #include <cstdint>
 #include <vector>
@@ -14,11 +15,17 @@ constexpr int kTypes = mxtyat;  // 18 ECEPP atom types
 
 struct HostSys {
   int nat = 0, nvt = 0, nlev = 0, words = 0, nheavy = 0;
-  int i2 = 0, i3 = 0;  // 0-based second/third backbone atoms (bldmol fnd3ba)
-  double gbpr[6] = {0, 0, 0, 0, 0, 0};
+  // chains (0-based atoms): first and last atom, second and third backbone
+  // atoms (bldmol fnd3ba), heavy-index range [h1, h2), global parameters
+  int nml = 0;
+  int16_t mol_a1[mxml] = {}, mol_a2[mxml] = {}, mol_i2[mxml] = {}, mol_i3[mxml] = {};
+  int16_t mol_h1[mxml] = {}, mol_h2[mxml] = {};
+  double gbpr[6 * mxml] = {};
+  double boxsize = 0;
+  std::vector<int16_t> mol_of;      // per atom: its chain
   // atoms (0-based)
-  std::vector<int16_t> parent;      // iowat-1, -1 for the first atom
-  std::vector<int16_t> lev_atoms;   // atoms in build order (first atom excluded)
+  std::vector<int16_t> parent;      // iowat-1, -1 for the first atom of a chain
+  std::vector<int16_t> lev_atoms;   // atoms in build order (first atoms excluded)
   std::vector<int32_t> lev_off;     // nlev+1 offsets into lev_atoms
   std::vector<int8_t> type;         // ityat-1
   std::vector<double> q, bl, cba, sba, toat0, tref, rvdw, sigma;
@@ -35,7 +42,7 @@ struct HostSys {
   // interaction set: bits[(i*words + w)*2 + plane], plane 0 = pair counted,
   // plane 1 = 1-4 pair (half repulsion); symmetric in (i, j)
   std::vector<uint64_t> bits;
-  int64_t npairs = 0, n14 = 0;
+  int64_t npairs = 0, n14 = 0, ninter = 0;  // ninter: pairs between chains
   std::vector<int16_t> heavy;       // atoms with rvdw > 0
   std::vector<int16_t> heavy_of;    // per atom: its index in 'heavy', or -1
   // parameters, [ti*18 + tj]: A (aij or ahb), A14 (a14 or ahb), C (cij or chb), hb flag
@@ -46,8 +53,8 @@ struct HostSys {
   std::vector<double> spoint;       // npnt*3, point-major (x,y,z)
 };
 
-// Builds the description for molecule 1 of s (single chain).  Throws Error
-// when a list pair occurs twice or a moving set is not contiguous.
+// Builds the description of every chain of s.  Throws Error when a list
+// pair occurs twice or a moving set is not contiguous.
 HostSys build_host_sys(const State& s);
 
 }  // namespace smmp

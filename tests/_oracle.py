@@ -30,12 +30,15 @@ class Topo(C.Structure):
                 + [(n, F64P) for n in ("e0to", "sgto", "rnto", "spoint")]
                 + [(n, C.c_double) for n in ("conv", "plt", "slp")]
                 + [(n, C.c_int) for n in ("epsd", "itysol", "i2", "i3")]
-                + [("gbpr", C.c_double * 6)])
+                + [("gbpr", C.c_double * 6), ("nml", C.c_int)]
+                + [(n, C.c_int * 10) for n in ("mol_a1", "mol_a2", "mol_i2", "mol_i3")]
+                + [("gbm", C.c_double * 60), ("boxsize", C.c_double)])
 
 
 class Conf(C.Structure):
     _fields_ = [(n, F64P) for n in ("toat", "sntoat", "cstoat", "snbaat", "csbaat", "xat", "yat", "zat",
-                                    "xtoat", "ytoat", "ztoat", "xbaat", "ybaat", "zbaat")] + [("e", C.c_double * 9)]
+                                    "xtoat", "ytoat", "ztoat", "xbaat", "ybaat", "zbaat")] + [
+        ("e", C.c_double * 9), ("ei", C.c_double * 4), ("gb", C.c_double * 60)]
 
 
 class MT(C.Structure):
@@ -104,7 +107,7 @@ class OracleMT:
 
 
 class OracleSystem:
-    """One chain for the oracle, from a product context (smmp_amd._ffi.Context)."""
+    """The chains of a product context (smmp_amd._ffi.Context) for the oracle."""
 
     def __init__(self, ctx):
         v = ctx.view
@@ -112,12 +115,16 @@ class OracleSystem:
         # init_molecule leaves from before the terminal groups were added
         vl0 = np.ascontiguousarray(v("var_r.vlvr"), dtype=np.float64)
         from smmp_amd import _ffi
-        ctx.check(_ffi.lib.smmp_setvar(ctx.h, 1, vl0.ctypes.data_as(F64P)))
-        nat = int(v("res_i.iatrs2")[int(v("mol_i.irsml2")[0]) - 1])
-        nvt = int(v("mol_i.ivrml1")[0] + v("mol_i.nvrml")[0] - 1)
-        nms = int(v("mol_i.imsml1")[0] + v("mol_i.nmsml")[0] - 1)
-        nvw = int(v("eny_i.ivwml1")[0] + v("eny_i.nvwml")[0] - 1)
-        n14 = int(v("eny_i.i14ml1")[0] + v("eny_i.n14ml")[0] - 1)
+        nml = int(v("mol_i.ntlml")[0])
+        for m in range(1, nml + 1):
+            ctx.check(_ffi.lib.smmp_setvar(ctx.h, m, vl0.ctypes.data_as(F64P)))
+        last = nml - 1
+        nat = int(v("res_i.iatrs2")[int(v("mol_i.irsml2")[last]) - 1])
+        nvt = int(v("mol_i.ivrml1")[last] + v("mol_i.nvrml")[last] - 1)
+        nms = int(v("mol_i.imsml1")[last] + v("mol_i.nmsml")[last] - 1)
+        nvw = int(v("eny_i.ivwml1")[last] + v("eny_i.nvwml")[last] - 1)
+        n14 = int(v("eny_i.i14ml1")[last] + v("eny_i.n14ml")[last] - 1)
+        self.nml = nml
         self.nat, self.nvt = nat, nvt
         npnt = int(v("pnt.npnt")[0])
         itysol = int(v("isolty.itysol")[0])
@@ -154,6 +161,14 @@ class OracleSystem:
         t.i2, t.i3 = int(ix[1, 0]), int(ix[2, 0])
         for k in range(6):
             t.gbpr[k] = float(v("mol_r.gbpr")[k, 0])
+        t.nml = nml
+        for m in range(nml):
+            t.mol_a1[m] = int(v("res_i.iatrs1")[int(v("mol_i.irsml1")[m]) - 1])
+            t.mol_a2[m] = int(v("res_i.iatrs2")[int(v("mol_i.irsml2")[m]) - 1])
+            t.mol_i2[m], t.mol_i3[m] = int(ix[1, m]), int(ix[2, m])
+            for k in range(6):
+                t.gbm[6 * m + k] = float(v("mol_r.gbpr")[k, m])
+        t.boxsize = float(v("bounds.boxsize")[0])
         self.t = t
         self.nvr = int(v("mol_par.nvr")[0])
         self.idvr = i32(v("var_i.idvr")[:self.nvr])
@@ -177,6 +192,17 @@ class OracleSystem:
         vl = f64(vlvr_all)
         lib.oracle_energy(C.byref(self.t), C.byref(self.c), vl.ctypes.data_as(F64P))
         return np.array(self.c.e[:], dtype=np.float64)
+
+    def inter(self) -> np.ndarray:
+        """[eysmi, eyeli, eyvwi, eyhbi] of the last energy"""
+        return np.array(self.c.ei[:], dtype=np.float64)
+
+    def gbpr(self) -> np.ndarray:
+        return np.array(self.c.gb[:6 * self.nml], dtype=np.float64)
+
+    def set_gbpr(self, gb) -> None:
+        for k, x in enumerate(np.asarray(gb, dtype=np.float64).ravel()):
+            self.c.gb[k] = float(x)
 
     def coords(self) -> np.ndarray:
         n = self.nat

@@ -21,8 +21,9 @@ def golden(name: str):
     return np.load(os.path.join(GOLDEN, name), allow_pickle=False)
 
 
-def make_ctx(seq=None, tag=None, itysol=1, varfile=" ", tmpdir=None, sh2=0):
-    """Product context (ECEPP/3 as universe.py sets it up) for a sequence."""
+def make_ctx(seq=None, tag=None, itysol=1, varfile=" ", tmpdir=None, sh2=0, seqfile=None):
+    """Product context (ECEPP/3 as universe.py sets it up) for a sequence
+    (a residue list, a fixture tag or a sequence file)."""
     from smmp_amd import _ffi, workloads
     c = _ffi.Context()
     c.view("epar_l.flex")[0] = 0
@@ -36,6 +37,8 @@ def make_ctx(seq=None, tag=None, itysol=1, varfile=" ", tmpdir=None, sh2=0):
             seqf = os.path.join(GOLDEN, "enkefa.seq")
         else:
             seq = list(golden(f"topo_{tag}.npz")["seq_in"])
+    if seqfile is not None:
+        seqf = seqfile
     if seq is not None:
         import tempfile
         d = tmpdir or tempfile.mkdtemp(prefix="smmp_t_")
@@ -55,3 +58,27 @@ def has_gpu() -> bool:
 @pytest.fixture(scope="session")
 def golden_dir():
     return GOLDEN
+
+
+def write_multi_seq(path: str, chains) -> str:
+    """A sequence file with one '#'-headed block per chain (redseq.f)."""
+    with open(path, "w") as f:
+        for k, ch in enumerate(chains):
+            f.write(f"#  chain {k + 1}\n")
+            f.write("   " + " ".join(r.capitalize() for r in ch) + "\n")
+    return path
+
+
+def make_multi_ctx(tag: str):
+    """Context of a multi-chain fixture (tests/golden/multi_<tag>.npz) with
+    the fixture's global parameters and box; returns (context, fixture)."""
+    import tempfile
+    g = golden(f"multi_{tag}.npz")
+    chains = [str(c).split("-") for c in g["chains"]]
+    d = tempfile.mkdtemp(prefix="smmp_m_")
+    c = make_ctx(itysol=int(g["itysol_in"][0]), seqfile=write_multi_seq(os.path.join(d, "m.seq"), chains))
+    gb = c.view("mol_r.gbpr")
+    nml = len(chains)
+    gb[:, :nml] = np.asarray(g["gbpr"]).reshape(nml, 6).T
+    c.view("bounds.boxsize")[0] = float(g["boxsize_in"][0])
+    return c, g

@@ -7,7 +7,8 @@ directory and stores its outputs as small ``.npz`` files (loadable with
 ``allow_pickle=False``).  Needs /root/reference; run it in the build container,
 never on a GPU box.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py            # every fixture
+    python tests/golden/make_golden.py multi      # only the named groups
 """
 from __future__ import annotations
 
@@ -112,10 +113,62 @@ def topo_fixture(tag: str, seq: list[str], tmp: str, nconf: int, seed: int,
     return topo
 
 
-def main() -> None:
+MULTI_CASES = {
+    # two Met-enkephalins (EXAMPLES/enkefatimes2.seq), OONS, a box that the
+    # second chain's position reaches (metropolis.f:100-106)
+    "enk2": dict(chains=[["tyr", "gly", "gly", "phe", "met"]] * 2, itysol=1, boxsize=9.2,
+                 gbpr={2: (9.0, -3.0, 2.5, 0.7, -0.4, 1.9)}, seed=4357, temp=300.0, nequi=30, nsweep=8),
+    # three chains of different lengths, vacuum, no box
+    "three": dict(chains=[["tyr", "gly", "gly", "phe", "met"], ["ala", "gly", "ser"], ["val", "lys", "trp", "glu"]],
+                  itysol=0, boxsize=0.0,
+                  gbpr={2: (7.5, 1.0, -2.0, -1.2, 0.3, 2.6), 3: (-2.0, 8.0, 3.5, 2.2, -1.1, -0.5)},
+                  seed=81236, temp=400.0, nequi=30, nsweep=8),
+}
+
+
+def write_multi_seq(path: str, chains: list[list[str]]) -> str:
+    """A sequence file with one '#'-headed block per chain (redseq.f)."""
+    with open(path, "w") as f:
+        for k, ch in enumerate(chains):
+            f.write(f"#  chain {k + 1}\n")
+            f.write("   " + " ".join(r.capitalize() for r in ch) + "\n")
+    return path
+
+
+def multi_fixture(tag: str, case: dict, tmp: str) -> None:
+    seqf = write_multi_seq(os.path.join(tmp, f"multi_{tag}.seq"), case["chains"])
+    gbf = os.path.join(tmp, f"multi_{tag}.gb")
+    with open(gbf, "w") as f:
+        for iml, vals in case["gbpr"].items():
+            for k, v in enumerate(vals):
+                f.write(f"{iml} {k + 1} {v!r}\n")
+    out = run(["multi", LIBDIR, seqf, str(case["itysol"]), gbf, repr(case["boxsize"]), str(case["seed"]), "4",
+               repr(case["temp"]), str(case["nequi"]), str(case["nsweep"])], tmp)
+    arrays = {k: v[0] for k, v in out.items() if len(v) == 1}
+    # energies: the start state, then the random conformations
+    arrays["energies"] = np.stack(out["energy"])
+    arrays["inter"] = np.stack(out["inter"])
+    arrays["asa_all"] = np.stack(out["asa"])
+    arrays["xyz_all"] = np.stack(out["xyz"])
+    arrays["conf_vlvr"] = np.stack(out["vlvr"])
+    arrays["sweeps"] = np.stack(out["sweep"])
+    arrays["chains"] = np.array(["-".join(c) for c in case["chains"]], dtype="<U64")
+    arrays["itysol_in"] = np.array([case["itysol"]])
+    arrays["boxsize_in"] = np.array([case["boxsize"]])
+    arrays["temp_in"] = np.array([case["temp"]])
+    arrays["gbpr_in"] = np.array([[iml] + list(v) for iml, v in case["gbpr"].items()])
+    save(f"multi_{tag}.npz", **arrays)
+
+
+def main(only: list[str]) -> None:
     if not os.path.exists(DRIVER):
         raise SystemExit("build the reference first: make -C oracle ref")
     tmp = tempfile.mkdtemp(prefix="smmp_golden_")
+    if not only or "multi" in only:
+        for tag, case in MULTI_CASES.items():
+            multi_fixture(tag, case, tmp)
+    if only:
+        return
 
     # MT19937 streams (twister.f): the default seed and a few others.
     mt = {}
@@ -188,4 +241,4 @@ def main() -> None:
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])
