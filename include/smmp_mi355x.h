@@ -136,6 +136,10 @@ typedef struct {
                              multiple); the graph stays with the context and later
                              calls with the same configuration replay it without
                              capturing.  0: launched directly. */
+  double *gbpr;           /* several chains: NULL, or R x ntlml x 6 global parameters
+                             (gbpr(1..6, chain) of each replica; in: start, out: final).
+                             NULL: every replica starts from the context's gbpr.  The
+                             sweeps end with the rigid trials of metropolis.f:94-168. */
 } smmp_sampler_cfg;
 int smmp_sample(smmp_ctx *ctx, const smmp_sampler_cfg *cfg, double *vlvr, double *eol,
                 double *acc, int32_t *ipoi, int32_t *hist, double *traj);

@@ -145,11 +145,11 @@ __device__ void energy_stage_torsions(const DevSys& S, const double* vl, EnergyL
   }
 }
 
-// bldmol.f:303-420 (setgbl) and the first atom of chain m from its global
-// parameters g[0..5]; one lane.
-__device__ void build_root(const DevSys& S, EnergyLds& L, const double* g, int m) {
-  double ca = cos(g[3]), sa = sin(g[3]), cb = cos(g[4]), sb = sin(g[4]), cg = cos(g[5]), sg = sin(g[5]);
-  double ag[3][3], d;
+// bldmol.f:303-369 (setgbl): the global axes J, K, L (columns of ag) from
+// the Euler angles e[0..2] = gbpr(4..6).
+__device__ void euler_frame(const double* e, double ag[3][3]) {
+  double ca = cos(e[0]), sa = sin(e[0]), cb = cos(e[1]), sb = sin(e[1]), cg = cos(e[2]), sg = sin(e[2]);
+  double d;
   double x1 = ca * cg - sa * sb * sg, x2 = sa * cg + ca * sb * sg, x3 = -cb * sg;
   d = sqrt(x1 * x1 + x2 * x2 + x3 * x3);
   ag[0][0] = x1 / d; ag[1][0] = x2 / d; ag[2][0] = x3 / d;
@@ -159,6 +159,14 @@ __device__ void build_root(const DevSys& S, EnergyLds& L, const double* g, int m
   double z1 = ca * sg + sa * sb * cg, z2 = sa * sg - ca * sb * cg, z3 = cb * cg;
   d = sqrt(z1 * z1 + z2 * z2 + z3 * z3);
   ag[0][2] = z1 / d; ag[1][2] = z2 / d; ag[2][2] = z3 / d;
+}
+
+// bldmol.f:303-420 (setgbl) and the first atom of chain m from its global
+// parameters g[0..5]; one lane.
+__device__ void build_root(const DevSys& S, EnergyLds& L, const double* g, int m) {
+  double ag[3][3];
+  euler_frame(g + 3, ag);
+  double x1, x2, x3, y1, y3, z1, z2, z3;
   const int a1 = S.mol_a1[m], i2 = S.mol_i2[m], i3 = S.mol_i3[m];
   double ct2 = L.fr[6 * i2 + 0], st2 = L.fr[6 * i2 + 1], sa2 = S.sba[i2];
   x1 = -S.cba[i2]; x2 = -sa2 * ct2; x3 = -sa2 * st2;

@@ -135,7 +135,10 @@ enum PhaseId { kPhPropose = 0, kPhRotate, kPhPairs, kPhSolvent, kPhReduce, kPhAc
                // pair change, per trial: slowest wave and wave sum of pair_delta,
                // wave sum of the contact marking, thread 0's torsion term
                kPhPairMax, kPhPairSum, kPhMarkSum, kPhTorsion, kNPhase };
-constexpr int kScDoubles = 160;  // broadcast scalars [0,32), phase clocks [32,64), per-wave stats [64,160)
+// broadcast scalars [0,32), phase clocks [32,64), per-wave stats [64,160),
+// rigid-move transform [160,176)
+constexpr int kScDoubles = 176;
+constexpr int kScRigid = 160;
 
 // Extra LDS of the sampler beyond the energy carve-up.
 __host__ __device__ inline size_t sampler_lds_bytes(int np, int npnt, int nheavy, int nvt, int nwaves, size_t nbits) {
@@ -329,10 +332,10 @@ __device__ void pair_delta(const DevSys& S, const SamplerLds& Q, int m1, int m2,
 // even a rigidly moved environment changes its discretised area) and every
 // fixed atom in contact with a moved one before or after the move.  Only
 // the fixed atoms of this block's share (h % G == g) are tested.
-__device__ void mark_affected(const DevSys& S, const SamplerLds& Q, int jv, int m1, int g, int G) {
+__device__ void mark_affected(const DevSys& S, const SamplerLds& Q, int h1, int h2, int m1, int g, int G) {
   const int nh = S.nheavy;
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave, nwaves = blockDim.x / kWave;
-  const int h1 = Q.V.h1[jv], h2 = Q.V.h2[jv];  // heavy atoms of the moved range
+  // h1, h2: heavy atoms of the moved range
   const int nM = h2 - h1, nown = (nh - g + G - 1) / G;
   for (int h = h1 + threadIdx.x; h < h2; h += blockDim.x) Q.aff[h] = 1;
   // a wave per moved heavy atom, lanes over this share's heavy atoms; the
@@ -595,9 +598,51 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
   const int traj_base = P.sweep_base ? *P.sweep_base : P.traj_base;
   for (int sw = 0; sw < P.nsweep; ++sw) {
     if (tid == 0) Q.sc[14] = P.beta[P.slot_of[r]];  // LDS: no global reload inside the loop
-    for (int nsw = 0; nsw < P.nvr; ++nsw) {
+    // torsion trials, then with several chains six rigid trials per chain
+    const int ntrial = P.nvr + (S.nml > 1 ? 6 * S.nml : 0);
+    for (int nsw = 0; nsw < ntrial; ++nsw) {
+      const bool rigid = nsw >= P.nvr;  // uniform
       // ---- proposal (leader): metropolis.f:42-66; identical in the G blocks
-      if (tid == 0) {
+      if (tid == 0 && rigid) {
+        // metropolis.f:94-168: chain m, parameter i of gbpr; the move is
+        // x -> c + M (x - c) + t over the chain's atoms
+        const int q = nsw - P.nvr, m = q / 6, i = q - 6 * m;
+        double* gm = Q.gb + 6 * m;
+        const double gbol = gm[i];
+        double gnew;
+        double* T = Q.sc + kScRigid;  // [0..2] c, [3..11] M (row-major), [12..14] t
+        for (int k = 0; k < 15; ++k) T[k] = 0.0;
+        if (i < 3) {  // shift of the position, kept in the box
+          gnew = gbol + (mt_grnd(Q.mt, &Q.mt[624]) - 0.5);
+          if (S.boxsize > 0.0) {
+            if (gnew > S.boxsize)
+              gnew = S.boxsize;
+            else if (gnew < -S.boxsize)
+              gnew = -S.boxsize;
+          }
+          T[3] = T[7] = T[11] = 1.0;
+          T[12 + i] = gnew - gbol;
+        } else {  // a new Euler angle: turn about the first atom from the old axes to the new
+          gnew = (mt_grnd(Q.mt, &Q.mt[624]) - 0.5) * (i == 4 ? kPi : kPi2);
+          double eo[3] = {gm[3], gm[4], gm[5]}, en[3] = {gm[3], gm[4], gm[5]};
+          en[i - 3] = gnew;
+          double ao[3][3], an[3][3];
+          euler_frame(eo, ao);
+          euler_frame(en, an);
+          for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) T[3 + 3 * a + b] = an[a][0] * ao[b][0] + an[a][1] * ao[b][1] + an[a][2] * ao[b][2];
+          const int a1 = S.mol_a1[m];
+          T[0] = L.x[a1];
+          T[1] = L.y[a1];
+          T[2] = L.z[a1];
+        }
+        Q.sc[2] = gnew;
+        Q.sc[15] = gbol;
+        Q.sc[6] = (double)(-1 - q);  // trial id: negative for rigid moves
+        Q.sc[5] = fabs(Q.sc[0]) >= 1e5 ? 1.0 : 0.0;
+        Q.sc[17] = Q.sc[18] = 0.0;  // no torsion term changes
+      }
+      if (tid == 0 && !rigid) {
         uint32_t* mti = &Q.mt[624];
         int ivar = nsw;
         if (P.rndord) {  // 1+int(nvr*grnd()); u = 1 exactly would index past the end
@@ -621,8 +666,20 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
       }
       __syncthreads();
       lap(kPhPropose);
-      const int jv = (int)Q.sc[6];
-      const int m1 = V.m1[jv], m2 = V.m2[jv];
+      const int jv = (int)Q.sc[6];  // variable, or -1 - (6 chain + parameter)
+      int m1, m2, h1, h2;
+      if (!rigid) {
+        m1 = V.m1[jv];
+        m2 = V.m2[jv];
+        h1 = V.h1[jv];
+        h2 = V.h2[jv];
+      } else {
+        const int m = (-1 - jv) / 6;
+        m1 = S.mol_a1[m];
+        m2 = S.mol_a2[m];
+        h1 = S.mol_h1[m];
+        h2 = S.mol_h2[m];
+      }
       const bool full = Q.sc[5] != 0.0;
       EnergyOut vref;  // verify mode (G = 1): full energy before the move (thread 0)
       if (P.verify && !full) {
@@ -630,8 +687,32 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
         vref = re.o;
         __syncthreads();
       }
-      // ---- rigid rotation of the moved atoms about the bond (Rodrigues)
-      {
+      if (rigid) {  // ---- rigid move of a whole chain
+        const double* T = Q.sc + kScRigid;
+        const double cx = T[0], cy = T[1], cz = T[2];
+        if (tid == 0 && g == 0) wk.moved += m2 - m1 + 1;
+        for (int a = m1 + tid; a <= m2; a += blockDim.x) {
+          const double vx = L.x[a] - cx, vy = L.y[a] - cy, vz = L.z[a] - cz;
+          Q.bx[a - m1] = L.x[a];
+          Q.by[a - m1] = L.y[a];
+          Q.bz[a - m1] = L.z[a];
+          const double xn = cx + (T[3] * vx + T[4] * vy + T[5] * vz) + T[12];
+          const double yn = cy + (T[6] * vx + T[7] * vy + T[8] * vz) + T[13];
+          const double zn = cz + (T[9] * vx + T[10] * vy + T[11] * vz) + T[14];
+          L.x[a] = xn;
+          L.y[a] = yn;
+          L.z[a] = zn;
+          const int h = L.hidx[a];
+          if (h >= 0) {
+            L.hp[4 * h + 0] = xn;
+            L.hp[4 * h + 1] = yn;
+            L.hp[4 * h + 2] = zn;
+          }
+        }
+        for (int h = tid; h < nh; h += blockDim.x) Q.aff[h] = 0;
+        if (tid == 0) Q.gb[-1 - jv] = Q.sc[2];
+      } else {
+        // ---- rigid rotation of the moved atoms about the bond (Rodrigues)
         double sn, cs;
         sincos(V.sign[jv] * Q.sc[3], &sn, &cs);
         const double px = Q.sc[8], py = Q.sc[9], pz = Q.sc[10];
@@ -694,7 +775,7 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
           pst[0] += now - tpa;
           tpa = now;
         }
-        if (S.itysol > 0) mark_affected(S, Q, jv, m1, g, G);
+        if (S.itysol > 0) mark_affected(S, Q, h1, h2, m1, g, G);
         if (pst && lane == 0) {
           const double now = (double)wall_clock64();
           pst[1] += now - tpa;
@@ -861,6 +942,8 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
         if (acc) {
           Q.sc[0] = enew;
           Q.sc[1] += 1.0;
+        } else if (rigid) {
+          Q.gb[-1 - jv] = Q.sc[15];
         } else {
           Q.vl[jv] = Q.sc[15];
         }

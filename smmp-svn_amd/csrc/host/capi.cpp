@@ -259,7 +259,7 @@ int smmp_sample(smmp_ctx* c, const smmp_sampler_cfg* cfg, double* vlvr, double* 
     if (!cfg || !vlvr || !eol || !acc || !ipoi) throw Error("sample> NULL argument");
     if (cfg->nrep < 1 || cfg->nsweep < 0 || !cfg->temps || !cfg->seeds) throw Error("sample> bad configuration");
     if (cfg->mode == 2 && !cfg->w) throw Error("sample> multicanonical mode needs a weight");
-    if (c->s.ntlml != 1) throw Error("sample> the replica sampler handles one chain");
+    if (c->s.ntlml < 1) throw Error("sample> no molecule initialised");
     return engine_sample(c->eng, c->s, *cfg, vlvr, eol, acc, ipoi, hist, traj);
   });
 }

@@ -82,7 +82,8 @@ HostSys build_host_sys(const State& s) {
   h.var_m1.resize(nvt); h.var_m2.resize(nvt); h.var_sign.resize(nvt);
   h.var_e0.resize(nvt); h.var_sg.resize(nvt); h.var_rn.resize(nvt); h.var_is_torsion.resize(nvt);
   std::vector<std::vector<int>> children(nat);
-  for (int i = 1; i < nat; ++i) children[h.parent[i]].push_back(i);
+  for (int i = 0; i < nat; ++i)
+    if (h.parent[i] >= 0) children[h.parent[i]].push_back(i);
   for (int v = 1; v <= nvt; ++v) {
     int k = v - 1;
     int iat = s.iatvr(v), ic = s.iclvr(v);

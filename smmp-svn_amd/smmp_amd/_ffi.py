@@ -36,7 +36,7 @@ class SamplerCfg(C.Structure):
     _fields_ = [("nrep", C.c_int32), ("nsweep", C.c_int32), ("mode", C.c_int32), ("precision", C.c_int32),
                 ("temps", C.POINTER(C.c_double)), ("seeds", C.POINTER(C.c_uint32)),
                 ("w", C.POINTER(Weight)), ("resync_every", C.c_int32), ("verify", C.c_int32),
-                ("groups", C.c_int32), ("graph_sweeps", C.c_int32)]
+                ("groups", C.c_int32), ("graph_sweeps", C.c_int32), ("gbpr", C.POINTER(C.c_double))]
 
 
 def _load():
@@ -253,10 +253,12 @@ class Context:
 
     def sample(self, temps, seeds, nsweep: int, vlvr, mode: int = 0, weight: Weight | None = None,
                precision: int = 0, with_traj: bool = True, allow_drift: bool = False, verify: bool = False,
-               groups: int = 0, graph_sweeps: int = 0):
+               groups: int = 0, graph_sweeps: int = 0, gbpr=None):
         """Independent-stream replica sampler (smmp_sample).  groups: workgroups
         per replica (0 automatic, -1 measured); graph_sweeps: sweeps per
-        replay of a captured graph (0: direct launches)."""
+        replay of a captured graph (0: direct launches); gbpr: several chains,
+        R x ntlml x 6 start values of the chains' global parameters (returned
+        as "gbpr"; None: every replica starts from the context's)."""
         temps = np.ascontiguousarray(temps, dtype=np.float64)
         R = len(temps)
         seeds = np.ascontiguousarray(seeds, dtype=np.uint32)
@@ -274,12 +276,17 @@ class Context:
                          temps=dptr(temps), seeds=seeds.ctypes.data_as(C.POINTER(C.c_uint32)),
                          w=C.pointer(weight) if weight is not None else None, resync_every=1,
                          verify=1 if verify else 0, groups=groups, graph_sweeps=graph_sweeps)
+        nml = int(self.view("mol_i.ntlml")[0])
+        gb = None
+        if gbpr is not None:
+            gb = np.ascontiguousarray(gbpr, dtype=np.float64).reshape(R, nml, 6).copy()
+            cfg.gbpr = dptr(gb)
         rc = lib.smmp_sample(self.h, C.byref(cfg), dptr(vl), dptr(eol), dptr(acc), iptr(ipoi),
                              iptr(hist) if weight is not None else None,
                              dptr(traj) if traj is not None else None)
         if rc != 0 and not (allow_drift and rc == -4):
             self.check(rc)
-        return {"vlvr": vl, "eol": eol, "acc": acc, "ipoi": ipoi, "hist": hist, "rc": rc,
+        return {"vlvr": vl, "eol": eol, "acc": acc, "ipoi": ipoi, "hist": hist, "rc": rc, "gbpr": gb,
                 "traj": traj[:nsweep * R].reshape(nsweep, R) if traj is not None else None}
 
 
