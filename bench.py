@@ -49,10 +49,10 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def make_system(tmp):
+def make_system(tmp, length=LENGTH, seed=SEQ_SEED, tag="c2"):
     from smmp_amd import _ffi, workloads
-    seq = workloads.random_sequence(LENGTH, SEQ_SEED)
-    seqf = workloads.write_seq_file(os.path.join(tmp, "c2.seq"), seq, "c2")
+    seq = workloads.random_sequence(length, seed)
+    seqf = workloads.write_seq_file(os.path.join(tmp, f"{tag}.seq"), seq, tag)
     c = _ffi.Context()
     for k, v in (("epar_l.flex", 0), ("epar_l.sh2", 0), ("epar_l.epsd", 0), ("epar_l.ientyp", 0),
                  ("isolty.itysol", 1)):
@@ -104,6 +104,91 @@ def cpu_baseline(seqf, budget_s: float):
                       f"(ECEPP/3+OONS, {TMIN:g}-{TMAX:g} K), one host core, {dt:.1f} s"}
 
 
+def run_da(args):
+    """Delayed acceptance against the plain sampler on parallel tempering
+    with 128 replicas of a random 36-residue sequence (seed 5), ECEPP/3 +
+    OONS, 250-1000 K: the network is trained online on the sampler's own
+    energies, then both samplers run the same sweeps from the same state."""
+    from scipy.stats import ks_2samp
+    from smmp_amd import _ffi, workloads
+    from smmp_amd.surrogate import Surrogate
+    R, L = 128, 36
+    tmp = tempfile.mkdtemp(prefix="smmp_da_")
+    c, seq, _ = make_system(tmp, length=L, seed=5, tag="c5")
+    temps = workloads.geometric_temperatures(R, TMIN, TMAX)
+    seeds = lambda k: np.arange(k, k + R + 1, dtype=np.uint32)  # noqa: E731
+    log(f"delayed acceptance: {R} replicas x {L} residues, {c.nvr} free torsions")
+    t0 = time.perf_counter()
+    state = c.sample(temps, seeds(1), args.da_warmup, random_starts(c, R, 12345), mode=1, with_traj=False)["vlvr"]
+    log(f"warm-up {args.da_warmup} sweeps: {time.perf_counter() - t0:.1f} s")
+    net = Surrogate(c, seed=3)
+    Vs, Es = [], []
+    t_collect = t_train = 0.0
+    hist = np.zeros(0)
+    for rnd in range(args.da_rounds):
+        t0 = time.perf_counter()
+        d = c.sample(temps, seeds(100 + 7 * rnd), args.da_collect, state, mode=1, with_traj=False, snap_every=1)
+        t_collect += time.perf_counter() - t0
+        state = d["vlvr"]
+        Vs.append(d["snap_vlvr"].reshape(-1, c.nvt))
+        Es.append(d["snap_e"].ravel())
+        V, E = np.concatenate(Vs), np.concatenate(Es)
+        t0 = time.perf_counter()
+        hist = net.fit(V, E, epochs=args.da_epochs, batch=512, lr=2e-3, seed=rnd,
+                       clip=float(np.quantile(E, 0.99)), rescale=(rnd == 0))
+        t_train += time.perf_counter() - t0
+        log(f"round {rnd}: {len(E)} energies, loss {hist[0]:.4f} -> {hist[-1]:.4f} (network units^2), "
+            f"scale {net.scale:.2f} kcal/mol")
+    K = args.steps
+    runs = {}
+    for name, fn in (("plain", lambda: c.sample(temps, seeds(900), K, state, mode=1)),
+                     ("da", lambda: net.sample(temps, seeds(900), K, state, mode=1))):
+        c.profile(1)
+        out = fn()
+        ks = c.profile_read(_ffi.KERNEL_SWEEP)
+        c.profile(False)
+        runs[name] = (out, ks["ms"] / 1e3)
+        log(f"{name}: {K} sweeps in {ks['ms']:.1f} ms of sweep kernel, accepted {out['acc'].sum():.0f}")
+    (plain, tp), (da, td) = runs["plain"], runs["da"]
+    burn, thin = K // 5, 5
+    a, b = plain["traj"][burn::thin], da["traj"][burn::thin]
+    ks_pooled = float(ks_2samp(a.ravel(), b.ravel()).pvalue)
+    slot_p = np.array([ks_2samp(a[:, k], b[:, k]).pvalue for k in range(R)])
+    st = da["da_stats"]
+    ntrial = float(R * K * c.nvr)
+    gain = (da["acc"].sum() / td) / (plain["acc"].sum() / tp)
+    line = {
+        "metric": "DA-surrogate vs plain GPU: exact-energy-equivalent accepted moves/s",
+        "value": round(float(gain), 3),
+        "unit": "x",
+        "n_gpus": 1,
+        "steps": K,
+        "higher_is_better": True,
+        "dtype": "f64 (energies), bf16 (network)",
+        "data": "synthetic: random 36-residue sequence (seed 5), random initial dihedrals; network trained on the "
+                "sampler's own energies",
+        "config": {"workload": f"parallel tempering, {R} replicas x {L}-residue random sequence, ECEPP/3+OONS, "
+                               f"T geometric {TMIN:g}-{TMAX:g} K, exchange every sweep",
+                   "replicas": R, "residues": L, "free_torsions": c.nvr},
+        "accepted_per_s_plain": round(float(plain["acc"].sum() / tp), 1),
+        "accepted_per_s_da": round(float(da["acc"].sum() / td), 1),
+        "sweeps_per_s_plain": round(R * K / tp, 1),
+        "sweeps_per_s_da": round(R * K / td, 1),
+        "acceptance_plain": round(float(plain["acc"].sum() / ntrial), 4),
+        "acceptance_da": round(float(da["acc"].sum() / ntrial), 4),
+        "rejected_by_network": round(float(st[:, 1].sum() / st[:, 0].sum()), 4),
+        "exact_energies_per_accept_plain": round(ntrial / float(plain["acc"].sum()), 3),
+        "exact_energies_per_accept_da": round(float(st[:, 2].sum() / max(st[:, 3].sum(), 1.0)), 3),
+        "ks_pooled_p": round(ks_pooled, 4),
+        "ks_per_temperature_median_p": round(float(np.median(slot_p)), 4),
+        "ks_per_temperature_frac_p_above_0.05": round(float(np.mean(slot_p > 0.05)), 3),
+        "training": {"energies": int(sum(len(e) for e in Es)), "rounds": args.da_rounds,
+                     "epochs_per_round": args.da_epochs, "final_loss": round(float(hist[-1]), 5),
+                     "collect_s": round(t_collect, 2), "train_s": round(t_train, 2)},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,6 +202,13 @@ def main():
                     help="workgroups per replica: 0 automatic, -1 measured (autotuner), >0 upper bound")
     ap.add_argument("--graph-sweeps", type=int, default=0,
                     help="replay a captured HIP graph of this many sweeps (must divide --steps and --warmup)")
+    ap.add_argument("--da", action="store_true",
+                    help="delayed acceptance with the trained surrogate against the plain sampler "
+                         "(128 replicas x 36 residues); --steps sets the compared sweeps")
+    ap.add_argument("--da-warmup", type=int, default=100)
+    ap.add_argument("--da-rounds", type=int, default=4, help="collect-and-train rounds")
+    ap.add_argument("--da-collect", type=int, default=150, help="sweeps per round (snapshots every sweep)")
+    ap.add_argument("--da-epochs", type=int, default=8, help="epochs per round")
     ap.add_argument("--phases", action="store_true",
                     help="also time the sweep kernel's phases on the device clock (adds barriers)")
     args = ap.parse_args()
@@ -128,6 +220,9 @@ def main():
     from smmp_amd import _ffi, workloads
 
     torch.cuda.set_device(0)
+    if args.da:
+        run_da(args)
+        return
     if args.copy_probe:
         x = torch.ones(args.copy_probe << 18, dtype=torch.float32, device="cuda")
         y = torch.empty_like(x)

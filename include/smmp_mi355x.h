@@ -140,9 +140,44 @@ typedef struct {
                              (gbpr(1..6, chain) of each replica; in: start, out: final).
                              NULL: every replica starts from the context's gbpr.  The
                              sweeps end with the rigid trials of metropolis.f:94-168. */
+  int32_t da;             /* 1: delayed acceptance (modes 0 and 1): the context's surrogate
+                             network (smmp_mlp_set) screens every torsion trial before its
+                             exact energy; 0: plain Metropolis */
+  double da_scale;        /* kcal/mol per network unit */
+  double *da_stats;       /* NULL or R x 4 out: trials screened by the network, rejected by
+                             it, judged with the exact energy, accepted */
+  double *snap_vlvr;      /* NULL or (nsweep / snap_every) x R x nvt out: every replica's
+                             variables after every snap_every-th sweep (training data) */
+  double *snap_e;         /* with snap_vlvr: (nsweep / snap_every) x R energies */
+  int32_t snap_every;
 } smmp_sampler_cfg;
 int smmp_sample(smmp_ctx *ctx, const smmp_sampler_cfg *cfg, double *vlvr, double *eol,
                 double *acc, int32_t *ipoi, int32_t *hist, double *traj);
+
+/* ---- the delayed-acceptance surrogate (no reference counterpart) -------
+ * A small MLP over residue windows (sin/cos of the variables of residues
+ * r-1, r, r+1 and embeddings of their types, three hidden layers of 64):
+ * E~ = scale * sum over residues of its output.  The context holds one
+ * network (smmp_mlp_size() FP32 parameters) and its Adam moments on the
+ * device; training and evaluation run BF16 MFMA kernels with FP32
+ * accumulation.  All arrays below are host arrays. */
+int smmp_mlp_size(void);
+/* Sets the context's network (and clears its Adam moments) / reads it. */
+int smmp_mlp_set(smmp_ctx *ctx, const float *theta);
+int smmp_mlp_get(smmp_ctx *ctx, float *theta);
+/* Outputs for nconf conformations of the context's system (vlvr: nconf x
+ * nvt): pred (nconf, may be NULL) in network units.  With target (nconf):
+ * *loss = mean squared error and grad (smmp_mlp_size(), may be NULL) its
+ * gradient. */
+int smmp_mlp_grad(smmp_ctx *ctx, const double *vlvr, int nconf, const float *target, float *pred, float *grad,
+                  double *loss);
+/* Minibatch training of the context's network on n conformations (vlvr: n
+ * x nvt) with targets (network units): 'epochs' passes in random order
+ * (seeded), batches of 'batch', one Adam step each (beta1 0.9, beta2 0.999,
+ * eps 1e-8); epoch_loss (epochs, may be NULL): mean squared error of each
+ * epoch's batches. */
+int smmp_mlp_fit(smmp_ctx *ctx, const double *vlvr, const float *target, int n, int epochs, int batch, float lr,
+                 uint32_t seed, double *epoch_loss);
 
 /* How the last smmp_sample call ran (no reference counterpart):
  * out[0] workgroups per replica, out[1] 1 if the whole run was one

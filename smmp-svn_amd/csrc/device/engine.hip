@@ -15,6 +15,7 @@
 #include "../host/engine.hpp"
 #include "common.hpp"
 #include "energy.hpp"
+#include "mlp_host.hpp"
 #include "sampler.hpp"
 
 namespace smmp {
@@ -117,7 +118,7 @@ struct DevMem {
 struct SamplerBuffers {
   DevMem idvr, axvr, vlvr, xyz, nfree, valid, eol, acc, mt, rng_of, slot_of, beta, err, dbg;
   DevMem wb, walpha, hist, traj, ipoi, iswitch, work, trace, trace_n, phase, part, sync, gsync;
-  DevMem sweep_base, tune_vl, tune_mt, tune_gb, gb;
+  DevMem sweep_base, tune_vl, tune_mt, tune_gb, gb, da_stats, snap, snap_e;
 };
 
 // One system uploaded to the device: a single allocation holding every
@@ -260,7 +261,9 @@ struct Engine {
   std::vector<uint8_t> gkey;
   std::map<std::vector<int64_t>, int> tuned_groups;  // (system, R, precision, mode) -> G
   int32_t info[4] = {0, 0, 0, 0};                     // how the last sampler call ran
+  MlpWork* mlp = nullptr;                             // surrogate tables and training workspace
   ~Engine() {
+    mlp_work_destroy(mlp);
     if (gexec) (void)hipGraphExecDestroy(gexec);
     for (hipEvent_t ev : prof.pool) (void)hipEventDestroy(ev);
     for (auto& p : prof.pending) {
@@ -437,6 +440,25 @@ int engine_profile_phases(Engine*& e, double* out) {
     out[k] = (count || E.prof.tick_khz <= 0.0) ? t : t / E.prof.tick_khz;
   }
   return SMMP_OK;
+}
+
+void engine_mlp_set(Engine*& e, const float* theta) {
+  Engine& E = engine_of(e);
+  mlp_set_theta(E.mlp, E.stream, theta);
+}
+void engine_mlp_get(Engine*& e, float* theta) {
+  Engine& E = engine_of(e);
+  mlp_get_theta(E.mlp, E.stream, theta);
+}
+double engine_mlp_grad(Engine*& e, const State& s, const double* vlvr, int nconf, const float* target, float* pred,
+                       float* grad) {
+  Engine& E = engine_of(e);
+  return mlp_grad(E.mlp, s, E.stream, vlvr, nconf, target, pred, grad);
+}
+void engine_mlp_fit(Engine*& e, const State& s, const double* vlvr, const float* target, int n, int epochs, int batch,
+                    float lr, uint32_t seed, double* epoch_loss) {
+  Engine& E = engine_of(e);
+  mlp_fit(E.mlp, s, E.stream, vlvr, target, n, epochs, batch, lr, seed, epoch_loss);
 }
 
 int engine_sampler_info(Engine*& e, int32_t* out) {

@@ -15,6 +15,7 @@
 #pragma once
 #include "common.hpp"
 #include "energy.hpp"
+#include "surrogate.hpp"
 
 namespace smmp {
 
@@ -109,6 +110,19 @@ struct SamplerParams {
   uint32_t* mt_x;        // the exchange stream (625 words)
   uint32_t* gsync;       // grid-barrier arrival counter, zero at launch
   double* gbpr;          // per replica: 6 global parameters per chain (kMaxMol chains)
+  // delayed acceptance (null: plain Metropolis): the surrogate's
+  // parameters, kcal/mol per network unit, residue windows, per-replica
+  // counts (screened, rejected by the network, judged exactly, accepted)
+  const float* da_theta;
+  double da_scale;
+  MlpResidues da_res;
+  double* da_stats;
+  size_t da_lds_off;     // byte offset of the surrogate's LDS region
+  // snapshots for training: every snap_every sweeps, each replica's
+  // variables (snap[k][r][nvt]) and energy (snap_e[k][r])
+  double* snap;
+  double* snap_e;
+  int32_t snap_every;
   // Sweeps already done when launches are replayed from a graph (a kernel
   // argument cannot change between replays); null: traj_base.
   int32_t* sweep_base;
@@ -515,8 +529,9 @@ __device__ void grid_sync(const SamplerParams& P, uint32_t& gepoch) {
   ++gepoch;
 }
 
-// P.nsweep Metropolis sweeps of one replica per group of G blocks.
-template <bool MIXED>
+// P.nsweep Metropolis sweeps of one replica per group of G blocks; DA:
+// delayed acceptance with the surrogate network (P.da_theta).
+template <bool MIXED, bool DA>
 __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
   extern __shared__ double smem[];
   const DevSys& S = *P.sys;
@@ -555,6 +570,14 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
   if (valid && S.itysol > 0) {  // the packed heavy copy of the loaded coordinates
     fill_heavy_packed(S, L);
     __syncthreads();
+  }
+  DaLds D{};
+  const int nres = P.da_res.nres;
+  if (DA) {  // the surrogate's weights, tables and outputs at the start state
+    D = carve_da_lds(reinterpret_cast<char*>(smem) + P.da_lds_off, nres, nvt);
+    stage_surrogate(D, P.da_theta, P.da_res);
+    __syncthreads();
+    surrogate_fill_cache(D, P.da_lds_off, nres, nvt, Q.vl);
   }
   // sc: [0] eol, [1] acz, [2] trial value, [3] dphi, [4] accept, [5] full-trial flag,
   //     [6] jv, [7] drift flag, [8..10] axis point, [11..13] axis unit, [15] old value,
@@ -681,6 +704,26 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
         h2 = S.mol_h2[m];
       }
       const bool full = Q.sc[5] != 0.0;
+      if (DA && !rigid) {
+        // ---- delayed acceptance, first stage: the network's energy change
+        if (tid == 0) surrogate_window_rows(D, nres, jv);
+        __syncthreads();
+        surrogate_rows(P.da_lds_off, nres, nvt, Q.vl, jv, Q.sc[2]);
+        if (tid == 0) {
+          double ds = 0.0;
+          for (int k = 0; k < D.rows[kDaRows]; ++k) ds += (double)D.ynew[k] - (double)D.ycache[D.rows[k]];
+          ds *= P.da_scale;
+          const double d1 = Q.sc[14] * ds;
+          bool pass = true;
+          if (d1 > 0.0) pass = exp(-fmin(d1, 100.0)) >= mt_grnd(Q.mt, &Q.mt[624]);
+          Q.sc[19] = ds;
+          Q.sc[20] = pass ? 1.0 : 0.0;
+          D.stats[0] += 1.0;
+          if (!pass) D.stats[1] += 1.0;
+        }
+        __syncthreads();
+        if (Q.sc[20] == 0.0) continue;  // rejected without the exact energy (uniform)
+      }
       EnergyOut vref;  // verify mode (G = 1): full energy before the move (thread 0)
       if (P.verify && !full) {
         const ReplicaEnergy re = replica_energy<MIXED>(P.sys, nvt, P.bits_lds, 0, false, 0, 1);
@@ -922,6 +965,8 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
         const double eol = Q.sc[0];
         const double beta = Q.sc[14];
         double delta = weight_of(P, beta, enew) - weight_of(P, beta, eol);
+        const bool da = DA && !rigid;
+        if (da) delta -= beta * Q.sc[19];  // second stage: exact change against the network's
         bool acc;
         if (delta <= 0.0) {
           acc = true;
@@ -938,6 +983,13 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
           t[2] = eol;
           t[3] = acc ? 1.0 : 0.0;
           *P.trace_n += 1;
+        }
+        if (da) {
+          D.stats[2] += 1.0;
+          if (acc) {
+            D.stats[3] += 1.0;
+            for (int k = 0; k < D.rows[kDaRows]; ++k) D.ycache[D.rows[k]] = D.ynew[k];
+          }
         }
         if (acc) {
           Q.sc[0] = enew;
@@ -996,6 +1048,11 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
       if (g == 0) {
         const int slot = P.slot_of[r];
         if (P.traj) P.traj[(size_t)(traj_base + sw) * P.nrep + slot] = enw;
+        if (P.snap && (traj_base + sw + 1) % P.snap_every == 0) {
+          const size_t k = (size_t)((traj_base + sw + 1) / P.snap_every - 1);
+          P.snap_e[k * P.nrep + r] = enw;
+          for (int v = 0; v < nvt; ++v) P.snap[(k * P.nrep + r) * nvt + v] = Q.vl[v];
+        }
         if (P.hist) {
           double hv = enw / P.ebin + copysign(0.5, enw);
           hv = fmin(fmax(hv, (double)P.kmin), (double)P.kmax);
@@ -1036,6 +1093,8 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
       P.eol[r] = Q.sc[0];
       P.acc[r] = Q.sc[1];
       P.valid[r] = 1;
+      if (DA && P.da_stats)
+        for (int k = 0; k < 4; ++k) P.da_stats[4 * (size_t)r + k] += D.stats[k];
     }
   }
   for (int h = tid; h < nh; h += blockDim.x)

@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "../../../include/smmp_mi355x.h"
+#include "../device/mlp_common.hpp"
 #include "engine.hpp"
 #include "state.hpp"
 
@@ -275,6 +276,45 @@ int smmp_profile(smmp_ctx* c, int enable) {
 
 int smmp_profile_read(smmp_ctx* c, int kernel, double* total_ms, int64_t* launches, double* work) {
   return guarded(c, SMMP_EDEVICE, [&] { return engine_profile_read(c->eng, kernel, total_ms, launches, work); });
+}
+
+int smmp_mlp_size(void) { return smmp::kThN; }
+
+int smmp_mlp_set(smmp_ctx* c, const float* theta) {
+  if (!theta) return SMMP_EINVAL;
+  return guarded(c, SMMP_EDEVICE, [&] {
+    engine_mlp_set(c->eng, theta);
+    return SMMP_OK;
+  });
+}
+
+int smmp_mlp_get(smmp_ctx* c, float* theta) {
+  if (!theta) return SMMP_EINVAL;
+  return guarded(c, SMMP_ESTATE, [&] {
+    engine_mlp_get(c->eng, theta);
+    return SMMP_OK;
+  });
+}
+
+int smmp_mlp_grad(smmp_ctx* c, const double* vlvr, int nconf, const float* target, float* pred, float* grad,
+                  double* loss) {
+  if (!vlvr || nconf <= 0) return SMMP_EINVAL;
+  return guarded(c, SMMP_ESTATE, [&] {
+    if (c->s.ntlml < 1) throw Error("surrogate> no molecule initialised");
+    const double l = engine_mlp_grad(c->eng, c->s, vlvr, nconf, target, pred, grad);
+    if (loss) *loss = l;
+    return SMMP_OK;
+  });
+}
+
+int smmp_mlp_fit(smmp_ctx* c, const double* vlvr, const float* target, int n, int epochs, int batch, float lr,
+                 uint32_t seed, double* epoch_loss) {
+  if (!vlvr || !target || n <= 0 || epochs < 0 || batch <= 0) return SMMP_EINVAL;
+  return guarded(c, SMMP_ESTATE, [&] {
+    if (c->s.ntlml < 1) throw Error("surrogate> no molecule initialised");
+    engine_mlp_fit(c->eng, c->s, vlvr, target, n, epochs, batch, lr, seed, epoch_loss);
+    return SMMP_OK;
+  });
 }
 
 int smmp_sampler_info(smmp_ctx* c, int32_t* out) {

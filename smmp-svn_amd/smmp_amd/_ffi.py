@@ -36,7 +36,9 @@ class SamplerCfg(C.Structure):
     _fields_ = [("nrep", C.c_int32), ("nsweep", C.c_int32), ("mode", C.c_int32), ("precision", C.c_int32),
                 ("temps", C.POINTER(C.c_double)), ("seeds", C.POINTER(C.c_uint32)),
                 ("w", C.POINTER(Weight)), ("resync_every", C.c_int32), ("verify", C.c_int32),
-                ("groups", C.c_int32), ("graph_sweeps", C.c_int32), ("gbpr", C.POINTER(C.c_double))]
+                ("groups", C.c_int32), ("graph_sweeps", C.c_int32), ("gbpr", C.POINTER(C.c_double)),
+                ("da", C.c_int32), ("da_scale", C.c_double), ("da_stats", C.POINTER(C.c_double)),
+                ("snap_vlvr", C.POINTER(C.c_double)), ("snap_e", C.POINTER(C.c_double)), ("snap_every", C.c_int32)]
 
 
 def _load():
@@ -46,6 +48,7 @@ def _load():
     P = C.c_void_p
     dp = C.POINTER(C.c_double)
     ip = C.POINTER(C.c_int32)
+    fp = C.POINTER(C.c_float)
     sig = {
         "smmp_ctx_create": (C.c_int, [C.POINTER(P)]),
         "smmp_ctx_destroy": (None, [P]),
@@ -70,6 +73,11 @@ def _load():
         "smmp_profile_read": (C.c_int, [P, C.c_int, dp, C.POINTER(C.c_int64), dp]),
         "smmp_profile_phases": (C.c_int, [P, dp]),
         "smmp_sampler_info": (C.c_int, [P, ip]),
+        "smmp_mlp_size": (C.c_int, []),
+        "smmp_mlp_set": (C.c_int, [P, fp]),
+        "smmp_mlp_get": (C.c_int, [P, fp]),
+        "smmp_mlp_grad": (C.c_int, [P, dp, C.c_int, fp, fp, fp, dp]),
+        "smmp_mlp_fit": (C.c_int, [P, dp, fp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_uint32, dp]),
         "smmp_muca_update": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, ip, dp, dp, dp, dp, dp]),
         "smmp_muca_final": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, dp, dp]),
     }
@@ -87,7 +95,8 @@ EXPORTS = ("smmp_ctx_create", "smmp_ctx_destroy", "smmp_last_error", "smmp_state
            "smmp_state_name", "smmp_init_energy", "smmp_init_molecule", "smmp_setvar", "smmp_energy",
            "smmp_enysol", "smmp_sgrnd", "smmp_grnd", "smmp_outvar", "smmp_metropolis", "smmp_partem_s",
            "smmp_energy_batch", "smmp_sample", "smmp_build_info", "smmp_profile", "smmp_profile_read",
-           "smmp_profile_phases", "smmp_muca_update", "smmp_muca_final", "smmp_sampler_info")
+           "smmp_profile_phases", "smmp_muca_update", "smmp_muca_final", "smmp_sampler_info",
+           "smmp_mlp_size", "smmp_mlp_set", "smmp_mlp_get", "smmp_mlp_grad", "smmp_mlp_fit")
 
 KERNEL_SWEEP, KERNEL_EXCHANGE, KERNEL_ENERGY = 0, 1, 2
 
@@ -253,12 +262,17 @@ class Context:
 
     def sample(self, temps, seeds, nsweep: int, vlvr, mode: int = 0, weight: Weight | None = None,
                precision: int = 0, with_traj: bool = True, allow_drift: bool = False, verify: bool = False,
-               groups: int = 0, graph_sweeps: int = 0, gbpr=None):
+               groups: int = 0, graph_sweeps: int = 0, gbpr=None, da: bool = False,
+               da_scale: float = 1.0, snap_every: int = 0):
         """Independent-stream replica sampler (smmp_sample).  groups: workgroups
         per replica (0 automatic, -1 measured); graph_sweeps: sweeps per
         replay of a captured graph (0: direct launches); gbpr: several chains,
         R x ntlml x 6 start values of the chains' global parameters (returned
-        as "gbpr"; None: every replica starts from the context's)."""
+        as "gbpr"; None: every replica starts from the context's); da: delayed
+        acceptance with the context's network (per-replica counts returned as
+        "da_stats"); snap_every > 0: each
+        replica's variables and energy every snap_every sweeps ("snap_vlvr",
+        "snap_e")."""
         temps = np.ascontiguousarray(temps, dtype=np.float64)
         R = len(temps)
         seeds = np.ascontiguousarray(seeds, dtype=np.uint32)
@@ -276,6 +290,18 @@ class Context:
                          temps=dptr(temps), seeds=seeds.ctypes.data_as(C.POINTER(C.c_uint32)),
                          w=C.pointer(weight) if weight is not None else None, resync_every=1,
                          verify=1 if verify else 0, groups=groups, graph_sweeps=graph_sweeps)
+        da_stats = None
+        if da:
+            da_stats = np.zeros((R, 4))
+            cfg.da = 1
+            cfg.da_scale = float(da_scale)
+            cfg.da_stats = dptr(da_stats)
+        snap_v = snap_e = None
+        if snap_every > 0:
+            ns = nsweep // snap_every
+            snap_v = np.zeros((max(ns, 1), R, self.nvt))
+            snap_e = np.zeros((max(ns, 1), R))
+            cfg.snap_vlvr, cfg.snap_e, cfg.snap_every = dptr(snap_v), dptr(snap_e), snap_every
         nml = int(self.view("mol_i.ntlml")[0])
         gb = None
         if gbpr is not None:
@@ -286,7 +312,10 @@ class Context:
                              dptr(traj) if traj is not None else None)
         if rc != 0 and not (allow_drift and rc == -4):
             self.check(rc)
+        if snap_v is not None:
+            snap_v, snap_e = snap_v[:nsweep // snap_every], snap_e[:nsweep // snap_every]
         return {"vlvr": vl, "eol": eol, "acc": acc, "ipoi": ipoi, "hist": hist, "rc": rc, "gbpr": gb,
+                "da_stats": da_stats, "snap_vlvr": snap_v, "snap_e": snap_e,
                 "traj": traj[:nsweep * R].reshape(nsweep, R) if traj is not None else None}
 
 

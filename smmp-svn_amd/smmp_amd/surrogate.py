@@ -1,0 +1,128 @@
+"""The delayed-acceptance surrogate from Python.  The network lives with the
+context on the device (smmp_mlp_set / smmp_mlp_fit train it there with the
+library's BF16 MFMA kernels); this class keeps the scale and offset that
+map network units to kcal/mol and drives training and delayed-acceptance
+sampling.
+
+The network predicts E~(x) = scale * sum_r y(f_r(x)) + offset over residue
+windows (see include/smmp_mi355x.h).  It is fitted to exact energies the
+sampler produced, and in the sampler it only screens trials: the exact
+energy judges every trial that passes, so the sampled distribution is the
+exact one whatever the network's quality (delayed acceptance, Christen and
+Fox, 2005)."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _ffi
+
+IN, HID, TYPES, EMB = 128, 64, 32, 16
+
+
+def init_theta(seed: int = 0, init_scale: float = 1.0) -> np.ndarray:
+    """Fan-in scaled normal weights, zero biases, embeddings of spread 0.5."""
+    n = _ffi.lib.smmp_mlp_size()
+    rng = np.random.default_rng(seed)
+    th = np.zeros(n, dtype=np.float32)
+    o = 0
+    th[o:o + IN * HID] = rng.normal(0, init_scale / np.sqrt(IN), IN * HID)
+    o += IN * HID + HID
+    th[o:o + HID * HID] = rng.normal(0, init_scale / np.sqrt(HID), HID * HID)
+    o += HID * HID + HID
+    th[o:o + HID * HID] = rng.normal(0, init_scale / np.sqrt(HID), HID * HID)
+    o += HID * HID + HID
+    th[o:o + HID] = rng.normal(0, init_scale / np.sqrt(HID), HID)
+    o += HID + 16
+    th[o:o + TYPES * EMB] = rng.normal(0, 0.5, TYPES * EMB)
+    return th
+
+
+class Surrogate:
+    """The network of one context, with its energy scale."""
+
+    def __init__(self, ctx, theta=None, seed: int = 0, init_scale: float = 1.0):
+        self.ctx = ctx
+        self.n = _ffi.lib.smmp_mlp_size()
+        self.scale = 1.0   # kcal/mol per network unit
+        self.offset = 0.0  # kcal/mol
+        self.set_theta(init_theta(seed, init_scale) if theta is None else theta)
+
+    @classmethod
+    def zero(cls, ctx) -> "Surrogate":
+        """A network whose output is zero everywhere (delayed acceptance with
+        it is plain Metropolis, draw for draw)."""
+        return cls(ctx, theta=np.zeros(_ffi.lib.smmp_mlp_size(), dtype=np.float32))
+
+    def set_theta(self, th) -> None:
+        a = np.ascontiguousarray(th, dtype=np.float32)
+        if a.size != self.n:
+            raise ValueError(f"expected {self.n} parameters")
+        self.ctx.check(_ffi.lib.smmp_mlp_set(self.ctx.h, a.ctypes.data_as(C.POINTER(C.c_float))))
+
+    def theta(self) -> np.ndarray:
+        a = np.zeros(self.n, dtype=np.float32)
+        self.ctx.check(_ffi.lib.smmp_mlp_get(self.ctx.h, a.ctypes.data_as(C.POINTER(C.c_float))))
+        return a
+
+    def grad(self, vlvr, target=None):
+        """(loss, predictions in network units, gradient) for conformations
+        vlvr (n x nvt) and targets in network units (or None)."""
+        v = np.ascontiguousarray(vlvr, dtype=np.float64)
+        n = len(v)
+        pred = np.zeros(n, dtype=np.float32)
+        fp = C.POINTER(C.c_float)
+        if target is None:
+            self.ctx.check(_ffi.lib.smmp_mlp_grad(self.ctx.h, _ffi.dptr(v), n, None, pred.ctypes.data_as(fp), None,
+                                                  None))
+            return None, pred, None
+        t = np.ascontiguousarray(target, dtype=np.float32)
+        g = np.zeros(self.n, dtype=np.float32)
+        loss = C.c_double(0.0)
+        self.ctx.check(_ffi.lib.smmp_mlp_grad(self.ctx.h, _ffi.dptr(v), n, t.ctypes.data_as(fp),
+                                              pred.ctypes.data_as(fp), g.ctypes.data_as(fp), C.byref(loss)))
+        return loss.value, pred, g
+
+    def predict(self, vlvr) -> np.ndarray:
+        """E~ in kcal/mol."""
+        _, pred, _ = self.grad(vlvr)
+        return pred.astype(np.float64) * self.scale + self.offset
+
+    def fit(self, vlvr, energies, epochs: int = 20, batch: int = 256, lr: float = 2e-3, seed: int = 0,
+            clip: float | None = None, rescale: bool = True) -> np.ndarray:
+        """Fits the network to exact energies (n x nvt conformations, n
+        energies in kcal/mol); energies above clip are left out.  With
+        rescale the targets are standardised first (offset = mean, scale =
+        standard deviation).  Returns the loss of every epoch."""
+        e = np.asarray(energies, dtype=np.float64)
+        v = np.asarray(vlvr, dtype=np.float64)
+        if clip is not None:
+            keep = e <= clip
+            e, v = e[keep], v[keep]
+        if rescale:
+            self.offset = float(e.mean())
+            self.scale = float(max(e.std(), 1e-6))
+        t = np.ascontiguousarray((e - self.offset) / self.scale, dtype=np.float32)
+        v = np.ascontiguousarray(v)
+        hist = np.zeros(epochs)
+        self.ctx.check(_ffi.lib.smmp_mlp_fit(self.ctx.h, _ffi.dptr(v), t.ctypes.data_as(C.POINTER(C.c_float)),
+                                             len(t), epochs, batch, C.c_float(lr), C.c_uint32(seed & 0xFFFFFFFF),
+                                             _ffi.dptr(hist)))
+        return hist
+
+    def sample(self, temps, seeds, nsweep, vlvr, **kw):
+        """smmp_sample with delayed acceptance screened by this network."""
+        return self.ctx.sample(temps, seeds, nsweep, vlvr, da=True, da_scale=self.scale, **kw)
+
+
+def da_gain(plain: dict, da: dict, t_plain: float, t_da: float) -> dict:
+    """Accepted moves per second with and without the network (every move a
+    delayed-acceptance run accepts has passed the exact energy)."""
+    a_plain = float(np.sum(plain["acc"]))
+    a_da = float(np.sum(da["acc"]))
+    st = da["da_stats"]
+    return {"accepted_per_s_plain": a_plain / t_plain, "accepted_per_s_da": a_da / t_da,
+            "gain": (a_da / t_da) / (a_plain / t_plain),
+            "screened": float(st[:, 0].sum()), "rejected_by_network": float(st[:, 1].sum()),
+            "exact_evaluations": float(st[:, 2].sum()), "accepted_da": float(st[:, 3].sum())}

@@ -1,0 +1,95 @@
+"""The delayed-acceptance surrogate on the CPU: the oracle's network
+gradient against finite differences, its BF16 rounding against closed
+forms, and the two-stage acceptance rule of the sampler (stage 1 with the
+network's energy change, stage 2 with the exact change minus the network's)
+against the stationarity it must keep."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import make_ctx
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import surrogate_oracle as so  # noqa: E402
+
+
+def test_bf16_rounding_closed_forms():
+    one = 1.0
+    ulp = 2.0 ** -7  # BF16 spacing at 1
+    assert so.bf16(np.array([one + ulp / 2]))[0] == one            # tie: to even
+    assert so.bf16(np.array([one + 3 * ulp / 2]))[0] == one + 2 * ulp  # tie: to even (up)
+    assert so.bf16(np.array([one + ulp / 2 + 2.0 ** -20]))[0] == one + ulp
+    assert so.bf16(np.array([-3.0]))[0] == -3.0
+    x = np.random.default_rng(0).normal(size=1000)
+    r = so.bf16(x)
+    assert np.all(np.abs(r - x) <= np.abs(x) * 2.0 ** -8 * 1.0001)
+
+
+def test_network_gradient_matches_finite_differences():
+    c = make_ctx(tag="enkefa", itysol=1)
+    R = so.Residues.from_context(c)
+    assert list(R.type) == [19, 8, 8, 14, 13]  # tyr gly gly phe met
+    rng = np.random.default_rng(0)
+    V = np.tile(c.vlvr(), (4, 1)) + rng.normal(0, 1, (4, len(c.vlvr())))
+    tgt = np.array([0.3, -1.2, 2.0, 0.1])
+    th = so.init_theta(2).astype(np.float64)
+    loss, g, _ = so.loss_grad(R, th, V, tgt, bf=False)
+    idx = [so.W1 + 5, so.W1 + 61 * 64 + 3, so.B1 + 3, so.W2 + 77, so.B2 + 9, so.W3 + 1000, so.B3 + 1,
+           so.W4 + 7, so.B4, so.EMBO + 16 * 8 + 3, so.EMBO + 16 * 13 + 5, so.EMBO + 16 * 0 + 2]
+    for i in idx:
+        h = 1e-6
+        tp, tm = th.copy(), th.copy()
+        tp[i] += h
+        tm[i] -= h
+        fd = (so.loss_grad(R, tp, V, tgt, bf=False)[0] - so.loss_grad(R, tm, V, tgt, bf=False)[0]) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(g[i])), (i, g[i], fd)
+
+
+def _da_kernel(E, S, beta):
+    """Transition matrix of one trial: uniform proposal to another state,
+    stage 1 with the surrogate S, stage 2 with E - S (the sampler's rule)."""
+    n = len(E)
+    T = np.zeros((n, n))
+    for i in range(n):
+        for j in range(n):
+            if i == j:
+                continue
+            d1 = beta * (S[j] - S[i])
+            a1 = 1.0 if d1 <= 0 else np.exp(-min(d1, 100.0))
+            d2 = beta * (E[j] - E[i]) - beta * (S[j] - S[i])
+            a2 = 1.0 if d2 <= 0 else np.exp(-min(d2, 100.0))
+            T[i, j] = a1 * a2 / (n - 1)
+        T[i, i] = 1.0 - T[i].sum()
+    return T
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_two_stage_rule_keeps_the_boltzmann_distribution(seed):
+    rng = np.random.default_rng(seed)
+    E = rng.normal(0, 2.0, 7)
+    beta = 1.3
+    pi = np.exp(-beta * (E - E.min()))
+    pi /= pi.sum()
+    for S in (rng.normal(0, 3.0, 7), E + rng.normal(0, 0.3, 7), np.zeros(7)):
+        T = _da_kernel(E, S, beta)
+        assert np.abs(pi @ T - pi).max() < 1e-14
+    # with a zero network the rule is Metropolis: acceptance min(1, exp(-beta dE))
+    T0 = _da_kernel(E, np.zeros(7), beta)
+    for i in range(7):
+        for j in range(7):
+            if i != j:
+                assert abs(T0[i, j] * 6 - min(1.0, np.exp(-beta * (E[j] - E[i])))) < 1e-15
+    # a wrong sign in stage 2 breaks stationarity (the test can tell)
+    S = rng.normal(0, 3.0, 7)
+    n = 7
+    Tw = np.zeros((n, n))
+    for i in range(n):
+        for j in range(n):
+            if i != j:
+                d1 = beta * (S[j] - S[i])
+                d2 = beta * (E[j] - E[i]) + beta * (S[j] - S[i])
+                Tw[i, j] = min(1.0, np.exp(-d1)) * min(1.0, np.exp(-d2)) / (n - 1)
+        Tw[i, i] = 1.0 - Tw[i].sum()
+    assert np.abs(pi @ Tw - pi).max() > 1e-4
