@@ -111,7 +111,7 @@ def run_da(args):
     energies, then both samplers run the same sweeps from the same state."""
     from scipy.stats import ks_2samp
     from smmp_amd import _ffi, workloads
-    from smmp_amd.surrogate import Surrogate
+    from smmp_amd.surrogate import Surrogate, move_pairs
     R, L = 128, 36
     tmp = tempfile.mkdtemp(prefix="smmp_da_")
     c, seq, _ = make_system(tmp, length=L, seed=5, tag="c5")
@@ -122,8 +122,8 @@ def run_da(args):
     state = c.sample(temps, seeds(1), args.da_warmup, random_starts(c, R, 12345), mode=1, with_traj=False)["vlvr"]
     log(f"warm-up {args.da_warmup} sweeps: {time.perf_counter() - t0:.1f} s")
     net = Surrogate(c, seed=3)
-    Vs, Es = [], []
-    t_collect = t_train = 0.0
+    Ps, Ds, Vs, Es = [], [], [], []
+    t_collect = t_pairs = t_train = 0.0
     hist = np.zeros(0)
     for rnd in range(args.da_rounds):
         t0 = time.perf_counter()
@@ -133,12 +133,27 @@ def run_da(args):
         Vs.append(d["snap_vlvr"].reshape(-1, c.nvt))
         Es.append(d["snap_e"].ravel())
         V, E = np.concatenate(Vs), np.concatenate(Es)
-        t0 = time.perf_counter()
-        hist = net.fit(V, E, epochs=args.da_epochs, batch=512, lr=2e-3, seed=rnd,
-                       clip=float(np.quantile(E, 0.99)), rescale=(rnd == 0))
-        t_train += time.perf_counter() - t0
-        log(f"round {rnd}: {len(E)} energies, loss {hist[0]:.4f} -> {hist[-1]:.4f} (network units^2), "
-            f"scale {net.scale:.2f} kcal/mol")
+        if args.da_train in ("energy", "both"):  # total energies of the sampled conformations
+            t0 = time.perf_counter()
+            hist = net.fit(V, E, epochs=args.da_epochs, batch=512, lr=2e-3, seed=rnd,
+                           clip=float(np.quantile(E, 0.99)), rescale=(rnd == 0))
+            t_train += time.perf_counter() - t0
+            log(f"round {rnd}: {len(E)} energies, loss {hist[0]:.4f} -> {hist[-1]:.4f} (network units^2), "
+                f"scale {net.scale:.2f} kcal/mol")
+        if args.da_train in ("moves", "both"):  # exact changes of single moves from them
+            t0 = time.perf_counter()
+            pv, de, _ = move_pairs(c, d["snap_vlvr"].reshape(-1, c.nvt), seed=rnd)
+            t_pairs += time.perf_counter() - t0
+            Ps.append(pv)
+            Ds.append(de)
+            P, D = np.concatenate(Ps), np.concatenate(Ds)
+            t0 = time.perf_counter()
+            hist = net.fit_moves(P, D, epochs=args.da_epochs, batch=512, lr=2e-3 if args.da_train == "moves" else 5e-4,
+                                 seed=rnd, rescale=(args.da_train == "moves" and rnd == 0))
+            t_train += time.perf_counter() - t0
+            log(f"round {rnd}: {len(D)} moves (|dE| <= 20: {np.mean(np.abs(D) <= 20):.2f}), loss {hist[0]:.4f} -> "
+                f"{hist[-1]:.4f} (network units^2), scale {net.scale:.2f} kcal/mol")
+    c.sample(temps, seeds(700), 10, state, mode=1, with_traj=False)  # the system back on the device
     K = args.steps
     runs = {}
     for name, fn in (("plain", lambda: c.sample(temps, seeds(900), K, state, mode=1)),
@@ -150,10 +165,13 @@ def run_da(args):
         runs[name] = (out, ks["ms"] / 1e3)
         log(f"{name}: {K} sweeps in {ks['ms']:.1f} ms of sweep kernel, accepted {out['acc'].sum():.0f}")
     (plain, tp), (da, td) = runs["plain"], runs["da"]
+    control = c.sample(temps, seeds(4242), K, state, mode=1)  # a second plain run: the KS test's own spread
     burn, thin = K // 5, 5
-    a, b = plain["traj"][burn::thin], da["traj"][burn::thin]
+    a, b, z = plain["traj"][burn::thin], da["traj"][burn::thin], control["traj"][burn::thin]
     ks_pooled = float(ks_2samp(a.ravel(), b.ravel()).pvalue)
+    ks_control = float(ks_2samp(a.ravel(), z.ravel()).pvalue)
     slot_p = np.array([ks_2samp(a[:, k], b[:, k]).pvalue for k in range(R)])
+    slot_c = np.array([ks_2samp(a[:, k], z[:, k]).pvalue for k in range(R)])
     st = da["da_stats"]
     ntrial = float(R * K * c.nvr)
     gain = (da["acc"].sum() / td) / (plain["acc"].sum() / tp)
@@ -182,9 +200,14 @@ def run_da(args):
         "ks_pooled_p": round(ks_pooled, 4),
         "ks_per_temperature_median_p": round(float(np.median(slot_p)), 4),
         "ks_per_temperature_frac_p_above_0.05": round(float(np.mean(slot_p > 0.05)), 3),
-        "training": {"energies": int(sum(len(e) for e in Es)), "rounds": args.da_rounds,
+        "ks_control_plain_vs_plain": {"pooled_p": round(ks_control, 4),
+                                      "per_temperature_median_p": round(float(np.median(slot_c)), 4),
+                                      "per_temperature_frac_p_above_0.05": round(float(np.mean(slot_c > 0.05)), 3)},
+        "training": {"targets": args.da_train, "energies": int(sum(len(e) for e in Es)),
+                     "moves": int(sum(len(d) for d in Ds)), "rounds": args.da_rounds,
                      "epochs_per_round": args.da_epochs, "final_loss": round(float(hist[-1]), 5),
-                     "collect_s": round(t_collect, 2), "train_s": round(t_train, 2)},
+                     "scale_kcal_per_unit": round(net.scale, 3), "collect_s": round(t_collect, 2),
+                     "pairs_s": round(t_pairs, 2), "train_s": round(t_train, 2)},
     }
     print(json.dumps(line), flush=True)
 
@@ -209,6 +232,8 @@ def main():
     ap.add_argument("--da-rounds", type=int, default=4, help="collect-and-train rounds")
     ap.add_argument("--da-collect", type=int, default=150, help="sweeps per round (snapshots every sweep)")
     ap.add_argument("--da-epochs", type=int, default=8, help="epochs per round")
+    ap.add_argument("--da-train", choices=("energy", "moves", "both"), default="moves",
+                    help="fit total energies, exact changes of single moves, or energies then moves")
     ap.add_argument("--phases", action="store_true",
                     help="also time the sweep kernel's phases on the device clock (adds barriers)")
     args = ap.parse_args()

@@ -166,18 +166,21 @@ int smmp_mlp_size(void);
 int smmp_mlp_set(smmp_ctx *ctx, const float *theta);
 int smmp_mlp_get(smmp_ctx *ctx, float *theta);
 /* Outputs for nconf conformations of the context's system (vlvr: nconf x
- * nvt): pred (nconf, may be NULL) in network units.  With target (nconf):
- * *loss = mean squared error and grad (smmp_mlp_size(), may be NULL) its
- * gradient. */
-int smmp_mlp_grad(smmp_ctx *ctx, const double *vlvr, int nconf, const float *target, float *pred, float *grad,
-                  double *loss);
-/* Minibatch training of the context's network on n conformations (vlvr: n
- * x nvt) with targets (network units): 'epochs' passes in random order
- * (seeded), batches of 'batch', one Adam step each (beta1 0.9, beta2 0.999,
- * eps 1e-8); epoch_loss (epochs, may be NULL): mean squared error of each
- * epoch's batches. */
-int smmp_mlp_fit(smmp_ctx *ctx, const double *vlvr, const float *target, int n, int epochs, int batch, float lr,
-                 uint32_t seed, double *epoch_loss);
+ * nvt): pred (nconf, may be NULL) in network units.  With targets: *loss =
+ * mean squared error and grad (smmp_mlp_size(), may be NULL) its gradient;
+ * pairs = 0: one target per conformation (its output), pairs = 1: the
+ * conformations come in pairs (x, x') and target[c] is for the difference
+ * of the outputs of conformations 2c+1 and 2c (nconf even). */
+int smmp_mlp_grad(smmp_ctx *ctx, const double *vlvr, int nconf, const float *target, int pairs, float *pred,
+                  float *grad, double *loss);
+/* Minibatch training of the context's network on n samples with targets
+ * (network units); a sample is one conformation (vlvr: n x nvt), or with
+ * pairs two (vlvr: 2n x nvt) as in smmp_mlp_grad.  'epochs' passes in
+ * random order (seeded), batches of 'batch' samples, one Adam step each
+ * (beta1 0.9, beta2 0.999, eps 1e-8); epoch_loss (epochs, may be NULL):
+ * mean squared error of each epoch's batches. */
+int smmp_mlp_fit(smmp_ctx *ctx, const double *vlvr, const float *target, int pairs, int n, int epochs, int batch,
+                 float lr, uint32_t seed, double *epoch_loss);
 
 /* How the last smmp_sample call ran (no reference counterpart):
  * out[0] workgroups per replica, out[1] 1 if the whole run was one

@@ -125,16 +125,26 @@ def forward(res: Residues, theta, vlvr, bf: bool = True):
     return pred, dict(X=X, T=T, Z1=Z1, H1=H1, Z2=Z2, H2=H2, Z3=Z3, H3=H3, y=y, p=p)
 
 
-def loss_grad(res: Residues, theta, vlvr, target, bf: bool = True):
-    """Mean squared error of the predictions and its gradient (NPARAM)."""
+def loss_grad(res: Residues, theta, vlvr, target, bf: bool = True, pairs: bool = False):
+    """Mean squared error of the predictions and its gradient (NPARAM).
+    With pairs the conformations come in pairs (x, x') and the targets are
+    for the differences pred(x') - pred(x)."""
     pred, c = forward(res, theta, vlvr, bf)
     target = np.asarray(target, dtype=np.float64)
-    n = len(pred)
-    e = pred - target
+    if pairs:
+        n = len(pred) // 2
+        e = pred[1::2] - pred[0::2] - target
+        gconf = np.empty(2 * n)
+        gconf[0::2] = -2.0 * e / n
+        gconf[1::2] = 2.0 * e / n
+    else:
+        n = len(pred)
+        e = pred - target
+        gconf = 2.0 * e / n
     loss = float(np.mean(e * e))
     r = (lambda a: bf16(a)) if bf else (lambda a: a)
     p = c["p"]
-    dy = np.repeat(2.0 * e / n, res.nres)
+    dy = np.repeat(gconf, res.nres)
     g = np.zeros(NPARAM)
     g[W4:B4] = c["H3"].T @ dy
     g[B4] = dy.sum()

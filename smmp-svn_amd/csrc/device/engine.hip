@@ -451,14 +451,14 @@ void engine_mlp_get(Engine*& e, float* theta) {
   mlp_get_theta(E.mlp, E.stream, theta);
 }
 double engine_mlp_grad(Engine*& e, const State& s, const double* vlvr, int nconf, const float* target, float* pred,
-                       float* grad) {
+                       float* grad, int pairs) {
   Engine& E = engine_of(e);
-  return mlp_grad(E.mlp, s, E.stream, vlvr, nconf, target, pred, grad);
+  return mlp_grad(E.mlp, s, E.stream, vlvr, nconf, target, pred, grad, pairs);
 }
 void engine_mlp_fit(Engine*& e, const State& s, const double* vlvr, const float* target, int n, int epochs, int batch,
-                    float lr, uint32_t seed, double* epoch_loss) {
+                    float lr, uint32_t seed, double* epoch_loss, int pairs) {
   Engine& E = engine_of(e);
-  mlp_fit(E.mlp, s, E.stream, vlvr, target, n, epochs, batch, lr, seed, epoch_loss);
+  mlp_fit(E.mlp, s, E.stream, vlvr, target, n, epochs, batch, lr, seed, epoch_loss, pairs);
 }
 
 int engine_sampler_info(Engine*& e, int32_t* out) {

@@ -158,20 +158,45 @@ __global__ void k_mlp_out(const bf16* __restrict__ h3, const float* __restrict__
   y[m] = s;
 }
 
-// Per conformation: pred = sum of its nres rows; with targets, dy[row] =
-// 2 (pred - target) / nconf and sq[c] = (pred - target)^2.
+// Per conformation: pred = sum of its nres rows.  With targets, either
+// per conformation (e = pred - target, dy[row] = 2 e / nconf), or with
+// pairs per pair c of conformations (2c, 2c+1): e = pred[2c+1] - pred[2c] -
+// target[c], dy = +-2 e / npairs on the rows of 2c+1 and 2c; sq = e^2.
 __global__ void k_mlp_conf(const float* __restrict__ y, int nres, int nconf, const float* __restrict__ target,
-                           float* __restrict__ pred, float* __restrict__ dy, float* __restrict__ sq) {
+                           float* __restrict__ pred, float* __restrict__ dy, float* __restrict__ sq, int pairs) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nconf) return;
-  float s = 0.0f;
-  for (int r = 0; r < nres; ++r) s += y[(size_t)c * nres + r];
-  if (pred) pred[c] = s;
+  if (!pairs) {
+    if (c >= nconf) return;
+    float s = 0.0f;
+    for (int r = 0; r < nres; ++r) s += y[(size_t)c * nres + r];
+    if (pred) pred[c] = s;
+    if (target) {
+      const float e = s - target[c];
+      sq[c] = e * e;
+      const float g = 2.0f * e / (float)nconf;
+      for (int r = 0; r < nres; ++r) dy[(size_t)c * nres + r] = g;
+    }
+    return;
+  }
+  const int np = nconf / 2;
+  if (c >= np) return;
+  float s0 = 0.0f, s1 = 0.0f;
+  for (int r = 0; r < nres; ++r) {
+    s0 += y[(size_t)(2 * c) * nres + r];
+    s1 += y[(size_t)(2 * c + 1) * nres + r];
+  }
+  if (pred) {
+    pred[2 * c] = s0;
+    pred[2 * c + 1] = s1;
+  }
   if (target) {
-    const float e = s - target[c];
+    const float e = (s1 - s0) - target[c];
     sq[c] = e * e;
-    const float g = 2.0f * e / (float)nconf;
-    for (int r = 0; r < nres; ++r) dy[(size_t)c * nres + r] = g;
+    const float g = 2.0f * e / (float)np;
+    for (int r = 0; r < nres; ++r) {
+      dy[(size_t)(2 * c) * nres + r] = -g;
+      dy[(size_t)(2 * c + 1) * nres + r] = g;
+    }
   }
 }
 
@@ -291,13 +316,16 @@ void gemm(hipStream_t st, const GemmArgs& a, int splits) {
 
 }  // namespace
 
-// Rows idx[0..nb) of the training set into contiguous batch buffers.
+// Samples idx[0..nb) of the training set into contiguous batch buffers;
+// a sample is one conformation, or with pairs two consecutive ones.
 __global__ void k_gather(const double* __restrict__ v, const float* __restrict__ t, const int32_t* __restrict__ idx,
-                         int nb, int nvt, double* __restrict__ bv, float* __restrict__ bt) {
+                         int nb, int nvt, int per, double* __restrict__ bv, float* __restrict__ bt) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= (int64_t)nb * nvt) return;
-  const int c = (int)(k / nvt), j = (int)(k - (int64_t)c * nvt);
-  bv[k] = v[(size_t)idx[c] * nvt + j];
+  const int64_t row_len = (int64_t)per * nvt;
+  if (k >= (int64_t)nb * row_len) return;
+  const int c = (int)(k / row_len);
+  const int64_t j = k - (int64_t)c * row_len;
+  bv[k] = v[(size_t)idx[c] * row_len + j];
   if (j == 0) bt[c] = t[idx[c]];
 }
 
@@ -404,7 +432,7 @@ const float* mlp_theta_dev(MlpWork*& wp) {
 // FP64), and with targets (device) the squared errors (sq, device) and the
 // gradient of the mean squared error (grad, device).  Enqueued on st only.
 static void mlp_pass(MlpWork& w, const MlpResidues& R, hipStream_t st, const float* theta, const double* vlvr,
-                     int nconf, const float* target, float* pred, float* grad, float* sq) {
+                     int nconf, const float* target, float* pred, float* grad, float* sq, int pairs) {
   const int64_t rows = (int64_t)nconf * R.nres;
   bf16* x = w.x.get<bf16>(rows * kMlpIn);
   int16_t* rt = w.rtype.get<int16_t>(rows * 3);
@@ -432,7 +460,8 @@ static void mlp_pass(MlpWork& w, const MlpResidues& R, hipStream_t st, const flo
   hidden(h2, kMlpHid, w3, theta + kThB3, z3, h3);
   hipLaunchKernelGGL(k_mlp_out, dim3(nblocks(rows, 256)), dim3(256), 0, st, h3, theta, rows, y);
   float* dy = target ? w.dy.get<float>(rows) : nullptr;
-  hipLaunchKernelGGL(k_mlp_conf, dim3(nblocks(nconf, 256)), dim3(256), 0, st, y, R.nres, nconf, target, pred, dy, sq);
+  hipLaunchKernelGGL(k_mlp_conf, dim3(nblocks(nconf, 256)), dim3(256), 0, st, y, R.nres, nconf, target, pred, dy, sq,
+                     pairs);
   HIPCHK(hipGetLastError());
   if (!target) return;
   // ---- backward
@@ -493,51 +522,54 @@ static void adam_step(MlpWork& w, hipStream_t st, float lr) {
 // Predictions (and with targets the loss and gradient) of the context's
 // network for host conformations.
 double mlp_grad(MlpWork*& wp, const State& s, hipStream_t st, const double* vlvr, int nconf, const float* target,
-                float* pred, float* grad) {
+                float* pred, float* grad, int pairs) {
   const MlpResidues R = mlp_residues(wp, s, st);
   MlpWork& w = *wp;
   if (!w.have_theta) throw Error("surrogate> no network set (smmp_mlp_set)");
   double* dv = w.tmp_v.get<double>((size_t)nconf * R.nvt);
   HIPCHK(hipMemcpyAsync(dv, vlvr, sizeof(double) * nconf * R.nvt, hipMemcpyHostToDevice, st));
+  if (pairs && nconf % 2) throw Error("surrogate> pairs need an even number of conformations");
+  const int nt = pairs ? nconf / 2 : nconf;  // targets
   float* dt = nullptr;
   if (target) {
-    dt = w.tmp_t.get<float>(nconf);
-    HIPCHK(hipMemcpyAsync(dt, target, sizeof(float) * nconf, hipMemcpyHostToDevice, st));
+    dt = w.tmp_t.get<float>(nt);
+    HIPCHK(hipMemcpyAsync(dt, target, sizeof(float) * nt, hipMemcpyHostToDevice, st));
   }
   float* dp = w.pred.get<float>(nconf);
   float* sq = w.sq.get<float>(nconf);
   float* g = w.grad.get<float>(kThN);
-  mlp_pass(w, R, st, static_cast<const float*>(w.theta.p), dv, nconf, dt, dp, g, sq);
+  mlp_pass(w, R, st, static_cast<const float*>(w.theta.p), dv, nconf, dt, dp, g, sq, pairs);
   if (pred) HIPCHK(hipMemcpyAsync(pred, dp, sizeof(float) * nconf, hipMemcpyDeviceToHost, st));
-  std::vector<float> hsq(target ? nconf : 0);
+  std::vector<float> hsq(target ? nt : 0);
   if (target) {
     if (grad) HIPCHK(hipMemcpyAsync(grad, g, sizeof(float) * kThN, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(hsq.data(), sq, sizeof(float) * nconf, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(hsq.data(), sq, sizeof(float) * nt, hipMemcpyDeviceToHost, st));
   }
   HIPCHK(hipStreamSynchronize(st));
   double loss = 0.0;
   for (float v : hsq) loss += v;
-  return target ? loss / nconf : 0.0;
+  return target ? loss / nt : 0.0;
 }
 
 // Minibatch Adam on the context's network: n conformations (host) with
 // targets, 'epochs' passes in random order (std::mt19937 with seed), batches
 // of 'batch' (the last partial batch of an epoch is left out).
 void mlp_fit(MlpWork*& wp, const State& s, hipStream_t st, const double* vlvr, const float* target, int n, int epochs,
-             int batch, float lr, uint32_t seed, double* epoch_loss) {
+             int batch, float lr, uint32_t seed, double* epoch_loss, int pairs) {
   const MlpResidues R = mlp_residues(wp, s, st);
   MlpWork& w = *wp;
   if (!w.have_theta) throw Error("surrogate> no network set (smmp_mlp_set)");
   if (batch < 1 || n < batch) throw Error("surrogate> need at least one batch of training data");
-  double* dv = w.data_v.get<double>((size_t)n * R.nvt);
+  const int per = pairs ? 2 : 1;  // conformations per sample
+  double* dv = w.data_v.get<double>((size_t)n * per * R.nvt);
   float* dtg = w.data_t.get<float>(n);
-  HIPCHK(hipMemcpyAsync(dv, vlvr, sizeof(double) * (size_t)n * R.nvt, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dv, vlvr, sizeof(double) * (size_t)n * per * R.nvt, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(dtg, target, sizeof(float) * n, hipMemcpyHostToDevice, st));
   int32_t* dperm = w.perm.get<int32_t>(n);
-  double* bv = w.bv.get<double>((size_t)batch * R.nvt);
+  double* bv = w.bv.get<double>((size_t)batch * per * R.nvt);
   float* bt = w.bt.get<float>(batch);
-  float* sq = w.sq.get<float>(batch);
-  float* pred = w.pred.get<float>(batch);
+  float* sq = w.sq.get<float>(batch * per);
+  float* pred = w.pred.get<float>(batch * per);
   float* g = w.grad.get<float>(kThN);
   double* acc = w.lossacc.get<double>(1);
   std::vector<int32_t> perm(n);
@@ -549,9 +581,9 @@ void mlp_fit(MlpWork*& wp, const State& s, hipStream_t st, const double* vlvr, c
     HIPCHK(hipMemcpyAsync(dperm, perm.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemsetAsync(acc, 0, sizeof(double), st));
     for (int b = 0; b < nbatch; ++b) {
-      hipLaunchKernelGGL(k_gather, dim3(nblocks((int64_t)batch * R.nvt, 256)), dim3(256), 0, st, dv, dtg,
-                         dperm + (size_t)b * batch, batch, R.nvt, bv, bt);
-      mlp_pass(w, R, st, static_cast<const float*>(w.theta.p), bv, batch, bt, pred, g, sq);
+      hipLaunchKernelGGL(k_gather, dim3(nblocks((int64_t)batch * per * R.nvt, 256)), dim3(256), 0, st, dv, dtg,
+                         dperm + (size_t)b * batch, batch, R.nvt, per, bv, bt);
+      mlp_pass(w, R, st, static_cast<const float*>(w.theta.p), bv, batch * per, bt, pred, g, sq, pairs);
       hipLaunchKernelGGL(k_sum_sq, dim3(1), dim3(64), 0, st, sq, batch, acc);
       adam_step(w, st, lr);
     }

@@ -17,9 +17,10 @@ void mlp_get_theta(MlpWork*& w, hipStream_t st, float* host);
 const float* mlp_theta_dev(MlpWork*& w);
 // Predictions of nconf host conformations; with targets returns the mean
 // squared error and writes its gradient (host, may be null).
+// pairs: the conformations come in pairs (x, x') with targets E(x') - E(x).
 double mlp_grad(MlpWork*& w, const State& s, hipStream_t st, const double* vlvr, int nconf, const float* target,
-                float* pred, float* grad);
+                float* pred, float* grad, int pairs);
 void mlp_fit(MlpWork*& w, const State& s, hipStream_t st, const double* vlvr, const float* target, int n, int epochs,
-             int batch, float lr, uint32_t seed, double* epoch_loss);
+             int batch, float lr, uint32_t seed, double* epoch_loss, int pairs);
 
 }  // namespace smmp

@@ -296,23 +296,23 @@ int smmp_mlp_get(smmp_ctx* c, float* theta) {
   });
 }
 
-int smmp_mlp_grad(smmp_ctx* c, const double* vlvr, int nconf, const float* target, float* pred, float* grad,
-                  double* loss) {
-  if (!vlvr || nconf <= 0) return SMMP_EINVAL;
+int smmp_mlp_grad(smmp_ctx* c, const double* vlvr, int nconf, const float* target, int pairs, float* pred,
+                  float* grad, double* loss) {
+  if (!vlvr || nconf <= 0 || (pairs && nconf % 2)) return SMMP_EINVAL;
   return guarded(c, SMMP_ESTATE, [&] {
     if (c->s.ntlml < 1) throw Error("surrogate> no molecule initialised");
-    const double l = engine_mlp_grad(c->eng, c->s, vlvr, nconf, target, pred, grad);
+    const double l = engine_mlp_grad(c->eng, c->s, vlvr, nconf, target, pred, grad, pairs);
     if (loss) *loss = l;
     return SMMP_OK;
   });
 }
 
-int smmp_mlp_fit(smmp_ctx* c, const double* vlvr, const float* target, int n, int epochs, int batch, float lr,
-                 uint32_t seed, double* epoch_loss) {
+int smmp_mlp_fit(smmp_ctx* c, const double* vlvr, const float* target, int pairs, int n, int epochs, int batch,
+                 float lr, uint32_t seed, double* epoch_loss) {
   if (!vlvr || !target || n <= 0 || epochs < 0 || batch <= 0) return SMMP_EINVAL;
   return guarded(c, SMMP_ESTATE, [&] {
     if (c->s.ntlml < 1) throw Error("surrogate> no molecule initialised");
-    engine_mlp_fit(c->eng, c->s, vlvr, target, n, epochs, batch, lr, seed, epoch_loss);
+    engine_mlp_fit(c->eng, c->s, vlvr, target, n, epochs, batch, lr, seed, epoch_loss, pairs);
     return SMMP_OK;
   });
 }

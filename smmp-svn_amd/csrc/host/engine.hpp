@@ -45,9 +45,9 @@ int engine_sampler_info(Engine*& e, int32_t* out);
 void engine_mlp_set(Engine*& e, const float* theta);
 void engine_mlp_get(Engine*& e, float* theta);
 double engine_mlp_grad(Engine*& e, const State& s, const double* vlvr, int nconf, const float* target, float* pred,
-                       float* grad);
+                       float* grad, int pairs);
 void engine_mlp_fit(Engine*& e, const State& s, const double* vlvr, const float* target, int n, int epochs, int batch,
-                    float lr, uint32_t seed, double* epoch_loss);
+                    float lr, uint32_t seed, double* epoch_loss, int pairs);
 
 // twister.f:104 grnd() on the context's stream (host side; the draws of the
 // device samplers come from the same state copied to the device)

@@ -76,8 +76,8 @@ def _load():
         "smmp_mlp_size": (C.c_int, []),
         "smmp_mlp_set": (C.c_int, [P, fp]),
         "smmp_mlp_get": (C.c_int, [P, fp]),
-        "smmp_mlp_grad": (C.c_int, [P, dp, C.c_int, fp, fp, fp, dp]),
-        "smmp_mlp_fit": (C.c_int, [P, dp, fp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_uint32, dp]),
+        "smmp_mlp_grad": (C.c_int, [P, dp, C.c_int, fp, C.c_int, fp, fp, dp]),
+        "smmp_mlp_fit": (C.c_int, [P, dp, fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_uint32, dp]),
         "smmp_muca_update": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, ip, dp, dp, dp, dp, dp]),
         "smmp_muca_final": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, dp, dp]),
     }

@@ -66,21 +66,23 @@ class Surrogate:
         self.ctx.check(_ffi.lib.smmp_mlp_get(self.ctx.h, a.ctypes.data_as(C.POINTER(C.c_float))))
         return a
 
-    def grad(self, vlvr, target=None):
+    def grad(self, vlvr, target=None, pairs: bool = False):
         """(loss, predictions in network units, gradient) for conformations
-        vlvr (n x nvt) and targets in network units (or None)."""
+        vlvr (n x nvt) and targets in network units (or None); with pairs
+        the conformations come in pairs (x, x') and the targets are for
+        the differences of the outputs."""
         v = np.ascontiguousarray(vlvr, dtype=np.float64)
         n = len(v)
         pred = np.zeros(n, dtype=np.float32)
         fp = C.POINTER(C.c_float)
         if target is None:
-            self.ctx.check(_ffi.lib.smmp_mlp_grad(self.ctx.h, _ffi.dptr(v), n, None, pred.ctypes.data_as(fp), None,
-                                                  None))
+            self.ctx.check(_ffi.lib.smmp_mlp_grad(self.ctx.h, _ffi.dptr(v), n, None, 0, pred.ctypes.data_as(fp),
+                                                  None, None))
             return None, pred, None
         t = np.ascontiguousarray(target, dtype=np.float32)
         g = np.zeros(self.n, dtype=np.float32)
         loss = C.c_double(0.0)
-        self.ctx.check(_ffi.lib.smmp_mlp_grad(self.ctx.h, _ffi.dptr(v), n, t.ctypes.data_as(fp),
+        self.ctx.check(_ffi.lib.smmp_mlp_grad(self.ctx.h, _ffi.dptr(v), n, t.ctypes.data_as(fp), int(pairs),
                                               pred.ctypes.data_as(fp), g.ctypes.data_as(fp), C.byref(loss)))
         return loss.value, pred, g
 
@@ -103,17 +105,55 @@ class Surrogate:
         if rescale:
             self.offset = float(e.mean())
             self.scale = float(max(e.std(), 1e-6))
-        t = np.ascontiguousarray((e - self.offset) / self.scale, dtype=np.float32)
-        v = np.ascontiguousarray(v)
+        t = (e - self.offset) / self.scale
+        return self._fit(v, t, 0, epochs, batch, lr, seed)
+
+    def fit_moves(self, pairs_v, delta_e, epochs: int = 20, batch: int = 256, lr: float = 2e-3, seed: int = 0,
+                  clip: float = 20.0, rescale: bool = True) -> np.ndarray:
+        """Fits the network's energy change to exact changes of single moves:
+        pairs_v (2n x nvt: x, x' alternating), delta_e (n, kcal/mol), the
+        changes clipped to [-clip, clip] (beyond a few kT only the sign of a
+        change matters to the first stage).  Returns the loss of every epoch."""
+        d = np.clip(np.asarray(delta_e, dtype=np.float64), -clip, clip)
+        if rescale:
+            self.offset = 0.0
+            self.scale = float(max(d.std(), 1e-6))
+        return self._fit(np.asarray(pairs_v, dtype=np.float64), d / self.scale, 1, epochs, batch, lr, seed)
+
+    def _fit(self, v, t, pairs, epochs, batch, lr, seed):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        t = np.ascontiguousarray(t, dtype=np.float32)
         hist = np.zeros(epochs)
         self.ctx.check(_ffi.lib.smmp_mlp_fit(self.ctx.h, _ffi.dptr(v), t.ctypes.data_as(C.POINTER(C.c_float)),
-                                             len(t), epochs, batch, C.c_float(lr), C.c_uint32(seed & 0xFFFFFFFF),
-                                             _ffi.dptr(hist)))
+                                             int(pairs), len(t), epochs, batch, C.c_float(lr),
+                                             C.c_uint32(seed & 0xFFFFFFFF), _ffi.dptr(hist)))
         return hist
 
     def sample(self, temps, seeds, nsweep, vlvr, **kw):
         """smmp_sample with delayed acceptance screened by this network."""
         return self.ctx.sample(temps, seeds, nsweep, vlvr, da=True, da_scale=self.scale, **kw)
+
+
+def move_pairs(ctx, vlvr, seed: int = 0):
+    """Training pairs of single moves from sampled conformations: for each
+    conformation x, one free variable changed as the sampler proposes it
+    (addang(v, axvr (u - 1/2)), metropolis.f:64-66) gives x'; both energies
+    come from smmp_energy_batch (FP64).  Returns (pairs (2n x nvt, x and x'
+    alternating), E(x') - E(x), E(x))."""
+    v = np.asarray(vlvr, dtype=np.float64)
+    n, nvt = v.shape
+    rng = np.random.default_rng(seed)
+    idvr = ctx.idvr - 1
+    ax = np.asarray(ctx.view("var_r.axvr")[:nvt], dtype=np.float64)
+    j = idvr[rng.integers(0, len(idvr), n)]
+    x1 = v.copy()
+    a = x1[np.arange(n), j] + ax[j] * (rng.random(n) - 0.5)
+    x1[np.arange(n), j] = np.mod(a + np.pi, 2 * np.pi) - np.pi
+    both = np.empty((2 * n, nvt))
+    both[0::2] = v
+    both[1::2] = x1
+    e = _ffi.energy_batch([ctx], np.zeros(2 * n, dtype=np.int32), both.ravel(), precision=0)[:, 0]
+    return both, e[1::2] - e[0::2], e[0::2]
 
 
 def da_gain(plain: dict, da: dict, t_plain: float, t_da: float) -> dict:

@@ -56,6 +56,26 @@ def test_forward_and_gradient_kernels_match_oracle():
         assert rel <= 2e-2, (name, rel)
 
 
+def test_pair_gradient_kernels_match_oracle():
+    from smmp_amd.surrogate import Surrogate
+    c = make_ctx(tag="c2_20mer", itysol=1)
+    R = so.Residues.from_context(c)
+    V = _confs(c, 64, 9)
+    th = so.init_theta(5)
+    tgt = np.random.default_rng(6).normal(0, 1, 32).astype(np.float32)
+    s = Surrogate(c, theta=th)
+    say("smmp_mlp_grad c2_20mer 32 pairs")
+    loss, pg, g = s.grad(V, tgt, pairs=True)
+    lo, go, po = so.loss_grad(R, th.astype(np.float64), V, tgt.astype(np.float64), bf=True, pairs=True)
+    assert abs(loss - lo) <= 5e-3 * lo
+    for name, (a, b) in _blocks().items():
+        if name == "b4":
+            assert abs(g[a]) < 1e-4
+            continue
+        rel = np.linalg.norm(g[a:b] - go[a:b]) / (np.linalg.norm(go[a:b]) + 1e-12)
+        assert rel <= 2e-2, (name, rel)
+
+
 def test_fit_step_is_adam():
     """One epoch with the whole set as one batch is one Adam step with the
     full-batch gradient: theta - lr g / (|g| + eps) at the first step."""

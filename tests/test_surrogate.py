@@ -47,6 +47,28 @@ def test_network_gradient_matches_finite_differences():
         assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(g[i])), (i, g[i], fd)
 
 
+def test_pair_loss_gradient_matches_finite_differences():
+    """The loss on differences of pairs (x, x'), the form the network is
+    trained in for the first stage."""
+    c = make_ctx(tag="enkefa", itysol=1)
+    R = so.Residues.from_context(c)
+    rng = np.random.default_rng(1)
+    V = np.tile(c.vlvr(), (6, 1)) + rng.normal(0, 1, (6, len(c.vlvr())))
+    tgt = np.array([0.5, -0.7, 1.1])
+    th = so.init_theta(4).astype(np.float64)
+    _, g, _ = so.loss_grad(R, th, V, tgt, bf=False, pairs=True)
+    for i in (so.W1 + 11, so.B1, so.W2 + 500, so.W3 + 3, so.W4 + 2, so.EMBO + 16 * 19 + 1):
+        h = 1e-6
+        tp, tm = th.copy(), th.copy()
+        tp[i] += h
+        tm[i] -= h
+        fd = (so.loss_grad(R, tp, V, tgt, bf=False, pairs=True)[0]
+              - so.loss_grad(R, tm, V, tgt, bf=False, pairs=True)[0]) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(g[i])), (i, g[i], fd)
+    # the bias of the output cancels in a difference
+    assert abs(g[so.B4]) < 1e-12
+
+
 def _da_kernel(E, S, beta):
     """Transition matrix of one trial: uniform proposal to another state,
     stage 1 with the surrogate S, stage 2 with E - S (the sampler's rule)."""
