@@ -12,6 +12,10 @@
 !   canon  libdir seqfile varfile itysol seed temp nsweep  Metropolis run
 !   partem libdir seqfile itysol seed nrep nequi nswp [switch]  partem_s run
 !   muca   libdir seqfile itysol seed nsweep nup temp kmin kmax ebin
+!   esolan libdir seqfile radfile seed nconf       analytic solvent (itysol -1):
+!          energies of the start and of random conformations, and the
+!          twister position after each (esolan.f draws on degenerate geometry);
+!          radfile ('-' or 'index radians' lines) sets start variables
 !   multi  libdir seqfile itysol gbfile boxsize seed nconf temp nequi nsweep
 !          several chains: gbfile holds 'molecule k value' lines for gbpr;
 !          topology, energies of random conformations with the
@@ -24,10 +28,11 @@ program ref_driver
   double precision energy, grnd, addang
   external can_weight
   character(len=256) :: mode, arg
-  character(len=80) :: libdir, seqf, varf
+  character(len=80) :: libdir, seqf, varf, radf
   character(len=4) :: grpn, grpc
   integer :: iseed, n, i, j, k, nconf, nsw, isol, nrep, nequi, nswp
-  integer :: kmn, kmx, nup, iswt
+  integer :: kmn, kmx, nup, iswt, ios
+  double precision :: vrad
   double precision :: e, eolx, acz, tmp, ebw
   double precision :: ggasc
   parameter (ggasc = 1.98773d-3)
@@ -125,6 +130,48 @@ program ref_driver
     call mulcan_par(nsw, nup, tmp, kmn, kmx, ebw, .false.)
     write(*,'(a,i0)') '@done int 1'
     write(*,'(i0)') 1
+
+  case ('esolan')
+    call get_command_argument(2, arg); libdir = arg
+    call get_command_argument(3, arg); seqf = arg
+    call get_command_argument(4, arg); radf = arg
+    varf = ' '
+    call iarg(5, iseed)
+    call iarg(6, nconf)
+    call setup_sol(-1)
+    ! 'index radians' lines go straight into vlvr: the start energy's own
+    ! setvar applies them (one rebuild from the initial torsions)
+    if (trim(radf) .ne. '-') then
+      open(79, file=trim(radf), status='old')
+      do
+        read(79, *, iostat=ios) j, vrad
+        if (ios .ne. 0) exit
+        vlvr(j) = vrad
+      end do
+      close(79)
+    end if
+    call sgrnd(iseed)
+    e = energy()
+    call dump_energy(e)
+    write(*,'(a,i0)') '@mti int 1'
+    write(*,'(i0)') mtstate_pos()
+    call dump_coords()
+    write(*,'(a,i0)') '@nconf int 1'
+    write(*,'(i0)') nconf
+    do k = 1, nconf
+      do i = 1, nvr
+        j = idvr(i)
+        vlvr(j) = addang(pi, axvr(j)*(grnd()-0.5))
+      end do
+      e = energy()
+      write(*,'(a,i0)') '@vlvr double ', nvr
+      do i = 1, nvr
+        write(*,'(es25.17)') vlvr(idvr(i))
+      end do
+      call dump_energy(e)
+      write(*,'(a,i0)') '@mti int 1'
+      write(*,'(i0)') mtstate_pos()
+    end do
 
   case ('multi')
     call setup(2, .false.)
@@ -239,6 +286,28 @@ contains
     ientyp = 0
     tesgrd = .false.
     itysol = isol
+    isolscl = .false.
+    upchswitch = 0
+    rndord = .false.
+    bgsprob = 0.0d0
+    ntlml = 0
+    call init_energy(libdir)
+    nov = ' '
+    call init_molecule(1, grpn, grpc, seqf, nov)
+    if (len_trim(varf) .gt. 0) call apply_vars(varf)
+  end subroutine
+
+  ! setup() with the library, sequence and variable files already in
+  ! libdir/seqf/varf and the solvent model isol given
+  subroutine setup_sol(isol_in)
+    integer, intent(in) :: isol_in
+    character(len=80) :: nov
+    flex = .false.
+    sh2 = .false.
+    epsd = .false.
+    ientyp = 0
+    tesgrd = .false.
+    itysol = isol_in
     isolscl = .false.
     upchswitch = 0
     rndord = .false.

@@ -455,12 +455,22 @@ double oracle_eninteract(const oracle_topo *t, oracle_conf *c) {
 
 /* energy.f:12-119: intra-chain terms of all chains, solvent over all atoms,
  * then the terms between chains (added to the total only, energy.f:114) */
+/* The generator of esolan.f's random rotations in oracle_energy (the
+ * reference draws them from the one global twister). */
+static oracle_mt *g_sol_mt = NULL;
+void oracle_set_solvent_mt(oracle_mt *g) { g_sol_mt = g; }
+
+double oracle_esolan(const oracle_topo *t, oracle_conf *c, oracle_mt *g, double *as_out);
+
 double oracle_energy(const oracle_topo *t, oracle_conf *c, const double *vlvr) {
   oracle_setvar(t, c, vlvr);
   double esm = 0.0;
   esm = esm + oracle_enyshe(t, c);
   if (t->itysol > 0) {
     esm = esm + oracle_enysol(t, c);
+  } else if (t->itysol < 0) { /* energy.f:79-82: esolan(0) over all chains */
+    c->e[6] = oracle_esolan(t, c, g_sol_mt, NULL);
+    esm = esm + c->e[6];
   } else {
     c->e[6] = 0.0;
   }
@@ -703,4 +713,417 @@ void oracle_muca_update(int kmin, int kmax, double ebin, double beta, const int3
 #undef G
 #undef B
 #undef AL
+}
+
+/* ------------------------------------------------ esolan.f:12-1050 (energy) */
+
+/* Test hook: >= 0 forces the random rotation of esolan.f:200-232 for every
+ * atom with these (u_gamma, u_phi) in [0, 1) instead of drawing them. */
+static double g_es_force_uga = -1.0, g_es_force_ufi = -1.0;
+void oracle_esolan_force_rotation(double uga, double ufi) {
+  g_es_force_uga = uga;
+  g_es_force_ufi = ufi;
+}
+
+#define ES_MAXNB 256
+#define ES_MAXV (4 * ES_MAXNB * ES_MAXNB)
+
+/* Analytic solvent accessible surface of every atom (projection method,
+ * esolan.f), energy part: returns sum sigma_i As_i and writes As (nat, may be
+ * NULL).  The random rotations of esolan.f:200-232 draw from g.  Returns NAN
+ * if two atom centres coincide (esolan.f:127-133 stops there). */
+double oracle_esolan(const oracle_topo *t, oracle_conf *c, oracle_mt *g, double *as_out) {
+  const int numat = t->nat;
+  static int neib[ES_MAXNB + 2], neibp[ES_MAXNB + 2], fullarc[ES_MAXNB + 2], al[2 * ES_MAXNB + 2];
+  static int ta2[ES_MAXNB + 2], ivrx[ES_MAXV + 2];
+  static double vertex[ES_MAXV + 1][4];
+  static double ayx[2 * ES_MAXNB + 2][2], ayx1[2 * ES_MAXNB + 2], ax[2 * ES_MAXNB + 2][2], probe[2 * ES_MAXNB + 2];
+  double *ddat = (double *)malloc(sizeof(double) * 4 * (size_t)(numat + 1));
+  double *neibor = (double *)malloc(sizeof(double) * 4 * (size_t)(numat + 1));
+  double sss = 0.0;
+#define DD(k, m) ddat[4 * (k) + (m) - 1]
+#define NB(k, m) neibor[4 * (k) + (m) - 1]
+  for (int i = 1; i <= numat; ++i) {
+    const double R = A(t->rvdw, i), poli = R * R;
+    double As = 4.0 * PI * poli;
+    const double xi = A(c->xat, i), yi = A(c->yat, i), zi = A(c->zat, i);
+    int jj = 0, bad = 0;
+    for (int j = 1; j <= numat; ++j) {
+      if (j == i) continue;
+      double ddp = (A(c->xat, j) - xi) * (A(c->xat, j) - xi) + (A(c->yat, j) - yi) * (A(c->yat, j) - yi) +
+                   (A(c->zat, j) - zi) * (A(c->zat, j) - zi);
+      if (ddp < 1e-10) {
+        free(ddat);
+        free(neibor);
+        return NAN;
+      }
+      ddp = sqrt(ddp);
+      if (ddp + R <= A(t->rvdw, j)) {  /* atom i inside atom j */
+        As = 0.0;
+        bad = 1;
+        break;
+      }
+      if (!(ddp <= R - A(t->rvdw, j) || ddp >= R + A(t->rvdw, j))) {
+        if (jj >= ES_MAXNB) {
+          free(ddat);
+          free(neibor);
+          return NAN;
+        }
+        neib[++jj] = j;
+      }
+    }
+    if (bad || jj == 0) goto atom_done;
+    neib[0] = jj;
+    {
+      const double R2 = R * R, R22 = R2 + R2, R42 = R22 + R22, R22p = R22 * PI;
+      for (int q = 1; q <= neib[0]; ++q) {
+        const int k = neib[q];
+        DD(k, 2) = A(c->xat, k) - xi;
+        DD(k, 3) = A(c->yat, k) - yi;
+        DD(k, 4) = A(c->zat, k) - zi;
+        DD(k, 1) = A(t->rvdw, k);
+      }
+      /* the north point of sphere i against the neighbouring surfaces (esolan.f:188-200) */
+      int jjj = 0;
+      double cf = 1.0, sf = 0.0, cg = 1.0, sg = 0.0;
+      {
+        const int k = neib[1];
+        const double polk = A(t->rvdw, k) * A(t->rvdw, k);
+        double ddp = fabs(DD(k, 2) * DD(k, 2) + DD(k, 3) * DD(k, 3) + (R - DD(k, 4)) * (R - DD(k, 4)) - polk);
+        for (int q = 2; q <= neib[0]; ++q) {
+          const int kk = neib[q];
+          const double pk = A(t->rvdw, kk) * A(t->rvdw, kk);
+          const double d2 = fabs(DD(kk, 2) * DD(kk, 2) + DD(kk, 3) * DD(kk, 3) + (R - DD(kk, 4)) * (R - DD(kk, 4)) - pk);
+          if (d2 < ddp) ddp = d2;
+        }
+        while (ddp < 1.0e-6 || (g_es_force_uga >= 0.0 && jjj == 0)) {
+          ++jjj;
+          if (g_es_force_uga < 0.0 && !g) {
+            free(ddat);
+            free(neibor);
+            return NAN;
+          }
+          double uga = g_es_force_uga >= 0.0 ? g_es_force_uga : oracle_grnd(g);
+          double ufi = g_es_force_uga >= 0.0 ? g_es_force_ufi : oracle_grnd(g);
+          uga = PI * uga / 2;
+          ufi = PI * ufi / 2;
+          cf = cos(ufi);
+          sf = sin(ufi);
+          cg = cos(uga);
+          sg = sin(uga);
+          const double cfsg = cf * sg, cfcg = cf * cg;
+          const double xx = R * cfsg, yy = R * sf, zz = R * cfcg;
+          const int k1 = neib[1];
+          const double p1 = A(t->rvdw, k1) * A(t->rvdw, k1);
+          ddp = fabs((xx - DD(k1, 2)) * (xx - DD(k1, 2)) + (yy - DD(k1, 3)) * (yy - DD(k1, 3)) +
+                     (zz - DD(k1, 4)) * (zz - DD(k1, 4)) - p1);
+          for (int q = 2; q <= neib[0]; ++q) {
+            const int kk = neib[q];
+            const double pk = A(t->rvdw, kk) * A(t->rvdw, kk);
+            const double d2 = fabs((xx - DD(kk, 2)) * (xx - DD(kk, 2)) + (yy - DD(kk, 3)) * (yy - DD(kk, 3)) +
+                                   (zz - DD(kk, 4)) * (zz - DD(kk, 4)) - pk);
+            if (d2 < ddp) ddp = d2;
+          }
+          if (g_es_force_uga >= 0.0) break;
+        }
+      }
+      if (jjj != 0) { /* rotate the neighbours (esolan.f:219-232) */
+        const double sfsg = sf * sg, sfcg = sf * cg, cfsg = cf * sg, cfcg = cf * cg;
+        for (int q = 1; q <= neib[0]; ++q) {
+          const int k = neib[q];
+          const double xx = DD(k, 2), yy = DD(k, 3), zz = DD(k, 4);
+          DD(k, 2) = xx * cg - zz * sg;
+          DD(k, 3) = -xx * sfsg + yy * cf - zz * sfcg;
+          DD(k, 4) = xx * cfsg + yy * sf + zz * cfcg;
+        }
+      }
+      /* circles a (t^2 + s^2) + b t + c s + d = 0 of the projection (esolan.f:238-247) */
+      const double pom8 = 8.0 * poli;
+      for (int q = 1; q <= neib[0]; ++q) {
+        const int j = neib[q];
+        const double pj = A(t->rvdw, j) * A(t->rvdw, j);
+        NB(j, 1) = DD(j, 2) * DD(j, 2) + DD(j, 3) * DD(j, 3) + (DD(j, 4) - R) * (DD(j, 4) - R) - pj;
+        NB(j, 2) = -pom8 * DD(j, 2);
+        NB(j, 3) = -pom8 * DD(j, 3);
+        NB(j, 4) = 4.0 * poli * (DD(j, 2) * DD(j, 2) + DD(j, 3) * DD(j, 3) + (R + DD(j, 4)) * (R + DD(j, 4)) - pj);
+      }
+      /* pairs of circles (esolan.f:251-451): enclosed ones dropped, crossing ones give vertices */
+      int iv = 0;
+      const int nb = neib[0];
+      int k = neib[0];
+      while (k > 1) {
+        k = k - 1;
+        const int lstart = neib[0];
+        const int count = lstart - k > 0 ? lstart - k : 0; /* trip count fixed at entry */
+        for (int it = 0, L = lstart; it < count; ++it, --L) {
+          const double ak = NB(neib[k], 1), aL = NB(neib[L], 1);
+          int kind = 0;
+          if (ak > 0.0 && aL > 0.0) kind = 1;
+          else if (ak > 0.0 && aL < 0.0) kind = 2;
+          else if (ak < 0.0 && aL > 0.0) kind = 3;
+          else if (ak < 0.0 && aL < 0.0) kind = 4;
+          if (!kind) continue;
+          const double b1 = NB(neib[k], 2) / ak, c1 = NB(neib[k], 3) / ak, d1 = NB(neib[k], 4) / ak;
+          const double b2 = NB(neib[L], 2) / aL, c2 = NB(neib[L], 3) / aL, d2 = NB(neib[L], 4) / aL;
+          const double D = 4.0 * ((b1 - b2) * (b2 * d1 - b1 * d2) + (c1 - c2) * (c2 * d1 - c1 * d2) - (d1 - d2) * (d1 - d2)) +
+                           (b1 * c2 - b2 * c1) * (b1 * c2 - b2 * c1);
+          const double dist = sqrt((b1 - b2) * (b1 - b2) + (c1 - c2) * (c1 - c2));
+          const double r1 = sqrt(b1 * b1 + c1 * c1 - 4.0 * d1), r2 = sqrt(b2 * b2 + c2 * c2 - 4.0 * d2);
+          int drop = 0; /* 1: drop k and leave the inner loop, 2: drop L */
+          int vert = 0, covered = 0, leave = 0;
+          if (kind == 1) {
+            if (D <= 0.0 && dist <= r2 - r1) drop = 1;
+            else if (D <= 0.0 && dist <= -r2 + r1) drop = 2;
+            else if (D > 0.0) vert = 1;
+          } else if (kind == 2) {
+            if (D <= 0.0 && dist <= -r2 + r1) covered = 1;
+            else if (D <= 0.0 && dist >= r2 + r1) drop = 1;
+            else if (D > 0.0) vert = 1;
+          } else if (kind == 3) {
+            if (D <= 0.0 && dist <= r2 - r1) {
+              As = 0.0; /* esolan.f:361-364: and the loop goes on with the next k */
+              leave = 1;
+            } else if (D <= 0.0 && dist >= r2 + r1) drop = 2;
+            else if (D > 0.0) vert = 1;
+          } else {
+            if (D <= 0.0 && dist <= r2 - r1) drop = 2;
+            else if (D <= 0.0 && dist <= -r2 + r1) drop = 1;
+            else if (D <= 0.0) covered = 1;
+            else vert = 1;
+          }
+          if (covered) {
+            As = 0.0;
+            goto arcs_done;
+          }
+          if (drop == 1) {
+            neib[0] = neib[0] - 1;
+            for (int q = k; q <= neib[0]; ++q) neib[q] = neib[q + 1];
+            break;
+          }
+          if (drop == 2) {
+            neib[0] = neib[0] - 1;
+            for (int q = L; q <= neib[0]; ++q) neib[q] = neib[q + 1];
+          }
+          if (leave) break;
+          if (vert) {
+            if (iv + 2 > ES_MAXV) {
+              free(ddat);
+              free(neibor);
+              return NAN;
+            }
+            const double am = 2.0 * ((b1 - b2) * (b1 - b2) + (c1 - c2) * (c1 - c2));
+            const double tt = -2.0 * (b1 - b2) * (d1 - d2) - (b2 * c1 - b1 * c2) * (c1 - c2);
+            const double ss = -2.0 * (c1 - c2) * (d1 - d2) + (b2 * c1 - b1 * c2) * (b1 - b2);
+            const double pm = sqrt(D), p1 = (c1 - c2) * pm, p2 = (b1 - b2) * pm;
+            ++iv;
+            vertex[iv][0] = (tt + p1) / am;
+            vertex[iv][1] = (ss - p2) / am;
+            vertex[iv][2] = neib[k];
+            vertex[iv][3] = neib[L];
+            ++iv;
+            vertex[iv][0] = (tt - p1) / am;
+            vertex[iv][1] = (ss + p2) / am;
+            vertex[iv][2] = neib[k];
+            vertex[iv][3] = neib[L];
+          }
+        }
+      }
+      /* lines through the north point, inner circles (esolan.f:453-503) */
+      int ita2 = 0, ita3 = 0;
+      for (int q = 1; q <= neib[0]; ++q) {
+        if (NB(neib[q], 1) == 0.0) ta2[++ita2] = q;
+        if (NB(neib[q], 1) < 0.0) ++ita3;
+      }
+      if (ita2 > 0 && ita3 < 1) {
+        As = R22p;
+        if (ita2 > 1) {
+          double amial = PI, amaal = -PI, amibe = PI, amabe = -PI;
+          for (int q = 1; q <= ita2; ++q) {
+            const int jq = ta2[q];
+            const double alp = atan2(NB(neib[jq], 3), NB(neib[jq], 2));
+            double bet = alp;
+            if (alp <= 0.0) bet = bet + PI2;
+            if (amial > alp) amial = alp;
+            if (amaal < alp) amaal = alp;
+            if (amibe > bet) amibe = bet;
+            if (amabe < bet) amabe = bet;
+          }
+          const double dal = amaal - amial, dbe = amabe - amibe;
+          if (dal < PI) As = R22 * (PI - dal);
+          else if (dbe < PI) As = R22 * (PI - dbe);
+          else As = 0.0;
+        }
+      } else if ((nb > 0 && neib[0] < 1) || ita3 > 0) {
+        As = 0.0;
+      }
+      /* circles without vertices (full arcs) and with them (esolan.f:505-526) */
+      neibp[0] = neib[0];
+      for (int q = 1; q <= neib[0]; ++q) neibp[q] = neib[q];
+      int ifu = 0, ine = 0;
+      for (int q = 1; q <= neibp[0]; ++q) {
+        int has = 0;
+        for (int v = 1; v <= iv; ++v)
+          if (neibp[q] == (int)vertex[v][2] || neibp[q] == (int)vertex[v][3]) {
+            has = 1;
+            break;
+          }
+        if (has) neib[++ine] = neibp[q];
+        else {
+          ++ifu;
+          fullarc[ifu] = neibp[q];
+          al[ifu] = neibp[q];
+        }
+      }
+      neib[0] = ine;
+      fullarc[0] = ifu;
+      al[0] = ifu;
+      for (int q = 1; q <= ifu; ++q) { /* esolan.f:529-560 */
+        const double a = NB(fullarc[q], 1), b = NB(fullarc[q], 2), cc = NB(fullarc[q], 3), d = NB(fullarc[q], 4);
+        const double aia = a < 0.0 ? -1.0 : 1.0;
+        if (a != 0.0)
+          As = As - aia * R22p * (1.0 + (-d / a - R42) * fabs(a) / sqrt((R42 * a - d) * (R42 * a - d) + R42 * (b * b + cc * cc)));
+        else
+          As = As + R22p * d / sqrt(R42 * (b * b + cc * cc) + d * d);
+      }
+      for (int q = 1; q <= ine; ++q) { /* esolan.f:563-957 */
+        const int jk = neib[q];
+        const double a = NB(jk, 1), b = NB(jk, 2), cc = NB(jk, 3), d = NB(jk, 4);
+        const double ak2 = a * a, daba = fabs(a), aia = a < 0.0 ? -1.0 : 1.0, b2c2 = b * b + cc * cc;
+        const double vv = sqrt((R42 * a - d) * (R42 * a - d) + R42 * b2c2), vv1 = 1.0 / vv;
+        const double wv = sqrt(b2c2 - 4.0 * a * d);
+        int nyx = 0, ivr = 0;
+        for (int v = 1; v <= iv; ++v) {
+          if ((int)vertex[v][2] == jk) {
+            ++nyx;
+            ayx[nyx][0] = vertex[v][0];
+            ayx[nyx][1] = vertex[v][1];
+            ivrx[++ivr] = v;
+          }
+          if ((int)vertex[v][3] == jk) {
+            ++nyx;
+            ayx[nyx][0] = vertex[v][0];
+            ayx[nyx][1] = vertex[v][1];
+            const double sw = vertex[v][2];
+            vertex[v][2] = vertex[v][3];
+            vertex[v][3] = sw;
+            ivrx[++ivr] = v;
+          }
+        }
+        int ial = ifu;
+        for (int u = 1; u <= ine; ++u)
+          if (neib[u] != jk) al[++ial] = neib[u];
+        al[0] = ial;
+        if (a != 0.0) {
+          const double aa = 0.5 / a, ba = b * aa, ca = cc * aa, bcd = b2c2 - 2.0 * a * d;
+          for (int u = 1; u <= nyx; ++u) {
+            ayx[u][0] += ba;
+            ayx[u][1] += ca;
+          }
+          const double ct = -ba, cs = -ca, rr = 0.5 * wv / daba;
+          for (int u = 1; u <= nyx; ++u) ayx1[u] = atan2(ayx[u][1], ayx[u][0]);
+          for (int u = 1; u <= nyx - 1; ++u) { /* selection sort of the polar angles */
+            double a1 = ayx1[u];
+            int jm = u;
+            for (int w = u + 1; w <= nyx; ++w)
+              if (a1 > ayx1[w]) {
+                a1 = ayx1[w];
+                jm = w;
+              }
+            if (jm != u) {
+              a1 = ayx1[jm];
+              ayx1[jm] = ayx1[u];
+              ayx1[u] = a1;
+              const int tmp = ivrx[jm];
+              ivrx[jm] = ivrx[u];
+              ivrx[u] = tmp;
+            }
+          }
+          ayx1[nyx + 1] = ayx1[1] + PI2;
+          for (int u = 1; u <= nyx; ++u) {
+            if (fabs(ayx1[u + 1] - ayx1[u]) < 1e-8) continue;
+            const double prob = (ayx1[u] + ayx1[u + 1]) / 2.0;
+            const double ap1 = ct + rr * cos(prob), ap2 = cs + rr * sin(prob);
+            int cov = 0;
+            for (int w = 1; w <= ial; ++w)
+              if (NB(al[w], 1) * (ap1 * ap1 + ap2 * ap2) + NB(al[w], 2) * ap1 + NB(al[w], 3) * ap2 + NB(al[w], 4) < 0.0) {
+                cov = 1;
+                break;
+              }
+            if (cov) continue;
+            const double alp = ayx1[u], bet = ayx1[u + 1];
+            const double dcbetmalp = cos(5e-1 * (bet - alp)), dcbetpalp = cos(5e-1 * (bet + alp));
+            const double dsbetmalp = sin(5e-1 * (bet - alp)), dsbetpalp = sin(5e-1 * (bet + alp));
+            const double uv = daba * (bcd + 2.0 * R42 * ak2) * dcbetmalp -
+                              a * sqrt(b2c2 - 4.0 * a * d) * (b * dcbetpalp + cc * dsbetpalp);
+            As = As + R2 * (aia * (alp - bet) + (d + R42 * a) * vv1 * (PI - 2.0 * atan(uv / (2.0 * ak2 * vv * dsbetmalp))));
+          }
+        } else { /* the circle is a line (esolan.f:858-957) */
+          const double ang = atan2(b, cc), s1 = sin(ang), s2 = cos(ang);
+          for (int u = 1; u <= nyx; ++u) {
+            ax[u][0] = ayx[u][0] * s2 - ayx[u][1] * s1;
+            ax[u][1] = ayx[u][0] * s1 + ayx[u][1] * s2;
+          }
+          for (int u = 1; u <= nyx - 1; ++u) {
+            double a1 = ax[u][0];
+            int jm = u;
+            for (int w = u + 1; w <= nyx; ++w)
+              if (a1 > ax[w][0]) {
+                a1 = ax[w][0];
+                jm = w;
+              }
+            if (jm != u) {
+              double t0 = ax[jm][0], t1 = ax[jm][1];
+              ax[jm][0] = ax[u][0];
+              ax[jm][1] = ax[u][1];
+              ax[u][0] = t0;
+              ax[u][1] = t1;
+              t0 = ayx[jm][0];
+              t1 = ayx[jm][1];
+              ayx[jm][0] = ayx[u][0];
+              ayx[jm][1] = ayx[u][1];
+              ayx[u][0] = t0;
+              ayx[u][1] = t1;
+            }
+          }
+          if (cc != 0.0)
+            for (int u = 1; u <= nyx; ++u) ayx1[u] = (ayx[u][0] - ayx[1][0]) / cc;
+          else
+            for (int u = 1; u <= nyx; ++u) ayx1[u] = -(ayx[u][1] - ayx[1][1]) / b;
+          probe[1] = -1.0;
+          for (int u = 1; u <= nyx - 1; ++u) probe[u + 1] = (ayx1[u] + ayx1[u + 1]) / 2.0;
+          probe[nyx + 1] = ayx1[nyx] + 1.0;
+          const double bc = b * b + cc * cc, ccs = sqrt(R42 * bc + d * d), caba = cc * ayx[1][0] - b * ayx[1][1];
+          for (int u = 1; u <= nyx + 1; ++u) {
+            const double p1 = ayx[1][0] + cc * probe[u], p2 = ayx[1][1] - b * probe[u];
+            int cov = 0;
+            for (int w = 1; w <= ial; ++w)
+              if (NB(al[w], 1) * (p1 * p1 + p2 * p2) + NB(al[w], 2) * p1 + NB(al[w], 3) * p2 + NB(al[w], 4) < 0.0) {
+                cov = 1;
+                break;
+              }
+            if (cov) continue;
+            if (u == 1) {
+              const double arg2 = bc * ayx1[1] + caba;
+              As = As + R22 * d * (atan(arg2 / ccs) + PI / 2.0) / ccs;
+            } else if (u <= nyx) {
+              const double arg1 = bc * ayx1[u - 1] + caba, arg2 = bc * ayx1[u] + caba;
+              As = As + R22 * d * (atan(arg2 / ccs) - atan(arg1 / ccs)) / ccs;
+            } else {
+              const double arg1 = bc * ayx1[nyx] + caba;
+              As = As + R22 * d * (PI / 2.0 - atan(arg1 / ccs)) / ccs;
+            }
+          }
+        }
+      }
+    arcs_done:;
+    }
+  atom_done:
+    if (as_out) A(as_out, i) = As;
+    sss = sss + As * A(t->sigma, i);
+  }
+#undef DD
+#undef NB
+  free(ddat);
+  free(neibor);
+  return sss;
 }

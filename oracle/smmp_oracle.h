@@ -73,6 +73,14 @@ void oracle_setvar(const oracle_topo *t, oracle_conf *c, const double *vlvr);
 double oracle_enyshe(const oracle_topo *t, oracle_conf *c);
 /* enysol.f: returns eysl; fills e[7], e[8]. */
 double oracle_enysol(const oracle_topo *t, oracle_conf *c);
+/* esolan.f:12-1050, the energy part: analytic solvent accessible areas
+ * (projection method) of all atoms; returns sum sigma_i As_i, As (nat, may
+ * be NULL).  Random rotations draw from g; NAN on coinciding atoms. */
+double oracle_esolan(const oracle_topo *t, oracle_conf *c, oracle_mt *g, double *as_out);
+/* The generator oracle_energy gives esolan (itysol < 0); NULL: a draw gives NAN. */
+void oracle_set_solvent_mt(oracle_mt *g);
+/* Test hook: uga >= 0 forces the rotation of every atom with these numbers. */
+void oracle_esolan_force_rotation(double uga, double ufi);
 /* eninteract.f:14-86: pairs between chains; fills ei[], returns eysmi. */
 double oracle_eninteract(const oracle_topo *t, oracle_conf *c);
 /* energy.f: setvar + enyshe + solvent + eninteract; returns total. */

@@ -74,6 +74,10 @@ def _load():
     lib.oracle_set_trace.argtypes = [F64P, C.c_int, C.c_int]
     lib.oracle_trace_count.restype = C.c_int
     lib.oracle_muca_update.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, I32P, F64P, F64P, F64P, F64P]
+    lib.oracle_esolan.argtypes = [P(Topo), P(Conf), P(MT), F64P]
+    lib.oracle_esolan.restype = C.c_double
+    lib.oracle_set_solvent_mt.argtypes = [P(MT)]
+    lib.oracle_esolan_force_rotation.argtypes = [C.c_double, C.c_double]
     lib.oracle_difang.argtypes = [C.c_double, C.c_double]
     lib.oracle_difang.restype = C.c_double
     lib.oracle_addang.argtypes = [C.c_double, C.c_double]
@@ -109,8 +113,13 @@ class OracleMT:
 class OracleSystem:
     """The chains of a product context (smmp_amd._ffi.Context) for the oracle."""
 
-    def __init__(self, ctx):
+    def __init__(self, ctx, initial_torsions: bool = False):
+        """With initial_torsions the oracle starts from the torsions as the
+        molecule was set up (before any rebuild wraps them into [-pi, pi]),
+        the state the reference is in when its first energy() applies new
+        variables."""
         v = ctx.view
+        toat_setup = np.array(v("atm_r.toat"), dtype=np.float64)
         # a rebuild refreshes mol_i.ixrfpt (the first backbone atoms), which
         # init_molecule leaves from before the terminal groups were added
         vl0 = np.ascontiguousarray(v("var_r.vlvr"), dtype=np.float64)
@@ -148,6 +157,8 @@ class OracleSystem:
             "e0to": f64(v("epar_r.e0to")), "sgto": f64(v("epar_r.sgto")), "rnto": f64(v("epar_r.rnto")),
             "spoint": f64(v("tes.spoint")[:max(npnt, 1), :].T.ravel()),
         }
+        if initial_torsions:
+            self.keep["toat"] = f64(toat_setup[:nat])
         t = Topo()
         t.nat, t.nvt, t.nms, t.npnt = nat, nvt, nms, npnt
         for k, a in self.keep.items():
@@ -187,11 +198,28 @@ class OracleSystem:
     def reset(self):
         lib.oracle_conf_init(C.byref(self.t), C.byref(self.c))
 
-    def energy(self, vlvr_all) -> np.ndarray:
-        """[total, eysm, eyel, eyvw, eyhb, eyvr, eysl, asa, vdvol]"""
+    def energy(self, vlvr_all, mt: "OracleMT | None" = None) -> np.ndarray:
+        """[total, eysm, eyel, eyvw, eyhb, eyvr, eysl, asa, vdvol]; mt gives
+        the draws of the analytic solvent's random rotations (itysol < 0)."""
         vl = f64(vlvr_all)
-        lib.oracle_energy(C.byref(self.t), C.byref(self.c), vl.ctypes.data_as(F64P))
+        lib.oracle_set_solvent_mt(C.byref(mt.s) if mt is not None else None)
+        try:
+            lib.oracle_energy(C.byref(self.t), C.byref(self.c), vl.ctypes.data_as(F64P))
+        finally:
+            lib.oracle_set_solvent_mt(None)
         return np.array(self.c.e[:], dtype=np.float64)
+
+    def setvar(self, vlvr_all) -> None:
+        """Applies the variables and rebuilds (setvar.f), without an energy."""
+        vl = f64(vlvr_all)
+        lib.oracle_setvar(C.byref(self.t), C.byref(self.c), vl.ctypes.data_as(F64P))
+
+    def esolan_areas(self, mt: "OracleMT | None" = None) -> tuple[float, np.ndarray]:
+        """Analytic solvent term and per-atom areas of the current coordinates."""
+        a = np.zeros(self.nat)
+        e = lib.oracle_esolan(C.byref(self.t), C.byref(self.c), C.byref(mt.s) if mt is not None else None,
+                              a.ctypes.data_as(F64P))
+        return e, a
 
     def inter(self) -> np.ndarray:
         """[eysmi, eyeli, eyvwi, eyhbi] of the last energy"""
@@ -235,3 +263,32 @@ class OracleSystem:
                                temps.ctypes.data_as(F64P), nequi, nswp, switch, eol.ctypes.data_as(F64P),
                                ipoi.ctypes.data_as(I32P), coor.ctypes.data_as(F64P), traj.ctypes.data_as(F64P))
         return rc, eol, ipoi, coor.reshape(R, self.nvr), traj[:nswp * R].reshape(nswp, R)
+
+
+class Spheres:
+    """A bare set of spheres (centres, radii, solvation parameters) for the
+    oracle's analytic surface, without a molecule around it."""
+
+    def __init__(self, xyz, r, sigma=None):
+        xyz = np.asarray(xyz, dtype=np.float64)
+        self.n = len(xyz)
+        self.keep = [f64(xyz[:, 0]), f64(xyz[:, 1]), f64(xyz[:, 2]), f64(r),
+                     f64(np.ones(self.n) if sigma is None else sigma)]
+        self.t = Topo()
+        self.t.nat = self.n
+        self.t.rvdw = self.keep[3].ctypes.data_as(F64P)
+        self.t.sigma = self.keep[4].ctypes.data_as(F64P)
+        self.c = Conf()
+        self.c.xat, self.c.yat, self.c.zat = (a.ctypes.data_as(F64P) for a in self.keep[:3])
+
+    def esolan(self, mt: OracleMT | None = None) -> tuple[float, np.ndarray]:
+        a = np.zeros(self.n)
+        e = lib.oracle_esolan(C.byref(self.t), C.byref(self.c), C.byref(mt.s) if mt is not None else None,
+                              a.ctypes.data_as(F64P))
+        return e, a
+
+
+def force_rotation(uga: float, ufi: float) -> None:
+    """Makes the oracle's esolan rotate every atom by these numbers (u_gamma,
+    u_phi in [0, 1)) instead of drawing; negative values switch it off."""
+    lib.oracle_esolan_force_rotation(uga, ufi)

@@ -52,8 +52,15 @@ def parse(text: str) -> dict[str, list]:
     return out
 
 
+def _big_stack():
+    # esolan.f keeps arrays of mxat*(mxat+1) elements as locals
+    import resource
+    resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+
+
 def run(args: list[str], cwd: str) -> dict[str, list]:
-    res = subprocess.run([DRIVER] + args, cwd=cwd, capture_output=True, text=True)
+    res = subprocess.run([DRIVER] + args, cwd=cwd, capture_output=True, text=True,
+                         preexec_fn=_big_stack if args[0] == "esolan" else None)
     if res.returncode != 0:
         raise RuntimeError(f"ref_driver {' '.join(args)} failed:\n{res.stdout[-2000:]}\n{res.stderr[-2000:]}")
     return parse(res.stdout)
@@ -160,6 +167,74 @@ def multi_fixture(tag: str, case: dict, tmp: str) -> None:
     save(f"multi_{tag}.npz", **arrays)
 
 
+CDR = 1.745329251994330e-2  # degrees to radians as the reference's INCL.H has it
+
+
+def esolan_rotation_fixture(tmp: str) -> None:
+    """A Met-enkephalin conformation where one neighbour's surface passes
+    through an atom's north point (esolan.f:188-216 then draws a random
+    rotation): one torsion is bisected (on the CPU oracle's coordinates,
+    which equal the reference's) onto the root of |N_i - c_k|^2 - R_k^2;
+    the reference then evaluates the conformation (its variables in
+    radians, applied by the start energy's own rebuild) and two random ones
+    after it."""
+    sys.path.insert(0, os.path.dirname(HERE))
+    from conftest import make_ctx
+    from _oracle import OracleSystem
+
+    c = make_ctx(tag="enkefa", itysol=-1)
+    o = OracleSystem(c)
+    R = np.asarray(c.view("solvent.rvdw")[:o.nat], dtype=np.float64)
+    rng = np.random.default_rng(1)
+
+    def fmat(v):
+        o.energy(v)
+        x = o.coords()
+        north = x.copy()
+        north[:, 2] += R
+        d = np.sqrt(((x[:, None, :] - x[None, :, :]) ** 2).sum(-1))
+        nb = (d < R[:, None] + R[None, :]) & (d > np.abs(R[:, None] - R[None, :]))
+        np.fill_diagonal(nb, False)
+        return ((north[:, None, :] - x[None, :, :]) ** 2).sum(-1) - R[None, :] ** 2, nb
+
+    while True:
+        deg = o.vlvr0 / CDR
+        deg[o.idvr - 1] = rng.uniform(-180.0, 180.0, o.nvr)
+        j = o.idvr[rng.integers(o.nvr)] - 1
+
+        def at(dj):
+            dd = deg.copy()
+            dd[j] = dj
+            v = o.vlvr0.copy()
+            v[o.idvr - 1] = dd[o.idvr - 1] * CDR
+            return fmat(v)
+        fa, nba = at(deg[j])
+        fb, nbb = at(deg[j] + 10.0)
+        cand = np.argwhere(nba & nbb & (np.sign(fa) != np.sign(fb)))
+        if len(cand):
+            break
+    i, k = cand[0]
+    lo, hi, flo = deg[j], deg[j] + 10.0, fa[i, k]
+    for _ in range(80):
+        mid = 0.5 * (lo + hi)
+        fm = at(mid)[0][i, k]
+        if np.sign(fm) == np.sign(flo):
+            lo, flo = mid, fm
+        else:
+            hi = mid
+    deg[j] = lo
+    assert abs(at(lo)[0][i, k]) < 1e-9
+    rad = deg[o.idvr - 1] * CDR
+    vf = os.path.join(tmp, "esolan_rot.rad")
+    with open(vf, "w") as f:
+        for iv, x in zip(o.idvr, rad):
+            f.write(f"{iv} {float(x)!r}\n")
+    out = run(["esolan", LIBDIR, os.path.join(HERE, "enkefa.seq"), vf, "4357", "2"], tmp)
+    save("esolan_rotation_enkefa.npz", var_idx=o.idvr.astype(np.int64), var_rad=rad,
+         atoms=np.array([i + 1, k + 1]), seed=np.array([4357]), energy=np.stack(out["energy"]),
+         mti=np.concatenate(out["mti"]), xyz0=out["xyz"][0], conf_vlvr=np.stack(out["vlvr"]))
+
+
 def main(only: list[str]) -> None:
     if not os.path.exists(DRIVER):
         raise SystemExit("build the reference first: make -C oracle ref")
@@ -167,6 +242,19 @@ def main(only: list[str]) -> None:
     if not only or "multi" in only:
         for tag, case in MULTI_CASES.items():
             multi_fixture(tag, case, tmp)
+    if not only or "esolan" in only:
+        # the analytic solvent (itysol -1, esolan.f): start and random conformations
+        cases = {"enkefa": ([s.lower() for s in open(os.path.join(HERE, "enkefa.seq")).read().split()[2:]], 8, 4357),
+                 "c2_20mer": (workloads.random_sequence(20, 2), 6, 1234),
+                 "len10": (workloads.random_sequence(10, 10), 6, 99)}
+        for tag, (seq, nconf, seed) in cases.items():
+            seqf = workloads.write_seq_file(os.path.join(tmp, f"esolan_{tag}.seq"), seq, tag)
+            out = run(["esolan", LIBDIR, seqf, "-", str(seed), str(nconf)], tmp)
+            save(f"esolan_{tag}.npz", seq_in=np.array(seq, dtype="<U8"), seed=np.array([seed]),
+                 energy=np.stack(out["energy"]), mti=np.concatenate(out["mti"]), xyz0=out["xyz"][0],
+                 conf_vlvr=np.stack(out["vlvr"]))
+    if not only or "esolan" in only:
+        esolan_rotation_fixture(tmp)
     if only:
         return
 
