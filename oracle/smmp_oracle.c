@@ -717,6 +717,14 @@ void oracle_muca_update(int kmin, int kmax, double ebin, double beta, const int3
 
 /* ------------------------------------------------ esolan.f:12-1050 (energy) */
 
+/* Largest neighbour count and vertex count of the last oracle_esolan call
+ * (sizes the device's per-atom buffers in the tests). */
+static int g_es_max_nb = 0, g_es_max_iv = 0;
+void oracle_esolan_stats(int *max_nb, int *max_iv) {
+  *max_nb = g_es_max_nb;
+  *max_iv = g_es_max_iv;
+}
+
 /* Test hook: >= 0 forces the random rotation of esolan.f:200-232 for every
  * atom with these (u_gamma, u_phi) in [0, 1) instead of drawing them. */
 static double g_es_force_uga = -1.0, g_es_force_ufi = -1.0;
@@ -741,6 +749,8 @@ double oracle_esolan(const oracle_topo *t, oracle_conf *c, oracle_mt *g, double 
   double *ddat = (double *)malloc(sizeof(double) * 4 * (size_t)(numat + 1));
   double *neibor = (double *)malloc(sizeof(double) * 4 * (size_t)(numat + 1));
   double sss = 0.0;
+  g_es_max_nb = 0;
+  g_es_max_iv = 0;
 #define DD(k, m) ddat[4 * (k) + (m) - 1]
 #define NB(k, m) neibor[4 * (k) + (m) - 1]
   for (int i = 1; i <= numat; ++i) {
@@ -752,7 +762,7 @@ double oracle_esolan(const oracle_topo *t, oracle_conf *c, oracle_mt *g, double 
       if (j == i) continue;
       double ddp = (A(c->xat, j) - xi) * (A(c->xat, j) - xi) + (A(c->yat, j) - yi) * (A(c->yat, j) - yi) +
                    (A(c->zat, j) - zi) * (A(c->zat, j) - zi);
-      if (ddp < 1e-10) {
+      if (ddp < (double)1e-10f) { /* a default-real literal in esolan.f:127 */
         free(ddat);
         free(neibor);
         return NAN;
@@ -774,6 +784,7 @@ double oracle_esolan(const oracle_topo *t, oracle_conf *c, oracle_mt *g, double 
     }
     if (bad || jj == 0) goto atom_done;
     neib[0] = jj;
+    if (jj > g_es_max_nb) g_es_max_nb = jj;
     {
       const double R2 = R * R, R22 = R2 + R2, R42 = R22 + R22, R22p = R22 * PI;
       for (int q = 1; q <= neib[0]; ++q) {
@@ -928,6 +939,7 @@ double oracle_esolan(const oracle_topo *t, oracle_conf *c, oracle_mt *g, double 
           }
         }
       }
+      if (iv > g_es_max_iv) g_es_max_iv = iv;
       /* lines through the north point, inner circles (esolan.f:453-503) */
       int ita2 = 0, ita3 = 0;
       for (int q = 1; q <= neib[0]; ++q) {

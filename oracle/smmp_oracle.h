@@ -77,6 +77,8 @@ double oracle_enysol(const oracle_topo *t, oracle_conf *c);
  * (projection method) of all atoms; returns sum sigma_i As_i, As (nat, may
  * be NULL).  Random rotations draw from g; NAN on coinciding atoms. */
 double oracle_esolan(const oracle_topo *t, oracle_conf *c, oracle_mt *g, double *as_out);
+/* Largest neighbour and vertex counts of the last oracle_esolan call. */
+void oracle_esolan_stats(int *max_nb, int *max_iv);
 /* The generator oracle_energy gives esolan (itysol < 0); NULL: a draw gives NAN. */
 void oracle_set_solvent_mt(oracle_mt *g);
 /* Test hook: uga >= 0 forces the rotation of every atom with these numbers. */

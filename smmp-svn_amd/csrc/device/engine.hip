@@ -15,6 +15,7 @@
 #include "../host/engine.hpp"
 #include "common.hpp"
 #include "energy.hpp"
+#include "esolan_host.hpp"
 #include "mlp_host.hpp"
 #include "sampler.hpp"
 
@@ -38,8 +39,9 @@ __global__ void __launch_bounds__(kEnergyThreads) k_energy(const DevSys* __restr
                                                            const int32_t* __restrict__ conf_sys,
                                                            const int64_t* __restrict__ conf_off,
                                                            const double* __restrict__ vlvr, double* __restrict__ out,
-                                                           double* __restrict__ xyz_out, int32_t* __restrict__ err,
-                                                           double* __restrict__ work, int bits_lds) {
+                                                           double* __restrict__ xyz_out, int xyz_stride,
+                                                           int32_t* __restrict__ err, double* __restrict__ work,
+                                                           int bits_lds) {
   extern __shared__ double smem[];
   const int c = blockIdx.x;
   const DevSys& S = systems[conf_sys[c]];
@@ -67,11 +69,12 @@ __global__ void __launch_bounds__(kEnergyThreads) k_energy(const DevSys* __restr
     r[11] = o.hbi;
     if (o.err) atomicOr(err, 1);
   }
-  if (xyz_out) {
+  if (xyz_out) {  // conformation c at xyz_out + 3 * xyz_stride * c
+    double* xo = xyz_out + (size_t)3 * xyz_stride * c;
     for (int a = threadIdx.x; a < S.nat; a += blockDim.x) {
-      xyz_out[3 * a + 0] = L.x[a];
-      xyz_out[3 * a + 1] = L.y[a];
-      xyz_out[3 * a + 2] = L.z[a];
+      xo[3 * a + 0] = L.x[a];
+      xo[3 * a + 1] = L.y[a];
+      xo[3 * a + 2] = L.z[a];
     }
   }
   if (work) {  // block totals of the work counters (one row per conformation)
@@ -229,7 +232,7 @@ static void upload_system(DevSystem& ds, hipStream_t st) {
 
 // Kernel timing with HIP events recorded on the engine's stream around each
 // launch, plus the algorithmic work the kernels report (see Work).
-enum KernelId { kKSweep = 0, kKExchange = 1, kKEnergy = 2, kNKernels = 3 };
+enum KernelId { kKSweep = 0, kKExchange = 1, kKEnergy = 2, kKEsolan = 3, kNKernels = 4 };
 struct KernelStat {
   double ms = 0.0;
   int64_t launches = 0;
@@ -262,8 +265,10 @@ struct Engine {
   std::map<std::vector<int64_t>, int> tuned_groups;  // (system, R, precision, mode) -> G
   int32_t info[4] = {0, 0, 0, 0};                     // how the last sampler call ran
   MlpWork* mlp = nullptr;                             // surrogate tables and training workspace
+  EsolanWork* esolan = nullptr;                       // analytic solvent workspace
   ~Engine() {
     mlp_work_destroy(mlp);
+    esolan_work_destroy(esolan);
     if (gexec) (void)hipGraphExecDestroy(gexec);
     for (hipEvent_t ev : prof.pool) (void)hipEventDestroy(ev);
     for (auto& p : prof.pending) {
@@ -393,7 +398,8 @@ static void check_lds(size_t bytes) {
 
 // Launches k_energy for nconf conformations of the systems at d_systems.
 static void launch_energy(Engine& E, const DevSys* d_systems, const int32_t* d_cs, const int64_t* d_off,
-                          const double* d_vl, double* d_out, double* d_xyz, int nconf, LdsPlan plan, int precision) {
+                          const double* d_vl, double* d_out, double* d_xyz, int xyz_stride, int nconf, LdsPlan plan,
+                          int precision) {
   const size_t lds = plan.bytes;
   check_lds(lds);
   int32_t* d_err = static_cast<int32_t*>(E.d_err.ensure(sizeof(int32_t)));
@@ -404,11 +410,11 @@ static void launch_energy(Engine& E, const DevSys* d_systems, const int32_t* d_c
   if (precision == 0) {
     HIPCHK(hipFuncSetAttribute((const void*)k_energy<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(k_energy<false>, dim3(nconf), dim3(kEnergyThreads), lds, E.stream, d_systems, d_cs, d_off, d_vl,
-                       d_out, d_xyz, d_err, d_work, plan.bits_lds);
+                       d_out, d_xyz, xyz_stride, d_err, d_work, plan.bits_lds);
   } else {
     HIPCHK(hipFuncSetAttribute((const void*)k_energy<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(k_energy<true>, dim3(nconf), dim3(kEnergyThreads), lds, E.stream, d_systems, d_cs, d_off, d_vl,
-                       d_out, d_xyz, d_err, d_work, plan.bits_lds);
+                       d_out, d_xyz, xyz_stride, d_err, d_work, plan.bits_lds);
   }
   HIPCHK(hipGetLastError());
   prof_mark(E, kKEnergy, false);
@@ -488,6 +494,24 @@ static int read_err(Engine& E) {
 
 // Single energy of the context's current variables; updates the host state
 // (components, coordinates) like energy.f does for the COMMON blocks.
+// Analytic solvent (itysol < 0) of conformations whose coordinates k_energy
+// left at d_xyz: adds eysl to their rows of d_out, drawing rotations from
+// the twister of each conformation's context (esolan.f:199-200).
+static void add_esolan(Engine& E, const DevSys* d_systems, const std::vector<const HostSys*>& hsys,
+                       const std::vector<int32_t>& conf_sys, const int32_t* d_cs, int nconf, int stride,
+                       const double* d_xyz, double* d_out, const std::function<double(int)>& draw) {
+  esolan_energies(E.esolan, E.stream, d_systems, hsys, conf_sys, d_cs, nconf, stride, d_xyz, d_out, kEnergyOut, draw);
+  if (E.prof.on) {
+    const EsolanStats st = esolan_last_stats(E.esolan);
+    KernelStat& k = E.prof.stat[kKEsolan];
+    k.ms += st.ms;
+    k.launches += st.passes + 1;
+    k.work[0] += st.atoms;
+    k.work[1] += st.pairs;
+    k.work[2] += st.arcs;
+  }
+}
+
 static void energy_into_state(Engine& E, State& s) {
   DevSystem& ds = current_system(E, s);
   const int nvt = ds.h.nvt, nat = ds.h.nat;
@@ -501,8 +525,11 @@ static void energy_into_state(Engine& E, State& s) {
   HIPCHK(hipMemcpyAsync(d_vl, s.vlvr.data(), sizeof(double) * nvt, hipMemcpyHostToDevice, E.stream));
   HIPCHK(hipMemcpyAsync(d_cs, &zero, sizeof(zero), hipMemcpyHostToDevice, E.stream));
   HIPCHK(hipMemcpyAsync(d_off, &zoff, sizeof(zoff), hipMemcpyHostToDevice, E.stream));
-  launch_energy(E, static_cast<const DevSys*>(E.d_sys.p), d_cs, d_off, d_vl, d_out, d_xyz, 1,
+  launch_energy(E, static_cast<const DevSys*>(E.d_sys.p), d_cs, d_off, d_vl, d_out, d_xyz, nat, 1,
                 [&] { const DevSys* one = &ds.d; return plan_energy_lds(&one, 1, kEnergyThreads); }(), 0);
+  if (s.itysol < 0)
+    add_esolan(E, static_cast<const DevSys*>(E.d_sys.p), {&ds.h}, {0}, d_cs, 1, nat, d_xyz, d_out,
+               [&](int) { return host_grnd(s); });
   double o[kEnergyOut];
   std::vector<double> xyz(3 * (size_t)nat);
   HIPCHK(hipMemcpyAsync(o, d_out, sizeof o, hipMemcpyDeviceToHost, E.stream));
@@ -537,6 +564,7 @@ double engine_energy(Engine*& e, State& s) {
 
 double engine_enysol(Engine*& e, State& s) {
   Engine& E = engine_of(e);
+  if (s.itysol < 0) throw Error("enysol> the context uses the analytic solvent (itysol < 0): call energy()");
   energy_into_state(E, s);
   return s.eysl;
 }
@@ -546,13 +574,21 @@ int engine_energy_batch(Engine*& e, std::vector<State*>& sys, const int32_t* con
   Engine& E = engine_of(e);
   if (nconf == 0) return SMMP_OK;
   const int nsys = (int)sys.size();
+  const bool analytic = sys[0]->itysol < 0;
+  for (int i = 1; i < nsys; ++i)
+    if ((sys[i]->itysol < 0) != analytic)
+      throw Error("energy_batch> systems with the analytic solvent (itysol < 0) and others in one batch");
   E.batch_sys.clear();
   std::vector<DevSys> dsys(nsys);
+  std::vector<const HostSys*> hsys(nsys);
+  int max_nat = 0;
   for (int i = 0; i < nsys; ++i) {
     auto p = std::make_unique<DevSystem>();
     p->h = build_host_sys(*sys[i]);
     upload_system(*p, E.stream);
     dsys[i] = p->d;
+    hsys[i] = &p->h;
+    max_nat = std::max(max_nat, p->h.nat);
     E.batch_sys.push_back(std::move(p));
   }
   std::vector<int64_t> off(nconf);
@@ -573,8 +609,16 @@ int engine_energy_batch(Engine*& e, std::vector<State*>& sys, const int32_t* con
   HIPCHK(hipMemcpyAsync(d_vl, vlvr, sizeof(double) * tot, hipMemcpyHostToDevice, E.stream));
   std::vector<const DevSys*> host_sys(nsys);
   for (int i = 0; i < nsys; ++i) host_sys[i] = &dsys[i];
-  launch_energy(E, d_systems, d_cs, d_off, d_vl, d_out, nullptr, nconf,
+  double* d_xyz = analytic ? static_cast<double*>(E.d_xyz.ensure(sizeof(double) * 3 * max_nat * (size_t)nconf))
+                           : nullptr;
+  launch_energy(E, d_systems, d_cs, d_off, d_vl, d_out, d_xyz, max_nat, nconf,
                 plan_energy_lds(host_sys.data(), nsys, kEnergyThreads), precision);
+  if (analytic) {
+    // rotations draw from each conformation's context, conformations in order
+    std::vector<int32_t> cs(conf_sys, conf_sys + nconf);
+    add_esolan(E, d_systems, hsys, cs, d_cs, nconf, max_nat, d_xyz, d_out,
+               [&](int c) { return host_grnd(*sys[conf_sys[c]]); });
+  }
   std::vector<double> full((size_t)kEnergyOut * nconf);
   HIPCHK(hipMemcpyAsync(full.data(), d_out, sizeof(double) * full.size(), hipMemcpyDeviceToHost, E.stream));
   HIPCHK(hipStreamSynchronize(E.stream));

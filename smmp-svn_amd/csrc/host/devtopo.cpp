@@ -49,7 +49,7 @@ HostSys build_host_sys(const State& s) {
     h.toat0[i] = s.toat(a);
     h.rvdw[i] = s.rvdw(a);
     h.sigma[i] = s.sigma(a);
-    if (s.itysol != 0 && s.rvdw(a) > 0.0) h.heavy.push_back((int16_t)i);
+    if (s.itysol > 0 && s.rvdw(a) > 0.0) h.heavy.push_back((int16_t)i);  // the numerical solvent's atoms
   }
   h.nheavy = (int)h.heavy.size();
   h.heavy_of.assign(nat, (int16_t)-1);
@@ -192,7 +192,6 @@ HostSys build_host_sys(const State& s) {
   h.slp = s.slp;
   h.epsd = s.epsd;
   h.itysol = s.itysol;
-  if (s.itysol < 0) throw Error("device engine: the analytic solvent (itysol < 0) is not implemented");
   h.npnt = s.itysol > 0 ? s.npnt : 0;
   if (h.npnt > 128) throw Error("device engine: tessellations of more than 128 points are not supported");
   h.spoint.resize((size_t)h.npnt * 3);

@@ -118,6 +118,7 @@ class OracleSystem:
         molecule was set up (before any rebuild wraps them into [-pi, pi]),
         the state the reference is in when its first energy() applies new
         variables."""
+        self.ctx = ctx  # the arrays below may be views of the context's memory
         v = ctx.view
         toat_setup = np.array(v("atm_r.toat"), dtype=np.float64)
         # a rebuild refreshes mol_i.ixrfpt (the first backbone atoms), which

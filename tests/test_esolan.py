@@ -144,3 +144,19 @@ def test_rotation_leaves_the_areas(u):
     assert np.allclose(a1, a0, rtol=1e-9, atol=1e-9)
     assert np.allclose(b1, b0, rtol=1e-9, atol=1e-9)
     assert abs(e1 - e0) < 1e-8 * max(1.0, abs(e0))
+
+
+def test_samplers_reject_the_analytic_solvent():
+    """The samplers run the numerical solvent only; with itysol < 0 they say
+    so before touching the device (energies take the analytic term)."""
+    from smmp_amd import _ffi
+    c = make_ctx(tag="enkefa", itysol=-1)
+    temps = np.array([300.0, 400.0])
+    with pytest.raises(_ffi.SmmpError, match="analytic solvent"):
+        c.sample(temps, np.array([1, 2, 3], dtype=np.uint32), 2, np.tile(c.vlvr(), (2, 1)))
+    with pytest.raises(_ffi.SmmpError, match="analytic solvent"):
+        c.partem_s(temps, 1, 1)
+    w = _ffi.Weight()
+    w.kind, w.beta = 0, 1.0 / (1.98773e-3 * 300.0)
+    with pytest.raises(_ffi.SmmpError, match="analytic solvent"):
+        c.metropolis(0.0, 0.0, w)

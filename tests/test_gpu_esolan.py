@@ -1,0 +1,142 @@
+"""The analytic solvent term (itysol < 0, esolan.f) on the GPU through the
+C-ABI: smmp_energy against the Fortran reference's own outputs (start and
+random conformations of three sequences, with the twister's position after
+each energy, and a conformation whose degenerate north point draws a random
+rotation from the context's twister), and smmp_energy_batch against the CPU
+oracle on many conformations of one and of two systems."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import golden, make_ctx
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__))))
+from _oracle import OracleMT, OracleSystem, lib as olib  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+RTOL = 1e-9  # FP64: areas from the device's atan2/sin/cos, sums in wave order
+
+
+def say(*a):
+    print("[gpu-esolan]", *a, flush=True)
+
+
+def _ctx(tag, d):
+    if tag == "enkefa":
+        return make_ctx(tag="enkefa", itysol=-1)
+    return make_ctx(seq=list(d["seq_in"]), itysol=-1)
+
+
+def _components(c):
+    v = c.view
+    return np.array([float(v(f"eny_r.{k}")[0]) for k in ("eysm", "eyel", "eyvw", "eyhb", "eyvr", "eysl")])
+
+
+def _close(a, b, rtol=RTOL):
+    return abs(a - b) <= rtol * max(1.0, abs(b))
+
+
+@pytest.mark.parametrize("tag", ["enkefa", "c2_20mer", "len10"])
+def test_energy_matches_fortran(tag):
+    d = golden(f"esolan_{tag}.npz")
+    c = _ctx(tag, d)
+    c.sgrnd(int(d["seed"][0]))
+    say(f"smmp_energy {tag} start")
+    e = c.energy()
+    comp = _components(c)
+    ref = d["energy"][0]
+    assert _close(comp[5], ref[6]), (comp[5], ref[6])
+    assert _close(e, ref[0]), (e, ref[0])
+    assert int(c.view("myblock.mti")[0]) == d["mti"][0]
+    idvr = c.idvr - 1
+    ax = np.asarray(c.view("var_r.axvr")[:c.nvt], dtype=np.float64)
+    for k, want in enumerate(d["conf_vlvr"]):
+        v = c.vlvr()
+        for j in idvr:
+            v[j] = olib.oracle_addang(np.pi, ax[j] * (c.grnd() - 0.5))
+        assert np.array_equal(v[idvr], want)
+        c.set_vlvr(v)
+        say(f"smmp_energy {tag} conformation {k}")
+        e = c.energy()
+        comp = _components(c)
+        ref = d["energy"][k + 1]
+        assert _close(comp[5], ref[6]), (k, comp[5], ref[6])
+        assert _close(e, ref[0]), (k, e, ref[0])
+        assert int(c.view("myblock.mti")[0]) == d["mti"][k + 1]
+
+
+def test_degenerate_north_point_draws_like_fortran():
+    d = golden("esolan_rotation_enkefa.npz")
+    c = make_ctx(tag="enkefa", itysol=-1)
+    v = c.vlvr()
+    v[d["var_idx"] - 1] = d["var_rad"]
+    c.set_vlvr(v)
+    c.sgrnd(int(d["seed"][0]))
+    say("smmp_energy enkefa, a neighbour's surface through an atom's north point")
+    e = c.energy()
+    assert int(c.view("myblock.mti")[0]) == 2  # one rotation: u_gamma, u_phi
+    comp = _components(c)
+    assert _close(comp[5], d["energy"][0][6]), (comp[5], d["energy"][0][6])
+    assert _close(e, d["energy"][0][0])
+
+
+def _random_confs(c, n, seed):
+    rng = np.random.default_rng(seed)
+    v = np.tile(c.vlvr(), (n, 1))
+    v[:, c.idvr - 1] = rng.uniform(-np.pi, np.pi, size=(n, c.nvr))
+    return v
+
+
+def test_energy_batch_matches_oracle():
+    from smmp_amd import _ffi
+    d = golden("esolan_c2_20mer.npz")
+    c = make_ctx(seq=list(d["seq_in"]), itysol=-1)
+    o = OracleSystem(c)
+    V = _random_confs(c, 48, 7)
+    c.sgrnd(11)
+    c.profile(True)
+    say("smmp_energy_batch c2_20mer 48 conformations")
+    out = _ffi.energy_batch([c], np.zeros(len(V), dtype=np.int32), V.ravel(), precision=0)
+    st = c.profile_read(3)
+    say(f"esolan kernels {st['ms']:.2f} ms for {st['pairs']:.0f} atoms, launches {st['launches']}")
+    mt = OracleMT(11)
+    for k in range(len(V)):
+        ref = o.energy(V[k], mt)
+        assert _close(out[k, 6], ref[6]), (k, out[k, 6], ref[6])
+        assert _close(out[k, 0], ref[0]), (k, out[k, 0], ref[0])
+    assert int(c.view("myblock.mti")[0]) == mt.s.mti
+
+
+def test_energy_batch_two_systems():
+    from smmp_amd import _ffi
+    a = make_ctx(tag="enkefa", itysol=-1)
+    d = golden("esolan_len10.npz")
+    b = make_ctx(seq=list(d["seq_in"]), itysol=-1)
+    Va, Vb = _random_confs(a, 5, 1), _random_confs(b, 7, 2)
+    cs = np.array([0, 1, 1, 0, 1, 0, 0, 1, 1, 0, 1, 1], dtype=np.int32)
+    flat, ia, ib = [], 0, 0
+    for s in cs:
+        if s == 0:
+            flat.append(Va[ia]); ia += 1
+        else:
+            flat.append(Vb[ib]); ib += 1
+    say("smmp_energy_batch enkefa + len10, 12 conformations")
+    out = _ffi.energy_batch([a, b], cs, np.concatenate(flat), precision=0)
+    oa, ob = OracleSystem(a), OracleSystem(b)
+    for k, s in enumerate(cs):
+        ref = (oa if s == 0 else ob).energy(flat[k], OracleMT(1))
+        assert _close(out[k, 6], ref[6]), (k, out[k, 6], ref[6])
+        assert _close(out[k, 0], ref[0]), (k, out[k, 0], ref[0])
+
+
+def test_mixed_solvent_batch_and_samplers_are_rejected():
+    from smmp_amd import _ffi
+    a = make_ctx(tag="enkefa", itysol=-1)
+    b = make_ctx(tag="enkefa", itysol=1)
+    V = np.tile(a.vlvr(), (2, 1))
+    with pytest.raises(_ffi.SmmpError, match="analytic solvent"):
+        _ffi.energy_batch([a, b], np.array([0, 1], dtype=np.int32), V.ravel())
+    with pytest.raises(_ffi.SmmpError, match="analytic solvent"):
+        a.enysol()
