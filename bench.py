@@ -38,6 +38,11 @@ FLOP_PAIR = 22      # ECEPP pair term: 3 sub + 5 r^2 + 1 rsqrt + 3 powers + 2 Co
 FLOP_POINT = 9      # one tessellation point against one solvent neighbour
 FLOP_BUILD = 60     # one atom placed by the Eyring step
 FLOP_ROTATE = 21    # one atom rotated rigidly about a bond (Rodrigues)
+# the analytic solvent (esolan.f), per unit of its kernel's work counters:
+FLOP_ES_SCAN = 9    # one atom pair of the neighbour search: 3 sub, 3 mul, 2 add, 1 compare
+FLOP_ES_PAIR = 48   # one pair of circles classified: 6 div, 21 discriminant, 3 sqrt with their 12, 6 tests
+FLOP_ES_ARC = 30    # one arc: midpoint (cos, sin, 6) and, when exposed, its term (2 cos, 2 sin, atan, 15)
+FLOP_ES_COVER = 7   # one circle tested against an arc midpoint: 3 mul, 3 add, 1 compare
 # HBM-side bytes per sweep of this workload in FP64: FETCH_SIZE x 2.0 plus
 # WRITE_SIZE x 1.0 (the factors make a 256 MiB device copy read 256 MiB and
 # write 256 MiB) of the timed launch, rocprofv3 --pmc, 1000 sweeps:
@@ -49,13 +54,13 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def make_system(tmp, length=LENGTH, seed=SEQ_SEED, tag="c2"):
+def make_system(tmp, length=LENGTH, seed=SEQ_SEED, tag="c2", itysol=1):
     from smmp_amd import _ffi, workloads
     seq = workloads.random_sequence(length, seed)
     seqf = workloads.write_seq_file(os.path.join(tmp, f"{tag}.seq"), seq, tag)
     c = _ffi.Context()
     for k, v in (("epar_l.flex", 0), ("epar_l.sh2", 0), ("epar_l.epsd", 0), ("epar_l.ientyp", 0),
-                 ("isolty.itysol", 1)):
+                 ("isolty.itysol", itysol)):
         c.view(k)[0] = v
     c.init_energy(workloads.default_libdir())
     c.init_molecule(1, "nh2", "cooh", seqf, " ")
@@ -212,6 +217,86 @@ def run_da(args):
     print(json.dumps(line), flush=True)
 
 
+def run_esolan(args):
+    """Energies with the analytic solvent (itysol -1, esolan.f) of conformations
+    sampled at the parallel-tempering workload (64 replicas of the random
+    20-residue sequence, seed 2, 250-1000 K): --es-warmup sweeps from random
+    dihedrals with OONS, then one snapshot of every replica per sweep until
+    --es-confs conformations; each timed step is one smmp_energy_batch call
+    over all of them (ECEPP/3 + analytic solvent, FP64)."""
+    import torch
+    from smmp_amd import _ffi, workloads
+    tmp = tempfile.mkdtemp(prefix="smmp_es_")
+    c, seq, seqf = make_system(tmp)
+    ca, _, _ = make_system(tmp, itysol=-1, tag="c2a")
+    temps = workloads.geometric_temperatures(NREP, TMIN, TMAX)
+    nsnap = max(1, args.es_confs // NREP)
+    log(f"analytic solvent: sampling {NREP} replicas, {args.es_warmup} + {nsnap} sweeps")
+    w = c.sample(temps, np.arange(1, NREP + 2, dtype=np.uint32), args.es_warmup, random_starts(c, NREP, 12345),
+                 mode=1, with_traj=False)
+    d = c.sample(temps, np.arange(500, 500 + NREP + 1, dtype=np.uint32), nsnap, w["vlvr"], mode=1, with_traj=False,
+                 snap_every=1)
+    V = np.ascontiguousarray(d["snap_vlvr"].reshape(-1, c.nvt))
+    n = len(V)
+    cs = np.zeros(n, dtype=np.int32)
+    ca.sgrnd(4357)
+    _ffi.energy_batch([ca], cs, V.ravel())  # untimed: device buffers and the intersection capacity
+    ca.profile(True)
+    torch.cuda.synchronize()
+    K = args.steps
+    log(f"timed {K} batches of {n} conformations")
+    t0 = time.perf_counter()
+    for _ in range(K):
+        out = _ffi.energy_batch([ca], cs, V.ravel())
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ke = ca.profile_read(_ffi.KERNEL_ENERGY)
+    ks = ca.profile_read(_ffi.KERNEL_ESOLAN)
+    wk = list(ks["work"])  # atom pairs scanned, circle pairs, arcs, coverage tests
+    flops = FLOP_ES_SCAN * wk[0] + FLOP_ES_PAIR * wk[1] + FLOP_ES_ARC * wk[2] + FLOP_ES_COVER * wk[3]
+    achieved = flops / (ks["ms"] * 1e-3) / 1e12
+    log(f"esolan kernels {ks['ms'] / K:.2f} ms per batch ({ks['launches'] / K:.1f} launches), k_energy "
+        f"{ke['ms'] / K:.2f} ms; per conformation: scan {wk[0] / K / n:.0f}, circle pairs {wk[1] / K / n:.0f}, "
+        f"arcs {wk[2] / K / n:.0f}, coverage tests {wk[3] / K / n:.0f}")
+    log(f"eysl mean {out[:, 6].mean():.3f}, total mean {out[:, 0].mean():.3f}")
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(REPO, "tests"))
+        from _oracle import OracleMT, OracleSystem
+        o = OracleSystem(ca)
+        mt = OracleMT(4357)
+        m, t1 = 0, time.perf_counter()
+        while m < n and time.perf_counter() - t1 < 10.0:
+            o.energy(V[m], mt)
+            m += 1
+        tc = time.perf_counter() - t1
+        cpu = {"value": round(m / tc, 3), "unit": "energy evaluations/s", "cores": 1, "kind": "port",
+               "sample": f"the first {m} of the same conformations through the C oracle (ECEPP/3 + esolan "
+                         f"restatement), one host core, {tc:.1f} s"}
+    line = {
+        "metric": "energy evaluations/s (ECEPP/3 + analytic solvent)",
+        "value": round(K * n / dt, 2),
+        "unit": "energy evaluations/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": 1,
+        "ms_per_step": round(1e3 * dt / K, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: random 20-residue sequence (seed 2); conformations sampled by parallel tempering",
+        "config": {"workload": f"smmp_energy_batch of {n} conformations of the {LENGTH}-residue sequence sampled at "
+                               f"{NREP} temperatures {TMIN:g}-{TMAX:g} K, ECEPP/3 + analytic solvent (itysol -1)",
+                   "conformations": n, "atoms": int(ca.view("res_i.iatrs2")[LENGTH - 1])},
+        "roofline": {"bound": "valu-fp64", "achieved": round(achieved, 4), "peak": PEAK_FP64, "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK_FP64, 5), "traffic": None,
+                     "kernel": "k_esolan (both passes) + k_esolan_sum"},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -234,6 +319,11 @@ def main():
     ap.add_argument("--da-epochs", type=int, default=8, help="epochs per round")
     ap.add_argument("--da-train", choices=("energy", "moves", "both"), default="moves",
                     help="fit total energies, exact changes of single moves, or energies then moves")
+    ap.add_argument("--esolan", action="store_true",
+                    help="energies with the analytic solvent (itysol -1) of sampled conformations; "
+                         "--steps sets the timed batches")
+    ap.add_argument("--es-warmup", type=int, default=200, help="sampling sweeps before the snapshots")
+    ap.add_argument("--es-confs", type=int, default=1024, help="conformations per batch")
     ap.add_argument("--phases", action="store_true",
                     help="also time the sweep kernel's phases on the device clock (adds barriers)")
     args = ap.parse_args()
@@ -247,6 +337,9 @@ def main():
     torch.cuda.set_device(0)
     if args.da:
         run_da(args)
+        return
+    if args.esolan:
+        run_esolan(args)
         return
     if args.copy_probe:
         x = torch.ones(args.copy_probe << 18, dtype=torch.float32, device="cuda")

@@ -209,8 +209,9 @@ const char *smmp_build_info(void);
  * 1 = exchange, 2 = energy, 3 = analytic solvent (itysol < 0).  work[4]:
  * pair-term evaluations, solvent contacts (each tested against all
  * tessellation points), atoms placed by chain builds, atoms rotated by trial
- * moves, summed over the launches; for kernel 3: atoms analysed, circle
- * pairs classified, arcs tested against the other circles. */
+ * moves, summed over the launches; for kernel 3: atom pairs scanned for
+ * neighbours, circle pairs classified, arcs tested, circles tested against
+ * arc midpoints. */
 int smmp_profile(smmp_ctx *ctx, int enable);
 int smmp_profile_read(smmp_ctx *ctx, int kernel, double *total_ms, int64_t *launches, double *work);
 /* With smmp_profile(ctx, 2) the sweep kernel also times its phases on the

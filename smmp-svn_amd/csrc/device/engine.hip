@@ -506,9 +506,7 @@ static void add_esolan(Engine& E, const DevSys* d_systems, const std::vector<con
     KernelStat& k = E.prof.stat[kKEsolan];
     k.ms += st.ms;
     k.launches += st.passes + 1;
-    k.work[0] += st.atoms;
-    k.work[1] += st.pairs;
-    k.work[2] += st.arcs;
+    for (int w = 0; w < 4; ++w) k.work[w] += st.work[w];
   }
 }
 

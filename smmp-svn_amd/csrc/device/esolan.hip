@@ -90,7 +90,7 @@ struct EsParams {
   char* scratch;         // per wavefront slot: wave_bytes
   size_t wave_bytes;
   int cap;               // intersection points per atom
-  double* counters;      // atoms, pairs, arcs
+  double* counters;      // atom pairs scanned, circle pairs classified, arcs tested, coverage tests
 };
 
 struct EsScratch {
@@ -178,7 +178,7 @@ __device__ __forceinline__ int es_classify(const EsLds& L, int K, int Lc, double
 // One atom of one conformation.  Returns false if the item was left for the
 // rotation launch.
 __device__ void es_item(const EsParams& P, EsLds& L, const EsScratch& X, int item, const double* rot, int lane,
-                        double& n_pairs, double& n_arcs) {
+                        double* work) {
   const int c = item / P.stride, i = item - c * P.stride;
   const DevSys& S = P.systems[P.conf_sys[c]];
   if (i >= S.nat) {
@@ -195,6 +195,7 @@ __device__ void es_item(const EsParams& P, EsLds& L, const EsScratch& X, int ite
   bool enclosed = false;
   for (int base = 0; base < S.nat; base += kWave) {
     const int j = base + lane;
+    work[0] += (j < S.nat && j != i) ? 1.0 : 0.0;
     bool coin = false, encl = false, nbr = false;
     if (j < S.nat && j != i) {
       const double dx = xyz[3 * j] - xi, dy = xyz[3 * j + 1] - yi, dz = xyz[3 * j + 2] - zi;
@@ -297,7 +298,7 @@ __device__ void es_item(const EsParams& P, EsLds& L, const EsScratch& X, int ite
         Lc = L.lst[lp - 1];
         code = es_classify(L, K, Lc, v);
       }
-      n_pairs += valid ? 1.0 : 0.0;
+      work[1] += valid ? 1.0 : 0.0;
       const uint64_t bm = __ballot(code >= kDropK);
       const int first = bm ? __ffsll((unsigned long long)bm) - 1 : kWave;
       const bool live = valid && lane < first;
@@ -469,11 +470,13 @@ __device__ void es_item(const EsParams& P, EsLds& L, const EsScratch& X, int ite
     const double ap1 = -ba + rr * cos(prob), ap2 = -ca + rr * sin(prob);
     const double q2 = ap1 * ap1 + ap2 * ap2;
     bool cov = false;
-    for (int q = 0; q < n && !cov; ++q) {
+    int q = 0;
+    for (; q < n && !cov; ++q) {
       const int t = L.lst[q];
       if (t != s && L.A[t] * q2 + L.B[t] * ap1 + L.C[t] * ap2 + L.D[t] < 0.0) cov = true;
     }
-    n_arcs += 1.0;
+    work[2] += 1.0;
+    work[3] += (double)q;
     if (cov) continue;
     const double hm = 5e-1 * (bet - alp), hp = 5e-1 * (bet + alp);
     const double dcbetmalp = cos(hm), dcbetpalp = cos(hp), dsbetmalp = sin(hm), dsbetpalp = sin(hp);
@@ -490,24 +493,19 @@ __global__ void __launch_bounds__(kEsThreads) k_esolan(EsParams P) {
   EsLds& L = reinterpret_cast<EsLds*>(es_smem)[wave];
   const size_t slot = (size_t)blockIdx.x * kEsWaves + wave;
   const EsScratch X = es_carve(P.scratch + slot * P.wave_bytes, P.cap);
-  double n_atoms = 0.0, n_pairs = 0.0, n_arcs = 0.0;
+  double n[4] = {0.0, 0.0, 0.0, 0.0};
   for (;;) {
     int idx = 0;
     if (lane == 0) idx = atomicAdd(P.next, 1);
     idx = __shfl(idx, 0, kWave);
     if (idx >= P.nitems) break;
     const int item = P.items ? P.items[idx] : idx;
-    es_item(P, L, X, item, P.items ? P.rot + 4 * (size_t)idx : nullptr, lane, n_pairs, n_arcs);
-    n_atoms += 1.0;
+    es_item(P, L, X, item, P.items ? P.rot + 4 * (size_t)idx : nullptr, lane, n);
     wave_lds_sync();
   }
-  n_pairs = wave_sum(n_pairs);
-  n_arcs = wave_sum(n_arcs);
-  if (lane == 0 && P.counters) {
-    atomicAdd(&P.counters[0], n_atoms);
-    atomicAdd(&P.counters[1], n_pairs);
-    atomicAdd(&P.counters[2], n_arcs);
-  }
+  for (int k = 0; k < 4; ++k) n[k] = wave_sum(n[k]);
+  if (lane == 0 && P.counters)
+    for (int k = 0; k < 4; ++k) atomicAdd(&P.counters[k], n[k]);
 }
 
 // eysl of each conformation (one wavefront per conformation) and the total
@@ -619,11 +617,11 @@ void esolan_energies(EsolanWork*& Wp, hipStream_t st, const DevSys* d_systems, c
   const int nitems = nconf * stride;
   int32_t* d_flag = static_cast<int32_t*>(W.flag.ensure(sizeof(int32_t) * nitems));
   double* d_asig = static_cast<double*>(W.asig.ensure(sizeof(double) * nitems));
-  char* small = static_cast<char*>(W.small.ensure(64));
+  char* small = static_cast<char*>(W.small.ensure(96));
   int32_t* d_small = reinterpret_cast<int32_t*>(small);
   double* d_counters = reinterpret_cast<double*>(small + 32);
   HIPCHK(hipMemsetAsync(d_flag, 0, sizeof(int32_t) * nitems, st));
-  HIPCHK(hipMemsetAsync(d_counters, 0, 3 * sizeof(double), st));
+  HIPCHK(hipMemsetAsync(d_counters, 0, 4 * sizeof(double), st));
   HIPCHK(hipEventRecord(W.ev0, st));
 
   EsParams P{};
@@ -703,15 +701,13 @@ void esolan_energies(EsolanWork*& Wp, hipStream_t st, const DevSys* d_systems, c
                      nconf, stride, d_out, out_stride);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(W.ev1, st));
-  double cnt[3];
+  double cnt[4];
   HIPCHK(hipMemcpyAsync(cnt, d_counters, sizeof cnt, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   float ms = 0.f;
   HIPCHK(hipEventElapsedTime(&ms, W.ev0, W.ev1));
   W.last.ms = ms;
-  W.last.atoms = cnt[0];
-  W.last.pairs = cnt[1];
-  W.last.arcs = cnt[2];
+  for (int k = 0; k < 4; ++k) W.last.work[k] = cnt[k];
 }
 
 }  // namespace smmp

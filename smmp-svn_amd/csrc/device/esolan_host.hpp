@@ -28,10 +28,12 @@ void esolan_energies(EsolanWork*& W, hipStream_t st, const DevSys* d_systems, co
                      const double* d_xyz, double* d_out, int out_stride, const std::function<double(int)>& draw);
 
 // Kernel time of the last esolan_energies call (ms, both passes and the
-// sums) and its work counters: atoms analysed, circle pairs classified,
-// arcs tested (sums over all conformations).
+// sums) and its work counters, summed over all conformations: atom pairs
+// scanned for neighbours, circle pairs classified, arcs tested, circles
+// tested against arc midpoints.
 struct EsolanStats {
-  double ms = 0.0, atoms = 0.0, pairs = 0.0, arcs = 0.0;
+  double ms = 0.0;
+  double work[4] = {0.0, 0.0, 0.0, 0.0};
   int passes = 0;
 };
 EsolanStats esolan_last_stats(EsolanWork* W);

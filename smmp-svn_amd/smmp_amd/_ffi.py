@@ -98,7 +98,7 @@ EXPORTS = ("smmp_ctx_create", "smmp_ctx_destroy", "smmp_last_error", "smmp_state
            "smmp_profile_phases", "smmp_muca_update", "smmp_muca_final", "smmp_sampler_info",
            "smmp_mlp_size", "smmp_mlp_set", "smmp_mlp_get", "smmp_mlp_grad", "smmp_mlp_fit")
 
-KERNEL_SWEEP, KERNEL_EXCHANGE, KERNEL_ENERGY = 0, 1, 2
+KERNEL_SWEEP, KERNEL_EXCHANGE, KERNEL_ENERGY, KERNEL_ESOLAN = 0, 1, 2, 3
 
 
 def dptr(a: np.ndarray):
@@ -234,7 +234,7 @@ class Context:
         work = np.zeros(4)
         self.check(lib.smmp_profile_read(self.h, kernel, C.byref(ms), C.byref(n), dptr(work)))
         return {"ms": ms.value, "launches": n.value, "pairs": work[0], "contacts": work[1],
-                "builds": work[2], "moved": work[3]}
+                "builds": work[2], "moved": work[3], "work": work.copy()}
 
     def sgrnd(self, seed: int) -> None:
         self.check(lib.smmp_sgrnd(self.h, int(np.int32(np.uint32(seed & 0xffffffff)))))
