@@ -519,6 +519,8 @@ int oracle_metropolis(const oracle_topo *t, oracle_conf *c, oracle_mt *g, double
                       int nvr, const int32_t *idvr, const double *axvr,
                       const oracle_weight *w, double *eol1, double *acz) {
   double *vrol = (double *)malloc(sizeof(double) * (size_t)t->nvt);
+  oracle_mt *sol_prev = g_sol_mt;
+  g_sol_mt = g; /* esolan's rotations draw from the trials' twister */
   double eol = oracle_energy(t, c, vlvr);
   for (int nsw = 1; nsw <= nvr; ++nsw) {
     int jv = idvr[nsw - 1];
@@ -591,6 +593,7 @@ int oracle_metropolis(const oracle_topo *t, oracle_conf *c, oracle_mt *g, double
   int bad = fabs(eol - enw) > 0.000001 && eol < 100000;
   *eol1 = eol;
   free(vrol);
+  g_sol_mt = sol_prev;
   return bad;
 }
 
@@ -624,6 +627,8 @@ int oracle_partem(const oracle_topo *t, oracle_mt *g, const double *vlvr0, int n
   oracle_weight w = {0};
   int bad = 0;
   double acz = 0.0;
+  oracle_mt *sol_prev = g_sol_mt;
+  g_sol_mt = g;
   for (int k = 0; k < nrep; ++k) {
     g_cur_rep = k;
     w.beta = pbe[k];
@@ -674,6 +679,7 @@ int oracle_partem(const oracle_topo *t, oracle_mt *g, const double *vlvr0, int n
   free(vlvr);
   free(pbe);
   oracle_conf_free(&c);
+  g_sol_mt = sol_prev;
   return bad;
 }
 

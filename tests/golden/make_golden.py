@@ -59,8 +59,7 @@ def _big_stack():
 
 
 def run(args: list[str], cwd: str) -> dict[str, list]:
-    res = subprocess.run([DRIVER] + args, cwd=cwd, capture_output=True, text=True,
-                         preexec_fn=_big_stack if args[0] == "esolan" else None)
+    res = subprocess.run([DRIVER] + args, cwd=cwd, capture_output=True, text=True, preexec_fn=_big_stack)
     if res.returncode != 0:
         raise RuntimeError(f"ref_driver {' '.join(args)} failed:\n{res.stdout[-2000:]}\n{res.stderr[-2000:]}")
     return parse(res.stdout)
@@ -235,6 +234,35 @@ def esolan_rotation_fixture(tmp: str) -> None:
          mti=np.concatenate(out["mti"]), xyz0=out["xyz"][0], conf_vlvr=np.stack(out["vlvr"]))
 
 
+def esolan_mc_fixtures(tmp: str) -> None:
+    """Metropolis (canon) and serial parallel tempering (partem_s) with the
+    analytic solvent, itysol -1: every trial's energy() runs esolan, whose
+    random rotations share the trials' twister."""
+    enk = [s.lower() for s in open(os.path.join(HERE, "enkefa.seq")).read().split()[2:]]
+    seqf = os.path.join(HERE, "enkefa.seq")
+    t_enk = first(run(["topo", LIBDIR, seqf, "-", "1"], tmp))
+    vf = vars_file(os.path.join(tmp, "enk_es.vars"), t_enk, os.path.join(HERE, "enkefa.var"), False)
+    out = run(["canon", LIBDIR, seqf, vf, "-1", "4357", "300.0", "8"], tmp)
+    vars_in = np.loadtxt(vf, ndmin=2)
+    save("canon_esolan_enkefa.npz", e0=out["canon_e0"][0], sweeps=np.stack(out["sweep"]), vlvr=out["vlvr"][0],
+         mti=out["mti"][0], seed=np.array([4357]), temp=np.array([300.0]),
+         var_idx=vars_in[:, 0].astype(np.int64), var_deg=vars_in[:, 1])
+    from smmp_amd import workloads as wl
+    seq = wl.random_sequence(10, 10)
+    seqf = wl.write_seq_file(os.path.join(tmp, "pt_es.seq"), seq, "pt")
+    nrep, nequi, nswp = 3, 2, 6
+    temps = wl.geometric_temperatures(nrep, 250.0, 1000.0)
+    wl.write_temperatures(os.path.join(tmp, "temperatures"), temps)
+    run(["partem", LIBDIR, seqf, "-1", "4242", str(nrep), str(nequi), str(nswp), "-1"], tmp)
+    td = np.loadtxt(os.path.join(tmp, "time.d"))
+    raw = open(os.path.join(tmp, "par_R.in"), "rb").read()
+    rec = 4 + 4 + 4 + 4 + 8 * nrep + 4
+    assert len(raw) == nrep * rec, len(raw)
+    eol_final = np.frombuffer(raw[16:16 + 8 * nrep], dtype="<f8").copy()
+    save("partem_esolan_len10.npz", seq_in=np.array(seq, dtype="<U8"), temps=temps, time_d=td, eol_final=eol_final,
+         seed=np.array([4242]), nequi=np.array([nequi]), nswp=np.array([nswp]), switch=np.array([-1]))
+
+
 def main(only: list[str]) -> None:
     if not os.path.exists(DRIVER):
         raise SystemExit("build the reference first: make -C oracle ref")
@@ -255,6 +283,8 @@ def main(only: list[str]) -> None:
                  conf_vlvr=np.stack(out["vlvr"]))
     if not only or "esolan" in only:
         esolan_rotation_fixture(tmp)
+    if not only or "esolan_mc" in only:
+        esolan_mc_fixtures(tmp)
     if only:
         return
 

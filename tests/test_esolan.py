@@ -146,17 +146,57 @@ def test_rotation_leaves_the_areas(u):
     assert abs(e1 - e0) < 1e-8 * max(1.0, abs(e0))
 
 
-def test_samplers_reject_the_analytic_solvent():
-    """The samplers run the numerical solvent only; with itysol < 0 they say
-    so before touching the device (energies take the analytic term)."""
+def test_batched_sampler_rejects_the_analytic_solvent():
+    """The batched sampler runs the numerical solvent inside its sweep
+    kernel; with itysol < 0 it says so before touching the device
+    (smmp_metropolis and smmp_partem_s take the analytic term)."""
     from smmp_amd import _ffi
     c = make_ctx(tag="enkefa", itysol=-1)
     temps = np.array([300.0, 400.0])
     with pytest.raises(_ffi.SmmpError, match="analytic solvent"):
         c.sample(temps, np.array([1, 2, 3], dtype=np.uint32), 2, np.tile(c.vlvr(), (2, 1)))
-    with pytest.raises(_ffi.SmmpError, match="analytic solvent"):
-        c.partem_s(temps, 1, 1)
-    w = _ffi.Weight()
-    w.kind, w.beta = 0, 1.0 / (1.98773e-3 * 300.0)
-    with pytest.raises(_ffi.SmmpError, match="analytic solvent"):
-        c.metropolis(0.0, 0.0, w)
+
+
+GASC = 1.98773e-3
+CDR = 1.745329251994330e-2
+
+
+def canon_start(o, d):
+    """The reference's canon start: the variable file applied with a rebuild
+    from the set-up torsions, the variables read back from the torsions
+    (vlvr(a) = toat(iatvr(a)))."""
+    v = o.vlvr0.copy()
+    v[d["var_idx"] - 1] = d["var_deg"] * CDR
+    o.setvar(v)
+    toat = np.ctypeslib.as_array(o.c.toat, (o.nat,))
+    return toat[o.keep["iatvr"] - 1].copy()
+
+
+def test_metropolis_with_esolan_matches_fortran():
+    """canon with itysol -1: every trial's energy runs esolan on the trials'
+    twister; sweep energies, acceptances, final variables and the twister's
+    position bit for bit."""
+    d = golden("canon_esolan_enkefa.npz")
+    c = make_ctx(tag="enkefa", itysol=-1)
+    o = OracleSystem(c, initial_torsions=True)
+    v = canon_start(o, d)
+    mt = OracleMT(int(d["seed"][0]))
+    e0 = o.energy(v, mt)[0]
+    assert e0 == d["e0"][0]
+    vl, tr = o.metropolis(mt, v, 1.0 / (float(d["temp"][0]) * GASC), len(d["sweeps"]))
+    for k, (eol, acz, bad) in enumerate(tr):
+        assert eol == d["sweeps"][k][0] and acz == d["sweeps"][k][1] and not bad, (k, eol, d["sweeps"][k])
+    assert np.array_equal(vl[o.idvr - 1], d["vlvr"])
+    assert mt.s.mti == d["mti"][0]
+
+
+def test_partem_with_esolan_matches_fortran():
+    d = golden("partem_esolan_len10.npz")
+    c = make_ctx(seq=list(d["seq_in"]), itysol=-1)
+    o = OracleSystem(c)
+    nswp, R = int(d["nswp"][0]), len(d["temps"])
+    rc, eol, ipoi, coor, traj = o.partem(OracleMT(int(d["seed"][0])), d["temps"], int(d["nequi"][0]), nswp,
+                                         int(d["switch"][0]))
+    assert rc == 0
+    assert np.abs(traj - d["time_d"][:, 3].reshape(nswp, R)).max() <= 5e-4  # time.d prints 3 decimals
+    assert np.abs(eol - d["eol_final"]).max() < 1e-9

@@ -2,8 +2,9 @@
 C-ABI: smmp_energy against the Fortran reference's own outputs (start and
 random conformations of three sequences, with the twister's position after
 each energy, and a conformation whose degenerate north point draws a random
-rotation from the context's twister), and smmp_energy_batch against the CPU
-oracle on many conformations of one and of two systems."""
+rotation from the context's twister), smmp_energy_batch against the CPU
+oracle on many conformations of one and of two systems, and smmp_metropolis
+and smmp_partem_s against the Fortran's canon and partem_s runs."""
 import os
 import sys
 
@@ -100,7 +101,7 @@ def test_energy_batch_matches_oracle():
     say("smmp_energy_batch c2_20mer 48 conformations")
     out = _ffi.energy_batch([c], np.zeros(len(V), dtype=np.int32), V.ravel(), precision=0)
     st = c.profile_read(3)
-    say(f"esolan kernels {st['ms']:.2f} ms for {st['pairs']:.0f} atoms, launches {st['launches']}")
+    say(f"esolan kernels {st['ms']:.2f} ms, launches {st['launches']}, circle pairs {st['work'][1]:.0f}")
     mt = OracleMT(11)
     for k in range(len(V)):
         ref = o.energy(V[k], mt)
@@ -129,6 +130,47 @@ def test_energy_batch_two_systems():
         ref = (oa if s == 0 else ob).energy(flat[k], OracleMT(1))
         assert _close(out[k, 6], ref[6]), (k, out[k, 6], ref[6])
         assert _close(out[k, 0], ref[0]), (k, out[k, 0], ref[0])
+
+
+GASC = 1.98773e-3
+CDR = 1.745329251994330e-2
+
+
+def test_metropolis_matches_fortran():
+    """smmp_metropolis with itysol -1 (trial loop on the host, every energy
+    on the device, rotations on the trials' twister) against the Fortran's
+    canon run: sweep energies, acceptances, final variables, twister."""
+    from smmp_amd import _ffi
+    d = golden("canon_esolan_enkefa.npz")
+    c = make_ctx(tag="enkefa", itysol=-1)
+    v = c.vlvr()
+    v[d["var_idx"] - 1] = d["var_deg"] * CDR
+    c.setvar(1, v)  # the reference applies the variable file with a rebuild,
+    iat = np.asarray(c.view("var_i.iatvr")[:c.nvt]) - 1
+    c.set_vlvr(np.asarray(c.view("atm_r.toat"))[iat])  # then reads the variables back from the torsions
+    c.sgrnd(int(d["seed"][0]))
+    say("smmp_energy + 8 x smmp_metropolis enkefa itysol -1")
+    e0 = c.energy()
+    assert _close(e0, d["e0"][0])
+    w = _ffi.Weight()
+    w.kind, w.beta = 0, 1.0 / (float(d["temp"][0]) * GASC)
+    eol, acz = e0, 0.0
+    for k, (we, wa) in enumerate(d["sweeps"]):
+        eol, acz = c.metropolis(eol, acz, w)
+        assert _close(eol, we) and acz == wa, (k, eol, we, acz, wa)
+    assert np.abs(c.vlvr()[c.idvr - 1] - d["vlvr"]).max() < 1e-9
+    assert int(c.view("myblock.mti")[0]) == d["mti"][0]
+
+
+def test_partem_s_matches_fortran():
+    d = golden("partem_esolan_len10.npz")
+    c = make_ctx(seq=list(d["seq_in"]), itysol=-1)
+    nswp, R = int(d["nswp"][0]), len(d["temps"])
+    c.sgrnd(int(d["seed"][0]))
+    say(f"smmp_partem_s len10 itysol -1, {R} replicas, {nswp} sweeps")
+    eol, ipoi, traj = c.partem_s(d["temps"], int(d["nequi"][0]), nswp, int(d["switch"][0]))
+    assert np.abs(traj - d["time_d"][:, 3].reshape(nswp, R)).max() <= 6e-4
+    assert np.abs(eol - d["eol_final"]).max() <= 1e-8 * max(1.0, np.abs(d["eol_final"]).max())
 
 
 def test_mixed_solvent_batch_and_samplers_are_rejected():
