@@ -88,9 +88,17 @@ int smmp_metropolis(smmp_ctx *ctx, double *eol, double *acz, const smmp_weight *
  * 'temperatures' file).  Serial-compatible mode: every replica draws from
  * the context's single MT stream in slot order, as the reference does.
  * Outputs (may be NULL): eol[R] per replica, ipoi[R] slot->replica (1-based),
- * traj[nswp*R] energy per sweep and slot (the energies of time.d). */
+ * traj[nswp*R] energy per sweep and slot (the energies of time.d),
+ * obs[nswp*R*2] per sweep and slot the radius of gyration and the C-alpha
+ * end-to-end distance after the slot's sweep (rgyr(0,..) of
+ * partem_s.f:186-193, the last two columns of time.d). */
 int smmp_partem_s(smmp_ctx *ctx, int num_rep, int nequi, int nswp, int switch_, const double *temps,
-                  double *eol, int32_t *ipoi, double *traj);
+                  double *eol, int32_t *ipoi, double *traj, double *obs);
+
+/* rgyr(nml, rgy, ee) rgyr.f:13: radius of gyration over the atoms of chain
+ * nml (0: all chains) and the distance between the C-alpha atoms of its first
+ * and last residue, for the context's current coordinates. */
+int smmp_rgyr(smmp_ctx *ctx, int nml, double *rgy, double *ee);
 
 /* ---- batched entry points (no reference counterpart: the configs'
  *      many-replica and many-conformation workloads) ------------------- */
@@ -150,6 +158,8 @@ typedef struct {
                              variables after every snap_every-th sweep (training data) */
   double *snap_e;         /* with snap_vlvr: (nsweep / snap_every) x R energies */
   int32_t snap_every;
+  double *obs;            /* NULL or nsweep x R x 2 out: radius of gyration and C-alpha
+                             end-to-end distance after every sweep, slot order (rgyr(0,..)) */
 } smmp_sampler_cfg;
 int smmp_sample(smmp_ctx *ctx, const smmp_sampler_cfg *cfg, double *vlvr, double *eol,
                 double *acc, int32_t *ipoi, int32_t *hist, double *traj);

@@ -19,6 +19,7 @@ struct DevSys {
   int16_t mol_a1[kMaxMol], mol_a2[kMaxMol];  // first and last atom of each chain
   int16_t mol_i2[kMaxMol], mol_i3[kMaxMol];  // its second and third backbone atoms
   int16_t mol_h1[kMaxMol], mol_h2[kMaxMol];  // its heavy atoms: heavy indices [h1, h2)
+  int16_t ca_first, ca_last;             // C-alpha atoms of the first and last residue, -1 if none
   double conv, plt, slp, boxsize;
   double gbpr[6 * kMaxMol];              // global parameters of the chains (the context's)
   const int16_t* mol_of;                 // per atom: its chain

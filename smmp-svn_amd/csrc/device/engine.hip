@@ -121,7 +121,7 @@ struct DevMem {
 struct SamplerBuffers {
   DevMem idvr, axvr, vlvr, xyz, nfree, valid, eol, acc, mt, rng_of, slot_of, beta, err, dbg;
   DevMem wb, walpha, hist, traj, ipoi, iswitch, work, trace, trace_n, phase, part, sync, gsync;
-  DevMem sweep_base, tune_vl, tune_mt, tune_gb, gb, da_stats, snap, snap_e;
+  DevMem sweep_base, tune_vl, tune_mt, tune_gb, gb, da_stats, snap, snap_e, obs;
 };
 
 // One system uploaded to the device: a single allocation holding every
@@ -192,6 +192,8 @@ static void upload_system(DevSystem& ds, hipStream_t st) {
     d.mol_h1[m] = h.mol_h1[m]; d.mol_h2[m] = h.mol_h2[m];
   }
   d.boxsize = h.boxsize;
+  d.ca_first = h.ca_first;
+  d.ca_last = h.ca_last;
   d.np = (h.nat + 63) / 64 * 64;
   d.conv = h.conv; d.plt = h.plt; d.slp = h.slp;
   for (int k = 0; k < 6 * kMaxMol; ++k) d.gbpr[k] = h.gbpr[k];

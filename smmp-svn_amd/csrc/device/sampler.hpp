@@ -80,6 +80,7 @@ struct SamplerParams {
   const double* walpha;
   int32_t* hist;         // multicanonical histogram (kmax-kmin+1) or null
   double* traj;          // [sweep][slot] energies or null
+  double* obs;           // [sweep][slot][2] radius of gyration and C-alpha end-to-end distance, or null
   int32_t traj_base;     // first sweep index of this launch
   int32_t nsweep;        // sweeps per launch
   int32_t rep0;          // replica of block 0
@@ -1027,6 +1028,39 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
     ReplicaEnergy re = replica_energy<MIXED>(P.sys, nvt, P.bits_lds, 1, true, g, G);
     if (g != 0) re.w.builds = 0.0;
     add_work(wk, re.w);
+    if (P.obs && g == 0) {  // rgyr(0, ...) of the rebuilt chain (partem_s.f:186-193, rgyr.f:54-140)
+      double m[4] = {0.0, 0.0, 0.0, 0.0};
+      for (int a = tid; a < S.nat; a += blockDim.x) {
+        const double x = L.x[a], y = L.y[a], z = L.z[a];
+        m[0] += x;
+        m[1] += y;
+        m[2] += z;
+        m[3] += x * x + y * y + z * z;
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) m[k] = wave_sum(m[k]);
+      __syncthreads();
+      if (lane == 0)
+        for (int k = 0; k < 4; ++k) L.red[8 * wave + k] = m[k];
+      __syncthreads();
+      if (tid == 0) {
+        for (int k = 0; k < 4; ++k) m[k] = 0.0;
+        for (int w = 0; w < nwaves; ++w)
+          for (int k = 0; k < 4; ++k) m[k] += L.red[8 * w + k];
+        const double dn = (double)S.nat, dx = m[0] / dn, dy = m[1] / dn, dz = m[2] / dn, d2 = m[3] / dn;
+        const double rg2 = d2 - (dx * dx + dy * dy + dz * dz);
+        double ee = 0.0;
+        if (S.ca_first >= 0 && S.ca_last >= 0) {
+          const double ex = L.x[S.ca_last] - L.x[S.ca_first], ey = L.y[S.ca_last] - L.y[S.ca_first],
+                       ez = L.z[S.ca_last] - L.z[S.ca_first];
+          ee = sqrt(ex * ex + ey * ey + ez * ez);
+        }
+        double* o = P.obs + 2 * ((size_t)(traj_base + sw) * P.nrep + P.slot_of[r]);
+        o[0] = rg2 > 0.0 ? sqrt(rg2) : 0.0;
+        o[1] = ee;
+      }
+      __syncthreads();
+    }
     if (tid == 0) {
       double v[3] = {re.o.el + re.o.vw + re.o.hb + re.o.tor + re.o.sol, (double)re.o.err, re.o.mag};
       group_sum(P, r, g, epoch, v, 3);

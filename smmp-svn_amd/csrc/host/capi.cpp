@@ -229,11 +229,21 @@ int smmp_metropolis(smmp_ctx* c, double* eol, double* acz, const smmp_weight* w)
 }
 
 int smmp_partem_s(smmp_ctx* c, int num_rep, int nequi, int nswp, int switch_, const double* temps, double* eol,
-                  int32_t* ipoi, double* traj) {
+                  int32_t* ipoi, double* traj, double* obs) {
   return guarded(c, SMMP_ESTATE, [&] {
     if (num_rep < 1 || num_rep > 2048 || nequi < 0 || nswp < 0 || !temps) throw Error("partem_s> bad arguments");
     if (c->s.ntlml < 1) throw Error("partem_s> no molecule initialised");
-    return engine_partem_s(c->eng, c->s, num_rep, nequi, nswp, switch_, temps, eol, ipoi, traj);
+    return engine_partem_s(c->eng, c->s, num_rep, nequi, nswp, switch_, temps, eol, ipoi, traj, obs);
+  });
+}
+
+int smmp_rgyr(smmp_ctx* c, int nml, double* rgy, double* ee) {
+  return guarded(c, SMMP_ESTATE, [&] {
+    if (!rgy || !ee) throw Error("rgyr> NULL argument");
+    if (c->s.ntlml < 1) throw Error("rgyr> no molecule initialised");
+    if (nml < 0 || nml > c->s.ntlml) throw Error("rgyr> no such molecule");
+    rgyr(c->s, nml, rgy, ee);
+    return SMMP_OK;
   });
 }
 

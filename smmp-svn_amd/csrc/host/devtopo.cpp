@@ -15,6 +15,16 @@ HostSys build_host_sys(const State& s) {
   h.words = (nat + 63) / 64;
   h.nml = s.ntlml;
   h.boxsize = s.boxsize;
+  {  // ind_alf of c_alfa: the last atom of the residue whose name starts with "ca"
+    auto ca_of = [&](int res) {
+      int ia = -1;
+      for (int a = s.iatrs1(res); a <= s.iatrs2(res); ++a)
+        if (s.nmat.get(a).compare(0, 2, "ca") == 0) ia = a - 1;
+      return (int16_t)ia;
+    };
+    h.ca_first = ca_of(s.irsml1(1));
+    h.ca_last = ca_of(s.irsml2(s.ntlml));
+  }
   h.mol_of.assign(nat, 0);
   for (int m = 1; m <= h.nml; ++m) {
     const int a1 = s.iatrs1(s.irsml1(m)), a2 = s.iatrs2(s.irsml2(m));

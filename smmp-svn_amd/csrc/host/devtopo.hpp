@@ -22,6 +22,9 @@ struct HostSys {
   int16_t mol_h1[mxml] = {}, mol_h2[mxml] = {};
   double gbpr[6 * mxml] = {};
   double boxsize = 0;
+  // C-alpha atoms (c_alfa, contacts.f:65-97) of the first residue of the first
+  // chain and the last residue of the last chain, -1 if none: rgyr's ee
+  int16_t ca_first = -1, ca_last = -1;
   std::vector<int16_t> mol_of;      // per atom: its chain
   // atoms (0-based)
   std::vector<int16_t> parent;      // iowat-1, -1 for the first atom of a chain

@@ -7,7 +7,8 @@ protein.py:42); atom lists include the last atom of the range (the
 reference stops one short); residues() returns whole residue names (the
 reference takes one character of each); saveVariables names fixed torsions
 correctly (the reference reuses the previous variable's name); PDB input and
-the observables (hbond, rgyr, contacts, savePDB) are outside this engine."""
+the observables hbond, contacts and savePDB are outside this engine
+(rgyr is provided)."""
 from __future__ import annotations
 
 import sys

@@ -14,9 +14,9 @@ Differences from the f2py module (each stated where it applies):
     ``smmp_amd.weights``; other callables are rejected, because the weight is
     evaluated inside the GPU sweep;
   * ``partem_s`` is exported (partem_s.f:13 has no f2py entry);
-  * observables and minimisers (rgyr, helix, hbond, contacts, zimmer,
-    outpdb, minim, rmsdfun, anneal) are outside the energy-and-move engine
-    and raise NotImplementedError.
+  * observables and minimisers (helix, hbond, contacts, zimmer, outpdb,
+    minim, rmsdfun, anneal) are outside the energy-and-move engine and raise
+    NotImplementedError; rgyr is provided.
 """
 from __future__ import annotations
 
@@ -138,10 +138,11 @@ def partem_s(num_rep: int, nequi: int, nswp: int, nmes: int = 1, nsave: int = 1,
              switch: int = 1):
     """partem_s.f:13: parallel tempering on one random-number stream.
     Reads the 'temperatures' file of the current directory (partem_s.f:88-95)
-    and appends the energies to 'time.d' every nmes sweeps in the reference's
-    format (end-to-end distance and radius of gyration, outside this engine,
-    are written as 0).  Returns (eol, ipoi, energies per sweep and slot).
-    Restarts (newsta false, par_R.in) are not provided."""
+    and writes 'time.d' every nmes sweeps in the reference's format (sweep,
+    slot, temperature, energy, radius of gyration, C-alpha end-to-end
+    distance: rgyr(0, ..) after the slot's sweep, partem_s.f:186-193).
+    Returns (eol, ipoi, energies per sweep and slot).  Restarts (newsta
+    false, par_R.in) are not provided."""
     if not newsta:
         raise NotImplementedError("partem_s: restart from par_R.in is not provided")
     temps = np.zeros(num_rep)
@@ -149,13 +150,21 @@ def partem_s(num_rep: int, nequi: int, nswp: int, nmes: int = 1, nsave: int = 1,
         for _ in range(num_rep):
             j, t = f.readline().split()[:2]
             temps[int(j) - 1] = float(t)
-    eol, ipoi, traj = _ctx.partem_s(temps, int(nequi), int(nswp), int(switch))
+    eol, ipoi, traj, obs = _ctx.partem_s(temps, int(nequi), int(nswp), int(switch), with_obs=True)
     with open("time.d", "w") as out:
         for sw in range(nswp):
             if (sw + 1) % max(int(nmes), 1) == 0:
                 for k1 in range(num_rep):
-                    out.write(f"{sw + 1:8d}{k1 + 1:3d}{temps[k1]:12.3f}{traj[sw, k1]:12.3f}{0.0:12.3f}{0.0:12.3f}\n")
+                    out.write(f"{sw + 1:8d}{k1 + 1:3d}{temps[k1]:12.3f}{traj[sw, k1]:12.3f}"
+                              f"{obs[sw, k1, 0]:12.3f}{obs[sw, k1, 1]:12.3f}\n")
     return eol, ipoi, traj
+
+
+def rgyr(nml: int) -> tuple[float, float]:
+    """smmp.pyf rgyr(nml) -> (rgy, ee), rgyr.f:13: radius of gyration and
+    C-alpha end-to-end distance of chain nml (0: all) at the current
+    coordinates."""
+    return _ctx.rgyr(int(nml))
 
 
 def _outside(name: str):
@@ -165,7 +174,7 @@ def _outside(name: str):
     return f
 
 
-for _n in ("rgyr", "helix", "hbond", "interhbond", "contacts", "zimmer", "outpdb", "minim", "rmsdfun",
+for _n in ("helix", "hbond", "interhbond", "contacts", "zimmer", "outpdb", "minim", "rmsdfun",
            "anneal", "canon", "regul", "gradient", "esolan", "eninteract"):
     globals()[_n] = _outside(_n)
 

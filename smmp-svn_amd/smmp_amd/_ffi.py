@@ -38,7 +38,8 @@ class SamplerCfg(C.Structure):
                 ("w", C.POINTER(Weight)), ("resync_every", C.c_int32), ("verify", C.c_int32),
                 ("groups", C.c_int32), ("graph_sweeps", C.c_int32), ("gbpr", C.POINTER(C.c_double)),
                 ("da", C.c_int32), ("da_scale", C.c_double), ("da_stats", C.POINTER(C.c_double)),
-                ("snap_vlvr", C.POINTER(C.c_double)), ("snap_e", C.POINTER(C.c_double)), ("snap_every", C.c_int32)]
+                ("snap_vlvr", C.POINTER(C.c_double)), ("snap_e", C.POINTER(C.c_double)), ("snap_every", C.c_int32),
+                ("obs", C.POINTER(C.c_double))]
 
 
 def _load():
@@ -65,7 +66,8 @@ def _load():
         "smmp_grnd": (C.c_int, [P, dp]),
         "smmp_outvar": (C.c_int, [P, C.c_int, C.c_char_p]),
         "smmp_metropolis": (C.c_int, [P, dp, dp, C.POINTER(Weight)]),
-        "smmp_partem_s": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp, ip, dp]),
+        "smmp_partem_s": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp, ip, dp, dp]),
+        "smmp_rgyr": (C.c_int, [P, C.c_int, dp, dp]),
         "smmp_energy_batch": (C.c_int, [C.POINTER(P), C.c_int, ip, C.c_int, dp, C.c_int, dp]),
         "smmp_sample": (C.c_int, [P, C.POINTER(SamplerCfg), dp, dp, dp, ip, ip, dp]),
         "smmp_build_info": (C.c_char_p, []),
@@ -93,7 +95,7 @@ lib = _load()
 #: every symbol the header declares (checked by the CPU tests)
 EXPORTS = ("smmp_ctx_create", "smmp_ctx_destroy", "smmp_last_error", "smmp_state_view", "smmp_state_count",
            "smmp_state_name", "smmp_init_energy", "smmp_init_molecule", "smmp_setvar", "smmp_energy",
-           "smmp_enysol", "smmp_sgrnd", "smmp_grnd", "smmp_outvar", "smmp_metropolis", "smmp_partem_s",
+           "smmp_enysol", "smmp_sgrnd", "smmp_grnd", "smmp_outvar", "smmp_metropolis", "smmp_partem_s", "smmp_rgyr",
            "smmp_energy_batch", "smmp_sample", "smmp_build_info", "smmp_profile", "smmp_profile_read",
            "smmp_profile_phases", "smmp_muca_update", "smmp_muca_final", "smmp_sampler_info",
            "smmp_mlp_size", "smmp_mlp_set", "smmp_mlp_get", "smmp_mlp_grad", "smmp_mlp_fit")
@@ -250,20 +252,32 @@ class Context:
         self.check(lib.smmp_metropolis(self.h, C.byref(e), C.byref(a), C.byref(weight)))
         return e.value, a.value
 
-    def partem_s(self, temps, nequi: int, nswp: int, switch: int = 1):
+    def partem_s(self, temps, nequi: int, nswp: int, switch: int = 1, with_obs: bool = False):
+        """(eol, ipoi, energies per sweep and slot[, radius of gyration and
+        end-to-end distance per sweep and slot])."""
         temps = np.ascontiguousarray(temps, dtype=np.float64)
         R = len(temps)
         eol = np.zeros(R)
         ipoi = np.zeros(R, dtype=np.int32)
         traj = np.zeros(max(nswp, 1) * R)
-        rc = lib.smmp_partem_s(self.h, R, nequi, nswp, switch, dptr(temps), dptr(eol), iptr(ipoi), dptr(traj))
+        obs = np.zeros(max(nswp, 1) * R * 2) if with_obs else None
+        rc = lib.smmp_partem_s(self.h, R, nequi, nswp, switch, dptr(temps), dptr(eol), iptr(ipoi), dptr(traj),
+                               dptr(obs) if obs is not None else None)
         self.check(rc)
+        if with_obs:
+            return eol, ipoi, traj[:nswp * R].reshape(nswp, R), obs[:nswp * R * 2].reshape(nswp, R, 2)
         return eol, ipoi, traj[:nswp * R].reshape(nswp, R)
+
+    def rgyr(self, nml: int = 0) -> tuple[float, float]:
+        """(radius of gyration, C-alpha end-to-end distance) of the current coordinates."""
+        rgy, ee = C.c_double(), C.c_double()
+        self.check(lib.smmp_rgyr(self.h, int(nml), C.byref(rgy), C.byref(ee)))
+        return rgy.value, ee.value
 
     def sample(self, temps, seeds, nsweep: int, vlvr, mode: int = 0, weight: Weight | None = None,
                precision: int = 0, with_traj: bool = True, allow_drift: bool = False, verify: bool = False,
                groups: int = 0, graph_sweeps: int = 0, gbpr=None, da: bool = False,
-               da_scale: float = 1.0, snap_every: int = 0):
+               da_scale: float = 1.0, snap_every: int = 0, obs: bool = False):
         """Independent-stream replica sampler (smmp_sample).  groups: workgroups
         per replica (0 automatic, -1 measured); graph_sweeps: sweeps per
         replay of a captured graph (0: direct launches); gbpr: several chains,
@@ -272,7 +286,8 @@ class Context:
         acceptance with the context's network (per-replica counts returned as
         "da_stats"); snap_every > 0: each
         replica's variables and energy every snap_every sweeps ("snap_vlvr",
-        "snap_e")."""
+        "snap_e"); obs: radius of gyration and end-to-end distance after every
+        sweep ("obs", nsweep x R x 2, slot order)."""
         temps = np.ascontiguousarray(temps, dtype=np.float64)
         R = len(temps)
         seeds = np.ascontiguousarray(seeds, dtype=np.uint32)
@@ -302,6 +317,10 @@ class Context:
             snap_v = np.zeros((max(ns, 1), R, self.nvt))
             snap_e = np.zeros((max(ns, 1), R))
             cfg.snap_vlvr, cfg.snap_e, cfg.snap_every = dptr(snap_v), dptr(snap_e), snap_every
+        ob = None
+        if obs:
+            ob = np.zeros((max(nsweep, 1), R, 2))
+            cfg.obs = dptr(ob)
         nml = int(self.view("mol_i.ntlml")[0])
         gb = None
         if gbpr is not None:
@@ -316,6 +335,7 @@ class Context:
             snap_v, snap_e = snap_v[:nsweep // snap_every], snap_e[:nsweep // snap_every]
         return {"vlvr": vl, "eol": eol, "acc": acc, "ipoi": ipoi, "hist": hist, "rc": rc, "gbpr": gb,
                 "da_stats": da_stats, "snap_vlvr": snap_v, "snap_e": snap_e,
+                "obs": ob[:nsweep] if ob is not None else None,
                 "traj": traj[:nsweep * R].reshape(nsweep, R) if traj is not None else None}
 
 

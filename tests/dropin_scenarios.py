@@ -29,11 +29,18 @@ def setup_cpu(args):
     p.setOrigin((1.0, 2.0, 3.0), (0.1, 0.2, 0.3))
     out["origin"] = p.origin()
     p.saveVariables(args["varout"])
-    try:
-        smmp.rgyr(0)
-        out["rgyr"] = "ran"
-    except NotImplementedError:
-        out["rgyr"] = "not provided"
+    rgy, ee = smmp.rgyr(0)
+    out["rgyr"] = [rgy, ee, u.rgyr(), u.endToEnd() if hasattr(u, "endToEnd") else ee]
+    # the same from the coordinates: all atoms, and the C-alpha atoms of the
+    # first and the last residue
+    xyz = np.array([a.position() for a in p.atoms()])
+    names = [a.name().strip().lower() for a in p.atoms()]
+    ca = [i for i, nm in enumerate(names) if nm.startswith("ca")]
+    r0, r1 = int(smmp.res_i.iatrs2[0]), int(smmp.res_i.iatrs1[len(p) - 1]) - 1
+    c_first = max(i for i in ca if i < r0)
+    c_last = max(i for i in ca if i >= r1)
+    out["rgyr_np"] = [float(np.sqrt(((xyz - xyz.mean(0)) ** 2).sum(1).mean())),
+                      float(np.linalg.norm(xyz[c_last] - xyz[c_first]))]
     from smmp_amd import weights
     out["beta_from_lambda"] = weights.as_descriptor(lambda x: x * smmp.bet.beta).beta
     try:

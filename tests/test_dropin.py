@@ -38,7 +38,9 @@ def test_universe_and_protein_state(tmp_path):
     assert abs(out["psi2"] - 156.18) < 1e-9    # enkefa.var: 2 : psi : 156.18 (first gly)
     assert out["itysol"] == 1 and out["nvr"] == 24
     assert out["origin"] == [[1.0, 2.0, 3.0], [0.1, 0.2, 0.3]]
-    assert out["rgyr"] == "not provided"
+    rgy, ee, urgy, uee = out["rgyr"]
+    assert abs(rgy - out["rgyr_np"][0]) < 1e-9 and abs(ee - out["rgyr_np"][1]) < 1e-9
+    assert urgy == rgy and uee == ee
     assert out["nonlinear"] == "rejected"
     assert abs(out["beta_from_lambda"] - out["beta"]) < 1e-12
     # saveVariables writes every value of the .var file it was read from

@@ -168,8 +168,10 @@ def test_partem_s_matches_fortran():
     nswp, R = int(d["nswp"][0]), len(d["temps"])
     c.sgrnd(int(d["seed"][0]))
     say(f"smmp_partem_s len10 itysol -1, {R} replicas, {nswp} sweeps")
-    eol, ipoi, traj = c.partem_s(d["temps"], int(d["nequi"][0]), nswp, int(d["switch"][0]))
+    eol, ipoi, traj, obs = c.partem_s(d["temps"], int(d["nequi"][0]), nswp, int(d["switch"][0]), with_obs=True)
     assert np.abs(traj - d["time_d"][:, 3].reshape(nswp, R)).max() <= 6e-4
+    assert np.abs(obs[:, :, 0] - d["time_d"][:, 4].reshape(nswp, R)).max() <= 6e-4
+    assert np.abs(obs[:, :, 1] - d["time_d"][:, 5].reshape(nswp, R)).max() <= 6e-4
     assert np.abs(eol - d["eol_final"]).max() <= 1e-8 * max(1.0, np.abs(d["eol_final"]).max())
 
 

@@ -58,7 +58,9 @@ def test_metropolis_with_solvent_vs_reference():
 @pytest.mark.parametrize("tag", ["len10", "c2_20mer"])
 def test_partem_s_serial_compatible_vs_reference(tag):
     """partem_s.f on one stream from the extended chain (switch < 0): the
-    energies of time.d (3 decimals) and of par_R.in.  (From random starts the
+    energies of time.d (3 decimals), its radius of gyration and C-alpha
+    end-to-end distance (rgyr after each slot's sweep), and the energies of
+    par_R.in.  (From random starts the
     first sweeps sit at clash energies ~1e13 kcal/mol where a move's energy
     change is below the rounding of the total, so the reference's own
     decisions there depend on its summation order.)"""
@@ -67,9 +69,11 @@ def test_partem_s_serial_compatible_vs_reference(tag):
     c = make_ctx(tag=tag, itysol=1)
     c.sgrnd(int(p["seed"][0]))
     say(f"smmp_partem_s {tag} {R} replicas {nswp} sweeps")
-    eol, ipoi, traj = c.partem_s(p["temps"], int(p["nequi"][0]), nswp, int(p["switch"][0]))
+    eol, ipoi, traj, obs = c.partem_s(p["temps"], int(p["nequi"][0]), nswp, int(p["switch"][0]), with_obs=True)
     ref = p["time_d"][:, 3].reshape(nswp, R)
     assert np.abs(traj - ref).max() <= 6e-4
+    assert np.abs(obs[:, :, 0] - p["time_d"][:, 4].reshape(nswp, R)).max() <= 6e-4
+    assert np.abs(obs[:, :, 1] - p["time_d"][:, 5].reshape(nswp, R)).max() <= 6e-4
     assert np.abs(eol - p["eol_final"]).max() <= 1e-8 * max(1.0, np.abs(p["eol_final"]).max())
 
 
@@ -325,3 +329,23 @@ def test_measured_groups_keep_the_start_state():
     t2 = c.sample(temps, seeds, 5, v0, mode=1, groups=-1)
     assert c.sampler_info()["groups"] == G
     _same_run(t2, e)
+
+
+def test_sample_observables_match_the_final_coordinates():
+    """smmp_sample's per-sweep radius of gyration and end-to-end distance:
+    the last sweep's values equal rgyr of each replica's final variables."""
+    c = make_ctx(tag="len10", itysol=1)
+    R, nsw = 4, 3
+    temps = np.array([300.0, 450.0, 650.0, 900.0])
+    rng = np.random.default_rng(4)
+    v0 = np.tile(c.vlvr(), (R, 1))
+    v0[:, c.idvr - 1] = np.pi + rng.normal(0, 0.3, size=(R, c.nvr))
+    say("smmp_sample canonical len10 R=4 3 sweeps with observables")
+    out = c.sample(temps, np.arange(7, 7 + R, dtype=np.uint32), nsw, v0, mode=0, obs=True)
+    assert out["obs"].shape == (nsw, R, 2)
+    for r in range(R):
+        c.set_vlvr(out["vlvr"][r])
+        c.energy()  # rebuilds the coordinates of these variables
+        rgy, ee = c.rgyr(0)
+        assert abs(out["obs"][-1, r, 0] - rgy) <= 1e-9 * max(1.0, rgy), r
+        assert abs(out["obs"][-1, r, 1] - ee) <= 1e-9 * max(1.0, ee), r
