@@ -117,8 +117,12 @@ int smmp_energy_batch(smmp_ctx *const *ctxs, int nsys, const int32_t *conf_sys, 
  * each with its own twister.f MT19937 stream seeded sgrnd(seeds[r]).
  *   mode 0: canonical at temps[r]; mode 1: parallel tempering over temps
  *   (exchange attempt after every sweep, partem_s.f:209-232 rule, exchange
- *   draws from the stream of the lower slot); mode 2: multicanonical with
- *   the weight w (muca_weight2) shared by all replicas.
+ *   draws from its own stream seeded sgrnd(seeds[R])); mode 2:
+ *   multicanonical with the weight w (muca_weight2) shared by all replicas.
+ * With the analytic solvent (itysol < 0; one chain) the replicas take their
+ * trials in lockstep with one device energy pass per trial step, and each
+ * replica's solvent rotations draw from its own stream; delayed acceptance,
+ * verify and graph_sweeps do not apply there.
  * vlvr[r*nvt..] in: start conformations (all variables), out: final.
  * eol[r] out: final energies; acc[r] out: accepted moves; ipoi[R] out.
  * hist (mode 2, may be NULL): kmax-kmin+1 int32 counts of sweep-end bins.

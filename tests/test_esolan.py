@@ -146,15 +146,16 @@ def test_rotation_leaves_the_areas(u):
     assert abs(e1 - e0) < 1e-8 * max(1.0, abs(e0))
 
 
-def test_batched_sampler_rejects_the_analytic_solvent():
-    """The batched sampler runs the numerical solvent inside its sweep
-    kernel; with itysol < 0 it says so before touching the device
-    (smmp_metropolis and smmp_partem_s take the analytic term)."""
+def test_batched_sampler_limits_with_the_analytic_solvent():
+    """The batched sampler's analytic-solvent path has no delayed acceptance
+    and no verify mode; it says so before touching the device."""
     from smmp_amd import _ffi
     c = make_ctx(tag="enkefa", itysol=-1)
     temps = np.array([300.0, 400.0])
     with pytest.raises(_ffi.SmmpError, match="analytic solvent"):
-        c.sample(temps, np.array([1, 2, 3], dtype=np.uint32), 2, np.tile(c.vlvr(), (2, 1)))
+        c.sample(temps, np.array([1, 2, 3], dtype=np.uint32), 2, np.tile(c.vlvr(), (2, 1)), verify=True)
+    with pytest.raises(_ffi.SmmpError, match="analytic solvent"):
+        c.sample(temps, np.array([1, 2, 3], dtype=np.uint32), 2, np.tile(c.vlvr(), (2, 1)), da=True)
 
 
 GASC = 1.98773e-3

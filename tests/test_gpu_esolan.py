@@ -184,3 +184,69 @@ def test_mixed_solvent_batch_and_samplers_are_rejected():
         _ffi.energy_batch([a, b], np.array([0, 1], dtype=np.int32), V.ravel())
     with pytest.raises(_ffi.SmmpError, match="analytic solvent"):
         a.enysol()
+
+
+def _near_extended(c, R, seed):
+    rng = np.random.default_rng(seed)
+    v0 = np.tile(c.vlvr(), (R, 1))
+    v0[:, c.idvr - 1] = np.pi + rng.normal(0, 0.25, size=(R, c.nvr))
+    return v0
+
+
+def test_batched_canonical_replicas_match_oracle_streams():
+    """smmp_sample mode 0 with itysol -1 (replicas in lockstep, one device
+    pass per trial step): replica r is the oracle's Metropolis with esolan on
+    stream seeds[r]."""
+    c = _ctx("len10", golden("esolan_len10.npz"))
+    o = OracleSystem(c)
+    R, nsw = 4, 2
+    temps = np.array([250.0, 400.0, 650.0, 1000.0])
+    seeds = np.array([11, 22, 33, 44], dtype=np.uint32)
+    v0 = _near_extended(c, R, 5)
+    say(f"smmp_sample canonical len10 itysol -1 R={R} {nsw} sweeps")
+    out = c.sample(temps, seeds, nsw, v0, mode=0, obs=True)
+    for r in range(R):
+        mt = OracleMT(int(seeds[r]))
+        o.reset()
+        vl, tr = o.metropolis(mt, v0[r], 1.0 / (temps[r] * GASC), nsw)
+        assert out["acc"][r] == tr[-1][1], r
+        for k in range(nsw):
+            assert _close(out["traj"][k, r], tr[k][0]), (r, k, out["traj"][k, r], tr[k][0])
+        assert np.abs(out["vlvr"][r] - vl).max() < 1e-9
+    assert out["obs"].shape == (nsw, R, 2) and np.all(out["obs"][:, :, 0] > 0)
+
+
+def test_batched_parallel_tempering_matches_oracle():
+    """smmp_sample mode 1 with itysol -1: per-replica streams for the sweeps,
+    the exchange stream seeds[R] for partem_s.f's exchange rule."""
+    c = _ctx("len10", golden("esolan_len10.npz"))
+    o = OracleSystem(c)
+    R, nsw = 3, 3
+    temps = np.array([300.0, 550.0, 1000.0])
+    seeds = np.array([5, 6, 7, 8], dtype=np.uint32)
+    v0 = _near_extended(c, R, 9)
+    say(f"smmp_sample parallel tempering len10 itysol -1 R={R} {nsw} sweeps")
+    out = c.sample(temps, seeds, nsw, v0, mode=1)
+    beta = 1.0 / (temps * GASC)
+    mts = [OracleMT(int(s)) for s in seeds[:R]]
+    mx = OracleMT(int(seeds[R]))
+    cur, eol, acc = v0.copy(), np.zeros(R), np.zeros(R)
+    ipoi, slot = list(range(1, R + 1)), list(range(R))
+    traj = np.zeros((nsw, R))
+    for sw in range(nsw):
+        for r in range(R):
+            vl, tr = o.metropolis(mts[r], cur[r], beta[slot[r]], 1)
+            cur[r], eol[r] = vl, tr[-1][0]
+            acc[r] += tr[-1][1]
+            traj[sw, slot[r]] = eol[r]
+        nu, top = (1, R - 1) if sw % 2 == 0 else (2, R)
+        for i in range(nu, top + 1, 2):
+            j = 1 if i == R else i + 1
+            a, b = ipoi[i - 1], ipoi[j - 1]
+            delta = -beta[i - 1] * eol[b - 1] - beta[j - 1] * eol[a - 1] + beta[i - 1] * eol[a - 1] + beta[j - 1] * eol[b - 1]
+            if mx.grnd() <= np.exp(delta):
+                ipoi[i - 1], ipoi[j - 1] = b, a
+                slot[b - 1], slot[a - 1] = i - 1, j - 1
+    assert list(out["ipoi"]) == ipoi
+    assert np.array_equal(out["acc"], acc)
+    assert np.all(np.abs(out["traj"] - traj) <= RTOL * np.maximum(1.0, np.abs(traj)))
