@@ -53,8 +53,8 @@ namespace smmp {
 
 constexpr int kEsThreads = 256;  // four wavefronts per workgroup, each on its own atom
 constexpr int kEsWaves = kEsThreads / kWave;
-constexpr int kEsSlots = 256;    // circle slots of one atom (at most 255 neighbours)
-constexpr int kEsMaxNb = 255;
+constexpr int kEsSlots = 192;    // circle slots of one atom (at most 191 neighbours)
+constexpr int kEsMaxNb = kEsSlots - 1;
 constexpr double kEsCoinc = (double)1e-10f;  // esolan.f:127 compares with a default-real literal
 constexpr double kPi4 = 4.0 * kPi;
 
@@ -66,6 +66,9 @@ constexpr int kNone = 0, kDropL = 1, kVert = 2, kDropK = 4, kZeroAs = 5, kCover 
 
 struct EsLds {
   double A[kEsSlots], B[kEsSlots], C[kEsSlots], D[kEsSlots];  // circle coefficients by slot
+  // b/a, c/a, d/a and sqrt((b/a)^2 + (c/a)^2 - 4 d/a) of each circle: the
+  // values esolan.f:255-268 forms for every pair, formed once
+  double nB[kEsSlots], nC[kEsSlots], nD[kEsSlots], nR[kEsSlots];
   int16_t atom[kEsSlots];                                     // neighbour atom of each slot
   uint8_t lst[kEsSlots], tmp[kEsSlots];                       // circle list (slots) and compaction buffer
   uint8_t rm[kEsSlots];                                       // per list position: removed in this step
@@ -137,12 +140,11 @@ __device__ __forceinline__ int es_classify(const EsLds& L, int K, int Lc, double
   else if (ak < 0.0 && al > 0.0) kind = 3;
   else if (ak < 0.0 && al < 0.0) kind = 4;
   if (!kind) return kNone;
-  const double b1 = L.B[K] / ak, c1 = L.C[K] / ak, d1 = L.D[K] / ak;
-  const double b2 = L.B[Lc] / al, c2 = L.C[Lc] / al, d2 = L.D[Lc] / al;
+  const double b1 = L.nB[K], c1 = L.nC[K], d1 = L.nD[K], r1 = L.nR[K];
+  const double b2 = L.nB[Lc], c2 = L.nC[Lc], d2 = L.nD[Lc], r2 = L.nR[Lc];
   const double D = 4.0 * ((b1 - b2) * (b2 * d1 - b1 * d2) + (c1 - c2) * (c2 * d1 - c1 * d2) - (d1 - d2) * (d1 - d2)) +
                    (b1 * c2 - b2 * c1) * (b1 * c2 - b2 * c1);
   const double dist = sqrt((b1 - b2) * (b1 - b2) + (c1 - c2) * (c1 - c2));
-  const double r1 = sqrt(b1 * b1 + c1 * c1 - 4.0 * d1), r2 = sqrt(b2 * b2 + c2 * c2 - 4.0 * d2);
   int code = kNone;
   if (kind == 1) {  // a01
     if (D <= 0.0 && dist <= r2 - r1) code = kDropK;
@@ -269,10 +271,19 @@ __device__ void es_item(const EsParams& P, EsLds& L, const EsScratch& X, int ite
       z = xx * cfsg + yy * sf + zz * cfcg;
     }
     const double rk = S.rvdw[a], pk = rk * rk;
-    L.A[s] = x * x + y * y + (z - R) * (z - R) - pk;
-    L.B[s] = -pom * x;
-    L.C[s] = -pom * y;
-    L.D[s] = 4.0 * pol * (x * x + y * y + (R + z) * (R + z) - pk);
+    const double ca = x * x + y * y + (z - R) * (z - R) - pk, cb = -pom * x, cc = -pom * y;
+    const double cd = 4.0 * pol * (x * x + y * y + (R + z) * (R + z) - pk);
+    L.A[s] = ca;
+    L.B[s] = cb;
+    L.C[s] = cc;
+    L.D[s] = cd;
+    if (ca != 0.0) {
+      const double nb = cb / ca, nc = cc / ca, nd = cd / ca;
+      L.nB[s] = nb;
+      L.nC[s] = nc;
+      L.nD[s] = nd;
+      L.nR[s] = sqrt(nb * nb + nc * nc - 4.0 * nd);
+    }
     L.lst[s] = (uint8_t)s;
     L.rm[s] = 0;
     L.hasv[s] = 0;
@@ -467,7 +478,9 @@ __device__ void es_item(const EsParams& P, EsLds& L, const EsScratch& X, int ite
     const double aa = 0.5 / a, ba = b * aa, ca = cc * aa, bcd = b2c2 - 2.0 * a * d;
     const double rr = 0.5 * wv / daba;
     const double prob = (alp + bet) / 2.0;
-    const double ap1 = -ba + rr * cos(prob), ap2 = -ca + rr * sin(prob);
+    double sp, cp;
+    sincos(prob, &sp, &cp);
+    const double ap1 = -ba + rr * cp, ap2 = -ca + rr * sp;
     const double q2 = ap1 * ap1 + ap2 * ap2;
     bool cov = false;
     int q = 0;
@@ -479,7 +492,9 @@ __device__ void es_item(const EsParams& P, EsLds& L, const EsScratch& X, int ite
     work[3] += (double)q;
     if (cov) continue;
     const double hm = 5e-1 * (bet - alp), hp = 5e-1 * (bet + alp);
-    const double dcbetmalp = cos(hm), dcbetpalp = cos(hp), dsbetmalp = sin(hm), dsbetpalp = sin(hp);
+    double dsbetmalp, dcbetmalp, dsbetpalp, dcbetpalp;
+    sincos(hm, &dsbetmalp, &dcbetmalp);
+    sincos(hp, &dsbetpalp, &dcbetpalp);
     const double uv = daba * (bcd + 2.0 * R42 * ak) * dcbetmalp - a * wv * (b * dcbetpalp + cc * dsbetpalp);
     arcs += R2 * (aia * (alp - bet) + (d + R42 * a) * vv1 * (kPi - 2.0 * atan(uv / (2.0 * ak * vv * dsbetmalp))));
   }
@@ -598,7 +613,7 @@ static int es_launch(EsolanWork& W, hipStream_t st, EsParams P, int32_t* d_small
 
 static void es_check(int err) {
   if (err & kEsErrCoinc) throw Error("esolan> centres of two atoms coincide");
-  if (err & kEsErrNb) throw Error("esolan> more than 255 solvent neighbours of one atom");
+  if (err & kEsErrNb) throw Error("esolan> more than 191 solvent neighbours of one atom");
   if (err & kEsErrCap) throw Error("esolan> more than 65536 circle intersections at one atom");
   if (err & kEsErrLine) throw Error("esolan> a circle of the projection is a line");
 }

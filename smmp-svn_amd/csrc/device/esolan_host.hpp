@@ -22,7 +22,7 @@ void esolan_work_destroy(EsolanWork* w);
 //          energy draws from (esolan.f:199-200 draws on a degenerate north
 //          point, in atom order, conformations in order).
 // Throws Error on coinciding atom centres (esolan.f:127-133 stops there) and
-// on more than 255 neighbours of one atom.
+// on more than 191 neighbours of one atom.
 void esolan_energies(EsolanWork*& W, hipStream_t st, const DevSys* d_systems, const std::vector<const HostSys*>& hsys,
                      const std::vector<int32_t>& conf_sys, const int32_t* d_conf_sys, int nconf, int stride,
                      const double* d_xyz, double* d_out, int out_stride, const std::function<double(int)>& draw);
