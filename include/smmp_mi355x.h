@@ -82,18 +82,30 @@ typedef struct {
  * GPU with the context's MT stream; eol/acz in,out. */
 int smmp_metropolis(smmp_ctx *ctx, double *eol, double *acz, const smmp_weight *w);
 
-/* partem_s(num_rep,nequi,nswp,nmes,nsave,newsta,switch) partem_s.f:13 with a
- * new start (newsta): switch > 0 random initial torsions, < 0 all 180
- * degrees, 0 all zero (partem_s.f:114-126).  temps: R temperatures (the
+/* partem_s(num_rep,nequi,nswp,nmes,nsave,newsta,switch) partem_s.f:13.
+ * New start (start NULL, newsta): switch > 0 random initial torsions, < 0
+ * all 180 degrees, 0 all zero (partem_s.f:114-126), then nequi sweeps of
+ * every replica.  Restart (start given, newsta false): start[R*nvr] holds
+ * every replica's free variables (idvr order, radians) and the run begins
+ * with the parallel-tempering sweeps (partem_s.f:97-111; the reference reads
+ * them from conf_NNNN.var but leaves its replicas at zero, which this does
+ * not replicate).  nswp sweeps follow.  temps: R temperatures (the
  * 'temperatures' file).  Serial-compatible mode: every replica draws from
  * the context's single MT stream in slot order, as the reference does.
  * Outputs (may be NULL): eol[R] per replica, ipoi[R] slot->replica (1-based),
  * traj[nswp*R] energy per sweep and slot (the energies of time.d),
  * obs[nswp*R*2] per sweep and slot the radius of gyration and the C-alpha
  * end-to-end distance after the slot's sweep (rgyr(0,..) of
- * partem_s.f:186-193, the last two columns of time.d). */
+ * partem_s.f:186-193, the last two columns of time.d), coor[R*nvr] every
+ * replica's final free variables (what partem_s.f:237-255 writes to
+ * conf_NNNN.var). */
 int smmp_partem_s(smmp_ctx *ctx, int num_rep, int nequi, int nswp, int switch_, const double *temps,
-                  double *eol, int32_t *ipoi, double *traj, double *obs);
+                  double *eol, int32_t *ipoi, double *traj, double *obs, const double *start, double *coor);
+
+/* redvar() redvar.f:13 on a copy of the context: the values of all nvt
+ * variables (radians) after reading the .var file at path over the
+ * context's current ones; the context itself is not changed. */
+int smmp_redvar(smmp_ctx *ctx, const char *path, double *vlvr);
 
 /* rgyr(nml, rgy, ee) rgyr.f:13: radius of gyration over the atoms of chain
  * nml (0: all chains) and the distance between the C-alpha atoms of its first

@@ -3,6 +3,7 @@
 // device/engine.hip and are reached through engine.hpp.
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -229,11 +230,25 @@ int smmp_metropolis(smmp_ctx* c, double* eol, double* acz, const smmp_weight* w)
 }
 
 int smmp_partem_s(smmp_ctx* c, int num_rep, int nequi, int nswp, int switch_, const double* temps, double* eol,
-                  int32_t* ipoi, double* traj, double* obs) {
+                  int32_t* ipoi, double* traj, double* obs, const double* start, double* coor) {
   return guarded(c, SMMP_ESTATE, [&] {
     if (num_rep < 1 || num_rep > 2048 || nequi < 0 || nswp < 0 || !temps) throw Error("partem_s> bad arguments");
     if (c->s.ntlml < 1) throw Error("partem_s> no molecule initialised");
-    return engine_partem_s(c->eng, c->s, num_rep, nequi, nswp, switch_, temps, eol, ipoi, traj, obs);
+    return engine_partem_s(c->eng, c->s, num_rep, nequi, nswp, switch_, temps, eol, ipoi, traj, obs, start, coor);
+  });
+}
+
+int smmp_redvar(smmp_ctx* c, const char* path, double* vlvr) {
+  return guarded(c, SMMP_EINPUT, [&] {
+    if (!path || !vlvr) throw Error("redvar> NULL argument");
+    if (c->s.ntlml < 1) throw Error("redvar> no molecule initialised");
+    auto t = std::make_unique<State>(c->s);
+    redvar_file(*t, path);
+    for (int i = 1; i <= t->nvars(); ++i) {  // as init_molecule.f:111-131 reads them back
+      const int it = t->ityvr(i), a = t->iatvr(i);
+      vlvr[i - 1] = it == 3 ? t->toat(a) : it == 2 ? t->baat(a) : t->blat(a);
+    }
+    return SMMP_OK;
   });
 }
 

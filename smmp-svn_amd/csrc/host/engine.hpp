@@ -32,7 +32,7 @@ double engine_enysol(Engine*& e, State& s);
 int engine_metropolis(Engine*& e, State& s, double* eol, double* acz, const smmp_weight& w);
 // partem_s.f:13, serial-compatible single stream
 int engine_partem_s(Engine*& e, State& s, int nrep, int nequi, int nswp, int switch_, const double* temps,
-                    double* eol, int32_t* ipoi, double* traj, double* obs);
+                    double* eol, int32_t* ipoi, double* traj, double* obs, const double* start, double* coor);
 int engine_energy_batch(Engine*& e, std::vector<State*>& sys, const int32_t* conf_sys, int nconf,
                         const double* vlvr, int precision, double* out);
 int engine_sample(Engine*& e, State& s, const smmp_sampler_cfg& cfg, double* vlvr, double* eol, double* acc,

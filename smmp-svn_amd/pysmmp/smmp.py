@@ -140,24 +140,77 @@ def partem_s(num_rep: int, nequi: int, nswp: int, nmes: int = 1, nsave: int = 1,
     Reads the 'temperatures' file of the current directory (partem_s.f:88-95)
     and writes 'time.d' every nmes sweeps in the reference's format (sweep,
     slot, temperature, energy, radius of gyration, C-alpha end-to-end
-    distance: rgyr(0, ..) after the slot's sweep, partem_s.f:186-193).
-    Returns (eol, ipoi, energies per sweep and slot).  Restarts (newsta
-    false, par_R.in) are not provided."""
-    if not newsta:
-        raise NotImplementedError("partem_s: restart from par_R.in is not provided")
+    distance: rgyr(0, ..) after the slot's sweep, partem_s.f:186-193).  At
+    the end it writes the restart files 'par_R.in' and 'conf_NNNN.var'
+    (partem_s.f:237-255); with newsta false it starts from them and runs
+    sweeps nstart..nswp, nstart read from par_R.in (partem_s.f:97-111; the
+    reference then leaves its replicas at zero torsions, this starts them
+    from their conf files, and takes eol(k) from record k rather than the
+    first value of every record).  Returns (eol, ipoi, energies per sweep and
+    slot)."""
     temps = np.zeros(num_rep)
     with open("temperatures") as f:
         for _ in range(num_rep):
             j, t = f.readline().split()[:2]
             temps[int(j) - 1] = float(t)
-    eol, ipoi, traj, obs = _ctx.partem_s(temps, int(nequi), int(nswp), int(switch), with_obs=True)
+    start, nstart = None, 1
+    if not newsta:
+        nstart, _ = read_par_r("par_R.in", num_rep)
+        idvr = _ctx.idvr - 1
+        start = np.stack([_ctx.redvar(conf_file_name(k + 1))[idvr] for k in range(num_rep)])
+    nrun = max(int(nswp) - nstart + 1, 0)
+    eol, ipoi, traj, obs, coor = _ctx.partem_s(temps, int(nequi), nrun, int(switch), with_obs=True, start=start,
+                                               with_coor=True)
     with open("time.d", "w") as out:
-        for sw in range(nswp):
-            if (sw + 1) % max(int(nmes), 1) == 0:
+        for sw in range(nrun):
+            nsw = nstart + sw
+            if nsw % max(int(nmes), 1) == 0:
                 for k1 in range(num_rep):
-                    out.write(f"{sw + 1:8d}{k1 + 1:3d}{temps[k1]:12.3f}{traj[sw, k1]:12.3f}"
+                    out.write(f"{nsw:8d}{k1 + 1:3d}{temps[k1]:12.3f}{traj[sw, k1]:12.3f}"
                               f"{obs[sw, k1, 0]:12.3f}{obs[sw, k1, 1]:12.3f}\n")
+    write_par_r("par_R.in", int(nswp), eol)
+    keep = _ctx.vlvr()
+    for k in range(num_rep):
+        v = keep.copy()
+        v[_ctx.idvr - 1] = coor[k]
+        _ctx.set_vlvr(v)
+        _ctx.outvar(0, conf_file_name(k + 1))
+    _ctx.set_vlvr(keep)
     return eol, ipoi, traj
+
+
+def conf_file_name(k: int) -> str:
+    """fileNameMP('conf_0000.var', 6, 9, k) of utilities.f:141-170."""
+    if k >= 10000:
+        raise ValueError("too few characters available to label filenames")
+    return f"conf_{k:04d}.var"
+
+
+def write_par_r(path: str, nswp: int, eol) -> None:
+    """par_R.in as partem_s.f:237-241 writes it: Fortran unformatted
+    sequential records, per replica [nswp] and [eol(1..R)]."""
+    e = np.ascontiguousarray(eol, dtype="<f8")
+    with open(path, "wb") as f:
+        for _ in range(len(e)):
+            f.write(np.array([4, nswp, 4], dtype="<i4").tobytes())
+            f.write(np.array([8 * len(e)], dtype="<i4").tobytes() + e.tobytes() + np.array([8 * len(e)], dtype="<i4").tobytes())
+
+
+def read_par_r(path: str, num_rep: int):
+    """(nstart, eol) from par_R.in: record 2k-1 holds nstart, record 2k the
+    replicas' energies, of which replica k's is element k."""
+    raw = open(path, "rb").read()
+    pos, recs = 0, []
+    while pos < len(raw):
+        n = int(np.frombuffer(raw, dtype="<i4", count=1, offset=pos)[0])
+        recs.append(raw[pos + 4:pos + 4 + n])
+        pos += n + 8
+    if len(recs) < 2 * num_rep:
+        raise ValueError("par_R.in holds fewer replicas than num_rep")
+    nstart = int(np.frombuffer(recs[0], dtype="<i4")[0])
+    eol = np.array([np.frombuffer(recs[2 * k + 1], dtype="<f8")[k] if len(recs[2 * k + 1]) >= 8 * (k + 1)
+                    else np.frombuffer(recs[2 * k + 1], dtype="<f8")[0] for k in range(num_rep)])
+    return nstart, eol
 
 
 def rgyr(nml: int) -> tuple[float, float]:

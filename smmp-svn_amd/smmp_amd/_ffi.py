@@ -66,7 +66,8 @@ def _load():
         "smmp_grnd": (C.c_int, [P, dp]),
         "smmp_outvar": (C.c_int, [P, C.c_int, C.c_char_p]),
         "smmp_metropolis": (C.c_int, [P, dp, dp, C.POINTER(Weight)]),
-        "smmp_partem_s": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp, ip, dp, dp]),
+        "smmp_partem_s": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp, ip, dp, dp, dp, dp]),
+        "smmp_redvar": (C.c_int, [P, C.c_char_p, dp]),
         "smmp_rgyr": (C.c_int, [P, C.c_int, dp, dp]),
         "smmp_energy_batch": (C.c_int, [C.POINTER(P), C.c_int, ip, C.c_int, dp, C.c_int, dp]),
         "smmp_sample": (C.c_int, [P, C.POINTER(SamplerCfg), dp, dp, dp, ip, ip, dp]),
@@ -95,7 +96,7 @@ lib = _load()
 #: every symbol the header declares (checked by the CPU tests)
 EXPORTS = ("smmp_ctx_create", "smmp_ctx_destroy", "smmp_last_error", "smmp_state_view", "smmp_state_count",
            "smmp_state_name", "smmp_init_energy", "smmp_init_molecule", "smmp_setvar", "smmp_energy",
-           "smmp_enysol", "smmp_sgrnd", "smmp_grnd", "smmp_outvar", "smmp_metropolis", "smmp_partem_s", "smmp_rgyr",
+           "smmp_enysol", "smmp_sgrnd", "smmp_grnd", "smmp_outvar", "smmp_metropolis", "smmp_partem_s", "smmp_rgyr", "smmp_redvar",
            "smmp_energy_batch", "smmp_sample", "smmp_build_info", "smmp_profile", "smmp_profile_read",
            "smmp_profile_phases", "smmp_muca_update", "smmp_muca_final", "smmp_sampler_info",
            "smmp_mlp_size", "smmp_mlp_set", "smmp_mlp_get", "smmp_mlp_grad", "smmp_mlp_fit")
@@ -252,21 +253,37 @@ class Context:
         self.check(lib.smmp_metropolis(self.h, C.byref(e), C.byref(a), C.byref(weight)))
         return e.value, a.value
 
-    def partem_s(self, temps, nequi: int, nswp: int, switch: int = 1, with_obs: bool = False):
+    def partem_s(self, temps, nequi: int, nswp: int, switch: int = 1, with_obs: bool = False, start=None,
+                 with_coor: bool = False):
         """(eol, ipoi, energies per sweep and slot[, radius of gyration and
-        end-to-end distance per sweep and slot])."""
+        end-to-end distance per sweep and slot][, final free variables per
+        replica]).  start (R x nvr free variables): restart from them (no
+        equilibration, no random start)."""
         temps = np.ascontiguousarray(temps, dtype=np.float64)
         R = len(temps)
         eol = np.zeros(R)
         ipoi = np.zeros(R, dtype=np.int32)
         traj = np.zeros(max(nswp, 1) * R)
         obs = np.zeros(max(nswp, 1) * R * 2) if with_obs else None
+        st = np.ascontiguousarray(start, dtype=np.float64).reshape(R, self.nvr) if start is not None else None
+        coor = np.zeros((R, self.nvr)) if with_coor else None
         rc = lib.smmp_partem_s(self.h, R, nequi, nswp, switch, dptr(temps), dptr(eol), iptr(ipoi), dptr(traj),
-                               dptr(obs) if obs is not None else None)
+                               dptr(obs) if obs is not None else None, dptr(st) if st is not None else None,
+                               dptr(coor) if coor is not None else None)
         self.check(rc)
+        out = [eol, ipoi, traj[:nswp * R].reshape(nswp, R)]
         if with_obs:
-            return eol, ipoi, traj[:nswp * R].reshape(nswp, R), obs[:nswp * R * 2].reshape(nswp, R, 2)
-        return eol, ipoi, traj[:nswp * R].reshape(nswp, R)
+            out.append(obs[:nswp * R * 2].reshape(nswp, R, 2))
+        if with_coor:
+            out.append(coor)
+        return tuple(out)
+
+    def redvar(self, path: str) -> np.ndarray:
+        """All variables after reading a .var file over the current ones
+        (the context is not changed)."""
+        v = np.zeros(self.nvt)
+        self.check(lib.smmp_redvar(self.h, path.encode(), dptr(v)))
+        return v
 
     def rgyr(self, nml: int = 0) -> tuple[float, float]:
         """(radius of gyration, C-alpha end-to-end distance) of the current coordinates."""

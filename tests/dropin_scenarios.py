@@ -92,6 +92,26 @@ def partem_gpu(args):
     return {"eol": [float(x) for x in eol], "time_d": open("time.d").read()}
 
 
+def partem_restart_gpu(args):
+    """smmp.partem_s for nswp sweeps, then again with newsta false and a larger
+    nswp: time.d of the second run starts at the stored sweep, the restart
+    files are rewritten."""
+    import smmp
+    import universe
+    import protein
+    os.chdir(args["dir"])
+    universe.Universe(T=300, st=1, libdir=LIBDIR)
+    protein.Protein(args["seq"])
+    smmp.sgrnd(args["seed"])
+    smmp.partem_s(args["nrep"], args["nequi"], args["nswp1"], 1, 1, True, args["switch"])
+    first = open("time.d").read()
+    nstart, eol1 = smmp.read_par_r("par_R.in", args["nrep"])
+    confs1 = [open(smmp.conf_file_name(k + 1)).read() for k in range(args["nrep"])]
+    eol, ipoi, traj = smmp.partem_s(args["nrep"], args["nequi"], args["nswp2"], 1, 1, False, args["switch"])
+    nstart2, eol2 = smmp.read_par_r("par_R.in", args["nrep"])
+    return {"first": first, "second": open("time.d").read(), "nstart": nstart, "nstart2": nstart2,
+            "eol1": [float(x) for x in eol1], "eol2": [float(x) for x in eol2], "eol": [float(x) for x in eol],
+            "confs1": confs1}
 
 
 def muca_gpu(args):

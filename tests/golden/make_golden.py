@@ -263,6 +263,16 @@ def esolan_mc_fixtures(tmp: str) -> None:
          seed=np.array([4242]), nequi=np.array([nequi]), nswp=np.array([nswp]), switch=np.array([-1]))
 
 
+def restart_fixture(tmp: str, nrep: int, nswp: int, eol_final) -> None:
+    """The restart files (par_R.in, conf_NNNN.var) of the partem_s run just
+    made in tmp, as the reference wrote them."""
+    confs = {f"conf_{k:04d}": np.frombuffer(open(os.path.join(tmp, f"conf_{k:04d}.var"), "rb").read(), dtype=np.uint8)
+             for k in range(1, nrep + 1)}
+    raw = open(os.path.join(tmp, "par_R.in"), "rb").read()
+    save("partem_restart_len10.npz", par_r=np.frombuffer(raw, dtype=np.uint8), eol_final=np.asarray(eol_final),
+         nswp=np.array([nswp]), **confs)
+
+
 def main(only: list[str]) -> None:
     if not os.path.exists(DRIVER):
         raise SystemExit("build the reference first: make -C oracle ref")
@@ -285,6 +295,16 @@ def main(only: list[str]) -> None:
         esolan_rotation_fixture(tmp)
     if not only or "esolan_mc" in only:
         esolan_mc_fixtures(tmp)
+    if only and "restart" in only:  # the len10 run of partem_ext_len10.npz again, for its files
+        seq = list(np.load(os.path.join(HERE, "topo_len10.npz"))["seq_in"])
+        seqf = workloads.write_seq_file(os.path.join(tmp, "pt_len10.seq"), seq, "len10")
+        temps = workloads.geometric_temperatures(4, 250.0, 1000.0)
+        workloads.write_temperatures(os.path.join(tmp, "temperatures"), temps)
+        run(["partem", LIBDIR, seqf, "1", "4242", "4", "3", "25", "-1"], tmp)
+        raw = open(os.path.join(tmp, "par_R.in"), "rb").read()
+        eol_final = np.frombuffer(raw[16:16 + 8 * 4], dtype="<f8").copy()
+        assert np.array_equal(eol_final, np.load(os.path.join(HERE, "partem_ext_len10.npz"))["eol_final"])
+        restart_fixture(tmp, 4, 25, eol_final)
     if only:
         return
 
@@ -356,6 +376,8 @@ def main(only: list[str]) -> None:
         eol_final = np.frombuffer(raw[16:16 + 8 * nrep], dtype="<f8").copy()
         save(f"partem_ext_{tag}.npz", temps=temps, time_d=td, eol_final=eol_final, seed=np.array([4242]),
              nequi=np.array([nequi]), nswp=np.array([nswp]), switch=np.array([-1]))
+        if tag == "len10":
+            restart_fixture(tmp, nrep, nswp, eol_final)
 
 
 if __name__ == "__main__":

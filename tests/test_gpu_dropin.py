@@ -44,6 +44,25 @@ def test_partem_s_files_vs_reference(tmp_path):
     assert np.abs(rows[:, 3:6] - p["time_d"][:, 3:6]).max() <= 6e-4
 
 
+def test_partem_s_restart_files(tmp_path):
+    """partem_s writes par_R.in and conf_NNNN.var; with newsta false it runs
+    sweeps nstart..nswp from those conformations (partem_s.f:97-111, 179-255)."""
+    from smmp_amd import workloads
+    seq = list(golden("topo_len10.npz")["seq_in"])
+    seqf = workloads.write_seq_file(str(tmp_path / "s.seq"), seq, "t")
+    temps = workloads.geometric_temperatures(3, 250.0, 1000.0)
+    workloads.write_temperatures(str(tmp_path / "temperatures"), temps)
+    out = run("partem_restart_gpu", {"dir": str(tmp_path), "seq": seqf, "seed": 4242, "nrep": 3, "nequi": 2,
+                                     "nswp1": 4, "nswp2": 7, "switch": -1})
+    assert out["nstart"] == 4 and out["nstart2"] == 7
+    first = [line.split() for line in out["first"].splitlines()]
+    second = [line.split() for line in out["second"].splitlines()]
+    assert [int(r[0]) for r in first] == [s for s in range(1, 5) for _ in range(3)]
+    assert [int(r[0]) for r in second] == [s for s in range(4, 8) for _ in range(3)]
+    assert np.allclose(out["eol2"], out["eol"])
+    assert all(len(text.splitlines()) > 0 for text in out["confs1"])
+
+
 def test_mulcan_par_matches_oracle_iteration(tmp_path):
     """smmp.multicanonical.mulcan_par (GPU sweeps + host recursion) =
     the same iteration driven by the oracle's Metropolis and recursion:

@@ -349,3 +349,21 @@ def test_sample_observables_match_the_final_coordinates():
         rgy, ee = c.rgyr(0)
         assert abs(out["obs"][-1, r, 0] - rgy) <= 1e-9 * max(1.0, rgy), r
         assert abs(out["obs"][-1, r, 1] - ee) <= 1e-9 * max(1.0, ee), r
+
+
+def test_partem_s_restart_continues_the_run():
+    """One replica (no exchanges): 3 sweeps, then a restart from the returned
+    variables for 2 more on the same twister, equal the 5 sweeps of one run
+    (the sweep-end energy and coordinates are full recomputations, so a
+    restart from the variables loses nothing)."""
+    c = make_ctx(tag="len10", itysol=1)
+    T = np.array([400.0])
+    c.sgrnd(99)
+    say("smmp_partem_s len10 one replica: 3 sweeps, restart for 2, and 5 in one run")
+    _, _, tra, ca = c.partem_s(T, 2, 3, -1, with_coor=True)
+    eb, _, trb, cb = c.partem_s(T, 0, 2, -1, start=ca, with_coor=True)
+    c.sgrnd(99)
+    ec, _, trc, cc = c.partem_s(T, 2, 5, -1, with_coor=True)
+    assert np.array_equal(tra, trc[:3])
+    assert np.array_equal(trb, trc[3:])
+    assert np.array_equal(cb, cc) and np.array_equal(eb, ec)
