@@ -112,6 +112,11 @@ int smmp_redvar(smmp_ctx *ctx, const char *path, double *vlvr);
  * and last residue, for the context's current coordinates. */
 int smmp_rgyr(smmp_ctx *ctx, int nml, double *rgy, double *ee);
 
+/* helix(nhel,mhel,nbet,mbet) helix.f:13: out[4] = helical residues, helical
+ * segments, sheet-like residues, sheet-like segments of the context's
+ * current variables. */
+int smmp_helix(smmp_ctx *ctx, int32_t *out);
+
 /* ---- batched entry points (no reference counterpart: the configs'
  *      many-replica and many-conformation workloads) ------------------- */
 

@@ -12,6 +12,7 @@
 !   canon  libdir seqfile varfile itysol seed temp nsweep  Metropolis run
 !   partem libdir seqfile itysol seed nrep nequi nswp [switch]  partem_s run
 !   muca   libdir seqfile itysol seed nsweep nup temp kmin kmax ebin
+!   helix  libdir seqfile itysol seed nconf         helix() of drawn conformations
 !   esolan libdir seqfile radfile seed nconf       analytic solvent (itysol -1):
 !          energies of the start and of random conformations, and the
 !          twister position after each (esolan.f draws on degenerate geometry);
@@ -31,7 +32,7 @@ program ref_driver
   character(len=80) :: libdir, seqf, varf, radf
   character(len=4) :: grpn, grpc
   integer :: iseed, n, i, j, k, nconf, nsw, isol, nrep, nequi, nswp
-  integer :: kmn, kmx, nup, iswt, ios
+  integer :: kmn, kmx, nup, iswt, ios, nhel, mhel, nbet, mbet
   double precision :: vrad
   double precision :: e, eolx, acz, tmp, ebw
   double precision :: ggasc
@@ -78,6 +79,41 @@ program ref_driver
       end do
       call dump_energy(e)
       if (k .le. 2) call dump_coords()
+    end do
+
+  case ('helix')
+    ! helix(nhel,mhel,nbet,mbet) of conformations whose free variables are
+    ! drawn near the helical (-70,-37) or sheet (-150,150) limits or at random
+    call setup(2, .false.)
+    call iarg(5, iseed)
+    call iarg(6, nconf)
+    call sgrnd(iseed)
+    write(*,'(a,i0)') '@nconf int 1'
+    write(*,'(i0)') nconf
+    do k = 1, nconf
+      do i = 1, nvr
+        j = idvr(i)
+        tmp = grnd()
+        if (tmp .lt. 0.35d0) then
+          vlvr(j) = (-70.0d0 + 70.0d0 * (grnd() - 0.5d0)) * cdr
+        else if (tmp .lt. 0.7d0) then
+          vlvr(j) = (-150.0d0 + 70.0d0 * (grnd() - 0.5d0)) * cdr
+        else if (tmp .lt. 0.85d0) then
+          vlvr(j) = (150.0d0 + 70.0d0 * (grnd() - 0.5d0)) * cdr
+        else
+          vlvr(j) = addang(pi, axvr(j)*(grnd()-0.5))
+        end if
+      end do
+      call helix(nhel, mhel, nbet, mbet)
+      write(*,'(a,i0)') '@vlvr double ', nvr
+      do i = 1, nvr
+        write(*,'(es25.17)') vlvr(idvr(i))
+      end do
+      write(*,'(a,i0)') '@helix int 4'
+      write(*,'(i0)') nhel
+      write(*,'(i0)') mhel
+      write(*,'(i0)') nbet
+      write(*,'(i0)') mbet
     end do
 
   case ('canon')

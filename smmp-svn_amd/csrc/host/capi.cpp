@@ -252,6 +252,15 @@ int smmp_redvar(smmp_ctx* c, const char* path, double* vlvr) {
   });
 }
 
+int smmp_helix(smmp_ctx* c, int32_t* out) {
+  return guarded(c, SMMP_ESTATE, [&] {
+    if (!out) throw Error("helix> NULL argument");
+    if (c->s.ntlml < 1) throw Error("helix> no molecule initialised");
+    helix(c->s, out);
+    return SMMP_OK;
+  });
+}
+
 int smmp_rgyr(smmp_ctx* c, int nml, double* rgy, double* ee) {
   return guarded(c, SMMP_ESTATE, [&] {
     if (!rgy || !ee) throw Error("rgyr> NULL argument");

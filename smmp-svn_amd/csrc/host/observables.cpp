@@ -43,4 +43,46 @@ void rgyr(const State& s, int nml, double* rgy, double* ee) {
   *ee = std::sqrt(ex * ex + ey * ey + ez * ez);
 }
 
+// helix.f:13-77: residues whose (phi, psi) lie within 30 degrees of
+// (-70, -37) (helical) or (-150, 150) (sheet-like), and the segments they
+// form, over the free variables; psi is the free variable after phi.
+void helix(const State& s, int32_t out[4]) {
+  constexpr double hlim = 30.0, philim = -70.0, psilim = -37.0;
+  constexpr double hlim2 = 30.0, philim2 = -150.0, psilim2 = 150.0;
+  int nhel = 0, mhel = 0, nbet = 0, mbet = 0;
+  bool lhel = false, lbet = false;
+  for (int i = 1; i <= s.nvr; ++i) {
+    const int iv = s.idvr(i);
+    if (s.nmvr.get(iv) != "phi") continue;
+    const double xphi = s.vlvr(iv) * crd;
+    const double xpsi = s.vlvr(s.idvr(i + 1)) * crd;
+    if (std::fabs(xphi - philim) <= hlim) {
+      lbet = false;
+      if (std::fabs(xpsi - psilim) <= hlim) {
+        nhel = nhel + 1;
+        if (!lhel) mhel = mhel + 1;
+        lhel = true;
+      } else {
+        lhel = false;
+      }
+    } else if (std::fabs(xphi - philim2) <= hlim2) {
+      lhel = false;
+      if (std::fabs(xpsi - psilim2) <= hlim2) {
+        nbet = nbet + 1;
+        if (!lbet) mbet = mbet + 1;
+        lbet = true;
+      } else {
+        lbet = false;
+      }
+    } else {
+      lhel = false;
+      lbet = false;
+    }
+  }
+  out[0] = nhel;
+  out[1] = mhel;
+  out[2] = nbet;
+  out[3] = mbet;
+}
+
 }  // namespace smmp

@@ -149,6 +149,7 @@ void init_molecule(State& s, int iabin, const std::string& grpn, const std::stri
                    const std::string& seqfile, const std::string& varfile);  // init_molecule.f:13-219
 void setvar(State& s, int nml, const double* vlvrx);  // setvar.f:13-73 (1-based vlvrx(1..))
 void rgyr(const State& s, int nml, double* rgy, double* ee);  // rgyr.f:13-140 (rgy, ee)
+void helix(const State& s, int32_t out[4]);                   // helix.f:13-77 (nhel, mhel, nbet, mbet)
 void bldmol(State& s, int nml);             // bldmol.f:13-93
 // fnd3ba (bldmol.f:95-173) on the current topology; ixrfpt keeps the atoms
 // of the last bldmol call, which precede addend's renumbering after init.

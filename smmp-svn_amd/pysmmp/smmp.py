@@ -14,9 +14,9 @@ Differences from the f2py module (each stated where it applies):
     ``smmp_amd.weights``; other callables are rejected, because the weight is
     evaluated inside the GPU sweep;
   * ``partem_s`` is exported (partem_s.f:13 has no f2py entry);
-  * observables and minimisers (helix, hbond, contacts, zimmer, outpdb,
-    minim, rmsdfun, anneal) are outside the energy-and-move engine and raise
-    NotImplementedError; rgyr is provided.
+  * observables and minimisers (hbond, contacts, zimmer, outpdb, minim,
+    rmsdfun, anneal) are outside the energy-and-move engine and raise
+    NotImplementedError; rgyr and helix are provided.
 """
 from __future__ import annotations
 
@@ -213,6 +213,11 @@ def read_par_r(path: str, num_rep: int):
     return nstart, eol
 
 
+def helix() -> tuple[int, int, int, int]:
+    """smmp.pyf helix() -> (nhel, mhel, nbet, mbet), helix.f:13."""
+    return _ctx.helix()
+
+
 def rgyr(nml: int) -> tuple[float, float]:
     """smmp.pyf rgyr(nml) -> (rgy, ee), rgyr.f:13: radius of gyration and
     C-alpha end-to-end distance of chain nml (0: all) at the current
@@ -227,7 +232,7 @@ def _outside(name: str):
     return f
 
 
-for _n in ("helix", "hbond", "interhbond", "contacts", "zimmer", "outpdb", "minim", "rmsdfun",
+for _n in ("hbond", "interhbond", "contacts", "zimmer", "outpdb", "minim", "rmsdfun",
            "anneal", "canon", "regul", "gradient", "esolan", "eninteract"):
     globals()[_n] = _outside(_n)
 

@@ -69,6 +69,7 @@ def _load():
         "smmp_partem_s": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp, ip, dp, dp, dp, dp]),
         "smmp_redvar": (C.c_int, [P, C.c_char_p, dp]),
         "smmp_rgyr": (C.c_int, [P, C.c_int, dp, dp]),
+        "smmp_helix": (C.c_int, [P, ip]),
         "smmp_energy_batch": (C.c_int, [C.POINTER(P), C.c_int, ip, C.c_int, dp, C.c_int, dp]),
         "smmp_sample": (C.c_int, [P, C.POINTER(SamplerCfg), dp, dp, dp, ip, ip, dp]),
         "smmp_build_info": (C.c_char_p, []),
@@ -96,7 +97,7 @@ lib = _load()
 #: every symbol the header declares (checked by the CPU tests)
 EXPORTS = ("smmp_ctx_create", "smmp_ctx_destroy", "smmp_last_error", "smmp_state_view", "smmp_state_count",
            "smmp_state_name", "smmp_init_energy", "smmp_init_molecule", "smmp_setvar", "smmp_energy",
-           "smmp_enysol", "smmp_sgrnd", "smmp_grnd", "smmp_outvar", "smmp_metropolis", "smmp_partem_s", "smmp_rgyr", "smmp_redvar",
+           "smmp_enysol", "smmp_sgrnd", "smmp_grnd", "smmp_outvar", "smmp_metropolis", "smmp_partem_s", "smmp_rgyr", "smmp_redvar", "smmp_helix",
            "smmp_energy_batch", "smmp_sample", "smmp_build_info", "smmp_profile", "smmp_profile_read",
            "smmp_profile_phases", "smmp_muca_update", "smmp_muca_final", "smmp_sampler_info",
            "smmp_mlp_size", "smmp_mlp_set", "smmp_mlp_get", "smmp_mlp_grad", "smmp_mlp_fit")
@@ -277,6 +278,12 @@ class Context:
         if with_coor:
             out.append(coor)
         return tuple(out)
+
+    def helix(self) -> tuple[int, int, int, int]:
+        """(helical residues, helical segments, sheet residues, sheet segments)."""
+        out = np.zeros(4, dtype=np.int32)
+        self.check(lib.smmp_helix(self.h, iptr(out)))
+        return tuple(int(x) for x in out)
 
     def redvar(self, path: str) -> np.ndarray:
         """All variables after reading a .var file over the current ones

@@ -295,6 +295,13 @@ def main(only: list[str]) -> None:
         esolan_rotation_fixture(tmp)
     if not only or "esolan_mc" in only:
         esolan_mc_fixtures(tmp)
+    if not only or "helix" in only:  # helix.f on the 20-mer and the 10-mer
+        for tag in ("c2_20mer", "len10"):
+            seq = list(np.load(os.path.join(HERE, f"topo_{tag}.npz"))["seq_in"])
+            seqf = workloads.write_seq_file(os.path.join(tmp, f"hx_{tag}.seq"), seq, tag)
+            out = run(["helix", LIBDIR, seqf, "1", "31", "40"], tmp)
+            save(f"helix_{tag}.npz", seq_in=np.array(seq, dtype="<U8"), vlvr=np.stack(out["vlvr"]),
+                 helix=np.stack(out["helix"]))
     if only and "restart" in only:  # the len10 run of partem_ext_len10.npz again, for its files
         seq = list(np.load(os.path.join(HERE, "topo_len10.npz"))["seq_in"])
         seqf = workloads.write_seq_file(os.path.join(tmp, "pt_len10.seq"), seq, "len10")
