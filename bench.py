@@ -259,7 +259,16 @@ def run_esolan(args):
         f"{ke['ms'] / K:.2f} ms; per conformation: scan {wk[0] / K / n:.0f}, circle pairs {wk[1] / K / n:.0f}, "
         f"arcs {wk[2] / K / n:.0f}, coverage tests {wk[3] / K / n:.0f}")
     log(f"eysl mean {out[:, 6].mean():.3f}, total mean {out[:, 0].mean():.3f}")
+    # the replica sampler with the analytic solvent: the replicas' trials in
+    # lockstep, one device energy pass per trial step (parallel tempering)
+    nsw2 = 2
+    t0 = time.perf_counter()
+    ca.sample(temps, np.arange(900, 900 + NREP + 1, dtype=np.uint32), nsw2, V[:NREP], mode=1, with_traj=False)
+    torch.cuda.synchronize()
+    t_smp = time.perf_counter() - t0
+    log(f"analytic-solvent parallel tempering: {NREP} replicas x {nsw2} sweeps in {t_smp:.2f} s")
     cpu = None
+    smp_cpu = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(REPO, "tests"))
         from _oracle import OracleMT, OracleSystem
@@ -273,6 +282,9 @@ def run_esolan(args):
         cpu = {"value": round(m / tc, 3), "unit": "energy evaluations/s", "cores": 1, "kind": "port",
                "sample": f"the first {m} of the same conformations through the C oracle (ECEPP/3 + esolan "
                          f"restatement), one host core, {tc:.1f} s"}
+        t1 = time.perf_counter()
+        o.metropolis(OracleMT(5), V[0], 1.0 / (TMIN * 1.98773e-3), 1)
+        smp_cpu = round(1.0 / (time.perf_counter() - t1), 4)
     line = {
         "metric": "energy evaluations/s (ECEPP/3 + analytic solvent)",
         "value": round(K * n / dt, 2),
@@ -293,6 +305,9 @@ def run_esolan(args):
                      "frac": round(achieved / PEAK_FP64, 5), "traffic": None,
                      "kernel": "k_esolan (both passes) + k_esolan_sum"},
         "cpu_baseline": cpu,
+        "sampler": {"replica_sweeps_per_s": round(NREP * nsw2 / t_smp, 2), "replicas": NREP, "sweeps": nsw2,
+                    "mode": "parallel tempering (smmp_sample mode 1), analytic solvent",
+                    "cpu_oracle_replica_sweeps_per_s": smp_cpu},
     }
     print(json.dumps(line), flush=True)
 

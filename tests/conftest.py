@@ -69,14 +69,16 @@ def write_multi_seq(path: str, chains) -> str:
     return path
 
 
-def make_multi_ctx(tag: str):
+def make_multi_ctx(tag: str, itysol=None):
     """Context of a multi-chain fixture (tests/golden/multi_<tag>.npz) with
-    the fixture's global parameters and box; returns (context, fixture)."""
+    the fixture's global parameters and box (the fixture's solvent unless
+    itysol is given); returns (context, fixture)."""
     import tempfile
     g = golden(f"multi_{tag}.npz")
     chains = [str(c).split("-") for c in g["chains"]]
     d = tempfile.mkdtemp(prefix="smmp_m_")
-    c = make_ctx(itysol=int(g["itysol_in"][0]), seqfile=write_multi_seq(os.path.join(d, "m.seq"), chains))
+    c = make_ctx(itysol=int(g["itysol_in"][0]) if itysol is None else itysol,
+                 seqfile=write_multi_seq(os.path.join(d, "m.seq"), chains))
     gb = c.view("mol_r.gbpr")
     nml = len(chains)
     gb[:, :nml] = np.asarray(g["gbpr"]).reshape(nml, 6).T

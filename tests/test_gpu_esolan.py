@@ -250,3 +250,35 @@ def test_batched_parallel_tempering_matches_oracle():
     assert list(out["ipoi"]) == ipoi
     assert np.array_equal(out["acc"], acc)
     assert np.all(np.abs(out["traj"] - traj) <= RTOL * np.maximum(1.0, np.abs(traj)))
+
+
+def test_several_chains_energy_and_metropolis_match_oracle():
+    """Two chains with the analytic solvent: esolan(0) over the atoms of all
+    chains (energy.f:79-82), and smmp_metropolis with the chains' rigid moves
+    (metropolis.f:94-168) on the context's twister, against the oracle."""
+    from conftest import make_multi_ctx
+    from smmp_amd import _ffi
+    c, g = make_multi_ctx("enk2", itysol=-1)
+    o = OracleSystem(c)
+    rng = np.random.default_rng(2)
+    v = c.vlvr()
+    v[c.idvr - 1] = np.pi + rng.normal(0, 0.3, c.nvr)
+    c.set_vlvr(v)
+    say("smmp_energy two chains itysol -1")
+    e = c.energy()
+    ref = o.energy(v, OracleMT(1))
+    assert _close(e, ref[0]) and _close(_components(c)[5], ref[6])
+    c.sgrnd(77)
+    w = _ffi.Weight()
+    w.kind, w.beta = 0, 1.0 / (400.0 * GASC)
+    say("2 x smmp_metropolis two chains itysol -1")
+    eol, acz = e, 0.0
+    mt = OracleMT(77)
+    o.reset()
+    vl, tr = o.metropolis(mt, v, w.beta, 2)
+    for k in range(2):
+        eol, acz = c.metropolis(eol, acz, w)
+        assert _close(eol, tr[k][0]) and acz == tr[k][1], (k, eol, tr[k])
+    assert np.abs(c.vlvr() - vl).max() < 1e-9
+    assert np.abs(np.asarray(c.view("mol_r.gbpr"))[:, :2].T.ravel() - o.gbpr()[:12]).max() < 1e-9
+    assert int(c.view("myblock.mti")[0]) == mt.s.mti
