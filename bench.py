@@ -33,6 +33,7 @@ sys.path.insert(0, PKG)
 NREP, LENGTH, SEQ_SEED = 64, 20, 2
 TMIN, TMAX = 250.0, 1000.0
 PEAK_FP64 = 78.6  # TFLOP/s, MI355X FP64 vector
+PEAK_FP32 = 157.3  # TFLOP/s, MI355X FP32 vector
 # algorithmic flops per unit of work (divide and sqrt count 1):
 FLOP_PAIR = 22      # ECEPP pair term: 3 sub + 5 r^2 + 1 rsqrt + 3 powers + 2 Coulomb + 4 LJ/HB + 3 sums
 FLOP_POINT = 9      # one tessellation point against one solvent neighbour
@@ -312,6 +313,241 @@ def run_esolan(args):
     print(json.dumps(line), flush=True)
 
 
+def run_c3(args):
+    """Batched energy-only evaluation: 4,096 conformations over 256 random
+    sequences of 10-60 residues (16 conformations each, dihedrals uniform in
+    [-pi, pi)), ECEPP/3 + OONS.  Each timed step is one smmp_batch_run over
+    all of them with the conformations resident on the device (the same
+    through smmp_energy_batch from host buffers is reported beside it); the
+    headline is the mixed-precision path (FP32 pair terms,
+    FP64 accumulation), with every structure's error against the FP64 path
+    reported, and FP64 throughput beside it."""
+    from smmp_amd import _ffi, workloads
+    nseq, per = args.c3_seqs, args.c3_per_seq
+    tmp = tempfile.mkdtemp(prefix="smmp_c3_")
+    seqs = workloads.c3_sequences(nseq, 10, 60, seed=3)
+    t0 = time.perf_counter()
+    ctxs = []
+    for i, s in enumerate(seqs):
+        seqf = workloads.write_seq_file(os.path.join(tmp, f"s{i}.seq"), s, f"s{i}")
+        c = _ffi.Context()
+        for k, v in (("epar_l.flex", 0), ("epar_l.sh2", 0), ("epar_l.epsd", 0), ("epar_l.ientyp", 0),
+                     ("isolty.itysol", 1)):
+            c.view(k)[0] = v
+        c.init_energy(workloads.default_libdir())
+        c.init_molecule(1, "nh2", "cooh", seqf, " ")
+        ctxs.append(c)
+    nat = np.array([int(c.view("res_i.iatrs2")[len(s) - 1]) for c, s in zip(ctxs, seqs)])
+    log(f"C3: {nseq} sequences ({min(map(len, seqs))}-{max(map(len, seqs))} residues, {nat.min()}-{nat.max()} atoms, "
+        f"mean {nat.mean():.0f}) set up in {time.perf_counter() - t0:.1f} s")
+    rows, cs = [], []
+    for i, c in enumerate(ctxs):
+        v = np.tile(c.vlvr(), (per, 1))
+        v[:, c.idvr - 1] = workloads.uniform_dihedrals(per, c.nvr, 1000 + i)
+        rows.append(v.ravel())
+        cs += [i] * per
+    V = np.concatenate(rows)
+    cs = np.asarray(cs, dtype=np.int32)
+    n = len(cs)
+    K = args.steps
+    B = _ffi.EnergyBatch(ctxs, cs)  # systems uploaded once; conformations resident
+    B.load(V)
+    res = {}
+    for prec in (1, 0):
+        B.run(prec)
+        B.read()  # untimed first run
+        ctxs[0].profile(True)
+        t0 = time.perf_counter()
+        for _ in range(K):
+            B.run(prec)
+        out = B.read()
+        dt = time.perf_counter() - t0
+        ke = ctxs[0].profile_read(_ffi.KERNEL_ENERGY)
+        ctxs[0].profile(False)
+        # the host-buffer entry point (conformations in, energies out over PCIe each call)
+        _ffi.energy_batch(ctxs, cs, V, prec)
+        t1 = time.perf_counter()
+        for _ in range(K):
+            _ffi.energy_batch(ctxs, cs, V, prec)
+        dth = time.perf_counter() - t1
+        res[prec] = (out, dt, ke, dth)
+        log(f"precision {prec}: {K} resident runs {1e3 * dt / K:.2f} ms each, k_energy {ke['ms'] / K:.2f} ms per batch; "
+            f"smmp_energy_batch from host buffers {1e3 * dth / K:.2f} ms per call")
+    (om, dtm, kem, dthm), (od, dtd, ked, dthd) = res[1], res[0]
+    err = np.abs(om[:, 0] - od[:, 0])
+    rel = err / np.maximum(np.abs(od[:, 0]), 1.0)
+    # the mixed mode's stated bound: 2e-5 x (|E_el| + |E_vw| + |E_hb|) + 1e-3
+    bound = 2e-5 * (np.abs(od[:, 2]) + np.abs(od[:, 3]) + np.abs(od[:, 4])) + 1e-3
+    log(f"mixed vs FP64 per structure: max |dE| {err.max():.3e}, max rel {rel.max():.3e}, median rel "
+        f"{np.median(rel):.3e}; within bound {np.mean(err <= bound):.4f}")
+
+    def roof(ke, peak):
+        wk = ke["work"]
+        flops = FLOP_PAIR * wk[0] + FLOP_POINT * 122 * wk[1] + FLOP_BUILD * wk[2]
+        a = flops / (ke["ms"] * 1e-3) / 1e12
+        return flops / K, a, {"achieved": round(a, 4), "peak": peak, "frac": round(a / peak, 5)}
+
+    fl, am, rm = roof(kem, PEAK_FP32)
+    _, ad, rd = roof(ked, PEAK_FP64)
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(REPO, "tests"))
+        from _oracle import OracleSystem
+        oracles = {}
+        m, worst, t1 = 0, 0.0, time.perf_counter()
+        step = max(1, n // 512)  # spread the sample over the sequences
+        for k in range(0, n, step):
+            if time.perf_counter() - t1 >= 15.0:
+                break
+            s = int(cs[k])
+            o = oracles.get(s) or oracles.setdefault(s, OracleSystem(ctxs[s]))
+            off = int(sum(ctxs[int(j)].nvt for j in cs[:k]))
+            e = o.energy(V[off:off + ctxs[s].nvt])
+            worst = max(worst, abs(od[k, 0] - e[0]) / max(abs(e[0]), 1.0))
+            m += 1
+        tc = time.perf_counter() - t1
+        cpu = {"value": round(m / tc, 3), "unit": "energy evaluations/s", "cores": 1, "kind": "port",
+               "sample": f"{m} of the conformations (every {step}th) through the C oracle (FP64), one host core, "
+                         f"{tc:.1f} s (incl. system set-up); GPU FP64 vs oracle max rel error {worst:.2e}"}
+    line = {
+        "metric": "energy evals/s",
+        "value": round(K * n / dtm, 1),
+        "unit": "energy evaluations/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": 1,
+        "ms_per_step": round(1e3 * dtm / K, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 pair terms + f64 accumulation",
+        "data": f"synthetic: {nseq} random sequences of 10-60 residues (seed 3), {per} conformations each with "
+                "dihedrals uniform in [-pi, pi)",
+        "config": {"workload": f"energies of {n} resident conformations (smmp_batch_run) over {nseq} sequences of "
+                               "uneven length, ECEPP/3 + OONS", "conformations": n, "sequences": nseq,
+                   "atoms_min_mean_max": [int(nat.min()), round(float(nat.mean()), 1), int(nat.max())]},
+        "fp64": {"value": round(K * n / dtd, 1), "ms_per_step": round(1e3 * dtd / K, 3),
+                 "kernel_ms": round(ked["ms"] / K, 3), "roofline": rd,
+                 "host_buffer_calls_per_s": round(K * n / dthd, 1)},
+        "host_buffer_calls_evals_per_s": round(K * n / dthm, 1),
+        "error_mixed_vs_fp64": {"max_abs": float(f"{err.max():.4e}"), "max_rel": float(f"{rel.max():.4e}"),
+                                "median_rel": float(f"{np.median(rel):.4e}"),
+                                "frac_within_bound": round(float(np.mean(err <= bound)), 5)},
+        "roofline": {"bound": "valu-fp32", "unit": "TFLOP/s", "traffic": None, "kernel": "k_energy<mixed>",
+                     "avg_launch_ms": round(kem["ms"] / max(kem["launches"], 1), 4),
+                     "flop_per_launch": round(fl / max(kem["launches"] / K, 1), 1), **rm},
+        "kernel_evals_per_s": round(K * n / (kem["ms"] * 1e-3), 1),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def run_muca(args):
+    """Multicanonical sampling: 128 independent replicas of a random
+    36-residue sequence (seed 5), ECEPP/3 in vacuo, 200 energy bins.  The
+    weights come from mulcan_par-style iterations on the device sampler
+    (every replica's sweep-end histogram pooled per update, Berg's recursion
+    in smmp_muca_update, muca.d tables from smmp_muca_final); the timed run
+    samples with those weights (muca_weight2) in mixed precision (FP32 pair
+    terms of full recomputations, FP64 chain, trial changes and decisions)
+    with the sweeps replayed from a captured HIP graph."""
+    from smmp_amd import _ffi, workloads
+    R, L, NB = 128, 36, 200
+    tmp = tempfile.mkdtemp(prefix="smmp_muca_")
+    c, seq, _ = make_system(tmp, length=L, seed=5, tag="c4", itysol=0)
+    seeds = lambda k: np.arange(k, k + R + 1, dtype=np.uint32)  # noqa: E731
+    log(f"multicanonical: {R} replicas x {L} residues, {c.nvr} free torsions, ECEPP/3 in vacuo")
+    temps = workloads.geometric_temperatures(R, TMIN, TMAX)
+    t0 = time.perf_counter()
+    d = c.sample(temps, seeds(1), args.mu_warmup, random_starts(c, R, 12345), mode=1, precision=1)
+    e = d["traj"][args.mu_warmup // 2:]
+    lo, hi = float(np.quantile(e[:, 0], 0.01)), float(np.quantile(e[:, -1], 0.99))
+    ebin = (hi - lo) / (NB - 2)
+    kmin = int(math.floor(lo / ebin)) - 1
+    kmax = kmin + NB - 1
+    log(f"warm-up {args.mu_warmup} PT sweeps in {time.perf_counter() - t0:.1f} s: E {lo:.1f} .. {hi:.1f}, "
+        f"ebin {ebin:.3f}, bins {kmin}..{kmax}")
+    state = d["vlvr"]
+    # weight iteration (mulcan_par_mod.f90:23-190), sweeps at (beta + b(k)) x + alpha(k)
+    beta_t = 1.0 / (TMAX * 1.98773e-3)
+    b, alpha, xhist, g1, ent = (np.zeros(NB) for _ in range(5))
+    ih = np.zeros(NB, dtype=np.int32)
+    t0 = time.perf_counter()
+    for it in range(args.mu_iters):
+        w = _ffi.Weight(kind=1, beta=beta_t, kmin=kmin, kmax=kmax, ebin=ebin, b=_ffi.dptr(b), alpha=_ffi.dptr(alpha))
+        d = c.sample(np.full(R, TMAX), seeds(100 + 3 * it), args.mu_nup, state, mode=2, weight=w, precision=1,
+                     with_traj=False)
+        state = d["vlvr"]
+        ih[:] = d["hist"]
+        _ffi.lib.smmp_muca_update(kmin, kmax, ebin, beta_t, _ffi.iptr(ih), _ffi.dptr(xhist), _ffi.dptr(g1),
+                                  _ffi.dptr(b), _ffi.dptr(alpha), _ffi.dptr(ent))
+    _ffi.lib.smmp_muca_final(kmin, kmax, ebin, beta_t, _ffi.dptr(b), _ffi.dptr(alpha))
+    log(f"{args.mu_iters} weight updates of {args.mu_nup} sweeps x {R} replicas in {time.perf_counter() - t0:.1f} s; "
+        f"bins visited {np.mean(xhist > 0):.2f}")
+    w2 = _ffi.Weight(kind=2, beta=0.0, kmin=kmin, kmax=kmax, ebin=ebin, b=_ffi.dptr(b), alpha=_ffi.dptr(alpha))
+    K, W = args.steps, args.warmup
+    g = args.graph_sweeps or math.gcd(K, W) or K
+    kw = dict(mode=2, weight=w2, precision=1, with_traj=False, graph_sweeps=g)
+    state = c.sample(np.full(R, TMAX), seeds(500), W, state, **kw)["vlvr"]
+    c.profile(1)
+    t0 = time.perf_counter()
+    out = c.sample(np.full(R, TMAX), seeds(600), K, state, **kw)
+    dt = time.perf_counter() - t0
+    ks = c.profile_read(_ffi.KERNEL_SWEEP)
+    info = c.sampler_info()
+    c.profile(False)
+    hist = out["hist"].astype(np.float64)
+    vis = hist > 0
+    span = np.flatnonzero(vis)
+    flat = float(hist[vis].min() / hist[vis].mean()) if vis.any() else 0.0
+    flops = FLOP_PAIR * ks["pairs"] + FLOP_POINT * 122 * ks["contacts"] + FLOP_BUILD * ks["builds"] \
+        + FLOP_ROTATE * ks["moved"]
+    achieved = flops / (ks["ms"] * 1e-3) / 1e12
+    log(f"timed {K} sweeps: {R * K / dt:.1f} sweeps/s; kernel {ks['launches']} launches, {ks['ms']:.1f} ms; "
+        f"graph replays {info['graph_replays']}; bins visited {vis.mean():.2f} ({span[0] if vis.any() else -1}.."
+        f"{span[-1] if vis.any() else -1}), min/mean {flat:.3f}")
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(REPO, "tests"))
+        from _oracle import OracleMT, OracleSystem
+        o = OracleSystem(c)
+        m, t1 = 0, time.perf_counter()
+        while time.perf_counter() - t1 < 15.0 and m < R:
+            o.metropolis(OracleMT(7 + m), state[m], 1.0 / (TMAX * 1.98773e-3), 1)
+            m += 1
+        tc = time.perf_counter() - t1
+        cpu = {"value": round(m / tc, 4), "unit": "sweeps/s", "cores": 1, "kind": "port",
+               "sample": f"{m} single sweeps of the C oracle's Metropolis (FP64, canonical weight at {TMAX:g} K: the "
+                         f"same trial work as a multicanonical sweep) from the replicas' start states, one host "
+                         f"core, {tc:.1f} s"}
+    line = {
+        "metric": "MC sweeps/s over all replicas",
+        "value": round(R * K / dt, 2),
+        "unit": "sweeps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": round(1e3 * dt / K, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 pair terms of full recomputations + f64 chain, trial changes and decisions",
+        "data": "synthetic: random 36-residue sequence (seed 5), random initial dihedrals, weights iterated on the "
+                "device sampler",
+        "config": {"workload": f"multicanonical sampling, {R} independent replicas x {L}-residue random sequence, "
+                               f"ECEPP/3 in vacuo, {NB} energy bins, hipGraph replay of {g} sweeps",
+                   "replicas": R, "residues": L, "free_torsions": c.nvr, "bins": NB, "ebin": round(ebin, 4),
+                   "kmin": kmin, "weight_updates": args.mu_iters, "sweeps_per_update": args.mu_nup,
+                   "workgroups_per_replica": info["groups"], "graph_replays": info["graph_replays"]},
+        "histogram": {"bins_visited": round(float(vis.mean()), 3), "min_over_mean_visited": round(flat, 4)},
+        "roofline": {"bound": "valu", "achieved": round(achieved, 4), "peak": PEAK_FP64, "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK_FP64, 5), "traffic": None, "kernel": "k_sweep<mixed>",
+                     "avg_launch_ms": round(ks["ms"] / max(ks["launches"], 1), 4)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -339,6 +575,16 @@ def main():
                          "--steps sets the timed batches")
     ap.add_argument("--es-warmup", type=int, default=200, help="sampling sweeps before the snapshots")
     ap.add_argument("--es-confs", type=int, default=1024, help="conformations per batch")
+    ap.add_argument("--c3", action="store_true",
+                    help="batched energies of 4,096 conformations over 256 sequences of 10-60 residues; "
+                         "--steps sets the timed batches")
+    ap.add_argument("--muca", action="store_true",
+                    help="multicanonical sampling, 128 replicas x 36 residues, 200 bins, mixed precision, graph replay")
+    ap.add_argument("--mu-warmup", type=int, default=200, help="parallel-tempering sweeps that set the energy range")
+    ap.add_argument("--mu-iters", type=int, default=20, help="weight updates")
+    ap.add_argument("--mu-nup", type=int, default=50, help="sweeps per weight update")
+    ap.add_argument("--c3-seqs", type=int, default=256)
+    ap.add_argument("--c3-per-seq", type=int, default=16)
     ap.add_argument("--phases", action="store_true",
                     help="also time the sweep kernel's phases on the device clock (adds barriers)")
     args = ap.parse_args()
@@ -355,6 +601,12 @@ def main():
         return
     if args.esolan:
         run_esolan(args)
+        return
+    if args.c3:
+        run_c3(args)
+        return
+    if args.muca:
+        run_muca(args)
         return
     if args.copy_probe:
         x = torch.ones(args.copy_probe << 18, dtype=torch.float32, device="cuda")

@@ -129,6 +129,29 @@ int smmp_helix(smmp_ctx *ctx, int32_t *out);
  * out[c*8 + k] = total, eysm, eyel, eyvw, eyhb, eyvr, eysl, asa. */
 int smmp_energy_batch(smmp_ctx *const *ctxs, int nsys, const int32_t *conf_sys, int nconf,
                       const double *vlvr, int precision, double *out);
+/* The call above keeps its systems on the device and reuses them while it is
+ * called again with the same contexts and conf_sys (a system is rebuilt when
+ * its context's topology changes: init_energy, init_molecule).
+ *
+ * Resident batch: the same computation with the conformations and results
+ * kept in device memory between calls, for callers that evaluate a fixed set
+ * of conformations repeatedly or refill it in place.  The contexts must
+ * outlive the batch; it runs on the stream of ctxs[0].
+ *   smmp_batch_create: uploads the systems; *nvals = doubles of vlvr it takes
+ *   smmp_batch_load:   conformations (layout as smmp_energy_batch) -> device;
+ *                      nvals must equal what the batch takes now (it
+ *                      changes when a context's topology does)
+ *   smmp_batch_run:    energies of the loaded conformations (enqueued; with
+ *                      the analytic solvent it waits for the kernels)
+ *   smmp_batch_read:   waits, out[c*8 + k] as smmp_energy_batch
+ *   smmp_batch_destroy */
+typedef struct smmp_batch smmp_batch;
+int smmp_batch_create(smmp_ctx *const *ctxs, int nsys, const int32_t *conf_sys, int nconf, smmp_batch **out,
+                      int64_t *nvals);
+int smmp_batch_load(smmp_batch *b, const double *vlvr, int64_t nvals);
+int smmp_batch_run(smmp_batch *b, int precision);
+int smmp_batch_read(smmp_batch *b, double *out);
+void smmp_batch_destroy(smmp_batch *b);
 
 /* Independent-stream replica sampler: R replicas of the context's system,
  * each with its own twister.f MT19937 stream seeded sgrnd(seeds[r]).

@@ -41,9 +41,9 @@ __global__ void __launch_bounds__(kEnergyThreads) k_energy(const DevSys* __restr
                                                            const double* __restrict__ vlvr, double* __restrict__ out,
                                                            double* __restrict__ xyz_out, int xyz_stride,
                                                            int32_t* __restrict__ err, double* __restrict__ work,
-                                                           int bits_lds) {
+                                                           int bits_lds, const int32_t* __restrict__ order) {
   extern __shared__ double smem[];
-  const int c = blockIdx.x;
+  const int c = order ? order[blockIdx.x] : (int)blockIdx.x;  // block b evaluates conformation order[b]
   const DevSys& S = systems[conf_sys[c]];
   const double* vl = vlvr + conf_off[c];
   EnergyLds L = carve_energy_lds(smem, S, blockDim.x / kWave, bits_lds != 0);
@@ -257,7 +257,8 @@ struct Engine {
   DevSystem sys;
   DevMem d_sys, d_confsys, d_confoff, d_vlvr, d_out, d_xyz, d_err, d_work;
   SamplerBuffers smp;           // replica state of the samplers
-  std::vector<std::unique_ptr<DevSystem>> batch_sys;  // energy_batch systems
+  EnergyBatch* batch = nullptr;            // kept by energy_batch for repeated calls on the same set
+  std::vector<EnergyBatch*> user_batches;  // smmp_batch_create handles running on this engine
   Profiler prof;
   uint64_t sys_gen = 0;         // counts uploads of the context's system
   bool capturing = false;       // launches are being recorded into a graph
@@ -268,7 +269,9 @@ struct Engine {
   int32_t info[4] = {0, 0, 0, 0};                     // how the last sampler call ran
   MlpWork* mlp = nullptr;                             // surrogate tables and training workspace
   EsolanWork* esolan = nullptr;                       // analytic solvent workspace
+  void release_batches();                             // drops 'batch', detaches user batches
   ~Engine() {
+    release_batches();
     mlp_work_destroy(mlp);
     esolan_work_destroy(esolan);
     if (gexec) (void)hipGraphExecDestroy(gexec);
@@ -398,27 +401,31 @@ static void check_lds(size_t bytes) {
   if (bytes > kLdsBytes) throw DeviceError("system too large for one workgroup's LDS");
 }
 
+// Launches k_energy on stream st for nblocks conformations: block b takes
+// conformation d_order[b] (all of them in order when d_order is null).
+static void launch_energy_on(hipStream_t st, const DevSys* d_systems, const int32_t* d_cs, const int64_t* d_off,
+                             const double* d_vl, double* d_out, double* d_xyz, int xyz_stride, int nblocks,
+                             const int32_t* d_order, LdsPlan plan, int precision, int32_t* d_err, double* d_work) {
+  const size_t lds = plan.bytes;
+  check_lds(lds);
+  auto kern = precision == 0 ? k_energy<false> : k_energy<true>;
+  HIPCHK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(kern, dim3(nblocks), dim3(kEnergyThreads), lds, st, d_systems, d_cs, d_off, d_vl, d_out, d_xyz,
+                     xyz_stride, d_err, d_work, plan.bits_lds, d_order);
+  HIPCHK(hipGetLastError());
+}
+
 // Launches k_energy for nconf conformations of the systems at d_systems.
 static void launch_energy(Engine& E, const DevSys* d_systems, const int32_t* d_cs, const int64_t* d_off,
                           const double* d_vl, double* d_out, double* d_xyz, int xyz_stride, int nconf, LdsPlan plan,
                           int precision) {
-  const size_t lds = plan.bytes;
-  check_lds(lds);
   int32_t* d_err = static_cast<int32_t*>(E.d_err.ensure(sizeof(int32_t)));
   HIPCHK(hipMemsetAsync(d_err, 0, sizeof(int32_t), E.stream));
   double* d_work = E.prof.on ? static_cast<double*>(E.d_work.ensure(sizeof(double) * 4 * nconf)) : nullptr;
   if (d_work) HIPCHK(hipMemsetAsync(d_work, 0, sizeof(double) * 4 * nconf, E.stream));
   prof_mark(E, kKEnergy, true);
-  if (precision == 0) {
-    HIPCHK(hipFuncSetAttribute((const void*)k_energy<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(k_energy<false>, dim3(nconf), dim3(kEnergyThreads), lds, E.stream, d_systems, d_cs, d_off, d_vl,
-                       d_out, d_xyz, xyz_stride, d_err, d_work, plan.bits_lds);
-  } else {
-    HIPCHK(hipFuncSetAttribute((const void*)k_energy<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(k_energy<true>, dim3(nconf), dim3(kEnergyThreads), lds, E.stream, d_systems, d_cs, d_off, d_vl,
-                       d_out, d_xyz, xyz_stride, d_err, d_work, plan.bits_lds);
-  }
-  HIPCHK(hipGetLastError());
+  launch_energy_on(E.stream, d_systems, d_cs, d_off, d_vl, d_out, d_xyz, xyz_stride, nconf, nullptr, plan, precision,
+                   d_err, d_work);
   prof_mark(E, kKEnergy, false);
   if (d_work) {
     prof_collect(E);
@@ -569,62 +576,296 @@ double engine_enysol(Engine*& e, State& s) {
   return s.eysl;
 }
 
-int engine_energy_batch(Engine*& e, std::vector<State*>& sys, const int32_t* conf_sys, int nconf, const double* vlvr,
-                        int precision, double* out) {
-  Engine& E = engine_of(e);
-  if (nconf == 0) return SMMP_OK;
-  const int nsys = (int)sys.size();
-  const bool analytic = sys[0]->itysol < 0;
-  for (int i = 1; i < nsys; ++i)
-    if ((sys[i]->itysol < 0) != analytic)
-      throw Error("energy_batch> systems with the analytic solvent (itysol < 0) and others in one batch");
-  E.batch_sys.clear();
+// ------------------------------------------------ resident energy batch
+//
+// The systems of a batch are uploaded once and its conformations and results
+// stay on the device between calls; a system is rebuilt only when its
+// context's topology changed (State::gen) and refreshed when only its global
+// parameters moved.  Conformations are grouped by the LDS their system needs:
+// class 1 holds the systems that fit two 512-thread workgroups per CU (the
+// register budget of k_energy allows two), class 0 the rest.  Each class is
+// one launch with its own LDS size, the two on separate streams so that they
+// fill the CUs together; within a class the largest systems go first.
+constexpr int kBatchClasses = 2;
+
+struct EnergyBatch {
+  Engine* E = nullptr;  // null once the engine is gone
+  std::vector<State*> states;
+  std::vector<uint64_t> gens;
+  std::vector<std::unique_ptr<DevSystem>> sys;
+  std::vector<int32_t> conf_sys;
+  std::vector<int64_t> off;
+  int64_t nvals = 0;
+  int nconf = 0, max_nat = 0;
+  bool analytic = false;
+  bool loaded = false;  // conformations on the device match the current layout
+  struct Class {
+    int begin = 0, count = 0;
+    LdsPlan plan;
+  };
+  std::vector<Class> classes;
+  hipStream_t side = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+  DevMem d_sys, d_cs, d_off, d_order, d_vl, d_out, d_xyz;
+  ~EnergyBatch() {
+    if (join) (void)hipEventDestroy(join);
+    if (fork) (void)hipEventDestroy(fork);
+    if (side) (void)hipStreamDestroy(side);
+  }
+};
+
+void Engine::release_batches() {
+  delete batch;
+  batch = nullptr;
+  for (EnergyBatch* b : user_batches) b->E = nullptr;
+  user_batches.clear();
+}
+
+static Engine& batch_engine(EnergyBatch* b) {
+  if (!b || !b->E) throw Error("batch> the batch's first context was destroyed");
+  return *b->E;
+}
+
+// (Re)builds what depends on the systems' topology: offsets, classes, order.
+static void batch_layout(EnergyBatch& B) {
+  Engine& E = *B.E;
+  const int nsys = (int)B.sys.size();
   std::vector<DevSys> dsys(nsys);
-  std::vector<const HostSys*> hsys(nsys);
-  int max_nat = 0;
+  B.max_nat = 0;
+  for (int i = 0; i < nsys; ++i) {
+    dsys[i] = B.sys[i]->d;
+    B.max_nat = std::max(B.max_nat, B.sys[i]->h.nat);
+  }
+  B.off.resize(B.nconf);
+  B.nvals = 0;
+  for (int c = 0; c < B.nconf; ++c) {
+    B.off[c] = B.nvals;
+    B.nvals += dsys[B.conf_sys[c]].nvt;
+  }
+  // class of each system: 1 when it fits half of the LDS (with or without
+  // its pair bits), else 0
+  const int nw = kEnergyThreads / kWave;
+  std::vector<int> cls(nsys);
+  for (int i = 0; i < nsys; ++i) {
+    const DevSys& d = dsys[i];
+    const size_t without = energy_lds_bytes(d.np, d.npnt, nw, d.nheavy, 0);
+    check_lds(without);
+    cls[i] = without <= kLdsBytes / 2 ? 1 : 0;
+  }
+  std::vector<int32_t> order(B.nconf);
+  for (int c = 0; c < B.nconf; ++c) order[c] = c;
+  std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+    const int sa = B.conf_sys[a], sb = B.conf_sys[b];
+    if (cls[sa] != cls[sb]) return cls[sa] < cls[sb];
+    return dsys[sa].nat > dsys[sb].nat;
+  });
+  B.classes.clear();
+  for (int k = 0; k < kBatchClasses; ++k) {
+    EnergyBatch::Class C;
+    C.begin = -1;
+    size_t with = 0, without = 0;
+    for (int j = 0; j < B.nconf; ++j) {
+      const int si = B.conf_sys[order[j]];
+      if (cls[si] != k) continue;
+      if (C.begin < 0) C.begin = j;
+      ++C.count;
+      const DevSys& d = dsys[si];
+      with = std::max(with, energy_lds_bytes(d.np, d.npnt, nw, d.nheavy, pair_bit_words(d.nat, d.words)));
+      without = std::max(without, energy_lds_bytes(d.np, d.npnt, nw, d.nheavy, 0));
+    }
+    if (C.count == 0) continue;
+    const size_t cap = k == 1 ? kLdsBytes / 2 : kLdsBytes;
+    C.plan.bits_lds = with <= cap ? 1 : 0;
+    C.plan.bytes = with <= cap ? with : without;
+    B.classes.push_back(C);
+  }
+  DevSys* d_systems = static_cast<DevSys*>(B.d_sys.ensure(sizeof(DevSys) * nsys));
+  int32_t* d_cs = static_cast<int32_t*>(B.d_cs.ensure(sizeof(int32_t) * std::max(B.nconf, 1)));
+  int64_t* d_off = static_cast<int64_t*>(B.d_off.ensure(sizeof(int64_t) * std::max(B.nconf, 1)));
+  int32_t* d_order = static_cast<int32_t*>(B.d_order.ensure(sizeof(int32_t) * std::max(B.nconf, 1)));
+  B.d_vl.ensure(sizeof(double) * std::max<int64_t>(B.nvals, 1));
+  B.d_out.ensure(sizeof(double) * kEnergyOut * std::max(B.nconf, 1));
+  if (B.analytic) B.d_xyz.ensure(sizeof(double) * 3 * (size_t)B.max_nat * std::max(B.nconf, 1));
+  HIPCHK(hipMemcpyAsync(d_systems, dsys.data(), sizeof(DevSys) * nsys, hipMemcpyHostToDevice, E.stream));
+  if (B.nconf > 0) {
+    HIPCHK(hipMemcpyAsync(d_cs, B.conf_sys.data(), sizeof(int32_t) * B.nconf, hipMemcpyHostToDevice, E.stream));
+    HIPCHK(hipMemcpyAsync(d_off, B.off.data(), sizeof(int64_t) * B.nconf, hipMemcpyHostToDevice, E.stream));
+    HIPCHK(hipMemcpyAsync(d_order, order.data(), sizeof(int32_t) * B.nconf, hipMemcpyHostToDevice, E.stream));
+  }
+  HIPCHK(hipStreamSynchronize(E.stream));
+  B.loaded = false;
+}
+
+// Brings the device systems in line with the contexts: rebuilds a system
+// whose topology changed (and then the layout), refreshes global parameters.
+static void batch_refresh(EnergyBatch& B) {
+  Engine& E = *B.E;
+  bool topo = false, params = false;
+  for (size_t i = 0; i < B.sys.size(); ++i) {
+    State& s = *B.states[i];
+    DevSystem& ds = *B.sys[i];
+    if (s.ntlml < 1) throw Error("energy_batch> context without a molecule");
+    if (B.gens[i] != s.gen || (s.itysol < 0) != B.analytic) {
+      if ((s.itysol < 0) != B.analytic)
+        throw Error("energy_batch> systems with the analytic solvent (itysol < 0) and others in one batch");
+      ds.h = build_host_sys(s);
+      upload_system(ds, E.stream);
+      B.gens[i] = s.gen;
+      topo = true;
+      continue;
+    }
+    bool same = ds.h.nml == s.ntlml && ds.h.boxsize == s.boxsize;
+    for (int m = 1; same && m <= s.ntlml; ++m)
+      for (int k = 0; k < 6; ++k)
+        if (ds.h.gbpr[6 * (m - 1) + k] != s.gbpr(k + 1, m)) same = false;
+    if (!same) {
+      for (int m = 1; m <= s.ntlml; ++m)
+        for (int k = 0; k < 6; ++k) ds.h.gbpr[6 * (m - 1) + k] = ds.d.gbpr[6 * (m - 1) + k] = s.gbpr(k + 1, m);
+      ds.h.boxsize = ds.d.boxsize = s.boxsize;
+      params = true;
+    }
+  }
+  if (topo) {
+    batch_layout(B);
+  } else if (params) {
+    std::vector<DevSys> dsys(B.sys.size());
+    for (size_t i = 0; i < B.sys.size(); ++i) dsys[i] = B.sys[i]->d;
+    HIPCHK(hipMemcpyAsync(B.d_sys.p, dsys.data(), sizeof(DevSys) * dsys.size(), hipMemcpyHostToDevice, E.stream));
+    HIPCHK(hipStreamSynchronize(E.stream));
+  }
+}
+
+static EnergyBatch* batch_new(Engine& E, std::vector<State*>& sys, const int32_t* conf_sys, int nconf) {
+  auto B = std::make_unique<EnergyBatch>();
+  B->E = &E;
+  const int nsys = (int)sys.size();
+  B->analytic = sys[0]->itysol < 0;
+  for (int i = 1; i < nsys; ++i)
+    if ((sys[i]->itysol < 0) != B->analytic)
+      throw Error("energy_batch> systems with the analytic solvent (itysol < 0) and others in one batch");
+  B->states = sys;
+  B->nconf = nconf;
+  B->conf_sys.assign(conf_sys, conf_sys + nconf);
   for (int i = 0; i < nsys; ++i) {
     auto p = std::make_unique<DevSystem>();
     p->h = build_host_sys(*sys[i]);
     upload_system(*p, E.stream);
-    dsys[i] = p->d;
-    hsys[i] = &p->h;
-    max_nat = std::max(max_nat, p->h.nat);
-    E.batch_sys.push_back(std::move(p));
+    B->sys.push_back(std::move(p));
+    B->gens.push_back(sys[i]->gen);
   }
-  std::vector<int64_t> off(nconf);
-  int64_t tot = 0;
-  for (int c = 0; c < nconf; ++c) {
-    off[c] = tot;
-    tot += dsys[conf_sys[c]].nvt;
+  batch_layout(*B);
+  return B.release();
+}
+
+EnergyBatch* engine_batch_create(Engine*& e, std::vector<State*>& sys, const int32_t* conf_sys, int nconf) {
+  Engine& E = engine_of(e);
+  EnergyBatch* b = batch_new(E, sys, conf_sys, nconf);
+  E.user_batches.push_back(b);
+  return b;
+}
+
+void engine_batch_destroy(EnergyBatch* b) {
+  if (!b) return;
+  if (b->E) {
+    auto& v = b->E->user_batches;
+    v.erase(std::remove(v.begin(), v.end(), b), v.end());
   }
-  DevSys* d_systems = static_cast<DevSys*>(E.d_sys.ensure(sizeof(DevSys) * nsys));
-  E.valid = false;  // d_sys now holds the batch systems
-  int32_t* d_cs = static_cast<int32_t*>(E.d_confsys.ensure(sizeof(int32_t) * nconf));
-  int64_t* d_off = static_cast<int64_t*>(E.d_confoff.ensure(sizeof(int64_t) * nconf));
-  double* d_vl = static_cast<double*>(E.d_vlvr.ensure(sizeof(double) * tot));
-  double* d_out = static_cast<double*>(E.d_out.ensure(sizeof(double) * kEnergyOut * nconf));
-  HIPCHK(hipMemcpyAsync(d_systems, dsys.data(), sizeof(DevSys) * nsys, hipMemcpyHostToDevice, E.stream));
-  HIPCHK(hipMemcpyAsync(d_cs, conf_sys, sizeof(int32_t) * nconf, hipMemcpyHostToDevice, E.stream));
-  HIPCHK(hipMemcpyAsync(d_off, off.data(), sizeof(int64_t) * nconf, hipMemcpyHostToDevice, E.stream));
-  HIPCHK(hipMemcpyAsync(d_vl, vlvr, sizeof(double) * tot, hipMemcpyHostToDevice, E.stream));
-  std::vector<const DevSys*> host_sys(nsys);
-  for (int i = 0; i < nsys; ++i) host_sys[i] = &dsys[i];
-  double* d_xyz = analytic ? static_cast<double*>(E.d_xyz.ensure(sizeof(double) * 3 * max_nat * (size_t)nconf))
-                           : nullptr;
-  launch_energy(E, d_systems, d_cs, d_off, d_vl, d_out, d_xyz, max_nat, nconf,
-                plan_energy_lds(host_sys.data(), nsys, kEnergyThreads), precision);
-  if (analytic) {
+  delete b;
+}
+
+int64_t engine_batch_values(const EnergyBatch* b) { return b->nvals; }
+
+void engine_batch_load(EnergyBatch* b, const double* vlvr, int64_t nvals) {
+  Engine& E = batch_engine(b);
+  batch_refresh(*b);
+  if (nvals != b->nvals)
+    throw Error("energy_batch> " + std::to_string(nvals) + " values given, the batch's systems take " +
+                std::to_string(b->nvals));
+  if (b->nvals > 0)
+    HIPCHK(hipMemcpyAsync(b->d_vl.p, vlvr, sizeof(double) * b->nvals, hipMemcpyHostToDevice, E.stream));
+  b->loaded = true;
+}
+
+void engine_batch_run(EnergyBatch* b, int precision) {
+  Engine& E = batch_engine(b);
+  EnergyBatch& B = *b;
+  batch_refresh(B);
+  if (!B.loaded) throw Error("energy_batch> load the conformations (again) after a topology change");
+  if (B.nconf == 0) return;
+  const DevSys* d_systems = static_cast<const DevSys*>(B.d_sys.p);
+  const int32_t* d_cs = static_cast<const int32_t*>(B.d_cs.p);
+  const int64_t* d_off = static_cast<const int64_t*>(B.d_off.p);
+  const int32_t* d_order = static_cast<const int32_t*>(B.d_order.p);
+  const double* d_vl = static_cast<const double*>(B.d_vl.p);
+  double* d_out = static_cast<double*>(B.d_out.p);
+  double* d_xyz = B.analytic ? static_cast<double*>(B.d_xyz.p) : nullptr;
+  int32_t* d_err = static_cast<int32_t*>(E.d_err.ensure(sizeof(int32_t)));
+  HIPCHK(hipMemsetAsync(d_err, 0, sizeof(int32_t), E.stream));
+  double* d_work = E.prof.on ? static_cast<double*>(E.d_work.ensure(sizeof(double) * 4 * B.nconf)) : nullptr;
+  if (d_work) HIPCHK(hipMemsetAsync(d_work, 0, sizeof(double) * 4 * B.nconf, E.stream));
+  prof_mark(E, kKEnergy, true);
+  const bool two = B.classes.size() > 1;
+  if (two) {
+    if (!B.side) {
+      HIPCHK(hipStreamCreateWithFlags(&B.side, hipStreamNonBlocking));
+      HIPCHK(hipEventCreateWithFlags(&B.fork, hipEventDisableTiming));
+      HIPCHK(hipEventCreateWithFlags(&B.join, hipEventDisableTiming));
+    }
+    HIPCHK(hipEventRecord(B.fork, E.stream));
+    HIPCHK(hipStreamWaitEvent(B.side, B.fork, 0));
+  }
+  for (size_t k = 0; k < B.classes.size(); ++k) {
+    const EnergyBatch::Class& C = B.classes[k];
+    launch_energy_on(k == 0 ? E.stream : B.side, d_systems, d_cs, d_off, d_vl, d_out, d_xyz, B.max_nat, C.count,
+                     d_order + C.begin, C.plan, precision, d_err, d_work);
+  }
+  if (two) {
+    HIPCHK(hipEventRecord(B.join, B.side));
+    HIPCHK(hipStreamWaitEvent(E.stream, B.join, 0));
+  }
+  prof_mark(E, kKEnergy, false);
+  if (d_work) {
+    prof_collect(E);
+    prof_add_work(E, kKEnergy, d_work, B.nconf);
+  }
+  if (B.analytic) {
     // rotations draw from each conformation's context, conformations in order
-    std::vector<int32_t> cs(conf_sys, conf_sys + nconf);
-    add_esolan(E, d_systems, hsys, cs, d_cs, nconf, max_nat, d_xyz, d_out,
-               [&](int c) { return host_grnd(*sys[conf_sys[c]]); });
+    std::vector<const HostSys*> hsys(B.sys.size());
+    for (size_t i = 0; i < B.sys.size(); ++i) hsys[i] = &B.sys[i]->h;
+    add_esolan(E, d_systems, hsys, B.conf_sys, d_cs, B.nconf, B.max_nat, d_xyz, d_out,
+               [&](int c) { return host_grnd(*B.states[B.conf_sys[c]]); });
   }
-  std::vector<double> full((size_t)kEnergyOut * nconf);
-  HIPCHK(hipMemcpyAsync(full.data(), d_out, sizeof(double) * full.size(), hipMemcpyDeviceToHost, E.stream));
+}
+
+void engine_batch_read(EnergyBatch* b, double* out) {
+  Engine& E = batch_engine(b);
+  EnergyBatch& B = *b;
+  if (B.nconf == 0) return;
+  std::vector<double> full((size_t)kEnergyOut * B.nconf);
+  HIPCHK(hipMemcpyAsync(full.data(), B.d_out.p, sizeof(double) * full.size(), hipMemcpyDeviceToHost, E.stream));
   HIPCHK(hipStreamSynchronize(E.stream));
   if (read_err(E)) throw Error("enysol> too many solvent neighbours for one atom");
-  for (int c = 0; c < nconf; ++c)
+  for (int c = 0; c < B.nconf; ++c)
     for (int k = 0; k < 8; ++k) out[8 * (size_t)c + k] = full[(size_t)kEnergyOut * c + k];
+}
+
+// energy_batch: the engine keeps the last batch and reuses it while the
+// contexts and the conformations' systems are the same.
+int engine_energy_batch(Engine*& e, std::vector<State*>& sys, const int32_t* conf_sys, int nconf, const double* vlvr,
+                        int precision, double* out) {
+  Engine& E = engine_of(e);
+  if (nconf == 0) return SMMP_OK;
+  EnergyBatch* B = E.batch;
+  const bool same = B && B->states == sys && B->nconf == nconf &&
+                    std::equal(B->conf_sys.begin(), B->conf_sys.end(), conf_sys);
+  if (!same) {
+    delete E.batch;
+    E.batch = nullptr;
+    E.batch = batch_new(E, sys, conf_sys, nconf);
+  }
+  engine_batch_load(E.batch, vlvr, E.batch->nvals);
+  engine_batch_run(E.batch, precision);
+  engine_batch_read(E.batch, out);
   return SMMP_OK;
 }
 

@@ -156,6 +156,7 @@ int smmp_init_energy(smmp_ctx* c, const char* libdir) {
   return guarded(c, SMMP_EINPUT, [&] {
     if (!libdir) throw Error("init_energy> libdir is NULL");
     init_energy(c->s, libdir);
+    c->s.gen = next_generation();
     engine_invalidate(c->eng);
     return SMMP_OK;
   });
@@ -166,6 +167,7 @@ int smmp_init_molecule(smmp_ctx* c, int iabin, const char* grpn, const char* grp
   return guarded(c, SMMP_EINPUT, [&] {
     init_molecule(c->s, iabin, grpn ? grpn : "nh2", grpc ? grpc : "cooh", seqfile ? seqfile : "",
                   varfile ? varfile : "");
+    c->s.gen = next_generation();
     engine_invalidate(c->eng);
     return SMMP_OK;
   });
@@ -286,6 +288,62 @@ int smmp_energy_batch(smmp_ctx* const* ctxs, int nsys, const int32_t* conf_sys, 
     if (precision != 0 && precision != 1) throw Error("energy_batch> precision must be 0 or 1");
     return engine_energy_batch(c0->eng, sys, conf_sys, nconf, vlvr, precision, out);
   });
+}
+
+struct smmp_batch {
+  smmp_ctx* c0;
+  EnergyBatch* b;
+};
+
+int smmp_batch_create(smmp_ctx* const* ctxs, int nsys, const int32_t* conf_sys, int nconf, smmp_batch** out,
+                      int64_t* nvals) {
+  if (!ctxs || nsys < 1 || !ctxs[0] || nconf < 0 || (nconf > 0 && !conf_sys) || !out) return SMMP_EINVAL;
+  *out = nullptr;
+  smmp_ctx* c0 = ctxs[0];
+  return guarded(c0, SMMP_ESTATE, [&] {
+    std::vector<State*> sys(nsys);
+    for (int i = 0; i < nsys; ++i) {
+      if (!ctxs[i] || ctxs[i]->s.ntlml < 1) throw Error("energy_batch> context without a molecule");
+      sys[i] = &ctxs[i]->s;
+    }
+    for (int k = 0; k < nconf; ++k)
+      if (conf_sys[k] < 0 || conf_sys[k] >= nsys) throw Error("energy_batch> conf_sys out of range");
+    EnergyBatch* b = engine_batch_create(c0->eng, sys, conf_sys, nconf);
+    *out = new smmp_batch{c0, b};
+    if (nvals) *nvals = engine_batch_values(b);
+    return SMMP_OK;
+  });
+}
+
+int smmp_batch_load(smmp_batch* b, const double* vlvr, int64_t nvals) {
+  if (!b || !vlvr) return SMMP_EINVAL;
+  return guarded(b->c0, SMMP_ESTATE, [&] {
+    engine_batch_load(b->b, vlvr, nvals);
+    return SMMP_OK;
+  });
+}
+
+int smmp_batch_run(smmp_batch* b, int precision) {
+  if (!b) return SMMP_EINVAL;
+  return guarded(b->c0, SMMP_ESTATE, [&] {
+    if (precision != 0 && precision != 1) throw Error("energy_batch> precision must be 0 or 1");
+    engine_batch_run(b->b, precision);
+    return SMMP_OK;
+  });
+}
+
+int smmp_batch_read(smmp_batch* b, double* out) {
+  if (!b || !out) return SMMP_EINVAL;
+  return guarded(b->c0, SMMP_ESTATE, [&] {
+    engine_batch_read(b->b, out);
+    return SMMP_OK;
+  });
+}
+
+void smmp_batch_destroy(smmp_batch* b) {
+  if (!b) return;
+  engine_batch_destroy(b->b);
+  delete b;
 }
 
 int smmp_sample(smmp_ctx* c, const smmp_sampler_cfg* cfg, double* vlvr, double* eol, double* acc, int32_t* ipoi,

@@ -35,6 +35,14 @@ int engine_partem_s(Engine*& e, State& s, int nrep, int nequi, int nswp, int swi
                     double* eol, int32_t* ipoi, double* traj, double* obs, const double* start, double* coor);
 int engine_energy_batch(Engine*& e, std::vector<State*>& sys, const int32_t* conf_sys, int nconf,
                         const double* vlvr, int precision, double* out);
+// Resident batch (smmp_batch_*): systems and conformations kept on the device.
+struct EnergyBatch;
+EnergyBatch* engine_batch_create(Engine*& e, std::vector<State*>& sys, const int32_t* conf_sys, int nconf);
+void engine_batch_destroy(EnergyBatch* b);
+void engine_batch_load(EnergyBatch* b, const double* vlvr, int64_t nvals);
+void engine_batch_run(EnergyBatch* b, int precision);
+void engine_batch_read(EnergyBatch* b, double* out);
+int64_t engine_batch_values(const EnergyBatch* b);  // doubles of vlvr the batch takes
 int engine_sample(Engine*& e, State& s, const smmp_sampler_cfg& cfg, double* vlvr, double* eol, double* acc,
                   int32_t* ipoi, int32_t* hist, double* traj);
 const char* engine_build_info();

@@ -5,6 +5,7 @@
 // 1-based convention through the F1/F2 accessors; xat/yat/zat keep their
 // lower bound -1 (INCL.H:237, virtual atoms 0 and -1 are used by rplgrp).
 #pragma once
+#include <atomic>
 #include <cstdint>
 #include <cstring>
 #include <string>
@@ -54,7 +55,16 @@ struct FStr {
 };
 
 // Everything the energy, the moves and the Python layer read or write.
+// A fresh number for every topology a State takes on (process-wide, never
+// reused), so device copies of a system can be matched to what they were
+// built from.
+inline uint64_t next_generation() {
+  static std::atomic<uint64_t> g{0};
+  return ++g;
+}
+
 struct State {
+  uint64_t gen = next_generation();  // renewed by init_energy and init_molecule
   // /epar_l/ flex, tesgrd, sh2, epsd (Fortran logicals as int32), ientyp
   int32_t flex = 0, tesgrd = 0, sh2 = 0, epsd = 0, ientyp = 0;
   // /isolty/

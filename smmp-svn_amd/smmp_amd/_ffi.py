@@ -71,6 +71,11 @@ def _load():
         "smmp_rgyr": (C.c_int, [P, C.c_int, dp, dp]),
         "smmp_helix": (C.c_int, [P, ip]),
         "smmp_energy_batch": (C.c_int, [C.POINTER(P), C.c_int, ip, C.c_int, dp, C.c_int, dp]),
+        "smmp_batch_create": (C.c_int, [C.POINTER(P), C.c_int, ip, C.c_int, C.POINTER(P), C.POINTER(C.c_int64)]),
+        "smmp_batch_load": (C.c_int, [P, dp, C.c_int64]),
+        "smmp_batch_run": (C.c_int, [P, C.c_int]),
+        "smmp_batch_read": (C.c_int, [P, dp]),
+        "smmp_batch_destroy": (None, [P]),
         "smmp_sample": (C.c_int, [P, C.POINTER(SamplerCfg), dp, dp, dp, ip, ip, dp]),
         "smmp_build_info": (C.c_char_p, []),
         "smmp_profile": (C.c_int, [P, C.c_int]),
@@ -98,7 +103,8 @@ lib = _load()
 EXPORTS = ("smmp_ctx_create", "smmp_ctx_destroy", "smmp_last_error", "smmp_state_view", "smmp_state_count",
            "smmp_state_name", "smmp_init_energy", "smmp_init_molecule", "smmp_setvar", "smmp_energy",
            "smmp_enysol", "smmp_sgrnd", "smmp_grnd", "smmp_outvar", "smmp_metropolis", "smmp_partem_s", "smmp_rgyr", "smmp_redvar", "smmp_helix",
-           "smmp_energy_batch", "smmp_sample", "smmp_build_info", "smmp_profile", "smmp_profile_read",
+           "smmp_energy_batch", "smmp_batch_create", "smmp_batch_load", "smmp_batch_run", "smmp_batch_read",
+           "smmp_batch_destroy", "smmp_sample", "smmp_build_info", "smmp_profile", "smmp_profile_read",
            "smmp_profile_phases", "smmp_muca_update", "smmp_muca_final", "smmp_sampler_info",
            "smmp_mlp_size", "smmp_mlp_set", "smmp_mlp_get", "smmp_mlp_grad", "smmp_mlp_fit")
 
@@ -372,3 +378,44 @@ def energy_batch(ctxs, conf_sys, vlvr_flat, precision: int = 0) -> np.ndarray:
     rc = lib.smmp_energy_batch(arr, len(ctxs), iptr(cs), len(cs), dptr(vl), precision, dptr(out))
     ctxs[0].check(rc)
     return out
+
+
+class EnergyBatch:
+    """Resident batch (smmp_batch_*): the systems of ctxs and the conformations
+    stay on the device; load() fills them, run() evaluates, read() returns
+    rows total, eysm, eyel, eyvw, eyhb, eyvr, eysl, asa.  The contexts must
+    outlive the batch."""
+
+    def __init__(self, ctxs, conf_sys):
+        self._ctxs = list(ctxs)
+        arr = (C.c_void_p * len(self._ctxs))(*[c.h for c in self._ctxs])
+        self.conf_sys = np.ascontiguousarray(conf_sys, dtype=np.int32)
+        h = C.c_void_p()
+        nv = C.c_int64()
+        self._ctxs[0].check(lib.smmp_batch_create(arr, len(self._ctxs), iptr(self.conf_sys), len(self.conf_sys),
+                                                  C.byref(h), C.byref(nv)))
+        self.h, self.nvals = h, int(nv.value)
+
+    def load(self, vlvr_flat) -> None:
+        vl = np.ascontiguousarray(vlvr_flat, dtype=np.float64).ravel()
+        self._vl = vl  # kept alive while the copy may still read it
+        self._ctxs[0].check(lib.smmp_batch_load(self.h, dptr(vl), vl.size))
+
+    def run(self, precision: int = 0) -> None:
+        self._ctxs[0].check(lib.smmp_batch_run(self.h, precision))
+
+    def read(self) -> np.ndarray:
+        out = np.zeros((len(self.conf_sys), 8))
+        self._ctxs[0].check(lib.smmp_batch_read(self.h, dptr(out)))
+        return out
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            lib.smmp_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -10,7 +10,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import TOPO_TAGS, GOLDEN, golden, make_ctx
+from conftest import LIBDIR, TOPO_TAGS, GOLDEN, golden, make_ctx
 
 pytestmark = pytest.mark.gpu
 
@@ -123,3 +123,87 @@ def test_single_energy_sets_coordinates():
     nat = osys.nat
     xyz = np.stack([c.view("atm_r.xat")[2:2 + nat], c.view("atm_r.yat")[2:2 + nat], c.view("atm_r.zat")[2:2 + nat]], 1)
     assert np.abs(xyz - osys.coords()).max() < 1e-11
+
+
+def _uneven_set(lengths, per, seed):
+    from smmp_amd import workloads
+    from _oracle import OracleSystem
+    ctxs = [make_ctx(seq=workloads.random_sequence(L, seed + L)) for L in lengths]
+    rng = np.random.default_rng(seed)
+    cs, flat, refs = [], [], []
+    for s, c in enumerate(ctxs):
+        o = OracleSystem(c)
+        for _ in range(per):
+            vl = c.vlvr()
+            vl[c.idvr - 1] = rng.uniform(-np.pi, np.pi, size=c.nvr)
+            cs.append(s)
+            flat.append(vl)
+            refs.append(o.energy(vl))
+    return ctxs, np.asarray(cs, dtype=np.int32), np.concatenate(flat), refs
+
+
+def test_resident_batch_both_lds_classes_vs_oracle():
+    """Resident batch over 10- to 60-residue systems (both LDS classes, two
+    streams): FP64 against the oracle, repeated runs identical, the mixed
+    mode within its bound, and energy_batch (kept batch) equal to it."""
+    from smmp_amd import _ffi
+    ctxs, cs, flat, refs = _uneven_set((10, 17, 33, 48, 60), 3, 300)
+    say("resident batch", len(cs), "conformations")
+    B = _ffi.EnergyBatch(ctxs, cs)
+    assert B.nvals == flat.size
+    B.load(flat)
+    B.run(0)
+    o1 = B.read()
+    B.run(0)
+    o2 = B.read()
+    assert np.array_equal(o1, o2)
+    for k, ref in enumerate(refs):
+        assert np.all(np.abs(o1[k, :7] - ref[:7]) <= comp_tol(ref, 1e-10)), (k, o1[k], ref)
+        assert abs(o1[k, 7] - ref[7]) <= 1e-9 * max(1.0, ref[7])
+    B.run(1)
+    om = B.read()
+    for k, ref in enumerate(refs):
+        mag = abs(ref[2]) + abs(ref[3]) + abs(ref[4])
+        assert abs(om[k, 0] - ref[0]) <= 5e-6 * mag + 1e-6, (k, om[k, 0], ref[0])
+    e1 = _ffi.energy_batch(ctxs, cs, flat, 0)
+    e2 = _ffi.energy_batch(ctxs, cs, flat, 0)  # the engine's kept batch
+    assert np.array_equal(e1, o1) and np.array_equal(e2, o1)
+    B.close()
+
+
+def test_resident_batch_follows_a_topology_change(tmp_path):
+    """A context re-initialised after the batch was made: the batch refuses
+    to run on the old layout, takes the new one, and matches smmp_energy."""
+    from smmp_amd import _ffi, workloads
+    ctxs, cs, flat, refs = _uneven_set((12, 20), 2, 700)
+    B = _ffi.EnergyBatch(ctxs, cs)
+    B.load(flat)
+    B.run(0)
+    B.read()
+    c = ctxs[0]
+    seqf = workloads.write_seq_file(str(tmp_path / "n.seq"), workloads.random_sequence(15, 4), "n")
+    c.init_energy(LIBDIR)
+    c.init_molecule(1, "nh2", "cooh", seqf, " ")
+    nml = int(c.view("mol_i.ntlml")[0])
+    g = c.view("mol_r.gbpr")
+    for m in range(1, nml):  # chains added to the context start at the origin: move them apart
+        g[0, m] += 40.0 * m
+    say("topology change: chains", nml, "nvt", c.nvt)
+    with pytest.raises(_ffi.SmmpError):
+        B.run(0)
+    with pytest.raises(_ffi.SmmpError):
+        B.load(flat)  # sized for the old systems
+    rows = []
+    rng = np.random.default_rng(9)
+    for s in cs:
+        v = ctxs[s].vlvr()
+        v[ctxs[s].idvr - 1] = rng.uniform(-np.pi, np.pi, size=ctxs[s].nvr)
+        rows.append(v)
+    B.load(np.concatenate(rows))
+    B.run(0)
+    out = B.read()
+    for k, s in enumerate(cs):
+        ctxs[s].set_vlvr(rows[k])
+        e = ctxs[s].energy()
+        assert abs(out[k, 0] - e) <= 1e-10 * max(1.0, abs(e)), (k, out[k, 0], e)
+    B.close()
