@@ -387,6 +387,9 @@ def run_c3(args):
         a = flops / (ke["ms"] * 1e-3) / 1e12
         return flops / K, a, {"achieved": round(a, 4), "peak": peak, "frac": round(a / peak, 5)}
 
+    wk = kem["work"] / (K * n)
+    log(f"work per conformation: pairs {wk[0]:.0f}, solvent contacts {wk[1]:.0f} (x122 point tests), "
+        f"atoms placed {wk[2]:.0f}")
     fl, am, rm = roof(kem, PEAK_FP32)
     _, ad, rd = roof(ked, PEAK_FP64)
     cpu = None

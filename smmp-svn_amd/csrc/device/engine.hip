@@ -28,6 +28,10 @@ namespace smmp {
   } while (0)
 
 constexpr int kEnergyThreads = 512;
+// k_energy is compiled for up to 1024 threads (it needs fewer than 128 VGPRs,
+// so four waves fit per SIMD); systems whose LDS allows one workgroup per CU
+// take 1024 threads, the rest 512.
+constexpr int kEnergyThreadsMax = 1024;
 // k_energy values per conformation: total, eysm, eyel, eyvw, eyhb, eyvr,
 // eysl, asa, then between chains eysmi, eyeli, eyvwi, eyhbi
 constexpr int kEnergyOut = 12;
@@ -35,7 +39,7 @@ constexpr int kEnergyOut = 12;
 // ------------------------------------------------------------- kernels
 
 template <bool MIXED>
-__global__ void __launch_bounds__(kEnergyThreads) k_energy(const DevSys* __restrict__ systems,
+__global__ void __launch_bounds__(kEnergyThreadsMax) k_energy(const DevSys* __restrict__ systems,
                                                            const int32_t* __restrict__ conf_sys,
                                                            const int64_t* __restrict__ conf_off,
                                                            const double* __restrict__ vlvr, double* __restrict__ out,
@@ -405,12 +409,13 @@ static void check_lds(size_t bytes) {
 // conformation d_order[b] (all of them in order when d_order is null).
 static void launch_energy_on(hipStream_t st, const DevSys* d_systems, const int32_t* d_cs, const int64_t* d_off,
                              const double* d_vl, double* d_out, double* d_xyz, int xyz_stride, int nblocks,
-                             const int32_t* d_order, LdsPlan plan, int precision, int32_t* d_err, double* d_work) {
+                             const int32_t* d_order, LdsPlan plan, int precision, int32_t* d_err, double* d_work,
+                             int threads = kEnergyThreads) {
   const size_t lds = plan.bytes;
   check_lds(lds);
   auto kern = precision == 0 ? k_energy<false> : k_energy<true>;
   HIPCHK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  hipLaunchKernelGGL(kern, dim3(nblocks), dim3(kEnergyThreads), lds, st, d_systems, d_cs, d_off, d_vl, d_out, d_xyz,
+  hipLaunchKernelGGL(kern, dim3(nblocks), dim3(threads), lds, st, d_systems, d_cs, d_off, d_vl, d_out, d_xyz,
                      xyz_stride, d_err, d_work, plan.bits_lds, d_order);
   HIPCHK(hipGetLastError());
 }
@@ -582,10 +587,11 @@ double engine_enysol(Engine*& e, State& s) {
 // stay on the device between calls; a system is rebuilt only when its
 // context's topology changed (State::gen) and refreshed when only its global
 // parameters moved.  Conformations are grouped by the LDS their system needs:
-// class 1 holds the systems that fit two 512-thread workgroups per CU (the
-// register budget of k_energy allows two), class 0 the rest.  Each class is
-// one launch with its own LDS size, the two on separate streams so that they
-// fill the CUs together; within a class the largest systems go first.
+// class 1 holds the systems that fit two 512-thread workgroups per CU, class
+// 0 the rest, which run one 1024-thread workgroup per CU (the register budget
+// of k_energy allows four waves per SIMD either way).  Each class is one
+// launch with its own LDS size, the two on separate streams so that they fill
+// the CUs together; within a class the largest systems go first.
 constexpr int kBatchClasses = 2;
 
 struct EnergyBatch {
@@ -600,7 +606,7 @@ struct EnergyBatch {
   bool analytic = false;
   bool loaded = false;  // conformations on the device match the current layout
   struct Class {
-    int begin = 0, count = 0;
+    int begin = 0, count = 0, threads = kEnergyThreads;
     LdsPlan plan;
   };
   std::vector<Class> classes;
@@ -642,9 +648,10 @@ static void batch_layout(EnergyBatch& B) {
     B.off[c] = B.nvals;
     B.nvals += dsys[B.conf_sys[c]].nvt;
   }
-  // class of each system: 1 when it fits half of the LDS (with or without
-  // its pair bits), else 0
+  // class of each system: 1 when it fits half of the LDS at 512 threads
+  // (with or without its pair bits), else 0 (1024 threads)
   const int nw = kEnergyThreads / kWave;
+  auto waves_of = [](int k) { return (k == 1 ? kEnergyThreads : kEnergyThreadsMax) / kWave; };
   std::vector<int> cls(nsys);
   for (int i = 0; i < nsys; ++i) {
     const DevSys& d = dsys[i];
@@ -670,10 +677,12 @@ static void batch_layout(EnergyBatch& B) {
       if (C.begin < 0) C.begin = j;
       ++C.count;
       const DevSys& d = dsys[si];
-      with = std::max(with, energy_lds_bytes(d.np, d.npnt, nw, d.nheavy, pair_bit_words(d.nat, d.words)));
-      without = std::max(without, energy_lds_bytes(d.np, d.npnt, nw, d.nheavy, 0));
+      with = std::max(with, energy_lds_bytes(d.np, d.npnt, waves_of(k), d.nheavy, pair_bit_words(d.nat, d.words)));
+      without = std::max(without, energy_lds_bytes(d.np, d.npnt, waves_of(k), d.nheavy, 0));
     }
     if (C.count == 0) continue;
+    check_lds(without);
+    C.threads = waves_of(k) * kWave;
     const size_t cap = k == 1 ? kLdsBytes / 2 : kLdsBytes;
     C.plan.bits_lds = with <= cap ? 1 : 0;
     C.plan.bytes = with <= cap ? with : without;
@@ -817,7 +826,7 @@ void engine_batch_run(EnergyBatch* b, int precision) {
   for (size_t k = 0; k < B.classes.size(); ++k) {
     const EnergyBatch::Class& C = B.classes[k];
     launch_energy_on(k == 0 ? E.stream : B.side, d_systems, d_cs, d_off, d_vl, d_out, d_xyz, B.max_nat, C.count,
-                     d_order + C.begin, C.plan, precision, d_err, d_work);
+                     d_order + C.begin, C.plan, precision, d_err, d_work, C.threads);
   }
   if (two) {
     HIPCHK(hipEventRecord(B.join, B.side));
