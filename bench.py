@@ -49,6 +49,9 @@ FLOP_ES_COVER = 7   # one circle tested against an arc midpoint: 3 mul, 3 add, 1
 # write 256 MiB) of the timed launch, rocprofv3 --pmc, 1000 sweeps:
 # 4,587,491 KiB + 9,686,131 KiB.
 HBM_BYTES_PER_SWEEP = 14.616e6
+# The same for one batch of the batched-energy workload (4,096 conformations,
+# mixed): (123,055 + 39,745) KiB FETCH_SIZE x 2.0 + 427 KiB WRITE_SIZE x 1.0.
+C3_HBM_BYTES_PER_BATCH = 333.8e6
 
 
 def log(*a):
@@ -436,7 +439,9 @@ def run_c3(args):
         "error_mixed_vs_fp64": {"max_abs": float(f"{err.max():.4e}"), "max_rel": float(f"{rel.max():.4e}"),
                                 "median_rel": float(f"{np.median(rel):.4e}"),
                                 "frac_within_bound": round(float(np.mean(err <= bound)), 5)},
-        "roofline": {"bound": "valu-fp32", "unit": "TFLOP/s", "traffic": None, "kernel": "k_energy<mixed>",
+        "roofline": {"bound": "valu-fp32", "unit": "TFLOP/s",
+                     "traffic": C3_HBM_BYTES_PER_BATCH if (nseq, per) == (256, 16) else None,
+                     "kernel": "k_energy<mixed> (both LDS classes, one batch)",
                      "avg_launch_ms": round(kem["ms"] / max(kem["launches"], 1), 4),
                      "flop_per_launch": round(fl / max(kem["launches"] / K, 1), 1), **rm},
         "kernel_evals_per_s": round(K * n / (kem["ms"] * 1e-3), 1),
