@@ -22,9 +22,11 @@ struct EnergyLds {
   double* fr;              // frames: 6*np (or the solvent scratch)
   double* ptab;            // pair types (18x18) x {A, A14, C, H-bond flag}: 4 * 324
   double* sp;              // npnt*3
-  double* red;             // nwaves*8
+  double* red;             // nwaves*8, then one slot holding the solvent loop's atom counter
+  int* ctr;                // that counter
   double *rv, *sg;         // solvent radius and sigma per atom (np each)
   double* hp;              // heavy atoms packed by heavy index: x, y, z, radius (4 * nheavy)
+  int32_t* nfs;            // free points per heavy atom of a full evaluation (nheavy)
   const uint64_t* bits;    // pair bit rows: an LDS copy, or the system's array in HBM
   const lds_u64* lbits;    // the LDS copy as an LDS pointer (ds_read), or null
   int8_t* t;               // np bytes
@@ -45,8 +47,8 @@ __host__ __device__ inline int energy_region1(int np, int nwaves) {
 }
 // nbits: u64 words of pair bits staged in LDS (0: read from HBM).
 __host__ __device__ inline size_t energy_lds_bytes(int np, int npnt, int nwaves, int nheavy, size_t nbits) {
-  size_t d = 6 * (size_t)np + energy_region1(np, nwaves) + 4 * 324 + 3 * (size_t)npnt + 8 * (size_t)nwaves + nbits +
-             4 * (size_t)nheavy;
+  size_t d = 6 * (size_t)np + energy_region1(np, nwaves) + 4 * 324 + 3 * (size_t)npnt + 8 * (size_t)nwaves + 8 +
+             nbits + 4 * (size_t)nheavy + ((size_t)nheavy + 1) / 2;
   return d * sizeof(double) + (size_t)np + 2 * (size_t)nheavy + 2 * (size_t)np;
 }
 
@@ -62,9 +64,11 @@ __device__ inline EnergyLds carve_energy_lds(double* base, const DevSys& S, int 
   L.ptab = p; p += 4 * 324;
   L.sp = p; p += 3 * npnt;
   L.red = p; p += 8 * nwaves;
+  L.ctr = reinterpret_cast<int*>(p); p += 8;
   L.rv = p; p += np;
   L.sg = p; p += np;
   L.hp = p; p += 4 * S.nheavy;
+  L.nfs = reinterpret_cast<int32_t*>(p); p += (S.nheavy + 1) / 2;
   if (bits_lds) {
     L.bits = reinterpret_cast<const uint64_t*>(p);
     L.lbits = (const lds_u64*)(p);
@@ -753,26 +757,42 @@ __device__ void energy_from_coords(const DevSys& S, const double* vl, EnergyLds&
     ps.mag += pi.mag;
   }
   double tor = part == 0 ? torsion_terms(S, vl) : 0.0;
+  if (tid == 0) *L.ctr = 0;
   __syncthreads();  // packed heavy atoms written; the solvent scratch overlays the FP32 copies
   double sol = 0.0, asa = 0.0;
   int err = 0;
   if (S.itysol > 0) {
+    // This block's heavy atoms are h = part + nparts k.  Waves take the next
+    // atom as they finish (an atom's cost varies with its contacts and with
+    // how soon its points are all occluded); the areas are then summed with
+    // a fixed thread-to-atom map, so the totals do not depend on which wave
+    // took which atom.
     double* nb = L.fr + (size_t)wave * kNbDoubles;
+    int32_t* nf = nfree_out ? nfree_out : L.nfs;
     const double c4 = 4.0 * kPi / (double)S.npnt;
-    for (int h = t0; h < S.nheavy; h += stride) {
+    const int nmine = S.nheavy > part ? (S.nheavy - part + nparts - 1) / nparts : 0;
+    for (;;) {
+      int k = 0;
+      if (lane == 0) k = atomicAdd(L.ctr, 1);
+      k = __shfl(k, 0, kWave);
+      if (k >= nmine) break;
+      const int h = part + nparts * k;
       const int i = L.hv[h];
       int nfree = solvent_atom(S, L, nb, i, wk, L.lbits != nullptr);
       if (nfree < 0) {
         err = 1;
         nfree = 0;
       }
-      if (lane == 0) {
-        if (nfree_out) nfree_out[h] = nfree;
-        const double trad = L.rv[i];
-        double area = c4 * trad * trad * (double)nfree;
-        sol += area * L.sg[i];
-        asa += area;
-      }
+      if (lane == 0) nf[h] = nfree;
+    }
+    __syncthreads();
+    for (int k = tid; k < nmine; k += blockDim.x) {
+      const int h = part + nparts * k;
+      const int i = L.hv[h];
+      const double trad = L.rv[i];
+      const double area = c4 * trad * trad * (double)nf[h];
+      sol += area * L.sg[i];
+      asa += area;
     }
   }
   double v[7] = {ps.el, ps.vw, ps.hb, tor, sol, asa, ps.mag};
