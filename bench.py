@@ -490,12 +490,20 @@ def run_muca(args):
         _ffi.lib.smmp_muca_update(kmin, kmax, ebin, beta_t, _ffi.iptr(ih), _ffi.dptr(xhist), _ffi.dptr(g1),
                                   _ffi.dptr(b), _ffi.dptr(alpha), _ffi.dptr(ent))
     _ffi.lib.smmp_muca_final(kmin, kmax, ebin, beta_t, _ffi.dptr(b), _ffi.dptr(alpha))
+    # The recursion leaves the slope near the top of the range (and, clamped,
+    # above it) at d ln g/dE, below beta(TMAX): walkers could drift to ever
+    # higher energies.  Slopes are kept at least canonical at TMAX, and alpha
+    # follows from them as in mulcan_par_mod.f90:170-174.
+    b[:] = np.maximum(b, beta_t)
+    alpha[-1] = 0.0
+    for i in range(NB - 2, -1, -1):
+        alpha[i] = alpha[i + 1] + (b[i + 1] - b[i]) * (kmin + i + 0.5) * ebin
     log(f"{args.mu_iters} weight updates of {args.mu_nup} sweeps x {R} replicas in {time.perf_counter() - t0:.1f} s; "
         f"bins visited {np.mean(xhist > 0):.2f}")
     w2 = _ffi.Weight(kind=2, beta=0.0, kmin=kmin, kmax=kmax, ebin=ebin, b=_ffi.dptr(b), alpha=_ffi.dptr(alpha))
     K, W = args.steps, args.warmup
     g = args.graph_sweeps or math.gcd(K, W) or K
-    kw = dict(mode=2, weight=w2, precision=1, with_traj=False, graph_sweeps=g)
+    kw = dict(mode=2, weight=w2, precision=1, with_traj=False, graph_sweeps=g, groups=args.groups)
     state = c.sample(np.full(R, TMAX), seeds(500), W, state, **kw)["vlvr"]
     c.profile(1)
     t0 = time.perf_counter()
