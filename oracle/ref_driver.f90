@@ -10,6 +10,11 @@
 !   topo   libdir seqfile varfile itysol           topology, lists, tables
 !   energy libdir seqfile varfile itysol nconf seed  random conformations
 !   canon  libdir seqfile varfile itysol seed temp nsweep  Metropolis run
+!   metro  libdir seqfile varfile itysol seed temp nsweep upch bgsprob rndord wfile
+!          Metropolis run with /updchois/ set (upchswitch, bgsprob, rndord
+!          0/1) and the weight can_weight (wfile '-') or b(k)x + alpha(k)
+!          from wfile ('kmin kmax ebin' then one 'b alpha' line per bin);
+!          prints every sweep's eol and acz, then /updstats/
 !   partem libdir seqfile itysol seed nrep nequi nswp [switch]  partem_s run
 !   muca   libdir seqfile itysol seed nsweep nup temp kmin kmax ebin
 !   helix  libdir seqfile itysol seed nconf         helix() of drawn conformations
@@ -23,8 +28,29 @@
 !          intermolecular terms (eninteract.f), then a Metropolis run with
 !          the rigid-body trials of metropolis.f:94-168 (nequi unrecorded
 !          sweeps, then the state and nsweep recorded sweeps)
+! Tables of a multicanonical weight b(k) x + alpha(k) for the 'metro' mode.
+! The reference keeps its own tables private to module multicanonical and
+! only fills them inside mulcan_par / mulcan_sim, which stop under this
+! compiler before their sweeps (an internal write of two records into one,
+! mulcan_par_mod.f90:105 and the same format in mulcan_sim), so the driver
+! passes this function, the formula of muca_weight2
+! (mulcan_par_mod.f90:398-412), to the reference's own metropolis.
+module drv_weights
+  implicit none
+  double precision, allocatable :: wb(:), wal(:)
+  double precision :: webin, wxmin, wxmax
+contains
+  double precision function drv_muca2(x)
+    double precision, intent(in) :: x
+    integer :: k
+    k = int(min(wxmax, max(wxmin, x/webin + sign(0.5d0, x))))
+    drv_muca2 = wb(k)*x + wal(k)
+  end function
+end module drv_weights
+
 program ref_driver
   use multicanonical
+  use drv_weights
   include 'INCL.H'
   double precision energy, grnd, addang
   external can_weight
@@ -37,6 +63,9 @@ program ref_driver
   double precision :: e, eolx, acz, tmp, ebw
   double precision :: ggasc
   parameter (ggasc = 1.98773d-3)
+  integer :: ncalls, nacalls, kk
+  common/updstats/ncalls(5),nacalls(5)
+  character(len=256) :: wfile
 
   call get_command_argument(1, mode)
   grpn = 'nh2 '
@@ -139,6 +168,56 @@ program ref_driver
     end do
     write(*,'(a,i0)') '@mti int 1'
     write(*,'(i0)') mtstate_pos()
+
+  case ('metro')
+    call setup(2, .true.)
+    call iarg(6, iseed)
+    call rarg(7, tmp)
+    call iarg(8, nsw)
+    call iarg(9, j)
+    upchswitch = j
+    call rarg(10, bgsprob)
+    call iarg(11, j)
+    rndord = (j .ne. 0)
+    call get_command_argument(12, wfile)
+    if (trim(wfile) .ne. '-') then
+      open(76, file=trim(wfile), status='old')
+      read(76, *) kmn, kmx, webin
+      allocate(wb(kmn:kmx), wal(kmn:kmx))
+      do kk = kmn, kmx
+        read(76, *) wb(kk), wal(kk)
+      end do
+      close(76)
+      wxmin = dble(kmn)
+      wxmax = dble(kmx)
+    end if
+    call sgrnd(iseed)
+    beta = 1.0d0 / (tmp * ggasc)
+    eolx = energy()
+    acz = 0.0d0
+    ncalls = 0
+    nacalls = 0
+    write(*,'(a,i0)') '@canon_e0 double 1'
+    write(*,'(es25.17)') eolx
+    do k = 1, nsw
+      if (trim(wfile) .ne. '-') then
+        call metropolis(eolx, acz, drv_muca2)
+      else
+        call metropolis(eolx, acz, can_weight)
+      end if
+      write(*,'(a,i0)') '@sweep double 2'
+      write(*,'(es25.17)') eolx
+      write(*,'(es25.17)') acz
+    end do
+    write(*,'(a,i0)') '@vlvr double ', nvr
+    do i = 1, nvr
+      write(*,'(es25.17)') vlvr(idvr(i))
+    end do
+    write(*,'(a,i0)') '@mti int 1'
+    write(*,'(i0)') mtstate_pos()
+    write(*,'(a,i0)') '@ncalls int 10'
+    write(*,'(i0)') (ncalls(i), i=1,5)
+    write(*,'(i0)') (nacalls(i), i=1,5)
 
   case ('partem')
     call setup(2, .false.)

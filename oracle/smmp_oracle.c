@@ -515,6 +515,38 @@ static void trace_trial(int jv, double enw, double eol, int acc) {
 
 /* ------------------------------------------------ metropolis.f:13-184 */
 
+/* Trial order and update-type choice of metropolis.f:42-57 (/updchois/):
+ * rndord draws the variable; with upchswitch 1 (updtch1, metropolis.f:
+ * 196-207) or 2 (updtch2, :209-220) a variable whose name does not start
+ * with 'x' draws once more and, below bgsprob or (beta - up2bmin) /
+ * (up2bmax - up2bmin), takes a BGS update, which the reference leaves inert
+ * (bgsnvar = 0, bgs.f:86-89: no move, no energy).  g_upd holds accanalyze's
+ * ncalls(1..5), nacalls(1..5) (metropolis.f:186-193). */
+static int g_rndord = 0, g_upch = 0;
+static double g_bgsprob = 0.0, g_up2bmax = 4.0, g_up2bmin = 0.5, g_ubeta = -1.0;
+static const int8_t *g_xname = NULL;
+static double g_upd[10];
+
+void oracle_set_updchois(int rndord, int upchswitch, double bgsprob, double up2bmax, double up2bmin,
+                         const int8_t *xname, double ubeta) {
+  g_rndord = rndord;
+  g_upch = upchswitch;
+  g_bgsprob = bgsprob;
+  g_up2bmax = up2bmax;
+  g_up2bmin = up2bmin;
+  g_xname = xname;
+  g_ubeta = ubeta;
+  memset(g_upd, 0, sizeof g_upd);
+}
+void oracle_updstats(double *out) { memcpy(out, g_upd, sizeof g_upd); }
+
+static void accanalyze(int iupt, int st) {
+  g_upd[4] += 1.0;
+  g_upd[iupt - 1] += 1.0;
+  g_upd[9] += st;
+  g_upd[4 + iupt] += st;
+}
+
 int oracle_metropolis(const oracle_topo *t, oracle_conf *c, oracle_mt *g, double *vlvr,
                       int nvr, const int32_t *idvr, const double *axvr,
                       const oracle_weight *w, double *eol1, double *acz) {
@@ -523,7 +555,20 @@ int oracle_metropolis(const oracle_topo *t, oracle_conf *c, oracle_mt *g, double
   g_sol_mt = g; /* esolan's rotations draw from the trials' twister */
   double eol = oracle_energy(t, c, vlvr);
   for (int nsw = 1; nsw <= nvr; ++nsw) {
-    int jv = idvr[nsw - 1];
+    int ivar = nsw;
+    if (g_rndord) { /* 1 + int(nvr*grnd()) (metropolis.f:42-44); u = 1 would pass the end */
+      ivar = 1 + (int)(nvr * oracle_grnd(g));
+      if (ivar > nvr) ivar = nvr;
+    }
+    int jv = idvr[ivar - 1];
+    if (g_upch != 0 && !(g_xname && g_xname[jv - 1])) {
+      const double bu = g_ubeta > 0.0 ? g_ubeta : w->beta;
+      const double prob = g_upch == 1 ? g_bgsprob : (bu - g_up2bmin) / (g_up2bmax - g_up2bmin);
+      if (oracle_grnd(g) < prob) { /* iupt = 2: bgs() returns 0 at once */
+        accanalyze(2, 0);
+        continue;
+      }
+    }
     memcpy(vrol, vlvr, sizeof(double) * (size_t)t->nvt);
     double dv = axvr[jv - 1] * (oracle_grnd(g) - 0.5);
     vlvr[jv - 1] = oracle_addang(vrol[jv - 1], dv);
@@ -547,6 +592,7 @@ int oracle_metropolis(const oracle_topo *t, oracle_conf *c, oracle_mt *g, double
     }
     trace_trial(jv, enw, eprev, up != 0.0);
     *acz = *acz + up;
+    accanalyze(1, up != 0.0);
   }
   /* rigid moves of each chain when there are several (metropolis.f:94-168):
    * three shifts of its position, then three new Euler angles */
@@ -586,6 +632,7 @@ int oracle_metropolis(const oracle_topo *t, oracle_conf *c, oracle_mt *g, double
         }
         trace_trial(-1 - (6 * m + i), enw, eprev, up != 0.0);
         *acz = *acz + up;
+        accanalyze(i < 3 ? 3 : 4, up != 0.0);
       }
     }
   }

@@ -106,6 +106,15 @@ int oracle_metropolis(const oracle_topo *t, oracle_conf *c, oracle_mt *g, double
                       int nvr, const int32_t *idvr, const double *axvr,
                       const oracle_weight *w, double *eol, double *acz);
 
+/* Trial order and update-type choice for the calls above (metropolis.f:
+ * 42-57; default: in order, upchswitch 0).  xname[nvt]: 1 where the
+ * variable's name starts with 'x'; ubeta > 0: the /bet/ beta updtch2 reads,
+ * else the weight's.  Clears the accanalyze counts; oracle_updstats gives
+ * ncalls(1..5), nacalls(1..5) as 10 doubles. */
+void oracle_set_updchois(int rndord, int upchswitch, double bgsprob, double up2bmax, double up2bmin,
+                         const int8_t *xname, double ubeta);
+void oracle_updstats(double *out);
+
 /* partem_s.f:13-258 (new start), all replicas on one stream.  switch_ > 0:
  * random initial torsions, < 0: all at 180 degrees, 0: all at zero.
  * temps: R temperatures; outputs: eol[R] (by replica), ipoi[R] (slot ->

@@ -125,7 +125,7 @@ struct DevMem {
 struct SamplerBuffers {
   DevMem idvr, axvr, vlvr, xyz, nfree, valid, eol, acc, mt, rng_of, slot_of, beta, err, dbg;
   DevMem wb, walpha, hist, traj, ipoi, iswitch, work, trace, trace_n, phase, part, sync, gsync;
-  DevMem sweep_base, tune_vl, tune_mt, tune_gb, gb, da_stats, snap, snap_e, obs;
+  DevMem sweep_base, tune_vl, tune_mt, tune_gb, gb, da_stats, snap, snap_e, obs, varx, upd;
 };
 
 // One system uploaded to the device: a single allocation holding every
@@ -872,6 +872,9 @@ int engine_energy_batch(Engine*& e, std::vector<State*>& sys, const int32_t* con
     E.batch = nullptr;
     E.batch = batch_new(E, sys, conf_sys, nconf);
   }
+  // a context whose topology changed since the last call is rebuilt first,
+  // so the size check below sees the values its systems take now
+  batch_refresh(*E.batch);
   engine_batch_load(E.batch, vlvr, E.batch->nvals);
   engine_batch_run(E.batch, precision);
   engine_batch_read(E.batch, out);

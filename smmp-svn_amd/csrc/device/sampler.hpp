@@ -62,7 +62,19 @@ __device__ inline double mt_grnd(uint32_t* mt, uint32_t* mti) {
 struct SamplerParams {
   const DevSys* sys;
   int32_t nrep, nvr, nvt, rndord;
+  // Update-type choice of metropolis.f:46-57 (/updchois/ upchswitch,
+  // bgsprob; /updtparam/ up2bmax, up2bmin): with upch 1 (updtch1) or 2
+  // (updtch2) every trial of a variable whose name does not start with 'x'
+  // draws one number first; below bgsprob (1) or (beta-up2bmin)/(up2bmax-
+  // up2bmin) (2) the trial is a BGS update, which the reference leaves
+  // inert (bgsnvar = 0, bgs.f:86-89): no move, no energy, not accepted.
+  int32_t upch;
+  double bgsprob, up2bmax, up2bmin;
+  double upch_beta;      // the beta updtch2 sees when upch_own is 0 (/bet/ of smmp_metropolis)
+  int32_t upch_own;      // 1: updtch2 sees the replica's beta (partem_s sets /bet/ per replica)
+  double* upd;           // [R][10] ncalls(1..5), nacalls(1..5) of accanalyze (metropolis.f:186-193), or null
   const int32_t* idvr;   // free variables, 0-based
+  const int8_t* var_x;   // [nvt] 1: name starts with 'x' (null: all, no draws)
   const double* axvr;    // nvt step widths
   double* vlvr;          // [R][nvt]
   double* xyz;           // [R][3][np] coordinates matching vlvr
@@ -134,6 +146,8 @@ __global__ void k_fill_u32(uint32_t* p, int n, uint32_t v) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
+// No-op: probes whether a cooperative launch is granted to this process.
+__global__ void k_coop_probe(int32_t unused) { (void)unused; }
 // Advances the sweep counter of graph replays past a finished launch.
 __global__ void k_add_i32(int32_t* p, int v) {
   if (threadIdx.x == 0) *p += v;
@@ -151,9 +165,10 @@ enum PhaseId { kPhPropose = 0, kPhRotate, kPhPairs, kPhSolvent, kPhReduce, kPhAc
                // wave sum of the contact marking, thread 0's torsion term
                kPhPairMax, kPhPairSum, kPhMarkSum, kPhTorsion, kNPhase };
 // broadcast scalars [0,32), phase clocks [32,64), per-wave stats [64,160),
-// rigid-move transform [160,176)
-constexpr int kScDoubles = 176;
+// rigid-move transform [160,176), accanalyze counters [176,186)
+constexpr int kScDoubles = 192;
 constexpr int kScRigid = 160;
+constexpr int kScUpd = 176;
 
 // Extra LDS of the sampler beyond the energy carve-up.
 __host__ __device__ inline size_t sampler_lds_bytes(int np, int npnt, int nheavy, int nvt, int nwaves, size_t nbits) {
@@ -169,6 +184,7 @@ __host__ __device__ inline size_t sampler_lds_bytes(int np, int npnt, int nheavy
   b += 8 * (size_t)nvt * sizeof(int16_t);
   b += 2 * (size_t)nheavy;                        // list of the affected heavy atoms
   b += (size_t)nheavy;                            // affected flags
+  b += (size_t)nvt;                               // variable names starting with 'x'
   return b + 64;
 }
 
@@ -178,6 +194,7 @@ __host__ __device__ inline size_t sampler_lds_bytes(int np, int npnt, int nheavy
 struct VarLds {
   double *ax, *sign, *e0, *sg, *rn, *t0, *tref;
   int16_t *m1, *m2, *base, *axis0, *tv, *idvr, *h1, *h2;
+  int8_t* xname;  // 1: the variable's name starts with 'x' (no update-type draw)
 };
 
 struct SamplerLds {
@@ -222,7 +239,8 @@ __device__ inline SamplerLds carve_sampler_lds(double* base, const DevSys& S, in
   Q.V.h1 = reinterpret_cast<int16_t*>(p); p += nvt * sizeof(int16_t);
   Q.V.h2 = reinterpret_cast<int16_t*>(p); p += nvt * sizeof(int16_t);
   Q.alist = reinterpret_cast<int16_t*>(p); p += S.nheavy * sizeof(int16_t);
-  Q.aff = reinterpret_cast<int8_t*>(p);
+  Q.aff = reinterpret_cast<int8_t*>(p); p += S.nheavy;
+  Q.V.xname = reinterpret_cast<int8_t*>(p);
   return Q;
 }
 
@@ -254,6 +272,7 @@ __device__ void stage_variables(const SamplerParams& P, const DevSys& S, const V
     V.tv[k] = tv;
   }
   for (int k = threadIdx.x; k < P.nvr; k += blockDim.x) V.idvr[k] = (int16_t)P.idvr[k];
+  for (int k = threadIdx.x; k < P.nvt; k += blockDim.x) V.xname[k] = P.var_x ? P.var_x[k] : (int8_t)1;
 }
 
 
@@ -665,6 +684,7 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
         Q.sc[6] = (double)(-1 - q);  // trial id: negative for rigid moves
         Q.sc[5] = fabs(Q.sc[0]) >= 1e5 ? 1.0 : 0.0;
         Q.sc[17] = Q.sc[18] = 0.0;  // no torsion term changes
+        Q.sc[21] = 0.0;
       }
       if (tid == 0 && !rigid) {
         uint32_t* mti = &Q.mt[624];
@@ -674,8 +694,20 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
           if (ivar >= P.nvr) ivar = P.nvr - 1;
         }
         const int jv = V.idvr[ivar];
+        // update type (metropolis.f:46-57): a BGS pick is an inert trial
+        bool bgs = false;
+        if (P.upch != 0 && V.xname[jv] == 0) {
+          const double bu = P.upch_own ? Q.sc[14] : P.upch_beta;
+          const double prob = P.upch == 1 ? P.bgsprob : (bu - P.up2bmin) / (P.up2bmax - P.up2bmin);
+          bgs = mt_grnd(Q.mt, mti) < prob;
+        }
+        Q.sc[21] = bgs ? 1.0 : 0.0;
+        if (bgs) {  // accanalyze(2, 0)
+          Q.sc[kScUpd + 1] += 1.0;
+          Q.sc[kScUpd + 4] += 1.0;
+        }
         const double vold = Q.vl[jv];
-        const double dv = V.ax[jv] * (mt_grnd(Q.mt, mti) - 0.5);
+        const double dv = bgs ? 0.0 : V.ax[jv] * (mt_grnd(Q.mt, mti) - 0.5);
         const double vnew = d_addang(vold, dv);
         Q.sc[2] = vnew;
         Q.sc[15] = vold;
@@ -690,6 +722,10 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
       }
       __syncthreads();
       lap(kPhPropose);
+      if (Q.sc[21] != 0.0) {  // inert BGS pick (uniform); the barrier keeps thread 0's
+        __syncthreads();      // next proposal from overwriting the flag before all read it
+        continue;
+      }
       const int jv = (int)Q.sc[6];  // variable, or -1 - (6 chain + parameter)
       int m1, m2, h1, h2;
       if (!rigid) {
@@ -720,7 +756,11 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
           Q.sc[19] = ds;
           Q.sc[20] = pass ? 1.0 : 0.0;
           D.stats[0] += 1.0;
-          if (!pass) D.stats[1] += 1.0;
+          if (!pass) {  // accanalyze(1, 0)
+            D.stats[1] += 1.0;
+            Q.sc[kScUpd + 0] += 1.0;
+            Q.sc[kScUpd + 4] += 1.0;
+          }
         }
         __syncthreads();
         if (Q.sc[20] == 0.0) continue;  // rejected without the exact energy (uniform)
@@ -977,6 +1017,15 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
           acc = exp(-delta) >= rd;
         }
         Q.sc[4] = acc ? 1.0 : 0.0;
+        {  // accanalyze(iupt, iupstate): 1 torsion, 3 shift, 4 new Euler angle
+          const int iu = rigid ? (((-1 - jv) % 6) < 3 ? 2 : 3) : 0;
+          Q.sc[kScUpd + iu] += 1.0;
+          Q.sc[kScUpd + 4] += 1.0;
+          if (acc) {
+            Q.sc[kScUpd + 5 + iu] += 1.0;
+            Q.sc[kScUpd + 9] += 1.0;
+          }
+        }
         if (P.trace && g == 0 && r == P.trace_rep && *P.trace_n < P.trace_cap) {
           double* t = P.trace + 4 * (size_t)(*P.trace_n);
           t[0] = jv;
@@ -1129,6 +1178,8 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
       P.valid[r] = 1;
       if (DA && P.da_stats)
         for (int k = 0; k < 4; ++k) P.da_stats[4 * (size_t)r + k] += D.stats[k];
+      if (P.upd)
+        for (int k = 0; k < 10; ++k) P.upd[10 * (size_t)r + k] += Q.sc[kScUpd + k];
     }
   }
   for (int h = tid; h < nh; h += blockDim.x)

@@ -42,6 +42,8 @@ const ViewDesc kViews[] = {
     VS(isolty, itysol, SMMP_I32), VS(isolty, isolscl, SMMP_I32),
     VS(updchois, rndord, SMMP_I32), VS(updchois, upchswitch, SMMP_I32), VS(updchois, bgsprob, SMMP_F64),
     VS(bet, beta, SMMP_F64),
+    V1(updstats, ncalls, SMMP_I32, 5, 1), V1(updstats, nacalls, SMMP_I32, 5, 1),
+    VS(updtparam, up2bmax, SMMP_F64), VS(updtparam, up2bmin, SMMP_F64),
     VS(mol_par, nvr, SMMP_I32),
     VS(mol_i, ntlml, SMMP_I32), V1(mol_i, irsml1, SMMP_I32, mxml, 1), V1(mol_i, irsml2, SMMP_I32, mxml, 1),
     V1(mol_i, ivrml1, SMMP_I32, mxml, 1), V1(mol_i, nvrml, SMMP_I32, mxml, 1), V1(mol_i, imsml1, SMMP_I32, mxml, 1),

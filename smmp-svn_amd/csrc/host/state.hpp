@@ -74,6 +74,11 @@ struct State {
   double bgsprob = 0.0;
   // /bet/
   double beta = 0.0;
+  // /updstats/ trials and accepted trials per update type (accanalyze,
+  // metropolis.f:186-193: 1 torsion, 2 BGS, 3 shift, 4 rotation, 5 all);
+  // /updtparam/ the beta range of updtch2 (BLOCK DATA, metropolis.f:221-229)
+  F1<int32_t, 5> ncalls, nacalls;
+  double up2bmax = 4.0, up2bmin = 0.5;
   // /mol_par/
   int32_t nvr = 0;
   // /mol_i/

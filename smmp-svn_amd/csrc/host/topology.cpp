@@ -1454,6 +1454,8 @@ void reset_state(State& s) {
   for (int i = 1; i <= mxvr; ++i) s.nmvr.set(i, "");
   std::memset(s.zimm, ' ', sizeof s.zimm);
   s.mti = 625;
+  s.up2bmax = 4.0;  // BLOCK DATA updtchs (metropolis.f:221-229)
+  s.up2bmin = 0.5;
   block_data(s);
 }
 

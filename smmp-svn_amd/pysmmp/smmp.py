@@ -66,6 +66,7 @@ _SCALARS = {
     "isolty": ("itysol", "isolscl"),
     "updchois": ("rndord", "upchswitch", "bgsprob"),
     "bet": ("beta",),
+    "updtparam": ("up2bmax", "up2bmin"),
     "mol_par": ("nvr",),
     "mol_i": ("ntlml",),
     "epar_r": ("conv", "plt", "slp"),

@@ -5,8 +5,10 @@ There is no fallback: a missing or unloadable library raises ImportError.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -119,6 +121,25 @@ def iptr(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_int32))
 
 
+# Live handles, torn down in order by close_all() at interpreter exit.
+_live_batches: "weakref.WeakSet" = weakref.WeakSet()
+_live_contexts: "weakref.WeakSet" = weakref.WeakSet()
+
+
+def close_all() -> None:
+    """Destroys every live resident batch, then every live context (graph
+    executables, events, streams, device memory).  Registered with atexit,
+    so this runs before the interpreter finalises objects in arbitrary order
+    and before the HIP runtime's and a profiler's own exit handlers."""
+    for b in list(_live_batches):
+        b.close()
+    for c in list(_live_contexts):
+        c.close()
+
+
+atexit.register(close_all)
+
+
 class Context:
     """Owns one smmp_ctx (one system: the reference's COMMON blocks)."""
 
@@ -126,12 +147,23 @@ class Context:
         h = C.c_void_p()
         self.check(lib.smmp_ctx_create(C.byref(h)), None)
         self.h = h
+        _live_contexts.add(self)
 
-    def __del__(self):
+    def close(self) -> None:
+        """Releases the context and its device state now (idempotent)."""
         h = getattr(self, "h", None)
         if h:
-            lib.smmp_ctx_destroy(h)
             self.h = None
+            lib.smmp_ctx_destroy(h)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        self.close()
 
     def check(self, rc: int, h=None) -> None:
         if rc != 0:
@@ -395,6 +427,7 @@ class EnergyBatch:
         self._ctxs[0].check(lib.smmp_batch_create(arr, len(self._ctxs), iptr(self.conf_sys), len(self.conf_sys),
                                                   C.byref(h), C.byref(nv)))
         self.h, self.nvals = h, int(nv.value)
+        _live_batches.add(self)
 
     def load(self, vlvr_flat) -> None:
         vl = np.ascontiguousarray(vlvr_flat, dtype=np.float64).ravel()
@@ -410,9 +443,10 @@ class EnergyBatch:
         return out
 
     def close(self) -> None:
-        if getattr(self, "h", None):
-            lib.smmp_batch_destroy(self.h)
+        h = getattr(self, "h", None)
+        if h:
             self.h = None
+            lib.smmp_batch_destroy(h)
 
     def __del__(self):
         try:

@@ -71,6 +71,9 @@ def _load():
     lib.oracle_metropolis.argtypes = [P(Topo), P(Conf), P(MT), F64P, C.c_int, I32P, F64P, P(Weight), F64P, F64P]
     lib.oracle_partem.argtypes = [P(Topo), P(MT), F64P, C.c_int, I32P, F64P, C.c_int, F64P, C.c_int, C.c_int,
                                   C.c_int, F64P, I32P, F64P, F64P]
+    lib.oracle_set_updchois.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                        C.POINTER(C.c_int8), C.c_double]
+    lib.oracle_updstats.argtypes = [F64P]
     lib.oracle_set_trace.argtypes = [F64P, C.c_int, C.c_int]
     lib.oracle_trace_count.restype = C.c_int
     lib.oracle_muca_update.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, I32P, F64P, F64P, F64P, F64P]
@@ -190,6 +193,7 @@ class OracleSystem:
         if lib.oracle_conf_alloc(C.byref(self.c), nat) != 0:
             raise MemoryError
         lib.oracle_conf_init(C.byref(self.t), C.byref(self.c))
+        lib.oracle_set_updchois(0, 0, 0.0, 4.0, 0.5, None, -1.0)  # trials in order, no update-type draws
 
     def __del__(self):
         if getattr(self, "c", None) is not None:
@@ -238,6 +242,22 @@ class OracleSystem:
         return np.stack([np.ctypeslib.as_array(self.c.xat, (n,)).copy(),
                          np.ctypeslib.as_array(self.c.yat, (n,)).copy(),
                          np.ctypeslib.as_array(self.c.zat, (n,)).copy()], 1)
+
+    def set_updchois(self, rndord: int = 0, upchswitch: int = 0, bgsprob: float = 0.0, ubeta: float = -1.0,
+                     up2bmax: float = 4.0, up2bmin: float = 0.5) -> None:
+        """Trial order and update-type draws of the following metropolis
+        calls (metropolis.f:42-57); clears the accanalyze counts."""
+        names = self.ctx.view("var_c.nmvr")  # (3, mxvr) characters
+        self._xname = np.array([names[0, k] == b"x" for k in range(self.nvt)], dtype=np.int8)
+        lib.oracle_set_updchois(rndord, upchswitch, bgsprob, up2bmax, up2bmin,
+                                self._xname.ctypes.data_as(C.POINTER(C.c_int8)), ubeta)
+
+    @staticmethod
+    def updstats() -> np.ndarray:
+        """ncalls(1..5), nacalls(1..5) since set_updchois."""
+        out = np.zeros(10)
+        lib.oracle_updstats(out.ctypes.data_as(F64P))
+        return out
 
     def metropolis(self, mt: OracleMT, vlvr, beta: float, nsweep: int, weight: Weight | None = None):
         vl = f64(vlvr).copy()

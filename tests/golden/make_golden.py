@@ -273,6 +273,48 @@ def restart_fixture(tmp: str, nrep: int, nswp: int, eol_final) -> None:
          nswp=np.array([nswp]), **confs)
 
 
+def muca_table(kmin: int, kmax: int, ebin: float, t_low: float, t_high: float) -> tuple[np.ndarray, np.ndarray]:
+    """A continuous piecewise-linear multicanonical weight: slope b(k) =
+    1/(k_B T_k) with T_k geometric from t_low (kmin) to t_high (kmax), alpha
+    by the continuity rule of mulcan_par_mod.f90:147-150 (alpha(kmax) = 0)."""
+    k = np.arange(kmin, kmax + 1)
+    temps = t_low * (t_high / t_low) ** ((k - kmin) / (kmax - kmin))
+    b = 1.0 / (temps * 1.98773e-3)
+    al = np.zeros_like(b)
+    for i in range(len(k) - 2, -1, -1):
+        al[i] = al[i + 1] + (b[i + 1] - b[i]) * (k[i] + 0.5) * ebin
+    return b, al
+
+
+def metro_fixtures(tmp: str) -> None:
+    """metropolis.f with the update-type draws (upchswitch 1 and 2, rndord)
+    and with a multicanonical weight b(k)x + alpha(k), on Met-enkephalin with
+    the 19 free torsions of enkefa.var (omegas fixed)."""
+    enk = [s.lower() for s in open(os.path.join(HERE, "enkefa.seq")).read().split()[2:]]
+    t_enk = topo_fixture("enkefa", enk, tmp, nconf=8, seed=11)
+    vf_fix = vars_file(os.path.join(tmp, "enk_fix.vars"), t_enk, os.path.join(HERE, "enkefa.var"), True)
+    seqf = os.path.join(tmp, "enkefa.seq")
+    kmin, kmax, ebin = -15, 120, 1.0
+    b, al = muca_table(kmin, kmax, ebin, 300.0, 1000.0)
+    wf = os.path.join(tmp, "muca_w.txt")
+    with open(wf, "w") as f:
+        f.write(f"{kmin} {kmax} {ebin!r}\n")
+        for x, y in zip(b, al):
+            f.write(f"{float(x):.17g} {float(y):.17g}\n")
+    runs = {"upch1": ("4357", "300.0", "20", "1", "0.3", "0", "-"),
+            "upch2": ("81236", "300.0", "20", "2", "0.0", "1", "-"),
+            "muca": ("81236", "300.0", "30", "0", "0.0", "0", wf),
+            "muca_upch1": ("2024", "300.0", "15", "1", "0.2", "1", wf)}
+    out = {"muca_kmin": np.array([kmin]), "muca_kmax": np.array([kmax]), "muca_ebin": np.array([ebin]),
+           "muca_b": b, "muca_alpha": al}
+    for tag, (seed, temp, nsw, upch, bgs, rnd, w) in runs.items():
+        r = run(["metro", LIBDIR, seqf, vf_fix, "0", seed, temp, nsw, upch, bgs, rnd, w], tmp)
+        out.update({f"{tag}_args": np.array([int(seed), float(temp), int(nsw), int(upch), float(bgs), int(rnd)]),
+                    f"{tag}_e0": r["canon_e0"][0], f"{tag}_sweeps": np.stack(r["sweep"]),
+                    f"{tag}_vlvr": r["vlvr"][0], f"{tag}_mti": r["mti"][0], f"{tag}_ncalls": r["ncalls"][0]})
+    save("metro_enkefa.npz", **out)
+
+
 def main(only: list[str]) -> None:
     if not os.path.exists(DRIVER):
         raise SystemExit("build the reference first: make -C oracle ref")
@@ -302,6 +344,8 @@ def main(only: list[str]) -> None:
             out = run(["helix", LIBDIR, seqf, "1", "31", "40"], tmp)
             save(f"helix_{tag}.npz", seq_in=np.array(seq, dtype="<U8"), vlvr=np.stack(out["vlvr"]),
                  helix=np.stack(out["helix"]))
+    if not only or "metro" in only:
+        metro_fixtures(tmp)
     if only and "restart" in only:  # the len10 run of partem_ext_len10.npz again, for its files
         seq = list(np.load(os.path.join(HERE, "topo_len10.npz"))["seq_in"])
         seqf = workloads.write_seq_file(os.path.join(tmp, "pt_len10.seq"), seq, "len10")

@@ -207,3 +207,34 @@ def test_resident_batch_follows_a_topology_change(tmp_path):
         e = ctxs[s].energy()
         assert abs(out[k, 0] - e) <= 1e-10 * max(1.0, abs(e)), (k, out[k, 0], e)
     B.close()
+
+
+def test_energy_batch_after_a_topology_change(tmp_path):
+    """smmp_energy_batch called again with the same contexts after one of
+    them changed its topology: the first call with values sized for the new
+    systems succeeds (the kept batch is rebuilt before the size check) and
+    matches smmp_energy."""
+    from smmp_amd import _ffi, workloads
+    ctxs, cs, flat, refs = _uneven_set((12, 20), 2, 701)
+    out0 = _ffi.energy_batch(ctxs, cs, flat, 0)
+    ref0 = np.array([r[0] for r in refs])
+    assert np.all(np.abs(out0[:, 0] - ref0) <= 1e-10 * np.maximum(1.0, np.abs(ref0)))
+    c = ctxs[0]
+    seqf = workloads.write_seq_file(str(tmp_path / "n.seq"), workloads.random_sequence(15, 4), "n")
+    c.init_energy(LIBDIR)
+    c.init_molecule(1, "nh2", "cooh", seqf, " ")
+    g = c.view("mol_r.gbpr")
+    for m in range(1, int(c.view("mol_i.ntlml")[0])):
+        g[0, m] += 40.0 * m
+    say("energy_batch after a topology change: nvt", c.nvt)
+    rows = []
+    rng = np.random.default_rng(10)
+    for s in cs:
+        v = ctxs[s].vlvr()
+        v[ctxs[s].idvr - 1] = rng.uniform(-np.pi, np.pi, size=ctxs[s].nvr)
+        rows.append(v)
+    out = _ffi.energy_batch(ctxs, cs, np.concatenate(rows), 0)
+    for k, s in enumerate(cs):
+        ctxs[s].set_vlvr(rows[k])
+        e = ctxs[s].energy()
+        assert abs(out[k, 0] - e) <= 1e-10 * max(1.0, abs(e)), (k, out[k, 0], e)

@@ -40,6 +40,42 @@ def test_metropolis_c1_trajectory_vs_reference():
     assert int(c.view("myblock.mti")[0]) == int(r["canon_mti"][0])
 
 
+@pytest.mark.parametrize("tag", ["upch1", "upch2", "muca", "muca_upch1"])
+def test_metropolis_variants_vs_reference(tag):
+    """metropolis.f with the update-type draws (upchswitch 1 and 2, rndord)
+    and a multicanonical weight b(k)x + alpha(k), move for move against the
+    Fortran's runs: acceptances and energies per sweep, the final torsions,
+    the twister position and accanalyze's counts (/updstats/)."""
+    from smmp_amd import _ffi
+    g = golden("metro_enkefa.npz")
+    seed, temp, nsw, upch, bgs, rnd = g[f"{tag}_args"]
+    c = make_ctx(tag="enkefa", itysol=0, varfile=os.path.join(GOLDEN, "enkefa_c1.var"))
+    beta = 1.0 / (temp * GASC)
+    c.view("bet.beta")[0] = beta
+    c.view("updchois.upchswitch")[0] = int(upch)
+    c.view("updchois.bgsprob")[0] = float(bgs)
+    c.view("updchois.rndord")[0] = int(rnd)
+    c.sgrnd(int(seed))
+    if tag.startswith("muca"):
+        b = np.ascontiguousarray(g["muca_b"])
+        al = np.ascontiguousarray(g["muca_alpha"])
+        w = _ffi.Weight(kind=2, beta=beta, kmin=int(g["muca_kmin"][0]), kmax=int(g["muca_kmax"][0]),
+                        ebin=float(g["muca_ebin"][0]), b=_ffi.dptr(b), alpha=_ffi.dptr(al))
+    else:
+        w = _ffi.Weight(kind=0, beta=beta)
+    sw = g[f"{tag}_sweeps"]
+    eol, acz = 0.0, 0.0
+    say(f"smmp_metropolis enkefa {tag} {int(nsw)} sweeps")
+    for k in range(int(nsw)):
+        eol, acz = c.metropolis(eol, acz, w)
+        assert acz == sw[k, 1], (tag, k)
+        assert abs(eol - sw[k, 0]) <= 1e-8 * max(1.0, abs(eol)), (tag, k)
+    assert np.abs(c.vlvr()[c.idvr - 1] - g[f"{tag}_vlvr"]).max() < 1e-12
+    assert int(c.view("myblock.mti")[0]) == int(g[f"{tag}_mti"][0])
+    counts = np.concatenate([c.view("updstats.ncalls"), c.view("updstats.nacalls")])
+    assert np.array_equal(counts.astype(np.int64), g[f"{tag}_ncalls"])
+
+
 def test_metropolis_with_solvent_vs_reference():
     from smmp_amd import _ffi
     r = golden("enkefa_runs.npz")

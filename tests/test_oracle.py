@@ -193,3 +193,81 @@ def test_muca_recursion_closed_form():
         exp_al[i] = exp_al[i + 1] + (exp_b[i + 1] - exp_b[i]) * (i + 0.5) * ebin
     assert np.allclose(b, exp_b, rtol=1e-6) and np.allclose(al, exp_al, rtol=1e-6, atol=1e-9)
     assert np.array_equal(xh, h.astype(float))
+
+
+def test_muca_recursion_two_updates_hand_computed():
+    """Two successive updates of Berg's recursion (mulcan_par_mod.f90:
+    122-155) from b = g1 = 0, with every branch written out bin by bin:
+    counts in both neighbours (g1 accumulates across the updates, so the
+    second update's step is weighted by g0 / (g0_1 + g0_2)), a neighbour
+    with too few counts (b kept), and a sparse bin (xhist < 2) that takes
+    max(0, b of the next bin) — once a negative one (-> 0), once a positive
+    one computed earlier in the same pass.  Logarithms are REAL (float32)
+    as in the reference."""
+    from _oracle import F64P, I32P, lib
+    kmin, kmax, ebin, beta = -1, 5, 1.5, 0.7
+    L = lambda n: float(np.log(np.float32(n)))  # noqa: E731  log(float(n))
+    h1 = np.array([0, 9, 4, 6, 2, 7, 3], dtype=np.int32)   # bins -1..5
+    h2 = np.array([1, 3, 30, 5, 8, 2, 1], dtype=np.int32)
+    xh, g1, b, al = np.zeros(7), np.zeros(7), np.zeros(7), np.zeros(7)
+    run = lambda h: lib.oracle_muca_update(  # noqa: E731
+        kmin, kmax, ebin, beta, h.copy().ctypes.data_as(I32P), xh.ctypes.data_as(F64P), g1.ctypes.data_as(F64P),
+        b.ctypes.data_as(F64P), al.ctypes.data_as(F64P))
+    ix = lambda k: k - kmin  # noqa: E731
+    # ---- first update (bins 3, 2, 1, 0 from their counts; bin -1 sparse)
+    run(h1)
+    g1_3, g1_2, g1_1, g1_0 = 2 * 7 / 9, 6 * 2 / 8, 4 * 6 / 10, 9 * 4 / 13
+    b3 = (L(7) - L(2)) / ebin
+    b2 = (L(2) - L(6)) / ebin
+    b1 = (L(6) - L(4)) / ebin
+    b0 = (L(4) - L(9)) / ebin          # negative
+    bm1 = 0.0                          # max(0, b0 < 0)
+    exp1 = np.array([bm1, b0, b1, b2, b3, b3, b3])  # b(kmax) = b(kmax-1) = b(kmax-2)
+    assert np.allclose(b, exp1, rtol=1e-6, atol=1e-12)
+    assert np.allclose(g1[[ix(0), ix(1), ix(2), ix(3)]], [g1_0, g1_1, g1_2, g1_3], rtol=1e-12)
+    a = [0.0] * 7                      # alpha(kmax) = 0, then downwards
+    for k in (4, 3, 2, 1, 0, -1):
+        a[ix(k)] = a[ix(k + 1)] + (exp1[ix(k + 1)] - exp1[ix(k)]) * (k + 0.5) * ebin
+    assert np.allclose(al, a, rtol=1e-6, atol=1e-9)
+    assert np.array_equal(xh, h1.astype(float))
+    # ---- second update: steps weighted by the accumulated g1
+    run(h2)
+    s3 = 8 * 2 / 10
+    b3 += s3 / (g1_3 + s3) * (L(2) - L(8)) / ebin
+    s2 = 5 * 8 / 13
+    b2 += s2 / (g1_2 + s2) * (L(8) - L(5)) / ebin
+    s1 = 30 * 5 / 35
+    b1 += s1 / (g1_1 + s1) * (L(5) - L(30)) / ebin
+    s0 = 3 * 30 / 33
+    b0 += s0 / (g1_0 + s0) * (L(30) - L(3)) / ebin
+    assert b0 > 0
+    bm1 = b0                           # sparse bin: max(0, b0 > 0), b0 of this same pass
+    exp2 = np.array([bm1, b0, b1, b2, b3, b3, b3])
+    assert np.allclose(b, exp2, rtol=1e-6, atol=1e-12)
+    assert np.allclose(g1[[ix(0), ix(1), ix(2), ix(3)]], [g1_0 + s0, g1_1 + s1, g1_2 + s2, g1_3 + s3], rtol=1e-12)
+    a = [0.0] * 7
+    for k in (4, 3, 2, 1, 0, -1):
+        a[ix(k)] = a[ix(k + 1)] + (exp2[ix(k + 1)] - exp2[ix(k)]) * (k + 0.5) * ebin
+    assert np.allclose(al, a, rtol=1e-6, atol=1e-9)
+    assert np.array_equal(xh, (h1 + h2).astype(float))
+
+
+def test_muca_recursion_product_matches_oracle_two_updates():
+    """The product's recursion (smmp_muca_update, host code behind the
+    C-ABI) equals the oracle's on the same two updates, bin for bin."""
+    from _oracle import F64P, I32P, lib
+    from smmp_amd import _ffi
+    kmin, kmax, ebin, beta = -1, 5, 1.5, 0.7
+    st_o = [np.zeros(7) for _ in range(4)]
+    st_p = [np.zeros(7) for _ in range(4)]
+    for h in (np.array([0, 9, 4, 6, 2, 7, 3], dtype=np.int32), np.array([1, 3, 30, 5, 8, 2, 1], dtype=np.int32)):
+        xh, g1, b, al = st_o
+        lib.oracle_muca_update(kmin, kmax, ebin, beta, h.copy().ctypes.data_as(I32P), xh.ctypes.data_as(F64P),
+                               g1.ctypes.data_as(F64P), b.ctypes.data_as(F64P), al.ctypes.data_as(F64P))
+        xh, g1, b, al = st_p
+        hp = h.copy()
+        rc = _ffi.lib.smmp_muca_update(kmin, kmax, ebin, beta, _ffi.iptr(hp), _ffi.dptr(xh), _ffi.dptr(g1),
+                                       _ffi.dptr(b), _ffi.dptr(al), None)
+        assert rc == 0 and not hp.any()  # the sweep histogram is cleared
+    for o, p in zip(st_o, st_p):
+        assert np.array_equal(o, p)
