@@ -129,6 +129,19 @@ int smmp_helix(smmp_ctx *ctx, int32_t *out);
  * out[c*8 + k] = total, eysm, eyel, eyvw, eyhb, eyvr, eysl, asa. */
 int smmp_energy_batch(smmp_ctx *const *ctxs, int nsys, const int32_t *conf_sys, int nconf,
                       const double *vlvr, int precision, double *out);
+/* The same energies as smmp_energy_batch (identical results) from an entry
+ * point that allocates no device memory.  No counterpart in the reference:
+ * it is for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Everything the
+ * call needs on the device (its systems, conformation tables, values,
+ * results, error flag) is laid out in a fixed 32 MiB array per device that
+ * the library holds until the process ends; a batch that needs more is
+ * refused with SMMP_EINVAL and a message naming the size.  Runs on the
+ * current device's null stream.  Calls on one GPU share the array: calls
+ * that overlap on the same GPU race.  Not for the analytic solvent
+ * (itysol < 0, which needs its own workspace): SMMP_EINVAL. */
+int smmp_energy_batch_noalloc(smmp_ctx *const *ctxs, int nsys, const int32_t *conf_sys, int nconf,
+                              const double *vlvr, int precision, double *out);
 /* The call above keeps its systems on the device and reuses them while it is
  * called again with the same contexts and conf_sys (a system is rebuilt when
  * its context's topology changes: init_energy, init_molecule).

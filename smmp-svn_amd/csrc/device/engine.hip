@@ -8,6 +8,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -136,8 +137,10 @@ struct DevSystem {
   DevMem mem;
 };
 
-static void upload_system(DevSystem& ds, hipStream_t st) {
-  const HostSys& h = ds.h;
+// Lays out one system's arrays in a blob (256-byte aligned parts) and fills
+// d with pointers into the device copy of that blob at 'base'.  base null:
+// only the size is computed (returned).
+static size_t system_blob(const HostSys& h, char* base, std::vector<uint8_t>& blob, DevSys& d) {
   size_t off = 0;
   std::vector<std::pair<size_t, std::pair<const void*, size_t>>> parts;
   auto add = [&](const void* src, size_t bytes) {
@@ -180,13 +183,11 @@ static void upload_system(DevSystem& ds, hipStream_t st) {
   size_t o_pC = add(h.pC.data(), h.pC.size() * 8);
   size_t o_pHB = add(h.pHB.data(), h.pHB.size() * 4);
   size_t o_sp = add(h.spoint.data(), h.spoint.size() * 8);
-  std::vector<uint8_t> blob(off, 0);
+  off = (off + 255) & ~size_t(255);
+  if (!base) return off;
+  blob.assign(off, 0);
   for (auto& pr : parts)
     if (pr.second.second) std::memcpy(blob.data() + pr.first, pr.second.first, pr.second.second);
-  char* base = static_cast<char*>(ds.mem.ensure(off));
-  HIPCHK(hipMemcpyAsync(base, blob.data(), off, hipMemcpyHostToDevice, st));
-  HIPCHK(hipStreamSynchronize(st));
-  DevSys& d = ds.d;
   d.nat = h.nat; d.nvt = h.nvt; d.nlev = h.nlev; d.words = h.words; d.nheavy = h.nheavy;
   d.npnt = h.npnt; d.epsd = h.epsd; d.itysol = h.itysol;
   d.nml = h.nml;
@@ -234,6 +235,17 @@ static void upload_system(DevSystem& ds, hipStream_t st) {
   d.pC = (const double*)(base + o_pC);
   d.pHB = (const int32_t*)(base + o_pHB);
   d.spoint = (const double*)(base + o_sp);
+  return off;
+}
+
+static void upload_system(DevSystem& ds, hipStream_t st) {
+  std::vector<uint8_t> blob;
+  DevSys probe;
+  const size_t bytes = system_blob(ds.h, nullptr, blob, probe);
+  char* base = static_cast<char*>(ds.mem.ensure(bytes));
+  system_blob(ds.h, base, blob, ds.d);
+  HIPCHK(hipMemcpyAsync(base, blob.data(), bytes, hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
 }
 
 // Kernel timing with HIP events recorded on the engine's stream around each
@@ -594,6 +606,60 @@ double engine_enysol(Engine*& e, State& s) {
 // the CUs together; within a class the largest systems go first.
 constexpr int kBatchClasses = 2;
 
+// One launch of k_energy over the conformations order[begin, begin+count).
+struct EnergyClass {
+  int begin = 0, count = 0, threads = kEnergyThreads;
+  LdsPlan plan;
+};
+
+// The classes of a set of conformations and the order that groups them
+// (largest systems first within a class).
+static std::vector<EnergyClass> plan_classes(const std::vector<DevSys>& dsys, const std::vector<int32_t>& conf_sys,
+                                             std::vector<int32_t>& order) {
+  const int nsys = (int)dsys.size(), nconf = (int)conf_sys.size();
+  // class of each system: 1 when it fits half of the LDS at 512 threads
+  // (with or without its pair bits), else 0 (1024 threads)
+  const int nw = kEnergyThreads / kWave;
+  auto waves_of = [](int k) { return (k == 1 ? kEnergyThreads : kEnergyThreadsMax) / kWave; };
+  std::vector<int> cls(nsys);
+  for (int i = 0; i < nsys; ++i) {
+    const DevSys& d = dsys[i];
+    const size_t without = energy_lds_bytes(d.np, d.npnt, nw, d.nheavy, 0);
+    check_lds(without);
+    cls[i] = without <= kLdsBytes / 2 ? 1 : 0;
+  }
+  order.resize(nconf);
+  for (int c = 0; c < nconf; ++c) order[c] = c;
+  std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+    const int sa = conf_sys[a], sb = conf_sys[b];
+    if (cls[sa] != cls[sb]) return cls[sa] < cls[sb];
+    return dsys[sa].nat > dsys[sb].nat;
+  });
+  std::vector<EnergyClass> classes;
+  for (int k = 0; k < kBatchClasses; ++k) {
+    EnergyClass C;
+    C.begin = -1;
+    size_t with = 0, without = 0;
+    for (int j = 0; j < nconf; ++j) {
+      const int si = conf_sys[order[j]];
+      if (cls[si] != k) continue;
+      if (C.begin < 0) C.begin = j;
+      ++C.count;
+      const DevSys& d = dsys[si];
+      with = std::max(with, energy_lds_bytes(d.np, d.npnt, waves_of(k), d.nheavy, pair_bit_words(d.nat, d.words)));
+      without = std::max(without, energy_lds_bytes(d.np, d.npnt, waves_of(k), d.nheavy, 0));
+    }
+    if (C.count == 0) continue;
+    check_lds(without);
+    C.threads = waves_of(k) * kWave;
+    const size_t cap = k == 1 ? kLdsBytes / 2 : kLdsBytes;
+    C.plan.bits_lds = with <= cap ? 1 : 0;
+    C.plan.bytes = with <= cap ? with : without;
+    classes.push_back(C);
+  }
+  return classes;
+}
+
 struct EnergyBatch {
   Engine* E = nullptr;  // null once the engine is gone
   std::vector<State*> states;
@@ -605,11 +671,7 @@ struct EnergyBatch {
   int nconf = 0, max_nat = 0;
   bool analytic = false;
   bool loaded = false;  // conformations on the device match the current layout
-  struct Class {
-    int begin = 0, count = 0, threads = kEnergyThreads;
-    LdsPlan plan;
-  };
-  std::vector<Class> classes;
+  std::vector<EnergyClass> classes;
   hipStream_t side = nullptr;
   hipEvent_t fork = nullptr, join = nullptr;
   DevMem d_sys, d_cs, d_off, d_order, d_vl, d_out, d_xyz;
@@ -648,46 +710,8 @@ static void batch_layout(EnergyBatch& B) {
     B.off[c] = B.nvals;
     B.nvals += dsys[B.conf_sys[c]].nvt;
   }
-  // class of each system: 1 when it fits half of the LDS at 512 threads
-  // (with or without its pair bits), else 0 (1024 threads)
-  const int nw = kEnergyThreads / kWave;
-  auto waves_of = [](int k) { return (k == 1 ? kEnergyThreads : kEnergyThreadsMax) / kWave; };
-  std::vector<int> cls(nsys);
-  for (int i = 0; i < nsys; ++i) {
-    const DevSys& d = dsys[i];
-    const size_t without = energy_lds_bytes(d.np, d.npnt, nw, d.nheavy, 0);
-    check_lds(without);
-    cls[i] = without <= kLdsBytes / 2 ? 1 : 0;
-  }
-  std::vector<int32_t> order(B.nconf);
-  for (int c = 0; c < B.nconf; ++c) order[c] = c;
-  std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
-    const int sa = B.conf_sys[a], sb = B.conf_sys[b];
-    if (cls[sa] != cls[sb]) return cls[sa] < cls[sb];
-    return dsys[sa].nat > dsys[sb].nat;
-  });
-  B.classes.clear();
-  for (int k = 0; k < kBatchClasses; ++k) {
-    EnergyBatch::Class C;
-    C.begin = -1;
-    size_t with = 0, without = 0;
-    for (int j = 0; j < B.nconf; ++j) {
-      const int si = B.conf_sys[order[j]];
-      if (cls[si] != k) continue;
-      if (C.begin < 0) C.begin = j;
-      ++C.count;
-      const DevSys& d = dsys[si];
-      with = std::max(with, energy_lds_bytes(d.np, d.npnt, waves_of(k), d.nheavy, pair_bit_words(d.nat, d.words)));
-      without = std::max(without, energy_lds_bytes(d.np, d.npnt, waves_of(k), d.nheavy, 0));
-    }
-    if (C.count == 0) continue;
-    check_lds(without);
-    C.threads = waves_of(k) * kWave;
-    const size_t cap = k == 1 ? kLdsBytes / 2 : kLdsBytes;
-    C.plan.bits_lds = with <= cap ? 1 : 0;
-    C.plan.bytes = with <= cap ? with : without;
-    B.classes.push_back(C);
-  }
+  std::vector<int32_t> order;
+  B.classes = plan_classes(dsys, B.conf_sys, order);
   DevSys* d_systems = static_cast<DevSys*>(B.d_sys.ensure(sizeof(DevSys) * nsys));
   int32_t* d_cs = static_cast<int32_t*>(B.d_cs.ensure(sizeof(int32_t) * std::max(B.nconf, 1)));
   int64_t* d_off = static_cast<int64_t*>(B.d_off.ensure(sizeof(int64_t) * std::max(B.nconf, 1)));
@@ -824,7 +848,7 @@ void engine_batch_run(EnergyBatch* b, int precision) {
     HIPCHK(hipStreamWaitEvent(B.side, B.fork, 0));
   }
   for (size_t k = 0; k < B.classes.size(); ++k) {
-    const EnergyBatch::Class& C = B.classes[k];
+    const EnergyClass& C = B.classes[k];
     launch_energy_on(k == 0 ? E.stream : B.side, d_systems, d_cs, d_off, d_vl, d_out, d_xyz, B.max_nat, C.count,
                      d_order + C.begin, C.plan, precision, d_err, d_work, C.threads);
   }
@@ -878,6 +902,107 @@ int engine_energy_batch(Engine*& e, std::vector<State*>& sys, const int32_t* con
   engine_batch_load(E.batch, vlvr, E.batch->nvals);
   engine_batch_run(E.batch, precision);
   engine_batch_read(E.batch, out);
+  return SMMP_OK;
+}
+
+// ------------------------------------------- energies without allocation
+//
+// engine_energy_batch_noalloc computes what engine_energy_batch does (the
+// same kernels, classes and thread counts, so the results are identical)
+// but allocates no device memory: the systems, the conformation tables, the
+// values, the results and the error flag of a call are laid out in
+// 'scratch', one file-scope array per device, which the device holds until
+// the process ends.  Calls on one device share that array: concurrent calls
+// on the same GPU race, and each call overwrites what the last one left.
+constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;  // 32 MiB per device
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};  // device address of 'scratch', per device
+
+// The current device's copy of 'scratch', looked up once per device with
+// that device current.  Host code passes this address, never the symbol.
+static char* noalloc_scratch() {
+  int dev = -1;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) throw DeviceError("energy_batch_noalloc> device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = p;
+  }
+  return static_cast<char*>(g_scratch_ptr[dev]);
+}
+
+int engine_energy_batch_noalloc(std::vector<State*>& sys, const int32_t* conf_sys, int nconf, const double* vlvr,
+                                int precision, double* out) {
+  if (nconf == 0) return SMMP_OK;
+  char* base = noalloc_scratch();
+  const int nsys = (int)sys.size();
+  std::vector<HostSys> hs(nsys);
+  for (int i = 0; i < nsys; ++i) {
+    if (sys[i]->itysol < 0)
+      throw Error("energy_batch_noalloc> the analytic solvent (itysol < 0) needs its own workspace: "
+                  "use smmp_energy_batch");
+    hs[i] = build_host_sys(*sys[i]);
+  }
+  // layout of this call in the scratch array
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t o = (off + 255) & ~size_t(255);
+    off = o + std::max<size_t>(bytes, 1);
+    return o;
+  };
+  std::vector<DevSys> dsys(nsys);
+  std::vector<std::vector<uint8_t>> blobs(nsys);
+  std::vector<size_t> o_sys(nsys);
+  for (int i = 0; i < nsys; ++i) o_sys[i] = take(system_blob(hs[i], nullptr, blobs[i], dsys[i]));
+  std::vector<int32_t> cs(conf_sys, conf_sys + nconf);
+  std::vector<int64_t> coff(nconf);
+  int64_t nvals = 0;
+  for (int c = 0; c < nconf; ++c) {
+    coff[c] = nvals;
+    nvals += hs[cs[c]].nvt;
+  }
+  const size_t o_dsys = take(sizeof(DevSys) * nsys), o_cs = take(sizeof(int32_t) * nconf),
+               o_off = take(sizeof(int64_t) * nconf), o_order = take(sizeof(int32_t) * nconf),
+               o_vl = take(sizeof(double) * nvals), o_out = take(sizeof(double) * kEnergyOut * nconf),
+               o_err = take(sizeof(int32_t));
+  if (off > kNoAllocScratchBytes)
+    throw Error("energy_batch_noalloc> this batch needs " + std::to_string(off) + " bytes of device scratch; the " +
+                "library holds " + std::to_string(kNoAllocScratchBytes) + " per device (use smmp_energy_batch)");
+  for (int i = 0; i < nsys; ++i) system_blob(hs[i], base + o_sys[i], blobs[i], dsys[i]);
+  std::vector<int32_t> order;
+  const std::vector<EnergyClass> classes = plan_classes(dsys, cs, order);
+  hipStream_t st = nullptr;  // the device's null stream: nothing to create
+  auto put = [&](size_t o, const void* src, size_t bytes) {
+    if (bytes) HIPCHK(hipMemcpyAsync(base + o, src, bytes, hipMemcpyHostToDevice, st));
+  };
+  for (int i = 0; i < nsys; ++i) put(o_sys[i], blobs[i].data(), blobs[i].size());
+  const int32_t zero = 0;
+  put(o_dsys, dsys.data(), sizeof(DevSys) * nsys);
+  put(o_cs, cs.data(), sizeof(int32_t) * nconf);
+  put(o_off, coff.data(), sizeof(int64_t) * nconf);
+  put(o_order, order.data(), sizeof(int32_t) * nconf);
+  put(o_vl, vlvr, sizeof(double) * nvals);
+  put(o_err, &zero, sizeof zero);  // the error flag starts at zero
+  const DevSys* d_systems = reinterpret_cast<const DevSys*>(base + o_dsys);
+  const int32_t* d_order = reinterpret_cast<const int32_t*>(base + o_order);
+  for (const EnergyClass& C : classes)
+    launch_energy_on(st, d_systems, reinterpret_cast<const int32_t*>(base + o_cs),
+                     reinterpret_cast<const int64_t*>(base + o_off), reinterpret_cast<const double*>(base + o_vl),
+                     reinterpret_cast<double*>(base + o_out), nullptr, 0, C.count, d_order + C.begin, C.plan,
+                     precision, reinterpret_cast<int32_t*>(base + o_err), nullptr, C.threads);
+  std::vector<double> full((size_t)kEnergyOut * nconf);
+  int32_t err = 0;
+  HIPCHK(hipMemcpyAsync(full.data(), base + o_out, sizeof(double) * full.size(), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(&err, base + o_err, sizeof err, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (err) throw Error("enysol> too many solvent neighbours for one atom");
+  for (int c = 0; c < nconf; ++c)
+    for (int k = 0; k < 8; ++k) out[8 * (size_t)c + k] = full[(size_t)kEnergyOut * c + k];
   return SMMP_OK;
 }
 

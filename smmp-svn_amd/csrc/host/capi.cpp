@@ -292,6 +292,23 @@ int smmp_energy_batch(smmp_ctx* const* ctxs, int nsys, const int32_t* conf_sys, 
   });
 }
 
+int smmp_energy_batch_noalloc(smmp_ctx* const* ctxs, int nsys, const int32_t* conf_sys, int nconf,
+                              const double* vlvr, int precision, double* out) {
+  if (!ctxs || nsys < 1 || !conf_sys || nconf < 0 || !vlvr || !out) return SMMP_EINVAL;
+  smmp_ctx* c0 = ctxs[0];
+  return guarded(c0, SMMP_EINVAL, [&] {
+    std::vector<State*> sys(nsys);
+    for (int i = 0; i < nsys; ++i) {
+      if (!ctxs[i] || ctxs[i]->s.ntlml < 1) throw Error("energy_batch_noalloc> context without a molecule");
+      sys[i] = &ctxs[i]->s;
+    }
+    for (int k = 0; k < nconf; ++k)
+      if (conf_sys[k] < 0 || conf_sys[k] >= nsys) throw Error("energy_batch_noalloc> conf_sys out of range");
+    if (precision != 0 && precision != 1) throw Error("energy_batch_noalloc> precision must be 0 or 1");
+    return engine_energy_batch_noalloc(sys, conf_sys, nconf, vlvr, precision, out);
+  });
+}
+
 struct smmp_batch {
   smmp_ctx* c0;
   EnergyBatch* b;

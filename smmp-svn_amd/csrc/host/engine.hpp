@@ -37,6 +37,8 @@ int engine_energy_batch(Engine*& e, std::vector<State*>& sys, const int32_t* con
                         const double* vlvr, int precision, double* out);
 // Resident batch (smmp_batch_*): systems and conformations kept on the device.
 struct EnergyBatch;
+int engine_energy_batch_noalloc(std::vector<State*>& sys, const int32_t* conf_sys, int nconf, const double* vlvr,
+                                int precision, double* out);
 EnergyBatch* engine_batch_create(Engine*& e, std::vector<State*>& sys, const int32_t* conf_sys, int nconf);
 void engine_batch_destroy(EnergyBatch* b);
 void engine_batch_load(EnergyBatch* b, const double* vlvr, int64_t nvals);

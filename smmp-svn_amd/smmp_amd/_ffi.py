@@ -73,6 +73,7 @@ def _load():
         "smmp_rgyr": (C.c_int, [P, C.c_int, dp, dp]),
         "smmp_helix": (C.c_int, [P, ip]),
         "smmp_energy_batch": (C.c_int, [C.POINTER(P), C.c_int, ip, C.c_int, dp, C.c_int, dp]),
+        "smmp_energy_batch_noalloc": (C.c_int, [C.POINTER(P), C.c_int, ip, C.c_int, dp, C.c_int, dp]),
         "smmp_batch_create": (C.c_int, [C.POINTER(P), C.c_int, ip, C.c_int, C.POINTER(P), C.POINTER(C.c_int64)]),
         "smmp_batch_load": (C.c_int, [P, dp, C.c_int64]),
         "smmp_batch_run": (C.c_int, [P, C.c_int]),
@@ -105,7 +106,7 @@ lib = _load()
 EXPORTS = ("smmp_ctx_create", "smmp_ctx_destroy", "smmp_last_error", "smmp_state_view", "smmp_state_count",
            "smmp_state_name", "smmp_init_energy", "smmp_init_molecule", "smmp_setvar", "smmp_energy",
            "smmp_enysol", "smmp_sgrnd", "smmp_grnd", "smmp_outvar", "smmp_metropolis", "smmp_partem_s", "smmp_rgyr", "smmp_redvar", "smmp_helix",
-           "smmp_energy_batch", "smmp_batch_create", "smmp_batch_load", "smmp_batch_run", "smmp_batch_read",
+           "smmp_energy_batch", "smmp_energy_batch_noalloc", "smmp_batch_create", "smmp_batch_load", "smmp_batch_run", "smmp_batch_read",
            "smmp_batch_destroy", "smmp_sample", "smmp_build_info", "smmp_profile", "smmp_profile_read",
            "smmp_profile_phases", "smmp_muca_update", "smmp_muca_final", "smmp_sampler_info",
            "smmp_mlp_size", "smmp_mlp_set", "smmp_mlp_get", "smmp_mlp_grad", "smmp_mlp_fit")
@@ -408,6 +409,18 @@ def energy_batch(ctxs, conf_sys, vlvr_flat, precision: int = 0) -> np.ndarray:
     vl = np.ascontiguousarray(vlvr_flat, dtype=np.float64)
     out = np.zeros((len(cs), 8))
     rc = lib.smmp_energy_batch(arr, len(ctxs), iptr(cs), len(cs), dptr(vl), precision, dptr(out))
+    ctxs[0].check(rc)
+    return out
+
+
+def energy_batch_noalloc(ctxs, conf_sys, vlvr_flat, precision: int = 0) -> np.ndarray:
+    """energy_batch through the entry point that allocates no device memory
+    (smmp_energy_batch_noalloc)."""
+    arr = (C.c_void_p * len(ctxs))(*[c.h for c in ctxs])
+    cs = np.ascontiguousarray(conf_sys, dtype=np.int32)
+    vl = np.ascontiguousarray(vlvr_flat, dtype=np.float64)
+    out = np.zeros((len(cs), 8))
+    rc = lib.smmp_energy_batch_noalloc(arr, len(ctxs), iptr(cs), len(cs), dptr(vl), precision, dptr(out))
     ctxs[0].check(rc)
     return out
 
