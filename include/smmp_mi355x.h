@@ -117,6 +117,23 @@ int smmp_rgyr(smmp_ctx *ctx, int nml, double *rgy, double *ee);
  * current variables. */
 int smmp_helix(smmp_ctx *ctx, int32_t *out);
 
+/* hbond(nml,mhb,ipr) smmp.pyf:1806, hbond.f:14: H-bonds of molecule nml
+ * (>= 1) by McDonald and Thornton's criteria over the energy's pair lists,
+ * for the context's current coordinates; *mhb counts those between residues
+ * more than four apart, h_bond.ntyhb/ixtyhb/nutyhb hold the type table.
+ * interhbond(mhb) smmp.pyf:2092, hbond.f:228: H-bonds between chains (all
+ * atom pairs), h_bond.mmhb per pair of chains. */
+int smmp_hbond(smmp_ctx *ctx, int nml, int32_t *mhb);
+int smmp_interhbond(smmp_ctx *ctx, int32_t *mhb);
+
+/* zimmer(nresi) smmp.pyf:3238, zimmer.f:15: Zimmerman code of residues
+ * 1..nresi from the current (phi, psi) into zimme.zimm. */
+int smmp_zimmer(smmp_ctx *ctx, int nresi);
+
+/* outpdb(nml,file) smmp.pyf:3810, outpdb.f:12: PDB file of molecule nml
+ * (0: all) from the current coordinates ('' = stdout). */
+int smmp_outpdb(smmp_ctx *ctx, int nml, const char *path);
+
 /* ---- batched entry points (no reference counterpart: the configs'
  *      many-replica and many-conformation workloads) ------------------- */
 

@@ -18,6 +18,8 @@
 !   partem libdir seqfile itysol seed nrep nequi nswp [switch]  partem_s run
 !   muca   libdir seqfile itysol seed nsweep nup temp kmin kmax ebin
 !   helix  libdir seqfile itysol seed nconf         helix() of drawn conformations
+!   obs    libdir seqfile itysol seed nconf prefix [gbfile boxsize]
+!          hbond, interhbond, zimmer and outpdb of drawn conformations
 !   esolan libdir seqfile radfile seed nconf       analytic solvent (itysol -1):
 !          energies of the start and of random conformations, and the
 !          twister position after each (esolan.f draws on degenerate geometry);
@@ -143,6 +145,62 @@ program ref_driver
       write(*,'(i0)') mhel
       write(*,'(i0)') nbet
       write(*,'(i0)') mbet
+    end do
+
+  case ('obs')
+    ! observables of conformations drawn as in 'helix' (coordinates built by
+    ! energy()): hbond of every chain (mhb and the type table), interhbond
+    ! (mmhb), zimmer, and outpdb(0) of the first two into <prefix>_k.pdb
+    call setup(2, .false.)
+    call iarg(5, iseed)
+    call iarg(6, nconf)
+    call get_command_argument(7, wfile)
+    if (command_argument_count() .ge. 9) then
+      call get_command_argument(8, arg)
+      call rarg(9, boxsize)
+      call read_gbpr(arg)
+    end if
+    call sgrnd(iseed)
+    write(*,'(a,i0)') '@nconf int 1'
+    write(*,'(i0)') nconf
+    do k = 1, nconf
+      do i = 1, nvr
+        j = idvr(i)
+        tmp = grnd()
+        if (tmp .lt. 0.35d0) then
+          vlvr(j) = (-70.0d0 + 70.0d0 * (grnd() - 0.5d0)) * cdr
+        else if (tmp .lt. 0.7d0) then
+          vlvr(j) = (-150.0d0 + 70.0d0 * (grnd() - 0.5d0)) * cdr
+        else if (tmp .lt. 0.85d0) then
+          vlvr(j) = (150.0d0 + 70.0d0 * (grnd() - 0.5d0)) * cdr
+        else
+          vlvr(j) = addang(pi, axvr(j)*(grnd()-0.5))
+        end if
+      end do
+      e = energy()
+      write(*,'(a,i0)') '@vlvr double ', nvr
+      do i = 1, nvr
+        write(*,'(es25.17)') vlvr(idvr(i))
+      end do
+      do kk = 1, ntlml
+        call hbond(kk, nhel, 0)
+        write(*,'(a,i0)') '@hbond int ', 2 + 2*ntyhb
+        write(*,'(i0)') nhel
+        write(*,'(i0)') ntyhb
+        write(*,'(i0)') (ixtyhb(i), i=1,ntyhb)
+        write(*,'(i0)') (nutyhb(i), i=1,ntyhb)
+      end do
+      call interhbond(nhel)
+      write(*,'(a,i0)') '@interhb int ', 1 + ntlml*ntlml
+      write(*,'(i0)') nhel
+      write(*,'(i0)') ((mmhb(i,j), i=1,ntlml), j=1,ntlml)
+      call zimmer(irsml2(ntlml))
+      write(*,'(a,i0)') '@zimm int ', irsml2(ntlml)
+      write(*,'(i0)') (ichar(zimm(i:i)), i=1,irsml2(ntlml))
+      if (k .le. 2) then
+        write(arg, '(a,a,i0,a)') trim(wfile), '_', k, '.pdb'
+        call outpdb(0, trim(arg))
+      end if
     end do
 
   case ('canon')

@@ -86,7 +86,9 @@ const ViewDesc kViews[] = {
     V1(solvent, rvdw, SMMP_F64, mxat, 1), V1(solvent, sigma, SMMP_F64, mxat, 1),
     V2(tes, spoint, SMMP_F64, mxpoint, 3), VS(pnt, npnt, SMMP_I32),
     VS(v_sol, vdvol, SMMP_F64), VS(v_sol, asa, SMMP_F64),
-    V2(h_bond, mmhb, SMMP_I32, mxml, mxml),
+    V2(h_bond, mmhb, SMMP_I32, mxml, mxml), VS(h_bond, ntyhb, SMMP_I32),
+    V1(h_bond, ixtyhb, SMMP_I32, State::mxtyhb, 1), V1(h_bond, nutyhb, SMMP_I32, State::mxtyhb, 1),
+    {"zimme.zimm", SMMP_CHR, 2, {mxrs, 1}, {1, 1}, [](State& s) -> void* { return (void*)s.zimm; }},
     VS(myblock, mti, SMMP_I32), V1(myblock, mt, SMMP_I32, 624, 0),
 };
 constexpr int kNViews = sizeof(kViews) / sizeof(kViews[0]);
@@ -261,6 +263,39 @@ int smmp_helix(smmp_ctx* c, int32_t* out) {
     if (!out) throw Error("helix> NULL argument");
     if (c->s.ntlml < 1) throw Error("helix> no molecule initialised");
     helix(c->s, out);
+    return SMMP_OK;
+  });
+}
+
+int smmp_hbond(smmp_ctx* c, int nml, int32_t* mhb) {
+  return guarded(c, SMMP_ESTATE, [&] {
+    if (!mhb) throw Error("hbond> NULL argument");
+    if (c->s.ntlml < 1) throw Error("hbond> no molecule initialised");
+    *mhb = hbond(c->s, nml);
+    return SMMP_OK;
+  });
+}
+
+int smmp_interhbond(smmp_ctx* c, int32_t* mhb) {
+  return guarded(c, SMMP_ESTATE, [&] {
+    if (!mhb) throw Error("interhbond> NULL argument");
+    *mhb = interhbond(c->s);
+    return SMMP_OK;
+  });
+}
+
+int smmp_zimmer(smmp_ctx* c, int nresi) {
+  return guarded(c, SMMP_ESTATE, [&] {
+    if (nresi < 0 || nresi > mxrs) throw Error("zimmer> nresi out of range");
+    zimmer(c->s, nresi);
+    return SMMP_OK;
+  });
+}
+
+int smmp_outpdb(smmp_ctx* c, int nml, const char* path) {
+  return guarded(c, SMMP_EINPUT, [&] {
+    if (c->s.ntlml < 1) throw Error("outpdb> no molecule initialised");
+    outpdb(c->s, nml, path ? std::string(path) : std::string());
     return SMMP_OK;
   });
 }

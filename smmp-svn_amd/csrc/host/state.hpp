@@ -131,7 +131,11 @@ struct State {
   F2<double, mxtysol, mxtyat> coef_sl, rad_vdw;
   double vdvol = 0, asa = 0, asap = 0, asah = 0;
   F1<double, mxrs> surfres;
-  // /h_bond/ mmhb, /zimme/ zimm, /myblock/ mt state
+  // /h_bond/ ntyhb, ixtyhb, nutyhb (hbond.f), mmhb (interhbond), /zimme/ zimm,
+  // /myblock/ mt state
+  static constexpr int mxtyhb = 100;
+  int32_t ntyhb = 0;
+  F1<int32_t, mxtyhb> ixtyhb, nutyhb;
   F2<int32_t, mxml, mxml> mmhb;
   char zimm[mxrs];
   int32_t mti = 625;
@@ -165,6 +169,10 @@ void init_molecule(State& s, int iabin, const std::string& grpn, const std::stri
 void setvar(State& s, int nml, const double* vlvrx);  // setvar.f:13-73 (1-based vlvrx(1..))
 void rgyr(const State& s, int nml, double* rgy, double* ee);  // rgyr.f:13-140 (rgy, ee)
 void helix(const State& s, int32_t out[4]);                   // helix.f:13-77 (nhel, mhel, nbet, mbet)
+int hbond(State& s, int nml);                                 // hbond.f:14-216 (mhb; ntyhb, ixtyhb, nutyhb)
+int interhbond(State& s);                                     // hbond.f:228-256 (mhb; mmhb)
+void zimmer(State& s, int nresi);                             // zimmer.f:15-93 (zimm)
+void outpdb(const State& s, int nml, const std::string& path);  // outpdb.f:12-185 ('' = stdout)
 void bldmol(State& s, int nml);             // bldmol.f:13-93
 // fnd3ba (bldmol.f:95-173) on the current topology; ixrfpt keeps the atoms
 // of the last bldmol call, which precede addend's renumbering after init.

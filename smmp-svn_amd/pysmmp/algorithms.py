@@ -8,7 +8,10 @@ import smmp
 class CanonicalMonteCarlo:
     """Canonical Monte Carlo with Metropolis weights."""
 
-    def __init__(self, myUniverse, nequi: int, sweeps: int, weight=lambda x: x * smmp.bet.beta):
+    def __init__(self, myUniverse, nequi: int, sweeps: int, weight=smmp.can_weight):
+        # the reference's default is lambda x: x * smmp.bet.beta (algorithms.py:19);
+        # smmp.can_weight has that meaning and runs the sweep on the GPU, a
+        # lambda passed here is honoured through the host-driven trial loop
         self._def_sweeps = sweeps
         self._def_equi = nequi
         self._univ = myUniverse

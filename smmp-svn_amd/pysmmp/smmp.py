@@ -8,15 +8,15 @@ the COMMON state on the host; energies and Metropolis sweeps run on the GPU.
 
 Differences from the f2py module (each stated where it applies):
   * errors raise ``smmp_amd._ffi.SmmpError`` instead of stopping the process;
-  * ``metropolis``'s weight callback must be linear in the energy (the
-    canonical weight, e.g. ``lambda x: x * smmp.bet.beta``: it is evaluated
-    on the host to read off beta) or a weight descriptor from
-    ``smmp_amd.weights``; other callables are rejected, because the weight is
-    evaluated inside the GPU sweep;
+  * ``metropolis``'s weight: ``smmp.can_weight`` and the objects of
+    ``smmp_amd.weights`` run the whole sweep in the GPU kernel; any other
+    Python callable is called twice per trial like the reference calls it,
+    from a host-driven trial loop with each energy on the GPU (exact, one
+    GPU energy per trial);
   * ``partem_s`` is exported (partem_s.f:13 has no f2py entry);
-  * observables and minimisers (hbond, contacts, zimmer, outpdb, minim,
-    rmsdfun, anneal) are outside the energy-and-move engine and raise
-    NotImplementedError; rgyr and helix are provided.
+  * observables rgyr, helix, hbond, interhbond, zimmer, outpdb and the
+    canon driver are provided; contacts, rmsdfun, minim, regul and anneal
+    raise NotImplementedError.
 """
 from __future__ import annotations
 
@@ -66,6 +66,8 @@ _SCALARS = {
     "isolty": ("itysol", "isolscl"),
     "updchois": ("rndord", "upchswitch", "bgsprob"),
     "bet": ("beta",),
+    "bounds": ("boxsize",),
+    "h_bond": ("ntyhb",),
     "updtparam": ("up2bmax", "up2bmin"),
     "mol_par": ("nvr",),
     "mol_i": ("ntlml",),
@@ -128,11 +130,101 @@ def outvar(nml: int, fileout: str = "") -> None:
     _ctx.outvar(int(nml), fileout)
 
 
+#: can_weight (canon.f:135-149): beta x with beta from /bet/; the sweep runs on the GPU
+can_weight = weights.CanonicalCommon(_ctx)
+
+
 def metropolis(eol1: float, acz: float, dummy) -> tuple[float, float]:
-    """smmp.pyf:22, metropolis.f:13: one sweep over the free variables on the
-    GPU.  ``dummy`` is the weight: a descriptor from smmp_amd.weights, or a
-    callable linear in the energy (its slope is beta)."""
-    return _ctx.metropolis(float(eol1), float(acz), weights.as_descriptor(dummy))
+    """smmp.pyf:22, metropolis.f:13: one sweep over the free variables.
+    ``dummy`` is the weight.  can_weight and the objects of smmp_amd.weights
+    (canonical, multicanonical) carry their weight as data and the whole
+    sweep runs in the GPU sweep kernel.  Any other callable is called as the
+    reference calls it, twice per trial, from the host-driven loop
+    (_metropolis_host): same draws, same decisions, one GPU energy per
+    trial."""
+    d = weights.as_descriptor(dummy)
+    if d is None:
+        return _metropolis_host(float(eol1), float(acz), dummy)
+    return _ctx.metropolis(float(eol1), float(acz), d)
+
+
+def _accanalyze(iupt: int, iupstate: int) -> None:
+    """metropolis.f:186-193: counts of trials and accepted trials per type."""
+    nc, na = updstats.ncalls, updstats.nacalls
+    nc[4] += 1
+    na[4] += iupstate
+    nc[iupt - 1] += 1
+    na[iupt - 1] += iupstate
+
+
+def _accept(delta: float) -> bool:
+    """metropolis.f:76-87: rd is drawn only when delta > 0."""
+    if delta <= 0.0:
+        return True
+    rd = grnd()
+    return float(np.exp(-min(delta, 100.0))) >= rd
+
+
+def _metropolis_host(eol1: float, acz: float, dummy) -> tuple[float, float]:
+    """metropolis.f:13-184 with a Python weight: the trial loop runs here in
+    the reference's order (variable choice, update type, proposal, the
+    callback on the new and the old energy, acceptance, restore), every
+    energy() on the GPU, every random number from the context's twister."""
+    nvr = int(mol_par.nvr)
+    idv = _ctx.idvr
+    ax = var_r.axvr
+    vl = var_r.vlvr
+    names = var_c.nmvr
+    rndord, upch = int(updchois.rndord), int(updchois.upchswitch)
+    eol = energy()
+    for nsw in range(1, nvr + 1):
+        ivar = nsw
+        if rndord:
+            ivar = min(1 + int(nvr * grnd()), nvr)  # u = 1 exactly would pass the end
+        jv = int(idv[ivar - 1])
+        if upch != 0 and names[0, jv - 1] != b"x":  # updtch1 / updtch2 (metropolis.f:196-220)
+            b = float(bet.beta)
+            prob = float(updchois.bgsprob) if upch == 1 else (b - float(updtparam.up2bmin)) / (
+                float(updtparam.up2bmax) - float(updtparam.up2bmin))
+            if grnd() < prob:  # BGS, inert in the reference (bgsnvar = 0)
+                _accanalyze(2, 0)
+                continue
+        vrol = float(vl[jv - 1])
+        dv = float(ax[jv - 1]) * (grnd() - 0.5)
+        vl[jv - 1] = _addang(vrol, dv)
+        enw = energy()
+        if _accept(float(dummy(enw)) - float(dummy(eol))):
+            eol, up = enw, 1
+        else:
+            vl[jv - 1] = vrol
+            up = 0
+        acz += up
+        _accanalyze(1, up)
+    gb = mol_r.gbpr
+    box = float(bounds.boxsize)
+    ntlml = int(mol_i.ntlml)
+    for iml in range(ntlml if ntlml > 1 else 0):  # rigid moves of each chain (metropolis.f:94-168)
+        for i in range(6):
+            gbol = float(gb[i, iml])
+            if i < 3:
+                g = gbol + (grnd() - 0.5)
+                if box > 0:
+                    g = min(g, box) if g > box else max(g, -box)
+            else:
+                g = (grnd() - 0.5) * (np.pi if i == 4 else 2.0 * np.pi)
+            gb[i, iml] = g
+            enw = energy()
+            if _accept(float(dummy(enw)) - float(dummy(eol))):
+                eol, up = enw, 1
+            else:
+                gb[i, iml] = gbol
+                up = 0
+            acz += up
+            _accanalyze(3 if i < 3 else 4, up)
+    enw = energy()
+    if abs(eol - enw) > 0.000001 and eol < 100000:
+        raise _ffi.SmmpError(-4, "metropolis: eol and enw difference unacceptable")
+    return eol, acz
 
 
 def partem_s(num_rep: int, nequi: int, nswp: int, nmes: int = 1, nsave: int = 1, newsta: bool = True,
@@ -233,8 +325,69 @@ def _outside(name: str):
     return f
 
 
-for _n in ("hbond", "interhbond", "contacts", "zimmer", "outpdb", "minim", "rmsdfun",
-           "anneal", "canon", "regul", "gradient", "esolan", "eninteract"):
+def hbond(nml: int, ipr: int = 0) -> int:
+    """smmp.pyf:1806, hbond.f:14: H-bonds of molecule nml between residues
+    more than four apart (mhb); the type table goes to h_bond.  ipr > 0
+    (printing the bonds) is not provided."""
+    return _ctx.hbond(int(nml))
+
+
+def interhbond() -> int:
+    """smmp.pyf:2092, hbond.f:228: H-bonds between chains (h_bond.mmhb)."""
+    return _ctx.interhbond()
+
+
+def zimmer(nresi: int) -> None:
+    """smmp.pyf:3238, zimmer.f:15: Zimmerman codes into zimme.zimm."""
+    _ctx.zimmer(int(nresi))
+
+
+def outpdb(nml: int, filename: str = "") -> None:
+    """smmp.pyf:3810, outpdb.f:12: PDB file of molecule nml (0: all)."""
+    _ctx.outpdb(int(nml), filename)
+
+
+def canon(nequi: int, nswp: int, nmes: int, temp: float, lrand: bool = True) -> None:
+    """smmp.pyf:3522, canon.f:12-132: canonical Metropolis simulation at temp
+    with can_weight.  Writes start.pdb, minen.pdb at each new minimum, every
+    nmes sweeps a time.d line (sweep, energy, radius of gyration of chain 1,
+    helix counts, H-bonds of the last chain, eyel, eyhb, eyvr, eysl in
+    canon.f:110-112's format), then lastconf.var and final.pdb."""
+    bet.beta = 1.0 / (temp * 1.98773e-3)
+    if lrand:  # canon.f:62-67
+        vl = _ctx.vlvr()
+        ax = var_r.axvr
+        for iv in _ctx.idvr:
+            vl[iv - 1] = _addang(np.pi, float(ax[iv - 1]) * (grnd() - 0.5))
+        _ctx.set_vlvr(vl)
+    eol = energy()
+    e_min = eol
+    outpdb(0, "start.pdb")
+    acz = 0.0
+    for _ in range(int(nequi)):
+        eol, acz = metropolis(eol, acz, can_weight)
+    acz = 0.0
+    ntlml = int(mol_i.ntlml)
+    with open("time.d", "w") as td:
+        for nsw in range(0, int(nswp) + 1):
+            eol, acz = metropolis(eol, acz, can_weight)
+            if eol < e_min:
+                e_min = eol
+                outpdb(0, "minen.pdb")
+            if nsw % int(nmes) == 0:
+                rgy, _ = rgyr(1)
+                nhel, mhel, nbet, mbet = helix()
+                mhb = 0
+                for i in range(1, ntlml + 1):
+                    mhb = hbond(i, 0)
+                td.write(f"{nsw:7d}{eol:15.3f}{rgy:15.3f}{nhel:7d}{mhel:7d}{nbet:7d}{mbet:7d}{mhb:7d}"
+                         f"{float(eny_r.eyel):15.3f}{float(eny_r.eyhb):15.3f}{float(eny_r.eyvr):15.3f}"
+                         f"{float(eny_r.eysl):15.3f}\n")
+    outvar(0, "lastconf.var")
+    outpdb(0, "final.pdb")
+
+
+for _n in ("contacts", "minim", "rmsdfun", "anneal", "regul", "gradient", "esolan", "eninteract"):
     globals()[_n] = _outside(_n)
 
 

@@ -72,6 +72,10 @@ def _load():
         "smmp_redvar": (C.c_int, [P, C.c_char_p, dp]),
         "smmp_rgyr": (C.c_int, [P, C.c_int, dp, dp]),
         "smmp_helix": (C.c_int, [P, ip]),
+        "smmp_hbond": (C.c_int, [P, C.c_int, ip]),
+        "smmp_interhbond": (C.c_int, [P, ip]),
+        "smmp_zimmer": (C.c_int, [P, C.c_int]),
+        "smmp_outpdb": (C.c_int, [P, C.c_int, C.c_char_p]),
         "smmp_energy_batch": (C.c_int, [C.POINTER(P), C.c_int, ip, C.c_int, dp, C.c_int, dp]),
         "smmp_energy_batch_noalloc": (C.c_int, [C.POINTER(P), C.c_int, ip, C.c_int, dp, C.c_int, dp]),
         "smmp_batch_create": (C.c_int, [C.POINTER(P), C.c_int, ip, C.c_int, C.POINTER(P), C.POINTER(C.c_int64)]),
@@ -106,6 +110,7 @@ lib = _load()
 EXPORTS = ("smmp_ctx_create", "smmp_ctx_destroy", "smmp_last_error", "smmp_state_view", "smmp_state_count",
            "smmp_state_name", "smmp_init_energy", "smmp_init_molecule", "smmp_setvar", "smmp_energy",
            "smmp_enysol", "smmp_sgrnd", "smmp_grnd", "smmp_outvar", "smmp_metropolis", "smmp_partem_s", "smmp_rgyr", "smmp_redvar", "smmp_helix",
+           "smmp_hbond", "smmp_interhbond", "smmp_zimmer", "smmp_outpdb",
            "smmp_energy_batch", "smmp_energy_batch_noalloc", "smmp_batch_create", "smmp_batch_load", "smmp_batch_run", "smmp_batch_read",
            "smmp_batch_destroy", "smmp_sample", "smmp_build_info", "smmp_profile", "smmp_profile_read",
            "smmp_profile_phases", "smmp_muca_update", "smmp_muca_final", "smmp_sampler_info",
@@ -323,6 +328,27 @@ class Context:
         out = np.zeros(4, dtype=np.int32)
         self.check(lib.smmp_helix(self.h, iptr(out)))
         return tuple(int(x) for x in out)
+
+    def hbond(self, nml: int) -> int:
+        """hbond.f: H-bonds of molecule nml (>= 1) between residues more than
+        four apart; the type table is left in h_bond.ntyhb/ixtyhb/nutyhb."""
+        m = C.c_int32()
+        self.check(lib.smmp_hbond(self.h, int(nml), C.byref(m)))
+        return int(m.value)
+
+    def interhbond(self) -> int:
+        """H-bonds between chains (h_bond.mmhb per pair of chains)."""
+        m = C.c_int32()
+        self.check(lib.smmp_interhbond(self.h, C.byref(m)))
+        return int(m.value)
+
+    def zimmer(self, nresi: int) -> str:
+        """Zimmerman code of residues 1..nresi (zimme.zimm)."""
+        self.check(lib.smmp_zimmer(self.h, int(nresi)))
+        return self.view("zimme.zimm").ravel(order="F")[:nresi].tobytes().decode()
+
+    def outpdb(self, nml: int, path: str = "") -> None:
+        self.check(lib.smmp_outpdb(self.h, int(nml), path.encode()))
 
     def redvar(self, path: str) -> np.ndarray:
         """All variables after reading a .var file over the current ones

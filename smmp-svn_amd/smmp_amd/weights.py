@@ -2,6 +2,8 @@
 callback, smmp.pyf:3-10; the sweep kernels need it as data).
 
   Canonical(beta)                       w(x) = beta x            (canon.f:135-149)
+  CanonicalCommon(ctx)                  w(x) = beta x, beta read from the
+                                        context's /bet/ at each call (can_weight)
   Multicanonical(b, alpha, kmin, kmax, ebin, beta=None)
       beta None:  w(x) = b(k) x + alpha(k)              (muca_weight2, mulcan_par_mod.f90:398-412)
       beta given: w(x) = (beta + b(k)) x + alpha(k)     (muca_weight, mulcan_par_mod.f90:380-395)
@@ -17,6 +19,24 @@ from . import _ffi
 class Canonical:
     def __init__(self, beta: float):
         self.beta = float(beta)
+
+    def descriptor(self) -> _ffi.Weight:
+        return _ffi.Weight(kind=0, beta=self.beta)
+
+    def __call__(self, x: float) -> float:
+        return self.beta * x
+
+
+class CanonicalCommon:
+    """can_weight of canon.f:135-149: beta x with beta taken from the
+    context's COMMON /bet/ when the sweep starts."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+
+    @property
+    def beta(self) -> float:
+        return float(self.ctx.view("bet.beta")[0])
 
     def descriptor(self) -> _ffi.Weight:
         return _ffi.Weight(kind=0, beta=self.beta)
@@ -51,20 +71,15 @@ class Multicanonical:
                            alpha=_ffi.dptr(self.alpha))
 
 
-def as_descriptor(w) -> _ffi.Weight:
-    """A weight descriptor for w: a Canonical/Multicanonical object, a
-    descriptor, or a callable linear in the energy (read off as beta)."""
+def as_descriptor(w) -> _ffi.Weight | None:
+    """The weight descriptor of w (a descriptor, or an object of this module),
+    or None for any other callable: its values are known only by calling it,
+    so the sweep must run where it can be called (the host-driven
+    Metropolis loop).  Nothing is inferred from sample values."""
     if isinstance(w, _ffi.Weight):
         return w
     if hasattr(w, "descriptor"):
         return w.descriptor()
     if callable(w):
-        k = float(w(1.0))
-        ok = float(w(0.0)) == 0.0
-        for x in (2.0, -3.5, 123.25):
-            ok = ok and abs(float(w(x)) - k * x) <= 1e-12 * max(1.0, abs(k * x))
-        if not ok:
-            raise TypeError("metropolis: the weight callable is not linear in the energy; pass a "
-                            "smmp_amd.weights.Multicanonical (the GPU sweep evaluates the weight itself)")
-        return _ffi.Weight(kind=0, beta=k)
+        return None
     raise TypeError(f"metropolis: unsupported weight {w!r}")

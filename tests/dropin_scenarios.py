@@ -42,12 +42,8 @@ def setup_cpu(args):
     out["rgyr_np"] = [float(np.sqrt(((xyz - xyz.mean(0)) ** 2).sum(1).mean())),
                       float(np.linalg.norm(xyz[c_last] - xyz[c_first]))]
     from smmp_amd import weights
-    out["beta_from_lambda"] = weights.as_descriptor(lambda x: x * smmp.bet.beta).beta
-    try:
-        weights.as_descriptor(lambda x: x * x)
-        out["nonlinear"] = "accepted"
-    except TypeError:
-        out["nonlinear"] = "rejected"
+    out["beta_from_can_weight"] = weights.as_descriptor(smmp.can_weight).beta
+    out["lambda_descriptor"] = repr(weights.as_descriptor(lambda x: x * smmp.bet.beta))  # None: host loop
     return out
 
 
@@ -60,13 +56,63 @@ def canonical_gpu(args):
     p = protein.Protein(os.path.join(GOLDEN, "enkefa.seq"), os.path.join(GOLDEN, "enkefa_c1.var"))
     u.add(p)
     smmp.sgrnd(args["seed"])
-    mc = algorithms.CanonicalMonteCarlo(u, 0, args["sweeps"])
+    if args.get("weight") == "lambda":  # the reference's default weight, through the host-driven loop
+        mc = algorithms.CanonicalMonteCarlo(u, 0, args["sweeps"], weight=lambda x: x * smmp.bet.beta)
+    else:
+        mc = algorithms.CanonicalMonteCarlo(u, 0, args["sweeps"])
     energies, accs = [], []
     for _ in range(args["sweeps"]):
         mc.sweep()
         energies.append(u.energy())
         accs.append(mc._acc)
     return {"energies": energies, "acc": accs, "rate": mc.acceptanceRate(), "e0": mc.minimumEnergy()[0]}
+
+
+def callback_gpu(args):
+    """smmp.metropolis with a Python weight callable (smmp.pyf:3-10) over the
+    table of tests/golden/metro_enkefa.npz, the update-type switches set."""
+    import smmp
+    import protein
+    import universe
+    g = np.load(os.path.join(GOLDEN, "metro_enkefa.npz"), allow_pickle=False)
+    tag = args["tag"]
+    seed, temp, nsw, upch, bgs, rnd = g[f"{tag}_args"]
+    u = universe.Universe(T=float(temp), st=0, libdir=LIBDIR)
+    u.add(protein.Protein(os.path.join(GOLDEN, "enkefa.seq"), os.path.join(GOLDEN, "enkefa_c1.var")))
+    smmp.updchois.upchswitch = int(upch)
+    smmp.updchois.bgsprob = float(bgs)
+    smmp.updchois.rndord = int(rnd)
+    kmin, ebin = int(g["muca_kmin"][0]), float(g["muca_ebin"][0])
+    b, al = g["muca_b"], g["muca_alpha"]
+    kmax = kmin + len(b) - 1
+
+    def muca(x):  # muca_weight2 as a plain Python function
+        k = int(min(kmax, max(kmin, x / ebin + np.copysign(0.5, x))))
+        return b[k - kmin] * x + al[k - kmin]
+
+    w = muca if tag.startswith("muca") else (lambda x: x * smmp.bet.beta)
+    smmp.sgrnd(int(seed))
+    eol, acz, sweeps = 0.0, 0.0, []
+    for _ in range(int(nsw)):
+        eol, acz = smmp.metropolis(eol, acz, w)
+        sweeps.append([eol, acz])
+    v = smmp.var_r.vlvr
+    return {"sweeps": sweeps, "vlvr": [float(v[i - 1]) for i in smmp.context().idvr],
+            "mti": int(smmp.myblock.mti), "ncalls": [int(x) for x in smmp.updstats.ncalls] +
+            [int(x) for x in smmp.updstats.nacalls]}
+
+
+def canon_gpu(args):
+    """smmp.canon (canon.f) from enkefa_c1.var without a random start."""
+    import smmp
+    import protein
+    import universe
+    os.chdir(args["dir"])
+    u = universe.Universe(T=300, st=0, libdir=LIBDIR)
+    u.add(protein.Protein(os.path.join(GOLDEN, "enkefa.seq"), os.path.join(GOLDEN, "enkefa_c1.var")))
+    smmp.sgrnd(args["seed"])
+    smmp.canon(0, args["nswp"], args["nmes"], 300.0, False)
+    return {"files": sorted(os.listdir("."))}
 
 
 def energy_gpu(args):

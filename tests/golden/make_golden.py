@@ -315,6 +315,47 @@ def metro_fixtures(tmp: str) -> None:
     save("metro_enkefa.npz", **out)
 
 
+OBS_CASES = {
+    # (chains, seed, conformations, gbpr of chains 2.. or None)
+    "c2_20mer": ([workloads.random_sequence(20, 2)], 77, 30, None),
+    "len10": ([workloads.random_sequence(10, 10)], 78, 30, None),
+    "pair": ([["gly", "ser", "asn", "gln", "ala"], ["thr", "gln", "asn", "ser"]], 79, 40,
+             {2: (2.5, 1.5, 0.5, 0.4, -0.3, 1.1)}),
+}
+
+
+def obs_fixtures(tmp: str) -> None:
+    """hbond, interhbond, zimmer and outpdb (hbond.f, zimmer.f, outpdb.f) of
+    conformations drawn near helix and sheet angles."""
+    for tag, (chains, seed, nconf, gb) in OBS_CASES.items():
+        seqf = write_multi_seq(os.path.join(tmp, f"obs_{tag}.seq"), chains)
+        prefix = os.path.join(tmp, f"obs_{tag}")
+        args = ["obs", LIBDIR, seqf, "1", str(seed), str(nconf), prefix]
+        if gb:
+            gbf = os.path.join(tmp, f"obs_{tag}.gb")
+            with open(gbf, "w") as f:
+                for iml, vals in gb.items():
+                    for k, v in enumerate(vals):
+                        f.write(f"{iml} {k + 1} {v!r}\n")
+            args += [gbf, "0.0"]
+        out = run(args, tmp)
+        nch = len(chains)
+        hb = out["hbond"]  # nconf * nch blocks: mhb, ntyhb, ixtyhb.., nutyhb..
+        mhb = np.array([h[0] for h in hb]).reshape(nconf, nch)
+        ntyhb = np.array([h[1] for h in hb]).reshape(nconf, nch)
+        types = np.zeros((nconf, nch, 2, 100), dtype=np.int64)
+        for k, h in enumerate(hb):
+            n = int(h[1])
+            types[k // nch, k % nch, 0, :n] = h[2:2 + n]
+            types[k // nch, k % nch, 1, :n] = h[2 + n:2 + 2 * n]
+        inter = np.stack(out["interhb"])
+        pdbs = [open(f"{prefix}_{k}.pdb").read() for k in (1, 2)]
+        save(f"obs_{tag}.npz", chains=np.array(["-".join(c) for c in chains], dtype="<U128"),
+             seed=np.array([seed]), gbpr_in=np.array([[iml] + list(v) for iml, v in (gb or {}).items()]).reshape(-1, 7),
+             vlvr=np.stack(out["vlvr"]), mhb=mhb, ntyhb=ntyhb, hbtypes=types, inter_mhb=inter[:, 0],
+             mmhb=inter[:, 1:], zimm=np.stack(out["zimm"]), pdb1=np.array(pdbs[0]), pdb2=np.array(pdbs[1]))
+
+
 def main(only: list[str]) -> None:
     if not os.path.exists(DRIVER):
         raise SystemExit("build the reference first: make -C oracle ref")
@@ -344,6 +385,8 @@ def main(only: list[str]) -> None:
             out = run(["helix", LIBDIR, seqf, "1", "31", "40"], tmp)
             save(f"helix_{tag}.npz", seq_in=np.array(seq, dtype="<U8"), vlvr=np.stack(out["vlvr"]),
                  helix=np.stack(out["helix"]))
+    if not only or "obs" in only:
+        obs_fixtures(tmp)
     if not only or "metro" in only:
         metro_fixtures(tmp)
     if only and "restart" in only:  # the len10 run of partem_ext_len10.npz again, for its files

@@ -29,6 +29,45 @@ def test_canonical_monte_carlo_trajectory_vs_reference():
     assert np.abs(e - r["canon_sweeps"][:, 0]).max() <= 1e-8 * np.abs(e).max()
 
 
+def test_canonical_monte_carlo_with_a_lambda_weight_vs_reference():
+    """The reference's default weight, lambda x: x * smmp.bet.beta, goes
+    through the host-driven trial loop and gives the same trajectory."""
+    r = golden("enkefa_runs.npz")
+    out = run("canonical_gpu", {"seed": 81236, "sweeps": 12, "weight": "lambda"})
+    assert np.array_equal(np.array(out["acc"]), r["canon_sweeps"][:12, 1])
+    e = np.array(out["energies"])
+    assert np.abs(e - r["canon_sweeps"][:12, 0]).max() <= 1e-8 * np.abs(e).max()
+
+
+@pytest.mark.parametrize("tag", ["muca", "upch1", "muca_upch1"])
+def test_python_weight_callback_vs_reference(tag):
+    """A Python weight callable (muca_weight2 written as a plain function, or
+    a lambda) with update-type draws: move for move the Fortran's run."""
+    g = golden("metro_enkefa.npz")
+    out = run("callback_gpu", {"tag": tag})
+    sw = np.array(out["sweeps"])
+    assert np.array_equal(sw[:, 1], g[f"{tag}_sweeps"][:, 1])
+    assert np.abs(sw[:, 0] - g[f"{tag}_sweeps"][:, 0]).max() <= 1e-8 * max(1.0, np.abs(sw[:, 0]).max())
+    assert np.abs(np.array(out["vlvr"]) - g[f"{tag}_vlvr"]).max() < 1e-12
+    assert out["mti"] == int(g[f"{tag}_mti"][0])
+    assert out["ncalls"] == [int(x) for x in g[f"{tag}_ncalls"]]
+
+
+def test_canon_time_d_vs_reference(tmp_path):
+    """canon.f's driver: time.d carries the energies of the reference's
+    Metropolis run (sweeps 0..nswp, canon.f:81-112, format i7,2f15.3,5i7,
+    4f15.3), and the files canon.f writes exist."""
+    r = golden("enkefa_runs.npz")
+    out = run("canon_gpu", {"dir": str(tmp_path), "seed": 81236, "nswp": 19, "nmes": 1})
+    for f in ("start.pdb", "final.pdb", "lastconf.var", "time.d"):
+        assert f in out["files"], f
+    td = [line for line in open(tmp_path / "time.d").read().splitlines()]
+    assert len(td) == 20 and all(len(line) == 7 + 30 + 35 + 60 for line in td)
+    e = np.array([float(line[7:22]) for line in td])
+    assert np.abs(e - r["canon_sweeps"][:20, 0]).max() <= 6e-4
+    assert [int(line[:7]) for line in td] == list(range(20))
+
+
 def test_partem_s_files_vs_reference(tmp_path):
     p = golden("partem_ext_len10.npz")
     from smmp_amd import workloads
