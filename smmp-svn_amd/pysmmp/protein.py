@@ -6,9 +6,14 @@ slots below it through the -1 lower bound of xat, INCL.H:237 vs
 protein.py:42); atom lists include the last atom of the range (the
 reference stops one short); residues() returns whole residue names (the
 reference takes one character of each); saveVariables names fixed torsions
-correctly (the reference reuses the previous variable's name); PDB input and
-the observables hbond, contacts and savePDB are outside this engine
-(rgyr is provided)."""
+correctly (the reference reuses the previous variable's name); savePDB
+writes this protein's chain (the reference passes its 0-based id to outpdb,
+so the first protein writes every chain); directionVector normalises by the
+vector's length (the reference's reduce leaves the first component
+unsquared); strand counts the free variables of this chain only (the
+reference's loop bound mixes variable and free-variable indices and runs
+into the next chain).  PDB input,
+contacts and rmsdfun are outside this engine."""
 from __future__ import annotations
 
 import sys
@@ -144,7 +149,54 @@ class Protein:
         return smmp.contacts()
 
     def savePDB(self, file: str = "") -> None:
-        smmp.outpdb(self._id, file)
+        """The chain's coordinates in PDB format (outpdb.f), to file or stdout."""
+        smmp.outpdb(self._id + 1, file)
+
+    def strand(self) -> int:
+        """Residues of this chain in a beta-strand-like configuration: free
+        phi within 30 degrees of -150 with the next free variable (psi)
+        within 30 of 150 (protein.py:184-205)."""
+        max_dev, phi_sheet, psi_sheet = 30.0, -150.0, 150.0
+        ifivr = int(smmp.mol_i.ivrml1[self._id])
+        idvr = smmp.context().idvr
+        nvr = len(idvr)
+        count = 0
+        i = 0
+        while i < nvr and idvr[i] < ifivr:
+            i += 1
+        last = ifivr + int(smmp.mol_i.nvrml[self._id])  # first variable of the next chain
+        for j in range(i, nvr):
+            if idvr[j] >= last:
+                break
+            iv = int(idvr[j]) - 1
+            if self._var_name(iv) == "phi" and j + 1 < nvr:
+                phi = float(smmp.var_r.vlvr[iv]) * CRD
+                psi = float(smmp.var_r.vlvr[int(idvr[j + 1]) - 1]) * CRD
+                if abs(phi - phi_sheet) < max_dev and abs(psi - psi_sheet) < max_dev:
+                    count += 1
+        return count
+
+    def directionVector(self):
+        """Unit vector from the first to the last C-alpha atom of the chain
+        (protein.py:207-221)."""
+        beginning = end = None
+        for a in self.atoms():
+            if a.name().strip() == "ca":
+                if beginning is None:
+                    beginning = a.position()
+                else:
+                    end = a.position()
+        if beginning is None or end is None:
+            raise ValueError("directionVector: the chain needs two C-alpha atoms")
+        v = np.array(end, dtype=np.float64) - np.array(beginning, dtype=np.float64)
+        return list(v / np.linalg.norm(v))
+
+    def zimmer(self) -> str:
+        """Zimmerman codes of the chain's residues (zimmer.f, protein.py:265-270)."""
+        first = int(smmp.mol_i.irsml1[self._id]) - 1
+        last = int(smmp.mol_i.irsml2[self._id])
+        smmp.zimmer(last)
+        return smmp.zimme.zimm.ravel(order="F")[first:last].tobytes().decode()
 
     def saveVariables(self, file: str = "") -> None:
         """The conformation in the .var format (residue : name : degrees,

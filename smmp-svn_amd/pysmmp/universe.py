@@ -51,6 +51,30 @@ class Universe:
     def helix(self):
         return smmp.helix()
 
+    def sheet(self):
+        """Candidate beta sheets between the proteins (universe.py:66-88):
+        pairs (i, j, +1 parallel / -1 antiparallel) of proteins with at least
+        two strand residues each, at least three H-bonds between them in
+        h_bond.mmhb (as the last interhbond left it) and end-to-end
+        directions with |cos| > 0.7."""
+        min_strand, min_hb = 2, 3
+        mmhb = smmp.h_bond.mmhb
+        res = []
+        objs = self._objects
+        for i in range(len(objs)):
+            if objs[i].strand() < min_strand:
+                continue
+            for j in range(i, len(objs)):
+                if mmhb[objs[i].id()][objs[j].id()] >= min_hb and objs[j].strand() >= min_strand:
+                    print(f"Found a candidate for a sheet between {i} and {j}.")
+                    d1, d2 = objs[i].directionVector(), objs[j].directionVector()
+                    c = sum(d1[k] * d2[k] for k in range(3))
+                    if c > 0.7:
+                        res.append([i, j, 1])
+                    elif c < -0.7:
+                        res.append([i, j, -1])
+        return res
+
     def temperature(self) -> float:
         return 1.0 / (float(smmp.bet.beta) * GASC)
 

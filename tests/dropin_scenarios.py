@@ -47,6 +47,41 @@ def setup_cpu(args):
     return out
 
 
+def sheet_cpu(args):
+    """Two strands (every free phi -150, psi 150) side by side: strand
+    counts, direction vectors, Zimmerman codes, hbond through Protein and
+    Universe, and sheet() with the chain-pair H-bond table set by hand."""
+    import smmp
+    import protein
+    import universe
+    u = universe.Universe(T=300, st=0, libdir=LIBDIR)
+    seq = os.path.join(args["dir"], "s.seq")
+    with open(seq, "w") as f:
+        f.write("#  strand\nGly Ala Ser Val Thr Ala Gly\n")  # '#': a new molecule per file (redseq.f:58-74)
+    p1 = protein.Protein(seq)
+    p2 = protein.Protein(seq)
+    u.add(p1)
+    u.add(p2)
+    v = smmp.var_r.vlvr
+    for i in smmp.context().idvr:
+        nm = b"".join(smmp.var_c.nmvr[:, i - 1]).decode()
+        if nm == "phi":
+            v[i - 1] = np.radians(-150.0)
+        elif nm == "psi":
+            v[i - 1] = np.radians(150.0)
+    smmp.mol_r.gbpr[0, 1] = 4.8   # the second chain 4.8 A along x
+    for m in (1, 2):              # coordinates of both chains (host rebuild)
+        smmp.setvar(m, v)
+    nphi = sum(1 for i in smmp.context().idvr if b"".join(smmp.var_c.nmvr[:, i - 1]) == b"phi")
+    out = {"strand": [p1.strand(), p2.strand()], "nphi": nphi,
+           "dir": [p1.directionVector(), p2.directionVector()], "zimmer": [p1.zimmer(), p2.zimmer()],
+           "hbond": [p1.hbond(), p2.hbond()], "u_hbond": u.hbond()}
+    out["sheet_none"] = u.sheet() if int(smmp.h_bond.mmhb[0, 1]) < 3 else "skip"
+    smmp.h_bond.mmhb[0, 1] = 3
+    out["sheet"] = u.sheet()
+    return out
+
+
 def canonical_gpu(args):
     import smmp
     import algorithms

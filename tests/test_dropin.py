@@ -79,3 +79,18 @@ def test_muca_update_matches_oracle_recursion():
     assert np.allclose(b2, b + beta) and a2[-1] == 0.0
     for i in range(n - 2, -1, -1):
         assert a2[i] == a2[i + 1] + (b2[i + 1] - b2[i]) * (kmin + i + 0.5) * ebin
+
+
+def test_strand_direction_zimmer_and_sheet(tmp_path):
+    """Protein.strand/directionVector/zimmer and Universe.sheet
+    (protein.py:184-221, 265-270; universe.py:66-88) on two extended chains."""
+    out = run("sheet_cpu", {"dir": str(tmp_path)})
+    per_chain = out["nphi"] // 2
+    assert out["strand"] == [per_chain, per_chain]   # every free phi/psi at (-150, 150)
+    for d in out["dir"]:
+        assert abs(np.linalg.norm(d) - 1.0) < 1e-12
+    assert abs(np.dot(out["dir"][0], out["dir"][1]) - 1.0) < 1e-9  # same shape, shifted chain
+    # Zimmerman: phi = -150 (<= -110), psi = 150 (> 110) is 'E'; gly 1 has no free phi ('-')
+    for z in out["zimmer"]:
+        assert len(z) == 7 and set(z[1:]) == {"E"}
+    assert out["sheet"] == [[0, 1, 1]]               # parallel strands with mmhb(1, 2) = 3
