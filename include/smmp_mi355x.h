@@ -239,19 +239,26 @@ int smmp_sample(smmp_ctx *ctx, const smmp_sampler_cfg *cfg, double *vlvr, double
                 double *acc, int32_t *ipoi, int32_t *hist, double *traj);
 
 /* ---- the delayed-acceptance surrogate (no reference counterpart) -------
- * A small MLP over residue windows (sin/cos of the variables of residues
- * r-1, r, r+1 and embeddings of their types, three hidden layers of 64):
- * E~ = scale * sum over residues of its output.  The context holds one
- * network (smmp_mlp_size() FP32 parameters) and its Adam moments on the
+ * An MLP over the whole system: per residue (all chains in order) sin and
+ * cos of its up to 10 variables and a 16-dimensional embedding of its type
+ * (36 inputs per residue), three SiLU hidden layers of 256, one output;
+ * E~ = scale * output.  For 36 residues that is 1296 inputs and 464,400
+ * parameters.  The context holds one network (smmp_mlp_size(ctx) FP32
+ * parameters, laid out as smmp_mlp_layout gives) and its Adam moments on the
  * device; training and evaluation run BF16 MFMA kernels with FP32
- * accumulation.  All arrays below are host arrays. */
-int smmp_mlp_size(void);
+ * accumulation.  All arrays below are host arrays.
+ * smmp_mlp_layout: out[16] = parameter count, residues, inputs, hidden
+ * width, offsets of W1 (inputs x hidden, input-major), b1, W2 (hidden x
+ * hidden), b2, W3, b3, w4, b4, embeddings (types x 16), embedding width,
+ * embedding rows, variables per residue. */
+int smmp_mlp_layout(const smmp_ctx *ctx, int32_t *out);
+int smmp_mlp_size(const smmp_ctx *ctx);
 /* Sets the context's network (and clears its Adam moments) / reads it. */
 int smmp_mlp_set(smmp_ctx *ctx, const float *theta);
 int smmp_mlp_get(smmp_ctx *ctx, float *theta);
 /* Outputs for nconf conformations of the context's system (vlvr: nconf x
  * nvt): pred (nconf, may be NULL) in network units.  With targets: *loss =
- * mean squared error and grad (smmp_mlp_size(), may be NULL) its gradient;
+ * mean squared error and grad (smmp_mlp_size(ctx), may be NULL) its gradient;
  * pairs = 0: one target per conformation (its output), pairs = 1: the
  * conformations come in pairs (x, x') and target[c] is for the difference
  * of the outputs of conformations 2c+1 and 2c (nconf even). */

@@ -474,13 +474,13 @@ int engine_profile_phases(Engine*& e, double* out) {
   return SMMP_OK;
 }
 
-void engine_mlp_set(Engine*& e, const float* theta) {
+void engine_mlp_set(Engine*& e, const State& s, const float* theta) {
   Engine& E = engine_of(e);
-  mlp_set_theta(E.mlp, E.stream, theta);
+  mlp_set_theta(E.mlp, E.stream, theta, mlp_residue_count(s));
 }
-void engine_mlp_get(Engine*& e, float* theta) {
+void engine_mlp_get(Engine*& e, const State& s, float* theta) {
   Engine& E = engine_of(e);
-  mlp_get_theta(E.mlp, E.stream, theta);
+  mlp_get_theta(E.mlp, E.stream, theta, mlp_residue_count(s));
 }
 double engine_mlp_grad(Engine*& e, const State& s, const double* vlvr, int nconf, const float* target, float* pred,
                        float* grad, int pairs) {

@@ -31,48 +31,41 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // ------------------------------------------------------------ features
 
-// Angle columns and type ids of every residue window of nconf conformations
-// (vlvr: nconf x nvt FP64).  One thread per (row, column < kMlpIn);
-// embedding columns are filled by k_mlp_embed from the current parameters.
-__global__ void k_mlp_features(MlpResidues R, const double* __restrict__ vlvr, int nconf, bf16* __restrict__ x,
-                               int16_t* __restrict__ rtype) {
+// Angle columns of nconf conformations (vlvr: nconf x nvt FP64) into x
+// (nconf x D, BF16); one thread per (conformation, column).  Embedding
+// columns are filled by k_mlp_embed from the current parameters.
+__global__ void k_mlp_features(MlpResidues R, const double* __restrict__ vlvr, int nconf, bf16* __restrict__ x) {
+  const int D = R.nres * kMlpResCols;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t rows = (int64_t)nconf * R.nres;
-  if (t >= rows * kMlpIn) return;
-  const int64_t row = t / kMlpIn;
-  const int col = (int)(t - row * kMlpIn);
-  const int c = (int)(row / R.nres), r = (int)(row - (int64_t)c * R.nres);
-  float v = 0.0f;
-  if (col < kMlpAngleCols) {
-    int var;
-    if (window_angle(R, r, col, var)) {
-      const double a = vlvr[(size_t)c * R.nvt + var];
-      v = (float)((col & 1) ? cos(a) : sin(a));
-    }
-  }
-  if (col < kMlpEmbCol || col >= kMlpEmbCol + 3 * kMlpEmb) x[row * kMlpIn + col] = (bf16)v;
-  if (col < 3) rtype[row * 3 + col] = (int16_t)window_type(R, r, col);
+  if (t >= (int64_t)nconf * D) return;
+  const int64_t c = t / D;
+  const int col = (int)(t - c * D), r = col / kMlpResCols, k = col - r * kMlpResCols;
+  if (k >= kMlpEmbCol) return;
+  const int s = k >> 1;
+  bf16 v = (bf16)0.0f;
+  if (s < R.nvar[r]) v = mlp_angle_feature(vlvr[(size_t)c * R.nvt + R.var0[r] + s], k & 1);
+  x[t] = v;
 }
 
 // Embedding columns from the parameters (BF16 copies of theta's table).
-__global__ void k_mlp_embed(const float* __restrict__ theta, const int16_t* __restrict__ rtype, int64_t rows,
+__global__ void k_mlp_embed(const float* __restrict__ theta, MlpLayout Lo, const int16_t* __restrict__ type, int nconf,
                             bf16* __restrict__ x) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= rows * 3 * kMlpEmb) return;
-  const int64_t row = t / (3 * kMlpEmb);
-  const int k = (int)(t - row * 3 * kMlpEmb), d = k / kMlpEmb, e = k - d * kMlpEmb;
-  const int ty = rtype[row * 3 + d];
-  x[row * kMlpIn + kMlpEmbCol + k] = (bf16)theta[kThEmb + ty * kMlpEmb + e];
+  const int per = Lo.nres * kMlpEmb;
+  if (t >= (int64_t)nconf * per) return;
+  const int64_t c = t / per;
+  const int q = (int)(t - c * per), r = q / kMlpEmb, e = q - r * kMlpEmb;
+  x[c * Lo.D + r * kMlpResCols + kMlpEmbCol + e] = (bf16)theta[Lo.emb + type[r] * kMlpEmb + e];
 }
 
-// BF16 copies of W1, W2, W3 (input-major, as in theta).
-__global__ void k_mlp_pack(const float* __restrict__ theta, bf16* __restrict__ w1, bf16* __restrict__ w2,
-                           bf16* __restrict__ w3) {
+// BF16 copies of W1 [D][H], W2 and W3 [H][H] (input-major, as in theta).
+__global__ void k_mlp_pack(const float* __restrict__ theta, MlpLayout Lo, bf16* __restrict__ w1,
+                           bf16* __restrict__ w2, bf16* __restrict__ w3) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < kMlpIn * kMlpHid) w1[t] = (bf16)theta[kThW1 + t];
+  if (t < Lo.D * kMlpHid) w1[t] = (bf16)theta[Lo.w1 + t];
   if (t < kMlpHid * kMlpHid) {
-    w2[t] = (bf16)theta[kThW2 + t];
-    w3[t] = (bf16)theta[kThW3 + t];
+    w2[t] = (bf16)theta[Lo.w2 + t];
+    w3[t] = (bf16)theta[Lo.w3 + t];
   }
 }
 
@@ -149,65 +142,55 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(GemmArgs g) {
 // ------------------------------------------------------------ small kernels
 
 // y[m] = w4 . H3[m] + b4
-__global__ void k_mlp_out(const bf16* __restrict__ h3, const float* __restrict__ theta, int64_t rows,
+__global__ void k_mlp_out(const bf16* __restrict__ h3, const float* __restrict__ theta, MlpLayout Lo, int64_t rows,
                           float* __restrict__ y) {
   const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= rows) return;
-  float s = theta[kThB4];
-  for (int n = 0; n < kMlpHid; ++n) s += (float)h3[m * kMlpHid + n] * theta[kThW4 + n];
+  float s = theta[Lo.b4];
+  for (int n = 0; n < kMlpHid; ++n) s += (float)h3[m * kMlpHid + n] * theta[Lo.w4 + n];
   y[m] = s;
 }
 
-// Per conformation: pred = sum of its nres rows.  With targets, either
-// per conformation (e = pred - target, dy[row] = 2 e / nconf), or with
-// pairs per pair c of conformations (2c, 2c+1): e = pred[2c+1] - pred[2c] -
-// target[c], dy = +-2 e / npairs on the rows of 2c+1 and 2c; sq = e^2.
-__global__ void k_mlp_conf(const float* __restrict__ y, int nres, int nconf, const float* __restrict__ target,
+// Predictions and the gradient of the loss with respect to them: with
+// per-conformation targets e = y - target, dy = 2 e / nconf; with pairs
+// (conformations 2c, 2c+1 and target[c] for y[2c+1] - y[2c]) e = y[2c+1] -
+// y[2c] - target[c], dy = +-2 e / npairs; sq = e^2.
+__global__ void k_mlp_conf(const float* __restrict__ y, int nconf, const float* __restrict__ target,
                            float* __restrict__ pred, float* __restrict__ dy, float* __restrict__ sq, int pairs) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (!pairs) {
     if (c >= nconf) return;
-    float s = 0.0f;
-    for (int r = 0; r < nres; ++r) s += y[(size_t)c * nres + r];
-    if (pred) pred[c] = s;
+    if (pred) pred[c] = y[c];
     if (target) {
-      const float e = s - target[c];
+      const float e = y[c] - target[c];
       sq[c] = e * e;
-      const float g = 2.0f * e / (float)nconf;
-      for (int r = 0; r < nres; ++r) dy[(size_t)c * nres + r] = g;
+      dy[c] = 2.0f * e / (float)nconf;
     }
     return;
   }
   const int np = nconf / 2;
   if (c >= np) return;
-  float s0 = 0.0f, s1 = 0.0f;
-  for (int r = 0; r < nres; ++r) {
-    s0 += y[(size_t)(2 * c) * nres + r];
-    s1 += y[(size_t)(2 * c + 1) * nres + r];
-  }
   if (pred) {
-    pred[2 * c] = s0;
-    pred[2 * c + 1] = s1;
+    pred[2 * c] = y[2 * c];
+    pred[2 * c + 1] = y[2 * c + 1];
   }
   if (target) {
-    const float e = (s1 - s0) - target[c];
+    const float e = (y[2 * c + 1] - y[2 * c]) - target[c];
     sq[c] = e * e;
     const float g = 2.0f * e / (float)np;
-    for (int r = 0; r < nres; ++r) {
-      dy[(size_t)(2 * c) * nres + r] = -g;
-      dy[(size_t)(2 * c + 1) * nres + r] = g;
-    }
+    dy[2 * c] = -g;
+    dy[2 * c + 1] = g;
   }
 }
 
 // dZ3 = (dy w4^T) * silu'(Z3)
-__global__ void k_mlp_dout(const float* __restrict__ dy, const float* __restrict__ theta, const float* __restrict__ z3,
-                           int64_t rows, bf16* __restrict__ dz) {
+__global__ void k_mlp_dout(const float* __restrict__ dy, const float* __restrict__ theta, MlpLayout Lo,
+                           const float* __restrict__ z3, int64_t rows, bf16* __restrict__ dz) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= rows * kMlpHid) return;
   const int64_t m = t / kMlpHid;
   const int n = (int)(t - m * kMlpHid);
-  dz[t] = (bf16)(dy[m] * theta[kThW4 + n] * silu_grad(z3[t]));
+  dz[t] = (bf16)(dy[m] * theta[Lo.w4 + n] * silu_grad(z3[t]));
 }
 
 // Column sums over row blocks of 256: partial[b][n] = sum over rows m of
@@ -243,24 +226,28 @@ __global__ void k_sum_partials(const float* __restrict__ partial, int nblk, int 
   out[i] = s;
 }
 
-// Embedding gradients: block t sums dX's embedding columns over the window
-// positions whose type is t (fixed order).
-__global__ void k_mlp_emb_grad(const float* __restrict__ dx, const int16_t* __restrict__ rtype, int64_t rows,
-                               float* __restrict__ grad) {
+// Embedding gradients: block t sums dX's embedding columns over the
+// (conformation, residue) pairs whose residue type is t (fixed order).
+__global__ void k_mlp_emb_grad(const float* __restrict__ dx, const int16_t* __restrict__ type, MlpLayout Lo,
+                               int64_t rows, float* __restrict__ grad) {
   __shared__ float red[256 * kMlpEmb];
   const int ty = blockIdx.x;
   float acc[kMlpEmb];
   for (int e = 0; e < kMlpEmb; ++e) acc[e] = 0.0f;
-  for (int64_t m = threadIdx.x; m < rows; m += 256)
-    for (int d = 0; d < 3; ++d)
-      if (rtype[m * 3 + d] == ty)
-        for (int e = 0; e < kMlpEmb; ++e) acc[e] += dx[m * kMlpIn + kMlpEmbCol + d * kMlpEmb + e];
+  const int64_t items = rows * Lo.nres;
+  for (int64_t q = threadIdx.x; q < items; q += 256) {
+    const int64_t m = q / Lo.nres;
+    const int r = (int)(q - m * Lo.nres);
+    if (type[r] != ty) continue;
+    const float* d = dx + m * Lo.D + r * kMlpResCols + kMlpEmbCol;
+    for (int e = 0; e < kMlpEmb; ++e) acc[e] += d[e];
+  }
   for (int e = 0; e < kMlpEmb; ++e) red[threadIdx.x * kMlpEmb + e] = acc[e];
   __syncthreads();
   if (threadIdx.x < kMlpEmb) {
     float s = 0.0f;
     for (int k = 0; k < 256; ++k) s += red[k * kMlpEmb + threadIdx.x];
-    grad[kThEmb + ty * kMlpEmb + threadIdx.x] = s;
+    grad[Lo.emb + ty * kMlpEmb + threadIdx.x] = s;
   }
 }
 
@@ -338,20 +325,22 @@ __global__ void k_sum_sq(const float* __restrict__ sq, int n, double* __restrict
 }
 
 // Workspace of the training passes and the context's network (FP32
-// parameters and Adam moments), kept with the context.
+// parameters and Adam moments), kept with the context, plus the BF16 copies
+// of the weights the sampler's screen reads.
 struct MlpWork {
-  DevBuf res, x, rtype, z1, z2, z3, h1, h2, h3, dz_a, dz_b, dx, y, dy, sq, pred, part, wb;
+  DevBuf res, x, z1, z2, z3, h1, h2, h3, dz_a, dz_b, dx, y, dy, sq, pred, part, wb;
   DevBuf theta, grad, m1, m2;
   DevBuf data_v, data_t, perm, bv, bt, lossacc, tmp_v, tmp_t;
+  DevBuf sw1, sw2t, sw3t;  // screen weights: W1 [D][H], W2^T and W3^T [out][in], BF16
   std::vector<int16_t> host_tables;
   int step = 0;
-  bool have_theta = false;
+  int n = 0;               // parameters of the network held (0: none)
+  int nres = 0;            // residues it was made for
+  bool screen_fresh = false;
 };
 
 void mlp_work_destroy(MlpWork* w) { delete w; }
 
-// Residue windows of the system: first variable, variable count, type and
-// chain of each residue (all chains), residue of each variable.
 static int residue_type_id(const std::string& name) {
   static const char* kStd[20] = {"ala", "arg", "asn", "asp", "cys", "gln", "glu", "gly", "his", "ile",
                                  "leu", "lys", "met", "phe", "pro", "ser", "thr", "trp", "tyr", "val"};
@@ -362,6 +351,10 @@ static int residue_type_id(const std::string& name) {
   return 21;  // any other residue of the library
 }
 
+int mlp_residue_count(const State& s) { return s.ntlml >= 1 ? s.irsml2(s.ntlml) : 0; }
+
+// Residue tables of the system: first variable, variable count, type and
+// chain of each residue (all chains), residue of each variable.
 MlpResidues mlp_residues(MlpWork*& w, const State& s, hipStream_t st) {
   if (!w) w = new MlpWork();
   const int nml = s.ntlml;
@@ -404,49 +397,93 @@ static MlpWork& work(MlpWork*& w) {
   return *w;
 }
 
-void mlp_set_theta(MlpWork*& wp, hipStream_t st, const float* host) {
+void mlp_set_theta(MlpWork*& wp, hipStream_t st, const float* host, int nres) {
   MlpWork& w = work(wp);
-  float* th = w.theta.get<float>(kThN);
-  HIPCHK(hipMemcpyAsync(th, host, sizeof(float) * kThN, hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemsetAsync(w.m1.get<float>(kThN), 0, sizeof(float) * kThN, st));
-  HIPCHK(hipMemsetAsync(w.m2.get<float>(kThN), 0, sizeof(float) * kThN, st));
+  const int n = mlp_layout(nres).n;
+  float* th = w.theta.get<float>(n);
+  HIPCHK(hipMemcpyAsync(th, host, sizeof(float) * n, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemsetAsync(w.m1.get<float>(n), 0, sizeof(float) * n, st));
+  HIPCHK(hipMemsetAsync(w.m2.get<float>(n), 0, sizeof(float) * n, st));
   HIPCHK(hipStreamSynchronize(st));
   w.step = 0;
-  w.have_theta = true;
+  w.n = n;
+  w.nres = nres;
+  w.screen_fresh = false;
 }
 
-void mlp_get_theta(MlpWork*& wp, hipStream_t st, float* host) {
+static void need_theta(const MlpWork& w, int nres) {
+  if (w.n == 0) throw Error("surrogate> no network set (smmp_mlp_set)");
+  if (w.nres != nres)
+    throw Error("surrogate> the network was made for " + std::to_string(w.nres) + " residues, the system has " +
+                std::to_string(nres));
+}
+
+void mlp_get_theta(MlpWork*& wp, hipStream_t st, float* host, int nres) {
   MlpWork& w = work(wp);
-  if (!w.have_theta) throw Error("surrogate> no network set (smmp_mlp_set)");
-  HIPCHK(hipMemcpyAsync(host, w.theta.p, sizeof(float) * kThN, hipMemcpyDeviceToHost, st));
+  need_theta(w, nres);
+  HIPCHK(hipMemcpyAsync(host, w.theta.p, sizeof(float) * w.n, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
 }
 
-const float* mlp_theta_dev(MlpWork*& wp) {
-  MlpWork& w = work(wp);
-  if (!w.have_theta) throw Error("surrogate> no network set (smmp_mlp_set)");
-  return static_cast<const float*>(w.theta.p);
+// W2^T, W3^T [out][in]: 8 consecutive inputs of one output are contiguous,
+// the B operand of a 16x16x32 MFMA step in one 16-byte load.
+__global__ void k_mlp_pack_screen(const float* __restrict__ theta, MlpLayout Lo, bf16* __restrict__ w1,
+                                  bf16* __restrict__ w2t, bf16* __restrict__ w3t) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < Lo.D * kMlpHid) w1[t] = (bf16)theta[Lo.w1 + t];
+  if (t < kMlpHid * kMlpHid) {
+    const int k = t / kMlpHid, n = t - k * kMlpHid;  // theta: [in k][out n]
+    w2t[n * kMlpHid + k] = (bf16)theta[Lo.w2 + t];
+    w3t[n * kMlpHid + k] = (bf16)theta[Lo.w3 + t];
+  }
 }
 
-// Forward pass over the windows of nconf conformations (vlvr on the device,
-// FP64), and with targets (device) the squared errors (sq, device) and the
-// gradient of the mean squared error (grad, device).  Enqueued on st only.
+MlpScreen mlp_screen(MlpWork*& wp, const State& s, hipStream_t st) {
+  MlpWork& w = work(wp);
+  const MlpResidues R = mlp_residues(wp, s, st);
+  need_theta(w, R.nres);
+  const MlpLayout Lo = mlp_layout(R.nres);
+  bf16* w1 = w.sw1.get<bf16>((size_t)Lo.D * kMlpHid);
+  bf16* w2t = w.sw2t.get<bf16>(kMlpHid * kMlpHid);
+  bf16* w3t = w.sw3t.get<bf16>(kMlpHid * kMlpHid);
+  if (!w.screen_fresh) {
+    hipLaunchKernelGGL(k_mlp_pack_screen, dim3(nblocks(std::max(Lo.D, kMlpHid) * (int64_t)kMlpHid, 256)), dim3(256),
+                       0, st, static_cast<const float*>(w.theta.p), Lo, w1, w2t, w3t);
+    HIPCHK(hipGetLastError());
+    w.screen_fresh = true;
+  }
+  MlpScreen S;
+  S.res = R;
+  S.lo = Lo;
+  S.theta = static_cast<const float*>(w.theta.p);
+  S.w1 = w1;
+  S.w2t = w2t;
+  S.w3t = w3t;
+  return S;
+}
+
+// Forward pass over nconf conformations (vlvr on the device, FP64), and
+// with targets (device) the squared errors (sq, device) and the gradient of
+// the mean squared error (grad, device).  Enqueued on st only.
 static void mlp_pass(MlpWork& w, const MlpResidues& R, hipStream_t st, const float* theta, const double* vlvr,
                      int nconf, const float* target, float* pred, float* grad, float* sq, int pairs) {
-  const int64_t rows = (int64_t)nconf * R.nres;
-  bf16* x = w.x.get<bf16>(rows * kMlpIn);
-  int16_t* rt = w.rtype.get<int16_t>(rows * 3);
+  const MlpLayout Lo = mlp_layout(R.nres);
+  const int D = Lo.D;
+  const int64_t rows = nconf;
+  bf16* x = w.x.get<bf16>(rows * D);
   float *z1 = w.z1.get<float>(rows * kMlpHid), *z2 = w.z2.get<float>(rows * kMlpHid),
         *z3 = w.z3.get<float>(rows * kMlpHid);
   bf16 *h1 = w.h1.get<bf16>(rows * kMlpHid), *h2 = w.h2.get<bf16>(rows * kMlpHid),
        *h3 = w.h3.get<bf16>(rows * kMlpHid);
   float* y = w.y.get<float>(rows);
-  if (target) HIPCHK(hipMemsetAsync(grad, 0, sizeof(float) * kThN, st));
-  bf16* wb = w.wb.get<bf16>(kMlpIn * kMlpHid + 2 * kMlpHid * kMlpHid);
-  bf16 *w1 = wb, *w2 = wb + kMlpIn * kMlpHid, *w3 = w2 + kMlpHid * kMlpHid;
-  hipLaunchKernelGGL(k_mlp_features, dim3(nblocks(rows * kMlpIn, 256)), dim3(256), 0, st, R, vlvr, nconf, x, rt);
-  hipLaunchKernelGGL(k_mlp_embed, dim3(nblocks(rows * 3 * kMlpEmb, 256)), dim3(256), 0, st, theta, rt, rows, x);
-  hipLaunchKernelGGL(k_mlp_pack, dim3(nblocks(kMlpIn * kMlpHid, 256)), dim3(256), 0, st, theta, w1, w2, w3);
+  if (target) HIPCHK(hipMemsetAsync(grad, 0, sizeof(float) * Lo.n, st));
+  bf16* wb = w.wb.get<bf16>((size_t)D * kMlpHid + 2 * kMlpHid * kMlpHid);
+  bf16 *w1 = wb, *w2 = wb + (size_t)D * kMlpHid, *w3 = w2 + kMlpHid * kMlpHid;
+  hipLaunchKernelGGL(k_mlp_features, dim3(nblocks(rows * D, 256)), dim3(256), 0, st, R, vlvr, nconf, x);
+  hipLaunchKernelGGL(k_mlp_embed, dim3(nblocks(rows * R.nres * kMlpEmb, 256)), dim3(256), 0, st, theta, Lo, R.type,
+                     nconf, x);
+  hipLaunchKernelGGL(k_mlp_pack, dim3(nblocks(std::max(D, kMlpHid) * (int64_t)kMlpHid, 256)), dim3(256), 0, st,
+                     theta, Lo, w1, w2, w3);
   HIPCHK(hipGetLastError());
   const int M = (int)rows;
   auto hidden = [&](const bf16* in, int K, const bf16* W, const float* b, float* Z, bf16* H) {
@@ -455,21 +492,23 @@ static void mlp_pass(MlpWork& w, const MlpResidues& R, hipStream_t st, const flo
     g.epi = kEpiHidden; g.bias = b; g.Z = Z; g.H = H;
     gemm(st, g, 1);
   };
-  hidden(x, kMlpIn, w1, theta + kThB1, z1, h1);
-  hidden(h1, kMlpHid, w2, theta + kThB2, z2, h2);
-  hidden(h2, kMlpHid, w3, theta + kThB3, z3, h3);
-  hipLaunchKernelGGL(k_mlp_out, dim3(nblocks(rows, 256)), dim3(256), 0, st, h3, theta, rows, y);
+  hidden(x, D, w1, theta + Lo.b1, z1, h1);
+  hidden(h1, kMlpHid, w2, theta + Lo.b2, z2, h2);
+  hidden(h2, kMlpHid, w3, theta + Lo.b3, z3, h3);
+  hipLaunchKernelGGL(k_mlp_out, dim3(nblocks(rows, 256)), dim3(256), 0, st, h3, theta, Lo, rows, y);
   float* dy = target ? w.dy.get<float>(rows) : nullptr;
-  hipLaunchKernelGGL(k_mlp_conf, dim3(nblocks(nconf, 256)), dim3(256), 0, st, y, R.nres, nconf, target, pred, dy, sq,
-                     pairs);
+  hipLaunchKernelGGL(k_mlp_conf, dim3(nblocks(nconf, 256)), dim3(256), 0, st, y, nconf, target, pred, dy, sq, pairs);
   HIPCHK(hipGetLastError());
   if (!target) return;
   // ---- backward
   bf16 *dza = w.dz_a.get<bf16>(rows * kMlpHid), *dzb = w.dz_b.get<bf16>(rows * kMlpHid);
-  float* dx = w.dx.get<float>(rows * kMlpIn);
+  float* dx = w.dx.get<float>(rows * D);
   const int nb = (int)nblocks(rows, 256);
-  const int splits = std::max(1, std::min(64, M / 512));
-  float* part = w.part.get<float>(std::max<size_t>((size_t)nb * (kMlpHid + 1), (size_t)splits * kMlpIn * kMlpHid));
+  const int splits = std::max(1, std::min(64, M / 256));
+  // partials of the weight gradients: splits x K x H for K = D (layer 1) and
+  // K = H (layers 2, 3), whichever is larger; column sums nb x (H + 1)
+  const size_t kmax = (size_t)std::max(D, kMlpHid);
+  float* part = w.part.get<float>(std::max<size_t>((size_t)nb * (kMlpHid + 1), (size_t)splits * kmax * kMlpHid));
   auto colsum = [&](const bf16* ab, const float* af, const float* wt, int N, float* out) {
     hipLaunchKernelGGL(k_colsum, dim3(nb), dim3(256), 0, st, ab, af, wt, rows, N, part);
     hipLaunchKernelGGL(k_sum_partials, dim3(nblocks(N, 64)), dim3(64), 0, st, part, nb, N, out);
@@ -480,7 +519,8 @@ static void mlp_pass(MlpWork& w, const MlpResidues& R, hipStream_t st, const flo
     g.epi = kEpiPartial; g.C = part;
     gemm(st, g, splits);
     const int chunk = ((M + splits - 1) / splits + 31) / 32 * 32, nz = (M + chunk - 1) / chunk;
-    hipLaunchKernelGGL(k_sum_partials, dim3(nblocks(K * kMlpHid, 256)), dim3(256), 0, st, part, nz, K * kMlpHid, out);
+    hipLaunchKernelGGL(k_sum_partials, dim3(nblocks((int64_t)K * kMlpHid, 256)), dim3(256), 0, st, part, nz,
+                       K * kMlpHid, out);
   };
   auto back = [&](const bf16* dz, const bf16* W, int K, const float* Zprev, bf16* dzprev) {  // (dz W^T) * silu'
     GemmArgs g{};
@@ -488,24 +528,24 @@ static void mlp_pass(MlpWork& w, const MlpResidues& R, hipStream_t st, const flo
     g.epi = kEpiBackHidden; g.Z = const_cast<float*>(Zprev); g.H = dzprev;
     gemm(st, g, 1);
   };
-  colsum(h3, nullptr, dy, kMlpHid, grad + kThW4);  // output layer
-  colsum(nullptr, dy, nullptr, 1, grad + kThB4);
-  hipLaunchKernelGGL(k_mlp_dout, dim3(nblocks(rows * kMlpHid, 256)), dim3(256), 0, st, dy, theta, z3, rows, dza);
-  wgrad(h2, kMlpHid, dza, grad + kThW3);  // layer 3
-  colsum(dza, nullptr, nullptr, kMlpHid, grad + kThB3);
+  colsum(h3, nullptr, dy, kMlpHid, grad + Lo.w4);  // output layer
+  colsum(nullptr, dy, nullptr, 1, grad + Lo.b4);
+  hipLaunchKernelGGL(k_mlp_dout, dim3(nblocks(rows * kMlpHid, 256)), dim3(256), 0, st, dy, theta, Lo, z3, rows, dza);
+  wgrad(h2, kMlpHid, dza, grad + Lo.w3);  // layer 3
+  colsum(dza, nullptr, nullptr, kMlpHid, grad + Lo.b3);
   back(dza, w3, kMlpHid, z2, dzb);
-  wgrad(h1, kMlpHid, dzb, grad + kThW2);  // layer 2
-  colsum(dzb, nullptr, nullptr, kMlpHid, grad + kThB2);
+  wgrad(h1, kMlpHid, dzb, grad + Lo.w2);  // layer 2
+  colsum(dzb, nullptr, nullptr, kMlpHid, grad + Lo.b2);
   back(dzb, w2, kMlpHid, z1, dza);
-  wgrad(x, kMlpIn, dza, grad + kThW1);  // layer 1, and the embeddings through the input
-  colsum(dza, nullptr, nullptr, kMlpHid, grad + kThB1);
+  wgrad(x, D, dza, grad + Lo.w1);  // layer 1, and the embeddings through the input
+  colsum(dza, nullptr, nullptr, kMlpHid, grad + Lo.b1);
   {
     GemmArgs g{};
-    g.A = dza; g.lda = kMlpHid; g.B = w1; g.ldb = kMlpHid; g.transB = 1; g.M = M; g.N = kMlpIn; g.K = kMlpHid;
+    g.A = dza; g.lda = kMlpHid; g.B = w1; g.ldb = kMlpHid; g.transB = 1; g.M = M; g.N = D; g.K = kMlpHid;
     g.epi = kEpiStore; g.C = dx;
     gemm(st, g, 1);
   }
-  hipLaunchKernelGGL(k_mlp_emb_grad, dim3(kMlpTypes), dim3(256), 0, st, dx, rt, rows, grad);
+  hipLaunchKernelGGL(k_mlp_emb_grad, dim3(kMlpTypes), dim3(256), 0, st, dx, R.type, Lo, rows, grad);
   HIPCHK(hipGetLastError());
 }
 
@@ -513,10 +553,11 @@ static void adam_step(MlpWork& w, hipStream_t st, float lr) {
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
   ++w.step;
   const float c1 = 1.0f - std::pow(b1, (float)w.step), c2 = 1.0f - std::pow(b2, (float)w.step);
-  hipLaunchKernelGGL(k_mlp_adam, dim3(nblocks(kThN, 256)), dim3(256), 0, st, static_cast<float*>(w.theta.p),
+  hipLaunchKernelGGL(k_mlp_adam, dim3(nblocks(w.n, 256)), dim3(256), 0, st, static_cast<float*>(w.theta.p),
                      static_cast<const float*>(w.grad.p), static_cast<float*>(w.m1.p), static_cast<float*>(w.m2.p),
-                     kThN, lr, b1, b2, eps, c1, c2);
+                     w.n, lr, b1, b2, eps, c1, c2);
   HIPCHK(hipGetLastError());
+  w.screen_fresh = false;
 }
 
 // Predictions (and with targets the loss and gradient) of the context's
@@ -525,7 +566,7 @@ double mlp_grad(MlpWork*& wp, const State& s, hipStream_t st, const double* vlvr
                 float* pred, float* grad, int pairs) {
   const MlpResidues R = mlp_residues(wp, s, st);
   MlpWork& w = *wp;
-  if (!w.have_theta) throw Error("surrogate> no network set (smmp_mlp_set)");
+  need_theta(w, R.nres);
   double* dv = w.tmp_v.get<double>((size_t)nconf * R.nvt);
   HIPCHK(hipMemcpyAsync(dv, vlvr, sizeof(double) * nconf * R.nvt, hipMemcpyHostToDevice, st));
   if (pairs && nconf % 2) throw Error("surrogate> pairs need an even number of conformations");
@@ -537,12 +578,12 @@ double mlp_grad(MlpWork*& wp, const State& s, hipStream_t st, const double* vlvr
   }
   float* dp = w.pred.get<float>(nconf);
   float* sq = w.sq.get<float>(nconf);
-  float* g = w.grad.get<float>(kThN);
+  float* g = w.grad.get<float>(w.n);
   mlp_pass(w, R, st, static_cast<const float*>(w.theta.p), dv, nconf, dt, dp, g, sq, pairs);
   if (pred) HIPCHK(hipMemcpyAsync(pred, dp, sizeof(float) * nconf, hipMemcpyDeviceToHost, st));
   std::vector<float> hsq(target ? nt : 0);
   if (target) {
-    if (grad) HIPCHK(hipMemcpyAsync(grad, g, sizeof(float) * kThN, hipMemcpyDeviceToHost, st));
+    if (grad) HIPCHK(hipMemcpyAsync(grad, g, sizeof(float) * w.n, hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(hsq.data(), sq, sizeof(float) * nt, hipMemcpyDeviceToHost, st));
   }
   HIPCHK(hipStreamSynchronize(st));
@@ -551,14 +592,14 @@ double mlp_grad(MlpWork*& wp, const State& s, hipStream_t st, const double* vlvr
   return target ? loss / nt : 0.0;
 }
 
-// Minibatch Adam on the context's network: n conformations (host) with
-// targets, 'epochs' passes in random order (std::mt19937 with seed), batches
-// of 'batch' (the last partial batch of an epoch is left out).
+// Minibatch Adam on the context's network: n samples (host) with targets,
+// 'epochs' passes in random order (std::mt19937 with seed), batches of
+// 'batch' (the last partial batch of an epoch is left out).
 void mlp_fit(MlpWork*& wp, const State& s, hipStream_t st, const double* vlvr, const float* target, int n, int epochs,
              int batch, float lr, uint32_t seed, double* epoch_loss, int pairs) {
   const MlpResidues R = mlp_residues(wp, s, st);
   MlpWork& w = *wp;
-  if (!w.have_theta) throw Error("surrogate> no network set (smmp_mlp_set)");
+  need_theta(w, R.nres);
   if (batch < 1 || n < batch) throw Error("surrogate> need at least one batch of training data");
   const int per = pairs ? 2 : 1;  // conformations per sample
   double* dv = w.data_v.get<double>((size_t)n * per * R.nvt);
@@ -570,7 +611,7 @@ void mlp_fit(MlpWork*& wp, const State& s, hipStream_t st, const double* vlvr, c
   float* bt = w.bt.get<float>(batch);
   float* sq = w.sq.get<float>(batch * per);
   float* pred = w.pred.get<float>(batch * per);
-  float* g = w.grad.get<float>(kThN);
+  float* g = w.grad.get<float>(w.n);
   double* acc = w.lossacc.get<double>(1);
   std::vector<int32_t> perm(n);
   std::mt19937 rng(seed);

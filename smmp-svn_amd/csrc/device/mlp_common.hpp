@@ -1,49 +1,72 @@
-// The delayed-acceptance surrogate: a small MLP over residue windows,
-// shared by the training kernels (mlp.hip) and the sampler's screening
-// stage (surrogate.hpp).
+// The delayed-acceptance surrogate: an MLP over the whole chain, shared by
+// the training kernels (mlp.hip) and the sampler's screening stage
+// (surrogate.hpp).
 //
-// Energy model: E~(x) = scale * sum_r y(f_r(x)), one row per residue r.
-// f_r has kIn = 128 columns: sin and cos of the (up to kSlots) variables of
-// residues r-1, r, r+1 of the same chain (60 columns, zero where a residue
-// or slot is absent), the kEmb-dimensional embeddings of those three
-// residues' types (48 columns), then zeros.  y = w4 . silu(W3 silu(W2
-// silu(W1^T f + b1) + b2) + b3) + b4 with kHid hidden units per layer.
-// Matrix products take BF16 operands and accumulate in FP32; biases, the
-// output layer and the sums over residues are FP32.
+// Energy model: E~(x) = scale * y(f(x)).  f(x) has D = nres * kMlpResCols
+// columns, kMlpResCols = 36 per residue r (all chains in order): sin and cos
+// of the residue's (up to kMlpSlots = 10) variables, slot s at columns
+// 36 r + 2 s and 36 r + 2 s + 1 (zero where the residue has no slot s), then
+// the kMlpEmb = 16 dimensional embedding of the residue's type.  y = w4 .
+// silu(W3 silu(W2 silu(W1^T f + b1) + b2) + b3) + b4 with kMlpHid = 256
+// units per hidden layer.  Matrix products take BF16 operands (f rounded as
+// bf16((float)sin a)) and accumulate in FP32; biases and the output layer
+// are FP32.  The parameter count depends on the system (W1 has D rows): for
+// 36 residues D = 1296 and the network has 464,400 parameters.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace smmp {
 
-constexpr int kMlpIn = 128;
-constexpr int kMlpHid = 64;
+constexpr int kMlpHid = 256;
 constexpr int kMlpSlots = 10;   // variables per residue (at most)
 constexpr int kMlpEmb = 16;
-constexpr int kMlpTypes = 32;   // embedding rows; row 0 = no residue
-constexpr int kMlpAngleCols = 3 * 2 * kMlpSlots;  // 60
-constexpr int kMlpEmbCol = kMlpAngleCols;          // embeddings start here
+constexpr int kMlpTypes = 32;   // embedding rows (1..20 the standard residues, 21 any other)
+constexpr int kMlpResCols = 2 * kMlpSlots + kMlpEmb;  // 36
+constexpr int kMlpEmbCol = 2 * kMlpSlots;             // embedding columns of a residue start here
 
-// Offsets of the parameters in one FP32 vector (theta): W1 is [kIn][kHid]
-// (input-major), W2 and W3 [kHid][kHid], embeddings [kTypes][kEmb].
-constexpr int kThW1 = 0;
-constexpr int kThB1 = kThW1 + kMlpIn * kMlpHid;
-constexpr int kThW2 = kThB1 + kMlpHid;
-constexpr int kThB2 = kThW2 + kMlpHid * kMlpHid;
-constexpr int kThW3 = kThB2 + kMlpHid;
-constexpr int kThB3 = kThW3 + kMlpHid * kMlpHid;
-constexpr int kThW4 = kThB3 + kMlpHid;
-constexpr int kThB4 = kThW4 + kMlpHid;
-constexpr int kThEmb = kThB4 + 16;
-constexpr int kThN = kThEmb + kMlpTypes * kMlpEmb;
+// Offsets of the parameters in one FP32 vector (theta) for nres residues:
+// W1 [D][H] (input-major), b1, W2 [H][H], b2, W3 [H][H], b3, w4, b4 (padded
+// to 16), embeddings [kMlpTypes][kMlpEmb].
+struct MlpLayout {
+  int nres, D;
+  int w1, b1, w2, b2, w3, b3, w4, b4, emb, n;
+};
+__host__ __device__ inline MlpLayout mlp_layout(int nres) {
+  MlpLayout L;
+  L.nres = nres;
+  L.D = nres * kMlpResCols;
+  L.w1 = 0;
+  L.b1 = L.w1 + L.D * kMlpHid;
+  L.w2 = L.b1 + kMlpHid;
+  L.b2 = L.w2 + kMlpHid * kMlpHid;
+  L.w3 = L.b2 + kMlpHid;
+  L.b3 = L.w3 + kMlpHid * kMlpHid;
+  L.w4 = L.b3 + kMlpHid;
+  L.b4 = L.w4 + kMlpHid;
+  L.emb = L.b4 + 16;
+  L.n = L.emb + kMlpTypes * kMlpEmb;
+  return L;
+}
 
-// Residue windows of one system (device arrays, nres entries each, 0-based):
-// first variable, number of variables, embedding row, chain; and per
-// variable its residue.
+// Residues of one system (device arrays, 0-based): first variable, number of
+// variables, embedding row, chain of each residue; residue of each variable.
 struct MlpResidues {
   int32_t nres, nvt;
   const int16_t *var0, *nvar, *type, *chain;
   const int16_t* res_of_var;
+};
+
+// What the sampler's screen reads: the residue tables, the layout, the FP32
+// parameters (biases, w4, b4, embeddings) and BF16 weights, W1 [D][H]
+// (input-major) and W2^T, W3^T [out][in].
+struct MlpScreen {
+  MlpResidues res;
+  MlpLayout lo;
+  const float* theta;
+  const __bf16* w1;
+  const __bf16* w2t;
+  const __bf16* w3t;
 };
 
 __device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
@@ -52,20 +75,10 @@ __device__ __forceinline__ float silu_grad(float z) {
   return s * (1.0f + z * (1.0f - s));
 }
 
-// Angle column c (< kMlpAngleCols) of the window of residue r: the value
-// of variable vl[...] behind it, or 'none' (returns false).
-__device__ __forceinline__ bool window_angle(const MlpResidues& R, int r, int c, int& var) {
-  const int d = c / (2 * kMlpSlots) - 1, s = (c % (2 * kMlpSlots)) / 2;
-  const int q = r + d;
-  if (q < 0 || q >= R.nres || R.chain[q] != R.chain[r] || s >= R.nvar[q]) return false;
-  var = R.var0[q] + s;
-  return true;
-}
-// Embedding row of window position d (0..2) of residue r (0 = no residue).
-__device__ __forceinline__ int window_type(const MlpResidues& R, int r, int d) {
-  const int q = r + d - 1;
-  if (q < 0 || q >= R.nres || R.chain[q] != R.chain[r]) return 0;
-  return R.type[q];
+// The angle feature of variable value a (sin for odd = 0, cos for odd = 1),
+// as the training kernels round it: bf16 of the FP32 value.
+__device__ __forceinline__ __bf16 mlp_angle_feature(double a, int odd) {
+  return (__bf16)(float)(odd ? cos(a) : sin(a));
 }
 
 }  // namespace smmp

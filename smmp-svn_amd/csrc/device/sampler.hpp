@@ -123,12 +123,11 @@ struct SamplerParams {
   uint32_t* mt_x;        // the exchange stream (625 words)
   uint32_t* gsync;       // grid-barrier arrival counter, zero at launch
   double* gbpr;          // per replica: 6 global parameters per chain (kMaxMol chains)
-  // delayed acceptance (null: plain Metropolis): the surrogate's
-  // parameters, kcal/mol per network unit, residue windows, per-replica
-  // counts (screened, rejected by the network, judged exactly, accepted)
-  const float* da_theta;
+  // delayed acceptance (da.theta null: plain Metropolis): the surrogate
+  // network, kcal/mol per network unit, per-replica counts (screened,
+  // rejected by the network, judged exactly, accepted)
+  MlpScreen da;
   double da_scale;
-  MlpResidues da_res;
   double* da_stats;
   size_t da_lds_off;     // byte offset of the surrogate's LDS region
   // snapshots for training: every snap_every sweeps, each replica's
@@ -550,7 +549,7 @@ __device__ void grid_sync(const SamplerParams& P, uint32_t& gepoch) {
 }
 
 // P.nsweep Metropolis sweeps of one replica per group of G blocks; DA:
-// delayed acceptance with the surrogate network (P.da_theta).
+// delayed acceptance with the surrogate network (P.da).
 template <bool MIXED, bool DA>
 __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
   extern __shared__ double smem[];
@@ -592,12 +591,9 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
     __syncthreads();
   }
   DaLds D{};
-  const int nres = P.da_res.nres;
-  if (DA) {  // the surrogate's weights, tables and outputs at the start state
-    D = carve_da_lds(reinterpret_cast<char*>(smem) + P.da_lds_off, nres, nvt);
-    stage_surrogate(D, P.da_theta, P.da_res);
-    __syncthreads();
-    surrogate_fill_cache(D, P.da_lds_off, nres, nvt, Q.vl);
+  if (DA) {  // the surrogate's tables, layer-1 sums and output at the start state
+    D = carve_da_lds(reinterpret_cast<char*>(smem) + P.da_lds_off, P.da.res.nres, nvt);
+    da_setup(P.da_lds_off, nvt, P.da, Q.vl);
   }
   // sc: [0] eol, [1] acz, [2] trial value, [3] dphi, [4] accept, [5] full-trial flag,
   //     [6] jv, [7] drift flag, [8..10] axis point, [11..13] axis unit, [15] old value,
@@ -743,13 +739,20 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
       const bool full = Q.sc[5] != 0.0;
       if (DA && !rigid) {
         // ---- delayed acceptance, first stage: the network's energy change
-        if (tid == 0) surrogate_window_rows(D, nres, jv);
-        __syncthreads();
-        surrogate_rows(P.da_lds_off, nres, nvt, Q.vl, jv, Q.sc[2]);
+        da_trial(P.da_lds_off, nvt, P.da, jv, Q.sc[15], Q.sc[2]);
         if (tid == 0) {
-          double ds = 0.0;
-          for (int k = 0; k < D.rows[kDaRows]; ++k) ds += (double)D.ynew[k] - (double)D.ycache[D.rows[k]];
-          ds *= P.da_scale;
+          const double ds = ((double)D.y[1] - (double)D.y[0]) * P.da_scale;
+          if (P.trace && g == 0 && r == P.trace_rep && *P.trace_n < P.trace_cap) {
+            // delayed acceptance traces its screens: variable, new value,
+            // network change, 0 (rejected by the network; the exact stage
+            // sets 1 or 2)
+            double* t = P.trace + 4 * (size_t)(*P.trace_n);
+            t[0] = jv;
+            t[1] = Q.sc[2];
+            t[2] = ds;
+            t[3] = 0.0;
+            *P.trace_n += 1;
+          }
           const double d1 = Q.sc[14] * ds;
           bool pass = true;
           if (d1 > 0.0) pass = exp(-fmin(d1, 100.0)) >= mt_grnd(Q.mt, &Q.mt[624]);
@@ -1026,20 +1029,21 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
             Q.sc[kScUpd + 9] += 1.0;
           }
         }
-        if (P.trace && g == 0 && r == P.trace_rep && *P.trace_n < P.trace_cap) {
-          double* t = P.trace + 4 * (size_t)(*P.trace_n);
-          t[0] = jv;
-          t[1] = enew;
-          t[2] = eol;
-          t[3] = acc ? 1.0 : 0.0;
-          *P.trace_n += 1;
+        if (P.trace && g == 0 && r == P.trace_rep) {
+          if (da) {  // the screen's record of this trial: 1 rejected exactly, 2 accepted
+            if (*P.trace_n > 0) P.trace[4 * (size_t)(*P.trace_n - 1) + 3] = acc ? 2.0 : 1.0;
+          } else if (*P.trace_n < P.trace_cap) {
+            double* t = P.trace + 4 * (size_t)(*P.trace_n);
+            t[0] = jv;
+            t[1] = enew;
+            t[2] = eol;
+            t[3] = acc ? 1.0 : 0.0;
+            *P.trace_n += 1;
+          }
         }
         if (da) {
           D.stats[2] += 1.0;
-          if (acc) {
-            D.stats[3] += 1.0;
-            for (int k = 0; k < D.rows[kDaRows]; ++k) D.ycache[D.rows[k]] = D.ynew[k];
-          }
+          if (acc) D.stats[3] += 1.0;
         }
         if (acc) {
           Q.sc[0] = enew;
@@ -1056,6 +1060,7 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
           for (int h = tid; h < nh; h += blockDim.x)
             if (full || Q.aff[h]) Q.nf[h] = Q.nf2[h];
         }
+        if (DA && !rigid) da_commit(D);  // the trial's layer-1 sums and output become the state's
       } else {
         for (int a = m1 + tid; a <= m2; a += blockDim.x) {
           const double xo = Q.bx[a - m1], yo = Q.by[a - m1], zo = Q.bz[a - m1];

@@ -424,12 +424,27 @@ int smmp_profile_read(smmp_ctx* c, int kernel, double* total_ms, int64_t* launch
   return guarded(c, SMMP_EDEVICE, [&] { return engine_profile_read(c->eng, kernel, total_ms, launches, work); });
 }
 
-int smmp_mlp_size(void) { return smmp::kThN; }
+int smmp_mlp_layout(const smmp_ctx* c, int32_t* out) {
+  if (!c || !out) return SMMP_EINVAL;
+  const State& s = c->s;
+  if (s.ntlml < 1) return SMMP_ESTATE;
+  const MlpLayout L = mlp_layout(s.irsml2(s.ntlml));
+  const int32_t v[16] = {L.n, L.nres, L.D, smmp::kMlpHid, L.w1, L.b1, L.w2, L.b2,
+                         L.w3, L.b3, L.w4, L.b4, L.emb, smmp::kMlpEmb, smmp::kMlpTypes, smmp::kMlpSlots};
+  for (int k = 0; k < 16; ++k) out[k] = v[k];
+  return SMMP_OK;
+}
+
+int smmp_mlp_size(const smmp_ctx* c) {
+  int32_t out[16];
+  return smmp_mlp_layout(c, out) == SMMP_OK ? out[0] : 0;
+}
 
 int smmp_mlp_set(smmp_ctx* c, const float* theta) {
   if (!theta) return SMMP_EINVAL;
   return guarded(c, SMMP_EDEVICE, [&] {
-    engine_mlp_set(c->eng, theta);
+    if (c->s.ntlml < 1) throw Error("surrogate> no molecule initialised");
+    engine_mlp_set(c->eng, c->s, theta);
     return SMMP_OK;
   });
 }
@@ -437,7 +452,8 @@ int smmp_mlp_set(smmp_ctx* c, const float* theta) {
 int smmp_mlp_get(smmp_ctx* c, float* theta) {
   if (!theta) return SMMP_EINVAL;
   return guarded(c, SMMP_ESTATE, [&] {
-    engine_mlp_get(c->eng, theta);
+    if (c->s.ntlml < 1) throw Error("surrogate> no molecule initialised");
+    engine_mlp_get(c->eng, c->s, theta);
     return SMMP_OK;
   });
 }

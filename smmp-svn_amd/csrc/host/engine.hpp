@@ -52,8 +52,8 @@ void engine_profile(Engine*& e, int enable);
 int engine_profile_read(Engine*& e, int kernel, double* ms, int64_t* launches, double* work);
 int engine_profile_phases(Engine*& e, double* out);
 int engine_sampler_info(Engine*& e, int32_t* out);
-void engine_mlp_set(Engine*& e, const float* theta);
-void engine_mlp_get(Engine*& e, float* theta);
+void engine_mlp_set(Engine*& e, const State& s, const float* theta);
+void engine_mlp_get(Engine*& e, const State& s, float* theta);
 double engine_mlp_grad(Engine*& e, const State& s, const double* vlvr, int nconf, const float* target, float* pred,
                        float* grad, int pairs);
 void engine_mlp_fit(Engine*& e, const State& s, const double* vlvr, const float* target, int n, int epochs, int batch,

@@ -89,7 +89,8 @@ def _load():
         "smmp_profile_read": (C.c_int, [P, C.c_int, dp, C.POINTER(C.c_int64), dp]),
         "smmp_profile_phases": (C.c_int, [P, dp]),
         "smmp_sampler_info": (C.c_int, [P, ip]),
-        "smmp_mlp_size": (C.c_int, []),
+        "smmp_mlp_size": (C.c_int, [P]),
+        "smmp_mlp_layout": (C.c_int, [P, ip]),
         "smmp_mlp_set": (C.c_int, [P, fp]),
         "smmp_mlp_get": (C.c_int, [P, fp]),
         "smmp_mlp_grad": (C.c_int, [P, dp, C.c_int, fp, C.c_int, fp, fp, dp]),
@@ -114,7 +115,7 @@ EXPORTS = ("smmp_ctx_create", "smmp_ctx_destroy", "smmp_last_error", "smmp_state
            "smmp_energy_batch", "smmp_energy_batch_noalloc", "smmp_batch_create", "smmp_batch_load", "smmp_batch_run", "smmp_batch_read",
            "smmp_batch_destroy", "smmp_sample", "smmp_build_info", "smmp_profile", "smmp_profile_read",
            "smmp_profile_phases", "smmp_muca_update", "smmp_muca_final", "smmp_sampler_info",
-           "smmp_mlp_size", "smmp_mlp_set", "smmp_mlp_get", "smmp_mlp_grad", "smmp_mlp_fit")
+           "smmp_mlp_size", "smmp_mlp_layout", "smmp_mlp_set", "smmp_mlp_get", "smmp_mlp_grad", "smmp_mlp_fit")
 
 KERNEL_SWEEP, KERNEL_EXCHANGE, KERNEL_ENERGY, KERNEL_ESOLAN = 0, 1, 2, 3
 

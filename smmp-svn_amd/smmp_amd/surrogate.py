@@ -4,12 +4,13 @@ library's BF16 MFMA kernels); this class keeps the scale and offset that
 map network units to kcal/mol and drives training and delayed-acceptance
 sampling.
 
-The network predicts E~(x) = scale * sum_r y(f_r(x)) + offset over residue
-windows (see include/smmp_mi355x.h).  It is fitted to exact energies the
-sampler produced, and in the sampler it only screens trials: the exact
-energy judges every trial that passes, so the sampled distribution is the
-exact one whatever the network's quality (delayed acceptance, Christen and
-Fox, 2005)."""
+The network predicts E~(x) = scale * y(f(x)) + offset from the whole
+chain (include/smmp_mi355x.h: per residue sin/cos of its variables and a
+16-d embedding of its type, three hidden layers of 256).  It is fitted to
+exact energies the sampler produced, and in the sampler it only screens
+trials: the exact energy judges every trial that passes, so the sampled
+distribution is the exact one whatever the network's quality (delayed
+acceptance, Christen and Fox, 2005)."""
 from __future__ import annotations
 
 import ctypes as C
@@ -18,24 +19,27 @@ import numpy as np
 
 from . import _ffi
 
-IN, HID, TYPES, EMB = 128, 64, 32, 16
+
+def layout(ctx) -> dict:
+    """The network's parameter layout for the context's system (smmp_mlp_layout)."""
+    out = np.zeros(16, dtype=np.int32)
+    ctx.check(_ffi.lib.smmp_mlp_layout(ctx.h, _ffi.iptr(out)))
+    keys = ("n", "nres", "D", "hid", "w1", "b1", "w2", "b2", "w3", "b3", "w4", "b4", "emb", "emb_dim", "types",
+            "slots")
+    return {k: int(v) for k, v in zip(keys, out)}
 
 
-def init_theta(seed: int = 0, init_scale: float = 1.0) -> np.ndarray:
+def init_theta(ctx, seed: int = 0, init_scale: float = 1.0) -> np.ndarray:
     """Fan-in scaled normal weights, zero biases, embeddings of spread 0.5."""
-    n = _ffi.lib.smmp_mlp_size()
+    L = layout(ctx)
     rng = np.random.default_rng(seed)
-    th = np.zeros(n, dtype=np.float32)
-    o = 0
-    th[o:o + IN * HID] = rng.normal(0, init_scale / np.sqrt(IN), IN * HID)
-    o += IN * HID + HID
-    th[o:o + HID * HID] = rng.normal(0, init_scale / np.sqrt(HID), HID * HID)
-    o += HID * HID + HID
-    th[o:o + HID * HID] = rng.normal(0, init_scale / np.sqrt(HID), HID * HID)
-    o += HID * HID + HID
-    th[o:o + HID] = rng.normal(0, init_scale / np.sqrt(HID), HID)
-    o += HID + 16
-    th[o:o + TYPES * EMB] = rng.normal(0, 0.5, TYPES * EMB)
+    th = np.zeros(L["n"], dtype=np.float32)
+    H, D = L["hid"], L["D"]
+    th[L["w1"]:L["b1"]] = rng.normal(0, init_scale / np.sqrt(D), D * H)
+    th[L["w2"]:L["b2"]] = rng.normal(0, init_scale / np.sqrt(H), H * H)
+    th[L["w3"]:L["b3"]] = rng.normal(0, init_scale / np.sqrt(H), H * H)
+    th[L["w4"]:L["b4"]] = rng.normal(0, init_scale / np.sqrt(H), H)
+    th[L["emb"]:] = rng.normal(0, 0.5, L["types"] * L["emb_dim"])
     return th
 
 
@@ -44,16 +48,16 @@ class Surrogate:
 
     def __init__(self, ctx, theta=None, seed: int = 0, init_scale: float = 1.0):
         self.ctx = ctx
-        self.n = _ffi.lib.smmp_mlp_size()
+        self.n = _ffi.lib.smmp_mlp_size(ctx.h)
         self.scale = 1.0   # kcal/mol per network unit
         self.offset = 0.0  # kcal/mol
-        self.set_theta(init_theta(seed, init_scale) if theta is None else theta)
+        self.set_theta(init_theta(ctx, seed, init_scale) if theta is None else theta)
 
     @classmethod
     def zero(cls, ctx) -> "Surrogate":
         """A network whose output is zero everywhere (delayed acceptance with
         it is plain Metropolis, draw for draw)."""
-        return cls(ctx, theta=np.zeros(_ffi.lib.smmp_mlp_size(), dtype=np.float32))
+        return cls(ctx, theta=np.zeros(_ffi.lib.smmp_mlp_size(ctx.h), dtype=np.float32))
 
     def set_theta(self, th) -> None:
         a = np.ascontiguousarray(th, dtype=np.float32)

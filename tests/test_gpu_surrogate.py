@@ -1,8 +1,10 @@
 """The delayed-acceptance surrogate on the GPU: the BF16 MFMA forward and
 backward kernels against the CPU oracle (which rounds to BF16 where the
-kernels do), Adam, delayed acceptance with a zero network against the plain
-sampler (identical draws and decisions), the sampled distribution with an
-untrained network (KS), and what a trained network gains over none."""
+kernels do), Adam, the sampler's in-kernel screen against the oracle's
+restatement of it, delayed acceptance with a zero network against the plain
+sampler (identical draws and decisions, also at the full C5 size), the
+sampled distribution with an untrained network (KS), and what a trained
+network gains over none."""
 import os
 import sys
 
@@ -29,10 +31,20 @@ def _confs(c, n, seed):
     return v
 
 
-def _blocks():
-    return {"W1": (so.W1, so.B1), "b1": (so.B1, so.W2), "W2": (so.W2, so.B2), "b2": (so.B2, so.W3),
-            "W3": (so.W3, so.B3), "b3": (so.B3, so.W4), "w4": (so.W4, so.B4), "b4": (so.B4, so.B4 + 1),
-            "emb": (so.EMBO, so.NPARAM)}
+def _blocks(L):
+    return {"W1": (L.W1, L.B1), "b1": (L.B1, L.W2), "W2": (L.W2, L.B2), "b2": (L.B2, L.W3),
+            "W3": (L.W3, L.B3), "b3": (L.B3, L.W4), "w4": (L.W4, L.B4), "b4": (L.B4, L.B4 + 1),
+            "emb": (L.EMBO, L.NPARAM)}
+
+
+def test_layout_matches_oracle():
+    from smmp_amd.surrogate import layout
+    c = make_ctx(tag="c5_36mer", itysol=1)
+    L, R = layout(c), so.Residues.from_context(c)
+    assert L["nres"] == 36 and L["D"] == 1296 and L["hid"] == 256
+    assert L["n"] == R.lay.NPARAM == 464400
+    assert (L["w1"], L["b1"], L["w2"], L["w3"], L["w4"], L["b4"], L["emb"]) == \
+        (R.lay.W1, R.lay.B1, R.lay.W2, R.lay.W3, R.lay.W4, R.lay.B4, R.lay.EMBO)
 
 
 def test_forward_and_gradient_kernels_match_oracle():
@@ -40,7 +52,7 @@ def test_forward_and_gradient_kernels_match_oracle():
     c = make_ctx(tag="c2_20mer", itysol=1)
     R = so.Residues.from_context(c)
     V = _confs(c, 96, 4)
-    th = so.init_theta(3)
+    th = so.init_theta(R.lay, 3)
     tgt = np.random.default_rng(5).normal(0, 1, len(V)).astype(np.float32)
     s = Surrogate(c, theta=th)
     say("smmp_mlp_grad c2_20mer 96 conformations")
@@ -49,7 +61,7 @@ def test_forward_and_gradient_kernels_match_oracle():
     lo, go, po = so.loss_grad(R, th.astype(np.float64), V, tgt.astype(np.float64), bf=True)
     assert np.abs(pg - po).max() <= 2e-3 * (np.abs(po).max() + 1.0), (pg[:4], po[:4])
     assert abs(loss - lo) <= 5e-3 * lo
-    for name, (a, b) in _blocks().items():
+    for name, (a, b) in _blocks(R.lay).items():
         ga, gb = g[a:b], go[a:b]
         rel = np.linalg.norm(ga - gb) / (np.linalg.norm(gb) + 1e-12)
         say(f"gradient {name}: relative L2 difference {rel:.2e}")
@@ -61,14 +73,14 @@ def test_pair_gradient_kernels_match_oracle():
     c = make_ctx(tag="c2_20mer", itysol=1)
     R = so.Residues.from_context(c)
     V = _confs(c, 64, 9)
-    th = so.init_theta(5)
+    th = so.init_theta(R.lay, 5)
     tgt = np.random.default_rng(6).normal(0, 1, 32).astype(np.float32)
     s = Surrogate(c, theta=th)
     say("smmp_mlp_grad c2_20mer 32 pairs")
     loss, pg, g = s.grad(V, tgt, pairs=True)
     lo, go, po = so.loss_grad(R, th.astype(np.float64), V, tgt.astype(np.float64), bf=True, pairs=True)
     assert abs(loss - lo) <= 5e-3 * lo
-    for name, (a, b) in _blocks().items():
+    for name, (a, b) in _blocks(R.lay).items():
         if name == "b4":
             assert abs(g[a]) < 1e-4
             continue
@@ -96,6 +108,41 @@ def test_fit_step_is_adam():
     assert np.all(np.abs(th1 - th0) <= lr * 1.0001)
 
 
+def test_screen_in_the_sweep_matches_oracle(tmp_path, monkeypatch):
+    """Every screened trial of one replica (traced by the sweep kernel):
+    the network's change equals the oracle's restatement of the screen,
+    scale * (y(x') - y(x)), on the trajectory the trace itself records."""
+    from smmp_amd.surrogate import Surrogate
+    c = make_ctx(tag="c2_20mer", itysol=1)
+    R = so.Residues.from_context(c)
+    th = so.init_theta(R.lay, 11, 0.5)
+    s = Surrogate(c, theta=th)
+    s.scale = 200.0  # large enough that the network rejects some trials by itself
+    v0 = _confs(c, 1, 12)
+    v0[0, c.idvr - 1] = np.pi + np.random.default_rng(13).normal(0, 0.3, c.nvr)  # near-extended
+    path = tmp_path / "trace.txt"
+    monkeypatch.setenv("SMMP_TRACE_REP", "0")
+    monkeypatch.setenv("SMMP_TRACE_FILE", str(path))
+    say("smmp_sample c2_20mer one replica, 2 sweeps, traced screen")
+    s.sample(np.array([600.0]), np.array([21], dtype=np.uint32), 2, v0, mode=0, groups=1)
+    tr = np.loadtxt(path, ndmin=2)
+    assert len(tr) == 2 * c.nvr
+    x = v0[0].copy()
+    states = []
+    for jv, vnew, ds, st in tr:
+        x1 = x.copy()
+        x1[int(jv)] = vnew
+        states += [x, x1]
+        if st == 2:
+            x = x1
+    y = so.screen_output(R, th, np.array(states))
+    want = s.scale * (y[1::2] - y[0::2])
+    err = np.abs(tr[:, 2] - want)
+    say(f"screen vs oracle: max |d| {err.max():.3g}, outcomes {np.bincount(tr[:, 3].astype(int), minlength=3)}")
+    assert err.max() <= 2e-3 * (np.abs(want).max() + 1.0)
+    assert (tr[:, 3] == 0).any() and (tr[:, 3] == 2).any()
+
+
 def test_zero_network_is_plain_metropolis():
     """Stage 1 passes without a draw when the network's change is zero, and
     stage 2 is then the plain rule: the runs agree bit for bit."""
@@ -114,6 +161,34 @@ def test_zero_network_is_plain_metropolis():
     st = b["da_stats"]
     assert st[:, 1].sum() == 0 and np.array_equal(st[:, 0], st[:, 2])
     assert np.array_equal(st[:, 3], a["acc"])
+
+
+def test_c5_full_size_zero_network_and_oracle():
+    """C5 at its full size: 128 replicas of the 36-residue sequence,
+    ECEPP/3 + OONS, parallel tempering 250-1000 K.  Delayed acceptance with
+    a zero network equals the plain sampler bit for bit, and the final
+    energies of a random-network run equal the FP64 oracle's."""
+    from _oracle import OracleSystem
+    from smmp_amd.surrogate import Surrogate
+    c = make_ctx(tag="c5_36mer", itysol=1)
+    R, nsw = 128, 3
+    temps = np.geomspace(250.0, 1000.0, R)
+    seeds = np.arange(40, 40 + R + 1, dtype=np.uint32)
+    v0 = np.tile(c.vlvr(), (R, 1))
+    v0[:, c.idvr - 1] = np.pi + np.random.default_rng(14).normal(0, 0.3, (R, c.nvr))
+    say(f"C5: {R} replicas x 36-mer, {nsw} sweeps: plain, zero network, random network")
+    a = c.sample(temps, seeds, nsw, v0, mode=1)
+    b = Surrogate.zero(c).sample(temps, seeds, nsw, v0, mode=1)
+    for k in ("acc", "ipoi", "eol", "vlvr", "traj"):
+        assert np.array_equal(a[k], b[k]), k
+    s = Surrogate(c, seed=15, init_scale=0.5)
+    s.scale = 3.0
+    d = s.sample(temps, seeds, nsw, v0, mode=1)
+    assert d["da_stats"][:, 1].sum() > 0
+    o = OracleSystem(c)
+    for r in range(0, R, 8):
+        ref = o.energy(d["vlvr"][r])
+        assert abs(d["eol"][r] - ref[0]) <= 1e-9 * max(1.0, abs(ref[0])), r
 
 
 def test_untrained_network_keeps_the_distribution_ks():
@@ -141,10 +216,10 @@ def test_untrained_network_keeps_the_distribution_ks():
 
 
 def test_trained_network_gains_over_none():
-    """Online training on the sampler's own energies, then the same
+    """Online training on the sampler's own moves, then the same
     delayed-acceptance run with the trained and with a zero network: the
     trained one needs fewer exact energies per accepted move."""
-    from smmp_amd.surrogate import Surrogate
+    from smmp_amd.surrogate import Surrogate, move_pairs
     c = make_ctx(tag="c2_20mer", itysol=1)
     R = 32
     temps = np.geomspace(250.0, 1000.0, R)
@@ -154,12 +229,12 @@ def test_trained_network_gains_over_none():
     d = c.sample(temps, np.arange(50, 50 + R + 1, dtype=np.uint32), 100, w["vlvr"], mode=1, with_traj=False,
                  snap_every=1)
     V = d["snap_vlvr"].reshape(-1, c.nvt)
-    E = d["snap_e"].ravel()
+    pv, de, _ = move_pairs(c, np.repeat(V, 8, axis=0), seed=1)
     s = Surrogate(c, seed=3)
-    say(f"fit on {len(E)} energies")
-    hist = s.fit(V, E, epochs=30, batch=128, lr=2e-3, clip=float(np.quantile(E, 0.98)))
+    say(f"fit on {len(de)} moves")
+    hist = s.fit_moves(pv, de, epochs=12, batch=512, lr=2e-3, clip=10.0)
     say(f"loss: first epoch {hist[0]:.3f}, last {hist[-1]:.3f}")
-    assert hist[-1] < 0.6 * hist[0]
+    assert hist[-1] < 0.9 * hist[0]
     seeds = np.arange(200, 200 + R + 1, dtype=np.uint32)
     t = s.sample(temps, seeds, 20, d["vlvr"], mode=1, with_traj=False)
     z = Surrogate.zero(c).sample(temps, seeds, 20, d["vlvr"], mode=1, with_traj=False)

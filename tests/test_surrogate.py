@@ -34,10 +34,12 @@ def test_network_gradient_matches_finite_differences():
     rng = np.random.default_rng(0)
     V = np.tile(c.vlvr(), (4, 1)) + rng.normal(0, 1, (4, len(c.vlvr())))
     tgt = np.array([0.3, -1.2, 2.0, 0.1])
-    th = so.init_theta(2).astype(np.float64)
+    L = R.lay
+    assert L.D == 5 * 36 and L.NPARAM == 180 * 256 + 2 * 256 * 256 + 4 * 256 + 16 + 32 * 16
+    th = so.init_theta(L, 2).astype(np.float64)
     loss, g, _ = so.loss_grad(R, th, V, tgt, bf=False)
-    idx = [so.W1 + 5, so.W1 + 61 * 64 + 3, so.B1 + 3, so.W2 + 77, so.B2 + 9, so.W3 + 1000, so.B3 + 1,
-           so.W4 + 7, so.B4, so.EMBO + 16 * 8 + 3, so.EMBO + 16 * 13 + 5, so.EMBO + 16 * 0 + 2]
+    idx = [L.W1 + 5, L.W1 + 61 * 256 + 3, L.W1 + 179 * 256 + 255, L.B1 + 3, L.W2 + 77, L.B2 + 9, L.W3 + 1000,
+           L.B3 + 1, L.W4 + 7, L.B4, L.EMBO + 16 * 8 + 3, L.EMBO + 16 * 13 + 5, L.EMBO + 16 * 0 + 2]
     for i in idx:
         h = 1e-6
         tp, tm = th.copy(), th.copy()
@@ -55,9 +57,10 @@ def test_pair_loss_gradient_matches_finite_differences():
     rng = np.random.default_rng(1)
     V = np.tile(c.vlvr(), (6, 1)) + rng.normal(0, 1, (6, len(c.vlvr())))
     tgt = np.array([0.5, -0.7, 1.1])
-    th = so.init_theta(4).astype(np.float64)
+    L = R.lay
+    th = so.init_theta(L, 4).astype(np.float64)
     _, g, _ = so.loss_grad(R, th, V, tgt, bf=False, pairs=True)
-    for i in (so.W1 + 11, so.B1, so.W2 + 500, so.W3 + 3, so.W4 + 2, so.EMBO + 16 * 19 + 1):
+    for i in (L.W1 + 11, L.B1, L.W2 + 500, L.W3 + 3, L.W4 + 2, L.EMBO + 16 * 19 + 1):
         h = 1e-6
         tp, tm = th.copy(), th.copy()
         tp[i] += h
@@ -66,7 +69,46 @@ def test_pair_loss_gradient_matches_finite_differences():
               - so.loss_grad(R, tm, V, tgt, bf=False, pairs=True)[0]) / (2 * h)
         assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(g[i])), (i, g[i], fd)
     # the bias of the output cancels in a difference
-    assert abs(g[so.B4]) < 1e-12
+    assert abs(g[L.B4]) < 1e-12
+
+
+def test_whole_chain_features_closed_form():
+    """Column 36 r + 2 s holds sin, the next cos, of slot s of residue r;
+    slots a residue lacks are zero; the embedding columns repeat the type's
+    row; one variable moves exactly two columns."""
+    c = make_ctx(tag="enkefa", itysol=1)
+    R = so.Residues.from_context(c)
+    L = R.lay
+    th = so.init_theta(L, 7)
+    v = c.vlvr()
+    X = R.features(v, th, bf=False)[0]
+    emb = th[L.EMBO:].reshape(32, 16)
+    for r in range(R.nres):
+        for s in range(10):
+            col = 36 * r + 2 * s
+            if s < R.nvar[r]:
+                a = v[R.var0[r] + s]
+                assert X[col] == np.float32(np.sin(a)) and X[col + 1] == np.float32(np.cos(a))
+            else:
+                assert X[col] == 0 and X[col + 1] == 0
+        assert np.array_equal(X[36 * r + 20:36 * r + 36], emb[R.type[r]].astype(np.float64))
+    v2 = v.copy()
+    v2[R.var0[2] + 1] += 0.7
+    d = np.nonzero(R.features(v2, th, bf=False)[0] != X)[0]
+    assert list(d) == [36 * 2 + 2, 36 * 2 + 3]
+
+
+def test_screen_output_matches_forward():
+    """The sampler's screen (integer layer-1 sums, then FP32 layers) gives
+    the network's output to within the rounding of the two forms."""
+    c = make_ctx(tag="enkefa", itysol=1)
+    R = so.Residues.from_context(c)
+    th = so.init_theta(R.lay, 3)
+    rng = np.random.default_rng(4)
+    V = np.tile(c.vlvr(), (5, 1)) + rng.normal(0, 1, (5, len(c.vlvr())))
+    y, _ = so.forward(R, th.astype(np.float64), V, bf=True)
+    ys = so.screen_output(R, th, V)
+    assert np.abs(ys - y).max() <= 2e-3 * (np.abs(y).max() + 1.0), (ys, y)
 
 
 def _da_kernel(E, S, beta):
