@@ -108,67 +108,108 @@ def cpu_baseline(seqf, budget_s: float):
         o.partem(OracleMT(777), temps, 0, nsw)
         dt = time.perf_counter() - t0
         kind = "port"
-    return {"value": round(nrep * nsw / dt, 4), "unit": "sweeps/s", "cores": 1, "kind": kind,
-            "sample": f"PT {nrep} replicas x {nsw} sweeps of the {LENGTH}-residue system "
-                      f"(ECEPP/3+OONS, {TMIN:g}-{TMAX:g} K), one host core, {dt:.1f} s"}
+    out = {"value": round(nrep * nsw / dt, 4), "unit": "sweeps/s", "cores": 1, "kind": kind,
+           "sample": f"PT {nrep} replicas x {nsw} sweeps of the {LENGTH}-residue system "
+                     f"(ECEPP/3+OONS, {TMIN:g}-{TMAX:g} K), one host core, {dt:.1f} s"}
+    if kind == "reference":  # the same sample in one process per host core (the reference is serial)
+        ncore = max(1, min(16, len(os.sched_getaffinity(0))))
+        with tempfile.TemporaryDirectory() as d:
+            procs = []
+            t0 = time.perf_counter()
+            for k in range(ncore):
+                wd = os.path.join(d, str(k))
+                os.mkdir(wd)
+                workloads.write_temperatures(os.path.join(wd, "temperatures"), temps)
+                procs.append(subprocess.Popen([drv, "partem", workloads.default_libdir(), seqf, "1", str(777 + k),
+                                               str(nrep), "0", str(nsw)], cwd=wd, stdout=subprocess.DEVNULL))
+            for pr in procs:
+                pr.wait(timeout=max(120.0, 20 * budget_s))
+            dtn = time.perf_counter() - t0
+        out["all_cores"] = {"value": round(ncore * nrep * nsw / dtn, 3), "unit": "sweeps/s", "cores": ncore,
+                            "sample": f"{ncore} independent processes of the same sample, {dtn:.1f} s"}
+    return out
 
 
-def run_da(args):
-    """Delayed acceptance against the plain sampler on parallel tempering
-    with 128 replicas of a random 36-residue sequence (seed 5), ECEPP/3 +
-    OONS, 250-1000 K: the network is trained online on the sampler's own
-    energies, then both samplers run the same sweeps from the same state."""
+def autocorr_time(x: np.ndarray) -> float:
+    """Integrated autocorrelation time of a series (Sokal's window: the
+    smallest W with W >= 5 tau(W))."""
+    x = np.asarray(x, dtype=np.float64) - np.mean(x)
+    n = len(x)
+    if n < 8 or not np.any(x):
+        return 1.0
+    f = np.fft.rfft(x, 2 * n)
+    acf = np.fft.irfft(f * np.conj(f))[:n]
+    acf /= acf[0]
+    tau = 1.0
+    for w in range(1, n):
+        tau = 1.0 + 2.0 * acf[1:w + 1].sum()
+        if w >= 5.0 * tau:
+            break
+    return max(tau, 1.0)
+
+
+def thinned(traj: np.ndarray, burn: int, taus) -> np.ndarray:
+    """Pooled samples of every slot's energy series after burn sweeps, each
+    slot thinned by twice its autocorrelation time."""
+    out = []
+    for k in range(traj.shape[1]):
+        step = max(1, int(np.ceil(2.0 * taus[k])))
+        out.append(traj[burn::step, k])
+    return np.concatenate(out)
+
+
+def da_experiment(c, R, temps, warmup, rounds, collect, moves_per_state, epochs, softs, timed, ks_sweeps, ks_burn,
+                  log_tag, keep_traj=False):
+    """Delayed acceptance on parallel tempering against the plain sampler.
+    Online training: 'rounds' times, 'collect' sweeps of the exact sampler
+    with a snapshot of every replica each sweep, 'moves_per_state' random
+    single moves per snapshot with their exact energy changes (FP64,
+    smmp_energy_batch: two energies per move), and a cosine-scheduled fit of
+    the network on all moves so far.  Then, from one state, 'timed' sweeps
+    of the plain sampler and of delayed acceptance at each softening factor
+    (timed by the sweep kernel's HIP events), and for the KS test on the
+    pooled energies 'ks_burn' sweeps of the plain sampler, then from that
+    state three independent runs of 'ks_sweeps' sweeps (plain, delayed
+    acceptance at the best factor, a second plain run as the control), each
+    slot thinned by twice the largest autocorrelation time of its three
+    runs."""
     from scipy.stats import ks_2samp
-    from smmp_amd import _ffi, workloads
+    from smmp_amd import _ffi
     from smmp_amd.surrogate import Surrogate, move_pairs
-    R, L = 128, 36
-    tmp = tempfile.mkdtemp(prefix="smmp_da_")
-    c, seq, _ = make_system(tmp, length=L, seed=5, tag="c5")
-    temps = workloads.geometric_temperatures(R, TMIN, TMAX)
     seeds = lambda k: np.arange(k, k + R + 1, dtype=np.uint32)  # noqa: E731
-    log(f"delayed acceptance: {R} replicas x {L} residues, {c.nvr} free torsions")
     t0 = time.perf_counter()
-    state = c.sample(temps, seeds(1), args.da_warmup, random_starts(c, R, 12345), mode=1, with_traj=False)["vlvr"]
-    log(f"warm-up {args.da_warmup} sweeps: {time.perf_counter() - t0:.1f} s")
+    state = c.sample(temps, seeds(1), warmup, random_starts(c, R, 12345), mode=1, with_traj=False)["vlvr"]
+    log(f"{log_tag}: warm-up {warmup} sweeps {time.perf_counter() - t0:.1f} s")
     net = Surrogate(c, seed=3)
-    Ps, Ds, Vs, Es = [], [], [], []
+    Ps, Ds = [], []
     t_collect = t_pairs = t_train = 0.0
-    hist = np.zeros(0)
-    for rnd in range(args.da_rounds):
+    hist = np.zeros(1)
+    for rnd in range(rounds):
         t0 = time.perf_counter()
-        d = c.sample(temps, seeds(100 + 7 * rnd), args.da_collect, state, mode=1, with_traj=False, snap_every=1)
+        d = c.sample(temps, seeds(100 + 7 * rnd), collect, state, mode=1, with_traj=False, snap_every=1)
         t_collect += time.perf_counter() - t0
         state = d["vlvr"]
-        Vs.append(d["snap_vlvr"].reshape(-1, c.nvt))
-        Es.append(d["snap_e"].ravel())
-        V, E = np.concatenate(Vs), np.concatenate(Es)
-        if args.da_train in ("energy", "both"):  # total energies of the sampled conformations
-            t0 = time.perf_counter()
-            hist = net.fit(V, E, epochs=args.da_epochs, batch=512, lr=2e-3, seed=rnd,
-                           clip=float(np.quantile(E, 0.99)), rescale=(rnd == 0))
-            t_train += time.perf_counter() - t0
-            log(f"round {rnd}: {len(E)} energies, loss {hist[0]:.4f} -> {hist[-1]:.4f} (network units^2), "
-                f"scale {net.scale:.2f} kcal/mol")
-        if args.da_train in ("moves", "both"):  # exact changes of single moves from them
-            t0 = time.perf_counter()
-            pv, de, _ = move_pairs(c, d["snap_vlvr"].reshape(-1, c.nvt), seed=rnd)
-            t_pairs += time.perf_counter() - t0
-            Ps.append(pv)
-            Ds.append(de)
-            P, D = np.concatenate(Ps), np.concatenate(Ds)
-            t0 = time.perf_counter()
-            hist = net.fit_moves(P, D, epochs=args.da_epochs, batch=512, lr=2e-3 if args.da_train == "moves" else 5e-4,
-                                 seed=rnd, rescale=(args.da_train == "moves" and rnd == 0))
-            t_train += time.perf_counter() - t0
-            log(f"round {rnd}: {len(D)} moves (|dE| <= 20: {np.mean(np.abs(D) <= 20):.2f}), loss {hist[0]:.4f} -> "
-                f"{hist[-1]:.4f} (network units^2), scale {net.scale:.2f} kcal/mol")
-    c.sample(temps, seeds(700), 10, state, mode=1, with_traj=False)  # the system back on the device
-    K = args.steps
+        t0 = time.perf_counter()
+        V = np.repeat(d["snap_vlvr"].reshape(-1, c.nvt), moves_per_state, axis=0)
+        pv, de, _ = move_pairs(c, V, seed=rnd)
+        t_pairs += time.perf_counter() - t0
+        Ps.append(pv)
+        Ds.append(de)
+        P, D = np.concatenate(Ps), np.concatenate(Ds)
+        t0 = time.perf_counter()
+        hs = []
+        for ep in range(epochs):  # cosine schedule, Adam moments kept across epochs
+            lr = 1e-4 + 0.5 * (2e-3 - 1e-4) * (1.0 + math.cos(math.pi * ep / epochs))
+            hs.append(net.fit_moves(P, D, epochs=1, batch=1024, lr=lr, seed=1000 * rnd + ep, clip=10.0,
+                                    rescale=(rnd == 0 and ep == 0))[0])
+        hist = np.array(hs)
+        t_train += time.perf_counter() - t0
+        log(f"{log_tag}: round {rnd}: {len(D)} moves ({2 * len(D)} exact energies), loss {hist[0]:.4f} -> "
+            f"{hist[-1]:.4f}, train {t_train:.1f} s")
     runs = {}
-    softs = [float(x) for x in str(args.da_soften).split(",")]
-    cases = [("plain", lambda: c.sample(temps, seeds(900), K, state, mode=1))]
+    cases = [("plain", lambda: c.sample(temps, seeds(900), timed, state, mode=1, with_traj=False))]
     for f in softs:  # the network's change scaled by f in both stages (Christen-Fox stays exact)
-        cases.append((f"da{f:g}", lambda f=f: c.sample(temps, seeds(900), K, state, mode=1, da=True,
+        cases.append((f"da{f:g}", lambda f=f: net.sample(temps, seeds(900), timed, state, mode=1, with_traj=False,
                                                          da_scale=net.scale * f)))
     for name, fn in cases:
         c.profile(1)
@@ -176,56 +217,83 @@ def run_da(args):
         ks = c.profile_read(_ffi.KERNEL_SWEEP)
         c.profile(False)
         runs[name] = (out, ks["ms"] / 1e3)
-        log(f"{name}: {K} sweeps in {ks['ms']:.1f} ms of sweep kernel, accepted {out['acc'].sum():.0f}")
-    (plain, tp), (da, td) = runs["plain"], runs[f"da{softs[0]:g}"]
-    soften = {f"{f:g}": round(float((runs[f"da{f:g}"][0]["acc"].sum() / runs[f"da{f:g}"][1])
-                                    / (plain["acc"].sum() / tp)), 3) for f in softs}
-    log(f"gain by softening factor: {soften}")
-    control = c.sample(temps, seeds(4242), K, state, mode=1)  # a second plain run: the KS test's own spread
-    burn, thin = K // 5, 5
-    a, b, z = plain["traj"][burn::thin], da["traj"][burn::thin], control["traj"][burn::thin]
-    ks_pooled = float(ks_2samp(a.ravel(), b.ravel()).pvalue)
-    ks_control = float(ks_2samp(a.ravel(), z.ravel()).pvalue)
-    slot_p = np.array([ks_2samp(a[:, k], b[:, k]).pvalue for k in range(R)])
-    slot_c = np.array([ks_2samp(a[:, k], z[:, k]).pvalue for k in range(R)])
+        log(f"{log_tag}: {name}: {timed} sweeps in {ks['ms']:.1f} ms of sweep kernel, accepted {out['acc'].sum():.0f}")
+    (plain, tp) = runs["plain"]
+    rate = lambda o, t: float(o["acc"].sum()) / t  # noqa: E731
+    gains = {f"{f:g}": round(rate(*runs[f"da{f:g}"]) / rate(plain, tp), 3) for f in softs}
+    best = max(softs, key=lambda f: gains[f"{f:g}"])
+    da, td = runs[f"da{best:g}"]
+    ntrial = float(R * timed * c.nvr)
     st = da["da_stats"]
-    ntrial = float(R * K * c.nvr)
-    gain = (da["acc"].sum() / td) / (plain["acc"].sum() / tp)
+    res = {"gain": gains[f"{best:g}"], "softening": best, "gain_by_softening": gains,
+           "accepted_per_s_plain": round(rate(plain, tp), 1), "accepted_per_s_da": round(rate(da, td), 1),
+           "sweeps_per_s_plain": round(R * timed / tp, 1), "sweeps_per_s_da": round(R * timed / td, 1),
+           "acceptance_plain": round(float(plain["acc"].sum()) / ntrial, 4),
+           "acceptance_da": round(float(da["acc"].sum()) / ntrial, 4),
+           "rejected_by_network": round(float(st[:, 1].sum() / st[:, 0].sum()), 4),
+           "exact_energies_per_accept_plain": round(ntrial / float(plain["acc"].sum()), 3),
+           "exact_energies_per_accept_da": round(float(st[:, 2].sum() / max(st[:, 3].sum(), 1.0)), 3),
+           "training": {"moves": int(sum(len(x) for x in Ds)), "exact_energies": int(2 * sum(len(x) for x in Ds)),
+                        "rounds": rounds, "sweeps_per_round": collect, "moves_per_snapshot": moves_per_state,
+                        "epochs_per_round": epochs, "final_loss": round(float(hist[-1]), 5),
+                        "scale_kcal_per_unit": round(net.scale, 3), "parameters": int(net.n),
+                        "collect_s": round(t_collect, 2), "pairs_s": round(t_pairs, 2), "train_s": round(t_train, 2)}}
+    if ks_sweeps > 0:
+        # an exact sampler started from the stationary distribution stays in
+        # it: equilibrate one state with the plain sampler, then run all
+        # three samplers from it and compare their energies at every sweep
+        t0 = time.perf_counter()
+        eq = c.sample(temps, seeds(1500), ks_burn, state, mode=1, with_traj=False)["vlvr"]
+        log(f"{log_tag}: KS equilibration {ks_burn} sweeps done, {time.perf_counter() - t0:.0f} s")
+        a = c.sample(temps, seeds(2000), ks_sweeps, eq, mode=1)["traj"]
+        log(f"{log_tag}: KS plain run done, {time.perf_counter() - t0:.0f} s")
+        b = net.sample(temps, seeds(3000), ks_sweeps, eq, mode=1, da_scale=net.scale * best)["traj"]
+        log(f"{log_tag}: KS delayed-acceptance run done, {time.perf_counter() - t0:.0f} s")
+        z = c.sample(temps, seeds(4000), ks_sweeps, eq, mode=1)["traj"]
+        log(f"{log_tag}: KS control run done, {time.perf_counter() - t0:.0f} s")
+        burn = 0
+        # each slot thinned by twice the largest of its three runs'
+        # autocorrelation times (the slowest slots hold few independent
+        # samples; no run's estimate is preferred)
+        taus = [max(autocorr_time(a[:, k]), autocorr_time(b[:, k]), autocorr_time(z[:, k])) for k in range(R)]
+        sa, sb, sz = thinned(a, burn, taus), thinned(b, burn, taus), thinned(z, burn, taus)
+        res["ks"] = {"sweeps": ks_sweeps, "equilibration_sweeps": ks_burn, "samples_per_run": int(len(sa)),
+                     "autocorr_sweeps_median": round(float(np.median(taus)), 2),
+                     "autocorr_sweeps_max": round(float(np.max(taus)), 2),
+                     "pooled_p_da_vs_plain": round(float(ks_2samp(sa, sb).pvalue), 4),
+                     "pooled_p_control_plain_vs_plain": round(float(ks_2samp(sa, sz).pvalue), 4),
+                     "run_s": round(time.perf_counter() - t0, 1)}
+        log(f"{log_tag}: KS {res['ks']}")
+        if keep_traj:
+            res["traj"] = {"plain": a, "da": b, "control": z}
+    return res
+
+
+def run_da(args):
+    """Delayed acceptance against the plain sampler on parallel tempering
+    with 128 replicas of a random 36-residue sequence (seed 5), ECEPP/3 +
+    OONS, 250-1000 K, the network trained online on the sampler's own
+    moves."""
+    from smmp_amd import workloads
+    R, L = 128, 36
+    tmp = tempfile.mkdtemp(prefix="smmp_da_")
+    c, seq, _ = make_system(tmp, length=L, seed=5, tag="c5")
+    temps = workloads.geometric_temperatures(R, TMIN, TMAX)
+    log(f"delayed acceptance: {R} replicas x {L} residues, {c.nvr} free torsions")
+    softs = [float(x) for x in str(args.da_soften).split(",")]
+    r = da_experiment(c, R, temps, args.da_warmup, args.da_rounds, args.da_collect, args.da_moves, args.da_epochs,
+                      softs, args.steps, args.da_ks, args.da_ks_burn, "da")
     line = {
         "metric": "DA-surrogate vs plain GPU: exact-energy-equivalent accepted moves/s",
-        "value": round(float(gain), 3),
-        "unit": "x",
-        "n_gpus": 1,
-        "steps": K,
-        "higher_is_better": True,
+        "value": r["gain"], "unit": "x", "n_gpus": 1, "steps": args.steps, "higher_is_better": True,
         "dtype": "f64 (energies), bf16 (network)",
         "data": "synthetic: random 36-residue sequence (seed 5), random initial dihedrals; network trained on the "
-                "sampler's own energies",
+                "sampler's own moves",
         "config": {"workload": f"parallel tempering, {R} replicas x {L}-residue random sequence, ECEPP/3+OONS, "
                                f"T geometric {TMIN:g}-{TMAX:g} K, exchange every sweep",
-                   "replicas": R, "residues": L, "free_torsions": c.nvr, "softening": softs[0]},
-        "accepted_per_s_plain": round(float(plain["acc"].sum() / tp), 1),
-        "accepted_per_s_da": round(float(da["acc"].sum() / td), 1),
-        "sweeps_per_s_plain": round(R * K / tp, 1),
-        "sweeps_per_s_da": round(R * K / td, 1),
-        "acceptance_plain": round(float(plain["acc"].sum() / ntrial), 4),
-        "acceptance_da": round(float(da["acc"].sum() / ntrial), 4),
-        "rejected_by_network": round(float(st[:, 1].sum() / st[:, 0].sum()), 4),
-        "exact_energies_per_accept_plain": round(ntrial / float(plain["acc"].sum()), 3),
-        "exact_energies_per_accept_da": round(float(st[:, 2].sum() / max(st[:, 3].sum(), 1.0)), 3),
-        "gain_by_softening": soften,
-        "ks_pooled_p": round(ks_pooled, 4),
-        "ks_per_temperature_median_p": round(float(np.median(slot_p)), 4),
-        "ks_per_temperature_frac_p_above_0.05": round(float(np.mean(slot_p > 0.05)), 3),
-        "ks_control_plain_vs_plain": {"pooled_p": round(ks_control, 4),
-                                      "per_temperature_median_p": round(float(np.median(slot_c)), 4),
-                                      "per_temperature_frac_p_above_0.05": round(float(np.mean(slot_c > 0.05)), 3)},
-        "training": {"targets": args.da_train, "energies": int(sum(len(e) for e in Es)),
-                     "moves": int(sum(len(d) for d in Ds)), "rounds": args.da_rounds,
-                     "epochs_per_round": args.da_epochs, "final_loss": round(float(hist[-1]), 5),
-                     "scale_kcal_per_unit": round(net.scale, 3), "collect_s": round(t_collect, 2),
-                     "pairs_s": round(t_pairs, 2), "train_s": round(t_train, 2)},
+                   "replicas": R, "residues": L, "free_torsions": c.nvr},
     }
+    line.update({k: v for k, v in r.items() if k != "gain"})
     print(json.dumps(line), flush=True)
 
 
@@ -322,6 +390,56 @@ def run_esolan(args):
                     "cpu_oracle_replica_sweeps_per_s": smp_cpu},
     }
     print(json.dumps(line), flush=True)
+
+
+def c3_headline(K: int) -> dict:
+    """Energy evaluations/s of the C3 batch (4,096 conformations over 256
+    random sequences of 10-60 residues, ECEPP/3 + OONS, resident on the
+    device) in the mixed and FP64 modes, with the energy kernel's roofline
+    fractions (mixed against the FP32 vector peak, FP64 against FP64)."""
+    from smmp_amd import _ffi, workloads
+    nseq, per = 256, 16
+    tmp = tempfile.mkdtemp(prefix="smmp_c3h_")
+    seqs = workloads.c3_sequences(nseq, 10, 60, seed=3)
+    ctxs, rows, cs = [], [], []
+    for i, sq in enumerate(seqs):
+        seqf = workloads.write_seq_file(os.path.join(tmp, f"s{i}.seq"), sq, f"s{i}")
+        c = _ffi.Context()
+        c.view("isolty.itysol")[0] = 1
+        c.init_energy(workloads.default_libdir())
+        c.init_molecule(1, "nh2", "cooh", seqf, " ")
+        ctxs.append(c)
+        v = np.tile(c.vlvr(), (per, 1))
+        v[:, c.idvr - 1] = workloads.uniform_dihedrals(per, c.nvr, 1000 + i)
+        rows.append(v.ravel())
+        cs += [i] * per
+    cs = np.asarray(cs, dtype=np.int32)
+    n = len(cs)
+    B = _ffi.EnergyBatch(ctxs, cs)
+    B.load(np.concatenate(rows))
+    out = {}
+    for prec, name, peak in ((1, "mixed", PEAK_FP32), (0, "fp64", PEAK_FP64)):
+        B.run(prec)
+        B.read()
+        ctxs[0].profile(True)
+        t0 = time.perf_counter()
+        for _ in range(K):
+            B.run(prec)
+        B.read()
+        dt = time.perf_counter() - t0
+        ke = ctxs[0].profile_read(_ffi.KERNEL_ENERGY)
+        ctxs[0].profile(False)
+        wk = ke["work"]
+        flops = FLOP_PAIR * wk[0] + FLOP_POINT * 122 * wk[1] + FLOP_BUILD * wk[2]
+        ach = flops / (ke["ms"] * 1e-3) / 1e12
+        out[name] = {"evals_per_s": round(K * n / dt, 1), "k_energy_ms_per_batch": round(ke["ms"] / K, 3),
+                     "roofline_frac": round(ach / peak, 5), "achieved_tflops": round(ach, 3), "peak_tflops": peak}
+    B.close()
+    for c in ctxs:
+        c.close()
+    log(f"C3 headline: {out}")
+    return {"workload": f"4,096 conformations over {nseq} random sequences of 10-60 residues, ECEPP/3+OONS, "
+                        f"resident batch, {K} timed batches", **out}
 
 
 def run_c3(args):
@@ -579,6 +697,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=30, help="untimed sweeps from the random start")
     ap.add_argument("--precision", type=int, default=0, help="0 FP64, 1 FP32 pair terms")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="leave out the energy-evaluation (C3) and delayed-acceptance (C5) figures of the default line")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-MiB device buffer 5 times (calibrates HBM byte counters) and exit")
     ap.add_argument("--groups", type=int, default=0,
@@ -588,15 +708,16 @@ def main():
     ap.add_argument("--da", action="store_true",
                     help="delayed acceptance with the trained surrogate against the plain sampler "
                          "(128 replicas x 36 residues); --steps sets the compared sweeps")
-    ap.add_argument("--da-warmup", type=int, default=100)
-    ap.add_argument("--da-soften", default="0.35",
-                    help="factor(s) on the network's energy change in both stages (comma list: the first is "
-                         "the reported value, the others are timed beside it)")
-    ap.add_argument("--da-rounds", type=int, default=4, help="collect-and-train rounds")
-    ap.add_argument("--da-collect", type=int, default=150, help="sweeps per round (snapshots every sweep)")
+    ap.add_argument("--da-warmup", type=int, default=300)
+    ap.add_argument("--da-soften", default="1,0.7,0.5,0.35",
+                    help="factors on the network's energy change in both stages (comma list; the best one is "
+                         "reported and used for the KS runs)")
+    ap.add_argument("--da-rounds", type=int, default=2, help="collect-and-train rounds")
+    ap.add_argument("--da-collect", type=int, default=40, help="sweeps per round (snapshots every sweep)")
+    ap.add_argument("--da-moves", type=int, default=50, help="random single moves per snapshot")
     ap.add_argument("--da-epochs", type=int, default=8, help="epochs per round")
-    ap.add_argument("--da-train", choices=("energy", "moves", "both"), default="moves",
-                    help="fit total energies, exact changes of single moves, or energies then moves")
+    ap.add_argument("--da-ks", type=int, default=8000, help="sweeps of each KS run (0: none)")
+    ap.add_argument("--da-ks-burn", type=int, default=4000, help="plain sweeps that equilibrate the KS runs' start")
     ap.add_argument("--esolan", action="store_true",
                     help="energies with the analytic solvent (itysol -1) of sampled conformations; "
                          "--steps sets the timed batches")
@@ -717,6 +838,20 @@ def main():
                      "flop_per_launch": round(flops / max(ks["launches"], 1), 1)},
         "cpu_baseline": cpu,
     }
+    if not args.no_extras:
+        # the other two parts of the metric: energy evaluations/s (C3) and
+        # delayed acceptance against the plain sampler (C5)
+        line["energy_evals_per_s"] = c3_headline(5)
+        R5 = 128
+        c5, _, _ = make_system(tmp, length=36, seed=5, tag="c5")
+        t0 = time.perf_counter()
+        da = da_experiment(c5, R5, workloads.geometric_temperatures(R5, TMIN, TMAX), 300, 2, 40, 50, 8,
+                           [1.0, 0.5, 0.35], 200, 0, 0, "da")
+        da["workload"] = ("parallel tempering, 128 replicas x 36-residue random sequence (seed 5), ECEPP/3+OONS, "
+                          "250-1000 K; network trained online on the sampler's own moves (1,024,000 exact "
+                          "energies); 200 timed sweeps per sampler")
+        da["total_s"] = round(time.perf_counter() - t0, 1)
+        line["da_gain"] = da
     print(json.dumps(line), flush=True)
 
 

@@ -739,9 +739,9 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
       const bool full = Q.sc[5] != 0.0;
       if (DA && !rigid) {
         // ---- delayed acceptance, first stage: the network's energy change
-        da_trial(P.da_lds_off, nvt, P.da, jv, Q.sc[15], Q.sc[2]);
+        const float ytrial = da_trial(D, P.da, jv, Q.sc[15], Q.sc[2]);
         if (tid == 0) {
-          const double ds = ((double)D.y[1] - (double)D.y[0]) * P.da_scale;
+          const double ds = ((double)ytrial - (double)D.y[0]) * P.da_scale;
           if (P.trace && g == 0 && r == P.trace_rep && *P.trace_n < P.trace_cap) {
             // delayed acceptance traces its screens: variable, new value,
             // network change, 0 (rejected by the network; the exact stage

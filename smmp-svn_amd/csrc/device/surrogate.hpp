@@ -75,72 +75,67 @@ __device__ __forceinline__ int64_t da_term(__bf16 w, __bf16 x) {
   return __double2ll_rn((double)(float)w * (double)(float)x * kDaFixScale);
 }
 
-// One 256 -> 256 SiLU layer on the single row 'in' (LDS): wave w computes
-// outputs 32w .. 32w+31 (two 16-column MFMA blocks over 8 k-steps of 32),
-// with W^T [out][in] from global memory.  Writes bf16(silu(z + b)) to out,
-// or (last layer) returns this wave's partial output sum w4 . h (lane 0).
-template <bool LAST>
-__device__ __forceinline__ float da_layer(const __bf16* in, const __bf16* __restrict__ wt, const float* __restrict__ b,
-                                          const float* __restrict__ w4, __bf16* out) {
-  const int wave = threadIdx.x / 64, lane = threadIdx.x & 63;
-  const int kq = 8 * (lane >> 4);
-  float part = 0.0f;
+// The B fragments of one 16-column block of W^T [out][in] for this lane:
+// 8 k-steps of 32 (input rows 8 (lane / 16) .. +7 of each step) of output
+// column n.  Issued as one batch of 16-byte loads.
+__device__ __forceinline__ void da_load_block(const __bf16* __restrict__ wt, int n, int kq, da_bf16x8 (&bw)[8]) {
 #pragma unroll
-  for (int half = 0; half < 2; ++half) {
-    const int n = 16 * (2 * wave + half) + (lane & 15);
-    da_bf16x8 bw[8];
-#pragma unroll
-    for (int kk = 0; kk < 8; ++kk)  // every load issued before the first product
-      bw[kk] = *reinterpret_cast<const da_bf16x8*>(&wt[(size_t)n * kMlpHid + 32 * kk + kq]);
-    da_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int kk = 0; kk < 8; ++kk) {
-      da_bf16x8 a;
-      if ((lane & 15) == 0) {
-        a = *reinterpret_cast<const da_bf16x8*>(&in[32 * kk + kq]);
-      } else {
-#pragma unroll
-        for (int j = 0; j < 8; ++j) a[j] = (__bf16)0.0f;
-      }
-      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw[kk], acc, 0, 0, 0);
-    }
-    // row 0 of the result sits in lanes 0..15, element 0
-    if (lane < 16) {
-      const __bf16 h = (__bf16)silu(acc[0] + b[n]);
-      if (LAST)
-        part += (float)h * w4[n];
-      else
-        out[n] = h;
-    }
-  }
-  if (LAST) {
-    for (int o = 8; o > 0; o >>= 1) part += __shfl_xor(part, o, 64);  // lanes 0..15
-  }
-  return part;
+  for (int kk = 0; kk < 8; ++kk) bw[kk] = *reinterpret_cast<const da_bf16x8*>(&wt[(size_t)n * kMlpHid + 32 * kk + kq]);
 }
 
-// Output of the network for the layer-1 sums z (LDS, kMlpHid) into D.y[slot]
-// (all threads; barriers inside; D.y[slot] is valid on return).
-__device__ __attribute__((noinline)) void da_output(size_t off, int nres, int nvt, MlpScreen S, int use_trial,
-                                                    int slot) {
-  extern __shared__ double smem[];
-  const DaLds D = carve_da_lds(reinterpret_cast<char*>(smem) + off, nres, nvt);
-  const int64_t* z = use_trial ? D.z1t : D.z1;
-  const float* th = S.theta;
-  const int t = threadIdx.x;
-  if (t < kMlpHid) D.h1[t] = (__bf16)silu((float)((double)z[t] * kDaFixInv) + th[S.lo.b1 + t]);
-  __syncthreads();
-  da_layer<false>(D.h1, S.w2t, th + S.lo.b2, nullptr, D.h2);
-  __syncthreads();
-  const float p = da_layer<true>(D.h2, S.w3t, th + S.lo.b3, th + S.lo.w4, nullptr);
-  if ((t & 63) == 0) D.red[t / 64] = p;
-  __syncthreads();
-  if (t == 0) {
-    float y = th[S.lo.b4];
-    for (int w = 0; w < 8; ++w) y += D.red[w];
-    D.y[slot] = y;
+// Row 0 of in (LDS, kMlpHid BF16) times the loaded block: the result for
+// output column n in lanes 0..15 (element 0).
+__device__ __forceinline__ float da_mfma_block(const __bf16* in, int lane, int kq, const da_bf16x8 (&bw)[8]) {
+  da_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int kk = 0; kk < 8; ++kk) {
+    da_bf16x8 a;
+    if ((lane & 15) == 0) {
+      a = *reinterpret_cast<const da_bf16x8*>(&in[32 * kk + kq]);
+    } else {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) a[j] = (__bf16)0.0f;
+    }
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw[kk], acc, 0, 0, 0);
   }
+  return acc[0];
+}
+
+// The network's output for the layer-1 sums z (LDS): h1 from z (threads <
+// kMlpHid), then the two 256 x 256 layers, wave w taking outputs 32 w ..
+// 32 w + 31 as two 16-column MFMA blocks over 8 k-steps of 32 with the
+// weight loads issued before each barrier.  All threads; on return thread
+// 0 holds the output (the other threads 0.0).
+__device__ __forceinline__ float da_forward(const DaLds& D, const MlpScreen& S, const int64_t* z) {
+  const int t = threadIdx.x, wave = t / 64, lane = t & 63, kq = 8 * (lane >> 4);
+  const float* th = S.theta;
+  if (t < kMlpHid) D.h1[t] = (__bf16)silu((float)((double)z[t] * kDaFixInv) + th[S.lo.b1 + t]);
+  da_bf16x8 bw[8];
+  const int n0 = 32 * wave + (lane & 15), n1 = n0 + 16;
+  da_load_block(S.w2t, n0, kq, bw);
+  __syncthreads();  // h1 complete
+  float v = da_mfma_block(D.h1, lane, kq, bw);
+  if (lane < 16) D.h2[n0] = (__bf16)silu(v + th[S.lo.b2 + n0]);
+  da_load_block(S.w2t, n1, kq, bw);
+  v = da_mfma_block(D.h1, lane, kq, bw);
+  if (lane < 16) D.h2[n1] = (__bf16)silu(v + th[S.lo.b2 + n1]);
+  da_load_block(S.w3t, n0, kq, bw);
+  __syncthreads();  // h2 complete
+  float part = 0.0f;
+  v = da_mfma_block(D.h2, lane, kq, bw);
+  if (lane < 16) part += (float)(__bf16)silu(v + th[S.lo.b3 + n0]) * th[S.lo.w4 + n0];
+  da_load_block(S.w3t, n1, kq, bw);
+  v = da_mfma_block(D.h2, lane, kq, bw);
+  if (lane < 16) part += (float)(__bf16)silu(v + th[S.lo.b3 + n1]) * th[S.lo.w4 + n1];
+  for (int o = 8; o > 0; o >>= 1) part += __shfl_xor(part, o, 64);  // lanes 0..15 of the wave
+  if (lane == 0) D.red[wave] = part;
   __syncthreads();
+  float y = 0.0f;
+  if (t == 0) {
+    y = th[S.lo.b4];
+    for (int w = 0; w < 8; ++w) y += D.red[w];
+  }
+  return y;
 }
 
 // Tables, features of the start state vl and its layer-1 sums, and the
@@ -174,16 +169,15 @@ __device__ __attribute__((noinline)) void da_setup(size_t off, int nvt, MlpScree
   __syncthreads();
   if (threadIdx.x < kMlpHid) D.z1[threadIdx.x] += D.z1t[threadIdx.x];
   __syncthreads();
-  da_output(off, nres, nvt, S, 0, 0);
+  const float y = da_forward(D, S, D.z1);
+  if (threadIdx.x == 0) D.y[0] = y;
+  __syncthreads();
 }
 
 // Layer-1 sums of the trial that sets variable jv from vold to vnew (z1t;
-// threads < kMlpHid), then its output into D.y[1].  All threads.
-__device__ __attribute__((noinline)) void da_trial(size_t off, int nvt, MlpScreen S, int jv, double vold,
-                                                   double vnew) {
-  extern __shared__ double smem[];
-  const int nres = S.res.nres;
-  const DaLds D = carve_da_lds(reinterpret_cast<char*>(smem) + off, nres, nvt);
+// threads < kMlpHid) and its output, which thread 0 returns (and keeps in
+// D.y[1]).  All threads.
+__device__ __forceinline__ float da_trial(const DaLds& D, const MlpScreen& S, int jv, double vold, double vnew) {
   const int t = threadIdx.x;
   if (t < kMlpHid) {
     int64_t z = D.z1[t];
@@ -197,7 +191,9 @@ __device__ __attribute__((noinline)) void da_trial(size_t off, int nvt, MlpScree
     D.z1t[t] = z;
   }
   __syncthreads();
-  da_output(off, nres, nvt, S, 1, 1);
+  const float y = da_forward(D, S, D.z1t);
+  if (t == 0) D.y[1] = y;
+  return y;
 }
 
 // The trial became the state (threads < kMlpHid; thread 0 the output).

@@ -135,7 +135,8 @@ class Surrogate:
 
     def sample(self, temps, seeds, nsweep, vlvr, **kw):
         """smmp_sample with delayed acceptance screened by this network."""
-        return self.ctx.sample(temps, seeds, nsweep, vlvr, da=True, da_scale=self.scale, **kw)
+        kw.setdefault("da_scale", self.scale)
+        return self.ctx.sample(temps, seeds, nsweep, vlvr, da=True, **kw)
 
 
 def move_pairs(ctx, vlvr, seed: int = 0):
