@@ -134,6 +134,37 @@ int smmp_zimmer(smmp_ctx *ctx, int nresi);
  * (0: all) from the current coordinates ('' = stdout). */
 int smmp_outpdb(smmp_ctx *ctx, int nml, const char *path);
 
+/* gradient.f:13-70 with opeshe.f:12-388 (called by minim.f; smmp.pyf
+ * exports the COMMON it fills, eny_r.gdeyvr at smmp.pyf:260): the energy of
+ * the context's variables within each chain (eysm, eyel, eyvw, eyhb, eyvr;
+ * eysl = 0) and its derivatives by every variable into eny_r.gdeyvr, on the
+ * device in FP64.  Vacuum only: itysol > 0 is refused as the reference stops
+ * there (gradient.f:59-62); the analytic solvent's derivatives (opesol.f)
+ * are not provided (itysol < 0: SMMP_ESTATE). */
+int smmp_gradient(smmp_ctx *ctx);
+
+/* minim(imin,maxit,eps) smmp.pyf:1234, minim.f:13: local minimisation of
+ * the energy by the free variables, imin 1 quasi-Newton (minqsn.f), 2
+ * conjugate gradients (mincjg.f), at most maxit function calls, eps the
+ * precision; derivatives from smmp_gradient (vacuum, ireg = 0).  Leaves the
+ * best variables (wrapped into [-pi, pi]) in var_r.vlvr and the energies of
+ * energy() at the last evaluated point in eny_r, like minim.f.  nfun (may
+ * be NULL): function calls used (maxit: the step limit was reached). */
+int smmp_minim(smmp_ctx *ctx, int imin, int maxit, double eps, int32_t *nfun);
+
+/* minqsn.f:12 and mincjg.f:12 as library routines for any function:
+ * fg(user, x, &f, g) evaluates f and its gradient at x (the reference's
+ * MOVE).  x, f, g: the start on entry, the result on return (minqsn: the
+ * best point; mincjg: as mincjg.f leaves them); scal (minqsn): per
+ * variable scale; acur: precision; maxfun: most calls; nfun: calls used;
+ * ier (mincjg, may be NULL): 0 normal end, 1 call limit, 2 step too small,
+ * 3 uphill direction, 4 no reduction. */
+typedef void (*smmp_fg_fn)(void *user, const double *x, double *f, double *g);
+int smmp_minqsn(int n, double *x, double *f, double *g, const double *scal, double acur, int maxfun,
+                smmp_fg_fn fg, void *user, int32_t *nfun);
+int smmp_mincjg(int n, double *x, double *f, double *g, double acur, int maxfun, smmp_fg_fn fg, void *user,
+                int32_t *nfun, int32_t *ier);
+
 /* ---- batched entry points (no reference counterpart: the configs'
  *      many-replica and many-conformation workloads) ------------------- */
 
@@ -146,6 +177,18 @@ int smmp_outpdb(smmp_ctx *ctx, int nml, const char *path);
  * out[c*8 + k] = total, eysm, eyel, eyvw, eyhb, eyvr, eysl, asa. */
 int smmp_energy_batch(smmp_ctx *const *ctxs, int nsys, const int32_t *conf_sys, int nconf,
                       const double *vlvr, int precision, double *out);
+/* gradient.f (vacuum) for nconf full variable vectors of the context's
+ * system, vlvr[nconf*nvt], in one launch: energies[5c..] = eysm, eyel, eyvw,
+ * eyhb, eyvr; grad[nvt c..] = derivatives by every variable. */
+int smmp_gradient_batch(smmp_ctx *ctx, int nconf, const double *vlvr, double *energies, double *grad);
+/* minim.f for nconf conformations side by side: each takes exactly the
+ * steps smmp_minim would take alone from that start, and every round of
+ * function calls is one device launch over the conformations still
+ * running.  vlvr[nconf*nvt] in and out (the best points, free variables
+ * wrapped into [-pi, pi]); fmin[c]: energy of gradient.f there; nfun[c]:
+ * calls used (either may be NULL). */
+int smmp_minim_batch(smmp_ctx *ctx, int imin, int nconf, double *vlvr, int maxit, double eps, double *fmin,
+                     int32_t *nfun);
 /* The same energies as smmp_energy_batch (identical results) from an entry
  * point that allocates no device memory.  No counterpart in the reference:
  * it is for callers that cannot allocate (code under HIP graph capture,

@@ -24,6 +24,14 @@
 !          energies of the start and of random conformations, and the
 !          twister position after each (esolan.f draws on degenerate geometry);
 !          radfile ('-' or 'index radians' lines) sets start variables
+!   grad   libdir seqfile varfile itysol nconf seed epsd
+!          gradient() (gradient.f, opeshe.f) at the start and at nconf
+!          random conformations, epsd 0/1 the distance-dependent dielectric
+!   minim  libdir seqfile varfile itysol imin maxit eps nconf seed
+!          the minimisation of minim(imin, maxit, eps) (minim.f:35-160 with the
+!          reference's minqsn.f / mincjg.f and move) from the start and from
+!          nconf random conformations: start, final variables, final
+!          energies, function calls
 !   multi  libdir seqfile itysol gbfile boxsize seed nconf temp nequi nsweep
 !          several chains: gbfile holds 'molecule k value' lines for gbpr;
 !          topology, energies of random conformations with the
@@ -68,6 +76,8 @@ program ref_driver
   integer :: ncalls, nacalls, kk
   common/updstats/ncalls(5),nacalls(5)
   character(len=256) :: wfile
+  integer :: iepsd, nvta, imin, maxit
+  double precision :: epsmin
 
   call get_command_argument(1, mode)
   grpn = 'nh2 '
@@ -226,6 +236,57 @@ program ref_driver
     end do
     write(*,'(a,i0)') '@mti int 1'
     write(*,'(i0)') mtstate_pos()
+
+  case ('grad')
+    call setup(2, .true.)
+    call iarg(6, nconf)
+    call iarg(7, iseed)
+    call iarg(8, iepsd)
+    epsd = iepsd .ne. 0
+    call sgrnd(iseed)
+    nvta = ivrml1(ntlml) + nvrml(ntlml) - 1
+    do k = 0, nconf
+      if (k .gt. 0) then
+        do i = 1, nvr
+          j = idvr(i)
+          vlvr(j) = addang(pi, axvr(j)*(grnd()-0.5))
+        end do
+      end if
+      write(*,'(a,i0)') '@vlvr double ', nvta
+      write(*,'(es25.17)') (vlvr(i), i=1,nvta)
+      call gradient()
+      write(*,'(a,i0)') '@grad_energy double 5'
+      write(*,'(es25.17)') eysm, eyel, eyvw, eyhb, eyvr
+      write(*,'(a,i0)') '@gdeyvr double ', nvta
+      write(*,'(es25.17)') (gdeyvr(i), i=1,nvta)
+    end do
+
+  case ('minim')
+    call setup(2, .true.)
+    call iarg(6, imin)
+    call iarg(7, maxit)
+    call rarg(8, epsmin)
+    call iarg(9, nconf)
+    call iarg(10, iseed)
+    call sgrnd(iseed)
+    nvta = ivrml1(ntlml) + nvrml(ntlml) - 1
+    do k = 0, nconf
+      if (k .gt. 0) then
+        do i = 1, nvr
+          j = idvr(i)
+          vlvr(j) = addang(pi, axvr(j)*(grnd()-0.5))
+        end do
+      end if
+      write(*,'(a,i0)') '@vlvr double ', nvta
+      write(*,'(es25.17)') (vlvr(i), i=1,nvta)
+      call drv_minim(imin, maxit, epsmin, n)
+      write(*,'(a,i0)') '@min_vlvr double ', nvta
+      write(*,'(es25.17)') (vlvr(i), i=1,nvta)
+      write(*,'(a,i0)') '@min_energy double 6'
+      write(*,'(es25.17)') eysm, eyel, eyvw, eyhb, eyvr, eysl
+      write(*,'(a,i0)') '@min_nfun int 1'
+      write(*,'(i0)') n
+    end do
 
   case ('metro')
     call setup(2, .true.)
@@ -433,6 +494,62 @@ contains
     character(len=256) :: s
     call get_command_argument(pos, s)
     read(s, *) val
+  end subroutine
+
+  ! minim.f:35-160 for ireg = 0 without its log output: minim writes its
+  ! headings with formats of several records into one internal record,
+  ! which stops under this compiler (minim.f:68), so the driver does its
+  ! steps (gradient at the start, the free variables with unit scales, the
+  ! reference's own minqsn or mincjg, which call the reference's move, then
+  ! energy() and the variables wrapped into [-pi, pi]).  nop: function calls.
+  subroutine drv_minim(imin, maxit, acc, nop)
+    integer, intent(in) :: imin, maxit
+    double precision, intent(in) :: acc
+    integer, intent(out) :: nop
+    double precision :: vlvrn(mxvr), gdvr(mxvr), scl(mxvr), esm, vr, energy
+    double precision, allocatable :: w(:)
+    integer :: i, nv, n, n1, n2, n3, n4, n5, n6, ntlvr
+    allocate(w(mxvr*(mxvr+13)/2))
+    call gradient()
+    ntlvr = ivrml1(ntlml) + nvrml(ntlml) - 1
+    nv = 0
+    do i = 1, ntlvr
+      if (.not. fxvr(i)) then
+        nv = nv + 1
+        vlvrn(nv) = vlvr(i)
+        scl(nv) = 1.d0
+        gdvr(nv) = gdeyvr(i)
+      end if
+    end do
+    n = nv
+    esm = eysm
+    if (imin .eq. 1) then
+      n1 = 1 + (n*(n+1))/2
+      n2 = n1 + n
+      n3 = n2 + n
+      n4 = n3 + n
+      n5 = n4 + n
+      n6 = n5 + n
+      call minqsn(n, mxvr, vlvrn, esm, gdvr, scl, acc, w, w(n1), w(n2), w(n3), w(n4), w(n5), w(n6), maxit, nop)
+    else
+      n1 = 1 + n
+      n2 = n1 + n
+      n3 = n2 + n
+      n4 = n3 + n
+      n5 = n4 + n
+      call mincjg(n, mxvr, vlvrn, esm, gdvr, acc, w, w(n1), w(n2), w(n3), w(n4), w(n5), maxit, nop)
+    end if
+    eysm = energy()
+    nv = 0
+    do i = 1, ntlvr
+      if (.not. fxvr(i)) then
+        nv = nv + 1
+        vr = vlvrn(nv)
+        if (abs(vr) .gt. pi) vr = vr - sign(pi2, vr)
+        vlvr(i) = vr
+      end if
+    end do
+    deallocate(w)
   end subroutine
 
   ! ECEPP/3 set-up as universe.py does it, then init_molecule(1,'nh2','cooh').

@@ -462,6 +462,150 @@ void oracle_set_solvent_mt(oracle_mt *g) { g_sol_mt = g; }
 
 double oracle_esolan(const oracle_topo *t, oracle_conf *c, oracle_mt *g, double *as_out);
 
+/* ------------------------------------------------ gradient.f, opeshe.f */
+
+/* One pair of opeshe.f:150-262: adds its terms to el, vw, hb and returns
+ * c = (dE/dr) / r, so that c (r_j - r_i) is the force on atom i. */
+static double pair_force(const oracle_topo *t, double cqi, double qj, double rij2, double A, double Cc, int hb,
+                         double *el, double *vw, double *ehb) {
+  double rij4 = rij2 * rij2, rij6 = rij4 * rij2, rij = sqrt(rij2);
+  double eel, deel;
+  if (t->epsd) { /* opeshe.f:168-176: d(q/(r eps(r)))/dr = -deel/r */
+    double sr = t->slp * rij, sr2 = sr * sr;
+    double xsr = (t->plt - 1.0) * exp(-sr) / 2.0;
+    double ep = t->plt - (sr2 + 2.0 * sr + 2.0) * xsr;
+    eel = cqi * qj / (rij * ep);
+    deel = eel + cqi * qj * (t->slp * sr2 * xsr) / (ep * ep);
+  } else {
+    eel = cqi * qj / rij;
+    deel = eel;
+  }
+  *el = *el + eel;
+  double rp = A / (rij6 * rij6), ds;
+  if (hb) { /* 10-12 hydrogen bond */
+    ds = Cc / (rij6 * rij4);
+    *ehb = *ehb + rp - ds;
+    return (-12.0 * rp + 10.0 * ds - deel) / rij2;
+  }
+  ds = Cc / rij6;
+  *vw = *vw + rp - ds;
+  return (-12.0 * rp + 6.0 * ds - deel) / rij2;
+}
+
+/* opeshe.f:12-388 over every chain at once (the moving sets of all chains
+ * follow each other, as enyshe.f's loop above uses them): variables in
+ * descending processing order; the forces on the atoms of a variable's own
+ * moving set are completed by their partners' earlier subtractions, and the
+ * sums of force and torque of the variables added to its set come from
+ * their own rows (the recursion of Abe, Braun, Noguti and Go 1984). */
+double oracle_gradient(const oracle_topo *t, oracle_conf *c, const double *vlvr, double *gd) {
+  if (t->itysol != 0) return NAN;
+  oracle_setvar(t, c, vlvr);
+  const int nat = t->nat, nvt = t->nvt;
+  double *fat = calloc(3 * (size_t)nat, sizeof(double));
+  double *fvr = calloc(6 * (size_t)nvt, sizeof(double)); /* force and torque sums per variable */
+  if (!fat || !fvr) {
+    free(fat);
+    free(fvr);
+    return NAN;
+  }
+  double eyel = 0.0, eyvw = 0.0, eyhb = 0.0, eyvr = 0.0;
+  int i1s = t->nms + 1, i1a = t->nad + 1;
+  for (int io = nvt; io >= 1; --io) {
+    const int iv = A(t->iorvr, io), ia = A(t->iatvr, iv), ib = A(t->iowat, ia);
+    const int it = A(t->ityvr, iv), ic = A(t->iclvr, iv);
+    const double xb = A(c->xat, ib), yb = A(c->yat, ib), zb = A(c->zat, ib);
+    double ex = 0.0, ey = 0.0, ez = 0.0, fv = 0.0;
+    if (it == 3) { /* torsion: axis along the bond to the base */
+      ex = A(c->xtoat, ib); ey = A(c->ytoat, ib); ez = A(c->ztoat, ib);
+      const double e0 = A(t->e0to, ic);
+      if (e0 != 0.0) {
+        const double vrn = A(c->toat, ia) * A(t->rnto, ic);
+        eyvr = eyvr + e0 * (1.0 + A(t->sgto, ic) * cos(vrn));
+        fv = e0 * A(t->sgto, ic) * A(t->rnto, ic) * sin(vrn); /* esnto (init_energy.f:309) */
+      }
+    } else if (it == 2) { /* bond angle */
+      ex = A(c->xbaat, ia); ey = A(c->ybaat, ia); ez = A(c->zbaat, ia);
+    } else if (it == 1) { /* bond length */
+      ex = A(c->xtoat, ia); ey = A(c->ytoat, ia); ez = A(c->ztoat, ia);
+    }
+    double F[3] = {0.0, 0.0, 0.0}, T[3] = {0.0, 0.0, 0.0};
+    const int i2s = i1s - 1;
+    i1s = A(t->imsvr1, iv);
+    for (int ims = i1s; ims <= i2s; ++ims) {
+      for (int i = A(t->latms1, ims); i <= A(t->latms2, ims); ++i) {
+        const int ity = A(t->ityat, i);
+        const double cqi = t->conv * A(t->cgat, i);
+        const double xi = A(c->xat, i), yi = A(c->yat, i), zi = A(c->zat, i);
+        double fi[3] = {fat[3 * (i - 1)], fat[3 * (i - 1) + 1], fat[3 * (i - 1) + 2]};
+        for (int pass = 0; pass < 2; ++pass) { /* vdW domains, then 1-4 partners */
+          const int k1 = pass == 0 ? A(t->ivwat1, i) : A(t->i14at1, i);
+          const int k2 = pass == 0 ? A(t->ivwat2, i) : A(t->i14at2, i);
+          for (int k = k1; k <= k2; ++k) {
+            const int j1 = pass == 0 ? A(t->lvwat1, k) : A(t->l14at, k);
+            const int j2 = pass == 0 ? A(t->lvwat2, k) : A(t->l14at, k);
+            for (int j = j1; j <= j2; ++j) {
+              const int jty = A(t->ityat, j), q = (ity - 1) + 18 * (jty - 1);
+              const double xij = A(c->xat, j) - xi, yij = A(c->yat, j) - yi, zij = A(c->zat, j) - zi;
+              const double rij2 = xij * xij + yij * yij + zij * zij;
+              const int hb = t->ihbty[q] != 0;
+              const double Aq = hb ? t->ahb[q] : (pass == 0 ? t->aij[q] : t->a14[q]);
+              const double Cq = hb ? t->chb[q] : t->cij[q];
+              const double cf = pair_force(t, cqi, A(t->cgat, j), rij2, Aq, Cq, hb, &eyel, &eyvw, &eyhb);
+              const double fjx = cf * xij, fjy = cf * yij, fjz = cf * zij;
+              fi[0] = fi[0] + fjx;
+              fi[1] = fi[1] + fjy;
+              fi[2] = fi[2] + fjz;
+              fat[3 * (j - 1)] = fat[3 * (j - 1)] - fjx;
+              fat[3 * (j - 1) + 1] = fat[3 * (j - 1) + 1] - fjy;
+              fat[3 * (j - 1) + 2] = fat[3 * (j - 1) + 2] - fjz;
+            }
+          }
+        }
+        fat[3 * (i - 1)] = fi[0];
+        fat[3 * (i - 1) + 1] = fi[1];
+        fat[3 * (i - 1) + 2] = fi[2];
+        F[0] = F[0] + fi[0];
+        F[1] = F[1] + fi[1];
+        F[2] = F[2] + fi[2];
+        T[0] = T[0] + (fi[1] * zi - fi[2] * yi); /* f x r */
+        T[1] = T[1] + (fi[2] * xi - fi[0] * zi);
+        T[2] = T[2] + (fi[0] * yi - fi[1] * xi);
+      }
+    }
+    const int i2a = i1a - 1;
+    i1a = A(t->iadvr1, iv);
+    for (int iad = i1a; iad <= i2a; ++iad) {
+      const double *s = fvr + 6 * (size_t)(A(t->ladvr, iad) - 1);
+      for (int d = 0; d < 3; ++d) {
+        F[d] = F[d] + s[d];
+        T[d] = T[d] + s[3 + d];
+      }
+    }
+    double *o = fvr + 6 * (size_t)(iv - 1);
+    for (int d = 0; d < 3; ++d) {
+      o[d] = F[d];
+      o[3 + d] = T[d];
+    }
+    if (it == 3 || it == 2)
+      gd[iv - 1] = (ey * zb - ez * yb) * F[0] + (ez * xb - ex * zb) * F[1] + (ex * yb - ey * xb) * F[2] +
+                   ex * T[0] + ey * T[1] + ez * T[2] - fv;
+    else
+      gd[iv - 1] = -(ex * F[0] + ey * F[1] + ez * F[2]) - fv;
+  }
+  free(fat);
+  free(fvr);
+  const double eysm = eyel + eyvw + eyhb + eyvr;
+  c->e[1] = eysm;
+  c->e[2] = eyel;
+  c->e[3] = eyvw;
+  c->e[4] = eyhb;
+  c->e[5] = eyvr;
+  c->e[6] = 0.0;
+  c->e[0] = eysm;
+  return eysm;
+}
+
 double oracle_energy(const oracle_topo *t, oracle_conf *c, const double *vlvr) {
   oracle_setvar(t, c, vlvr);
   double esm = 0.0;

@@ -37,6 +37,10 @@ typedef struct {
   int mol_a1[10], mol_a2[10], mol_i2[10], mol_i3[10];
   double gbm[60];
   double boxsize;   /* COMMON /bounds/: rigid-move positions stay in [-boxsize, boxsize] when > 0 */
+  /* variables added to each variable's moving set (setmvs.f): iadvr1 (nvt),
+   * ladvr (nad); read by oracle_gradient only */
+  const int32_t *iadvr1, *ladvr;
+  int nad;
 } oracle_topo;
 
 /* Mutable state of one conformation (allocated by the caller). */
@@ -85,6 +89,13 @@ void oracle_set_solvent_mt(oracle_mt *g);
 void oracle_esolan_force_rotation(double uga, double ufi);
 /* eninteract.f:14-86: pairs between chains; fills ei[], returns eysmi. */
 double oracle_eninteract(const oracle_topo *t, oracle_conf *c);
+/* gradient.f:13-70 with opeshe.f:12-388 (vacuum, itysol = 0): applies the
+ * variables, fills e[1..6] (eysm, eyel, eyvw, eyhb, eyvr, eysl = 0) and the
+ * derivatives gd[nvt] of eysm by every variable; the terms between chains
+ * are not part of it (gradient.f sums opeshe over the chains).  Returns
+ * eysm, or NAN (gd untouched) when itysol != 0: the reference stops for
+ * itysol > 0 and its analytic-solvent derivatives are not restated here. */
+double oracle_gradient(const oracle_topo *t, oracle_conf *c, const double *vlvr, double *gd);
 /* energy.f: setvar + enyshe + solvent + eninteract; returns total. */
 double oracle_energy(const oracle_topo *t, oracle_conf *c, const double *vlvr);
 

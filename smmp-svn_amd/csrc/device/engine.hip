@@ -17,6 +17,7 @@
 #include "common.hpp"
 #include "energy.hpp"
 #include "esolan_host.hpp"
+#include "gradient.hpp"
 #include "mlp_host.hpp"
 #include "sampler.hpp"
 
@@ -96,6 +97,58 @@ __global__ void __launch_bounds__(kEnergyThreadsMax) k_energy(const DevSys* __re
         for (int w = 0; w < nw; ++w) t += L.red[8 * w + k];
         work[4 * (size_t)c + k] = t;
       }
+  }
+}
+
+// Energy and derivatives by the variables of one conformation per block
+// (gradient.f in vacuum): eysm, eyel, eyvw, eyhb, eyvr at out + kGradOut c,
+// dE/dv at grad + conf_off[c] (the layout of vlvr).
+constexpr int kGradThreads = 256;
+__global__ void __launch_bounds__(kGradThreads) k_gradient(const DevSys* __restrict__ systems,
+                                                         const int32_t* __restrict__ conf_sys,
+                                                         const int64_t* __restrict__ conf_off,
+                                                         const double* __restrict__ vlvr, double* __restrict__ out,
+                                                         double* __restrict__ grad, double* __restrict__ work,
+                                                         int bits_lds) {
+  extern __shared__ double smem[];
+  const int c = blockIdx.x;
+  const DevSys& S = systems[conf_sys[c]];
+  const double* vl = vlvr + conf_off[c];
+  const int nw = blockDim.x / kWave, wave = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  EnergyLds L = carve_energy_lds(smem, S, nw, bits_lds != 0);
+  Work wk;
+  energy_stage_static(S, L);
+  energy_stage_torsions(S, vl, L);
+  __syncthreads();
+  build_chain(S, L, wk, S.gbpr);
+  double* f = L.fr;  // the frames are dead once the chain is built
+  double el = 0.0, vw = 0.0, hb = 0.0;
+  pair_forces(S, L, f, el, vw, hb, wk);
+  const double tor = torsion_terms(S, vl);
+  __syncthreads();  // forces complete
+  torsion_derivatives(S, vl, L, f, grad + conf_off[c]);
+  double v[5] = {el, vw, hb, tor, wk.pairs};
+#pragma unroll
+  for (int k = 0; k < 5; ++k) v[k] = wave_sum(v[k]);
+  if (lane == 0)
+    for (int k = 0; k < 5; ++k) L.red[8 * wave + k] = v[k];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int w = 0; w < nw; ++w)
+      for (int k = 0; k < 5; ++k) t[k] += L.red[8 * w + k];
+    double* r = out + (size_t)kGradOut * c;
+    r[0] = t[0] + t[1] + t[2] + t[3];  // eysm = eyel + eyvw + eyhb + eyvr (opeshe.f:384)
+    r[1] = t[0];
+    r[2] = t[1];
+    r[3] = t[2];
+    r[4] = t[3];
+    if (work) {  // pair terms, -, atoms built, - (the energy kernel's counters)
+      work[4 * (size_t)c + 0] = t[4];
+      work[4 * (size_t)c + 1] = 0.0;
+      work[4 * (size_t)c + 2] = (double)S.nat;
+      work[4 * (size_t)c + 3] = 0.0;
+    }
   }
 }
 
@@ -250,7 +303,7 @@ static void upload_system(DevSystem& ds, hipStream_t st) {
 
 // Kernel timing with HIP events recorded on the engine's stream around each
 // launch, plus the algorithmic work the kernels report (see Work).
-enum KernelId { kKSweep = 0, kKExchange = 1, kKEnergy = 2, kKEsolan = 3, kNKernels = 4 };
+enum KernelId { kKSweep = 0, kKExchange = 1, kKEnergy = 2, kKEsolan = 3, kKGradient = 4, kNKernels = 5 };
 struct KernelStat {
   double ms = 0.0;
   int64_t launches = 0;
@@ -272,6 +325,7 @@ struct Engine {
   bool valid = false;           // system upload matches the host state
   DevSystem sys;
   DevMem d_sys, d_confsys, d_confoff, d_vlvr, d_out, d_xyz, d_err, d_work;
+  DevMem g_cs, g_off, g_vl, g_out, g_grad;  // gradient batches
   SamplerBuffers smp;           // replica state of the samplers
   EnergyBatch* batch = nullptr;            // kept by energy_batch for repeated calls on the same set
   std::vector<EnergyBatch*> user_batches;  // smmp_batch_create handles running on this engine
@@ -584,6 +638,43 @@ double engine_energy(Engine*& e, State& s) {
   Engine& E = engine_of(e);
   energy_into_state(E, s);
   return s.eysm + s.eysl + s.eysmi;  // energy.f:114
+}
+
+void engine_gradient(Engine*& e, State& s, int nconf, const double* vlvr, double* out_e, double* out_g) {
+  gradient_supported(s);
+  if (nconf <= 0) return;
+  Engine& E = engine_of(e);
+  DevSystem& ds = current_system(E, s);
+  const int nvt = ds.h.nvt;
+  const size_t nv = (size_t)nconf * nvt;
+  int32_t* d_cs = static_cast<int32_t*>(E.g_cs.ensure(sizeof(int32_t) * nconf));
+  int64_t* d_off = static_cast<int64_t*>(E.g_off.ensure(sizeof(int64_t) * nconf));
+  double* d_vl = static_cast<double*>(E.g_vl.ensure(sizeof(double) * nv));
+  double* d_out = static_cast<double*>(E.g_out.ensure(sizeof(double) * kGradOut * nconf));
+  double* d_g = static_cast<double*>(E.g_grad.ensure(sizeof(double) * nv));
+  std::vector<int32_t> cs(nconf, 0);
+  std::vector<int64_t> off(nconf);
+  for (int c = 0; c < nconf; ++c) off[c] = (int64_t)c * nvt;
+  HIPCHK(hipMemcpyAsync(d_cs, cs.data(), sizeof(int32_t) * nconf, hipMemcpyHostToDevice, E.stream));
+  HIPCHK(hipMemcpyAsync(d_off, off.data(), sizeof(int64_t) * nconf, hipMemcpyHostToDevice, E.stream));
+  HIPCHK(hipMemcpyAsync(d_vl, vlvr, sizeof(double) * nv, hipMemcpyHostToDevice, E.stream));
+  const DevSys* one = &ds.d;
+  const LdsPlan plan = plan_energy_lds(&one, 1, kGradThreads);
+  check_lds(plan.bytes);
+  double* d_work = E.prof.on ? static_cast<double*>(E.d_work.ensure(sizeof(double) * 4 * nconf)) : nullptr;
+  HIPCHK(hipFuncSetAttribute((const void*)k_gradient, hipFuncAttributeMaxDynamicSharedMemorySize, (int)plan.bytes));
+  prof_mark(E, kKGradient, true);
+  hipLaunchKernelGGL(k_gradient, dim3(nconf), dim3(kGradThreads), plan.bytes, E.stream,
+                     static_cast<const DevSys*>(E.d_sys.p), d_cs, d_off, d_vl, d_out, d_g, d_work, plan.bits_lds);
+  HIPCHK(hipGetLastError());
+  prof_mark(E, kKGradient, false);
+  if (d_work) {
+    prof_collect(E);
+    prof_add_work(E, kKGradient, d_work, nconf);
+  }
+  HIPCHK(hipMemcpyAsync(out_e, d_out, sizeof(double) * kGradOut * nconf, hipMemcpyDeviceToHost, E.stream));
+  HIPCHK(hipMemcpyAsync(out_g, d_g, sizeof(double) * nv, hipMemcpyDeviceToHost, E.stream));
+  HIPCHK(hipStreamSynchronize(E.stream));
 }
 
 double engine_enysol(Engine*& e, State& s) {

@@ -10,6 +10,8 @@
 #include "../../../include/smmp_mi355x.h"
 #include "../device/mlp_common.hpp"
 #include "engine.hpp"
+#include "minim.hpp"
+#include "minimize.hpp"
 #include "state.hpp"
 
 using namespace smmp;
@@ -48,6 +50,7 @@ const ViewDesc kViews[] = {
     VS(mol_i, ntlml, SMMP_I32), V1(mol_i, irsml1, SMMP_I32, mxml, 1), V1(mol_i, irsml2, SMMP_I32, mxml, 1),
     V1(mol_i, ivrml1, SMMP_I32, mxml, 1), V1(mol_i, nvrml, SMMP_I32, mxml, 1), V1(mol_i, imsml1, SMMP_I32, mxml, 1),
     V1(mol_i, nmsml, SMMP_I32, mxml, 1), V2(mol_i, ixrfpt, SMMP_I32, 3, mxml), VS(mol_i, ireg, SMMP_I32),
+    V1(mol_i, iadml1, SMMP_I32, mxml, 1), V1(mol_i, nadml, SMMP_I32, mxml, 1),
     VC(mol_c, nmml, 80, mxml),
     V2(mol_r, gbpr, SMMP_F64, 6, mxml), V2(mol_r, rfpt, SMMP_F64, 3, mxml),
     V1(res_i, iatrs1, SMMP_I32, mxrs, 1), V1(res_i, iatrs2, SMMP_I32, mxrs, 1), V1(res_i, ixatrs, SMMP_I32, mxrs, 1),
@@ -81,6 +84,7 @@ const ViewDesc kViews[] = {
     V1(eny_i, n14ml, SMMP_I32, mxml, 1), V1(eny_i, i14ml1, SMMP_I32, mxml, 1),
     VS(eny_r, eyel, SMMP_F64), VS(eny_r, eyvw, SMMP_F64), VS(eny_r, eyhb, SMMP_F64), VS(eny_r, eyvr, SMMP_F64),
     VS(eny_r, eysm, SMMP_F64), VS(eny_r, eysl, SMMP_F64),
+    V1(eny_r, gdeyvr, SMMP_F64, mxvr, 1), V1(eny_r, gdeysl, SMMP_F64, mxvr, 1),
     VS(eny_r, eysmi, SMMP_F64), VS(eny_r, eyeli, SMMP_F64), VS(eny_r, eyvwi, SMMP_F64), VS(eny_r, eyhbi, SMMP_F64),
     VS(bounds, boxsize, SMMP_F64),
     V1(solvent, rvdw, SMMP_F64, mxat, 1), V1(solvent, sigma, SMMP_F64, mxat, 1),
@@ -308,6 +312,93 @@ int smmp_rgyr(smmp_ctx* c, int nml, double* rgy, double* ee) {
     rgyr(c->s, nml, rgy, ee);
     return SMMP_OK;
   });
+}
+
+int smmp_gradient(smmp_ctx* c) {
+  return guarded(c, SMMP_ESTATE, [&] {
+    if (c->s.ntlml < 1) throw Error("gradient> no molecule initialised");
+    gradient_supported(c->s);
+    const int nvt = c->s.nvars();
+    double e[kGradEnergies];
+    std::vector<double> g(nvt);
+    engine_gradient(c->eng, c->s, 1, c->s.vlvr.data(), e, g.data());
+    c->s.eysm = e[0];
+    c->s.eyel = e[1];
+    c->s.eyvw = e[2];
+    c->s.eyhb = e[3];
+    c->s.eyvr = e[4];
+    c->s.eysl = 0.0;
+    for (int k = 0; k < nvt; ++k) c->s.gdeyvr(k + 1) = g[k];
+    for (int m = 1; m <= c->s.ntlml; ++m) setvar(c->s, m, c->s.vlvr.data());  // gradient.f:33
+    return SMMP_OK;
+  });
+}
+
+int smmp_gradient_batch(smmp_ctx* c, int nconf, const double* vlvr, double* energies, double* grad) {
+  if (nconf < 0 || (nconf > 0 && (!vlvr || !energies || !grad))) return fail(c, SMMP_EINVAL, "gradient_batch> bad arguments");
+  return guarded(c, SMMP_ESTATE, [&] {
+    if (c->s.ntlml < 1) throw Error("gradient_batch> no molecule initialised");
+    engine_gradient(c->eng, c->s, nconf, vlvr, energies, grad);
+    return SMMP_OK;
+  });
+}
+
+int smmp_minim(smmp_ctx* c, int imin, int maxit, double eps, int32_t* nfun) {
+  return guarded(c, SMMP_ESTATE, [&] {
+    const int k = minim(c->eng, c->s, imin, maxit, eps);
+    if (nfun) *nfun = k;
+    return SMMP_OK;
+  });
+}
+
+int smmp_minim_batch(smmp_ctx* c, int imin, int nconf, double* vlvr, int maxit, double eps, double* fmin,
+                     int32_t* nfun) {
+  if (nconf < 0 || (nconf > 0 && !vlvr)) return fail(c, SMMP_EINVAL, "minim_batch> bad arguments");
+  return guarded(c, SMMP_ESTATE, [&] {
+    minim_batch(c->eng, c->s, imin, nconf, vlvr, maxit, eps, fmin, nfun);
+    return SMMP_OK;
+  });
+}
+
+int smmp_minqsn(int n, double* x, double* f, double* g, const double* scal, double acur, int maxfun, smmp_fg_fn fg,
+                void* user, int32_t* nfun) {
+  if (n < 1 || !x || !f || !g || !scal || !fg) return SMMP_EINVAL;
+  QuasiNewton m;
+  m.start(n, x, *f, g, scal, acur, maxfun);
+  std::vector<double> gg(n);
+  double ff = 0.0;
+  while (m.pending()) {
+    fg(user, m.point(), &ff, gg.data());
+    m.supply(ff, gg.data());
+  }
+  for (int i = 0; i < n; ++i) {
+    x[i] = m.x()[i];
+    g[i] = m.g()[i];
+  }
+  *f = m.f();
+  if (nfun) *nfun = m.nfun();
+  return SMMP_OK;
+}
+
+int smmp_mincjg(int n, double* x, double* f, double* g, double acur, int maxfun, smmp_fg_fn fg, void* user,
+                int32_t* nfun, int32_t* ier) {
+  if (n < 1 || !x || !f || !g || !fg) return SMMP_EINVAL;
+  ConjugateGradient m;
+  m.start(n, x, *f, g, acur, maxfun);
+  std::vector<double> gg(n);
+  double ff = 0.0;
+  while (m.pending()) {
+    fg(user, m.point(), &ff, gg.data());
+    m.supply(ff, gg.data());
+  }
+  for (int i = 0; i < n; ++i) {
+    x[i] = m.x()[i];
+    g[i] = m.g()[i];
+  }
+  *f = m.f();
+  if (nfun) *nfun = m.nfun();
+  if (ier) *ier = m.ier();
+  return SMMP_OK;
 }
 
 int smmp_energy_batch(smmp_ctx* const* ctxs, int nsys, const int32_t* conf_sys, int nconf, const double* vlvr,

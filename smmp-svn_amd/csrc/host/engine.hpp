@@ -26,6 +26,18 @@ void engine_invalidate(Engine* e);  // topology changed: rebuild device data on 
 
 // energy.f:12 for the context's current vlvr; writes eyel.. and coordinates
 double engine_energy(Engine*& e, State& s);
+// gradient.f:13-70 in vacuum for nconf full variable vectors (nvt each):
+// out_e[5 c ..] = eysm, eyel, eyvw, eyhb, eyvr; out_g[nvt c ..] = dE/dv
+void engine_gradient(Engine*& e, State& s, int nconf, const double* vlvr, double* out_e, double* out_g);
+// Throws unless the context's model has derivatives here: vacuum (itysol 0;
+// the reference stops for itysol > 0, gradient.f:59-62; the analytic
+// solvent's derivatives of opesol.f are not provided) and ECEPP (not FLEX).
+inline void gradient_supported(const State& s) {
+  if (s.itysol > 0) throw Error("gradient> Set itysol < 0");
+  if (s.itysol < 0)
+    throw Error("gradient> derivatives of the analytic solvent (opesol.f) are not provided: set itysol to 0");
+  if (s.flex) throw Error("gradient> the FLEX force field (opeflx.f) is not provided");
+}
 // enysol.f:13 for the current coordinates
 double engine_enysol(Engine*& e, State& s);
 // metropolis.f:13 (context MT stream)

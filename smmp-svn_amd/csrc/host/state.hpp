@@ -123,6 +123,7 @@ struct State {
   // /eny_r/ (energies of the last evaluation)
   double eyel = 0, eyvw = 0, eyhb = 0, eyvr = 0, eysm = 0, eysl = 0, eyrg = 0, eyslh = 0, eyslp = 0, eyab = 0;
   double eysmi = 0, eyeli = 0, eyvwi = 0, eyhbi = 0;  // between molecules (eninteract.f)
+  F1<double, mxvr> gdeyvr, gdeysl;                    // derivatives by the variables (gradient.f)
   double boxsize = 0;                                 // COMMON /bounds/ (metropolis.f:100-106)
   // /solvent/, /tes/, /pnt/, /sol/, /v_sol/, /ressurf/
   F1<double, mxat> rvdw, sigma;

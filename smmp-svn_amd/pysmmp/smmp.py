@@ -15,8 +15,8 @@ Differences from the f2py module (each stated where it applies):
     GPU energy per trial);
   * ``partem_s`` is exported (partem_s.f:13 has no f2py entry);
   * observables rgyr, helix, hbond, interhbond, zimmer, outpdb and the
-    canon driver are provided; contacts, rmsdfun, minim, regul and anneal
-    raise NotImplementedError.
+    canon driver are provided, and minim / gradient in vacuum; contacts,
+    rmsdfun, regul and anneal raise NotImplementedError.
 """
 from __future__ import annotations
 
@@ -387,7 +387,22 @@ def canon(nequi: int, nswp: int, nmes: int, temp: float, lrand: bool = True) -> 
     outpdb(0, "final.pdb")
 
 
-for _n in ("contacts", "minim", "rmsdfun", "anneal", "regul", "gradient", "esolan", "eninteract"):
+def minim(imin: int, maxit: int, eps: float) -> None:
+    """smmp.pyf:1234, minim.f:13: local minimisation (imin 1 quasi-Newton,
+    2 conjugate gradients) of the ECEPP/3 energy by the free variables, at
+    most maxit function calls; the derivatives come from the GPU in vacuum
+    (itysol 0; the reference stops for itysol > 0, the analytic solvent's
+    derivatives are not provided) and without regularisation (ireg 0)."""
+    _ctx.minim(int(imin), int(maxit), float(eps))
+
+
+def gradient() -> None:
+    """gradient.f:13 (vacuum): eny_r.eysm.. and eny_r.gdeyvr of the current
+    variables."""
+    _ctx.gradient()
+
+
+for _n in ("contacts", "rmsdfun", "anneal", "regul", "esolan", "eninteract"):
     globals()[_n] = _outside(_n)
 
 

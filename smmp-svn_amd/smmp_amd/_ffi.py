@@ -44,6 +44,10 @@ class SamplerCfg(C.Structure):
                 ("obs", C.POINTER(C.c_double))]
 
 
+#: fg(user, x, &f, g) of smmp_minqsn / smmp_mincjg
+FG_FN = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double))
+
+
 def _load():
     if not os.path.exists(LIBPATH):
         raise ImportError(f"libsmmp_mi355x.so not built ({LIBPATH}); run __graft_entry__.build()")
@@ -76,6 +80,12 @@ def _load():
         "smmp_interhbond": (C.c_int, [P, ip]),
         "smmp_zimmer": (C.c_int, [P, C.c_int]),
         "smmp_outpdb": (C.c_int, [P, C.c_int, C.c_char_p]),
+        "smmp_gradient": (C.c_int, [P]),
+        "smmp_gradient_batch": (C.c_int, [P, C.c_int, dp, dp, dp]),
+        "smmp_minim": (C.c_int, [P, C.c_int, C.c_int, C.c_double, ip]),
+        "smmp_minim_batch": (C.c_int, [P, C.c_int, C.c_int, dp, C.c_int, C.c_double, dp, ip]),
+        "smmp_minqsn": (C.c_int, [C.c_int, dp, dp, dp, dp, C.c_double, C.c_int, FG_FN, C.c_void_p, ip]),
+        "smmp_mincjg": (C.c_int, [C.c_int, dp, dp, dp, C.c_double, C.c_int, FG_FN, C.c_void_p, ip, ip]),
         "smmp_energy_batch": (C.c_int, [C.POINTER(P), C.c_int, ip, C.c_int, dp, C.c_int, dp]),
         "smmp_energy_batch_noalloc": (C.c_int, [C.POINTER(P), C.c_int, ip, C.c_int, dp, C.c_int, dp]),
         "smmp_batch_create": (C.c_int, [C.POINTER(P), C.c_int, ip, C.c_int, C.POINTER(P), C.POINTER(C.c_int64)]),
@@ -112,12 +122,13 @@ EXPORTS = ("smmp_ctx_create", "smmp_ctx_destroy", "smmp_last_error", "smmp_state
            "smmp_state_name", "smmp_init_energy", "smmp_init_molecule", "smmp_setvar", "smmp_energy",
            "smmp_enysol", "smmp_sgrnd", "smmp_grnd", "smmp_outvar", "smmp_metropolis", "smmp_partem_s", "smmp_rgyr", "smmp_redvar", "smmp_helix",
            "smmp_hbond", "smmp_interhbond", "smmp_zimmer", "smmp_outpdb",
+           "smmp_gradient", "smmp_gradient_batch", "smmp_minim", "smmp_minim_batch", "smmp_minqsn", "smmp_mincjg",
            "smmp_energy_batch", "smmp_energy_batch_noalloc", "smmp_batch_create", "smmp_batch_load", "smmp_batch_run", "smmp_batch_read",
            "smmp_batch_destroy", "smmp_sample", "smmp_build_info", "smmp_profile", "smmp_profile_read",
            "smmp_profile_phases", "smmp_muca_update", "smmp_muca_final", "smmp_sampler_info",
            "smmp_mlp_size", "smmp_mlp_layout", "smmp_mlp_set", "smmp_mlp_get", "smmp_mlp_grad", "smmp_mlp_fit")
 
-KERNEL_SWEEP, KERNEL_EXCHANGE, KERNEL_ENERGY, KERNEL_ESOLAN = 0, 1, 2, 3
+KERNEL_SWEEP, KERNEL_EXCHANGE, KERNEL_ENERGY, KERNEL_ESOLAN, KERNEL_GRADIENT = 0, 1, 2, 3, 4
 
 
 def dptr(a: np.ndarray):
@@ -348,6 +359,38 @@ class Context:
         self.check(lib.smmp_zimmer(self.h, int(nresi)))
         return self.view("zimme.zimm").ravel(order="F")[:nresi].tobytes().decode()
 
+    def gradient(self) -> tuple[float, np.ndarray]:
+        """gradient.f (vacuum): eysm and the derivatives by all variables
+        (also left in eny_r)."""
+        self.check(lib.smmp_gradient(self.h))
+        return float(self.view("eny_r.eysm")[0]), np.array(self.view("eny_r.gdeyvr")[:self.nvt])
+
+    def gradient_batch(self, vlvr) -> tuple[np.ndarray, np.ndarray]:
+        """Energies (eysm, eyel, eyvw, eyhb, eyvr) and derivatives of many
+        full variable vectors (rows of vlvr) in one launch."""
+        vl = np.ascontiguousarray(vlvr, dtype=np.float64).reshape(-1, self.nvt)
+        n = len(vl)
+        e = np.zeros((n, 5))
+        g = np.zeros((n, self.nvt))
+        self.check(lib.smmp_gradient_batch(self.h, n, dptr(vl), dptr(e), dptr(g)))
+        return e, g
+
+    def minim(self, imin: int, maxit: int, eps: float) -> int:
+        """minim.f: minimises from the current variables; returns the calls used."""
+        k = C.c_int32(0)
+        self.check(lib.smmp_minim(self.h, int(imin), int(maxit), float(eps), C.byref(k)))
+        return int(k.value)
+
+    def minim_batch(self, imin: int, vlvr, maxit: int, eps: float):
+        """minim.f for many starts side by side: (best variables, their
+        energies, calls used)."""
+        vl = np.ascontiguousarray(vlvr, dtype=np.float64).reshape(-1, self.nvt).copy()
+        n = len(vl)
+        f = np.zeros(n)
+        k = np.zeros(n, dtype=np.int32)
+        self.check(lib.smmp_minim_batch(self.h, int(imin), n, dptr(vl), int(maxit), float(eps), dptr(f), iptr(k)))
+        return vl, f, k
+
     def outpdb(self, nml: int, path: str = "") -> None:
         self.check(lib.smmp_outpdb(self.h, int(nml), path.encode()))
 
@@ -493,3 +536,50 @@ class EnergyBatch:
             self.close()
         except Exception:
             pass
+
+
+def _minimise(kind: int, fg, x, scal, acur: float, maxfun: int):
+    x = np.ascontiguousarray(x, dtype=np.float64).copy()
+    n = len(x)
+    f0, g0 = fg(x.copy())
+    g = np.ascontiguousarray(g0, dtype=np.float64).copy()
+    f = C.c_double(float(f0))
+    err = []
+
+    def cb(_user, xp, fp, gp):
+        try:
+            fv, gv = fg(np.ctypeslib.as_array(xp, (n,)).copy())
+            fp[0] = float(fv)
+            gv = np.asarray(gv, dtype=np.float64)
+            for i in range(n):
+                gp[i] = float(gv[i])
+        except Exception as e:  # noqa: BLE001  (reported after the call)
+            err.append(e)
+            fp[0] = float("nan")
+
+    cfun = FG_FN(cb)
+    k = C.c_int32(0)
+    ier = C.c_int32(0)
+    if kind == 1:
+        s = np.ascontiguousarray(np.ones(n) if scal is None else scal, dtype=np.float64)
+        rc = lib.smmp_minqsn(n, dptr(x), C.byref(f), dptr(g), dptr(s), float(acur), int(maxfun), cfun, None,
+                             C.byref(k))
+    else:
+        rc = lib.smmp_mincjg(n, dptr(x), C.byref(f), dptr(g), float(acur), int(maxfun), cfun, None, C.byref(k),
+                             C.byref(ier))
+    if err:
+        raise err[0]
+    if rc != 0:
+        raise SmmpError(rc, "minimiser arguments")
+    return x, float(f.value), g, int(k.value), int(ier.value)
+
+
+def minqsn(fg, x, scal=None, acur: float = 1e-9, maxfun: int = 1000):
+    """minqsn.f on fg(x) -> (f, gradient): (x, f, g, calls) at the best point."""
+    x, f, g, k, _ = _minimise(1, fg, x, scal, acur, maxfun)
+    return x, f, g, k
+
+
+def mincjg(fg, x, acur: float = 1e-9, maxfun: int = 1000):
+    """mincjg.f on fg(x) -> (f, gradient): (x, f, g, calls, ier)."""
+    return _minimise(2, fg, x, None, acur, maxfun)

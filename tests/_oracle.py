@@ -32,7 +32,8 @@ class Topo(C.Structure):
                 + [(n, C.c_int) for n in ("epsd", "itysol", "i2", "i3")]
                 + [("gbpr", C.c_double * 6), ("nml", C.c_int)]
                 + [(n, C.c_int * 10) for n in ("mol_a1", "mol_a2", "mol_i2", "mol_i3")]
-                + [("gbm", C.c_double * 60), ("boxsize", C.c_double)])
+                + [("gbm", C.c_double * 60), ("boxsize", C.c_double)]
+                + [(n, I32P) for n in ("iadvr1", "ladvr")] + [("nad", C.c_int)])
 
 
 class Conf(C.Structure):
@@ -66,6 +67,8 @@ def _load():
     lib.oracle_setvar.argtypes = [P(Topo), P(Conf), F64P]
     lib.oracle_energy.argtypes = [P(Topo), P(Conf), F64P]
     lib.oracle_energy.restype = C.c_double
+    lib.oracle_gradient.argtypes = [P(Topo), P(Conf), F64P, F64P]
+    lib.oracle_gradient.restype = C.c_double
     lib.oracle_weight_eval.argtypes = [P(Weight), C.c_double]
     lib.oracle_weight_eval.restype = C.c_double
     lib.oracle_metropolis.argtypes = [P(Topo), P(Conf), P(MT), F64P, C.c_int, I32P, F64P, P(Weight), F64P, F64P]
@@ -137,6 +140,7 @@ class OracleSystem:
         nms = int(v("mol_i.imsml1")[last] + v("mol_i.nmsml")[last] - 1)
         nvw = int(v("eny_i.ivwml1")[last] + v("eny_i.nvwml")[last] - 1)
         n14 = int(v("eny_i.i14ml1")[last] + v("eny_i.n14ml")[last] - 1)
+        nad = int(v("mol_i.iadml1")[last] + v("mol_i.nadml")[last] - 1)
         self.nml = nml
         self.nat, self.nvt = nat, nvt
         npnt = int(v("pnt.npnt")[0])
@@ -160,11 +164,13 @@ class OracleSystem:
             "ahb": f64(v("epar_r.ahb").ravel(order="F")), "ihbty": i32(v("epar_i.ihbty").ravel(order="F")),
             "e0to": f64(v("epar_r.e0to")), "sgto": f64(v("epar_r.sgto")), "rnto": f64(v("epar_r.rnto")),
             "spoint": f64(v("tes.spoint")[:max(npnt, 1), :].T.ravel()),
+            "iadvr1": i32(v("var_i.iadvr1")[:nvt]), "ladvr": i32(v("var_i.ladvr")[:max(nad, 1)]),
         }
         if initial_torsions:
             self.keep["toat"] = f64(toat_setup[:nat])
         t = Topo()
         t.nat, t.nvt, t.nms, t.npnt = nat, nvt, nms, npnt
+        t.nad = nad
         for k, a in self.keep.items():
             setattr(t, k, a.ctypes.data_as(I32P if a.dtype == np.int32 else F64P))
         t.conv = float(v("epar_r.conv")[0])
@@ -213,6 +219,15 @@ class OracleSystem:
         finally:
             lib.oracle_set_solvent_mt(None)
         return np.array(self.c.e[:], dtype=np.float64)
+
+    def gradient(self, vlvr_all) -> tuple[np.ndarray, np.ndarray]:
+        """gradient.f in vacuum: ([eysm, eyel, eyvw, eyhb, eyvr], gdeyvr[nvt])."""
+        vl = f64(vlvr_all)
+        gd = np.zeros(self.nvt)
+        e = lib.oracle_gradient(C.byref(self.t), C.byref(self.c), vl.ctypes.data_as(F64P), gd.ctypes.data_as(F64P))
+        if not np.isfinite(e):
+            raise ValueError("oracle gradient: vacuum only (itysol = 0)")
+        return np.array(self.c.e[1:6], dtype=np.float64), gd
 
     def setvar(self, vlvr_all) -> None:
         """Applies the variables and rebuilds (setvar.f), without an energy."""

@@ -150,6 +150,22 @@ def canon_gpu(args):
     return {"files": sorted(os.listdir("."))}
 
 
+def minimization_gpu(args):
+    """EXAMPLES/Python/minimization.py: Universe(st=0), Met-enkephalin from
+    enkefa.var, smmp.minim(1, 10000, 1e-9); the energies and variables after."""
+    import smmp
+    import protein
+    import universe
+    u = universe.Universe(T=300, ff="ecepp3", st=0, libdir=LIBDIR)
+    p = protein.Protein(os.path.join(GOLDEN, "enkefa.seq"), os.path.join(GOLDEN, "enkefa.var"))
+    u.add(p)
+    smmp.minim(args.get("imin", 1), args.get("maxit", 10000), args.get("eps", 1e-9))
+    nvt = int(smmp.mol_i.ivrml1[0] + smmp.mol_i.nvrml[0] - 1)
+    return {"eysm": float(smmp.eny_r.eysm), "eyel": float(smmp.eny_r.eyel), "eyvw": float(smmp.eny_r.eyvw),
+            "eyhb": float(smmp.eny_r.eyhb), "eyvr": float(smmp.eny_r.eyvr), "energy": u.energy(),
+            "vlvr": [float(x) for x in smmp.var_r.vlvr[:nvt]]}
+
+
 def energy_gpu(args):
     import universe
     import protein

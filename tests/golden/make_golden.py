@@ -13,6 +13,7 @@ never on a GPU box.
 from __future__ import annotations
 
 import os
+import re
 import subprocess
 import sys
 import tempfile
@@ -356,6 +357,50 @@ def obs_fixtures(tmp: str) -> None:
              mmhb=inter[:, 1:], zimm=np.stack(out["zimm"]), pdb1=np.array(pdbs[0]), pdb2=np.array(pdbs[1]))
 
 
+def grad_fixtures(tmp: str) -> None:
+    """gradient.f / opeshe.f derivatives and minim.f runs (quasi-Newton
+    minqsn.f and conjugate gradients mincjg.f) in vacuum: Met-enkephalin
+    from enkefa.var (the README's minimisation) and random conformations of
+    the 10-residue sequence."""
+    enk = [s.lower() for s in open(os.path.join(HERE, "enkefa.seq")).read().split()[2:]]
+    t_enk = topo_fixture("enkefa", enk, tmp, nconf=8, seed=11)
+    vf = vars_file(os.path.join(tmp, "enk.vars"), t_enk, os.path.join(HERE, "enkefa.var"), False)
+    seqf = os.path.join(tmp, "enkefa.seq")
+    seq10 = list(np.load(os.path.join(HERE, "topo_len10.npz"))["seq_in"])
+    seqf10 = workloads.write_seq_file(os.path.join(tmp, "grad_len10.seq"), seq10, "len10")
+    out = {}
+    for tag, (sf, varf, nconf, seed, epsd) in {"enkefa": (seqf, vf, 6, 515, 0), "enkefa_epsd": (seqf, vf, 4, 516, 1),
+                                              "len10": (seqf10, "-", 6, 517, 0)}.items():
+        r = run(["grad", LIBDIR, sf, varf, "0", str(nconf), str(seed), str(epsd)], tmp)
+        out[f"{tag}_vlvr"] = np.stack(r["vlvr"])
+        out[f"{tag}_energy"] = np.stack(r["grad_energy"])
+        out[f"{tag}_grad"] = np.stack(r["gdeyvr"])
+    save("grad_vacuum.npz", **out)
+    out = {}
+    for tag, (sf, varf, imin, maxit, eps, nconf, seed) in {
+            "enkefa_qn": (seqf, vf, 1, 10000, 1e-9, 0, 1), "enkefa_cg": (seqf, vf, 2, 10000, 1e-9, 0, 1),
+            "enkefa_cg60": (seqf, vf, 2, 60, 1e-9, 0, 1), "enkefa_qn_rand": (seqf, vf, 1, 25, 1e-9, 3, 520),
+            "len10_qn": (seqf10, "-", 1, 30, 1e-9, 2, 518), "len10_cg": (seqf10, "-", 2, 30, 1e-9, 2, 519)}.items():
+        r = run(["minim", LIBDIR, sf, varf, "0", str(imin), str(maxit), repr(eps), str(nconf), str(seed)], tmp)
+        out[f"{tag}_args"] = np.array([imin, maxit, eps])
+        out[f"{tag}_start"] = np.stack(r["vlvr"])
+        out[f"{tag}_vlvr"] = np.stack(r["min_vlvr"])
+        out[f"{tag}_energy"] = np.stack(r["min_energy"])
+        out[f"{tag}_nfun"] = np.concatenate(r["min_nfun"])
+    save("minim_vacuum.npz", **out)
+    # the README's printed minimisation of Met-enkephalin (minim.f's move
+    # lines): step, energy, sum of squared derivatives
+    readme = "/root/reference/trunk/README"
+    rows = re.findall(r"Step\s+(\d+): energy\s+(\S+)\s+\(\s*(\S+)\s*\)", open(readme).read())
+    path = os.path.join(HERE, "readme_minim_steps.txt")
+    with open(path, "w") as f:
+        f.write("# the reference README's minimisation of Met-enkephalin (enkefa.var, vacuum, minim(1, 10000, 1e-9)):\n"
+                "# step, energy, sum of squared derivatives as printed\n")
+        for k, e, g in rows:
+            f.write(f"{k} {float(e):.6E} {float(g):.6E}\n")
+    print(f"wrote {path} ({len(rows)} steps)")
+
+
 def main(only: list[str]) -> None:
     if not os.path.exists(DRIVER):
         raise SystemExit("build the reference first: make -C oracle ref")
@@ -389,6 +434,8 @@ def main(only: list[str]) -> None:
         obs_fixtures(tmp)
     if not only or "metro" in only:
         metro_fixtures(tmp)
+    if not only or "grad" in only:
+        grad_fixtures(tmp)
     if only and "restart" in only:  # the len10 run of partem_ext_len10.npz again, for its files
         seq = list(np.load(os.path.join(HERE, "topo_len10.npz"))["seq_in"])
         seqf = workloads.write_seq_file(os.path.join(tmp, "pt_len10.seq"), seq, "len10")

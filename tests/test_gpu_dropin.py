@@ -19,6 +19,20 @@ def test_universe_energy_readme_known_answer():
     assert round(out["eyvr"], 4) == 0.4511
 
 
+def test_minimization_example_readme_known_answer():
+    """EXAMPLES/Python/minimization.py through the drop-in API: README's
+    final energies (Total -0.12429E+02, Coulomb 0.2143E+02, Lennard-Jones
+    -0.2923E+02, HB -0.6706E+01, Variables 0.2084E+01) and the Fortran's
+    minimum (tests/golden/minim_vacuum.npz)."""
+    m = golden("minim_vacuum.npz")
+    out = run("minimization_gpu", {})
+    assert float(f"{out['eysm']:.4E}") == -12.429
+    assert float(f"{out['eyel']:.3E}") == 21.43 and float(f"{out['eyvw']:.3E}") == -29.23
+    assert float(f"{out['eyhb']:.3E}") == -6.706 and float(f"{out['eyvr']:.3E}") == 2.084
+    assert abs(out["energy"] - m["enkefa_qn_energy"][0][0]) < 1e-8
+    assert np.abs(np.array(out["vlvr"]) - m["enkefa_qn_vlvr"][0]).max() < 1e-7
+
+
 def test_canonical_monte_carlo_trajectory_vs_reference():
     """algorithms.CanonicalMonteCarlo sweeps = the reference's metropolis
     trajectory (C1: 19 free torsions, 300 K, seed 81236)."""
