@@ -39,6 +39,7 @@ FLOP_PAIR = 22      # ECEPP pair term: 3 sub + 5 r^2 + 1 rsqrt + 3 powers + 2 Co
 FLOP_POINT = 9      # one tessellation point against one solvent neighbour
 FLOP_BUILD = 60     # one atom placed by the Eyring step
 FLOP_ROTATE = 21    # one atom rotated rigidly about a bond (Rodrigues)
+FLOP_PAIR_GRAD = 36  # one pair with its force: the 22 of the energy, dE/dr / r (5), c r_ij (3), both atoms' sums (6)
 # the analytic solvent (esolan.f), per unit of its kernel's work counters:
 FLOP_ES_SCAN = 9    # one atom pair of the neighbour search: 3 sub, 3 mul, 2 add, 1 compare
 FLOP_ES_PAIR = 48   # one pair of circles classified: 6 div, 21 discriminant, 3 sqrt with their 12, 6 tests
@@ -442,6 +443,60 @@ def c3_headline(K: int) -> dict:
                         f"resident batch, {K} timed batches", **out}
 
 
+def minim_headline(K: int, nconf: int = 4096, nmin: int = 1024, maxit: int = 300) -> dict:
+    """Derivatives and local minimisation in vacuum (gradient.f, minim.f)
+    on the C2 sequence (random 20-mer, seed 2): K timed smmp_gradient_batch
+    calls over nconf random conformations (k_gradient's time from its HIP
+    events, with its FP64 roofline: FLOP_PAIR_GRAD per pair counted once
+    and FLOP_BUILD per atom placed), then nmin quasi-Newton minimisations
+    from random starts side by side (smmp_minim_batch, maxit calls each)."""
+    from smmp_amd import _ffi, workloads
+    tmp = tempfile.mkdtemp(prefix="smmp_min_")
+    c, _, _ = make_system(tmp, itysol=0)
+    V = random_starts(c, nconf, 4242)
+    c.gradient_batch(V)
+    c.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(K):
+        c.gradient_batch(V)
+    dt = time.perf_counter() - t0
+    kg = c.profile_read(_ffi.KERNEL_GRADIENT)
+    c.profile(False)
+    wk = kg["work"]
+    flops = FLOP_PAIR_GRAD * wk[0] + FLOP_BUILD * wk[2]
+    ach = flops / (kg["ms"] * 1e-3) / 1e12
+    out = {"gradient_evals_per_s": round(K * nconf / (kg["ms"] * 1e-3), 1),
+           "gradient_evals_per_s_with_transfers": round(K * nconf / dt, 1),
+           "k_gradient_ms_per_launch": round(kg["ms"] / max(kg["launches"], 1), 4),
+           "roofline": {"bound": "valu", "achieved": round(ach, 4), "peak": PEAK_FP64, "unit": "TFLOP/s",
+                        "frac": round(ach / PEAK_FP64, 5), "flop_per_launch": round(flops / max(kg["launches"], 1), 1)}}
+    S = random_starts(c, nmin, 777)
+    e0, _ = c.gradient_batch(S)
+    t0 = time.perf_counter()
+    Vb, fb, nb = c.minim_batch(1, S, maxit, 1e-9)
+    dtm = time.perf_counter() - t0
+    out["minimisation"] = {"starts": nmin, "maxit": maxit, "minimisations_per_s": round(nmin / dtm, 1),
+                           "function_calls_per_s": round(float(nb.sum()) / dtm, 1),
+                           "median_calls": float(np.median(nb)), "converged": int((nb < maxit).sum()),
+                           "median_energy_start": float(np.median(e0[:, 0])), "median_energy_end": float(np.median(fb)),
+                           "wall_s": round(dtm, 2)}
+    c.close()
+    log(f"minimisation headline: {out}")
+    return {"workload": f"{LENGTH}-residue random sequence (seed {SEQ_SEED}), ECEPP/3 in vacuum: {nconf} "
+                        f"conformations per derivative launch, {K} timed launches; {nmin} quasi-Newton "
+                        f"minimisations from random starts", **out}
+
+
+def run_minim(args):
+    line = {"metric": "derivative evaluations/s (gradient.f) and minimisations/s (minim.f)", "n_gpus": 1,
+            "higher_is_better": True, "dtype": "f64", "data": "synthetic: random sequence and dihedrals"}
+    r = minim_headline(max(args.steps, 1))
+    line["value"] = r["gradient_evals_per_s"]
+    line["unit"] = "gradient evals/s"
+    line.update(r)
+    print(json.dumps(line), flush=True)
+
+
 def run_c3(args):
     """Batched energy-only evaluation: 4,096 conformations over 256 random
     sequences of 10-60 residues (16 conformations each, dihedrals uniform in
@@ -726,6 +781,9 @@ def main():
     ap.add_argument("--c3", action="store_true",
                     help="batched energies of 4,096 conformations over 256 sequences of 10-60 residues; "
                          "--steps sets the timed batches")
+    ap.add_argument("--minim", action="store_true",
+                    help="derivatives (gradient.f) of 4,096 conformations per launch and 1,024 batched minimisations "
+                         "(minim.f) of the 20-residue system in vacuum; --steps sets the timed launches")
     ap.add_argument("--muca", action="store_true",
                     help="multicanonical sampling, 128 replicas x 36 residues, 200 bins, mixed precision, graph replay")
     ap.add_argument("--mu-warmup", type=int, default=200, help="parallel-tempering sweeps that set the energy range")
@@ -755,6 +813,9 @@ def main():
         return
     if args.muca:
         run_muca(args)
+        return
+    if args.minim:
+        run_minim(args)
         return
     if args.copy_probe:
         x = torch.ones(args.copy_probe << 18, dtype=torch.float32, device="cuda")
@@ -842,6 +903,7 @@ def main():
         # the other two parts of the metric: energy evaluations/s (C3) and
         # delayed acceptance against the plain sampler (C5)
         line["energy_evals_per_s"] = c3_headline(5)
+        line["minimisation"] = minim_headline(5)
         R5 = 128
         c5, _, _ = make_system(tmp, length=36, seed=5, tag="c5")
         t0 = time.perf_counter()

@@ -41,14 +41,20 @@ __host__ __device__ inline size_t pair_bit_words(int nat, int words) { return (s
 // Per-wave solvent scratch: kMaxNb neighbours x kNbFloats floats.
 constexpr int kNbFloats = 8;  // per neighbour: 2d (3), 1/M, -t/M, j, 2 unused (see solvent_points)
 constexpr int kNbDoubles = kMaxNb * kNbFloats / 2;
-__host__ __device__ inline int energy_region1(int np, int nwaves) {
-  int a = 6 * np, b = nwaves * kNbDoubles;
+// The frames of the chain build (6 per atom), overlaid after the build by
+// the solvent scratch of every wave when the system has a numerical solvent
+// (npnt > 0).
+__host__ __device__ inline int energy_region1(int np, int nwaves, int npnt) {
+  int a = 6 * np, b = npnt > 0 ? nwaves * kNbDoubles : 0;
   return a > b ? a : b;
 }
+// Per-atom solvent radius and sigma: only with a numerical solvent.
+__host__ __device__ inline int energy_solvent_atoms(int np, int npnt) { return npnt > 0 ? np : 0; }
 // nbits: u64 words of pair bits staged in LDS (0: read from HBM).
 __host__ __device__ inline size_t energy_lds_bytes(int np, int npnt, int nwaves, int nheavy, size_t nbits) {
-  size_t d = 6 * (size_t)np + energy_region1(np, nwaves) + 4 * 324 + 3 * (size_t)npnt + 8 * (size_t)nwaves + 8 +
-             nbits + 4 * (size_t)nheavy + ((size_t)nheavy + 1) / 2;
+  size_t d = 4 * (size_t)np + 2 * (size_t)energy_solvent_atoms(np, npnt) + energy_region1(np, nwaves, npnt) +
+             4 * 324 + 3 * (size_t)npnt + 8 * (size_t)nwaves + 8 + nbits + 4 * (size_t)nheavy +
+             ((size_t)nheavy + 1) / 2;
   return d * sizeof(double) + (size_t)np + 2 * (size_t)nheavy + 2 * (size_t)np;
 }
 
@@ -60,13 +66,13 @@ __device__ inline EnergyLds carve_energy_lds(double* base, const DevSys& S, int 
   L.y = p; p += np;
   L.z = p; p += np;
   L.q = p; p += np;
-  L.fr = p; p += energy_region1(np, nwaves);
+  L.fr = p; p += energy_region1(np, nwaves, npnt);
   L.ptab = p; p += 4 * 324;
   L.sp = p; p += 3 * npnt;
   L.red = p; p += 8 * nwaves;
   L.ctr = reinterpret_cast<int*>(p); p += 8;
-  L.rv = p; p += np;
-  L.sg = p; p += np;
+  L.rv = p; p += energy_solvent_atoms(np, npnt);
+  L.sg = p; p += energy_solvent_atoms(np, npnt);
   L.hp = p; p += 4 * S.nheavy;
   L.nfs = reinterpret_cast<int32_t*>(p); p += (S.nheavy + 1) / 2;
   if (bits_lds) {
@@ -123,14 +129,18 @@ __device__ void energy_stage_static(const DevSys& S, EnergyLds& L) {
     if (a < S.nat) {
       L.q[a] = S.q[a];
       L.t[a] = S.type[a];
-      L.rv[a] = S.rvdw[a];
-      L.sg[a] = S.sigma[a];
+      if (S.npnt > 0) {
+        L.rv[a] = S.rvdw[a];
+        L.sg[a] = S.sigma[a];
+      }
       L.hidx[a] = S.heavy_of[a];
     } else {
       L.q[a] = 0.0;
       L.t[a] = 0;
-      L.rv[a] = 0.0;
-      L.sg[a] = 0.0;
+      if (S.npnt > 0) {
+        L.rv[a] = 0.0;
+        L.sg[a] = 0.0;
+      }
       L.hidx[a] = -1;
       L.x[a] = 0.0;
       L.y[a] = 0.0;

@@ -658,8 +658,17 @@ void engine_gradient(Engine*& e, State& s, int nconf, const double* vlvr, double
   HIPCHK(hipMemcpyAsync(d_cs, cs.data(), sizeof(int32_t) * nconf, hipMemcpyHostToDevice, E.stream));
   HIPCHK(hipMemcpyAsync(d_off, off.data(), sizeof(int64_t) * nconf, hipMemcpyHostToDevice, E.stream));
   HIPCHK(hipMemcpyAsync(d_vl, vlvr, sizeof(double) * nv, hipMemcpyHostToDevice, E.stream));
-  const DevSys* one = &ds.d;
-  const LdsPlan plan = plan_energy_lds(&one, 1, kGradThreads);
+  // the pair bits go to LDS only where three workgroups per CU still fit
+  // (the kernel is latency-bound: occupancy first); else they are read
+  // through the caches
+  LdsPlan plan;
+  {
+    const DevSys& d = ds.d;
+    const size_t with = energy_lds_bytes(d.np, d.npnt, kGradThreads / kWave, d.nheavy, pair_bit_words(d.nat, d.words));
+    const size_t without = energy_lds_bytes(d.np, d.npnt, kGradThreads / kWave, d.nheavy, 0);
+    plan.bits_lds = 3 * with <= kLdsBytes ? 1 : 0;
+    plan.bytes = plan.bits_lds ? with : without;
+  }
   check_lds(plan.bytes);
   double* d_work = E.prof.on ? static_cast<double*>(E.d_work.ensure(sizeof(double) * 4 * nconf)) : nullptr;
   HIPCHK(hipFuncSetAttribute((const void*)k_gradient, hipFuncAttributeMaxDynamicSharedMemorySize, (int)plan.bytes));

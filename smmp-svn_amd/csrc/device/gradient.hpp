@@ -1,9 +1,10 @@
 // Energy and its derivatives by the torsions of one conformation in vacuum
 // (gradient.f:13-70 with opeshe.f:12-388), by one workgroup, FP64:
 //   1. per-atom forces f_i = -dE/dr_i over the pairs within each chain:
-//      wave w owns row blocks w, w + nwaves, ... of the pair bit rows and
-//      each lane sums the whole row of its atom (every pair is evaluated
-//      from both ends, so no two lanes write one atom and the sums are
+//      the items (row block of 64 atoms, half of the column words) go to
+//      the waves in turn and each lane sums its atom's part of the row into
+//      one of two partial force arrays (every pair is evaluated from both
+//      ends, so no two lanes write one value and the sums are
 //      deterministic); the pair energies are counted from the lower end;
 //   2. per variable v, the sums over its moved atoms of f and f x r; then
 //      dE/dv = s [(e x b) . F + e . T] - dE_tors/dv, e the unit bond axis,
@@ -91,21 +92,25 @@ __device__ __forceinline__ void row_forces(const DevSys& S, const EnergyLds& L, 
   }
 }
 
-// Forces into f (LDS, 3 per atom, all atoms of the system) and the pair
-// energies of this wave (all threads; a barrier must follow before f is
-// read).
+// Forces into f (LDS, 6 per atom slot: the partial sums over the first and
+// the second half of the column words, at f[3 i] and f[3 np + 3 i]) and the
+// pair energies of this wave; the waves take the (row block, half) items in
+// turn.  All threads; a barrier must follow before f is read.
 __device__ __forceinline__ void pair_forces(const DevSys& S, const EnergyLds& L, double* f, double& el, double& vw,
                                             double& hb, Work& wk) {
   const int nwaves = blockDim.x / kWave, wave = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
-  const int nt = (S.nat + 63) / 64;
-  for (int bi = wave; bi < nt; bi += nwaves) {
+  const int nt = (S.nat + 63) / 64, wh = (S.words + 1) / 2;
+  for (int item = wave; item < 2 * nt; item += nwaves) {
+    const int bi = item >> 1, h = item & 1;
     const int i = 64 * bi + lane;
     double fx = 0.0, fy = 0.0, fz = 0.0;
-    for (int bj = 0; bj < S.words; ++bj) row_forces(S, L, i, bj, fx, fy, fz, el, vw, hb, wk);
+    const int b0 = h ? wh : 0, b1 = h ? S.words : wh;
+    for (int bj = b0; bj < b1; ++bj) row_forces(S, L, i, bj, fx, fy, fz, el, vw, hb, wk);
     if (i < S.nat) {
-      f[3 * i + 0] = fx;
-      f[3 * i + 1] = fy;
-      f[3 * i + 2] = fz;
+      double* o = f + (size_t)3 * S.np * h + 3 * i;
+      o[0] = fx;
+      o[1] = fy;
+      o[2] = fz;
     }
   }
 }
@@ -118,8 +123,9 @@ __device__ __forceinline__ void torsion_derivatives(const DevSys& S, const doubl
   for (int v = wave; v < S.nvt; v += nwaves) {
     const int m1 = S.var_m1[v], m2 = S.var_m2[v];
     double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const double* f2 = f + (size_t)3 * S.np;
     for (int a = m1 + lane; a <= m2; a += kWave) {
-      const double fx = f[3 * a], fy = f[3 * a + 1], fz = f[3 * a + 2];
+      const double fx = f[3 * a] + f2[3 * a], fy = f[3 * a + 1] + f2[3 * a + 1], fz = f[3 * a + 2] + f2[3 * a + 2];
       const double x = L.x[a], y = L.y[a], z = L.z[a];
       s[0] += fx;
       s[1] += fy;

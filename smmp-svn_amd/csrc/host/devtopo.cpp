@@ -204,6 +204,7 @@ HostSys build_host_sys(const State& s) {
   h.itysol = s.itysol;
   h.npnt = s.itysol > 0 ? s.npnt : 0;
   if (h.npnt > 128) throw Error("device engine: tessellations of more than 128 points are not supported");
+  if (s.itysol > 0 && h.npnt < 1) throw Error("device engine: the numerical solvent needs tessellation points");
   h.spoint.resize((size_t)h.npnt * 3);
   for (int p = 1; p <= h.npnt; ++p)
     for (int d = 1; d <= 3; ++d) h.spoint[(size_t)(p - 1) * 3 + (d - 1)] = s.spoint(p, d);

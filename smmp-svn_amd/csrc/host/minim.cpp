@@ -2,7 +2,9 @@
 // variables that are not fixed, with the derivatives from the device
 // (engine_gradient), and a batched form that minimises many conformations
 // side by side, one device launch per round of function calls.
+#include <algorithm>
 #include <cmath>
+#include <thread>
 #include <vector>
 
 #include "engine.hpp"
@@ -55,6 +57,26 @@ struct Minimiser {
 
 double wrap(double v) { return std::fabs(v) > kPiM ? v - std::copysign(kPi2M, v) : v; }
 
+// fn(c) for c in [0, n): on up to 16 host threads when there is enough work
+// (the quasi-Newton factor updates cost O(nvr^2) per conformation and round),
+// each thread a contiguous range.
+template <typename F>
+void parallel_for(int n, F&& fn) {
+  const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
+  const int nt = std::min({16, hw, n / 32});
+  if (nt <= 1) {
+    for (int c = 0; c < n; ++c) fn(c);
+    return;
+  }
+  std::vector<std::thread> th;
+  th.reserve(nt);
+  for (int t = 0; t < nt; ++t)
+    th.emplace_back([&, t] {
+      for (int c = (int)((int64_t)n * t / nt); c < (int)((int64_t)n * (t + 1) / nt); ++c) fn(c);
+    });
+  for (auto& x : th) x.join();
+}
+
 }  // namespace
 
 int minim(Engine*& e, State& s, int imin, int maxit, double eps) {
@@ -102,17 +124,17 @@ void minim_batch(Engine*& e, State& s, int imin, int nconf, double* vlvr, int ma
   const std::vector<int> fv = free_variables(s);
   const int n = (int)fv.size();
   std::vector<Minimiser> ms(nconf);
-  std::vector<double> ge((size_t)kGradEnergies * nconf), gd((size_t)nvt * nconf), g(n);
+  std::vector<double> ge((size_t)kGradEnergies * nconf), gd((size_t)nvt * nconf);
   engine_gradient(e, s, nconf, vlvr, ge.data(), gd.data());
-  std::vector<double> x(n);
-  for (int c = 0; c < nconf; ++c) {
+  parallel_for(nconf, [&](int c) {
+    std::vector<double> x(n), g(n);
     for (int k = 0; k < n; ++k) {
       x[k] = vlvr[(size_t)c * nvt + fv[k]];
       g[k] = gd[(size_t)c * nvt + fv[k]];
     }
     ms[c].imin = imin;
     ms[c].start(n, x.data(), ge[(size_t)kGradEnergies * c], g.data(), eps, maxit);
-  }
+  });
   // rounds: every minimisation still running asks for one point
   std::vector<int> active;
   std::vector<double> pts;
@@ -123,17 +145,18 @@ void minim_batch(Engine*& e, State& s, int imin, int nconf, double* vlvr, int ma
     if (active.empty()) break;
     const int na = (int)active.size();
     pts.resize((size_t)na * nvt);
-    for (int a = 0; a < na; ++a) {
+    parallel_for(na, [&](int a) {
       double* p = pts.data() + (size_t)a * nvt;
       for (int k = 0; k < nvt; ++k) p[k] = vlvr[(size_t)active[a] * nvt + k];  // fixed variables
       const double* q = ms[active[a]].point();
       for (int k = 0; k < n; ++k) p[fv[k]] = q[k];
-    }
+    });
     engine_gradient(e, s, na, pts.data(), ge.data(), gd.data());
-    for (int a = 0; a < na; ++a) {
+    parallel_for(na, [&](int a) {
+      std::vector<double> g(n);
       for (int k = 0; k < n; ++k) g[k] = gd[(size_t)a * nvt + fv[k]];
       ms[active[a]].supply(ge[(size_t)kGradEnergies * a], g.data());
-    }
+    });
   }
   for (int c = 0; c < nconf; ++c) {
     for (int k = 0; k < n; ++k) vlvr[(size_t)c * nvt + fv[k]] = wrap(ms[c].x()[k]);
