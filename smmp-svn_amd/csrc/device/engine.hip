@@ -179,7 +179,7 @@ struct DevMem {
 struct SamplerBuffers {
   DevMem idvr, axvr, vlvr, xyz, nfree, valid, eol, acc, mt, rng_of, slot_of, beta, err, dbg;
   DevMem wb, walpha, hist, traj, ipoi, iswitch, work, trace, trace_n, phase, part, sync, gsync;
-  DevMem sweep_base, tune_vl, tune_mt, tune_gb, gb, da_stats, snap, snap_e, obs, varx, upd;
+  DevMem sweep_base, tune_vl, tune_mt, tune_gb, gb, da_stats, snap, snap_e, obs, varx, upd, spec_nf;
 };
 
 // One system uploaded to the device: a single allocation holding every

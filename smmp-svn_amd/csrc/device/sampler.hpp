@@ -105,6 +105,14 @@ struct SamplerParams {
   int32_t* trace_n;
   double* phase;         // optional: [R][8] wall-clock ticks per trial phase (see PhaseId)
   int32_t bits_lds;      // stage the pair bit rows in LDS
+  int32_t var_global;    // per-variable FP64 tables read from global memory instead of LDS
+  // Speculative batches (G > 1): the G blocks of a replica evaluate the next
+  // G torsion trials at once, block g trial g as if every earlier trial of
+  // the batch is rejected, and then take the decisions in order (see
+  // k_sweep).  spec_nf: [R][2][G][1 + nheavy] the free-point counts of each
+  // block's trial (count, then h | nf << 16).
+  int32_t spec;
+  int32_t* spec_nf;
   // Workgroups per replica (G).  With G > 1 the G blocks of a replica hold
   // the same state and split every energy change between them; they meet
   // once per trial in group_sum.  The launch is cooperative (all blocks
@@ -168,9 +176,18 @@ enum PhaseId { kPhPropose = 0, kPhRotate, kPhPairs, kPhSolvent, kPhReduce, kPhAc
 constexpr int kScDoubles = 192;
 constexpr int kScRigid = 160;
 constexpr int kScUpd = 176;
+// Speculative batches: per trial of the batch 8 doubles (jv, old value, new
+// value, angle change, acceptance number, BGS flag, draws before the
+// acceptance number, draws after it), the G gathered results (8 each), this
+// block's result (8), batch scalars (8); then a copy of the twister.
+constexpr int kSpecMaxG = 8;
+constexpr int kSpecDoubles = 16 * kSpecMaxG + 16;
 
-// Extra LDS of the sampler beyond the energy carve-up.
-__host__ __device__ inline size_t sampler_lds_bytes(int np, int npnt, int nheavy, int nvt, int nwaves, size_t nbits) {
+// Extra LDS of the sampler beyond the energy carve-up.  var_global: the
+// per-variable FP64 tables stay in global memory (a smaller carve-up, so
+// that two workgroups can share a CU).
+__host__ __device__ inline size_t sampler_lds_bytes(int np, int npnt, int nheavy, int nvt, int nwaves, size_t nbits,
+                                                    bool var_global = false, bool spec = false) {
   size_t b = energy_lds_bytes(np, npnt, nwaves, nheavy, nbits);
   b = (b + 15) & ~size_t(15);
   b += 3 * (size_t)np * sizeof(double);           // backup of moved coordinates
@@ -179,17 +196,21 @@ __host__ __device__ inline size_t sampler_lds_bytes(int np, int npnt, int nheavy
   b += 625 * sizeof(uint32_t);                    // MT state
   b += kScDoubles * sizeof(double);               // broadcast scalars and phase clocks
   b += 6 * kMaxMol * sizeof(double);              // global parameters of the chains
-  b += 7 * (size_t)nvt * sizeof(double);          // per-variable tables (VarLds)
+  if (!var_global) b += 7 * (size_t)nvt * sizeof(double);  // per-variable tables (VarLds)
   b += 8 * (size_t)nvt * sizeof(int16_t);
   b += 2 * (size_t)nheavy;                        // list of the affected heavy atoms
   b += (size_t)nheavy;                            // affected flags
   b += (size_t)nvt;                               // variable names starting with 'x'
+  if (spec) b = ((b + 7) & ~size_t(7)) + kSpecDoubles * sizeof(double) + 625 * sizeof(uint32_t);
   return b + 64;
 }
 
 // Per-variable tables read by every trial, staged in LDS: step width,
 // rotation sign, torsion-energy parameters, the torsion atom's snapshot
-// (toat0, tref) and controlling variable, moved range, axis atoms.
+// (toat0, tref) and controlling variable, moved range, axis atoms.  With
+// the compact carve-up the FP64 tables point at the system's arrays in
+// global memory and t0, tref are null (the torsion atom's values are read
+// where they are used).
 struct VarLds {
   double *ax, *sign, *e0, *sg, *rn, *t0, *tref;
   int16_t *m1, *m2, *base, *axis0, *tv, *idvr, *h1, *h2;
@@ -205,10 +226,16 @@ struct SamplerLds {
   uint32_t* mt;
   int16_t* alist;
   int8_t* aff;
+  double* spec;      // speculative batches (kSpecDoubles; carved only with P.spec)
+  uint32_t* mtsave;  // the twister before a batch's draws (625 words)
 };
 
-__device__ inline SamplerLds carve_sampler_lds(double* base, const DevSys& S, int nvt, int nwaves, bool bits_lds) {
+// lds_flags: bit 0 pair bits in LDS, bit 1 per-variable FP64 tables in
+// global memory (axvr: the step widths there; may be null where V is unused).
+__device__ inline SamplerLds carve_sampler_lds(double* base, const DevSys& S, int nvt, int nwaves, int lds_flags,
+                                               const double* axvr) {
   SamplerLds Q;
+  const bool bits_lds = (lds_flags & 1) != 0, var_global = (lds_flags & 2) != 0;
   Q.L = carve_energy_lds(base, S, nwaves, bits_lds);
   const size_t nbits = bits_lds ? pair_bit_words(S.nat, S.words) : 0;
   char* p = reinterpret_cast<char*>(base) + ((energy_lds_bytes(S.np, S.npnt, nwaves, S.nheavy, nbits) + 15) & ~size_t(15));
@@ -219,13 +246,23 @@ __device__ inline SamplerLds carve_sampler_lds(double* base, const DevSys& S, in
   Q.sc = reinterpret_cast<double*>(p); p += kScDoubles * sizeof(double);
   Q.gb = reinterpret_cast<double*>(p); p += 6 * kMaxMol * sizeof(double);
   // doubles first (8-byte aligned), then 32-, 16- and 8-bit arrays
-  Q.V.ax = reinterpret_cast<double*>(p); p += nvt * sizeof(double);
-  Q.V.sign = reinterpret_cast<double*>(p); p += nvt * sizeof(double);
-  Q.V.e0 = reinterpret_cast<double*>(p); p += nvt * sizeof(double);
-  Q.V.sg = reinterpret_cast<double*>(p); p += nvt * sizeof(double);
-  Q.V.rn = reinterpret_cast<double*>(p); p += nvt * sizeof(double);
-  Q.V.t0 = reinterpret_cast<double*>(p); p += nvt * sizeof(double);
-  Q.V.tref = reinterpret_cast<double*>(p); p += nvt * sizeof(double);
+  if (var_global) {
+    Q.V.ax = const_cast<double*>(axvr);
+    Q.V.sign = const_cast<double*>(S.var_sign);
+    Q.V.e0 = const_cast<double*>(S.var_e0);
+    Q.V.sg = const_cast<double*>(S.var_sg);
+    Q.V.rn = const_cast<double*>(S.var_rn);
+    Q.V.t0 = nullptr;
+    Q.V.tref = nullptr;
+  } else {
+    Q.V.ax = reinterpret_cast<double*>(p); p += nvt * sizeof(double);
+    Q.V.sign = reinterpret_cast<double*>(p); p += nvt * sizeof(double);
+    Q.V.e0 = reinterpret_cast<double*>(p); p += nvt * sizeof(double);
+    Q.V.sg = reinterpret_cast<double*>(p); p += nvt * sizeof(double);
+    Q.V.rn = reinterpret_cast<double*>(p); p += nvt * sizeof(double);
+    Q.V.t0 = reinterpret_cast<double*>(p); p += nvt * sizeof(double);
+    Q.V.tref = reinterpret_cast<double*>(p); p += nvt * sizeof(double);
+  }
   Q.nf = reinterpret_cast<int32_t*>(p); p += S.nheavy * sizeof(int32_t);
   Q.nf2 = reinterpret_cast<int32_t*>(p); p += S.nheavy * sizeof(int32_t);
   Q.mt = reinterpret_cast<uint32_t*>(p); p += 625 * sizeof(uint32_t);
@@ -240,18 +277,25 @@ __device__ inline SamplerLds carve_sampler_lds(double* base, const DevSys& S, in
   Q.alist = reinterpret_cast<int16_t*>(p); p += S.nheavy * sizeof(int16_t);
   Q.aff = reinterpret_cast<int8_t*>(p); p += S.nheavy;
   Q.V.xname = reinterpret_cast<int8_t*>(p);
+  p += nvt;
+  p = reinterpret_cast<char*>((reinterpret_cast<uintptr_t>(p) + 7) & ~uintptr_t(7));
+  Q.spec = reinterpret_cast<double*>(p);
+  Q.mtsave = reinterpret_cast<uint32_t*>(p + kSpecDoubles * sizeof(double));
   return Q;
 }
 
 // Stages the per-variable tables (all threads; a barrier must follow).
 __device__ void stage_variables(const SamplerParams& P, const DevSys& S, const VarLds& V) {
+  const bool in_lds = V.t0 != nullptr;  // else the FP64 tables are the global ones
   for (int k = threadIdx.x; k < P.nvt; k += blockDim.x) {
-    V.ax[k] = P.axvr[k];
-    V.sign[k] = S.var_sign[k];
     const double e0 = S.var_e0[k];
-    V.e0[k] = e0;
-    V.sg[k] = S.var_sg[k];
-    V.rn[k] = S.var_rn[k];
+    if (in_lds) {
+      V.ax[k] = P.axvr[k];
+      V.sign[k] = S.var_sign[k];
+      V.e0[k] = e0;
+      V.sg[k] = S.var_sg[k];
+      V.rn[k] = S.var_rn[k];
+    }
     V.m1[k] = S.var_m1[k];
     V.m2[k] = S.var_m2[k];
     V.base[k] = S.var_base[k];
@@ -266,8 +310,10 @@ __device__ void stage_variables(const SamplerParams& P, const DevSys& S, const V
       tref = S.tref[ia];
       tv = S.tvar[ia];
     }
-    V.t0[k] = t0;
-    V.tref[k] = tref;
+    if (in_lds) {
+      V.t0[k] = t0;
+      V.tref[k] = tref;
+    }
     V.tv[k] = tv;
   }
   for (int k = threadIdx.x; k < P.nvr; k += blockDim.x) V.idvr[k] = (int16_t)P.idvr[k];
@@ -424,11 +470,11 @@ struct ReplicaEnergy {
   Work w;
 };
 template <bool MIXED>
-__device__ __attribute__((noinline)) ReplicaEnergy replica_energy(const DevSys* sys, int nvt, int bits_lds,
+__device__ __attribute__((noinline)) ReplicaEnergy replica_energy(const DevSys* sys, int nvt, int lds_flags,
                                                                   int nf_sel, bool rebuild, int part, int nparts) {
   extern __shared__ double smem[];
   const DevSys& S = *sys;
-  SamplerLds Q = carve_sampler_lds(smem, S, nvt, blockDim.x / kWave, bits_lds != 0);
+  SamplerLds Q = carve_sampler_lds(smem, S, nvt, blockDim.x / kWave, lds_flags, nullptr);
   int32_t* nfree_out = nf_sel == 1 ? Q.nf : (nf_sel == 2 ? Q.nf2 : nullptr);
   ReplicaEnergy r;
   if (rebuild)
@@ -470,12 +516,39 @@ __device__ __forceinline__ void replica_of_block(const SamplerParams& P, int b, 
 // (it waits for all of them at each reduction), so it never overwrites
 // values still being read.  The wait is bounded on the device clock; on
 // expiry the error flag gets bit 3 and the block goes on.
+__device__ void group_wait(const SamplerParams& P, int r, uint32_t epoch);
 __device__ void group_sum(const SamplerParams& P, int r, int g, uint32_t& epoch, double* v, int nv) {
   const int G = P.G;
   if (G == 1) return;
   double* base = P.part + ((size_t)r * 2 + (epoch & 1u)) * G * kPartVals;
   double* mine = base + (size_t)g * kPartVals;
   for (int k = 0; k < nv; ++k) mine[k] = v[k];
+  group_wait(P, r, epoch);
+  for (int k = 0; k < nv; ++k) v[k] = 0.0;
+  for (int q = 0; q < G; ++q) {
+    const double* src = base + (size_t)q * kPartVals;
+    for (int k = 0; k < nv; ++k) v[k] += src[k];
+  }
+  ++epoch;
+}
+
+// Like group_sum, but every block receives all G rows of kPartVals values
+// (out[kPartVals q + k]) instead of their sum.
+__device__ void group_gather(const SamplerParams& P, int r, int g, uint32_t& epoch, const double* v, double* out) {
+  const int G = P.G;
+  double* base = P.part + ((size_t)r * 2 + (epoch & 1u)) * G * kPartVals;
+  double* mine = base + (size_t)g * kPartVals;
+  for (int k = 0; k < kPartVals; ++k) mine[k] = v[k];
+  group_wait(P, r, epoch);
+  for (int k = 0; k < G * kPartVals; ++k) out[k] = base[k];
+  ++epoch;
+}
+
+// Arrival of this block at meeting 'epoch' of replica r and the bounded
+// wait for all G blocks (thread 0): on return the rows written before every
+// block's arrival are visible.
+__device__ void group_wait(const SamplerParams& P, int r, uint32_t epoch) {
+  const int G = P.G;
   __hip_atomic_fetch_add(&P.sync[r], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
   const uint32_t target = (epoch + 1u) * (uint32_t)G;
   const uint64_t t0 = wall_clock64();
@@ -490,12 +563,6 @@ __device__ void group_sum(const SamplerParams& P, int r, int g, uint32_t& epoch,
     }
   }
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the partials are visible now
-  for (int k = 0; k < nv; ++k) v[k] = 0.0;
-  for (int q = 0; q < G; ++q) {
-    const double* src = base + (size_t)q * kPartVals;
-    for (int k = 0; k < nv; ++k) v[k] += src[k];
-  }
-  ++epoch;
 }
 
 // Temperature exchanges between neighbouring slots (partem_s.f:209-232):
@@ -548,6 +615,216 @@ __device__ void grid_sync(const SamplerParams& P, uint32_t& gepoch) {
   ++gepoch;
 }
 
+// Turns the atoms [m1, m2] by the angle whose sine and cosine are sn, cs
+// about the axis through p along the unit vector k (Rodrigues), with their
+// packed heavy copies; backup: keep the old coordinates in Q.bx/by/bz.
+// All threads.
+__device__ __forceinline__ void rotate_range(const SamplerLds& Q, int m1, int m2, double px, double py, double pz,
+                                             double kx, double ky, double kz, double sn, double cs, bool backup) {
+  const EnergyLds& L = Q.L;
+  for (int a = m1 + threadIdx.x; a <= m2; a += blockDim.x) {
+    const double vx = L.x[a] - px, vy = L.y[a] - py, vz = L.z[a] - pz;
+    if (backup) {
+      Q.bx[a - m1] = L.x[a];
+      Q.by[a - m1] = L.y[a];
+      Q.bz[a - m1] = L.z[a];
+    }
+    const double kv = kx * vx + ky * vy + kz * vz;
+    const double cx = ky * vz - kz * vy, cy = kz * vx - kx * vz, cz = kx * vy - ky * vx;
+    const double t = kv * (1.0 - cs);
+    const double xn = px + vx * cs + cx * sn + kx * t;
+    const double yn = py + vy * cs + cy * sn + ky * t;
+    const double zn = pz + vz * cs + cz * sn + kz * t;
+    L.x[a] = xn;
+    L.y[a] = yn;
+    L.z[a] = zn;
+    const int h = L.hidx[a];
+    if (h >= 0) {
+      L.hp[4 * h + 0] = xn;
+      L.hp[4 * h + 1] = yn;
+      L.hp[4 * h + 2] = zn;
+    }
+  }
+}
+
+// The rotation of a torsion trial of variable jv by dphi (the proposal's
+// axis from the current coordinates); every thread computes the same values.
+__device__ __forceinline__ void torsion_rotation(const DevSys& S, const SamplerLds& Q, int jv, double dphi, bool backup) {
+  const EnergyLds& L = Q.L;
+  const VarLds& V = Q.V;
+  const int b = V.base[jv], a0 = V.axis0[jv];
+  const double ux = L.x[b] - L.x[a0], uy = L.y[b] - L.y[a0], uz = L.z[b] - L.z[a0];
+  const double un = rsqrt_f64(ux * ux + uy * uy + uz * uz);
+  double sn, cs;
+  sincos(V.sign[jv] * dphi, &sn, &cs);
+  rotate_range(Q, V.m1[jv], V.m2[jv], L.x[b], L.y[b], L.z[b], ux * un, uy * un, uz * un, sn, cs, backup);
+}
+
+// Free-point counts after a full recomputation, where each block computed
+// its share (h % G == g): publish (before the meeting that follows: this
+// block's share to gnf, fenced) or collect (after it: all of gnf).  All
+// threads; ends with a barrier.
+__device__ __forceinline__ void share_free_points(const DevSys& S, const SamplerLds& Q, int32_t* gnf, int g, int G,
+                                                  bool publish) {
+  if (publish) {
+    for (int h = threadIdx.x; h < S.nheavy; h += blockDim.x)
+      if (h % G == g) gnf[h] = Q.nf[h];
+    __threadfence();
+  } else {
+    __syncthreads();  // thread 0 has passed the meeting
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    for (int h = threadIdx.x; h < S.nheavy; h += blockDim.x)
+      Q.nf[h] = __hip_atomic_load(&gnf[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+}
+
+// One trial of a speculative batch, evaluated by this block alone on its
+// copy of the replica state, which it leaves as it found it: variable jv
+// from vold to vnew (angle change dphi).  Thread 0 receives the energy
+// change's parts in res[0..6) (Coulomb, 6-12, H-bond, solvent, torsion
+// term, error flag); nf_out (global) receives the trial's free-point counts
+// of its affected heavy atoms (count, then h | nf << 16).  All threads.
+template <bool MIXED>
+__device__ __attribute__((noinline)) Work spec_trial(const DevSys* sys, int nvt, int lds_flags, const double* axvr,
+                                                     int jv, double vold, double vnew, double dphi, int32_t* nf_out,
+                                                     double* res) {
+  extern __shared__ double smem[];
+  const DevSys& S = *sys;
+  const int tid = threadIdx.x, nwaves = blockDim.x / kWave, wave = tid / kWave, lane = tid & (kWave - 1);
+  SamplerLds Q = carve_sampler_lds(smem, S, nvt, nwaves, lds_flags, axvr);
+  const EnergyLds& L = Q.L;
+  const VarLds& V = Q.V;
+  Work wk;
+  const int m1 = V.m1[jv], m2 = V.m2[jv], h1 = V.h1[jv], h2 = V.h2[jv], nh = S.nheavy;
+  if (tid == 0) wk.moved += m2 - m1 + 1;
+  torsion_rotation(S, Q, jv, dphi, true);
+  // torsion term of the variable, the new and the old angle, on two lanes
+  // of the last wave (enyshe.f:73-79)
+  const int tl = tid - ((int)blockDim.x - kWave);
+  if (tl == 0 || tl == 1) {
+    const double e0 = V.e0[jv];
+    double e = 0.0;
+    if (e0 != 0.0) {
+      const int v = V.tv[jv];
+      const int ia = V.t0 ? 0 : S.var_atom[jv];
+      double to = V.t0 ? V.t0[jv] : S.toat0[ia];
+      if (v >= 0) {
+        const double val = v == jv ? (tl == 0 ? vnew : vold) : Q.vl[v];
+        to = to + d_difang(V.t0 ? V.tref[jv] : S.tref[ia], val);
+        if (fabs(to) > kPi) to = to - copysign(kPi2, to);
+      }
+      e = e0 * (1.0 + V.sg[jv] * cos(to * V.rn[jv]));
+    }
+    Q.sc[17 + tl] = e;  // [17] new, [18] old
+  }
+  for (int h = tid; h < nh; h += blockDim.x) Q.aff[h] = 0;
+  __syncthreads();
+  PairE d = {0.0, 0.0, 0.0};
+  pair_delta(S, Q, m1, m2, wave, nwaves, d, wk);
+  if (S.itysol > 0) mark_affected(S, Q, h1, h2, m1, 0, 1);
+  __syncthreads();
+  int* next_atom = reinterpret_cast<int*>(Q.sc + 30);
+  if (S.itysol > 0) {
+    const int na = wave == 0 ? list_affected(S, Q, 0, 1) : 0;
+    if (tid == 0) {
+      Q.sc[29] = (double)na;
+      *next_atom = 0;
+    }
+    __syncthreads();
+  }
+  int err = 0;
+  if (S.itysol > 0) {
+    double* nb = L.fr + (size_t)wave * kNbDoubles;
+    const int na = (int)Q.sc[29];
+    for (;;) {  // waves take the next atom as they finish
+      int idx = 0;
+      if (lane == 0) idx = atomicAdd(next_atom, 1);
+      idx = __shfl(idx, 0, kWave);
+      if (idx >= na) break;
+      const int h = Q.alist[idx];
+      int nfree = solvent_atom(S, L, nb, L.hv[h], wk, lds_flags & 1, nullptr);
+      if (nfree < 0) {
+        err = 1;
+        nfree = Q.nf[h];
+      }
+      if (lane == 0) Q.nf2[h] = nfree;
+    }
+  }
+  double v[3] = {d.el, d.vw, d.hb};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) v[k] = wave_sum(v[k]);
+  const int errw = __any(err);
+  if (lane == 0) {
+    for (int k = 0; k < 3; ++k) L.red[8 * wave + k] = v[k];
+    L.red[8 * wave + 6] = (double)errw;
+  }
+  __syncthreads();
+  double dsol = 0.0;
+  const int na = S.itysol > 0 ? (int)Q.sc[29] : 0;
+  if (S.itysol > 0 && wave == 0) {  // in list order: independent of which wave took which atom
+    const double c4 = 4.0 * kPi / (double)S.npnt;
+    for (int k = lane; k < na; k += kWave) {
+      const int h = Q.alist[k];
+      const int i = L.hv[h];
+      const double trad = L.rv[i];
+      dsol += L.sg[i] * (c4 * trad * trad * (double)(Q.nf2[h] - Q.nf[h]));
+    }
+    dsol = wave_sum(dsol);
+  }
+  if (tid == 0) {
+    double t[6] = {0.0, 0.0, 0.0, dsol, Q.sc[17] - Q.sc[18], 0.0};
+    for (int w = 0; w < nwaves; ++w) {
+      for (int k = 0; k < 3; ++k) t[k] += L.red[8 * w + k];
+      t[5] += L.red[8 * w + 6];
+    }
+    for (int k = 0; k < 6; ++k) res[k] = t[k];
+    nf_out[0] = na;
+  }
+  for (int k = tid; k < na; k += blockDim.x) {
+    const int h = Q.alist[k];
+    nf_out[1 + k] = h | (Q.nf2[h] << 16);
+  }
+  __threadfence();  // the counts are visible before thread 0 reports arrival
+  // back to the state before the trial
+  for (int a = m1 + tid; a <= m2; a += blockDim.x) {
+    const double xo = Q.bx[a - m1], yo = Q.by[a - m1], zo = Q.bz[a - m1];
+    L.x[a] = xo;
+    L.y[a] = yo;
+    L.z[a] = zo;
+    const int h = L.hidx[a];
+    if (h >= 0) {
+      L.hp[4 * h + 0] = xo;
+      L.hp[4 * h + 1] = yo;
+      L.hp[4 * h + 2] = zo;
+    }
+  }
+  __syncthreads();
+  return wk;
+}
+
+// The accepted trial of a speculative batch becomes the state of this
+// block: the same rotation as in spec_trial (bit-identical coordinates),
+// the variable, and the evaluating block's free-point counts nf_in (global,
+// visible after the batch's meeting).  All threads.
+__device__ __attribute__((noinline)) void spec_apply(const DevSys* sys, int nvt, int lds_flags, const double* axvr,
+                                                     int jv, double vnew, double dphi, const int32_t* nf_in) {
+  extern __shared__ double smem[];
+  const DevSys& S = *sys;
+  SamplerLds Q = carve_sampler_lds(smem, S, nvt, blockDim.x / kWave, lds_flags, axvr);
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  torsion_rotation(S, Q, jv, dphi, false);
+  if (threadIdx.x == 0) Q.vl[jv] = vnew;
+  if (S.itysol > 0) {
+    const int na = nf_in[0];
+    for (int k = threadIdx.x; k < na; k += blockDim.x) {
+      const int32_t e = nf_in[1 + k];
+      Q.nf[e & 0xffff] = e >> 16;
+    }
+  }
+  __syncthreads();
+}
+
 // P.nsweep Metropolis sweeps of one replica per group of G blocks; DA:
 // delayed acceptance with the surrogate network (P.da).
 template <bool MIXED, bool DA>
@@ -559,7 +836,11 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
   replica_of_block(P, blockIdx.x, r, g);
   const int tid = threadIdx.x, nwaves = blockDim.x / kWave, wave = tid / kWave, lane = tid & (kWave - 1);
   const int nvt = P.nvt, np = S.np, nat = S.nat, nh = S.nheavy;
-  SamplerLds Q = carve_sampler_lds(smem, S, nvt, nwaves, P.bits_lds != 0);
+  const int lds_flags = (P.bits_lds ? 1 : 0) | (P.var_global ? 2 : 0);
+  SamplerLds Q = carve_sampler_lds(smem, S, nvt, nwaves, lds_flags, P.axvr);
+  // speculative batches: every block keeps the free-point counts of all
+  // heavy atoms (shared through gnf after each full recomputation)
+  const bool spec_on = P.spec != 0 && !DA && !P.verify && G > 1 && G <= kSpecMaxG;
   Work wk;
   EnergyLds& L = Q.L;
   double* gvl = P.vlvr + (size_t)r * nvt;
@@ -599,15 +880,17 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
   //     [6] jv, [7] drift flag, [8..10] axis point, [11..13] axis unit, [15] old value,
   //     [16] trial energy, [24..28] incremental components (verify), [32..40] phase clocks
   if (!valid) {
-    ReplicaEnergy re = replica_energy<MIXED>(P.sys, nvt, P.bits_lds, 1, true, g, G);
+    ReplicaEnergy re = replica_energy<MIXED>(P.sys, nvt, lds_flags, 1, true, g, G);
     if (g != 0) re.w.builds = 0.0;
     add_work(wk, re.w);
+    if (spec_on) share_free_points(S, Q, gnf, g, G, true);
     if (tid == 0) {
       double v[2] = {re.o.el + re.o.vw + re.o.hb + re.o.tor + re.o.sol, (double)re.o.err};
       group_sum(P, r, g, epoch, v, 2);
       Q.sc[0] = v[0];
       if (v[1] != 0.0) atomicOr(P.err, 2);
     }
+    if (spec_on) share_free_points(S, Q, gnf, g, G, false);
   } else if (tid == 0) {
     Q.sc[0] = P.eol[r];
   }
@@ -639,8 +922,153 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
     if (tid == 0) Q.sc[14] = P.beta[P.slot_of[r]];  // LDS: no global reload inside the loop
     // torsion trials, then with several chains six rigid trials per chain
     const int ntrial = P.nvr + (S.nml > 1 ? 6 * S.nml : 0);
+    // speculative batches for this sweep's torsion trials: not from a
+    // clashing state (those trials take the full recomputation)
+    __syncthreads();
+    const bool spec_sweep = spec_on && fabs(Q.sc[0]) < 1e5;
     for (int nsw = 0; nsw < ntrial; ++nsw) {
       const bool rigid = nsw >= P.nvr;  // uniform
+      if (spec_sweep && !rigid) {
+        // ---- speculative batch of the next nb torsion trials.  The G
+        //      blocks hold the same state and twister; each draws all nb
+        //      proposals as if every earlier trial of the batch is rejected
+        //      (a rejected trial always consumed its acceptance number, so
+        //      the positions are fixed), block t evaluates trial t, the
+        //      blocks exchange the energy changes and take the decisions in
+        //      order with the numbers drawn; the first accepted trial ends
+        //      the batch and the twister is set back to the draws the
+        //      decisions consumed.  Same decisions, same stream as one
+        //      trial at a time.
+        double* Z = Q.spec;  // [0,8G) proposals, [8G,16G) results, [16G,16G+8) own, then scalars
+        const int nb = min(G, P.nvr - nsw);
+        for (int k = tid; k < 625; k += blockDim.x) Q.mtsave[k] = Q.mt[k];
+        __syncthreads();
+        if (tid == 0) {
+          uint32_t* mti = &Q.mt[624];
+          int drawn = 0;
+          for (int t = 0; t < nb; ++t) {
+            double* T = Z + 8 * t;
+            int ivar = nsw + t;
+            if (P.rndord) {
+              ivar = (int)(P.nvr * mt_grnd(Q.mt, mti));
+              ++drawn;
+              if (ivar >= P.nvr) ivar = P.nvr - 1;
+            }
+            const int jv = V.idvr[ivar];
+            bool bgs = false;
+            if (P.upch != 0 && V.xname[jv] == 0) {
+              const double bu = P.upch_own ? Q.sc[14] : P.upch_beta;
+              const double prob = P.upch == 1 ? P.bgsprob : (bu - P.up2bmin) / (P.up2bmax - P.up2bmin);
+              bgs = mt_grnd(Q.mt, mti) < prob;
+              ++drawn;
+            }
+            T[0] = (double)jv;
+            T[5] = bgs ? 1.0 : 0.0;
+            if (!bgs) {
+              const double vold = Q.vl[jv];
+              const double dv = V.ax[jv] * (mt_grnd(Q.mt, mti) - 0.5);
+              ++drawn;
+              const double vnew = d_addang(vold, dv);
+              T[1] = vold;
+              T[2] = vnew;
+              T[3] = d_difang(vold, vnew);
+              T[6] = (double)drawn;
+              T[4] = mt_grnd(Q.mt, mti);  // its acceptance number, used when the change is uphill
+              ++drawn;
+              T[7] = (double)drawn;
+            } else {
+              T[6] = T[7] = (double)drawn;
+            }
+          }
+          Z[16 * G + 8] = (double)drawn;
+          Z[16 * G + 9] = (double)(epoch & 1u);
+        }
+        __syncthreads();
+        lap(kPhPropose);
+        const bool mine = g < nb && Z[8 * g + 5] == 0.0;
+        const int par = (int)Z[16 * G + 9];
+        int32_t* const nf_rows = P.spec_nf + (size_t)(r * 2 + par) * G * (nh + 1);
+        Work wt;
+        if (mine) {
+          const double* T = Z + 8 * g;
+          wt = spec_trial<MIXED>(P.sys, nvt, lds_flags, P.axvr, (int)T[0], T[1], T[2], T[3],
+                                 nf_rows + (size_t)g * (nh + 1), Z + 16 * G);
+        }
+        lap(kPhSolvent);
+        if (tid == 0) {
+          double v[kPartVals];
+          for (int k = 0; k < kPartVals; ++k) v[k] = (mine && k < 6) ? Z[16 * G + k] : 0.0;
+          group_gather(P, r, g, epoch, v, Z + 8 * G);
+        }
+        __syncthreads();
+        lap(kPhReduce);
+        if (tid == 0) {  // the decisions in order, identical in the G blocks (metropolis.f:72-87)
+          const double eol = Q.sc[0], beta = Q.sc[14];
+          int used = nb, kstar = -1, cons = (int)Z[16 * G + 8];
+          for (int t = 0; t < nb; ++t) {
+            const double* T = Z + 8 * t;
+            if (T[5] != 0.0) {  // inert BGS pick: accanalyze(2, 0)
+              Q.sc[kScUpd + 1] += 1.0;
+              Q.sc[kScUpd + 4] += 1.0;
+              continue;
+            }
+            const double* R = Z + 8 * G + kPartVals * t;
+            if (R[5] != 0.0) atomicOr(P.err, 2);
+            const double enew = eol + (R[4] + R[0] + R[1] + R[2] + R[3]);
+            double delta = weight_of(P, beta, enew) - weight_of(P, beta, eol);
+            bool acc, drew = false;
+            if (delta <= 0.0) {
+              acc = true;
+            } else {
+              drew = true;
+              delta = fmin(delta, 100.0);
+              acc = exp(-delta) >= T[4];
+            }
+            Q.sc[kScUpd + 0] += 1.0;  // accanalyze(1, acc)
+            Q.sc[kScUpd + 4] += 1.0;
+            if (acc) {
+              Q.sc[kScUpd + 5] += 1.0;
+              Q.sc[kScUpd + 9] += 1.0;
+            }
+            if (P.trace && g == 0 && r == P.trace_rep && *P.trace_n < P.trace_cap) {
+              double* tr = P.trace + 4 * (size_t)(*P.trace_n);
+              tr[0] = T[0];
+              tr[1] = enew;
+              tr[2] = eol;
+              tr[3] = acc ? 1.0 : 0.0;
+              *P.trace_n += 1;
+            }
+            if (acc) {
+              kstar = t;
+              used = t + 1;
+              cons = (int)(drew ? T[7] : T[6]);
+              Q.sc[0] = enew;
+              Q.sc[1] += 1.0;
+              break;
+            }
+          }
+          Z[16 * G + 10] = (double)kstar;
+          Z[16 * G + 11] = (double)used;
+          Z[16 * G + 12] = (double)cons;
+        }
+        __syncthreads();
+        const int kstar = (int)Z[16 * G + 10], used = (int)Z[16 * G + 11], cons = (int)Z[16 * G + 12];
+        if (cons < (int)Z[16 * G + 8]) {  // the twister back to the draws the decisions consumed
+          for (int k = tid; k < 625; k += blockDim.x) Q.mt[k] = Q.mtsave[k];
+          __syncthreads();
+          if (tid == 0)
+            for (int i = 0; i < cons; ++i) (void)mt_grnd(Q.mt, &Q.mt[624]);
+        }
+        if (kstar >= 0) {
+          const double* T = Z + 8 * kstar;
+          spec_apply(P.sys, nvt, lds_flags, P.axvr, (int)T[0], T[2], T[3], nf_rows + (size_t)kstar * (nh + 1));
+        }
+        if (mine && g < used) add_work(wk, wt);
+        __syncthreads();
+        lap(kPhAccept);
+        nsw += used - 1;
+        continue;
+      }
       // ---- proposal (leader): metropolis.f:42-66; identical in the G blocks
       if (tid == 0 && rigid) {
         // metropolis.f:94-168: chain m, parameter i of gbpr; the move is
@@ -770,7 +1198,7 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
       }
       EnergyOut vref;  // verify mode (G = 1): full energy before the move (thread 0)
       if (P.verify && !full) {
-        const ReplicaEnergy re = replica_energy<MIXED>(P.sys, nvt, P.bits_lds, 0, false, 0, 1);
+        const ReplicaEnergy re = replica_energy<MIXED>(P.sys, nvt, lds_flags, 0, false, 0, 1);
         vref = re.o;
         __syncthreads();
       }
@@ -834,10 +1262,11 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
           double e = 0.0;
           if (e0 != 0.0) {
             const int v = V.tv[jv];
-            double to = V.t0[jv];
+            const int ia = V.t0 ? 0 : S.var_atom[jv];
+            double to = V.t0 ? V.t0[jv] : S.toat0[ia];
             if (v >= 0) {
               const double val = v == jv ? (tl == 0 ? Q.sc[2] : Q.sc[15]) : Q.vl[v];
-              to = to + d_difang(V.tref[jv], val);
+              to = to + d_difang(V.t0 ? V.tref[jv] : S.tref[ia], val);
               if (fabs(to) > kPi) to = to - copysign(kPi2, to);
             }
             e = e0 * (1.0 + V.sg[jv] * cos(to * V.rn[jv]));
@@ -966,7 +1395,7 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
         lap(kPhReduce);
       } else {
         // ---- full recomputation from the trial coordinates
-        const ReplicaEnergy re = replica_energy<MIXED>(P.sys, nvt, P.bits_lds, 2, false, g, G);
+        const ReplicaEnergy re = replica_energy<MIXED>(P.sys, nvt, lds_flags, 2, false, g, G);
         add_work(wk, re.w);
         if (tid == 0) {
           double v[2] = {re.o.el + re.o.vw + re.o.hb + re.o.tor + re.o.sol, (double)re.o.err};
@@ -979,7 +1408,7 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
         lap(kPhFull);
       }
       if (P.verify && !full) {  // compare each incremental component with a full recomputation
-        const ReplicaEnergy rn = replica_energy<MIXED>(P.sys, nvt, P.bits_lds, 0, false, 0, 1);
+        const ReplicaEnergy rn = replica_energy<MIXED>(P.sys, nvt, lds_flags, 0, false, 0, 1);
         const EnergyOut& vnew = rn.o;
         if (tid == 0 && P.dbg[16 * r] == 0.0) {
           const double fd[5] = {vnew.el - vref.el, vnew.vw - vref.vw, vnew.hb - vref.hb, vnew.tor - vref.tor,
@@ -1079,9 +1508,10 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
       lap(kPhAccept);
     }
     // ---- sweep end: rebuild from the variables and check (metropolis.f:172-179)
-    ReplicaEnergy re = replica_energy<MIXED>(P.sys, nvt, P.bits_lds, 1, true, g, G);
+    ReplicaEnergy re = replica_energy<MIXED>(P.sys, nvt, lds_flags, 1, true, g, G);
     if (g != 0) re.w.builds = 0.0;
     add_work(wk, re.w);
+    if (spec_on) share_free_points(S, Q, gnf, g, G, true);
     if (P.obs && g == 0) {  // rgyr(0, ...) of the rebuilt chain (partem_s.f:186-193, rgyr.f:54-140)
       double m[4] = {0.0, 0.0, 0.0, 0.0};
       for (int a = tid; a < S.nat; a += blockDim.x) {
@@ -1150,6 +1580,7 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
       }
     }
     __syncthreads();
+    if (spec_on) share_free_points(S, Q, gnf, g, G, false);
     if (P.pt_exchange) {  // exchange attempt of this sweep (partem_s.f:204-232)
       if (tid == 0) grid_sync(P, gepoch);
       __syncthreads();

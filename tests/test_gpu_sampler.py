@@ -273,6 +273,33 @@ def test_group_split_matches_single_workgroup(groups):
     assert np.abs(a["vlvr"] - b["vlvr"]).max() < 1e-12
 
 
+@pytest.mark.parametrize("groups", [2, 4])
+def test_speculative_batches_match_sequential(groups, monkeypatch):
+    """SMMP_SPEC=1: the G workgroups of a replica evaluate the next G trials
+    at once, each assuming the earlier ones rejected, and take the decisions
+    in order: the same acceptances, exchanges and twister positions as one
+    trial at a time (energies to rounding), with the solvent term and the
+    update-type draws."""
+    c = make_ctx(tag="c2_20mer", itysol=1)
+    R = 8
+    temps = np.geomspace(250.0, 1000.0, R)
+    seeds = np.arange(11, R + 12, dtype=np.uint32)
+    v0 = np.tile(c.vlvr(), (R, 1))
+    v0[:, c.idvr - 1] = np.pi + np.random.default_rng(9).normal(0.0, 0.5, size=(R, c.nvr))
+    c.view("updchois.upchswitch")[0] = 1
+    c.view("updchois.bgsprob")[0] = 0.2
+    say(f"smmp_sample PT c2_20mer R=8 5 sweeps sequential vs speculative G={groups}")
+    a = c.sample(temps, seeds, 5, v0, mode=1, groups=1)
+    monkeypatch.setenv("SMMP_SPEC", "1")
+    b = c.sample(temps, seeds, 5, v0, mode=1, groups=groups)
+    monkeypatch.delenv("SMMP_SPEC")
+    assert c.sampler_info()["groups"] == groups
+    assert np.array_equal(a["acc"], b["acc"])
+    assert np.array_equal(a["ipoi"], b["ipoi"])
+    assert np.abs(a["traj"] - b["traj"]).max() <= 1e-8 * max(1.0, np.abs(a["traj"]).max())
+    assert np.abs(a["vlvr"] - b["vlvr"]).max() < 1e-12
+
+
 def test_mixed_precision_sampler_runs_and_tracks_fp64():
     """precision=1: FP32 pair terms in the full recomputations, FP64 trial
     changes; the drift check uses the mixed bound.  Final energies against
