@@ -130,6 +130,16 @@ int smmp_interhbond(smmp_ctx *ctx, int32_t *mhb);
  * 1..nresi from the current (phi, psi) into zimme.zimm. */
 int smmp_zimmer(smmp_ctx *ctx, int nresi);
 
+/* rmsdfun(nml,ir1,ir2,ixat,xrf,yrf,zrf,isl) smmp.pyf:4095, rmsdfun.f:12:
+ * rmsd of the heavy atoms of residues ir1..ir2 of molecule nml (counted
+ * within it) from the reference coordinates after the optimal
+ * superposition (quaternion method, rmsdfun.f's Jacobi), current
+ * coordinates; ixat[a-1] (atoms numbered over all molecules): index of atom
+ * a's reference (1-based, 0 none); isl 0 all heavy atoms, 1 n/ca/c, 2 ca.
+ * Fewer than three atoms selected: SMMP_EINPUT. */
+int smmp_rmsdfun(smmp_ctx *ctx, int nml, int ir1, int ir2, const int32_t *ixat, const double *xrf,
+                 const double *yrf, const double *zrf, int isl, double *rmsd);
+
 /* outpdb(nml,file) smmp.pyf:3810, outpdb.f:12: PDB file of molecule nml
  * (0: all) from the current coordinates ('' = stdout). */
 int smmp_outpdb(smmp_ctx *ctx, int nml, const char *path);

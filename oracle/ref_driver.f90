@@ -32,6 +32,10 @@
 !          reference's minqsn.f / mincjg.f and move) from the start and from
 !          nconf random conformations: start, final variables, final
 !          energies, function calls
+!   rmsd   libdir seqfile varfile itysol seed
+!          rmsdfun (rmsdfun.f) of the start against a reference made from
+!          it (turned, shifted, perturbed by the twister; a few atoms
+!          without a reference), for isl 0, 1, 2 and two residue ranges
 !   multi  libdir seqfile itysol gbfile boxsize seed nconf temp nequi nsweep
 !          several chains: gbfile holds 'molecule k value' lines for gbpr;
 !          topology, energies of random conformations with the
@@ -78,6 +82,8 @@ program ref_driver
   character(len=256) :: wfile
   integer :: iepsd, nvta, imin, maxit
   double precision :: epsmin
+  double precision :: rmsdfun, xrf(10000), yrf(10000), zrf(10000), ang(3), rot(3,3), px, py, pz
+  integer :: ixat(mxat), nat, isl, ir1s(2), ir2s(2), q
 
   call get_command_argument(1, mode)
   grpn = 'nh2 '
@@ -287,6 +293,52 @@ program ref_driver
       write(*,'(a,i0)') '@min_nfun int 1'
       write(*,'(i0)') n
     end do
+
+  case ('rmsd')
+    call setup(2, .true.)
+    call iarg(6, iseed)
+    call sgrnd(iseed)
+    e = energy()
+    nat = iatrs2(irsml2(ntlml))
+    do i = 1, 3
+      ang(i) = (grnd() - 0.5d0) * 2.0d0 * pi
+    end do
+    ! a rotation from three angles about z, y, z
+    rot(1,1) = cos(ang(1))*cos(ang(2))*cos(ang(3)) - sin(ang(1))*sin(ang(3))
+    rot(1,2) = -cos(ang(1))*cos(ang(2))*sin(ang(3)) - sin(ang(1))*cos(ang(3))
+    rot(1,3) = cos(ang(1))*sin(ang(2))
+    rot(2,1) = sin(ang(1))*cos(ang(2))*cos(ang(3)) + cos(ang(1))*sin(ang(3))
+    rot(2,2) = -sin(ang(1))*cos(ang(2))*sin(ang(3)) + cos(ang(1))*cos(ang(3))
+    rot(2,3) = sin(ang(1))*sin(ang(2))
+    rot(3,1) = -sin(ang(2))*cos(ang(3))
+    rot(3,2) = sin(ang(2))*sin(ang(3))
+    rot(3,3) = cos(ang(2))
+    do i = 1, nat
+      px = rot(1,1)*xat(i) + rot(1,2)*yat(i) + rot(1,3)*zat(i)
+      py = rot(2,1)*xat(i) + rot(2,2)*yat(i) + rot(2,3)*zat(i)
+      pz = rot(3,1)*xat(i) + rot(3,2)*yat(i) + rot(3,3)*zat(i)
+      xrf(i) = px + 3.0d0 + (grnd() - 0.5d0) * 0.6d0
+      yrf(i) = py - 1.5d0 + (grnd() - 0.5d0) * 0.6d0
+      zrf(i) = pz + 0.5d0 + (grnd() - 0.5d0) * 0.6d0
+      ixat(i) = i
+      if (mod(i, 11) .eq. 5) ixat(i) = 0
+    end do
+    write(*,'(a,i0)') '@rmsd_ref double ', 3*nat
+    write(*,'(es25.17)') (xrf(i), yrf(i), zrf(i), i=1,nat)
+    write(*,'(a,i0)') '@rmsd_ixat int ', nat
+    write(*,'(i0)') (ixat(i), i=1,nat)
+    ir1s(1) = 1
+    ir2s(1) = irsml2(1) - irsml1(1) + 1
+    ir1s(2) = 2
+    ir2s(2) = min(4, ir2s(1))
+    write(*,'(a,i0)') '@rmsd double ', 6
+    do isl = 0, 2
+      do q = 1, 2
+        write(*,'(es25.17)') rmsdfun(1, ir1s(q), ir2s(q), ixat, xrf, yrf, zrf, isl)
+      end do
+    end do
+    write(*,'(a,i0)') '@xyz double ', 3*nat
+    write(*,'(es25.17)') (xat(i), yat(i), zat(i), i=1,nat)
 
   case ('metro')
     call setup(2, .true.)

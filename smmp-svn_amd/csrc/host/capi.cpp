@@ -296,6 +296,16 @@ int smmp_zimmer(smmp_ctx* c, int nresi) {
   });
 }
 
+int smmp_rmsdfun(smmp_ctx* c, int nml, int ir1, int ir2, const int32_t* ixat, const double* xrf, const double* yrf,
+                 const double* zrf, int isl, double* rmsd) {
+  return guarded(c, SMMP_EINPUT, [&] {
+    if (!ixat || !xrf || !yrf || !zrf || !rmsd) throw Error("rmsdfun> NULL argument");
+    if (c->s.ntlml < 1) throw Error("rmsdfun> no molecule initialised");
+    *rmsd = rmsdfun(c->s, nml, ir1, ir2, ixat, xrf, yrf, zrf, isl);
+    return SMMP_OK;
+  });
+}
+
 int smmp_outpdb(smmp_ctx* c, int nml, const char* path) {
   return guarded(c, SMMP_EINPUT, [&] {
     if (c->s.ntlml < 1) throw Error("outpdb> no molecule initialised");

@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdio>
 #include <string>
+#include <vector>
 
 #include "state.hpp"
 
@@ -251,6 +252,135 @@ void zimmer(State& s, int nresi) {
 // outpdb (outpdb.f:12-185): the coordinates of molecule nml (0: all) in PDB
 // format, hydrogens with their trailing digit moved to the front of the
 // name, TER after each chain, CONECT records of the bonds i-j with i < j.
+// Eigenvalues d and eigenvectors (columns of v) of a symmetric 4x4 matrix
+// by cyclic Jacobi rotations, in the steps of rmsdfun.f's jacobi (the
+// sweeps stop once the off-diagonal magnitudes sum to at most 1e-6; the
+// upper triangle of a is destroyed).
+static void jacobi4(double a[4][4], double d[4], double v[4][4]) {
+  constexpr int n = 4;
+  double b[n], z[n];
+  for (int p = 0; p < n; ++p) {
+    b[p] = d[p] = a[p][p];
+    z[p] = 0.0;
+    for (int q = 0; q < n; ++q) v[p][q] = 0.0;
+    v[p][p] = 1.0;
+  }
+  for (int it = 1; it <= 500; ++it) {
+    double sm = 0.0;
+    for (int p = 0; p < n - 1; ++p)
+      for (int q = p + 1; q < n; ++q) sm += std::fabs(a[p][q]);
+    if (sm <= 1e-6) return;
+    const double tresh = it < 4 ? 0.2 * sm / (n * n) : 0.0;
+    for (int p = 0; p < n - 1; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        double g = 100.0 * std::fabs(a[p][q]);
+        if (it > 4 && std::fabs(d[p]) + g == std::fabs(d[p]) && std::fabs(d[q]) + g == std::fabs(d[q])) {
+          a[p][q] = 0.0;
+        } else if (std::fabs(a[p][q]) > tresh) {
+          double h = d[q] - d[p], t;
+          if (std::fabs(h) + g == std::fabs(h)) {
+            t = a[p][q] / h;
+          } else {
+            const double theta = 0.5 * h / a[p][q];
+            t = 1.0 / (std::fabs(theta) + std::sqrt(1.0 + theta * theta));
+            if (theta < 0.0) t = -t;
+          }
+          const double c = 1.0 / std::sqrt(1.0 + t * t), sn = t * c, tau = sn / (1.0 + c);
+          h = t * a[p][q];
+          z[p] -= h;
+          z[q] += h;
+          d[p] -= h;
+          d[q] += h;
+          a[p][q] = 0.0;
+          auto rot = [&](double& x, double& y) {
+            const double gx = x, hy = y;
+            x = gx - sn * (hy + gx * tau);
+            y = hy + sn * (gx - hy * tau);
+          };
+          for (int j = 0; j < p; ++j) rot(a[j][p], a[j][q]);
+          for (int j = p + 1; j < q; ++j) rot(a[p][j], a[j][q]);
+          for (int j = q + 1; j < n; ++j) rot(a[p][j], a[q][j]);
+          for (int j = 0; j < n; ++j) rot(v[j][p], v[j][q]);
+        }
+      }
+    for (int p = 0; p < n; ++p) {
+      b[p] += z[p];
+      d[p] = b[p];
+      z[p] = 0.0;
+    }
+  }
+  throw Error("jacobi> too many iterations");
+}
+
+double rmsdfun(const State& s, int nml, int ir1, int ir2, const int32_t* ixat, const double* xrf, const double* yrf,
+               const double* zrf, int isl) {
+  if (nml < 1 || nml > s.ntlml) throw Error("rmsdopt> Sorry, there is no molecule #" + std::to_string(nml));
+  // atom selection of rmsdopt (rmsdfun.f:64-99): residues ir1..ir2 of chain
+  // nml (counted within the chain), atoms with a reference (ixat of the
+  // running atom number > 0), no hydrogens; isl 1 backbone n, ca, c, 2 ca
+  std::vector<double> x1, x2;
+  int na = 0;
+  for (int im = 1; im <= s.ntlml; ++im) {
+    int nr = 0;
+    for (int ir = s.irsml1(im); ir <= s.irsml2(im); ++ir) {
+      if (im == nml) ++nr;
+      for (int ia = s.iatrs1(ir); ia <= s.iatrs2(ir); ++ia) {
+        ++na;
+        if (im != nml || nr < ir1 || nr > ir2) continue;
+        const int ix = ixat[na - 1];
+        const std::string nm = s.nmat.get(ia);
+        if (ix <= 0 || nm[0] == 'h') continue;
+        const bool take = isl == 0 || (isl == 1 && (nm == "n   " || nm == "ca  " || nm == "c   ")) ||
+                          (isl == 2 && nm == "ca  ");
+        if (!take) continue;
+        x1.insert(x1.end(), {s.xat(ia), s.yat(ia), s.zat(ia)});
+        x2.insert(x2.end(), {xrf[ix - 1], yrf[ix - 1], zrf[ix - 1]});
+      }
+    }
+  }
+  const int n = (int)x1.size() / 3;
+  if (n < 3) throw Error("rmsdopt> <3 atoms selected !");
+  // fitmol (rmsdfun.f:104-175): the smallest eigenvalue of Kearsley's
+  // quaternion matrix of the centred sets is n rmsd^2
+  const double dn = (double)n;
+  double a1[3] = {0.0, 0.0, 0.0}, a2[3] = {0.0, 0.0, 0.0};
+  for (int i = 0; i < 3; ++i) {
+    for (int j = 0; j < n; ++j) {
+      a1[i] += x1[3 * j + i];
+      a2[i] += x2[3 * j + i];
+    }
+    a1[i] /= dn;
+    a2[i] /= dn;
+  }
+  double q[4][4] = {};
+  for (int j = 0; j < n; ++j) {
+    double dm[3], dp[3];
+    for (int i = 0; i < 3; ++i) {
+      dm[i] = x1[3 * j + i] - a1[i];
+      dp[i] = x2[3 * j + i] - a2[i];
+    }
+    const double xm = dp[0] - dm[0], ym = dp[1] - dm[1], zm = dp[2] - dm[2];
+    const double xp = dp[0] + dm[0], yp = dp[1] + dm[1], zp = dp[2] + dm[2];
+    q[0][0] += xm * xm + ym * ym + zm * zm;
+    q[0][1] += yp * zm - ym * zp;
+    q[0][2] += xm * zp - xp * zm;
+    q[0][3] += xp * ym - xm * yp;
+    q[1][1] += yp * yp + zp * zp + xm * xm;
+    q[1][2] += xm * ym - xp * yp;
+    q[1][3] += xm * zm - xp * zp;
+    q[2][2] += xp * xp + zp * zp + ym * ym;
+    q[2][3] += ym * zm - yp * zp;
+    q[3][3] += xp * xp + yp * yp + zm * zm;
+  }
+  for (int i = 0; i < 3; ++i)
+    for (int j = i + 1; j < 4; ++j) q[j][i] = q[i][j];
+  double e[4], v[4][4];
+  jacobi4(q, e, v);
+  double em = e[0];
+  for (int i = 1; i < 4; ++i) em = std::min(em, e[i]);
+  return std::sqrt(em / dn);
+}
+
 void outpdb(const State& s, int nml, const std::string& path) {
   if (nml < 0 || nml > s.ntlml) throw Error("outpdb> No such molecule #" + std::to_string(nml));
   const int im1 = nml > 0 ? nml : 1, im2 = nml > 0 ? nml : s.ntlml;

@@ -174,6 +174,10 @@ int hbond(State& s, int nml);                                 // hbond.f:14-216 
 int interhbond(State& s);                                     // hbond.f:228-256 (mhb; mmhb)
 void zimmer(State& s, int nresi);                             // zimmer.f:15-93 (zimm)
 void outpdb(const State& s, int nml, const std::string& path);  // outpdb.f:12-185 ('' = stdout)
+// rmsdfun.f:12-261: rmsd of chain nml's residues ir1..ir2 after the optimal
+// superposition on the reference (ixat: reference index per atom, 1-based)
+double rmsdfun(const State& s, int nml, int ir1, int ir2, const int32_t* ixat, const double* xrf, const double* yrf,
+               const double* zrf, int isl);
 void bldmol(State& s, int nml);             // bldmol.f:13-93
 // fnd3ba (bldmol.f:95-173) on the current topology; ixrfpt keeps the atoms
 // of the last bldmol call, which precede addend's renumbering after init.

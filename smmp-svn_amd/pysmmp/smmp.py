@@ -15,8 +15,8 @@ Differences from the f2py module (each stated where it applies):
     GPU energy per trial);
   * ``partem_s`` is exported (partem_s.f:13 has no f2py entry);
   * observables rgyr, helix, hbond, interhbond, zimmer, outpdb and the
-    canon driver are provided, and minim / gradient in vacuum; contacts,
-    rmsdfun, regul and anneal raise NotImplementedError.
+    canon driver are provided, rmsdfun, and minim / gradient in vacuum;
+    contacts, regul and anneal raise NotImplementedError.
 """
 from __future__ import annotations
 
@@ -387,6 +387,13 @@ def canon(nequi: int, nswp: int, nmes: int, temp: float, lrand: bool = True) -> 
     outpdb(0, "final.pdb")
 
 
+def rmsdfun(nml: int, ir1: int, ir2: int, ixat, xrf, yrf, zrf, isl: int) -> float:
+    """smmp.pyf:4095, rmsdfun.f:12: rmsd of residues ir1..ir2 of molecule nml
+    from the reference xrf, yrf, zrf (ixat: reference index per atom) after
+    the optimal superposition; isl 0 heavy atoms, 1 backbone, 2 C-alpha."""
+    return _ctx.rmsdfun(nml, ir1, ir2, ixat, xrf, yrf, zrf, isl)
+
+
 def minim(imin: int, maxit: int, eps: float) -> None:
     """smmp.pyf:1234, minim.f:13: local minimisation (imin 1 quasi-Newton,
     2 conjugate gradients) of the ECEPP/3 energy by the free variables, at
@@ -402,7 +409,7 @@ def gradient() -> None:
     _ctx.gradient()
 
 
-for _n in ("contacts", "rmsdfun", "anneal", "regul", "esolan", "eninteract"):
+for _n in ("contacts", "anneal", "regul", "esolan", "eninteract"):
     globals()[_n] = _outside(_n)
 
 

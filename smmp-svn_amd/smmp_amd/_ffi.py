@@ -80,6 +80,7 @@ def _load():
         "smmp_interhbond": (C.c_int, [P, ip]),
         "smmp_zimmer": (C.c_int, [P, C.c_int]),
         "smmp_outpdb": (C.c_int, [P, C.c_int, C.c_char_p]),
+        "smmp_rmsdfun": (C.c_int, [P, C.c_int, C.c_int, C.c_int, ip, dp, dp, dp, C.c_int, dp]),
         "smmp_gradient": (C.c_int, [P]),
         "smmp_gradient_batch": (C.c_int, [P, C.c_int, dp, dp, dp]),
         "smmp_minim": (C.c_int, [P, C.c_int, C.c_int, C.c_double, ip]),
@@ -121,7 +122,7 @@ lib = _load()
 EXPORTS = ("smmp_ctx_create", "smmp_ctx_destroy", "smmp_last_error", "smmp_state_view", "smmp_state_count",
            "smmp_state_name", "smmp_init_energy", "smmp_init_molecule", "smmp_setvar", "smmp_energy",
            "smmp_enysol", "smmp_sgrnd", "smmp_grnd", "smmp_outvar", "smmp_metropolis", "smmp_partem_s", "smmp_rgyr", "smmp_redvar", "smmp_helix",
-           "smmp_hbond", "smmp_interhbond", "smmp_zimmer", "smmp_outpdb",
+           "smmp_hbond", "smmp_interhbond", "smmp_zimmer", "smmp_outpdb", "smmp_rmsdfun",
            "smmp_gradient", "smmp_gradient_batch", "smmp_minim", "smmp_minim_batch", "smmp_minqsn", "smmp_mincjg",
            "smmp_energy_batch", "smmp_energy_batch_noalloc", "smmp_batch_create", "smmp_batch_load", "smmp_batch_run", "smmp_batch_read",
            "smmp_batch_destroy", "smmp_sample", "smmp_build_info", "smmp_profile", "smmp_profile_read",
@@ -390,6 +391,15 @@ class Context:
         k = np.zeros(n, dtype=np.int32)
         self.check(lib.smmp_minim_batch(self.h, int(imin), n, dptr(vl), int(maxit), float(eps), dptr(f), iptr(k)))
         return vl, f, k
+
+    def rmsdfun(self, nml: int, ir1: int, ir2: int, ixat, xrf, yrf, zrf, isl: int) -> float:
+        """rmsdfun.f: rmsd after superposition on the reference coordinates."""
+        ix = np.ascontiguousarray(ixat, dtype=np.int32)
+        xs = [np.ascontiguousarray(a, dtype=np.float64) for a in (xrf, yrf, zrf)]
+        out = C.c_double(0.0)
+        self.check(lib.smmp_rmsdfun(self.h, int(nml), int(ir1), int(ir2), iptr(ix), dptr(xs[0]), dptr(xs[1]),
+                                    dptr(xs[2]), int(isl), C.byref(out)))
+        return float(out.value)
 
     def outpdb(self, nml: int, path: str = "") -> None:
         self.check(lib.smmp_outpdb(self.h, int(nml), path.encode()))

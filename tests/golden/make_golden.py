@@ -401,6 +401,25 @@ def grad_fixtures(tmp: str) -> None:
     print(f"wrote {path} ({len(rows)} steps)")
 
 
+def rmsd_fixtures(tmp: str) -> None:
+    """rmsdfun.f (quaternion superposition, rmsdfun/rmsdopt/fitmol/jacobi)
+    of a start conformation against a turned, shifted and perturbed copy."""
+    enk = [s.lower() for s in open(os.path.join(HERE, "enkefa.seq")).read().split()[2:]]
+    t_enk = topo_fixture("enkefa", enk, tmp, nconf=8, seed=11)
+    vf = vars_file(os.path.join(tmp, "enk.vars"), t_enk, os.path.join(HERE, "enkefa.var"), False)
+    seq10 = list(np.load(os.path.join(HERE, "topo_len10.npz"))["seq_in"])
+    seqf10 = workloads.write_seq_file(os.path.join(tmp, "rmsd_len10.seq"), seq10, "len10")
+    out = {}
+    for tag, sf, varf, seed in (("enkefa", os.path.join(tmp, "enkefa.seq"), vf, 31),
+                                ("len10", seqf10, "-", 32)):
+        r = run(["rmsd", LIBDIR, sf, varf, "0", str(seed)], tmp)
+        out[f"{tag}_ref"] = np.asarray(r["rmsd_ref"][0]).reshape(-1, 3)
+        out[f"{tag}_ixat"] = np.asarray(r["rmsd_ixat"][0], dtype=np.int32)
+        out[f"{tag}_rmsd"] = np.asarray(r["rmsd"][0])
+        out[f"{tag}_xyz"] = np.asarray(r["xyz"][0]).reshape(-1, 3)
+    save("rmsd.npz", **out)
+
+
 def main(only: list[str]) -> None:
     if not os.path.exists(DRIVER):
         raise SystemExit("build the reference first: make -C oracle ref")
@@ -436,6 +455,8 @@ def main(only: list[str]) -> None:
         metro_fixtures(tmp)
     if not only or "grad" in only:
         grad_fixtures(tmp)
+    if not only or "rmsd" in only:
+        rmsd_fixtures(tmp)
     if only and "restart" in only:  # the len10 run of partem_ext_len10.npz again, for its files
         seq = list(np.load(os.path.join(HERE, "topo_len10.npz"))["seq_in"])
         seqf = workloads.write_seq_file(os.path.join(tmp, "pt_len10.seq"), seq, "len10")

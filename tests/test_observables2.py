@@ -9,7 +9,7 @@ import tempfile
 import numpy as np
 import pytest
 
-from conftest import golden, make_ctx, write_multi_seq
+from conftest import GOLDEN, golden, make_ctx, write_multi_seq
 
 TAGS = ("c2_20mer", "len10", "pair")
 
@@ -83,3 +83,60 @@ def test_hbond_counts_a_planted_helix_bond():
     c.hbond(1)
     assert int(c.view("h_bond.ntyhb")[0]) == n_types - 1
     assert n0 >= 0
+
+
+@pytest.mark.parametrize("tag", ["enkefa", "len10"])
+def test_rmsdfun_matches_reference(tag):
+    """rmsdfun.f (Kearsley's quaternion superposition with its own Jacobi)
+    against the Fortran's values (tests/golden/rmsd.npz: the start
+    conformation against a turned, shifted and perturbed copy, some atoms
+    without a reference; isl 0, 1, 2 over the whole chain and residues
+    2-4), and against the closed form: the rmsd after the optimal rotation
+    from the singular value decomposition (Kabsch)."""
+    g = golden("rmsd.npz")
+    if tag == "enkefa":
+        c = make_ctx(tag="enkefa", itysol=0, varfile=os.path.join(GOLDEN, "enkefa.var"))
+    else:
+        c = make_ctx(tag="len10", itysol=0)
+    nml = 1
+    c.setvar(nml, c.vlvr())
+    v = c.view
+    nat = len(g[f"{tag}_ixat"])
+    # xat, yat, zat have lower bound -1 (INCL.H): atom a at index a + 1
+    xyz = np.stack([v("atm_r.xat")[2:nat + 2], v("atm_r.yat")[2:nat + 2], v("atm_r.zat")[2:nat + 2]], 1)
+    assert np.abs(xyz - g[f"{tag}_xyz"]).max() < 1e-10
+    ref, ix = g[f"{tag}_ref"], g[f"{tag}_ixat"]
+    nres = int(v("mol_i.irsml2")[0] - v("mol_i.irsml1")[0] + 1)
+    names = [bytes(np.asarray(v("atm_c.nmat")[:, a]).tobytes()).decode() if v("atm_c.nmat").ndim == 2
+             else str(v("atm_c.nmat")[a]) for a in range(nat)]
+    k = 0
+    for isl in (0, 1, 2):
+        for ir1, ir2 in ((1, nres), (2, min(4, nres))):
+            r = c.rmsdfun(nml, ir1, ir2, ix, ref[:, 0], ref[:, 1], ref[:, 2], isl)
+            assert abs(r - g[f"{tag}_rmsd"][k]) < 1e-10, (isl, ir1, ir2, r, g[f"{tag}_rmsd"][k])
+            k += 1
+            # the closed form on the same selection
+            sel = []
+            for res in range(ir1, ir2 + 1):
+                a1, a2 = int(v("res_i.iatrs1")[res - 1]), int(v("res_i.iatrs2")[res - 1])
+                for a in range(a1, a2 + 1):
+                    nm = names[a - 1]
+                    ok = ix[a - 1] > 0 and not nm.startswith("h") and (
+                        isl == 0 or (isl == 1 and nm.strip() in ("n", "ca", "c")) or (isl == 2 and nm.strip() == "ca"))
+                    if ok:
+                        sel.append(a - 1)
+            P = xyz[sel] - xyz[sel].mean(0)
+            Qr = ref[ix[sel] - 1] - ref[ix[sel] - 1].mean(0)
+            U, S, Vt = np.linalg.svd(P.T @ Qr)
+            dsign = np.sign(np.linalg.det(U @ Vt))
+            S[-1] *= dsign
+            r_closed = np.sqrt(max(((P ** 2).sum() + (Qr ** 2).sum() - 2 * S.sum()) / len(sel), 0.0))
+            assert abs(r - r_closed) < 1e-6, (r, r_closed)
+
+
+def test_rmsdfun_needs_three_atoms():
+    from smmp_amd import _ffi
+    c = make_ctx(tag="len10", itysol=0)
+    nat = 149
+    with pytest.raises(_ffi.SmmpError, match="3 atoms"):
+        c.rmsdfun(1, 1, 1, np.zeros(nat, dtype=np.int32), np.zeros(nat), np.zeros(nat), np.zeros(nat), 0)
