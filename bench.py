@@ -422,6 +422,9 @@ def c3_headline(K: int) -> dict:
     for prec, name, peak in ((1, "mixed", PEAK_FP32), (0, "fp64", PEAK_FP64)):
         B.run(prec)
         B.read()
+        shapes = B.tune(prec)  # the autotuner's launch shapes
+        B.run(prec)
+        B.read()
         ctxs[0].profile(True)
         t0 = time.perf_counter()
         for _ in range(K):
@@ -434,6 +437,7 @@ def c3_headline(K: int) -> dict:
         flops = FLOP_PAIR * wk[0] + FLOP_POINT * 122 * wk[1] + FLOP_BUILD * wk[2]
         ach = flops / (ke["ms"] * 1e-3) / 1e12
         out[name] = {"evals_per_s": round(K * n / dt, 1), "k_energy_ms_per_batch": round(ke["ms"] / K, 3),
+                     "launch_shapes": shapes,
                      "roofline_frac": round(ach / peak, 5), "achieved_tflops": round(ach, 3), "peak_tflops": peak}
     B.close()
     for c in ctxs:
@@ -537,9 +541,18 @@ def run_c3(args):
     B = _ffi.EnergyBatch(ctxs, cs)  # systems uploaded once; conformations resident
     B.load(V)
     res = {}
+    shapes = {}
     for prec in (1, 0):
         B.run(prec)
         B.read()  # untimed first run
+        t0 = time.perf_counter()
+        for _ in range(K):
+            B.run(prec)
+        B.read()
+        dt_fixed = time.perf_counter() - t0
+        shapes[prec] = B.tune(prec)  # the autotuner's launch shapes for this batch
+        B.run(prec)
+        B.read()
         ctxs[0].profile(True)
         t0 = time.perf_counter()
         for _ in range(K):
@@ -548,6 +561,8 @@ def run_c3(args):
         dt = time.perf_counter() - t0
         ke = ctxs[0].profile_read(_ffi.KERNEL_ENERGY)
         ctxs[0].profile(False)
+        log(f"precision {prec}: tuned shapes (threads, pair bits in LDS) per class {shapes[prec]}: "
+            f"{1e3 * dt / K:.2f} ms per run, fixed rule {1e3 * dt_fixed / K:.2f} ms")
         # the host-buffer entry point (conformations in, energies out over PCIe each call)
         _ffi.energy_batch(ctxs, cs, V, prec)
         t1 = time.perf_counter()
@@ -626,6 +641,8 @@ def run_c3(args):
                      "avg_launch_ms": round(kem["ms"] / max(kem["launches"], 1), 4),
                      "flop_per_launch": round(fl / max(kem["launches"] / K, 1), 1), **rm},
         "kernel_evals_per_s": round(K * n / (kem["ms"] * 1e-3), 1),
+        "launch_shapes": {"mixed": shapes[1], "fp64": shapes[0],
+                          "note": "(threads per workgroup, pair bits in LDS) per LDS class, by smmp_batch_tune"},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

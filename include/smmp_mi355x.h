@@ -234,6 +234,17 @@ int smmp_batch_create(smmp_ctx *const *ctxs, int nsys, const int32_t *conf_sys, 
 int smmp_batch_load(smmp_batch *b, const double *vlvr, int64_t nvals);
 int smmp_batch_run(smmp_batch *b, int precision);
 int smmp_batch_read(smmp_batch *b, double *out);
+/* The energy kernel's launch shapes for this batch picked by measurement
+ * (the project's autotuner; no counterpart in the reference): for each of
+ * the batch's LDS classes, 256, 512 and 1024 threads per workgroup with the
+ * pair bit rows staged in LDS (where they fit) or read through the caches
+ * are timed on the loaded conformations and the fastest is kept for later
+ * smmp_batch_run calls at this precision (until the systems' layout
+ * changes).  The shapes change the order of the FP64 sums, so energies may
+ * differ from untuned runs in the last bits.  shapes (may be NULL): threads
+ * and bits flag per class (2 ints each, at most 2 classes); nclasses (may
+ * be NULL): the number of classes. */
+int smmp_batch_tune(smmp_batch *b, int precision, int32_t *shapes, int32_t *nclasses);
 void smmp_batch_destroy(smmp_batch *b);
 
 /* Independent-stream replica sampler: R replicas of the context's system,

@@ -772,6 +772,9 @@ struct EnergyBatch {
   bool analytic = false;
   bool loaded = false;  // conformations on the device match the current layout
   std::vector<EnergyClass> classes;
+  // measured launch shapes per precision (engine_batch_tune; empty: the
+  // classes' own), dropped when the layout changes
+  std::vector<EnergyClass> tuned[2];
   hipStream_t side = nullptr;
   hipEvent_t fork = nullptr, join = nullptr;
   DevMem d_sys, d_cs, d_off, d_order, d_vl, d_out, d_xyz;
@@ -812,6 +815,8 @@ static void batch_layout(EnergyBatch& B) {
   }
   std::vector<int32_t> order;
   B.classes = plan_classes(dsys, B.conf_sys, order);
+  B.tuned[0].clear();
+  B.tuned[1].clear();
   DevSys* d_systems = static_cast<DevSys*>(B.d_sys.ensure(sizeof(DevSys) * nsys));
   int32_t* d_cs = static_cast<int32_t*>(B.d_cs.ensure(sizeof(int32_t) * std::max(B.nconf, 1)));
   int64_t* d_off = static_cast<int64_t*>(B.d_off.ensure(sizeof(int64_t) * std::max(B.nconf, 1)));
@@ -937,7 +942,8 @@ void engine_batch_run(EnergyBatch* b, int precision) {
   double* d_work = E.prof.on ? static_cast<double*>(E.d_work.ensure(sizeof(double) * 4 * B.nconf)) : nullptr;
   if (d_work) HIPCHK(hipMemsetAsync(d_work, 0, sizeof(double) * 4 * B.nconf, E.stream));
   prof_mark(E, kKEnergy, true);
-  const bool two = B.classes.size() > 1;
+  const std::vector<EnergyClass>& classes = B.tuned[precision].empty() ? B.classes : B.tuned[precision];
+  const bool two = classes.size() > 1;
   if (two) {
     if (!B.side) {
       HIPCHK(hipStreamCreateWithFlags(&B.side, hipStreamNonBlocking));
@@ -947,8 +953,8 @@ void engine_batch_run(EnergyBatch* b, int precision) {
     HIPCHK(hipEventRecord(B.fork, E.stream));
     HIPCHK(hipStreamWaitEvent(B.side, B.fork, 0));
   }
-  for (size_t k = 0; k < B.classes.size(); ++k) {
-    const EnergyClass& C = B.classes[k];
+  for (size_t k = 0; k < classes.size(); ++k) {
+    const EnergyClass& C = classes[k];
     launch_energy_on(k == 0 ? E.stream : B.side, d_systems, d_cs, d_off, d_vl, d_out, d_xyz, B.max_nat, C.count,
                      d_order + C.begin, C.plan, precision, d_err, d_work, C.threads);
   }
@@ -968,6 +974,83 @@ void engine_batch_run(EnergyBatch* b, int precision) {
     add_esolan(E, d_systems, hsys, B.conf_sys, d_cs, B.nconf, B.max_nat, d_xyz, d_out,
                [&](int c) { return host_grnd(*B.states[B.conf_sys[c]]); });
   }
+}
+
+// Measures, for each class of the batch alone, launches of 256, 512 and
+// 1024 threads with the pair bits in LDS (where they fit) or read through
+// the caches, and keeps the fastest for this precision (the best of two
+// timed launches each, on the batch's loaded conformations).  The shapes
+// change the order of the FP64 sums (rounding only).  out (may be null):
+// threads and bits flag per class, two ints each.
+int engine_batch_tune(EnergyBatch* b, int precision, int32_t* out) {
+  Engine& E = batch_engine(b);
+  EnergyBatch& B = *b;
+  batch_refresh(B);
+  if (!B.loaded) throw Error("energy_batch> load the conformations before tuning");
+  if (B.nconf == 0) return 0;
+  std::vector<DevSys> dsys(B.sys.size());
+  for (size_t i = 0; i < B.sys.size(); ++i) dsys[i] = B.sys[i]->d;
+  std::vector<int32_t> order(B.nconf);
+  HIPCHK(hipMemcpy(order.data(), B.d_order.p, sizeof(int32_t) * B.nconf, hipMemcpyDeviceToHost));
+  const DevSys* d_systems = static_cast<const DevSys*>(B.d_sys.p);
+  int32_t* d_err = static_cast<int32_t*>(E.d_err.ensure(sizeof(int32_t)));
+  HIPCHK(hipMemsetAsync(d_err, 0, sizeof(int32_t), E.stream));
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  std::vector<EnergyClass> best = B.classes;
+  try {
+    for (size_t k = 0; k < B.classes.size(); ++k) {
+      const EnergyClass& C0 = B.classes[k];
+      float best_ms = -1.0f;
+      for (int threads : {256, 512, 1024}) {
+        size_t with = 0, without = 0;
+        for (int j = 0; j < C0.count; ++j) {
+          const DevSys& d = dsys[B.conf_sys[order[C0.begin + j]]];
+          with = std::max(with, energy_lds_bytes(d.np, d.npnt, threads / kWave, d.nheavy, pair_bit_words(d.nat, d.words)));
+          without = std::max(without, energy_lds_bytes(d.np, d.npnt, threads / kWave, d.nheavy, 0));
+        }
+        for (int bits : {1, 0}) {
+          EnergyClass C = C0;
+          C.threads = threads;
+          C.plan.bits_lds = bits;
+          C.plan.bytes = bits ? with : without;
+          if (C.plan.bytes > kLdsBytes) continue;
+          float ms = 0.0f;
+          for (int rep = 0; rep < 3; ++rep) {  // the first launch warms up
+            HIPCHK(hipEventRecord(e0, E.stream));
+            launch_energy_on(E.stream, d_systems, static_cast<const int32_t*>(B.d_cs.p),
+                             static_cast<const int64_t*>(B.d_off.p), static_cast<const double*>(B.d_vl.p),
+                             static_cast<double*>(B.d_out.p), B.analytic ? static_cast<double*>(B.d_xyz.p) : nullptr,
+                             B.max_nat, C.count, static_cast<const int32_t*>(B.d_order.p) + C.begin, C.plan,
+                             precision, d_err, nullptr, C.threads);
+            HIPCHK(hipEventRecord(e1, E.stream));
+            HIPCHK(hipEventSynchronize(e1));
+            float t = 0.0f;
+            HIPCHK(hipEventElapsedTime(&t, e0, e1));
+            if (rep == 1 || (rep == 2 && t < ms)) ms = t;
+          }
+          if (best_ms < 0.0f || ms < best_ms) {
+            best_ms = ms;
+            best[k] = C;
+          }
+        }
+      }
+    }
+  } catch (...) {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    throw;
+  }
+  HIPCHK(hipEventDestroy(e0));
+  HIPCHK(hipEventDestroy(e1));
+  B.tuned[precision] = best;
+  if (out)
+    for (size_t k = 0; k < best.size(); ++k) {
+      out[2 * k] = best[k].threads;
+      out[2 * k + 1] = best[k].plan.bits_lds;
+    }
+  return (int)best.size();
 }
 
 void engine_batch_read(EnergyBatch* b, double* out) {

@@ -487,6 +487,16 @@ int smmp_batch_run(smmp_batch* b, int precision) {
   });
 }
 
+int smmp_batch_tune(smmp_batch* b, int precision, int32_t* shapes, int32_t* nclasses) {
+  if (!b) return SMMP_EINVAL;
+  return guarded(b->c0, SMMP_ESTATE, [&] {
+    if (precision != 0 && precision != 1) throw Error("energy_batch> precision must be 0 or 1");
+    const int n = engine_batch_tune(b->b, precision, shapes);
+    if (nclasses) *nclasses = n;
+    return SMMP_OK;
+  });
+}
+
 int smmp_batch_read(smmp_batch* b, double* out) {
   if (!b || !out) return SMMP_EINVAL;
   return guarded(b->c0, SMMP_ESTATE, [&] {

@@ -55,6 +55,7 @@ EnergyBatch* engine_batch_create(Engine*& e, std::vector<State*>& sys, const int
 void engine_batch_destroy(EnergyBatch* b);
 void engine_batch_load(EnergyBatch* b, const double* vlvr, int64_t nvals);
 void engine_batch_run(EnergyBatch* b, int precision);
+int engine_batch_tune(EnergyBatch* b, int precision, int32_t* out);  // measured launch shapes; returns the classes
 void engine_batch_read(EnergyBatch* b, double* out);
 int64_t engine_batch_values(const EnergyBatch* b);  // doubles of vlvr the batch takes
 int engine_sample(Engine*& e, State& s, const smmp_sampler_cfg& cfg, double* vlvr, double* eol, double* acc,

@@ -92,6 +92,7 @@ def _load():
         "smmp_batch_create": (C.c_int, [C.POINTER(P), C.c_int, ip, C.c_int, C.POINTER(P), C.POINTER(C.c_int64)]),
         "smmp_batch_load": (C.c_int, [P, dp, C.c_int64]),
         "smmp_batch_run": (C.c_int, [P, C.c_int]),
+        "smmp_batch_tune": (C.c_int, [P, C.c_int, ip, ip]),
         "smmp_batch_read": (C.c_int, [P, dp]),
         "smmp_batch_destroy": (None, [P]),
         "smmp_sample": (C.c_int, [P, C.POINTER(SamplerCfg), dp, dp, dp, ip, ip, dp]),
@@ -124,7 +125,7 @@ EXPORTS = ("smmp_ctx_create", "smmp_ctx_destroy", "smmp_last_error", "smmp_state
            "smmp_enysol", "smmp_sgrnd", "smmp_grnd", "smmp_outvar", "smmp_metropolis", "smmp_partem_s", "smmp_rgyr", "smmp_redvar", "smmp_helix",
            "smmp_hbond", "smmp_interhbond", "smmp_zimmer", "smmp_outpdb", "smmp_rmsdfun",
            "smmp_gradient", "smmp_gradient_batch", "smmp_minim", "smmp_minim_batch", "smmp_minqsn", "smmp_mincjg",
-           "smmp_energy_batch", "smmp_energy_batch_noalloc", "smmp_batch_create", "smmp_batch_load", "smmp_batch_run", "smmp_batch_read",
+           "smmp_energy_batch", "smmp_energy_batch_noalloc", "smmp_batch_create", "smmp_batch_load", "smmp_batch_run", "smmp_batch_tune", "smmp_batch_read",
            "smmp_batch_destroy", "smmp_sample", "smmp_build_info", "smmp_profile", "smmp_profile_read",
            "smmp_profile_phases", "smmp_muca_update", "smmp_muca_final", "smmp_sampler_info",
            "smmp_mlp_size", "smmp_mlp_layout", "smmp_mlp_set", "smmp_mlp_get", "smmp_mlp_grad", "smmp_mlp_fit")
@@ -529,6 +530,13 @@ class EnergyBatch:
 
     def run(self, precision: int = 0) -> None:
         self._ctxs[0].check(lib.smmp_batch_run(self.h, precision))
+
+    def tune(self, precision: int = 0) -> list[tuple[int, int]]:
+        """Measured launch shapes (smmp_batch_tune): (threads, bits in LDS) per class."""
+        shapes = np.zeros(4, dtype=np.int32)
+        n = C.c_int32(0)
+        self._ctxs[0].check(lib.smmp_batch_tune(self.h, precision, iptr(shapes), C.byref(n)))
+        return [(int(shapes[2 * k]), int(shapes[2 * k + 1])) for k in range(n.value)]
 
     def read(self) -> np.ndarray:
         out = np.zeros((len(self.conf_sys), 8))

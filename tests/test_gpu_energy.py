@@ -171,6 +171,35 @@ def test_resident_batch_both_lds_classes_vs_oracle():
     B.close()
 
 
+def test_batch_tuner_shapes_keep_the_energies():
+    """smmp_batch_tune: every class gets a measured shape among 256, 512,
+    1024 threads with the pair bits in LDS or through the caches; the
+    energies of the tuned launches agree with the fixed rule's to rounding
+    (the shapes only change the order of the sums) and with the oracle."""
+    from smmp_amd import _ffi
+    ctxs, cs, flat, refs = _uneven_set((10, 17, 33, 48, 60), 3, 301)
+    say("batch tuner", len(cs), "conformations")
+    B = _ffi.EnergyBatch(ctxs, cs)
+    B.load(flat)
+    B.run(0)
+    fixed = B.read()
+    for prec in (0, 1):
+        shapes = B.tune(prec)
+        assert 1 <= len(shapes) <= 2
+        assert all(t in (256, 512, 1024) and b in (0, 1) for t, b in shapes)
+    B.run(0)
+    tuned = B.read()
+    assert np.all(np.abs(tuned[:, :7] - fixed[:, :7]) <= 1e-12 * np.maximum(1.0, np.abs(fixed[:, :7])))
+    for k, ref in enumerate(refs):
+        assert np.all(np.abs(tuned[k, :7] - ref[:7]) <= comp_tol(ref, 1e-10)), k
+    B.run(1)
+    om = B.read()
+    for k, ref in enumerate(refs):
+        mag = abs(ref[2]) + abs(ref[3]) + abs(ref[4])
+        assert abs(om[k, 0] - ref[0]) <= 5e-6 * mag + 1e-6
+    B.close()
+
+
 def test_resident_batch_follows_a_topology_change(tmp_path):
     """A context re-initialised after the batch was made: the batch refuses
     to run on the old layout, takes the new one, and matches smmp_energy."""
