@@ -655,8 +655,9 @@ def run_muca(args):
     (every replica's sweep-end histogram pooled per update, Berg's recursion
     in smmp_muca_update, muca.d tables from smmp_muca_final); the timed run
     samples with those weights (muca_weight2) in mixed precision (FP32 pair
-    terms of full recomputations, FP64 chain, trial changes and decisions)
-    with the sweeps replayed from a captured HIP graph."""
+    terms in full recomputations and trial changes, FP64 sums, chain and
+    decisions) with the sweeps replayed from a captured HIP graph, then the
+    same run in FP64 and the rate of decisions the FP32 terms flip."""
     from smmp_amd import _ffi, workloads
     R, L, NB = 128, 36, 200
     tmp = tempfile.mkdtemp(prefix="smmp_muca_")
@@ -720,6 +721,26 @@ def run_muca(args):
     log(f"timed {K} sweeps: {R * K / dt:.1f} sweeps/s; kernel {ks['launches']} launches, {ks['ms']:.1f} ms; "
         f"graph replays {info['graph_replays']}; bins visited {vis.mean():.2f} ({span[0] if vis.any() else -1}.."
         f"{span[-1] if vis.any() else -1}), min/mean {flat:.3f}")
+    # the same timed run in FP64, and the acceptance decisions that the FP32
+    # pair terms flip: both precisions from the same states and streams, the
+    # first sweep at which a replica's variables differ holds its first flip
+    kw64 = dict(kw, precision=0)
+    c.sample(np.full(R, TMAX), seeds(500), W, state, **kw64)
+    t0 = time.perf_counter()
+    c.sample(np.full(R, TMAX), seeds(600), K, state, **kw64)
+    dt64 = time.perf_counter() - t0
+    S = 40
+    runs = [c.sample(np.full(R, TMAX), seeds(700), S, state, mode=2, weight=w2, precision=p, with_traj=False,
+                     snap_every=1, groups=args.groups)["snap_vlvr"] for p in (0, 1)]
+    diff = np.abs(runs[0] - runs[1]).max(axis=2) > 0.0          # [sweep][replica]
+    first = np.where(diff.any(axis=0), diff.argmax(axis=0), S)  # sweep of the first flip, or S
+    nflip = int((first < S).sum())
+    trials = int(np.minimum(first + 1, S).sum()) * c.nvr
+    flips = {"trials": trials, "flipped_replicas": nflip, "rate_per_trial": nflip / trials,
+             "sweeps": S, "replicas": R,
+             "estimate": "first differing sweep per replica; its sweep counted whole, so the rate is a lower bound"}
+    log(f"FP64 timed run: {R * K / dt64:.1f} sweeps/s (mixed {R * K / dt:.1f}); decision flips: {nflip} of {R} "
+        f"replicas within {S} sweeps, {nflip / trials:.2e} per trial")
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(REPO, "tests"))
@@ -745,7 +766,7 @@ def run_muca(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 pair terms of full recomputations + f64 chain, trial changes and decisions",
+        "dtype": "f32 pair terms (full recomputations and trial changes) + f64 sums, chain and decisions",
         "data": "synthetic: random 36-residue sequence (seed 5), random initial dihedrals, weights iterated on the "
                 "device sampler",
         "config": {"workload": f"multicanonical sampling, {R} independent replicas x {L}-residue random sequence, "
@@ -754,6 +775,8 @@ def run_muca(args):
                    "kmin": kmin, "weight_updates": args.mu_iters, "sweeps_per_update": args.mu_nup,
                    "workgroups_per_replica": info["groups"], "graph_replays": info["graph_replays"]},
         "histogram": {"bins_visited": round(float(vis.mean()), 3), "min_over_mean_visited": round(flat, 4)},
+        "fp64_same_run": {"value": round(R * K / dt64, 2), "unit": "sweeps/s", "mixed_over_fp64": round(dt64 / dt, 4)},
+        "mixed_decision_flips": flips,
         "roofline": {"bound": "valu", "achieved": round(achieved, 4), "peak": PEAK_FP64, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP64, 5), "traffic": None, "kernel": "k_sweep<mixed>",
                      "avg_launch_ms": round(ks["ms"] / max(ks["launches"], 1), 4)},

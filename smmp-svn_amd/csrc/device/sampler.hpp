@@ -350,11 +350,31 @@ __device__ __forceinline__ void pair_split(const DevSys& S, double dx, double dy
   el = qq * ri * ep_inv;
   lj = A * (r6i * r6i) - C * (hb ? r6i * r2i * r2i : r6i);
 }
+// The same pair terms in FP32 from FP64 coordinate differences (the form of
+// the full evaluation's mixed tiles).
+__device__ __forceinline__ void pair_split_f32(const DevSys& S, double dx64, double dy64, double dz64, float qq,
+                                               float A, float C, bool hb, float& el, float& lj) {
+  const float dx = (float)dx64, dy = (float)dy64, dz = (float)dz64;
+  const float r2 = dx * dx + dy * dy + dz * dz;
+  const float ri = __builtin_amdgcn_rsqf(r2);
+  const float r2i = ri * ri;
+  const float r6i = r2i * r2i * r2i;
+  float ep_inv = 1.f;
+  if (S.epsd) {  // sigmoidal dielectric (enyshe.f:129-134)
+    const float sr = (float)S.slp * (r2 * ri), plt = (float)S.plt;
+    ep_inv = 1.f / (plt - (sr * sr + 2.f * sr + 2.f) * (plt - 1.f) * expf(-sr) / 2.f);
+  }
+  el = qq * ri * ep_inv;
+  lj = A * (r6i * r6i) - C * (hb ? r6i * r2i * r2i : r6i);
+}
 
 // Energy change of the pairs between the moved range [m1,m2] (new
 // coordinates in L, old in the backup) and the fixed atoms.  The smaller
 // set is walked uniformly, the larger one across the lanes.  Work items
-// item0, item0 + stride, ... of the (row, 64-column block) grid.
+// item0, item0 + stride, ... of the (row, 64-column block) grid.  MIXED:
+// each pair's new and old terms in FP32, their difference and the sums in
+// FP64.
+template <bool MIXED>
 __device__ void pair_delta(const DevSys& S, const SamplerLds& Q, int m1, int m2, int item0, int stride, PairE& d,
                            Work& wk) {
   const EnergyLds& L = Q.L;
@@ -395,8 +415,18 @@ __device__ void pair_delta(const DevSys& S, const SamplerLds& Q, int m1, int m2,
     const bool hb = cf.y != 0.0;
     const double qq = S.conv * qa * qb;
     double eln, ljn, elo, ljo;
-    pair_split(S, xa - xb, ya - yb, za - zb, qq, A, C, hb, eln, ljn);
-    pair_split(S, xo - xb, yo - yb, zo - zb, qq, A, C, hb, elo, ljo);
+    if constexpr (MIXED) {
+      float fen, fln, feo, flo;
+      pair_split_f32(S, xa - xb, ya - yb, za - zb, (float)qq, (float)A, (float)C, hb, fen, fln);
+      pair_split_f32(S, xo - xb, yo - yb, zo - zb, (float)qq, (float)A, (float)C, hb, feo, flo);
+      eln = fen;
+      ljn = fln;
+      elo = feo;
+      ljo = flo;
+    } else {
+      pair_split(S, xa - xb, ya - yb, za - zb, qq, A, C, hb, eln, ljn);
+      pair_split(S, xo - xb, yo - yb, zo - zb, qq, A, C, hb, elo, ljo);
+    }
     const double del = on ? eln - elo : 0.0;
     const double dlj = on ? ljn - ljo : 0.0;
     d.el += del;
@@ -721,7 +751,7 @@ __device__ __attribute__((noinline)) Work spec_trial(const DevSys* sys, int nvt,
   for (int h = tid; h < nh; h += blockDim.x) Q.aff[h] = 0;
   __syncthreads();
   PairE d = {0.0, 0.0, 0.0};
-  pair_delta(S, Q, m1, m2, wave, nwaves, d, wk);
+  pair_delta<MIXED>(S, Q, m1, m2, wave, nwaves, d, wk);
   if (S.itysol > 0) mark_affected(S, Q, h1, h2, m1, 0, 1);
   __syncthreads();
   int* next_atom = reinterpret_cast<int*>(Q.sc + 30);
@@ -1285,7 +1315,7 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
         PairE d = {0.0, 0.0, 0.0};
         double* pst = P.phase ? Q.sc + 96 + 4 * wave : nullptr;  // this wave's pair timing (lane 0)
         double tpa = pst ? (double)wall_clock64() : 0.0;
-        pair_delta(S, Q, m1, m2, g + G * wave, G * nwaves, d, wk);
+        pair_delta<MIXED>(S, Q, m1, m2, g + G * wave, G * nwaves, d, wk);
         if (pst && lane == 0) {
           const double now = (double)wall_clock64();
           pst[0] += now - tpa;
@@ -1415,8 +1445,12 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
                                 vnew.sol - vref.sol};
           const double id[5] = {Q.sc[24], Q.sc[25], Q.sc[26], Q.sc[28], Q.sc[27]};
           const double scale = 1.0 + fabs(vref.el) + fabs(vref.vw) + fabs(vref.hb) + fabs(vref.sol);
+          // mixed: both sides carry FP32 pair-term rounding (each term within
+          // 2^-22 of its magnitude; the full sums' magnitudes bound the moved
+          // pairs' too)
+          const double mtol = MIXED ? 1e-5 * (vref.mag + vnew.mag) : 0.0;
           bool bad = false;
-          for (int k = 0; k < 5; ++k) bad |= fabs(fd[k] - id[k]) > 1e-9 * (scale + fabs(fd[k]));
+          for (int k = 0; k < 5; ++k) bad |= fabs(fd[k] - id[k]) > 1e-9 * (scale + fabs(fd[k])) + (k < 3 ? mtol : 0.0);
           if (bad) {
             double* d = P.dbg + 16 * r;
             d[0] = 1.0;
