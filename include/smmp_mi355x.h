@@ -144,13 +144,18 @@ int smmp_rmsdfun(smmp_ctx *ctx, int nml, int ir1, int ir2, const int32_t *ixat, 
  * (0: all) from the current coordinates ('' = stdout). */
 int smmp_outpdb(smmp_ctx *ctx, int nml, const char *path);
 
-/* gradient.f:13-70 with opeshe.f:12-388 (called by minim.f; smmp.pyf
- * exports the COMMON it fills, eny_r.gdeyvr at smmp.pyf:260): the energy of
- * the context's variables within each chain (eysm, eyel, eyvw, eyhb, eyvr;
- * eysl = 0) and its derivatives by every variable into eny_r.gdeyvr, on the
- * device in FP64.  Vacuum only: itysol > 0 is refused as the reference stops
- * there (gradient.f:59-62); the analytic solvent's derivatives (opesol.f)
- * are not provided (itysol < 0: SMMP_ESTATE). */
+/* gradient.f:13-70 with opeshe.f:12-388 and, for the analytic solvent,
+ * opesol.f:12-258 over esolan.f's Cartesian gradient (called by minim.f;
+ * smmp.pyf exports the COMMON it fills, eny_r.gdeyvr at smmp.pyf:260): the
+ * energy of the context's variables within each chain (eysm, eyel, eyvw,
+ * eyhb, eyvr, eysl; with itysol < 0 eysm includes eysl as gradient.f:47
+ * leaves it) and its derivatives by every variable into eny_r.gdeyvr, on
+ * the device in FP64.  itysol > 0 is refused as the reference stops there
+ * (gradient.f:59-62).  itysol < 0 is refused too (SMMP_ESTATE): the device
+ * form of the analytic solvent's derivatives is built but switched off until
+ * a fault of its first run is explained; when on, it takes one chain
+ * (esolan.f numbers a molecule's atoms from 1) and draws the rotations of
+ * degenerate north points from the context's twister like esolan.f:199-200. */
 int smmp_gradient(smmp_ctx *ctx);
 
 /* minim(imin,maxit,eps) smmp.pyf:1234, minim.f:13: local minimisation of
@@ -188,8 +193,8 @@ int smmp_mincjg(int n, double *x, double *f, double *g, double acur, int maxfun,
 int smmp_energy_batch(smmp_ctx *const *ctxs, int nsys, const int32_t *conf_sys, int nconf,
                       const double *vlvr, int precision, double *out);
 /* gradient.f (vacuum) for nconf full variable vectors of the context's
- * system, vlvr[nconf*nvt], in one launch: energies[5c..] = eysm, eyel, eyvw,
- * eyhb, eyvr; grad[nvt c..] = derivatives by every variable. */
+ * system, vlvr[nconf*nvt], in one launch: energies[5c..] = eysm, eyel,
+ * eyvw, eyhb, eyvr; grad[nvt c..] = derivatives by every variable. */
 int smmp_gradient_batch(smmp_ctx *ctx, int nconf, const double *vlvr, double *energies, double *grad);
 /* minim.f for nconf conformations side by side: each takes exactly the
  * steps smmp_minim would take alone from that start, and every round of

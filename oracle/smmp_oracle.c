@@ -498,8 +498,115 @@ static double pair_force(const oracle_topo *t, double cqi, double qj, double rij
  * moving set are completed by their partners' earlier subtractions, and the
  * sums of force and torque of the variables added to its set come from
  * their own rows (the recursion of Abe, Braun, Noguti and Go 1984). */
+/* opesol.f:12-258 (one chain): the analytic solvent term (esolan.f, its
+ * random rotations drawn from the solvent generator) and its derivatives
+ * by every variable, added to gd.  Forces f = -gradan and torques f x r per
+ * atom; variables in descending processing order summing their moving sets
+ * and added variables, except where the variable's base atom (or, for a
+ * bond length, its own atom) is the second reference atom of the chain
+ * frame: there the sum runs, with the opposite sign, over the atom's vdW
+ * and 1-4 partners (opesol.f:174-199).  Returns eysl (NAN on an esolan
+ * failure). */
+static double opesol(const oracle_topo *t, oracle_conf *c, double *gd) {
+  const int nat = t->nat, nvt = t->nvt;
+  double *ga = calloc(3 * (size_t)nat, sizeof(double));
+  double *fr = calloc(3 * (size_t)nat, sizeof(double));
+  double *fvr = calloc(6 * (size_t)nvt, sizeof(double));
+  if (!ga || !fr || !fvr) {
+    free(ga);
+    free(fr);
+    free(fvr);
+    return NAN;
+  }
+  const double eysl = oracle_esolan_grad(t, c, g_sol_mt, NULL, ga);
+  if (isnan(eysl)) {
+    free(ga);
+    free(fr);
+    free(fvr);
+    return NAN;
+  }
+  for (int i = 1; i <= nat; ++i) { /* f = -gradan, g = f x r */
+    double *f = ga + 3 * (i - 1), *g = fr + 3 * (i - 1);
+    f[0] = -f[0];
+    f[1] = -f[1];
+    f[2] = -f[2];
+    const double xi = A(c->xat, i), yi = A(c->yat, i), zi = A(c->zat, i);
+    g[0] = f[1] * zi - f[2] * yi;
+    g[1] = f[2] * xi - f[0] * zi;
+    g[2] = f[0] * yi - f[1] * xi;
+  }
+  const int ix2 = t->i2;
+  int i1s = t->nms + 1, i1a = t->nad + 1;
+  for (int io = nvt; io >= 1; --io) {
+    const int iv = A(t->iorvr, io), it = A(t->ityvr, iv), ia = A(t->iatvr, iv), ib = A(t->iowat, ia);
+    const double xb = A(c->xat, ib), yb = A(c->yat, ib), zb = A(c->zat, ib);
+    double ex = 0.0, ey = 0.0, ez = 0.0;
+    int lnb = 0;
+    if (it == 3) {
+      ex = A(c->xtoat, ib); ey = A(c->ytoat, ib); ez = A(c->ztoat, ib);
+      lnb = ib == ix2;
+    } else if (it == 2) {
+      ex = A(c->xbaat, ia); ey = A(c->ybaat, ia); ez = A(c->zbaat, ia);
+      lnb = ib == ix2;
+    } else if (it == 1) {
+      ex = A(c->xtoat, ia); ey = A(c->ytoat, ia); ez = A(c->ztoat, ia);
+      lnb = ia == ix2;
+    }
+    double F[3] = {0.0, 0.0, 0.0}, T[3] = {0.0, 0.0, 0.0};
+    if (!lnb) {
+      const int i2s = i1s - 1;
+      i1s = A(t->imsvr1, iv);
+      for (int ims = i1s; ims <= i2s; ++ims)
+        for (int i = A(t->latms1, ims); i <= A(t->latms2, ims); ++i)
+          for (int d = 0; d < 3; ++d) {
+            F[d] = F[d] + ga[3 * (i - 1) + d];
+            T[d] = T[d] + fr[3 * (i - 1) + d];
+          }
+      const int i2a = i1a - 1;
+      i1a = A(t->iadvr1, iv);
+      for (int iad = i1a; iad <= i2a; ++iad) {
+        const double *s = fvr + 6 * (size_t)(A(t->ladvr, iad) - 1);
+        for (int d = 0; d < 3; ++d) {
+          F[d] = F[d] + s[d];
+          T[d] = T[d] + s[3 + d];
+        }
+      }
+    } else {
+      for (int ivw = A(t->ivwat1, ia); ivw <= A(t->ivwat2, ia); ++ivw)
+        for (int j = A(t->lvwat1, ivw); j <= A(t->lvwat2, ivw); ++j)
+          for (int d = 0; d < 3; ++d) {
+            F[d] = F[d] - ga[3 * (j - 1) + d];
+            T[d] = T[d] - fr[3 * (j - 1) + d];
+          }
+      for (int i14 = A(t->i14at1, ia); i14 <= A(t->i14at2, ia); ++i14) {
+        const int j = A(t->l14at, i14);
+        for (int d = 0; d < 3; ++d) {
+          F[d] = F[d] - ga[3 * (j - 1) + d];
+          T[d] = T[d] - fr[3 * (j - 1) + d];
+        }
+      }
+    }
+    double *o = fvr + 6 * (size_t)(iv - 1);
+    for (int d = 0; d < 3; ++d) {
+      o[d] = F[d];
+      o[3 + d] = T[d];
+    }
+    double gs = 0.0;
+    if (it == 3 || it == 2)
+      gs = (ey * zb - ez * yb) * F[0] + (ez * xb - ex * zb) * F[1] + (ex * yb - ey * xb) * F[2] + ex * T[0] +
+           ey * T[1] + ez * T[2];
+    else if (it == 1)
+      gs = -(ex * F[0] + ey * F[1] + ez * F[2]);
+    gd[iv - 1] = gd[iv - 1] + gs;
+  }
+  free(ga);
+  free(fr);
+  free(fvr);
+  return eysl;
+}
+
 double oracle_gradient(const oracle_topo *t, oracle_conf *c, const double *vlvr, double *gd) {
-  if (t->itysol != 0) return NAN;
+  if (t->itysol > 0 || (t->itysol < 0 && t->nml > 1)) return NAN;
   oracle_setvar(t, c, vlvr);
   const int nat = t->nat, nvt = t->nvt;
   double *fat = calloc(3 * (size_t)nat, sizeof(double));
@@ -596,14 +703,19 @@ double oracle_gradient(const oracle_topo *t, oracle_conf *c, const double *vlvr,
   free(fat);
   free(fvr);
   const double eysm = eyel + eyvw + eyhb + eyvr;
+  double eysl = 0.0;
+  if (t->itysol < 0) { /* gradient.f:42-55 */
+    eysl = opesol(t, c, gd);
+    if (isnan(eysl)) return NAN;
+  }
   c->e[1] = eysm;
   c->e[2] = eyel;
   c->e[3] = eyvw;
   c->e[4] = eyhb;
   c->e[5] = eyvr;
-  c->e[6] = 0.0;
-  c->e[0] = eysm;
-  return eysm;
+  c->e[6] = eysl;
+  c->e[0] = eysm + eysl;
+  return eysm + eysl;
 }
 
 double oracle_energy(const oracle_topo *t, oracle_conf *c, const double *vlvr) {
@@ -938,7 +1050,106 @@ void oracle_esolan_force_rotation(double uga, double ufi) {
  * NULL).  The random rotations of esolan.f:200-232 draw from g.  Returns NAN
  * if two atom centres coincide (esolan.f:127-133 stops there). */
 double oracle_esolan(const oracle_topo *t, oracle_conf *c, oracle_mt *g, double *as_out) {
+  return oracle_esolan_grad(t, c, g, as_out, NULL);
+}
+
+/* d(area term)/d(a, b, c, d) of one circle -> d/d(x, y, z) of its atom
+ * (ddat of that atom after the rotation): the coefficients are
+ * a = x^2 + y^2 + (z - R)^2 - r^2, b = -8 R^2 x, c = -8 R^2 y,
+ * d = 4 R^2 (x^2 + y^2 + (z + R)^2 - r^2) (esolan.f:545-570, 813-862). */
+static void es_chain(const double dq[4], double x, double y, double z, double R, double out[3]) {
+  const double R2 = R * R, R42 = 4.0 * R2, m8 = -(R42 + R42);
+  out[0] += dq[0] * (2.0 * x) + dq[1] * m8 + dq[3] * (2.0 * x * R42);
+  out[1] += dq[0] * (2.0 * y) + dq[2] * m8 + dq[3] * (2.0 * y * R42);
+  out[2] += dq[0] * (2.0 * (z - R)) + dq[3] * (2.0 * (z + R) * R42);
+}
+
+/* Derivatives of one exposed arc's term (esolan.f:684-862).  The arc runs
+ * on circle q = (a, b, c, d) from polar angle alp, where circle q1 crosses it,
+ * to bet, where circle q2 does; its term depends on q's coefficients
+ * directly and through both end angles, and on q1's and q2's through alp and
+ * bet.  Returns d(term)/d(coefficients) of q, q1 and q2 in dq, dq1, dq2. */
+static void es_arc_grad(double R, double a, double b, double c, double d, double alp, double bet, const double q1[4],
+                        const double q2[4], double dq[4], double dq1[4], double dq2[4]) {
+  const double R2 = R * R, R42 = 4.0 * R2;
+  const double ak = a * a, daba = fabs(a), aia = a < 0.0 ? -1.0 : 1.0, b2c2 = b * b + c * c;
+  const double vv = sqrt((R42 * a - d) * (R42 * a - d) + R42 * b2c2), vv1 = 1.0 / vv, vv2 = vv1 * vv1;
+  const double wv = sqrt(b2c2 - 4.0 * a * d), bcd = b2c2 - 2.0 * a * d;
+  const double xv = (d + R42 * a) * vv1, yv = (bcd + 2.0 * R42 * ak) / daba * vv1, zv = wv / a * vv1;
+  const double sa = sin(alp), ca = cos(alp), sb = sin(bet), cb = cos(bet);
+  const double hm = 5e-1 * (bet - alp), hp = 5e-1 * (bet + alp);
+  const double cm = cos(hm), cp = cos(hp), sm = sin(hm), sp = sin(hp), cotm = 1.0 / tan(hm);
+  const double bcp = b * cp + c * sp, bsp = -b * sp + c * cp; /* and their derivatives by hp */
+  const double tt = yv * cotm - zv * bcp / sm;
+  const double a1 = q1[0], b1 = q1[1], c1 = q1[2], d1 = q1[3];
+  const double a2 = q2[0], b2 = q2[1], c2 = q2[2], d2 = q2[3];
+  const double ab1 = a * b1 - b * a1, ac1 = a * c1 - c * a1, ab2 = a * b2 - b * a2, ac2 = a * c2 - c * a2;
+  const double am1 = wv * aia * a1 * (ab1 * sa - ac1 * ca), am2 = wv * aia * a2 * (ab2 * sb - ac2 * cb);
+  /* the end angles by the arc circle's coefficients */
+  const double u1 = aia * a1 * (ab1 * ca + ac1 * sa) / wv, u2 = aia * a2 * (ab2 * cb + ac2 * sb) / wv;
+  double dal[4], dbe[4], dal1[4], dbe2[4];
+  dal[0] = (b1 * ab1 + c1 * ac1 - 2.0 * d * a1 * a1 - a * (b1 * b1 + c1 * c1 - 4.0 * a1 * d1) - 2.0 * d * u1 +
+            wv * aia * a1 * (b1 * ca + c1 * sa)) / am1;
+  dal[1] = (-a1 * ab1 + b * a1 * a1 + b * u1 - wv * aia * a1 * a1 * ca) / am1;
+  dal[2] = (-a1 * ac1 + c * a1 * a1 + c * u1 - wv * aia * a1 * a1 * sa) / am1;
+  dal[3] = (-2.0 * a * a1 * a1 - 2.0 * daba * a1 * (ab1 * ca + ac1 * sa) / wv) / am1;
+  dbe[0] = (b2 * ab2 + c2 * ac2 - 2.0 * d * a2 * a2 - a * (b2 * b2 + c2 * c2 - 4.0 * a2 * d2) - 2.0 * d * u2 +
+            wv * aia * a2 * (b2 * cb + c2 * sb)) / am2;
+  dbe[1] = (-a2 * ab2 + b * a2 * a2 + b * u2 - wv * aia * a2 * a2 * cb) / am2;
+  dbe[2] = (-a2 * ac2 + c * a2 * a2 + c * u2 - wv * aia * a2 * a2 * sb) / am2;
+  dbe[3] = (-2.0 * a * a2 * a2 - 2.0 * daba * a2 * (ab2 * cb + ac2 * sb) / wv) / am2;
+  /* the end angles by the crossing circles' coefficients */
+  dal1[0] = (-b * ab1 - c * ac1 + wv * wv * a1 + 2.0 * ak * d1 + wv * aia * ((ab1 - b * a1) * ca + (ac1 - c * a1) * sa)) / am1;
+  dal1[1] = (a * ab1 - ak * b1 + wv * daba * a1 * ca) / am1;
+  dal1[2] = (a * ac1 - ak * c1 + wv * daba * a1 * sa) / am1;
+  dal1[3] = (2.0 * ak * a1) / am1;
+  dbe2[0] = (-b * ab2 - c * ac2 + wv * wv * a2 + 2.0 * ak * d2 + wv * aia * ((ab2 - b * a2) * cb + (ac2 - c * a2) * sb)) / am2;
+  dbe2[1] = (a * ab2 - ak * b2 + wv * daba * a2 * cb) / am2;
+  dbe2[2] = (a * ac2 - ak * c2 + wv * daba * a2 * sb) / am2;
+  dbe2[3] = (2.0 * ak * a2) / am2;
+  /* vv, xv, yv, zv by (a, b, c, d) */
+  double dv[4], dx[4], dy[4], dz[4], dt[4];
+  dv[1] = R42 * b * vv1;
+  dv[2] = R42 * c * vv1;
+  dv[3] = (d - R42 * a) * vv1;
+  dv[0] = -dv[3] * R42;
+  const double xn = d + R42 * a, yn = b2c2 - 2.0 * a * d + 2.0 * R42 * ak;
+  dx[0] = R42 * vv1 - xn * dv[0] * vv2;
+  dx[1] = -xn * dv[1] * vv2;
+  dx[2] = -xn * dv[2] * vv2;
+  dx[3] = vv1 - xn * dv[3] * vv2;
+  dy[0] = (-2.0 * d + 4.0 * R42 * a) / daba * vv1 - yn * (aia * vv + daba * dv[0]) / ak * vv2;
+  dy[1] = 2.0 * b / daba * vv1 - yn * aia * dv[1] / a * vv2;
+  dy[2] = 2.0 * c / daba * vv1 - yn * aia * dv[2] / a * vv2;
+  dy[3] = -2.0 * a / daba * vv1 - yn * aia * dv[3] / a * vv2;
+  dz[0] = -2.0 * d / wv / a * vv1 - wv * (vv + a * dv[0]) / ak * vv2;
+  dz[1] = b / wv / a * vv1 - wv * dv[1] / a * vv2;
+  dz[2] = c / wv / a * vv1 - wv * dv[2] / a * vv2;
+  dz[3] = -2.0 / wv * vv1 - wv * dv[3] / a * vv2;
+  const double sm2 = sm * sm;
+  for (int m = 0; m < 4; ++m) {
+    const double extra = m == 1 ? cp * 2.0 * sm : (m == 2 ? sp * 2.0 * sm : 0.0); /* d(bcp)/db, d(bcp)/dc */
+    dt[m] = cotm * dy[m] - 5e-1 * yv * (dbe[m] - dal[m]) / sm2 - (bcp / sm) * dz[m] -
+            5e-1 * zv * (bsp * sm * (dal[m] + dbe[m]) - bcp * cm * (dbe[m] - dal[m]) + extra) / sm2;
+  }
+  const double w = 4.0 + tt * tt, at = PI - 2.0 * atan(5e-1 * tt);
+  for (int m = 0; m < 4; ++m) dq[m] = R2 * (aia * (dal[m] - dbe[m]) + at * dx[m] - 4.0 * xv * dt[m] / w);
+  for (int m = 0; m < 4; ++m) {
+    const double ta = 5e-1 * dal1[m] * ((yv - zv * (bsp * sm + bcp * cm)) / sm2);
+    const double tb = 5e-1 * dbe2[m] * ((-yv - zv * (bsp * sm - bcp * cm)) / sm2);
+    dq1[m] = R2 * (aia * dal1[m] - 4.0 * xv * ta / w);
+    dq2[m] = R2 * (-aia * dbe2[m] - 4.0 * xv * tb / w);
+  }
+}
+
+/* The same with the Cartesian gradient of esolan.f:533-1016: gradan (3 nat,
+ * atom j at 3 (j - 1); may be NULL) = sum over atoms i of sigma_i dAs_i/dr_j,
+ * accumulated over i in atom order as the reference sums it. */
+double oracle_esolan_grad(const oracle_topo *t, oracle_conf *c, oracle_mt *g, double *as_out, double *gradan) {
   const int numat = t->nat;
+  static int nb0[ES_MAXNB + 2];
+  double *gr = gradan ? (double *)calloc(3 * (size_t)(numat + 1), sizeof(double)) : NULL;
+  if (gradan) memset(gradan, 0, sizeof(double) * 3 * (size_t)numat);
   static int neib[ES_MAXNB + 2], neibp[ES_MAXNB + 2], fullarc[ES_MAXNB + 2], al[2 * ES_MAXNB + 2];
   static int ta2[ES_MAXNB + 2], ivrx[ES_MAXV + 2];
   static double vertex[ES_MAXV + 1][4];
@@ -954,7 +1165,7 @@ double oracle_esolan(const oracle_topo *t, oracle_conf *c, oracle_mt *g, double 
     const double R = A(t->rvdw, i), poli = R * R;
     double As = 4.0 * PI * poli;
     const double xi = A(c->xat, i), yi = A(c->yat, i), zi = A(c->zat, i);
-    int jj = 0, bad = 0;
+    int jj = 0, bad = 0, es_cov = 0, ifu_f = 0, ine_f = 0;
     for (int j = 1; j <= numat; ++j) {
       if (j == i) continue;
       double ddp = (A(c->xat, j) - xi) * (A(c->xat, j) - xi) + (A(c->yat, j) - yi) * (A(c->yat, j) - yi) +
@@ -962,6 +1173,7 @@ double oracle_esolan(const oracle_topo *t, oracle_conf *c, oracle_mt *g, double 
       if (ddp < (double)1e-10f) { /* a default-real literal in esolan.f:127 */
         free(ddat);
         free(neibor);
+        free(gr);
         return NAN;
       }
       ddp = sqrt(ddp);
@@ -974,6 +1186,7 @@ double oracle_esolan(const oracle_topo *t, oracle_conf *c, oracle_mt *g, double 
         if (jj >= ES_MAXNB) {
           free(ddat);
           free(neibor);
+          free(gr);
           return NAN;
         }
         neib[++jj] = j;
@@ -981,6 +1194,14 @@ double oracle_esolan(const oracle_topo *t, oracle_conf *c, oracle_mt *g, double 
     }
     if (bad || jj == 0) goto atom_done;
     neib[0] = jj;
+    nb0[0] = jj;
+    for (int q = 1; q <= jj; ++q) nb0[q] = neib[q];
+    if (gr) {
+      for (int q = 0; q <= jj; ++q) {
+        const int j = q == 0 ? i : nb0[q];
+        gr[3 * j] = gr[3 * j + 1] = gr[3 * j + 2] = 0.0;
+      }
+    }
     if (jj > g_es_max_nb) g_es_max_nb = jj;
     {
       const double R2 = R * R, R22 = R2 + R2, R42 = R22 + R22, R22p = R22 * PI;
@@ -1009,6 +1230,7 @@ double oracle_esolan(const oracle_topo *t, oracle_conf *c, oracle_mt *g, double 
           if (g_es_force_uga < 0.0 && !g) {
             free(ddat);
             free(neibor);
+            free(gr);
             return NAN;
           }
           double uga = g_es_force_uga >= 0.0 ? g_es_force_uga : oracle_grnd(g);
@@ -1101,6 +1323,7 @@ double oracle_esolan(const oracle_topo *t, oracle_conf *c, oracle_mt *g, double 
           }
           if (covered) {
             As = 0.0;
+            es_cov = 1;
             goto arcs_done;
           }
           if (drop == 1) {
@@ -1117,6 +1340,7 @@ double oracle_esolan(const oracle_topo *t, oracle_conf *c, oracle_mt *g, double 
             if (iv + 2 > ES_MAXV) {
               free(ddat);
               free(neibor);
+              free(gr);
               return NAN;
             }
             const double am = 2.0 * ((b1 - b2) * (b1 - b2) + (c1 - c2) * (c1 - c2));
@@ -1186,12 +1410,21 @@ double oracle_esolan(const oracle_topo *t, oracle_conf *c, oracle_mt *g, double 
       neib[0] = ine;
       fullarc[0] = ifu;
       al[0] = ifu;
+      ifu_f = ifu;
+      ine_f = ine;
       for (int q = 1; q <= ifu; ++q) { /* esolan.f:529-560 */
         const double a = NB(fullarc[q], 1), b = NB(fullarc[q], 2), cc = NB(fullarc[q], 3), d = NB(fullarc[q], 4);
         const double aia = a < 0.0 ? -1.0 : 1.0;
-        if (a != 0.0)
+        if (a != 0.0) {
           As = As - aia * R22p * (1.0 + (-d / a - R42) * fabs(a) / sqrt((R42 * a - d) * (R42 * a - d) + R42 * (b * b + cc * cc)));
-        else
+          if (gr) { /* esolan.f:547-568 */
+            const int s = fullarc[q];
+            const double div = R22 * (R22p + R22p) / pow((R42 * a - d) * (R42 * a - d) + R42 * (b * b + cc * cc), 1.5);
+            const double gp[4] = {(R42 * (b * b + cc * cc - 2.0 * a * d) + 2.0 * d * d) * div, -b * (d + R42 * a) * div,
+                                  -cc * (d + R42 * a) * div, (b * b + cc * cc - 2.0 * a * d + (R42 + R42) * a * a) * div};
+            es_chain(gp, DD(s, 2), DD(s, 3), DD(s, 4), R, gr + 3 * s);
+          }
+        } else
           As = As + R22p * d / sqrt(R42 * (b * b + cc * cc) + d * d);
       }
       for (int q = 1; q <= ine; ++q) { /* esolan.f:563-957 */
@@ -1265,6 +1498,16 @@ double oracle_esolan(const oracle_topo *t, oracle_conf *c, oracle_mt *g, double 
             const double uv = daba * (bcd + 2.0 * R42 * ak2) * dcbetmalp -
                               a * sqrt(b2c2 - 4.0 * a * d) * (b * dcbetpalp + cc * dsbetpalp);
             As = As + R2 * (aia * (alp - bet) + (d + R42 * a) * vv1 * (PI - 2.0 * atan(uv / (2.0 * ak2 * vv * dsbetmalp))));
+            if (gr) { /* esolan.f:684-862: the circles crossing at the arc's ends */
+              const int k1 = (int)vertex[ivrx[u]][3], k2 = (int)vertex[ivrx[u == nyx ? 1 : u + 1]][3];
+              const double q1[4] = {NB(k1, 1), NB(k1, 2), NB(k1, 3), NB(k1, 4)};
+              const double q2[4] = {NB(k2, 1), NB(k2, 2), NB(k2, 3), NB(k2, 4)};
+              double dq[4], dq1[4], dq2[4];
+              es_arc_grad(R, a, b, cc, d, alp, bet, q1, q2, dq, dq1, dq2);
+              es_chain(dq, DD(jk, 2), DD(jk, 3), DD(jk, 4), R, gr + 3 * jk);
+              es_chain(dq1, DD(k1, 2), DD(k1, 3), DD(k1, 4), R, gr + 3 * k1);
+              es_chain(dq2, DD(k2, 2), DD(k2, 3), DD(k2, 4), R, gr + 3 * k2);
+            }
           }
         } else { /* the circle is a line (esolan.f:858-957) */
           const double ang = atan2(b, cc), s1 = sin(ang), s2 = cos(ang);
@@ -1324,7 +1567,37 @@ double oracle_esolan(const oracle_topo *t, oracle_conf *c, oracle_mt *g, double 
           }
         }
       }
-    arcs_done:;
+    arcs_done:
+      /* esolan.f:966-1000: the atom's own derivative is minus the sum over
+       * the final list (full arcs, then circles with points); with a
+       * rotation, the final list's rows and its own turn back.  A circle
+       * that crossed others and was dropped afterwards keeps its rotated
+       * row (outside the list, as in the reference).  A covered atom has
+       * no rows. */
+      if (gr && !es_cov) {
+        double *self = gr + 3 * i;
+        for (int k3 = 0; k3 < 3; ++k3) {
+          double ss = 0.0;
+          for (int q = 1; q <= ifu_f; ++q) ss = ss + gr[3 * fullarc[q] + k3];
+          for (int q = 1; q <= ine_f; ++q) ss = ss + gr[3 * neib[q] + k3];
+          self[k3] = -ss;
+        }
+        if (jjj != 0) {
+          const double sfsg = sf * sg, sfcg = sf * cg, cfsg = cf * sg, cfcg = cf * cg;
+          for (int q = 0; q <= ifu_f + ine_f; ++q) {
+            double *v = q == 0 ? self : (q <= ifu_f ? gr + 3 * fullarc[q] : gr + 3 * neib[q - ifu_f]);
+            const double xx = v[0], yy = v[1], zz = v[2];
+            v[0] = xx * cg - yy * sfsg + zz * cfsg;
+            v[1] = yy * cf + zz * sf;
+            v[2] = -xx * sg - yy * sfcg + zz * cfcg;
+          }
+        }
+        const double sgi = A(t->sigma, i);
+        for (int q = 0; q <= nb0[0]; ++q) {
+          const int j = q == 0 ? i : nb0[q];
+          for (int k3 = 0; k3 < 3; ++k3) gradan[3 * (j - 1) + k3] = gradan[3 * (j - 1) + k3] + sgi * gr[3 * j + k3];
+        }
+      }
     }
   atom_done:
     if (as_out) A(as_out, i) = As;

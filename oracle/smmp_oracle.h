@@ -81,6 +81,10 @@ double oracle_enysol(const oracle_topo *t, oracle_conf *c);
  * (projection method) of all atoms; returns sum sigma_i As_i, As (nat, may
  * be NULL).  Random rotations draw from g; NAN on coinciding atoms. */
 double oracle_esolan(const oracle_topo *t, oracle_conf *c, oracle_mt *g, double *as_out);
+/* oracle_esolan with the Cartesian gradient of esolan.f:533-1016: gradan
+ * (3 nat, atom j at 3 (j - 1), may be NULL) = sum over atoms i of
+ * sigma_i dAs_i/dr_j, the reference's gradan. */
+double oracle_esolan_grad(const oracle_topo *t, oracle_conf *c, oracle_mt *g, double *as_out, double *gradan);
 /* Largest neighbour and vertex counts of the last oracle_esolan call. */
 void oracle_esolan_stats(int *max_nb, int *max_iv);
 /* The generator oracle_energy gives esolan (itysol < 0); NULL: a draw gives NAN. */
@@ -89,12 +93,15 @@ void oracle_set_solvent_mt(oracle_mt *g);
 void oracle_esolan_force_rotation(double uga, double ufi);
 /* eninteract.f:14-86: pairs between chains; fills ei[], returns eysmi. */
 double oracle_eninteract(const oracle_topo *t, oracle_conf *c);
-/* gradient.f:13-70 with opeshe.f:12-388 (vacuum, itysol = 0): applies the
- * variables, fills e[1..6] (eysm, eyel, eyvw, eyhb, eyvr, eysl = 0) and the
- * derivatives gd[nvt] of eysm by every variable; the terms between chains
- * are not part of it (gradient.f sums opeshe over the chains).  Returns
- * eysm, or NAN (gd untouched) when itysol != 0: the reference stops for
- * itysol > 0 and its analytic-solvent derivatives are not restated here. */
+/* gradient.f:13-70 with opeshe.f:12-388 and, for itysol < 0 (one chain),
+ * opesol.f:12-258 over esolan.f's gradient: applies the variables, fills
+ * e[0..6] (eysm + eysl, eysm, eyel, eyvw, eyhb, eyvr, eysl) and the
+ * derivatives gd[nvt] of eysm + eysl by every variable; the terms between
+ * chains are not part of it (gradient.f sums opeshe over the chains).
+ * esolan's random rotations draw from the generator of
+ * oracle_set_solvent_mt.  Returns eysm + eysl, or NAN when itysol > 0 (the
+ * reference stops), for several chains with itysol < 0, or when esolan
+ * fails. */
 double oracle_gradient(const oracle_topo *t, oracle_conf *c, const double *vlvr, double *gd);
 /* energy.f: setvar + enyshe + solvent + eninteract; returns total. */
 double oracle_energy(const oracle_topo *t, oracle_conf *c, const double *vlvr);

@@ -109,7 +109,7 @@ __global__ void __launch_bounds__(kGradThreads) k_gradient(const DevSys* __restr
                                                          const int64_t* __restrict__ conf_off,
                                                          const double* __restrict__ vlvr, double* __restrict__ out,
                                                          double* __restrict__ grad, double* __restrict__ work,
-                                                         int bits_lds) {
+                                                         int bits_lds, double* __restrict__ xyz_out, int xyz_stride) {
   extern __shared__ double smem[];
   const int c = blockIdx.x;
   const DevSys& S = systems[conf_sys[c]];
@@ -121,6 +121,14 @@ __global__ void __launch_bounds__(kGradThreads) k_gradient(const DevSys* __restr
   energy_stage_torsions(S, vl, L);
   __syncthreads();
   build_chain(S, L, wk, S.gbpr);
+  if (xyz_out) {  // the coordinates for the solvent term (x, y, z per atom)
+    double* o = xyz_out + (size_t)3 * xyz_stride * c;
+    for (int a = threadIdx.x; a < S.nat; a += blockDim.x) {
+      o[3 * a] = L.x[a];
+      o[3 * a + 1] = L.y[a];
+      o[3 * a + 2] = L.z[a];
+    }
+  }
   double* f = L.fr;  // the frames are dead once the chain is built
   double el = 0.0, vw = 0.0, hb = 0.0;
   pair_forces(S, L, f, el, vw, hb, wk);
@@ -148,6 +156,53 @@ __global__ void __launch_bounds__(kGradThreads) k_gradient(const DevSys* __restr
       work[4 * (size_t)c + 1] = 0.0;
       work[4 * (size_t)c + 2] = (double)S.nat;
       work[4 * (size_t)c + 3] = 0.0;
+    }
+  }
+}
+
+// The analytic solvent's part of dE/dv (opesol.f:74-247), added to grad:
+// forces f = -gradan per atom, and per variable, as in the vacuum kernel,
+// s [(e x b) . F + e . T] over its moved atoms (F the sum of f, T of f x r;
+// the moved range is the part before the axis where var_sign is -1: the
+// term does not change when the whole chain turns).  One workgroup per
+// conformation; waves over variables, lanes over atoms; coordinates and
+// gradan at xyz / gradan + 3 (stride c + atom).
+__global__ void __launch_bounds__(kGradThreads) k_sol_project(const DevSys* __restrict__ systems,
+                                                            const int32_t* __restrict__ conf_sys,
+                                                            const int64_t* __restrict__ conf_off,
+                                                            const double* __restrict__ xyz,
+                                                            const double* __restrict__ gradan, int stride,
+                                                            double* __restrict__ grad) {
+  const int c = blockIdx.x;
+  const DevSys& S = systems[conf_sys[c]];
+  const int nw = blockDim.x / kWave, wave = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  const double* X = xyz + (size_t)3 * stride * c;
+  const double* F = gradan + (size_t)3 * stride * c;
+  for (int v = wave; v < S.nvt; v += nw) {
+    const int m1 = S.var_m1[v], m2 = S.var_m2[v];
+    double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int a = m1 + lane; a <= m2; a += kWave) {
+      const double fx = -F[3 * a], fy = -F[3 * a + 1], fz = -F[3 * a + 2];
+      const double x = X[3 * a], y = X[3 * a + 1], z = X[3 * a + 2];
+      s[0] += fx;
+      s[1] += fy;
+      s[2] += fz;
+      s[3] += fy * z - fz * y;
+      s[4] += fz * x - fx * z;
+      s[5] += fx * y - fy * x;
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) s[k] = wave_sum(s[k]);
+    if (lane == 0) {
+      const int b = S.var_base[v], p = S.var_axis0[v];
+      const double xb = X[3 * b], yb = X[3 * b + 1], zb = X[3 * b + 2];
+      double ex = xb - X[3 * p], ey = yb - X[3 * p + 1], ez = zb - X[3 * p + 2];
+      const double en = rsqrt_f64(ex * ex + ey * ey + ez * ez);
+      ex *= en;
+      ey *= en;
+      ez *= en;
+      grad[conf_off[c] + v] += S.var_sign[v] * ((ey * zb - ez * yb) * s[0] + (ez * xb - ex * zb) * s[1] +
+                                               (ex * yb - ey * xb) * s[2] + ex * s[3] + ey * s[4] + ez * s[5]);
     }
   }
 }
@@ -326,6 +381,7 @@ struct Engine {
   DevSystem sys;
   DevMem d_sys, d_confsys, d_confoff, d_vlvr, d_out, d_xyz, d_err, d_work;
   DevMem g_cs, g_off, g_vl, g_out, g_grad;  // gradient batches
+  DevMem g_xyz, g_rows, g_gradan;           // their analytic solvent: coordinates, energy rows, gradan
   SamplerBuffers smp;           // replica state of the samplers
   EnergyBatch* batch = nullptr;            // kept by energy_batch for repeated calls on the same set
   std::vector<EnergyBatch*> user_batches;  // smmp_batch_create handles running on this engine
@@ -673,17 +729,38 @@ void engine_gradient(Engine*& e, State& s, int nconf, const double* vlvr, double
   double* d_work = E.prof.on ? static_cast<double*>(E.d_work.ensure(sizeof(double) * 4 * nconf)) : nullptr;
   HIPCHK(hipFuncSetAttribute((const void*)k_gradient, hipFuncAttributeMaxDynamicSharedMemorySize, (int)plan.bytes));
   prof_mark(E, kKGradient, true);
+  // the analytic solvent (itysol < 0, one chain): coordinates out of the
+  // kernel, esolan's term and gradient, its forces projected on the variables
+  const bool sol = s.itysol < 0;
+  const int nat = ds.h.nat;
+  double* d_xyz = sol ? static_cast<double*>(E.g_xyz.ensure(sizeof(double) * 3 * (size_t)nat * nconf)) : nullptr;
   hipLaunchKernelGGL(k_gradient, dim3(nconf), dim3(kGradThreads), plan.bytes, E.stream,
-                     static_cast<const DevSys*>(E.d_sys.p), d_cs, d_off, d_vl, d_out, d_g, d_work, plan.bits_lds);
+                     static_cast<const DevSys*>(E.d_sys.p), d_cs, d_off, d_vl, d_out, d_g, d_work, plan.bits_lds,
+                     d_xyz, nat);
   HIPCHK(hipGetLastError());
   prof_mark(E, kKGradient, false);
   if (d_work) {
     prof_collect(E);
     prof_add_work(E, kKGradient, d_work, nconf);
   }
+  std::vector<double> rows;
+  if (sol) {
+    double* d_rows = static_cast<double*>(E.g_rows.ensure(sizeof(double) * kEnergyOut * (size_t)nconf));
+    double* d_ga = static_cast<double*>(E.g_gradan.ensure(sizeof(double) * 3 * (size_t)nat * nconf));
+    HIPCHK(hipMemsetAsync(d_rows, 0, sizeof(double) * kEnergyOut * (size_t)nconf, E.stream));
+    esolan_energies(E.esolan, E.stream, static_cast<const DevSys*>(E.d_sys.p), {&ds.h}, cs, d_cs, nconf, nat, d_xyz,
+                    d_rows, kEnergyOut, [&](int) { return host_grnd(s); }, d_ga);
+    hipLaunchKernelGGL(k_sol_project, dim3(nconf), dim3(kGradThreads), 0, E.stream,
+                       static_cast<const DevSys*>(E.d_sys.p), d_cs, d_off, d_xyz, d_ga, nat, d_g);
+    HIPCHK(hipGetLastError());
+    rows.resize((size_t)kEnergyOut * nconf);
+    HIPCHK(hipMemcpyAsync(rows.data(), d_rows, sizeof(double) * rows.size(), hipMemcpyDeviceToHost, E.stream));
+  }
   HIPCHK(hipMemcpyAsync(out_e, d_out, sizeof(double) * kGradOut * nconf, hipMemcpyDeviceToHost, E.stream));
   HIPCHK(hipMemcpyAsync(out_g, d_g, sizeof(double) * nv, hipMemcpyDeviceToHost, E.stream));
   HIPCHK(hipStreamSynchronize(E.stream));
+  if (sol)  // gradient.f:47-48: eysm carries eysl
+    for (int c = 0; c < nconf; ++c) out_e[(size_t)kGradOut * c] += rows[(size_t)kEnergyOut * c + 6];
 }
 
 double engine_enysol(Engine*& e, State& s) {

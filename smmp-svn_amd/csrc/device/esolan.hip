@@ -27,9 +27,16 @@
 // The branch and list decisions use only +, -, *, /, sqrt with contraction
 // off, as in the reference (they match it bit for bit); the arc terms use
 // atan2/atan/sin/cos of the device library and are summed by wave trees, so
-// areas agree with the reference to rounding.  The Cartesian gradient
-// (gradan, esolan.f:533-1016) feeds only the minimiser (opesol.f) and is not
-// computed.
+// areas agree with the reference to rounding.
+//
+// The gradient form (k_esolan<true>, for the minimiser's opesol.f) also
+// forms the Cartesian derivatives of the atom's area by every neighbour
+// (esolan.f:547-568 for full arcs, 684-862 for arcs: the coefficients of
+// the arc's own circle and of the two circles crossing it at its ends),
+// rotates them back with the atom's rotation, takes the atom's own as minus
+// their sum over the final circle list (esolan.f:966-1000), and writes
+// sigma_i times each as a row of the item; k_esolan_grad_sum then forms
+// gradan_j = sum_i sigma_i dAs_i/dr_j in atom order for every atom j.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -77,6 +84,12 @@ struct EsLds {
   int32_t cnt[kEsSlots], off[kEsSlots], cur[kEsSlots];        // point buckets by slot
 };
 
+// Gradient form: per slot, the derivative of the atom's area by that
+// neighbour's position (frame of the rotated neighbours until the end).
+struct EsGradLds {
+  double gx[kEsSlots], gy[kEsSlots], gz[kEsSlots];
+};
+
 struct EsParams {
   const DevSys* systems;
   const int32_t* conf_sys;
@@ -94,6 +107,12 @@ struct EsParams {
   size_t wave_bytes;
   int cap;               // intersection points per atom
   double* counters;      // atom pairs scanned, circle pairs classified, arcs tested, coverage tests
+  // gradient form: per item kEsSlots rows (neighbour atom, sigma_i dAs_i/dr
+  // of it), slot kEsSlots - 1 the atom's own; grow[item] rows in use (0:
+  // none)
+  int32_t* gatom;
+  double* gval;
+  int32_t* grow;
 };
 
 struct EsScratch {
@@ -102,14 +121,20 @@ struct EsScratch {
   double *ang, *srt;  // bucket entries (2 cap): polar angle by entry, sorted within bucket
   uint16_t* ent;     // point of each entry
   uint8_t* entc;     // circle of each entry
+  // gradient form, per sorted entry (2 cap): the other circle through its
+  // point; the derivatives of the arc that starts there by its own circle's,
+  // its start circle's and its end circle's neighbour (9 values), with the
+  // start and end circles (255: no exposed arc)
+  uint8_t *oth, *gk1, *gk2;
+  double* gq;
 };
 
-__host__ __device__ inline size_t es_wave_bytes(int cap) {
+__host__ __device__ inline size_t es_wave_bytes(int cap, bool grad) {
   const size_t c = (size_t)cap;
-  return c * 8 * 2 + 2 * c * 8 * 2 + c * 2 + 2 * c * 2 + 2 * c + 256;
+  return c * 8 * 2 + 2 * c * 8 * 2 + c * 2 + 2 * c * 2 + 2 * c + 256 + (grad ? 2 * c * (9 * 8 + 3) + 256 : 0);
 }
 
-__device__ inline EsScratch es_carve(char* p, int cap) {
+__device__ inline EsScratch es_carve(char* p, int cap, bool grad) {
   EsScratch s;
   const size_t c = (size_t)cap;
   s.vt = (double*)p;
@@ -119,6 +144,15 @@ __device__ inline EsScratch es_carve(char* p, int cap) {
   s.vc = (uint16_t*)(s.srt + 2 * c);
   s.ent = s.vc + c;
   s.entc = (uint8_t*)(s.ent + 2 * c);
+  s.oth = s.gk1 = s.gk2 = nullptr;
+  s.gq = nullptr;
+  if (grad) {  // 8-byte aligned after the byte arrays
+    const size_t o = ((size_t)(s.entc + 2 * c - (uint8_t*)p) + 7) / 8 * 8;
+    s.gq = (double*)(p + o);
+    s.oth = (uint8_t*)(s.gq + 9 * 2 * c);
+    s.gk1 = s.oth + 2 * c;
+    s.gk2 = s.gk1 + 2 * c;
+  }
   return s;
 }
 
@@ -177,12 +211,116 @@ __device__ __forceinline__ int es_classify(const EsLds& L, int K, int Lc, double
   return code;
 }
 
+// ---------------------------------------------------------- gradient form
+//
+// d(term)/d(a, b, c, d) of one circle -> d/d(x, y, z) of its neighbour at
+// (x, y, z) relative to the atom (after the atom's rotation): the circle of
+// a neighbour of radius r is a = x^2 + y^2 + (z - R)^2 - r^2, b = -8 R^2 x,
+// c = -8 R^2 y, d = 4 R^2 (x^2 + y^2 + (z + R)^2 - r^2).
+__device__ __forceinline__ void es_to_xyz(const double dq[4], double x, double y, double z, double R, double& gx,
+                                          double& gy, double& gz) {
+  const double R42 = 4.0 * R * R, m8 = -(R42 + R42);
+  gx += dq[0] * (2.0 * x) + dq[1] * m8 + dq[3] * (2.0 * x * R42);
+  gy += dq[0] * (2.0 * y) + dq[2] * m8 + dq[3] * (2.0 * y * R42);
+  gz += dq[0] * (2.0 * (z - R)) + dq[3] * (2.0 * (z + R) * R42);
+}
+
+// The cap a full arc removes (esolan.f:545, a != 0) by the circle's
+// coefficients (esolan.f:559-563).
+__device__ __forceinline__ void es_full_grad(double R, double a, double b, double c, double d, double dq[4]) {
+  const double R2 = R * R, R22 = R2 + R2, R42 = R22 + R22, R42p = 2.0 * R22 * kPi;
+  const double den = (R42 * a - d) * (R42 * a - d) + R42 * (b * b + c * c);
+  const double div = R22 * R42p / (den * sqrt(den));
+  dq[0] = (R42 * (b * b + c * c - 2.0 * a * d) + 2.0 * d * d) * div;
+  dq[1] = -b * (d + R42 * a) * div;
+  dq[2] = -c * (d + R42 * a) * div;
+  dq[3] = (b * b + c * c - 2.0 * a * d + (R42 + R42) * a * a) * div;
+}
+
+// One exposed arc (esolan.f:684-862) on circle q = (a, b, c, d) from polar
+// angle alp, where circle q1 crosses it, to bet, where circle q2 does: the
+// term's derivatives by q's coefficients (directly and through both end
+// angles) and by q1's and q2's (through alp and bet).
+__device__ void es_arc_grad(double R, double a, double b, double c, double d, double alp, double bet,
+                            const double q1[4], const double q2[4], double dq[4], double dq1[4], double dq2[4]) {
+  const double R2 = R * R, R42 = 4.0 * R2;
+  const double ak = a * a, daba = fabs(a), aia = a < 0.0 ? -1.0 : 1.0, b2c2 = b * b + c * c;
+  const double vv = sqrt((R42 * a - d) * (R42 * a - d) + R42 * b2c2), vv1 = 1.0 / vv, vv2 = vv1 * vv1;
+  const double wv = sqrt(b2c2 - 4.0 * a * d), bcd = b2c2 - 2.0 * a * d;
+  const double xv = (d + R42 * a) * vv1, yv = (bcd + 2.0 * R42 * ak) / daba * vv1, zv = wv / a * vv1;
+  double sa, ca, sb, cb, sm, cm, sp, cp;
+  sincos(alp, &sa, &ca);
+  sincos(bet, &sb, &cb);
+  sincos(5e-1 * (bet - alp), &sm, &cm);
+  sincos(5e-1 * (bet + alp), &sp, &cp);
+  const double cotm = cm / sm;
+  const double bcp = b * cp + c * sp, bsp = -b * sp + c * cp;
+  const double tt = yv * cotm - zv * bcp / sm;
+  const double a1 = q1[0], b1 = q1[1], c1 = q1[2], d1 = q1[3];
+  const double a2 = q2[0], b2 = q2[1], c2 = q2[2], d2 = q2[3];
+  const double ab1 = a * b1 - b * a1, ac1 = a * c1 - c * a1, ab2 = a * b2 - b * a2, ac2 = a * c2 - c * a2;
+  const double am1 = wv * aia * a1 * (ab1 * sa - ac1 * ca), am2 = wv * aia * a2 * (ab2 * sb - ac2 * cb);
+  const double u1 = aia * a1 * (ab1 * ca + ac1 * sa) / wv, u2 = aia * a2 * (ab2 * cb + ac2 * sb) / wv;
+  double dal[4], dbe[4];
+  dal[0] = (b1 * ab1 + c1 * ac1 - 2.0 * d * a1 * a1 - a * (b1 * b1 + c1 * c1 - 4.0 * a1 * d1) - 2.0 * d * u1 +
+            wv * aia * a1 * (b1 * ca + c1 * sa)) / am1;
+  dal[1] = (-a1 * ab1 + b * a1 * a1 + b * u1 - wv * aia * a1 * a1 * ca) / am1;
+  dal[2] = (-a1 * ac1 + c * a1 * a1 + c * u1 - wv * aia * a1 * a1 * sa) / am1;
+  dal[3] = (-2.0 * a * a1 * a1 - 2.0 * daba * a1 * (ab1 * ca + ac1 * sa) / wv) / am1;
+  dbe[0] = (b2 * ab2 + c2 * ac2 - 2.0 * d * a2 * a2 - a * (b2 * b2 + c2 * c2 - 4.0 * a2 * d2) - 2.0 * d * u2 +
+            wv * aia * a2 * (b2 * cb + c2 * sb)) / am2;
+  dbe[1] = (-a2 * ab2 + b * a2 * a2 + b * u2 - wv * aia * a2 * a2 * cb) / am2;
+  dbe[2] = (-a2 * ac2 + c * a2 * a2 + c * u2 - wv * aia * a2 * a2 * sb) / am2;
+  dbe[3] = (-2.0 * a * a2 * a2 - 2.0 * daba * a2 * (ab2 * cb + ac2 * sb) / wv) / am2;
+  const double xn = d + R42 * a, yn = b2c2 - 2.0 * a * d + 2.0 * R42 * ak;
+  double dv[4];
+  dv[1] = R42 * b * vv1;
+  dv[2] = R42 * c * vv1;
+  dv[3] = (d - R42 * a) * vv1;
+  dv[0] = -dv[3] * R42;
+  const double sm2 = sm * sm, w = 4.0 + tt * tt, at = kPi - 2.0 * atan(5e-1 * tt);
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    const double dx = (m == 0 ? R42 * vv1 : (m == 3 ? vv1 : 0.0)) - xn * dv[m] * vv2;
+    double dy, dz;
+    if (m == 0) {
+      dy = (-2.0 * d + 4.0 * R42 * a) / daba * vv1 - yn * (aia * vv + daba * dv[0]) / ak * vv2;
+      dz = -2.0 * d / wv / a * vv1 - wv * (vv + a * dv[0]) / ak * vv2;
+    } else {
+      const double p = m == 1 ? 2.0 * b : (m == 2 ? 2.0 * c : -2.0 * a);
+      dy = p / daba * vv1 - yn * aia * dv[m] / a * vv2;
+      dz = (m == 1 ? b / wv / a : (m == 2 ? c / wv / a : -2.0 / wv)) * vv1 - wv * dv[m] / a * vv2;
+    }
+    const double extra = m == 1 ? cp * 2.0 * sm : (m == 2 ? sp * 2.0 * sm : 0.0);
+    const double dt = cotm * dy - 5e-1 * yv * (dbe[m] - dal[m]) / sm2 - (bcp / sm) * dz -
+                      5e-1 * zv * (bsp * sm * (dal[m] + dbe[m]) - bcp * cm * (dbe[m] - dal[m]) + extra) / sm2;
+    dq[m] = R2 * (aia * (dal[m] - dbe[m]) + at * dx - 4.0 * xv * dt / w);
+  }
+  double dal1[4], dbe2[4];
+  dal1[0] = (-b * ab1 - c * ac1 + wv * wv * a1 + 2.0 * ak * d1 + wv * aia * ((ab1 - b * a1) * ca + (ac1 - c * a1) * sa)) / am1;
+  dal1[1] = (a * ab1 - ak * b1 + wv * daba * a1 * ca) / am1;
+  dal1[2] = (a * ac1 - ak * c1 + wv * daba * a1 * sa) / am1;
+  dal1[3] = (2.0 * ak * a1) / am1;
+  dbe2[0] = (-b * ab2 - c * ac2 + wv * wv * a2 + 2.0 * ak * d2 + wv * aia * ((ab2 - b * a2) * cb + (ac2 - c * a2) * sb)) / am2;
+  dbe2[1] = (a * ab2 - ak * b2 + wv * daba * a2 * cb) / am2;
+  dbe2[2] = (a * ac2 - ak * c2 + wv * daba * a2 * sb) / am2;
+  dbe2[3] = (2.0 * ak * a2) / am2;
+  const double ta = (yv - zv * (bsp * sm + bcp * cm)) / sm2, tb = (-yv - zv * (bsp * sm - bcp * cm)) / sm2;
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    dq1[m] = R2 * (aia * dal1[m] - 4.0 * xv * (5e-1 * dal1[m] * ta) / w);
+    dq2[m] = R2 * (-aia * dbe2[m] - 4.0 * xv * (5e-1 * dbe2[m] * tb) / w);
+  }
+}
+
 // One atom of one conformation.  Returns false if the item was left for the
 // rotation launch.
-__device__ void es_item(const EsParams& P, EsLds& L, const EsScratch& X, int item, const double* rot, int lane,
-                        double* work) {
+template <bool GRAD>
+__device__ void es_item(const EsParams& P, EsLds& L, EsGradLds& G, const EsScratch& X, int item, const double* rot,
+                        int lane, double* work) {
   const int c = item / P.stride, i = item - c * P.stride;
   const DevSys& S = P.systems[P.conf_sys[c]];
+  if (GRAD && lane == 0) P.grow[item] = 0;
   if (i >= S.nat) {
     if (lane == 0) P.asig[item] = 0.0;
     return;
@@ -374,6 +512,24 @@ __device__ void es_item(const EsParams& P, EsLds& L, const EsScratch& X, int ite
     if (lane == 0) P.asig[item] = 0.0;
     return;
   }
+  // the position of a neighbour relative to the atom, after its rotation
+  auto slot_xyz = [&](int s, double& x, double& y, double& z) {
+    const int a = L.atom[s];
+    x = xyz[3 * a] - xi;
+    y = xyz[3 * a + 1] - yi;
+    z = xyz[3 * a + 2] - zi;
+    if (rot) {
+      const double cg = rot[0], sg = rot[1], cf = rot[2], sf = rot[3];
+      const double xx = x, yy = y, zz = z;
+      x = xx * cg - zz * sg;
+      y = -xx * (sf * sg) + yy * cf - zz * (sf * cg);
+      z = xx * (cf * sg) + yy * sf + zz * (cf * cg);
+    }
+  };
+  if constexpr (GRAD) {
+    for (int s = lane; s < nb; s += kWave) G.gx[s] = G.gy[s] = G.gz[s] = 0.0;
+    wave_lds_sync();
+  }
 
   // 5. areas (esolan.f:453-963)
   const double R2 = R * R, R22 = R2 + R2, R42 = R22 + R22, R22p = R22 * kPi;
@@ -388,6 +544,15 @@ __device__ void es_item(const EsParams& P, EsLds& L, const EsScratch& X, int ite
       if (a != 0.0) {
         const double aia = a < 0.0 ? -1.0 : 1.0;
         full += aia * R22p * (1.0 + (-d / a - R42) * fabs(a) / sqrt((R42 * a - d) * (R42 * a - d) + R42 * (b * b + cc * cc)));
+        if constexpr (GRAD) {  // each slot once in the list: this lane's own
+          double dq[4], x, y, z, gx = 0.0, gy = 0.0, gz = 0.0;
+          es_full_grad(R, a, b, cc, d, dq);
+          slot_xyz(s, x, y, z);
+          es_to_xyz(dq, x, y, z, R, gx, gy, gz);
+          G.gx[s] = gx;
+          G.gy[s] = gy;
+          G.gz[s] = gz;
+        }
       } else {
         full -= R22p * d / sqrt(R42 * (b * b + cc * cc) + d * d);
       }
@@ -463,6 +628,10 @@ __device__ void es_item(const EsParams& P, EsLds& L, const EsScratch& X, int ite
       r += (y < x || (y == x && q < e)) ? 1 : 0;
     }
     X.srt[lo + r] = x;
+    if constexpr (GRAD) {  // the other circle through this point
+      const int w = X.vc[X.ent[e]], c1 = w & 255, c2 = w >> 8;
+      X.oth[lo + r] = (uint8_t)(c1 == s ? c2 : c1);
+    }
   }
   wave_lds_sync();
   // arcs between consecutive points whose midpoint no other circle covers
@@ -470,6 +639,7 @@ __device__ void es_item(const EsParams& P, EsLds& L, const EsScratch& X, int ite
   for (int p = lane; p < E; p += kWave) {
     const int s = X.entc[p], lo = L.off[s], hi = lo + L.cnt[s];
     const double alp = X.srt[p], bet = (p + 1 < hi) ? X.srt[p + 1] : X.srt[lo] + kPi2;
+    if (GRAD) X.gk1[p] = 255;
     if (fabs(bet - alp) < 1e-8) continue;
     const double a = L.A[s], b = L.B[s], cc = L.C[s], d = L.D[s];
     const double ak = a * a, daba = fabs(a), aia = a < 0.0 ? -1.0 : 1.0, b2c2 = b * b + cc * cc;
@@ -497,17 +667,116 @@ __device__ void es_item(const EsParams& P, EsLds& L, const EsScratch& X, int ite
     sincos(hp, &dsbetpalp, &dcbetpalp);
     const double uv = daba * (bcd + 2.0 * R42 * ak) * dcbetmalp - a * wv * (b * dcbetpalp + cc * dsbetpalp);
     arcs += R2 * (aia * (alp - bet) + (d + R42 * a) * vv1 * (kPi - 2.0 * atan(uv / (2.0 * ak * vv * dsbetmalp))));
+    if constexpr (GRAD) {  // esolan.f:684-862
+      const int k1 = X.oth[p], k2 = X.oth[(p + 1 < hi) ? p + 1 : lo];
+      const double q1[4] = {L.A[k1], L.B[k1], L.C[k1], L.D[k1]}, q2[4] = {L.A[k2], L.B[k2], L.C[k2], L.D[k2]};
+      double dq[4], dq1[4], dq2[4], x, y, z;
+      es_arc_grad(R, a, b, cc, d, alp, bet, q1, q2, dq, dq1, dq2);
+      double g[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      slot_xyz(s, x, y, z);
+      es_to_xyz(dq, x, y, z, R, g[0], g[1], g[2]);
+      slot_xyz(k1, x, y, z);
+      es_to_xyz(dq1, x, y, z, R, g[3], g[4], g[5]);
+      slot_xyz(k2, x, y, z);
+      es_to_xyz(dq2, x, y, z, R, g[6], g[7], g[8]);
+      for (int m = 0; m < 9; ++m) X.gq[9 * (size_t)p + m] = g[m];
+      X.gk1[p] = (uint8_t)k1;
+      X.gk2[p] = (uint8_t)k2;
+    }
   }
   As = As - wave_sum(full) + wave_sum(arcs);
   if (lane == 0) P.asig[item] = As * S.sigma[i];
+  if constexpr (GRAD) {
+    wave_lds_sync();
+    // every slot's arc contributions in bucket order (the circles in list
+    // order, each one's arcs by angle: the order esolan.f adds them in)
+    for (int t = lane; t < nb; t += kWave) {
+      double gx = G.gx[t], gy = G.gy[t], gz = G.gz[t];
+      for (int p = 0; p < E; ++p) {
+        const int k1 = X.gk1[p];
+        if (k1 == 255) continue;
+        const double* q = X.gq + 9 * (size_t)p;
+        if (X.entc[p] == t) {
+          gx += q[0];
+          gy += q[1];
+          gz += q[2];
+        }
+        if (k1 == t) {
+          gx += q[3];
+          gy += q[4];
+          gz += q[5];
+        }
+        if (X.gk2[p] == t) {
+          gx += q[6];
+          gy += q[7];
+          gz += q[8];
+        }
+      }
+      G.gx[t] = gx;
+      G.gy[t] = gy;
+      G.gz[t] = gz;
+    }
+    wave_lds_sync();
+    // the atom's own: minus the sum over the final list (esolan.f:966-981)
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    for (int p = lane; p < n; p += kWave) {
+      const int t = L.lst[p];
+      sx += G.gx[t];
+      sy += G.gy[t];
+      sz += G.gz[t];
+    }
+    sx = -wave_sum(sx);
+    sy = -wave_sum(sy);
+    sz = -wave_sum(sz);
+    // back to the frame of the coordinates: the final list's rows and the
+    // atom's own (esolan.f:983-1001; a dropped circle's row stays as it is)
+    auto unrotate = [&](double& x, double& y, double& z) {
+      const double cg = rot[0], sg = rot[1], cf = rot[2], sf = rot[3];
+      const double xx = x, yy = y, zz = z;
+      x = xx * cg - yy * (sf * sg) + zz * (cf * sg);
+      y = yy * cf + zz * sf;
+      z = -xx * sg - yy * (sf * cg) + zz * (cf * cg);
+    };
+    if (rot) {
+      for (int p = lane; p < n; p += kWave) {
+        const int t = L.lst[p];
+        double x = G.gx[t], y = G.gy[t], z = G.gz[t];
+        unrotate(x, y, z);
+        G.gx[t] = x;
+        G.gy[t] = y;
+        G.gz[t] = z;
+      }
+      unrotate(sx, sy, sz);
+      wave_lds_sync();
+    }
+    const double sg = S.sigma[i];
+    const size_t row = (size_t)item * kEsSlots;
+    for (int t = lane; t < nb; t += kWave) {
+      P.gatom[row + t] = L.atom[t];
+      double* o = P.gval + 3 * (row + t);
+      o[0] = sg * G.gx[t];
+      o[1] = sg * G.gy[t];
+      o[2] = sg * G.gz[t];
+    }
+    if (lane == 0) {
+      P.gatom[row + kEsSlots - 1] = i;
+      double* o = P.gval + 3 * (row + kEsSlots - 1);
+      o[0] = sg * sx;
+      o[1] = sg * sy;
+      o[2] = sg * sz;
+      P.grow[item] = nb;
+    }
+  }
 }
 
+template <bool GRAD>
 __global__ void __launch_bounds__(kEsThreads) k_esolan(EsParams P) {
   extern __shared__ double es_smem[];
   const int wave = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
   EsLds& L = reinterpret_cast<EsLds*>(es_smem)[wave];
+  EsGradLds& G = reinterpret_cast<EsGradLds*>(reinterpret_cast<EsLds*>(es_smem) + kEsWaves)[GRAD ? wave : 0];
   const size_t slot = (size_t)blockIdx.x * kEsWaves + wave;
-  const EsScratch X = es_carve(P.scratch + slot * P.wave_bytes, P.cap);
+  const EsScratch X = es_carve(P.scratch + slot * P.wave_bytes, P.cap, GRAD);
   double n[4] = {0.0, 0.0, 0.0, 0.0};
   for (;;) {
     int idx = 0;
@@ -515,7 +784,7 @@ __global__ void __launch_bounds__(kEsThreads) k_esolan(EsParams P) {
     idx = __shfl(idx, 0, kWave);
     if (idx >= P.nitems) break;
     const int item = P.items ? P.items[idx] : idx;
-    es_item(P, L, X, item, P.items ? P.rot + 4 * (size_t)idx : nullptr, lane, n);
+    es_item<GRAD>(P, L, G, X, item, P.items ? P.rot + 4 * (size_t)idx : nullptr, lane, n);
     wave_lds_sync();
   }
   for (int k = 0; k < 4; ++k) n[k] = wave_sum(n[k]);
@@ -541,6 +810,60 @@ __global__ void k_esolan_sum(const DevSys* __restrict__ systems, const int32_t* 
   }
 }
 
+// gradan of every atom j of every conformation (esolan.f:1008-1016): the sum
+// over the atoms i of the conformation, in atom order per lane and by a wave
+// tree across lanes, of row j of item i (the neighbour rows are in
+// increasing atom order: a binary search) or of its own row (i = j).  One
+// wavefront per (conformation, atom); out at gradan + 3 (stride c + j).
+__global__ void k_esolan_grad_sum(const DevSys* __restrict__ systems, const int32_t* __restrict__ conf_sys,
+                                  const int32_t* __restrict__ gatom, const double* __restrict__ gval,
+                                  const int32_t* __restrict__ grow, int nconf, int stride, double* __restrict__ gradan) {
+  const int w = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  const int c = w / stride, j = w - c * stride;
+  if (c >= nconf) return;
+  const int nat = systems[conf_sys[c]].nat;
+  double* o = gradan + 3 * ((size_t)stride * c + j);
+  if (j >= nat) {
+    if (lane < 3) o[lane] = 0.0;
+    return;
+  }
+  double gx = 0.0, gy = 0.0, gz = 0.0;
+  for (int i = lane; i < nat; i += kWave) {
+    const size_t item = (size_t)stride * c + i, row = item * kEsSlots;
+    const int nr = grow[item];
+    if (nr == 0) continue;
+    int hit = -1;
+    if (i == j) {
+      hit = kEsSlots - 1;
+    } else {
+      int lo = 0, hi = nr - 1;
+      while (lo <= hi) {
+        const int mid = (lo + hi) >> 1, a = gatom[row + mid];
+        if (a == j) {
+          hit = mid;
+          break;
+        }
+        if (a < j) lo = mid + 1;
+        else hi = mid - 1;
+      }
+    }
+    if (hit >= 0) {
+      const double* v = gval + 3 * (row + hit);
+      gx += v[0];
+      gy += v[1];
+      gz += v[2];
+    }
+  }
+  gx = wave_sum(gx);
+  gy = wave_sum(gy);
+  gz = wave_sum(gz);
+  if (lane == 0) {
+    o[0] = gx;
+    o[1] = gy;
+    o[2] = gz;
+  }
+}
+
 // ------------------------------------------------------------------ host
 
 struct EsBuf {
@@ -562,7 +885,7 @@ struct EsBuf {
 };
 
 struct EsolanWork {
-  EsBuf flag, small, asig, items, rot, scratch;
+  EsBuf flag, small, asig, items, rot, scratch, gatom, gval, grow;
   int cap = 8192;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   EsolanStats last;
@@ -585,9 +908,9 @@ static int es_grid_waves(int nitems, size_t wave_bytes) {
 
 // One launch over nitems items; returns the error bits.  Grows the point
 // capacity and repeats when an atom has more intersection points.
-static int es_launch(EsolanWork& W, hipStream_t st, EsParams P, int32_t* d_small, double* d_counters) {
+static int es_launch(EsolanWork& W, hipStream_t st, EsParams P, int32_t* d_small, double* d_counters, bool grad) {
   for (;;) {
-    const size_t wb = (es_wave_bytes(W.cap) + 255) / 256 * 256;
+    const size_t wb = (es_wave_bytes(W.cap, grad) + 255) / 256 * 256;
     const int waves = es_grid_waves(P.nitems, wb);
     P.scratch = static_cast<char*>(W.scratch.ensure(wb * (size_t)waves));
     P.wave_bytes = wb;
@@ -597,7 +920,13 @@ static int es_launch(EsolanWork& W, hipStream_t st, EsParams P, int32_t* d_small
     P.next = d_small + 1;
     P.nflag = d_small + 2;
     P.counters = d_counters;
-    hipLaunchKernelGGL(k_esolan, dim3(waves / kEsWaves), dim3(kEsThreads), sizeof(EsLds) * kEsWaves, st, P);
+    if (grad) {
+      const size_t lds = (sizeof(EsLds) + sizeof(EsGradLds)) * kEsWaves;
+      HIPCHK(hipFuncSetAttribute((const void*)k_esolan<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      hipLaunchKernelGGL(k_esolan<true>, dim3(waves / kEsWaves), dim3(kEsThreads), lds, st, P);
+    } else {
+      hipLaunchKernelGGL(k_esolan<false>, dim3(waves / kEsWaves), dim3(kEsThreads), sizeof(EsLds) * kEsWaves, st, P);
+    }
     HIPCHK(hipGetLastError());
     int32_t h[3];
     HIPCHK(hipMemcpyAsync(h, d_small, sizeof h, hipMemcpyDeviceToHost, st));
@@ -620,7 +949,8 @@ static void es_check(int err) {
 
 void esolan_energies(EsolanWork*& Wp, hipStream_t st, const DevSys* d_systems, const std::vector<const HostSys*>& hsys,
                      const std::vector<int32_t>& conf_sys, const int32_t* d_conf_sys, int nconf, int stride,
-                     const double* d_xyz, double* d_out, int out_stride, const std::function<double(int)>& draw) {
+                     const double* d_xyz, double* d_out, int out_stride, const std::function<double(int)>& draw,
+                     double* d_gradan) {
   if (!Wp) Wp = new EsolanWork();
   EsolanWork& W = *Wp;
   if (!W.ev0) {
@@ -648,7 +978,14 @@ void esolan_energies(EsolanWork*& Wp, hipStream_t st, const DevSys* d_systems, c
   P.nitems = nitems;
   P.flag = d_flag;
   P.asig = d_asig;
-  int err = es_launch(W, st, P, d_small, d_counters);
+  const bool grad = d_gradan != nullptr;
+  if (grad) {
+    static_assert(kEsSlots - 1 == kEsMaxNb, "the atom's own row follows the neighbour rows");
+    P.gatom = static_cast<int32_t*>(W.gatom.ensure(sizeof(int32_t) * kEsSlots * (size_t)nitems));
+    P.gval = static_cast<double*>(W.gval.ensure(sizeof(double) * 3 * kEsSlots * (size_t)nitems));
+    P.grow = static_cast<int32_t*>(W.grow.ensure(sizeof(int32_t) * (size_t)nitems));
+  }
+  int err = es_launch(W, st, P, d_small, d_counters, grad);
   es_check(err & 255);
   const int nflag = err >> 8;
 
@@ -708,13 +1045,19 @@ void esolan_energies(EsolanWork*& Wp, hipStream_t st, const DevSys* d_systems, c
     P.nitems = (int)items.size();
     P.items = d_items;
     P.rot = d_rot;
-    err = es_launch(W, st, P, d_small, d_counters);
+    err = es_launch(W, st, P, d_small, d_counters, grad);
     es_check(err & 255);
   }
   const int per = 256 / kWave;
   hipLaunchKernelGGL(k_esolan_sum, dim3((nconf + per - 1) / per), dim3(256), 0, st, d_systems, d_conf_sys, d_asig,
                      nconf, stride, d_out, out_stride);
   HIPCHK(hipGetLastError());
+  if (grad) {
+    const int64_t waves = (int64_t)nconf * stride;
+    hipLaunchKernelGGL(k_esolan_grad_sum, dim3((unsigned)((waves + per - 1) / per)), dim3(256), 0, st, d_systems,
+                       d_conf_sys, P.gatom, P.gval, P.grow, nconf, stride, d_gradan);
+    HIPCHK(hipGetLastError());
+  }
   HIPCHK(hipEventRecord(W.ev1, st));
   double cnt[4];
   HIPCHK(hipMemcpyAsync(cnt, d_counters, sizeof cnt, hipMemcpyDeviceToHost, st));

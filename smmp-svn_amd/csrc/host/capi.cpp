@@ -337,7 +337,7 @@ int smmp_gradient(smmp_ctx* c) {
     c->s.eyvw = e[2];
     c->s.eyhb = e[3];
     c->s.eyvr = e[4];
-    c->s.eysl = 0.0;
+    c->s.eysl = c->s.itysol < 0 ? e[0] - (e[1] + e[2] + e[3] + e[4]) : 0.0;
     for (int k = 0; k < nvt; ++k) c->s.gdeyvr(k + 1) = g[k];
     for (int m = 1; m <= c->s.ntlml; ++m) setvar(c->s, m, c->s.vlvr.data());  // gradient.f:33
     return SMMP_OK;

@@ -26,16 +26,26 @@ void engine_invalidate(Engine* e);  // topology changed: rebuild device data on 
 
 // energy.f:12 for the context's current vlvr; writes eyel.. and coordinates
 double engine_energy(Engine*& e, State& s);
-// gradient.f:13-70 in vacuum for nconf full variable vectors (nvt each):
-// out_e[5 c ..] = eysm, eyel, eyvw, eyhb, eyvr; out_g[nvt c ..] = dE/dv
+// gradient.f:13-70 for nconf full variable vectors (nvt each): out_e[5 c ..]
+// = eysm, eyel, eyvw, eyhb, eyvr, where eysm includes eysl with the analytic
+// solvent as gradient.f:47 leaves it (eysl = eysm - the other four);
+// out_g[nvt c ..] = dE/dv.  With itysol < 0 esolan's rotations draw from
+// the context's twister, conformations in order.
 void engine_gradient(Engine*& e, State& s, int nconf, const double* vlvr, double* out_e, double* out_g);
 // Throws unless the context's model has derivatives here: vacuum (itysol 0;
-// the reference stops for itysol > 0, gradient.f:59-62; the analytic
-// solvent's derivatives of opesol.f are not provided) and ECEPP (not FLEX).
+// the reference stops for itysol > 0, gradient.f:59-62) and ECEPP (not
+// FLEX).  The analytic solvent's device derivatives (k_esolan's gradient
+// form, k_sol_project) are switched off: their first run faulted the GPU
+// and no cause has been shown yet, so itysol < 0 is refused.
+constexpr bool kSolventGradientOn = false;
 inline void gradient_supported(const State& s) {
   if (s.itysol > 0) throw Error("gradient> Set itysol < 0");
-  if (s.itysol < 0)
-    throw Error("gradient> derivatives of the analytic solvent (opesol.f) are not provided: set itysol to 0");
+  if (s.itysol < 0 && !kSolventGradientOn)
+    throw Error("gradient> derivatives of the analytic solvent (opesol.f) are switched off on the device: set itysol "
+                "to 0");
+  if (s.itysol < 0 && s.ntlml > 1)
+    throw Error("gradient> the analytic solvent's derivatives are provided for one chain (esolan.f numbers the atoms "
+                "of a molecule from 1)");
   if (s.flex) throw Error("gradient> the FLEX force field (opeflx.f) is not provided");
 }
 // enysol.f:13 for the current coordinates

@@ -94,7 +94,7 @@ int minim(Engine*& e, State& s, int imin, int maxit, double eps) {
     s.eyvw = ge[2];
     s.eyhb = ge[3];
     s.eyvr = ge[4];
-    s.eysl = 0.0;
+    s.eysl = s.itysol < 0 ? ge[0] - (ge[1] + ge[2] + ge[3] + ge[4]) : 0.0;
     for (int k = 0; k < nvt; ++k) s.gdeyvr(k + 1) = gd[k];
   };
   std::vector<double> x(n), g(n);

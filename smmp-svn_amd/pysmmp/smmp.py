@@ -398,8 +398,9 @@ def minim(imin: int, maxit: int, eps: float) -> None:
     """smmp.pyf:1234, minim.f:13: local minimisation (imin 1 quasi-Newton,
     2 conjugate gradients) of the ECEPP/3 energy by the free variables, at
     most maxit function calls; the derivatives come from the GPU in vacuum
-    (itysol 0; the reference stops for itysol > 0, the analytic solvent's
-    derivatives are not provided) and without regularisation (ireg 0)."""
+    (itysol 0; the reference stops for itysol > 0, and the device form of
+    the analytic solvent's derivatives is switched off), without
+    regularisation (ireg 0)."""
     _ctx.minim(int(imin), int(maxit), float(eps))
 
 

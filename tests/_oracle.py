@@ -82,6 +82,8 @@ def _load():
     lib.oracle_muca_update.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, I32P, F64P, F64P, F64P, F64P]
     lib.oracle_esolan.argtypes = [P(Topo), P(Conf), P(MT), F64P]
     lib.oracle_esolan.restype = C.c_double
+    lib.oracle_esolan_grad.argtypes = [P(Topo), P(Conf), P(MT), F64P, F64P]
+    lib.oracle_esolan_grad.restype = C.c_double
     lib.oracle_set_solvent_mt.argtypes = [P(MT)]
     lib.oracle_esolan_force_rotation.argtypes = [C.c_double, C.c_double]
     lib.oracle_difang.argtypes = [C.c_double, C.c_double]
@@ -220,19 +222,35 @@ class OracleSystem:
             lib.oracle_set_solvent_mt(None)
         return np.array(self.c.e[:], dtype=np.float64)
 
-    def gradient(self, vlvr_all) -> tuple[np.ndarray, np.ndarray]:
-        """gradient.f in vacuum: ([eysm, eyel, eyvw, eyhb, eyvr], gdeyvr[nvt])."""
+    def gradient(self, vlvr_all, mt: "OracleMT | None" = None) -> tuple[np.ndarray, np.ndarray]:
+        """gradient.f: ([eysm, eyel, eyvw, eyhb, eyvr], gdeyvr[nvt]) in vacuum;
+        with the analytic solvent (itysol < 0) ([eysm, eyel, eyvw, eyhb, eyvr,
+        eysl], gdeyvr) with esolan's rotations drawn from mt."""
         vl = f64(vlvr_all)
         gd = np.zeros(self.nvt)
-        e = lib.oracle_gradient(C.byref(self.t), C.byref(self.c), vl.ctypes.data_as(F64P), gd.ctypes.data_as(F64P))
+        lib.oracle_set_solvent_mt(C.byref(mt.s) if mt is not None else None)
+        try:
+            e = lib.oracle_gradient(C.byref(self.t), C.byref(self.c), vl.ctypes.data_as(F64P),
+                                    gd.ctypes.data_as(F64P))
+        finally:
+            lib.oracle_set_solvent_mt(None)
         if not np.isfinite(e):
-            raise ValueError("oracle gradient: vacuum only (itysol = 0)")
-        return np.array(self.c.e[1:6], dtype=np.float64), gd
+            raise ValueError("oracle gradient: vacuum or the analytic solvent of one chain only")
+        n = 6 if self.t.itysol < 0 else 5
+        return np.array(self.c.e[1:1 + n], dtype=np.float64), gd
 
     def setvar(self, vlvr_all) -> None:
         """Applies the variables and rebuilds (setvar.f), without an energy."""
         vl = f64(vlvr_all)
         lib.oracle_setvar(C.byref(self.t), C.byref(self.c), vl.ctypes.data_as(F64P))
+
+    def esolan_gradient(self, mt: "OracleMT | None" = None) -> tuple[float, np.ndarray]:
+        """Analytic solvent term and its Cartesian gradient (nat x 3,
+        esolan.f's gradan) of the current coordinates."""
+        g = np.zeros(3 * self.nat)
+        e = lib.oracle_esolan_grad(C.byref(self.t), C.byref(self.c), C.byref(mt.s) if mt is not None else None,
+                                   None, g.ctypes.data_as(F64P))
+        return e, g.reshape(self.nat, 3)
 
     def esolan_areas(self, mt: "OracleMT | None" = None) -> tuple[float, np.ndarray]:
         """Analytic solvent term and per-atom areas of the current coordinates."""
@@ -322,6 +340,17 @@ class Spheres:
         e = lib.oracle_esolan(C.byref(self.t), C.byref(self.c), C.byref(mt.s) if mt is not None else None,
                               a.ctypes.data_as(F64P))
         return e, a
+
+    def esolan_grad(self, mt: OracleMT | None = None) -> tuple[float, np.ndarray]:
+        """(sum sigma_i As_i, gradan n x 3): the reference's Cartesian gradient."""
+        g = np.zeros(3 * self.n)
+        e = lib.oracle_esolan_grad(C.byref(self.t), C.byref(self.c), C.byref(mt.s) if mt is not None else None,
+                                   None, g.ctypes.data_as(F64P))
+        return e, g.reshape(self.n, 3)
+
+    def move(self, j: int, k: int, h: float) -> None:
+        """Shifts coordinate k of sphere j by h (in place)."""
+        self.keep[k][j] += h
 
 
 def force_rotation(uga: float, ufi: float) -> None:

@@ -401,6 +401,37 @@ def grad_fixtures(tmp: str) -> None:
     print(f"wrote {path} ({len(rows)} steps)")
 
 
+def grad_esolan_fixtures(tmp: str) -> None:
+    """gradient.f with the analytic solvent (itysol -1: opeshe.f plus
+    opesol.f over esolan.f's gradient) and minim.f runs on it: Met-enkephalin
+    from enkefa.var and random conformations of it and of the 10-residue
+    sequence (the random conformations and esolan's rotations share the
+    driver's twister, so the vlvr sequence also pins the draws)."""
+    enk = [s.lower() for s in open(os.path.join(HERE, "enkefa.seq")).read().split()[2:]]
+    t_enk = topo_fixture("enkefa", enk, tmp, nconf=8, seed=11)
+    vf = vars_file(os.path.join(tmp, "enk.vars"), t_enk, os.path.join(HERE, "enkefa.var"), False)
+    seqf = os.path.join(tmp, "enkefa.seq")
+    seq10 = list(np.load(os.path.join(HERE, "topo_len10.npz"))["seq_in"])
+    seqf10 = workloads.write_seq_file(os.path.join(tmp, "grad_len10.seq"), seq10, "len10")
+    out = {}
+    for tag, (sf, varf, nconf, seed) in {"enkefa": (seqf, vf, 5, 531), "len10": (seqf10, "-", 5, 532)}.items():
+        r = run(["grad", LIBDIR, sf, varf, "-1", str(nconf), str(seed), "0"], tmp)
+        out[f"{tag}_seed"] = np.array([seed])
+        out[f"{tag}_vlvr"] = np.stack(r["vlvr"])
+        out[f"{tag}_energy"] = np.stack(r["grad_energy"])
+        out[f"{tag}_grad"] = np.stack(r["gdeyvr"])
+    for tag, (sf, varf, imin, maxit, eps, nconf, seed) in {
+            "enkefa_qn": (seqf, vf, 1, 40, 1e-9, 0, 1), "enkefa_cg": (seqf, vf, 2, 40, 1e-9, 0, 1),
+            "len10_qn": (seqf10, "-", 1, 20, 1e-9, 1, 533)}.items():
+        r = run(["minim", LIBDIR, sf, varf, "-1", str(imin), str(maxit), repr(eps), str(nconf), str(seed)], tmp)
+        out[f"min_{tag}_args"] = np.array([imin, maxit, eps, seed])
+        out[f"min_{tag}_start"] = np.stack(r["vlvr"])
+        out[f"min_{tag}_vlvr"] = np.stack(r["min_vlvr"])
+        out[f"min_{tag}_energy"] = np.stack(r["min_energy"])
+        out[f"min_{tag}_nfun"] = np.concatenate(r["min_nfun"])
+    save("grad_esolan.npz", **out)
+
+
 def rmsd_fixtures(tmp: str) -> None:
     """rmsdfun.f (quaternion superposition, rmsdfun/rmsdopt/fitmol/jacobi)
     of a start conformation against a turned, shifted and perturbed copy."""
@@ -455,6 +486,8 @@ def main(only: list[str]) -> None:
         metro_fixtures(tmp)
     if not only or "grad" in only:
         grad_fixtures(tmp)
+    if not only or "grad_esolan" in only:
+        grad_esolan_fixtures(tmp)
     if not only or "rmsd" in only:
         rmsd_fixtures(tmp)
     if only and "restart" in only:  # the len10 run of partem_ext_len10.npz again, for its files

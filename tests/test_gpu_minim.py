@@ -1,7 +1,9 @@
 """Derivatives (k_gradient, smmp_gradient / smmp_gradient_batch) and local
 minimisation (smmp_minim, smmp_minim_batch) on the GPU, against the oracle's
 derivatives (pinned to the Fortran's and to finite differences on the CPU)
-and the Fortran's minimisations (tests/golden/minim_vacuum.npz), vacuum."""
+and the Fortran's minimisations (tests/golden/minim_vacuum.npz), in vacuum
+and with the analytic solvent (k_esolan's gradient form, k_sol_project;
+tests/golden/grad_esolan.npz)."""
 import os
 
 import numpy as np
@@ -146,8 +148,116 @@ def test_minim_batch_full_size_36mer():
 
 
 def test_gradient_refuses_solvents():
+    """itysol > 0: gradient.f stops ('Set itysol < 0'); itysol < 0: the
+    device derivatives of the analytic solvent are switched off."""
     from smmp_amd import _ffi
     for ity in (1, -1):
         c = make_ctx(tag="enkefa", itysol=ity)
         with pytest.raises(_ffi.SmmpError, match="itysol"):
             c.gradient_batch(c.vlvr()[None, :])
+
+
+# ---- the analytic solvent (itysol -1): opesol.f over esolan.f's gradient
+#
+# Skipped: the first run of the device form (k_esolan's gradient form)
+# faulted the GPU (memory aperture violation in pass 1, enkefa, 14
+# conformations); a CPU emulation of the kernel under ASan with the same
+# input ran clean, so no cause has been shown and the path is switched off
+# (smmp_gradient refuses itysol < 0).  The oracle's derivatives are pinned
+# on the CPU (tests/test_esolan_grad.py).
+SOLVENT_GRAD_OFF = pytest.mark.skip(reason="device derivatives of the analytic solvent switched off: their first run "
+                                           "faulted the GPU and no cause has been shown")
+
+
+@pytest.mark.parametrize("tag", ["enkefa", "len10", "c2_20mer"])
+@SOLVENT_GRAD_OFF
+def test_gradient_with_esolan_vs_oracle(tag):
+    """Energies (eysm carries eysl, as gradient.f leaves it) and derivatives
+    of the start and random conformations against the oracle's (pinned to
+    the Fortran's and to finite differences): within 1e-9 of the largest
+    derivative (the arc terms use the device's atan/sin/cos, the areas agree
+    with the reference to 1e-9 relative)."""
+    from _oracle import OracleMT, OracleSystem
+    print(f"[gpu] gradient batch with esolan {tag}", flush=True)
+    c = make_ctx(tag=tag, itysol=-1)
+    o = OracleSystem(c)
+    V = np.vstack([c.vlvr()[None, :], random_confs(c, 7, 9)])
+    if tag in ("enkefa", "len10"):
+        V = np.vstack([V, golden("grad_esolan.npz")[f"{tag}_vlvr"]])
+    c.sgrnd(77)  # rotations of degenerate north points: conformations in order from the context's twister
+    mt = OracleMT(77)
+    e, g = c.gradient_batch(V)
+    for k in range(len(V)):
+        oe, og = o.gradient(V[k], mt)
+        ref = np.array([oe[0] + oe[5], oe[1], oe[2], oe[3], oe[4]])
+        assert np.abs(e[k] - ref).max() <= 1e-9 * max(1.0, np.abs(ref).max()), (k, e[k], ref)
+        assert np.abs(g[k] - og).max() <= 1e-9 * max(1.0, np.abs(og).max()), (k, np.abs(g[k] - og).max())
+
+
+@SOLVENT_GRAD_OFF
+def test_gradient_with_esolan_against_reference_fixture():
+    """The Fortran's own derivatives (gradient.f with opesol.f) of the
+    fixture's conformations, directly."""
+    print("[gpu] gradient with esolan vs the Fortran fixture", flush=True)
+    gz = golden("grad_esolan.npz")
+    for tag in ("enkefa", "len10"):
+        c = make_ctx(tag=tag, itysol=-1)
+        c.sgrnd(int(gz[f"{tag}_seed"][0]))  # the driver's stream: its draws of each conformation, then esolan's
+        for k in range(len(gz[f"{tag}_vlvr"])):
+            if k > 0:
+                for _ in range(c.nvr):
+                    c.grnd()
+            e, g = c.gradient_batch(gz[f"{tag}_vlvr"][k][None, :])
+            e, g = e[0], g[0]
+            ref, rg = gz[f"{tag}_energy"][k], gz[f"{tag}_grad"][k]
+            assert np.abs(e - ref).max() <= 1e-9 * max(1.0, np.abs(ref).max()), (tag, k)
+            assert np.abs(g - rg).max() <= 1e-9 * max(1.0, np.abs(rg).max()), (tag, k)
+
+
+@SOLVENT_GRAD_OFF
+def test_gradient_with_esolan_into_state():
+    """smmp_gradient: eysm and eysl in the state, eysl equal to energy()'s."""
+    print("[gpu] smmp_gradient with esolan", flush=True)
+    c = make_ctx(tag="enkefa", itysol=-1, varfile=os.path.join(GOLDEN, "enkefa.var"))
+    eysm, gd = c.gradient()
+    eysl = float(c.view("eny_r.eysl")[0])
+    e, g = c.gradient_batch(c.vlvr()[None, :])
+    assert eysm == e[0, 0] and np.array_equal(gd, g[0])
+    assert abs(eysm - golden("grad_esolan.npz")["enkefa_energy"][0][0]) < 1e-9
+    etot = c.energy()
+    assert abs(float(c.view("eny_r.eysl")[0]) - eysl) <= 1e-9 * max(1.0, abs(eysl))
+    assert abs(etot - eysm) <= 1e-9 * max(1.0, abs(etot))
+
+
+@SOLVENT_GRAD_OFF
+def test_minim_with_esolan_vs_reference():
+    """smmp_minim with the analytic solvent: the Fortran's quasi-Newton run
+    from enkefa.var (40 calls) and its conjugate-gradient run, the same call
+    counts and variables to rounding carried by the updates."""
+    print("[gpu] minim with esolan enkefa", flush=True)
+    gz = golden("grad_esolan.npz")
+    for tag, imin in (("enkefa_qn", 1), ("enkefa_cg", 2)):
+        c = make_ctx(tag="enkefa", itysol=-1, varfile=os.path.join(GOLDEN, "enkefa.var"))
+        assert np.abs(c.vlvr() - gz[f"min_{tag}_start"][0]).max() < 1e-12
+        n = c.minim(imin, 40, 1e-9)
+        assert n == int(gz[f"min_{tag}_nfun"][0]), tag
+        assert np.abs(c.vlvr() - gz[f"min_{tag}_vlvr"][0]).max() < 1e-6, tag
+        ref = gz[f"min_{tag}_energy"][0]
+        assert abs(float(c.view("eny_r.eysm")[0]) - ref[0]) <= 1e-6 * max(1.0, abs(ref[0])), tag
+
+
+@SOLVENT_GRAD_OFF
+def test_minim_batch_with_esolan_equals_single():
+    """Eight starts minimised side by side with the analytic solvent take the
+    same steps as each alone."""
+    print("[gpu] minim batch with esolan vs single", flush=True)
+    c = make_ctx(tag="len10", itysol=-1)
+    V = random_confs(c, 8, 12)
+    Vb, fb, nb = c.minim_batch(1, V, 60, 1e-9)
+    for k in (0, 7):
+        c1 = make_ctx(tag="len10", itysol=-1)
+        c1.set_vlvr(V[k])
+        n1 = c1.minim(1, 60, 1e-9)
+        assert n1 == nb[k] and np.array_equal(c1.vlvr(), Vb[k]), k
+    e, _ = c.gradient_batch(Vb)
+    assert np.abs(e[:, 0] - fb).max() <= 1e-9 * max(1.0, np.abs(fb).max())
