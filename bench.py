@@ -902,6 +902,8 @@ def main():
         log(f"pair change per trial: slowest wave {ph['pair_max_wave'] * 1e3 / ntrial:.2f} us, "
             f"wave average {ph['pair_sum'] * 1e3 / ntrial / 8:.2f} us; contact marking {ph['mark_sum'] * 1e3 / ntrial / 8:.2f} us; "
             f"torsion term (thread 0) {ph['torsion'] * 1e3 / ntrial:.2f} us")
+        log(f"reduction per trial (workgroup 0 of each replica): waiting for the last arrival "
+            f"{ph['sync_wait'] * 1e3 / ntrial:.2f} us, from the last arrival to release {ph['sync_latency'] * 1e3 / ntrial:.2f} us")
     c.profile(False)
     value = NREP * args.steps / dt
     npnt = int(c.view("pnt.npnt")[0])

@@ -387,7 +387,10 @@ int smmp_profile_read(smmp_ctx *ctx, int kernel, double *total_ms, int64_t *laun
  * summed over waves, out[14] atoms evaluated, out[15] unused; pair change
  * of the same workgroups: out[16] ms of the slowest wave, out[17] ms summed
  * over waves, out[18] ms of contact marking summed over waves, out[19] ms of
- * thread 0's torsion term.  out must hold 20 values. */
+ * thread 0's torsion term; reduction across a replica's workgroups (the
+ * same workgroups): out[20] ms waiting for the last workgroup to arrive,
+ * out[21] ms from that arrival until the wait ends.  out must hold 22
+ * values. */
 int smmp_profile_phases(smmp_ctx *ctx, double *out);
 
 #ifdef __cplusplus

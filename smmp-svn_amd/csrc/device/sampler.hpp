@@ -170,7 +170,11 @@ enum PhaseId { kPhPropose = 0, kPhRotate, kPhPairs, kPhSolvent, kPhReduce, kPhAc
                kPhSolMax, kPhSolGather, kPhSolPoints, kPhSolAtoms, kPhSolContacts,
                // pair change, per trial: slowest wave and wave sum of pair_delta,
                // wave sum of the contact marking, thread 0's torsion term
-               kPhPairMax, kPhPairSum, kPhMarkSum, kPhTorsion, kNPhase };
+               kPhPairMax, kPhPairSum, kPhMarkSum, kPhTorsion,
+               // reduction across the workgroups of a replica, per trial: the
+               // wait for the last workgroup's arrival, and the time from that
+               // arrival until this workgroup leaves the wait
+               kPhSyncWait, kPhSyncLatency, kNPhase };
 // broadcast scalars [0,32), phase clocks [32,64), per-wave stats [64,160),
 // rigid-move transform [160,176), accanalyze counters [176,186)
 constexpr int kScDoubles = 192;
@@ -547,13 +551,23 @@ __device__ __forceinline__ void replica_of_block(const SamplerParams& P, int b, 
 // values still being read.  The wait is bounded on the device clock; on
 // expiry the error flag gets bit 3 and the block goes on.
 __device__ void group_wait(const SamplerParams& P, int r, uint32_t epoch);
-__device__ void group_sum(const SamplerParams& P, int r, int g, uint32_t& epoch, double* v, int nv) {
+__device__ void group_sum(const SamplerParams& P, int r, int g, uint32_t& epoch, double* v, int nv,
+                          double* ph = nullptr) {
   const int G = P.G;
   if (G == 1) return;
   double* base = P.part + ((size_t)r * 2 + (epoch & 1u)) * G * kPartVals;
   double* mine = base + (size_t)g * kPartVals;
   for (int k = 0; k < nv; ++k) mine[k] = v[k];
+  const double arrive = ph ? (double)wall_clock64() : 0.0;
+  if (ph) mine[kPartVals - 1] = arrive;  // timing only: the arrival stamp rides in the last slot
   group_wait(P, r, epoch);
+  if (ph) {
+    const double now = (double)wall_clock64();
+    double last = arrive;
+    for (int q = 0; q < G; ++q) last = fmax(last, base[(size_t)q * kPartVals + kPartVals - 1]);
+    ph[kPhSyncWait] += last - arrive;
+    ph[kPhSyncLatency] += now - last;
+  }
   for (int k = 0; k < nv; ++k) v[k] = 0.0;
   for (int q = 0; q < G; ++q) {
     const double* src = base + (size_t)q * kPartVals;
@@ -1415,7 +1429,7 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
             for (int k = 0; k < 3; ++k) tot[k] += L.red[8 * w + k];
             tot[5] += L.red[8 * w + 6];
           }
-          group_sum(P, r, g, epoch, tot, 6);
+          group_sum(P, r, g, epoch, tot, 6, P.phase ? ph : nullptr);
           if (tot[5] != 0.0) atomicOr(P.err, 2);
           Q.sc[16] = Q.sc[0] + (tot[4] + tot[0] + tot[1] + tot[2] + tot[3]);
           for (int k = 0; k < 4; ++k) Q.sc[24 + k] = tot[k];  // el, vw, hb, sol deltas (verify mode)

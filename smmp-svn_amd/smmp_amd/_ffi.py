@@ -280,13 +280,13 @@ class Context:
                 "graph_reused": bool(out[3])}
 
     def profile_phases(self) -> dict:
-        out = np.zeros(20)
+        out = np.zeros(22)
         self.check(lib.smmp_profile_phases(self.h, dptr(out)))
         d = {k: out[i] for i, k in enumerate(self.PHASES)}
         d["full_trials"] = out[8]
         d["core_mhz"] = out[9] / (out[10] * 1e3) if out[10] > 0 else 0.0
         for i, k in enumerate(("sol_max_wave", "sol_gather", "sol_points", "sol_atoms", "sol_contacts",
-                               "pair_max_wave", "pair_sum", "mark_sum", "torsion")):
+                               "pair_max_wave", "pair_sum", "mark_sum", "torsion", "sync_wait", "sync_latency")):
             d[k] = out[11 + i]
         return d
 
