@@ -869,6 +869,29 @@ __device__ __attribute__((noinline)) void spec_apply(const DevSys* sys, int nvt,
   __syncthreads();
 }
 
+// Torsion term of variable jv (enyshe.f:73-79) at its trial angle (tl 0,
+// Q.sc[2]) and its old angle (tl 1, Q.sc[15]) into Q.sc[17 + tl], on block 0
+// of the replica; other lanes and blocks return.  The new value is read
+// from Q.sc[2], not Q.vl, which may already hold it.
+__device__ __forceinline__ void torsion_pair(const DevSys& S, const SamplerLds& Q, const VarLds& V, int jv, int tl,
+                                             int g) {
+  if (g != 0 || (tl != 0 && tl != 1)) return;
+  const double e0 = V.e0[jv];
+  double e = 0.0;
+  if (e0 != 0.0) {
+    const int v = V.tv[jv];
+    const int ia = V.t0 ? 0 : S.var_atom[jv];
+    double to = V.t0 ? V.t0[jv] : S.toat0[ia];
+    if (v >= 0) {
+      const double val = v == jv ? (tl == 0 ? Q.sc[2] : Q.sc[15]) : Q.vl[v];
+      to = to + d_difang(V.t0 ? V.tref[jv] : S.tref[ia], val);
+      if (fabs(to) > kPi) to = to - copysign(kPi2, to);
+    }
+    e = e0 * (1.0 + V.sg[jv] * cos(to * V.rn[jv]));
+  }
+  Q.sc[17 + tl] = e;  // [17] new, [18] old
+}
+
 // P.nsweep Metropolis sweeps of one replica per group of G blocks; DA:
 // delayed acceptance with the surrogate network (P.da).
 template <bool MIXED, bool DA>
@@ -1298,25 +1321,11 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
             L.hp[4 * h + 2] = zn;
           }
         }
-        // torsion term of the variable (enyshe.f:73-79), the new and the old
-        // angle on two idle lanes of the last wave
-        const int tl = tid - ((int)blockDim.x - kWave);
-        if (g == 0 && (tl == 0 || tl == 1)) {
-          const double e0 = V.e0[jv];
-          double e = 0.0;
-          if (e0 != 0.0) {
-            const int v = V.tv[jv];
-            const int ia = V.t0 ? 0 : S.var_atom[jv];
-            double to = V.t0 ? V.t0[jv] : S.toat0[ia];
-            if (v >= 0) {
-              const double val = v == jv ? (tl == 0 ? Q.sc[2] : Q.sc[15]) : Q.vl[v];
-              to = to + d_difang(V.t0 ? V.tref[jv] : S.tref[ia], val);
-              if (fabs(to) > kPi) to = to - copysign(kPi2, to);
-            }
-            e = e0 * (1.0 + V.sg[jv] * cos(to * V.rn[jv]));
-          }
-          Q.sc[17 + tl] = e;  // [17] new, [18] old
-        }
+        // torsion term of the variable, new and old angle: here on two idle
+        // lanes of the last wave in vacuo; with the solvent term that wave
+        // does it at the start of the solvent phase instead, whose atom queue
+        // absorbs it (the rotation phase waits for it otherwise)
+        if (S.itysol <= 0) torsion_pair(S, Q, V, jv, tid - ((int)blockDim.x - kWave), g);
         for (int h = tid; h < nh; h += blockDim.x) Q.aff[h] = 0;
         if (tid == 0) Q.vl[jv] = Q.sc[2];
       }
@@ -1341,7 +1350,6 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
           pst[1] += now - tpa;
           tpa = now;
         }
-        const double dt = g == 0 ? Q.sc[17] - Q.sc[18] : 0.0;  // torsion change, from the rotation phase
         __syncthreads();
         if (P.phase && tid == 0) {
           double mx = 0.0;
@@ -1372,6 +1380,7 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
           const int na = (int)Q.sc[29];
           double* wst = P.phase ? Q.sc + 64 + 4 * wave : nullptr;  // this wave's timing (lane 0)
           const double tw0 = wst ? (double)wall_clock64() : 0.0;
+          if (wave == nwaves - 1 && !rigid) torsion_pair(S, Q, V, jv, lane, g);  // then this wave joins the atom queue
           // waves take the next atom as they finish (atom costs vary with their contacts)
           for (;;) {
             int idx = 0;
@@ -1424,6 +1433,7 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
           ph[kPhSolMax] += mx;
         }
         if (tid == 0) {
+          const double dt = g == 0 ? Q.sc[17] - Q.sc[18] : 0.0;  // torsion change (torsion_pair)
           double tot[6] = {0, 0, 0, dsol, dt, 0};  // el, vw, hb, sol, torsion, error
           for (int w = 0; w < nwaves; ++w) {
             for (int k = 0; k < 3; ++k) tot[k] += L.red[8 * w + k];
