@@ -550,6 +550,21 @@ __device__ __forceinline__ void replica_of_block(const SamplerParams& P, int b, 
 // (it waits for all of them at each reduction), so it never overwrites
 // values still being read.  The wait is bounded on the device clock; on
 // expiry the error flag gets bit 3 and the block goes on.
+//
+// Every access to the shared partials and counters is an agent-scope atomic
+// (stores and loads reach the coherent level), the partials' stores are
+// complete (s_waitcnt vmcnt(0)) before the arrival is counted, and a block
+// reads the partials only after it has seen every arrival.  Nothing shared
+// is read through a non-coherent cache, so the agent-scope release and
+// acquire fences are not needed: on gfx950 they write back every dirty line
+// of the XCD's L2 (the spill slots of all its workgroups among them) and
+// invalidate it, once per trial and workgroup.
+__device__ __forceinline__ void part_store(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double part_load(const double* p) {
+  return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 __device__ void group_wait(const SamplerParams& P, int r, uint32_t epoch);
 __device__ void group_sum(const SamplerParams& P, int r, int g, uint32_t& epoch, double* v, int nv,
                           double* ph = nullptr) {
@@ -557,21 +572,21 @@ __device__ void group_sum(const SamplerParams& P, int r, int g, uint32_t& epoch,
   if (G == 1) return;
   double* base = P.part + ((size_t)r * 2 + (epoch & 1u)) * G * kPartVals;
   double* mine = base + (size_t)g * kPartVals;
-  for (int k = 0; k < nv; ++k) mine[k] = v[k];
+  for (int k = 0; k < nv; ++k) part_store(&mine[k], v[k]);
   const double arrive = ph ? (double)wall_clock64() : 0.0;
-  if (ph) mine[kPartVals - 1] = arrive;  // timing only: the arrival stamp rides in the last slot
+  if (ph) part_store(&mine[kPartVals - 1], arrive);  // timing only: the arrival stamp rides in the last slot
   group_wait(P, r, epoch);
   if (ph) {
     const double now = (double)wall_clock64();
     double last = arrive;
-    for (int q = 0; q < G; ++q) last = fmax(last, base[(size_t)q * kPartVals + kPartVals - 1]);
+    for (int q = 0; q < G; ++q) last = fmax(last, part_load(&base[(size_t)q * kPartVals + kPartVals - 1]));
     ph[kPhSyncWait] += last - arrive;
     ph[kPhSyncLatency] += now - last;
   }
   for (int k = 0; k < nv; ++k) v[k] = 0.0;
   for (int q = 0; q < G; ++q) {
     const double* src = base + (size_t)q * kPartVals;
-    for (int k = 0; k < nv; ++k) v[k] += src[k];
+    for (int k = 0; k < nv; ++k) v[k] += part_load(&src[k]);
   }
   ++epoch;
 }
@@ -582,21 +597,24 @@ __device__ void group_gather(const SamplerParams& P, int r, int g, uint32_t& epo
   const int G = P.G;
   double* base = P.part + ((size_t)r * 2 + (epoch & 1u)) * G * kPartVals;
   double* mine = base + (size_t)g * kPartVals;
-  for (int k = 0; k < kPartVals; ++k) mine[k] = v[k];
+  for (int k = 0; k < kPartVals; ++k) part_store(&mine[k], v[k]);
   group_wait(P, r, epoch);
-  for (int k = 0; k < G * kPartVals; ++k) out[k] = base[k];
+  for (int k = 0; k < G * kPartVals; ++k) out[k] = part_load(&base[k]);
   ++epoch;
 }
 
 // Arrival of this block at meeting 'epoch' of replica r and the bounded
-// wait for all G blocks (thread 0): on return the rows written before every
-// block's arrival are visible.
+// wait for all G blocks (thread 0): on return every block's partials
+// stored before its arrival can be read with part_load.
 __device__ void group_wait(const SamplerParams& P, int r, uint32_t epoch) {
   const int G = P.G;
-  __hip_atomic_fetch_add(&P.sync[r], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  // the signal fences keep the compiler from moving the partials' atomics
+  // across the arrival and the polls; they emit no instruction
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this block's partials are stored
+  __hip_atomic_fetch_add(&P.sync[r], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   const uint32_t target = (epoch + 1u) * (uint32_t)G;
   const uint64_t t0 = wall_clock64();
-  // relaxed polls (no cache invalidation per poll), one acquire fence after
   while (__hip_atomic_load(&P.sync[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
     __builtin_amdgcn_s_sleep(1);
     // after one expiry anywhere, no block waits again: the launch drains
@@ -606,7 +624,7 @@ __device__ void group_wait(const SamplerParams& P, int r, uint32_t epoch) {
       break;
     }
   }
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the partials are visible now
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
 // Temperature exchanges between neighbouring slots (partem_s.f:209-232):
