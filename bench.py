@@ -47,9 +47,9 @@ FLOP_ES_ARC = 30    # one arc: midpoint (cos, sin, 6) and, when exposed, its ter
 FLOP_ES_COVER = 7   # one circle tested against an arc midpoint: 3 mul, 3 add, 1 compare
 # HBM-side bytes per sweep of this workload in FP64: FETCH_SIZE x 2.0 plus
 # WRITE_SIZE x 1.0 (the factors make a 256 MiB device copy read 256 MiB and
-# write 256 MiB) of the timed launch, rocprofv3 --pmc, 1000 sweeps:
-# 4,587,491 KiB + 9,686,131 KiB.
-HBM_BYTES_PER_SWEEP = 14.616e6
+# write 256 MiB) of the timed launch, rocprofv3 --pmc, 200 sweeps:
+# 570,547 KiB x 2.0 + 2,670,886 KiB.
+HBM_BYTES_PER_SWEEP = 19.517e6
 # The same for one batch of the batched-energy workload (4,096 conformations,
 # mixed): (123,055 + 39,745) KiB FETCH_SIZE x 2.0 + 427 KiB WRITE_SIZE x 1.0.
 C3_HBM_BYTES_PER_BATCH = 333.8e6
