@@ -62,6 +62,7 @@ constexpr int kEsThreads = 256;  // four wavefronts per workgroup, each on its o
 constexpr int kEsWaves = kEsThreads / kWave;
 constexpr int kEsSlots = 192;    // circle slots of one atom (at most 191 neighbours)
 constexpr int kEsMaxNb = kEsSlots - 1;
+constexpr int kEsAngLds = 4 * kEsSlots;  // LDS doubles for the bucket angles
 constexpr double kEsCoinc = (double)1e-10f;  // esolan.f:127 compares with a default-real literal
 constexpr double kPi4 = 4.0 * kPi;
 
@@ -73,9 +74,16 @@ constexpr int kNone = 0, kDropL = 1, kVert = 2, kDropK = 4, kZeroAs = 5, kCover 
 
 struct EsLds {
   double A[kEsSlots], B[kEsSlots], C[kEsSlots], D[kEsSlots];  // circle coefficients by slot
-  // b/a, c/a, d/a and sqrt((b/a)^2 + (c/a)^2 - 4 d/a) of each circle: the
-  // values esolan.f:255-268 forms for every pair, formed once
-  double nB[kEsSlots], nC[kEsSlots], nD[kEsSlots], nR[kEsSlots];
+  union {
+    // b/a, c/a, d/a and sqrt((b/a)^2 + (c/a)^2 - 4 d/a) of each circle: the
+    // values esolan.f:255-268 forms for every pair, formed once
+    struct {
+      double nB[kEsSlots], nC[kEsSlots], nD[kEsSlots], nR[kEsSlots];
+    };
+    // after the pair loop: the polar angles of the bucket entries and, when
+    // both fit, their sorted copy (kEsAngLds entries in all)
+    double ang[4 * kEsSlots];
+  };
   int16_t atom[kEsSlots];                                     // neighbour atom of each slot
   uint8_t lst[kEsSlots], tmp[kEsSlots];                       // circle list (slots) and compaction buffer
   uint8_t rm[kEsSlots];                                       // per list position: removed in this step
@@ -154,6 +162,25 @@ __device__ inline EsScratch es_carve(char* p, int cap, bool grad) {
     s.gk2 = s.gk1 + 2 * c;
   }
   return s;
+}
+
+// Rank of entry e (angle x) within its bucket [lo, hi): the entries below
+// it, ties broken by entry order (esolan.f:642-659 sorts stably).
+template <typename T>
+__device__ __forceinline__ int es_rank(const T* ang, int lo, int hi, int e, double x) {
+  int r = 0, q = lo;
+  for (; q + 4 <= hi; q += 4) {
+    const double y0 = ang[q], y1 = ang[q + 1], y2 = ang[q + 2], y3 = ang[q + 3];
+    r += (y0 < x || (y0 == x && q < e)) ? 1 : 0;
+    r += (y1 < x || (y1 == x && q + 1 < e)) ? 1 : 0;
+    r += (y2 < x || (y2 == x && q + 2 < e)) ? 1 : 0;
+    r += (y3 < x || (y3 == x && q + 3 < e)) ? 1 : 0;
+  }
+  for (; q < hi; ++q) {
+    const double y = ang[q];
+    r += (y < x || (y == x && q < e)) ? 1 : 0;
+  }
+  return r;
 }
 
 __device__ __forceinline__ uint64_t lanes_below(int lane) { return (lane == 0) ? 0ull : (~0ull >> (64 - lane)); }
@@ -611,23 +638,36 @@ __device__ void es_item(const EsParams& P, EsLds& L, EsGradLds& G, const EsScrat
     }
   }
   wave_lds_sync();
-  // polar angles about each circle's centre (esolan.f:626-640)
+  // polar angles about each circle's centre (esolan.f:626-640); in LDS
+  // over the normalised coefficients (dead after the pair loop) when they
+  // fit, the sorted copy too when both do, else in the wave's scratch
+  const bool ang_lds = E <= kEsAngLds, srt_lds = 2 * E <= kEsAngLds;
   for (int e = lane; e < E; e += kWave) {
     const int s = X.entc[e], v = X.ent[e];
     const double a = L.A[s], aa = 0.5 / a;
-    X.ang[e] = atan2(X.vs[v] + L.C[s] * aa, X.vt[v] + L.B[s] * aa);
+    const double t = atan2(X.vs[v] + L.C[s] * aa, X.vt[v] + L.B[s] * aa);
+    if (ang_lds)
+      L.ang[e] = t;
+    else
+      X.ang[e] = t;
   }
   wave_lds_sync();
   // ascending within each bucket (esolan.f:642-659)
   for (int e = lane; e < E; e += kWave) {
     const int s = X.entc[e], lo = L.off[s], hi = lo + L.cnt[s];
-    const double x = X.ang[e];
-    int r = 0;
-    for (int q = lo; q < hi; ++q) {
-      const double y = X.ang[q];
-      r += (y < x || (y == x && q < e)) ? 1 : 0;
+    double x;
+    int r;
+    if (ang_lds) {
+      x = L.ang[e];
+      r = es_rank(L.ang, lo, hi, e, x);
+    } else {
+      x = X.ang[e];
+      r = es_rank(X.ang, lo, hi, e, x);
     }
-    X.srt[lo + r] = x;
+    if (srt_lds)
+      L.ang[E + lo + r] = x;
+    else
+      X.srt[lo + r] = x;
     if constexpr (GRAD) {  // the other circle through this point
       const int w = X.vc[X.ent[e]], c1 = w & 255, c2 = w >> 8;
       X.oth[lo + r] = (uint8_t)(c1 == s ? c2 : c1);
@@ -638,7 +678,16 @@ __device__ void es_item(const EsParams& P, EsLds& L, EsGradLds& G, const EsScrat
   double arcs = 0.0;
   for (int p = lane; p < E; p += kWave) {
     const int s = X.entc[p], lo = L.off[s], hi = lo + L.cnt[s];
-    const double alp = X.srt[p], bet = (p + 1 < hi) ? X.srt[p + 1] : X.srt[lo] + kPi2;
+    const int pn = (p + 1 < hi) ? p + 1 : lo;
+    double alp, bet;
+    if (srt_lds) {
+      alp = L.ang[E + p];
+      bet = L.ang[E + pn];
+    } else {
+      alp = X.srt[p];
+      bet = X.srt[pn];
+    }
+    if (pn == lo) bet = bet + kPi2;
     if (GRAD) X.gk1[p] = 255;
     if (fabs(bet - alp) < 1e-8) continue;
     const double a = L.A[s], b = L.B[s], cc = L.C[s], d = L.D[s];
@@ -652,12 +701,24 @@ __device__ void es_item(const EsParams& P, EsLds& L, EsGradLds& G, const EsScrat
     sincos(prob, &sp, &cp);
     const double ap1 = -ba + rr * cp, ap2 = -ca + rr * sp;
     const double q2 = ap1 * ap1 + ap2 * ap2;
+    // the first circle of the list (other than s) that covers the midpoint;
+    // four at a time, so their LDS reads overlap instead of chaining behind
+    // the test; q ends one past it (n when none does), as a scan one by one
     bool cov = false;
     int q = 0;
-    for (; q < n && !cov; ++q) {
-      const int t = L.lst[q];
-      if (t != s && L.A[t] * q2 + L.B[t] * ap1 + L.C[t] * ap2 + L.D[t] < 0.0) cov = true;
+    auto covers = [&](int t) {
+      return t != s && L.A[t] * q2 + L.B[t] * ap1 + L.C[t] * ap2 + L.D[t] < 0.0;
+    };
+    for (; q + 4 <= n; q += 4) {
+      const int t0 = L.lst[q], t1 = L.lst[q + 1], t2 = L.lst[q + 2], t3 = L.lst[q + 3];
+      const bool c0 = covers(t0), c1 = covers(t1), c2 = covers(t2), c3 = covers(t3);
+      if (c0 || c1 || c2 || c3) {
+        q += c0 ? 1 : (c1 ? 2 : (c2 ? 3 : 4));
+        cov = true;
+        break;
+      }
     }
+    for (; q < n && !cov; ++q) cov = covers(L.lst[q]);
     work[2] += 1.0;
     work[3] += (double)q;
     if (cov) continue;
