@@ -63,6 +63,7 @@ constexpr int kEsWaves = kEsThreads / kWave;
 constexpr int kEsSlots = 192;    // circle slots of one atom (at most 191 neighbours)
 constexpr int kEsMaxNb = kEsSlots - 1;
 constexpr int kEsAngLds = 4 * kEsSlots;  // LDS doubles for the bucket angles
+constexpr int kEsTri = 12 * kEsSlots;    // LDS bytes for the pair outcomes
 constexpr double kEsCoinc = (double)1e-10f;  // esolan.f:127 compares with a default-real literal
 constexpr double kPi4 = 4.0 * kPi;
 
@@ -89,7 +90,14 @@ struct EsLds {
   uint8_t rm[kEsSlots];                                       // per list position: removed in this step
   uint8_t hasv[kEsSlots];                                     // per slot: an intersection point lies on it
   uint8_t fin[kEsSlots];                                      // per slot: final circle with points
-  int32_t cnt[kEsSlots], off[kEsSlots], cur[kEsSlots];        // point buckets by slot
+  union {
+    struct {
+      int32_t cnt[kEsSlots], off[kEsSlots], cur[kEsSlots];  // point buckets by slot
+    };
+    // during the pair loop: the outcome of every pair of slots K < Lc at
+    // Lc (Lc - 1) / 2 + K (up to kEsTri pairs)
+    uint8_t code[12 * kEsSlots];
+  };
 };
 
 // Gradient form: per slot, the derivative of the atom's area by that
@@ -192,8 +200,24 @@ __device__ __forceinline__ double wave_min(double v) {
 }
 
 // Two circles of the projection (slot K at list position k, slot Lc at a
-// higher position): esolan.f:254-449.
-__device__ __forceinline__ int es_classify(const EsLds& L, int K, int Lc, double v[4]) {
+// higher position): esolan.f:254-449.  es_code gives the outcome,
+// es_vertices the two intersection points of a kVert pair (the same
+// operations on the same values, so either order gives the same bits).
+__device__ __forceinline__ void es_vertices(const EsLds& L, int K, int Lc, double v[4]) {
+  const double b1 = L.nB[K], c1 = L.nC[K], d1 = L.nD[K];
+  const double b2 = L.nB[Lc], c2 = L.nC[Lc], d2 = L.nD[Lc];
+  const double D = 4.0 * ((b1 - b2) * (b2 * d1 - b1 * d2) + (c1 - c2) * (c2 * d1 - c1 * d2) - (d1 - d2) * (d1 - d2)) +
+                   (b1 * c2 - b2 * c1) * (b1 * c2 - b2 * c1);
+  const double am = 2.0 * ((b1 - b2) * (b1 - b2) + (c1 - c2) * (c1 - c2));
+  const double t = -2.0 * (b1 - b2) * (d1 - d2) - (b2 * c1 - b1 * c2) * (c1 - c2);
+  const double s = -2.0 * (c1 - c2) * (d1 - d2) + (b2 * c1 - b1 * c2) * (b1 - b2);
+  const double pm = sqrt(D), p1 = (c1 - c2) * pm, p2 = (b1 - b2) * pm;
+  v[0] = (t + p1) / am;
+  v[1] = (s - p2) / am;
+  v[2] = (t - p1) / am;
+  v[3] = (s + p2) / am;
+}
+__device__ __forceinline__ int es_code(const EsLds& L, int K, int Lc) {
   const double ak = L.A[K], al = L.A[Lc];
   int kind = 0;
   if (ak > 0.0 && al > 0.0) kind = 1;
@@ -225,16 +249,11 @@ __device__ __forceinline__ int es_classify(const EsLds& L, int K, int Lc, double
     else if (D <= 0.0) code = kCover;
     else code = kVert;
   }
-  if (code == kVert) {
-    const double am = 2.0 * ((b1 - b2) * (b1 - b2) + (c1 - c2) * (c1 - c2));
-    const double t = -2.0 * (b1 - b2) * (d1 - d2) - (b2 * c1 - b1 * c2) * (c1 - c2);
-    const double s = -2.0 * (c1 - c2) * (d1 - d2) + (b2 * c1 - b1 * c2) * (b1 - b2);
-    const double pm = sqrt(D), p1 = (c1 - c2) * pm, p2 = (b1 - b2) * pm;
-    v[0] = (t + p1) / am;
-    v[1] = (s - p2) / am;
-    v[2] = (t - p1) / am;
-    v[3] = (s + p2) / am;
-  }
+  return code;
+}
+__device__ __forceinline__ int es_classify(const EsLds& L, int K, int Lc, double v[4]) {
+  const int code = es_code(L, K, Lc);
+  if (code == kVert) es_vertices(L, K, Lc, v);
   return code;
 }
 
@@ -456,9 +475,23 @@ __device__ void es_item(const EsParams& P, EsLds& L, EsGradLds& G, const EsScrat
   }
   wave_lds_sync();
 
-  // 4. pairs of circles (esolan.f:251-451)
+  // 4. pairs of circles (esolan.f:251-451).  A pair's outcome depends on
+  // its two circles only, and list positions keep the slots in increasing
+  // order (removal is stable), so when the codes of all pairs fit in LDS
+  // they are formed first, all lanes at once, and the loop below only
+  // reads them; the intersection points of the pairs it keeps follow in
+  // one parallel pass.
   int n = nb, k = nb, iv = 0;
   bool covered = false, zero_as = false, overflow = false;
+  const bool tab = nb * (nb - 1) / 2 <= kEsTri;
+  if (tab) {
+    const float inb = 1.0f / (float)nb;
+    for (int t = lane; t < nb * nb; t += kWave) {
+      const int K = (int)(((float)t + 0.5f) * inb), Lc = t - K * nb;  // exact: t < 191^2
+      if (K < Lc) L.code[Lc * (Lc - 1) / 2 + K] = (uint8_t)es_code(L, K, Lc);
+    }
+    wave_lds_sync();
+  }
   while (k > 1) {
     k = k - 1;
     const int cnt = n - k;  // positions n ... k+1, fixed when the loop starts
@@ -472,7 +505,10 @@ __device__ void es_item(const EsParams& P, EsLds& L, EsGradLds& G, const EsScrat
       double v[4] = {0.0, 0.0, 0.0, 0.0};
       if (valid) {
         Lc = L.lst[lp - 1];
-        code = es_classify(L, K, Lc, v);
+        if (tab)
+          code = L.code[Lc * (Lc - 1) / 2 + K];  // K < Lc
+        else
+          code = es_classify(L, K, Lc, v);
       }
       work[1] += valid ? 1.0 : 0.0;
       const uint64_t bm = __ballot(code >= kDropK);
@@ -485,10 +521,12 @@ __device__ void es_item(const EsParams& P, EsLds& L, EsGradLds& G, const EsScrat
         if (live && code == kVert) {
           const int o = iv + 2 * __popcll(vm & below);
           if (o + 2 <= P.cap) {
-            X.vt[o] = v[0];
-            X.vs[o] = v[1];
-            X.vt[o + 1] = v[2];
-            X.vs[o + 1] = v[3];
+            if (!tab) {
+              X.vt[o] = v[0];
+              X.vs[o] = v[1];
+              X.vt[o + 1] = v[2];
+              X.vs[o + 1] = v[3];
+            }
             X.vc[o] = X.vc[o + 1] = (uint16_t)(K | (Lc << 8));
           }
           L.hasv[K] = 1;
@@ -538,6 +576,19 @@ __device__ void es_item(const EsParams& P, EsLds& L, EsGradLds& G, const EsScrat
   if (covered) {  // esolan.f:325-326, 428-429
     if (lane == 0) P.asig[item] = 0.0;
     return;
+  }
+  if (tab) {  // the intersection points of the pairs kept, in their order
+    wave_lds_sync();
+    for (int o = 2 * lane; o < iv; o += 2 * kWave) {
+      const int w = X.vc[o];
+      double v[4];
+      es_vertices(L, w & 255, w >> 8, v);
+      X.vt[o] = v[0];
+      X.vs[o] = v[1];
+      X.vt[o + 1] = v[2];
+      X.vs[o + 1] = v[3];
+    }
+    wave_lds_sync();
   }
   // the position of a neighbour relative to the atom, after its rotation
   auto slot_xyz = [&](int s, double& x, double& y, double& z) {
