@@ -475,12 +475,16 @@ __device__ void es_item(const EsParams& P, EsLds& L, EsGradLds& G, const EsScrat
   }
   wave_lds_sync();
 
-  // 4. pairs of circles (esolan.f:251-451).  A pair's outcome depends on
-  // its two circles only, and list positions keep the slots in increasing
-  // order (removal is stable), so when the codes of all pairs fit in LDS
-  // they are formed first, all lanes at once, and the loop below only
-  // reads them; the intersection points of the pairs it keeps follow in
-  // one parallel pass.
+  // 4. pairs of circles (esolan.f:251-451).  List positions keep the slots
+  // in increasing order (removal is stable) and only the circle at position
+  // k or those above it are ever removed, so position k - 1 always holds
+  // slot k - 1, and the circles scanned at step k are the live slots above
+  // K = k - 1, from the highest down.  A pair's outcome depends on its two
+  // circles only.  With up to kEsTri pairs their outcomes are formed first,
+  // all lanes at once, and the loop keeps the live slots as a bit mask (at
+  // most 68 slots: two words) instead of a list; the intersection points of
+  // the pairs it keeps follow in one parallel pass.  More neighbours: the
+  // list form below.
   int n = nb, k = nb, iv = 0;
   bool covered = false, zero_as = false, overflow = false;
   const bool tab = nb * (nb - 1) / 2 <= kEsTri;
@@ -491,82 +495,136 @@ __device__ void es_item(const EsParams& P, EsLds& L, EsGradLds& G, const EsScrat
       if (K < Lc) L.code[Lc * (Lc - 1) / 2 + K] = (uint8_t)es_code(L, K, Lc);
     }
     wave_lds_sync();
-  }
-  while (k > 1) {
-    k = k - 1;
-    const int cnt = n - k;  // positions n ... k+1, fixed when the loop starts
-    if (cnt <= 0) continue;
-    const int K = L.lst[k - 1];
-    bool brk = false, anyrm = false;
-    for (int base = 0; base < cnt && !brk; base += kWave) {
-      const int lp = n - base - lane;  // 1-based list position of this lane
-      const bool valid = base + lane < cnt;
-      int code = kNone, Lc = 0;
-      double v[4] = {0.0, 0.0, 0.0, 0.0};
-      if (valid) {
-        Lc = L.lst[lp - 1];
-        if (tab)
-          code = L.code[Lc * (Lc - 1) / 2 + K];  // K < Lc
-        else
-          code = es_classify(L, K, Lc, v);
+    uint64_t alive[2] = {nb >= kWave ? ~0ull : (1ull << nb) - 1ull, nb > kWave ? (1ull << (nb - kWave)) - 1ull : 0ull};
+    uint64_t hv[2] = {0ull, 0ull};  // slots with an intersection point
+    double pairs = 0.0;             // pairs examined (esolan.f's count: up to the ending one)
+    for (int K = nb - 2; K >= 0 && !covered; --K) {
+      int code[2];
+      uint64_t act[2];
+#pragma unroll
+      for (int w = 0; w < 2; ++w) {  // live slots above K, their outcomes with K
+        const int lo = K + 1 - kWave * w;  // first bit above K in word w
+        act[w] = lo <= 0 ? alive[w] : (lo >= kWave ? 0ull : alive[w] & ~((1ull << lo) - 1ull));
+        const int Lc = kWave * w + lane;
+        code[w] = ((act[w] >> lane) & 1ull) ? L.code[Lc * (Lc - 1) / 2 + K] : kNone;
       }
-      work[1] += valid ? 1.0 : 0.0;
-      const uint64_t bm = __ballot(code >= kDropK);
-      const int first = bm ? __ffsll((unsigned long long)bm) - 1 : kWave;
-      const bool live = valid && lane < first;
-      if (live && code == kDropL) L.rm[lp - 1] = 1;
-      anyrm = anyrm || __ballot(live && code == kDropL) != 0;
-      const uint64_t vm = __ballot(live && code == kVert);
-      if (vm) {
-        if (live && code == kVert) {
-          const int o = iv + 2 * __popcll(vm & below);
-          if (o + 2 <= P.cap) {
-            if (!tab) {
+      for (int w = 1; w >= 0; --w) {  // the higher word first, each from its top bit
+        if (!act[w]) continue;
+        const uint64_t ev = __ballot(code[w] >= kDropK) & act[w];
+        const int f = ev ? 63 - __clzll(ev) : -1;                          // the first event in scan order
+        const uint64_t live = f < 0 ? act[w] : act[w] & ~((2ull << f) - 1ull);  // scanned before it (f = 63: none)
+        pairs += (double)(__popcll(live) + (f >= 0 ? 1 : 0));
+        alive[w] &= ~(__ballot(code[w] == kDropL) & live);
+        const uint64_t vm = __ballot(code[w] == kVert) & live;
+        if (vm) {
+          if ((vm >> lane) & 1ull) {  // descending slots: offset by the kept pairs above this lane
+            const uint64_t above = lane == 63 ? 0ull : ~((2ull << lane) - 1ull);
+            const int o = iv + 2 * __popcll(vm & above);
+            if (o + 2 <= P.cap) X.vc[o] = X.vc[o + 1] = (uint16_t)(K | ((kWave * w + lane) << 8));
+          }
+          hv[w] |= vm;
+          hv[K >> 6] |= 1ull << (K & 63);
+          iv += 2 * __popcll(vm);
+        }
+        if (f >= 0) {
+          const int bcode = __shfl(code[w], f, kWave);
+          if (bcode == kCover) covered = true;
+          else if (bcode == kDropK) alive[K >> 6] &= ~(1ull << (K & 63));
+          else zero_as = true;  // esolan.f:361-364: As = 0, next k
+          break;
+        }
+      }
+      if (iv > P.cap) {
+        overflow = true;
+        break;
+      }
+    }
+    if (lane == 0) work[1] += pairs;
+    if (!covered && !overflow) {  // the final list and the slots with points
+      const int n0 = __popcll(alive[0]);
+      n = n0 + __popcll(alive[1]);
+#pragma unroll
+      for (int w = 0; w < 2; ++w) {
+        const int sl = kWave * w + lane;
+        if (sl < nb) {
+          L.hasv[sl] = (uint8_t)((hv[w] >> lane) & 1ull);
+          if ((alive[w] >> lane) & 1ull) L.lst[(w ? n0 : 0) + __popcll(alive[w] & below)] = (uint8_t)sl;
+        }
+      }
+      wave_lds_sync();
+    }
+  } else {
+    while (k > 1) {
+      k = k - 1;
+      const int cnt = n - k;  // positions n ... k+1, fixed when the loop starts
+      if (cnt <= 0) continue;
+      const int K = L.lst[k - 1];
+      bool brk = false, anyrm = false;
+      for (int base = 0; base < cnt && !brk; base += kWave) {
+        const int lp = n - base - lane;  // 1-based list position of this lane
+        const bool valid = base + lane < cnt;
+        int code = kNone, Lc = 0;
+        double v[4] = {0.0, 0.0, 0.0, 0.0};
+        if (valid) {
+          Lc = L.lst[lp - 1];
+          code = es_classify(L, K, Lc, v);
+        }
+        work[1] += valid ? 1.0 : 0.0;
+        const uint64_t bm = __ballot(code >= kDropK);
+        const int first = bm ? __ffsll((unsigned long long)bm) - 1 : kWave;
+        const bool live = valid && lane < first;
+        if (live && code == kDropL) L.rm[lp - 1] = 1;
+        anyrm = anyrm || __ballot(live && code == kDropL) != 0;
+        const uint64_t vm = __ballot(live && code == kVert);
+        if (vm) {
+          if (live && code == kVert) {
+            const int o = iv + 2 * __popcll(vm & below);
+            if (o + 2 <= P.cap) {
               X.vt[o] = v[0];
               X.vs[o] = v[1];
               X.vt[o + 1] = v[2];
               X.vs[o + 1] = v[3];
+              X.vc[o] = X.vc[o + 1] = (uint16_t)(K | (Lc << 8));
             }
-            X.vc[o] = X.vc[o + 1] = (uint16_t)(K | (Lc << 8));
+            L.hasv[K] = 1;
+            L.hasv[Lc] = 1;
           }
-          L.hasv[K] = 1;
-          L.hasv[Lc] = 1;
+          iv += 2 * __popcll(vm);
         }
-        iv += 2 * __popcll(vm);
-      }
-      if (bm) {
-        const int bcode = __shfl(code, first, kWave);
-        if (bcode == kCover) {
-          covered = true;
-        } else if (bcode == kDropK) {
-          if (lane == 0) L.rm[k - 1] = 1;
-          anyrm = true;
-        } else {
-          zero_as = true;  // esolan.f:361-364: As = 0, next k
+        if (bm) {
+          const int bcode = __shfl(code, first, kWave);
+          if (bcode == kCover) {
+            covered = true;
+          } else if (bcode == kDropK) {
+            if (lane == 0) L.rm[k - 1] = 1;
+            anyrm = true;
+          } else {
+            zero_as = true;  // esolan.f:361-364: As = 0, next k
+          }
+          brk = true;
         }
-        brk = true;
       }
-    }
-    if (iv > P.cap) {
-      overflow = true;
-      break;
-    }
-    if (covered) break;
-    wave_lds_sync();
-    if (anyrm) {  // stable removal of the dropped positions
-      int nn = k - 1;
-      for (int base = k - 1; base < n; base += kWave) {
-        const int p = base + lane;
-        const bool keep = p < n && !L.rm[p];
-        const uint64_t km = __ballot(keep);
-        if (keep) L.tmp[nn + __popcll(km & below)] = L.lst[p];
-        if (p < n) L.rm[p] = 0;
-        nn += __popcll(km);
+      if (iv > P.cap) {
+        overflow = true;
+        break;
       }
+      if (covered) break;
       wave_lds_sync();
-      for (int p = k - 1 + lane; p < nn; p += kWave) L.lst[p] = L.tmp[p];
-      n = nn;
-      wave_lds_sync();
+      if (anyrm) {  // stable removal of the dropped positions
+        int nn = k - 1;
+        for (int base = k - 1; base < n; base += kWave) {
+          const int p = base + lane;
+          const bool keep = p < n && !L.rm[p];
+          const uint64_t km = __ballot(keep);
+          if (keep) L.tmp[nn + __popcll(km & below)] = L.lst[p];
+          if (p < n) L.rm[p] = 0;
+          nn += __popcll(km);
+        }
+        wave_lds_sync();
+        for (int p = k - 1 + lane; p < nn; p += kWave) L.lst[p] = L.tmp[p];
+        n = nn;
+        wave_lds_sync();
+      }
     }
   }
   if (overflow) {
