@@ -64,6 +64,7 @@ constexpr int kEsSlots = 192;    // circle slots of one atom (at most 191 neighb
 constexpr int kEsMaxNb = kEsSlots - 1;
 constexpr int kEsAngLds = 4 * kEsSlots;  // LDS doubles for the bucket angles
 constexpr int kEsTri = 12 * kEsSlots;    // LDS bytes for the pair outcomes
+constexpr int kEsVp = 512;               // kept pairs held in LDS
 constexpr double kEsCoinc = (double)1e-10f;  // esolan.f:127 compares with a default-real literal
 constexpr double kPi4 = 4.0 * kPi;
 
@@ -90,6 +91,8 @@ struct EsLds {
   uint8_t rm[kEsSlots];                                       // per list position: removed in this step
   uint8_t hasv[kEsSlots];                                     // per slot: an intersection point lies on it
   uint8_t fin[kEsSlots];                                      // per slot: final circle with points
+  uint16_t vp[kEsVp];        // kept pairs with intersection points (K | Lc << 8), in their order
+  uint8_t entc[kEsAngLds];   // circle of each bucket entry (entries in LDS)
   union {
     struct {
       int32_t cnt[kEsSlots], off[kEsSlots], cur[kEsSlots];  // point buckets by slot
@@ -99,6 +102,8 @@ struct EsLds {
     uint8_t code[12 * kEsSlots];
   };
 };
+
+static_assert(sizeof(EsLds) * kEsWaves <= 80 * 1024, "two workgroups of four wavefronts per CU");
 
 // Gradient form: per slot, the derivative of the atom's area by that
 // neighbour's position (frame of the rotated neighbours until the end).
@@ -203,9 +208,8 @@ __device__ __forceinline__ double wave_min(double v) {
 // higher position): esolan.f:254-449.  es_code gives the outcome,
 // es_vertices the two intersection points of a kVert pair (the same
 // operations on the same values, so either order gives the same bits).
-__device__ __forceinline__ void es_vertices(const EsLds& L, int K, int Lc, double v[4]) {
-  const double b1 = L.nB[K], c1 = L.nC[K], d1 = L.nD[K];
-  const double b2 = L.nB[Lc], c2 = L.nC[Lc], d2 = L.nD[Lc];
+__device__ __forceinline__ void es_vertices_of(double b1, double c1, double d1, double b2, double c2, double d2,
+                                               double v[4]) {
   const double D = 4.0 * ((b1 - b2) * (b2 * d1 - b1 * d2) + (c1 - c2) * (c2 * d1 - c1 * d2) - (d1 - d2) * (d1 - d2)) +
                    (b1 * c2 - b2 * c1) * (b1 * c2 - b2 * c1);
   const double am = 2.0 * ((b1 - b2) * (b1 - b2) + (c1 - c2) * (c1 - c2));
@@ -216,6 +220,16 @@ __device__ __forceinline__ void es_vertices(const EsLds& L, int K, int Lc, doubl
   v[1] = (s - p2) / am;
   v[2] = (t - p1) / am;
   v[3] = (s + p2) / am;
+}
+__device__ __forceinline__ void es_vertices(const EsLds& L, int K, int Lc, double v[4]) {
+  es_vertices_of(L.nB[K], L.nC[K], L.nD[K], L.nB[Lc], L.nC[Lc], L.nD[Lc], v);
+}
+// The same from the circles' own coefficients (the quotients formed again,
+// the same bits as those of step 3), for when the LDS of the quotients has
+// been reused.
+__device__ __forceinline__ void es_vertices_raw(const EsLds& L, int K, int Lc, double v[4]) {
+  const double a1 = L.A[K], a2 = L.A[Lc];
+  es_vertices_of(L.B[K] / a1, L.C[K] / a1, L.D[K] / a1, L.B[Lc] / a2, L.C[Lc] / a2, L.D[Lc] / a2, v);
 }
 __device__ __forceinline__ int es_code(const EsLds& L, int K, int Lc) {
   const double ak = L.A[K], al = L.A[Lc];
@@ -489,10 +503,11 @@ __device__ void es_item(const EsParams& P, EsLds& L, EsGradLds& G, const EsScrat
   bool covered = false, zero_as = false, overflow = false;
   const bool tab = nb * (nb - 1) / 2 <= kEsTri;
   if (tab) {
-    const float inb = 1.0f / (float)nb;
-    for (int t = lane; t < nb * nb; t += kWave) {
-      const int K = (int)(((float)t + 0.5f) * inb), Lc = t - K * nb;  // exact: t < 191^2
-      if (K < Lc) L.code[Lc * (Lc - 1) / 2 + K] = (uint8_t)es_code(L, K, Lc);
+    for (int t = lane; t < nb * (nb - 1) / 2; t += kWave) {  // t = Lc (Lc - 1) / 2 + K, K < Lc
+      int Lc = (int)(0.5f * (1.0f + sqrtf(1.0f + 8.0f * (float)t)));
+      Lc -= Lc * (Lc - 1) / 2 > t ? 1 : 0;  // the float root is within one of the integer one
+      Lc += (Lc + 1) * Lc / 2 <= t ? 1 : 0;
+      L.code[t] = (uint8_t)es_code(L, t - Lc * (Lc - 1) / 2, Lc);
     }
     wave_lds_sync();
     uint64_t alive[2] = {nb >= kWave ? ~0ull : (1ull << nb) - 1ull, nb > kWave ? (1ull << (nb - kWave)) - 1ull : 0ull};
@@ -520,7 +535,9 @@ __device__ void es_item(const EsParams& P, EsLds& L, EsGradLds& G, const EsScrat
           if ((vm >> lane) & 1ull) {  // descending slots: offset by the kept pairs above this lane
             const uint64_t above = lane == 63 ? 0ull : ~((2ull << lane) - 1ull);
             const int o = iv + 2 * __popcll(vm & above);
-            if (o + 2 <= P.cap) X.vc[o] = X.vc[o + 1] = (uint16_t)(K | ((kWave * w + lane) << 8));
+            const uint16_t pr = (uint16_t)(K | ((kWave * w + lane) << 8));
+            if (o + 2 <= P.cap) X.vc[o] = X.vc[o + 1] = pr;
+            if (o / 2 < kEsVp) L.vp[o / 2] = pr;
           }
           hv[w] |= vm;
           hv[K >> 6] |= 1ull << (K & 63);
@@ -635,7 +652,11 @@ __device__ void es_item(const EsParams& P, EsLds& L, EsGradLds& G, const EsScrat
     if (lane == 0) P.asig[item] = 0.0;
     return;
   }
-  if (tab) {  // the intersection points of the pairs kept, in their order
+  // with few kept pairs and entries the bucket step below takes them from
+  // LDS and forms each point and its angles at once; else the points go to
+  // the scratch here
+  const bool fastb = !GRAD && tab && iv / 2 <= kEsVp;
+  auto points_to_scratch = [&]() {
     wave_lds_sync();
     for (int o = 2 * lane; o < iv; o += 2 * kWave) {
       const int w = X.vc[o];
@@ -647,7 +668,8 @@ __device__ void es_item(const EsParams& P, EsLds& L, EsGradLds& G, const EsScrat
       X.vs[o + 1] = v[3];
     }
     wave_lds_sync();
-  }
+  };
+  if (tab && !fastb) points_to_scratch();
   // the position of a neighbour relative to the atom, after its rotation
   auto slot_xyz = [&](int s, double& x, double& y, double& z) {
     const int a = L.atom[s];
@@ -705,65 +727,111 @@ __device__ void es_item(const EsParams& P, EsLds& L, EsGradLds& G, const EsScrat
   }
   double As = zero_as ? 0.0 : kPi4 * pol;
   if ((nb > 0 && n < 1) || __ballot(inner)) As = 0.0;  // esolan.f:501-502
-  for (int s = lane; s < nb; s += kWave) L.cnt[s] = 0;
-  wave_lds_sync();
-
-  // intersection points bucketed by circle, circles in list order
-  for (int v = lane; v < iv; v += kWave) {
-    const int w = X.vc[v], c1 = w & 255, c2 = w >> 8;
-    if (L.fin[c1]) atomicAdd(&L.cnt[c1], 1);
-    if (L.fin[c2]) atomicAdd(&L.cnt[c2], 1);
-  }
-  wave_lds_sync();
-  int E = 0;
-  for (int base = 0; base < n; base += kWave) {
-    const int p = base + lane;
-    const int s = p < n ? L.lst[p] : 0;
-    int m = (p < n && L.fin[s]) ? L.cnt[s] : 0;
-    int incl = m;  // inclusive prefix over lanes
+  // intersection points bucketed by circle, circles in list order: bucket
+  // offsets from the counts
+  auto bucket_offsets = [&]() {
+    wave_lds_sync();
+    int tot = 0;
+    for (int base = 0; base < n; base += kWave) {
+      const int p = base + lane;
+      const int s = p < n ? L.lst[p] : 0;
+      int m = (p < n && L.fin[s]) ? L.cnt[s] : 0;
+      int incl = m;  // inclusive prefix over lanes
 #pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) {
-      const int t = __shfl_up(incl, o, kWave);
-      if (lane >= o) incl += t;
+      for (int o = 1; o < kWave; o <<= 1) {
+        const int t = __shfl_up(incl, o, kWave);
+        if (lane >= o) incl += t;
+      }
+      if (p < n && L.fin[s]) {
+        L.off[s] = tot + incl - m;
+        L.cur[s] = tot + incl - m;
+      }
+      tot += __shfl(incl, kWave - 1, kWave);
     }
-    if (p < n && L.fin[s]) {
-      L.off[s] = E + incl - m;
-      L.cur[s] = E + incl - m;
+    wave_lds_sync();
+    return tot;
+  };
+  for (int s = lane; s < nb; s += kWave) L.cnt[s] = 0;
+  int E = 0;
+  bool ent_lds = false;  // entries (circle, angle) in LDS
+  if (fastb) {
+    // both points of a kept pair lie on both its circles
+    wave_lds_sync();
+    for (int j = lane; j < iv / 2; j += kWave) {
+      const int w = L.vp[j], c1 = w & 255, c2 = w >> 8;
+      if (L.fin[c1]) atomicAdd(&L.cnt[c1], 2);
+      if (L.fin[c2]) atomicAdd(&L.cnt[c2], 2);
     }
-    E += __shfl(incl, kWave - 1, kWave);
+    E = bucket_offsets();
+    ent_lds = E <= kEsAngLds;
+    if (ent_lds) {  // each pair's points and their polar angles about both circles (esolan.f:626-640)
+      for (int j = lane; j < iv / 2; j += kWave) {
+        const int w = L.vp[j], c1 = w & 255, c2 = w >> 8;
+        double v[4];
+        es_vertices_raw(L, c1, c2, v);  // L.ang overlays the quotients
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+#pragma unroll
+          for (int u = 0; u < 2; ++u) {
+            const int c = u ? c2 : c1;
+            if (!L.fin[c]) continue;
+            const double aa = 0.5 / L.A[c];
+            const double t = atan2(v[2 * h + 1] + L.C[c] * aa, v[2 * h] + L.B[c] * aa);
+            const int q = atomicAdd(&L.cur[c], 1);
+            L.ang[q] = t;
+            L.entc[q] = (uint8_t)c;
+          }
+        }
+      }
+      wave_lds_sync();
+    } else {
+      points_to_scratch();
+      for (int s = lane; s < nb; s += kWave) L.cnt[s] = 0;
+    }
   }
-  wave_lds_sync();
-  for (int v = lane; v < iv; v += kWave) {
-    const int w = X.vc[v], c1 = w & 255, c2 = w >> 8;
-    if (L.fin[c1]) {
-      const int q = atomicAdd(&L.cur[c1], 1);
-      X.ent[q] = (uint16_t)v;
-      X.entc[q] = (uint8_t)c1;
+  if (!ent_lds) {
+    wave_lds_sync();
+    for (int v = lane; v < iv; v += kWave) {
+      const int w = X.vc[v], c1 = w & 255, c2 = w >> 8;
+      if (L.fin[c1]) atomicAdd(&L.cnt[c1], 1);
+      if (L.fin[c2]) atomicAdd(&L.cnt[c2], 1);
     }
-    if (L.fin[c2]) {
-      const int q = atomicAdd(&L.cur[c2], 1);
-      X.ent[q] = (uint16_t)v;
-      X.entc[q] = (uint8_t)c2;
+    E = bucket_offsets();
+    for (int v = lane; v < iv; v += kWave) {
+      const int w = X.vc[v], c1 = w & 255, c2 = w >> 8;
+      if (L.fin[c1]) {
+        const int q = atomicAdd(&L.cur[c1], 1);
+        X.ent[q] = (uint16_t)v;
+        X.entc[q] = (uint8_t)c1;
+      }
+      if (L.fin[c2]) {
+        const int q = atomicAdd(&L.cur[c2], 1);
+        X.ent[q] = (uint16_t)v;
+        X.entc[q] = (uint8_t)c2;
+      }
     }
+    wave_lds_sync();
   }
-  wave_lds_sync();
   // polar angles about each circle's centre (esolan.f:626-640); in LDS
   // over the normalised coefficients (dead after the pair loop) when they
   // fit, the sorted copy too when both do, else in the wave's scratch
   const bool ang_lds = E <= kEsAngLds, srt_lds = 2 * E <= kEsAngLds;
-  for (int e = lane; e < E; e += kWave) {
-    const int s = X.entc[e], v = X.ent[e];
-    const double a = L.A[s], aa = 0.5 / a;
-    const double t = atan2(X.vs[v] + L.C[s] * aa, X.vt[v] + L.B[s] * aa);
-    if (ang_lds)
-      L.ang[e] = t;
-    else
-      X.ang[e] = t;
+  if (!ent_lds) {
+    for (int e = lane; e < E; e += kWave) {
+      const int s = X.entc[e], v = X.ent[e];
+      const double a = L.A[s], aa = 0.5 / a;
+      const double t = atan2(X.vs[v] + L.C[s] * aa, X.vt[v] + L.B[s] * aa);
+      if (ang_lds)
+        L.ang[e] = t;
+      else
+        X.ang[e] = t;
+    }
+    wave_lds_sync();
   }
-  wave_lds_sync();
+  auto entry_circle = [&](int e) -> int { return ent_lds ? L.entc[e] : X.entc[e]; };
   // ascending within each bucket (esolan.f:642-659)
   for (int e = lane; e < E; e += kWave) {
-    const int s = X.entc[e], lo = L.off[s], hi = lo + L.cnt[s];
+    const int s = entry_circle(e), lo = L.off[s], hi = lo + L.cnt[s];
     double x;
     int r;
     if (ang_lds) {
@@ -786,7 +854,7 @@ __device__ void es_item(const EsParams& P, EsLds& L, EsGradLds& G, const EsScrat
   // arcs between consecutive points whose midpoint no other circle covers
   double arcs = 0.0;
   for (int p = lane; p < E; p += kWave) {
-    const int s = X.entc[p], lo = L.off[s], hi = lo + L.cnt[s];
+    const int s = entry_circle(p), lo = L.off[s], hi = lo + L.cnt[s];
     const int pn = (p + 1 < hi) ? p + 1 : lo;
     double alp, bet;
     if (srt_lds) {
@@ -1095,7 +1163,9 @@ static int es_launch(EsolanWork& W, hipStream_t st, EsParams P, int32_t* d_small
       HIPCHK(hipFuncSetAttribute((const void*)k_esolan<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
       hipLaunchKernelGGL(k_esolan<true>, dim3(waves / kEsWaves), dim3(kEsThreads), lds, st, P);
     } else {
-      hipLaunchKernelGGL(k_esolan<false>, dim3(waves / kEsWaves), dim3(kEsThreads), sizeof(EsLds) * kEsWaves, st, P);
+      const size_t lds = sizeof(EsLds) * kEsWaves;  // above the default 64 KiB: two workgroups per CU
+      HIPCHK(hipFuncSetAttribute((const void*)k_esolan<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      hipLaunchKernelGGL(k_esolan<false>, dim3(waves / kEsWaves), dim3(kEsThreads), lds, st, P);
     }
     HIPCHK(hipGetLastError());
     int32_t h[3];
