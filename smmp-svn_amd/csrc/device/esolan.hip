@@ -512,23 +512,31 @@ __device__ void es_item(const EsParams& P, EsLds& L, EsGradLds& G, const EsScrat
     wave_lds_sync();
     uint64_t alive[2] = {nb >= kWave ? ~0ull : (1ull << nb) - 1ull, nb > kWave ? (1ull << (nb - kWave)) - 1ull : 0ull};
     uint64_t hv[2] = {0ull, 0ull};  // slots with an intersection point
-    double pairs = 0.0;             // pairs examined (esolan.f's count: up to the ending one)
+    int pairs = 0;                  // pairs examined (esolan.f's count: up to the ending one)
+    // this lane's slot in each word and the outcomes of step K: read one
+    // step ahead (they do not depend on the live set), masked when used
+    const int tri0 = lane * (lane - 1) / 2, tri1 = (lane + kWave) * (lane + kWave - 1) / 2;
+    auto read_codes = [&](int K, int (&c)[2]) {
+      c[0] = L.code[(lane > K && lane < nb) ? tri0 + K : 0];
+      c[1] = L.code[(lane + kWave < nb) ? tri1 + K : 0];
+    };
+    int nxt[2] = {kNone, kNone};
+    if (nb >= 2) read_codes(nb - 2, nxt);
     for (int K = nb - 2; K >= 0 && !covered; --K) {
-      int code[2];
+      int code[2] = {nxt[0], nxt[1]};
+      if (K > 0) read_codes(K - 1, nxt);
       uint64_t act[2];
 #pragma unroll
-      for (int w = 0; w < 2; ++w) {  // live slots above K, their outcomes with K
+      for (int w = 0; w < 2; ++w) {  // live slots above K
         const int lo = K + 1 - kWave * w;  // first bit above K in word w
         act[w] = lo <= 0 ? alive[w] : (lo >= kWave ? 0ull : alive[w] & ~((1ull << lo) - 1ull));
-        const int Lc = kWave * w + lane;
-        code[w] = ((act[w] >> lane) & 1ull) ? L.code[Lc * (Lc - 1) / 2 + K] : kNone;
       }
       for (int w = 1; w >= 0; --w) {  // the higher word first, each from its top bit
         if (!act[w]) continue;
         const uint64_t ev = __ballot(code[w] >= kDropK) & act[w];
         const int f = ev ? 63 - __clzll(ev) : -1;                          // the first event in scan order
         const uint64_t live = f < 0 ? act[w] : act[w] & ~((2ull << f) - 1ull);  // scanned before it (f = 63: none)
-        pairs += (double)(__popcll(live) + (f >= 0 ? 1 : 0));
+        pairs += __popcll(live) + (f >= 0 ? 1 : 0);
         alive[w] &= ~(__ballot(code[w] == kDropL) & live);
         const uint64_t vm = __ballot(code[w] == kVert) & live;
         if (vm) {
@@ -543,10 +551,10 @@ __device__ void es_item(const EsParams& P, EsLds& L, EsGradLds& G, const EsScrat
           hv[K >> 6] |= 1ull << (K & 63);
           iv += 2 * __popcll(vm);
         }
-        if (f >= 0) {
-          const int bcode = __shfl(code[w], f, kWave);
-          if (bcode == kCover) covered = true;
-          else if (bcode == kDropK) alive[K >> 6] &= ~(1ull << (K & 63));
+        if (f >= 0) {  // the ending outcome, from the masks (no lane exchange)
+          const uint64_t fb = 1ull << f;
+          if (__ballot(code[w] == kCover) & fb) covered = true;
+          else if (__ballot(code[w] == kDropK) & fb) alive[K >> 6] &= ~(1ull << (K & 63));
           else zero_as = true;  // esolan.f:361-364: As = 0, next k
           break;
         }
@@ -556,7 +564,7 @@ __device__ void es_item(const EsParams& P, EsLds& L, EsGradLds& G, const EsScrat
         break;
       }
     }
-    if (lane == 0) work[1] += pairs;
+    if (lane == 0) work[1] += (double)pairs;
     if (!covered && !overflow) {  // the final list and the slots with points
       const int n0 = __popcll(alive[0]);
       n = n0 + __popcll(alive[1]);
@@ -899,10 +907,12 @@ __device__ void es_item(const EsParams& P, EsLds& L, EsGradLds& G, const EsScrat
     work[2] += 1.0;
     work[3] += (double)q;
     if (cov) continue;
-    const double hm = 5e-1 * (bet - alp), hp = 5e-1 * (bet + alp);
-    double dsbetmalp, dcbetmalp, dsbetpalp, dcbetpalp;
+    // (bet + alp) / 2 is prob to the bit (both halvings exact), so its sine
+    // and cosine are sp and cp
+    const double hm = 5e-1 * (bet - alp);
+    double dsbetmalp, dcbetmalp;
     sincos(hm, &dsbetmalp, &dcbetmalp);
-    sincos(hp, &dsbetpalp, &dcbetpalp);
+    const double dsbetpalp = sp, dcbetpalp = cp;
     const double uv = daba * (bcd + 2.0 * R42 * ak) * dcbetmalp - a * wv * (b * dcbetpalp + cc * dsbetpalp);
     arcs += R2 * (aia * (alp - bet) + (d + R42 * a) * vv1 * (kPi - 2.0 * atan(uv / (2.0 * ak * vv * dsbetmalp))));
     if constexpr (GRAD) {  // esolan.f:684-862
