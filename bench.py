@@ -51,8 +51,10 @@ FLOP_ES_COVER = 7   # one circle tested against an arc midpoint: 3 mul, 3 add, 1
 # 570,547 KiB x 2.0 + 2,670,886 KiB.
 HBM_BYTES_PER_SWEEP = 19.517e6
 # The same for one batch of the batched-energy workload (4,096 conformations,
-# mixed): (123,055 + 39,745) KiB FETCH_SIZE x 2.0 + 427 KiB WRITE_SIZE x 1.0.
-C3_HBM_BYTES_PER_BATCH = 333.8e6
+# the two dispatches): (18,224 + 6,910) KiB FETCH_SIZE x 2.0 + 427 KiB
+# WRITE_SIZE x 1.0 (each XCD takes runs of one system's conformations, so a
+# system's pair bits are fetched into one L2 rather than all eight).
+C3_HBM_BYTES_PER_BATCH = 51.9e6
 
 
 def log(*a):

@@ -783,14 +783,31 @@ double engine_enysol(Engine*& e, State& s) {
 // the CUs together; within a class the largest systems go first.
 constexpr int kBatchClasses = 2;
 
+constexpr int kXcdRun = 16;  // consecutive conformations of the sorted list per XCD
+
 // One launch of k_energy over the conformations order[begin, begin+count).
 struct EnergyClass {
   int begin = 0, count = 0, threads = kEnergyThreads;
   LdsPlan plan;
 };
 
+// XCD-aware block order of one launch: the hardware deals workgroups to the
+// 8 XCDs in turn (block b to XCD b mod 8), so block x + 8 p takes entry
+// r 8k + x k + i of the sorted list (p = r k + i): each XCD gets runs of k
+// consecutive entries (the conformations of one system, whose pair bits then
+// stay in that XCD's L2), and rounds of 8 runs keep the largest systems
+// first on every XCD.  The tail past the last full round keeps its order.
+static void xcd_interleave(int32_t* seg, int n, int k) {
+  const int round = 8 * k, full = n / round * round;
+  const std::vector<int32_t> src(seg, seg + full);
+  for (int b = 0; b < full; ++b) {
+    const int x = b % 8, p = b / 8;
+    seg[b] = src[(p / k) * round + x * k + p % k];
+  }
+}
+
 // The classes of a set of conformations and the order that groups them
-// (largest systems first within a class).
+// (largest systems first within a class, runs of one system per XCD).
 static std::vector<EnergyClass> plan_classes(const std::vector<DevSys>& dsys, const std::vector<int32_t>& conf_sys,
                                              std::vector<int32_t>& order) {
   const int nsys = (int)dsys.size(), nconf = (int)conf_sys.size();
@@ -834,6 +851,7 @@ static std::vector<EnergyClass> plan_classes(const std::vector<DevSys>& dsys, co
     C.plan.bytes = with <= cap ? with : without;
     classes.push_back(C);
   }
+  for (const EnergyClass& C : classes) xcd_interleave(order.data() + C.begin, C.count, kXcdRun);
   return classes;
 }
 
