@@ -440,7 +440,8 @@ def c3_headline(K: int) -> dict:
         ach = flops / (ke["ms"] * 1e-3) / 1e12
         out[name] = {"evals_per_s": round(K * n / dt, 1), "k_energy_ms_per_batch": round(ke["ms"] / K, 3),
                      "launch_shapes": shapes,
-                     "roofline_frac": round(ach / peak, 5), "achieved_tflops": round(ach, 3), "peak_tflops": peak}
+                     "roofline_frac": round(ach / peak, 5), "achieved_tflops": round(ach, 3), "peak_tflops": peak,
+                     "traffic_bytes_per_batch": C3_HBM_BYTES_PER_BATCH if (nseq, per) == (256, 16) else None}
     B.close()
     for c in ctxs:
         c.close()
