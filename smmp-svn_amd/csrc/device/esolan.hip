@@ -41,6 +41,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -127,6 +128,7 @@ struct EsParams {
   char* scratch;         // per wavefront slot: wave_bytes
   size_t wave_bytes;
   int cap;               // intersection points per atom
+  int forms;             // bit 0: pair loop in list form only; bit 1: bucket entries in the scratch only
   double* counters;      // atom pairs scanned, circle pairs classified, arcs tested, coverage tests
   // gradient form: per item kEsSlots rows (neighbour atom, sigma_i dAs_i/dr
   // of it), slot kEsSlots - 1 the atom's own; grow[item] rows in use (0:
@@ -501,7 +503,7 @@ __device__ void es_item(const EsParams& P, EsLds& L, EsGradLds& G, const EsScrat
   // list form below.
   int n = nb, k = nb, iv = 0;
   bool covered = false, zero_as = false, overflow = false;
-  const bool tab = nb * (nb - 1) / 2 <= kEsTri;
+  const bool tab = nb * (nb - 1) / 2 <= kEsTri && !(P.forms & 1);
   if (tab) {
     for (int t = lane; t < nb * (nb - 1) / 2; t += kWave) {  // t = Lc (Lc - 1) / 2 + K, K < Lc
       int Lc = (int)(0.5f * (1.0f + sqrtf(1.0f + 8.0f * (float)t)));
@@ -771,7 +773,7 @@ __device__ void es_item(const EsParams& P, EsLds& L, EsGradLds& G, const EsScrat
       if (L.fin[c2]) atomicAdd(&L.cnt[c2], 2);
     }
     E = bucket_offsets();
-    ent_lds = E <= kEsAngLds;
+    ent_lds = E <= kEsAngLds && !(P.forms & 2);
     if (ent_lds) {  // each pair's points and their polar angles about both circles (esolan.f:626-640)
       for (int j = lane; j < iv / 2; j += kWave) {
         const int w = L.vp[j], c1 = w & 255, c2 = w >> 8;
@@ -1220,6 +1222,9 @@ void esolan_energies(EsolanWork*& Wp, hipStream_t st, const DevSys* d_systems, c
   HIPCHK(hipEventRecord(W.ev0, st));
 
   EsParams P{};
+  // SMMP_ESOLAN_FORMS (tests): force the general forms of the pair loop
+  // (bit 0) and of the bucket step (bit 1) that large neighbour sets take
+  if (const char* env = std::getenv("SMMP_ESOLAN_FORMS")) P.forms = std::atoi(env) & 3;
   P.systems = d_systems;
   P.conf_sys = d_conf_sys;
   P.xyz = d_xyz;

@@ -110,6 +110,29 @@ def test_energy_batch_matches_oracle():
     assert int(c.view("myblock.mti")[0]) == mt.s.mti
 
 
+@pytest.mark.parametrize("forms", [1, 2, 3])
+def test_general_forms_match_oracle(forms, monkeypatch):
+    """The forms the kernel takes for large neighbour sets (the pair loop
+    on the circle list instead of a slot mask, bit 0; bucket entries and
+    angles in the wave's scratch, bit 1), forced on every atom: the same
+    energies as the oracle on sampled-size conformations."""
+    from smmp_amd import _ffi
+    d = golden("esolan_c2_20mer.npz")
+    c = make_ctx(seq=list(d["seq_in"]), itysol=-1)
+    o = OracleSystem(c)
+    V = _random_confs(c, 16, 9)
+    c.sgrnd(13)
+    monkeypatch.setenv("SMMP_ESOLAN_FORMS", str(forms))
+    say(f"smmp_energy_batch c2_20mer 16 conformations, general forms {forms}")
+    out = _ffi.energy_batch([c], np.zeros(len(V), dtype=np.int32), V.ravel(), precision=0)
+    mt = OracleMT(13)
+    for k in range(len(V)):
+        ref = o.energy(V[k], mt)
+        assert _close(out[k, 6], ref[6]), (k, out[k, 6], ref[6])
+        assert _close(out[k, 0], ref[0]), (k, out[k, 0], ref[0])
+    assert int(c.view("myblock.mti")[0]) == mt.s.mti
+
+
 def test_energy_batch_two_systems():
     from smmp_amd import _ffi
     a = make_ctx(tag="enkefa", itysol=-1)
