@@ -94,8 +94,11 @@ __device__ inline EnergyLds carve_energy_lds(double* base, const DevSys& S, int 
 }
 
 // Word k of the pair bit rows.
+// The two sources in their own address spaces: a select of the two
+// pointers would make every read a flat access.
 __device__ __forceinline__ uint64_t pair_bits(const EnergyLds& L, size_t k) {
-  return L.lbits ? L.lbits[k] : L.bits[k];
+  if (L.lbits) return ((const __attribute__((address_space(3))) uint64_t*)L.lbits)[k];
+  return ((const __attribute__((address_space(1))) uint64_t*)L.bits)[k];
 }
 
 // Torsion of atom a for the variable values vl (setvar.f:43-57 from the
