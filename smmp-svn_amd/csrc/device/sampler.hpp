@@ -912,8 +912,13 @@ __device__ __forceinline__ void torsion_pair(const DevSys& S, const SamplerLds& 
 
 // P.nsweep Metropolis sweeps of one replica per group of G blocks; DA:
 // delayed acceptance with the surrogate network (P.da).
-template <bool MIXED, bool DA>
+// LEAN: one chain, no speculative batches, verify mode or trace (the
+// common configuration): their code and the values it keeps live across a
+// trial are compiled out.
+template <bool MIXED, bool DA, bool LEAN>
 __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
+  const bool verify = !LEAN && P.verify != 0;
+  double* const trace = LEAN ? nullptr : P.trace;
   extern __shared__ double smem[];
   const DevSys& S = *P.sys;
   const int G = P.G;
@@ -925,7 +930,7 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
   SamplerLds Q = carve_sampler_lds(smem, S, nvt, nwaves, lds_flags, P.axvr);
   // speculative batches: every block keeps the free-point counts of all
   // heavy atoms (shared through gnf after each full recomputation)
-  const bool spec_on = P.spec != 0 && !DA && !P.verify && G > 1 && G <= kSpecMaxG;
+  const bool spec_on = !LEAN && P.spec != 0 && !DA && !verify && G > 1 && G <= kSpecMaxG;
   Work wk;
   EnergyLds& L = Q.L;
   double* gvl = P.vlvr + (size_t)r * nvt;
@@ -1006,13 +1011,13 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
   for (int sw = 0; sw < P.nsweep; ++sw) {
     if (tid == 0) Q.sc[14] = P.beta[P.slot_of[r]];  // LDS: no global reload inside the loop
     // torsion trials, then with several chains six rigid trials per chain
-    const int ntrial = P.nvr + (S.nml > 1 ? 6 * S.nml : 0);
+    const int ntrial = P.nvr + (!LEAN && S.nml > 1 ? 6 * S.nml : 0);
     // speculative batches for this sweep's torsion trials: not from a
     // clashing state (those trials take the full recomputation)
     __syncthreads();
     const bool spec_sweep = spec_on && fabs(Q.sc[0]) < 1e5;
     for (int nsw = 0; nsw < ntrial; ++nsw) {
-      const bool rigid = nsw >= P.nvr;  // uniform
+      const bool rigid = !LEAN && nsw >= P.nvr;  // uniform
       if (spec_sweep && !rigid) {
         // ---- speculative batch of the next nb torsion trials.  The G
         //      blocks hold the same state and twister; each draws all nb
@@ -1115,8 +1120,8 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
               Q.sc[kScUpd + 5] += 1.0;
               Q.sc[kScUpd + 9] += 1.0;
             }
-            if (P.trace && g == 0 && r == P.trace_rep && *P.trace_n < P.trace_cap) {
-              double* tr = P.trace + 4 * (size_t)(*P.trace_n);
+            if (trace && g == 0 && r == P.trace_rep && *P.trace_n < P.trace_cap) {
+              double* tr = trace + 4 * (size_t)(*P.trace_n);
               tr[0] = T[0];
               tr[1] = enew;
               tr[2] = eol;
@@ -1255,11 +1260,11 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
         const float ytrial = da_trial(D, P.da, jv, Q.sc[15], Q.sc[2]);
         if (tid == 0) {
           const double ds = ((double)ytrial - (double)D.y[0]) * P.da_scale;
-          if (P.trace && g == 0 && r == P.trace_rep && *P.trace_n < P.trace_cap) {
+          if (trace && g == 0 && r == P.trace_rep && *P.trace_n < P.trace_cap) {
             // delayed acceptance traces its screens: variable, new value,
             // network change, 0 (rejected by the network; the exact stage
             // sets 1 or 2)
-            double* t = P.trace + 4 * (size_t)(*P.trace_n);
+            double* t = trace + 4 * (size_t)(*P.trace_n);
             t[0] = jv;
             t[1] = Q.sc[2];
             t[2] = ds;
@@ -1282,7 +1287,7 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
         if (Q.sc[20] == 0.0) continue;  // rejected without the exact energy (uniform)
       }
       EnergyOut vref;  // verify mode (G = 1): full energy before the move (thread 0)
-      if (P.verify && !full) {
+      if (verify && !full) {
         const ReplicaEnergy re = replica_energy<MIXED>(P.sys, nvt, lds_flags, 0, false, 0, 1);
         vref = re.o;
         __syncthreads();
@@ -1479,7 +1484,7 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
         if (P.phase) __syncthreads();
         lap(kPhFull);
       }
-      if (P.verify && !full) {  // compare each incremental component with a full recomputation
+      if (verify && !full) {  // compare each incremental component with a full recomputation
         const ReplicaEnergy rn = replica_energy<MIXED>(P.sys, nvt, lds_flags, 0, false, 0, 1);
         const EnergyOut& vnew = rn.o;
         if (tid == 0 && P.dbg[16 * r] == 0.0) {
@@ -1534,11 +1539,11 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep(SamplerParams P) {
             Q.sc[kScUpd + 9] += 1.0;
           }
         }
-        if (P.trace && g == 0 && r == P.trace_rep) {
+        if (trace && g == 0 && r == P.trace_rep) {
           if (da) {  // the screen's record of this trial: 1 rejected exactly, 2 accepted
-            if (*P.trace_n > 0) P.trace[4 * (size_t)(*P.trace_n - 1) + 3] = acc ? 2.0 : 1.0;
+            if (*P.trace_n > 0) trace[4 * (size_t)(*P.trace_n - 1) + 3] = acc ? 2.0 : 1.0;
           } else if (*P.trace_n < P.trace_cap) {
-            double* t = P.trace + 4 * (size_t)(*P.trace_n);
+            double* t = trace + 4 * (size_t)(*P.trace_n);
             t[0] = jv;
             t[1] = enew;
             t[2] = eol;
