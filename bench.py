@@ -48,8 +48,8 @@ FLOP_ES_COVER = 7   # one circle tested against an arc midpoint: 3 mul, 3 add, 1
 # HBM-side bytes per sweep of this workload in FP64: FETCH_SIZE x 2.0 plus
 # WRITE_SIZE x 1.0 (the factors make a 256 MiB device copy read 256 MiB and
 # write 256 MiB) of the timed launch, rocprofv3 --pmc, 200 sweeps:
-# 570,547 KiB x 2.0 + 2,670,886 KiB.
-HBM_BYTES_PER_SWEEP = 19.517e6
+# 577,014 KiB x 2.0 + 2,632,251 KiB.
+HBM_BYTES_PER_SWEEP = 19.386e6
 # The same for one batch of the batched-energy workload (4,096 conformations,
 # the two dispatches): (18,224 + 6,910) KiB FETCH_SIZE x 2.0 + 427 KiB
 # WRITE_SIZE x 1.0 (each XCD takes runs of one system's conformations, so a
